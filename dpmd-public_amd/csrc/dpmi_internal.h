@@ -1,0 +1,60 @@
+// Internal helpers shared by the HIP translation units of libdpmi.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "dpmi.h"
+
+namespace dpmi {
+
+// Thread-local last error text, read through dpmi_last_error().
+void set_error(const std::string& msg);
+
+#define DPMI_CHECK_ARG(cond, msg)       \
+  do {                                  \
+    if (!(cond)) {                      \
+      ::dpmi::set_error(msg);           \
+      return DPMI_ERR_ARG;              \
+    }                                   \
+  } while (0)
+
+#define DPMI_HIP(call)                                                        \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ::dpmi::set_error(std::string(#call) + ": " + hipGetErrorString(e_));   \
+      return DPMI_ERR_HIP;                                                    \
+    }                                                                         \
+  } while (0)
+
+// Check the launch that was just queued.
+#define DPMI_LAUNCH_CHECK() DPMI_HIP(hipGetLastError())
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ----------------------------------------------------------- wave helpers
+// MI355X (gfx950) runs 64-wide wavefronts.
+constexpr int kWave = 64;
+
+__device__ inline int lane_id() { return __lane_id(); }
+
+// Lanes below this one whose bit is set in mask.
+__device__ inline int mask_prefix(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0));
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+}  // namespace dpmi

@@ -1,0 +1,525 @@
+// Environment matrix, force and virial kernels for se_e2_a on gfx950.
+//
+// Mapping: one 64-wide wavefront per centre atom, four atoms per 256-thread
+// workgroup.  A lane owns neighbor slots lane, lane+64, ...; the slot's four
+// em values (and twelve derivatives) are contiguous, so each store
+// instruction of a wave writes one contiguous span of the [nloc, nnei, 4]
+// (or [.., 12]) row: the stores are fully coalesced.  Centre-atom sums are
+// wavefront reductions; only neighbor forces use global atomics.  The virial
+// is reduced deterministically: per-workgroup partials in LDS, then one
+// compensated (Neumaier) sum per component.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "dpmi_internal.h"
+
+namespace dpmi {
+namespace {
+
+constexpr int kMaxTypes = 64;
+constexpr int kAtomsPerBlock = 4;
+
+struct SecParams {
+  int ntypes;
+  int nnei;
+  int sec[kMaxTypes + 1];
+};
+
+inline int make_sec(SecParams* sp, const int* sel_host, int ntypes) {
+  DPMI_CHECK_ARG(sel_host != nullptr, "sel is NULL");
+  DPMI_CHECK_ARG(ntypes > 0 && ntypes <= kMaxTypes, "ntypes out of range [1, 64]");
+  std::memset(sp, 0, sizeof(*sp));
+  sp->ntypes = ntypes;
+  for (int t = 0; t < ntypes; ++t) {
+    DPMI_CHECK_ARG(sel_host[t] >= 0, "sel must be >= 0");
+    sp->sec[t + 1] = sp->sec[t] + sel_host[t];
+  }
+  sp->nnei = sp->sec[ntypes];
+  return DPMI_OK;
+}
+
+__device__ inline int slot_type(const SecParams& sp, int s) {
+  int t = 0;
+  while (t + 1 < sp.ntypes && s >= sp.sec[t + 1]) ++t;
+  return t;
+}
+
+// spline5_switch, source/lib/include/switcher.h:60-85 (rmin, rmax are float).
+template <typename FP>
+__device__ inline void spline5(FP& vv, FP& dd, FP xx, float rmin, float rmax) {
+  if (xx < rmin) {
+    dd = (FP)0.;
+    vv = (FP)1.;
+  } else if (xx < rmax) {
+    FP uu = (xx - rmin) / (FP)(rmax - rmin);
+    FP du = (FP)1. / (FP)(rmax - rmin);
+    vv = uu * uu * uu * ((FP)-6. * uu * uu + (FP)15. * uu - (FP)10.) + (FP)1.;
+    dd = ((FP)3. * uu * uu * ((FP)-6. * uu * uu + (FP)15. * uu - (FP)10.) +
+          uu * uu * uu * ((FP)-12. * uu + (FP)15.)) *
+         du;
+  } else {
+    dd = (FP)0.;
+    vv = (FP)0.;
+  }
+}
+
+// env_mat_a_cpu for one neighbor (source/lib/src/env_mat.cc:159-219): the four
+// values and their derivative w.r.t. the centre atom, before normalisation.
+template <typename FP>
+__device__ inline void env_one(const FP rr[3], float rmin, float rmax, FP v[4], FP d[12]) {
+  FP nr2 = rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
+  FP inr = (FP)1. / sqrt(nr2);
+  FP nr = nr2 * inr;
+  FP inr2 = inr * inr;
+  FP inr4 = inr2 * inr2;
+  FP inr3 = inr4 * nr;
+  FP sw, dsw;
+  spline5(sw, dsw, nr, rmin, rmax);
+  v[0] = (FP)1. / nr;
+  v[1] = rr[0] / nr2;
+  v[2] = rr[1] / nr2;
+  v[3] = rr[2] / nr2;
+#pragma unroll
+  for (int dd = 0; dd < 3; ++dd) d[dd] = rr[dd] * inr3 * sw - v[0] * dsw * rr[dd] * inr;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int dd = 0; dd < 3; ++dd) {
+      FP t = (FP)2. * rr[c] * rr[dd] * inr4;
+      if (c == dd) t = t - inr2;
+      d[3 + 3 * c + dd] = t * sw - v[1 + c] * dsw * rr[dd] * inr;
+    }
+  v[0] *= sw;
+  v[1] *= sw;
+  v[2] *= sw;
+  v[3] *= sw;
+}
+
+// First -1 in each type section of the wave's row: slots at or after it are
+// empty (env_mat_a_cpu breaks at the first negative entry of a section).
+// lim must hold kMaxTypes ints for this wave.
+__device__ inline void section_limits(const int* __restrict__ nl, const SecParams& sp, int* lim) {
+  const int lane = lane_id();
+  for (int t = lane; t < sp.ntypes; t += kWave) lim[t] = sp.sec[t + 1];
+  __syncthreads();
+  for (int s = lane; s < sp.nnei; s += kWave)
+    if (nl[s] < 0) atomicMin(&lim[slot_type(sp, s)], s);
+  __syncthreads();
+}
+
+template <typename FP>
+__global__ void __launch_bounds__(256)
+    k_env_mat_a(FP* __restrict__ em, FP* __restrict__ em_deriv, FP* __restrict__ rij, const int* __restrict__ nlist,
+                const FP* __restrict__ ext_c, const int* __restrict__ ext_t, int nloc, const FP* __restrict__ avg,
+                const FP* __restrict__ std_, float rmin, float rmax, SecParams sp) {
+  __shared__ int lim_all[kAtomsPerBlock][kMaxTypes];
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int i = blockIdx.x * kAtomsPerBlock + w;
+  const bool active = i < nloc;
+  const int nnei = sp.nnei;
+  const int ii = active ? i : 0;
+  const int* nl = nlist + (int64_t)ii * nnei;
+  section_limits(nl, sp, lim_all[w]);
+  if (!active) return;
+  int ti = ext_t[i];
+  if (ti >= sp.ntypes) ti = -1;  // no statistics for such a type: treated as virtual
+  const FP xi0 = ext_c[3 * i + 0], xi1 = ext_c[3 * i + 1], xi2 = ext_c[3 * i + 2];
+  const int nem = nnei * 4;
+  for (int s = lane; s < nnei; s += kWave) {
+    int t = slot_type(sp, s);
+    int j = nl[s];
+    bool valid = s < lim_all[w][t];
+    FP rr[3] = {(FP)0., (FP)0., (FP)0.};
+    FP v[4] = {(FP)0., (FP)0., (FP)0., (FP)0.};
+    FP d[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) d[k] = (FP)0.;
+    if (valid) {
+      rr[0] = ext_c[3 * j + 0] - xi0;
+      rr[1] = ext_c[3 * j + 1] - xi1;
+      rr[2] = ext_c[3 * j + 2] - xi2;
+      env_one(rr, rmin, rmax, v, d);
+    }
+    int64_t o = (int64_t)i * nem + 4 * s;
+    if (ti >= 0) {
+      const FP* av = avg + (int64_t)ti * nem + 4 * s;
+      const FP* sd = std_ + (int64_t)ti * nem + 4 * s;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) em[o + c] = (v[c] - av[c]) / sd[c];
+      if (em_deriv) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) em_deriv[3 * o + k] = d[k] / sd[k / 3];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) em[o + c] = (FP)0.;
+      if (em_deriv) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) em_deriv[3 * o + k] = (FP)0.;
+      }
+    }
+    if (rij) {
+      int64_t r = (int64_t)i * nnei * 3 + 3 * s;
+      rij[r + 0] = rr[0];
+      rij[r + 1] = rr[1];
+      rij[r + 2] = rr[2];
+    }
+  }
+}
+
+template <typename FP>
+__global__ void k_fill(FP* p, int64_t n, FP v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// prod_force_a (source/lib/src/prod_force.cc:23-76): F_i -= sum_a nd*ed,
+// F_j += sum_{a in slot j} nd*ed.
+template <typename FP>
+__global__ void __launch_bounds__(256)
+    k_prod_force(FP* __restrict__ force, const FP* __restrict__ nd, const FP* __restrict__ ed,
+                 const int* __restrict__ nlist, int nloc, int nall, int nnei, int nframes) {
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int64_t gi = (int64_t)blockIdx.x * kAtomsPerBlock + w;
+  if (gi >= (int64_t)nloc * nframes) return;
+  const int64_t f = gi / nloc, li = gi % nloc;
+  const int64_t nd_off = gi * nnei * 4;
+  FP ci[3] = {(FP)0., (FP)0., (FP)0.};
+  FP* fr = force + f * nall * 3;
+  for (int s = lane; s < nnei; s += kWave) {
+    FP g[3] = {(FP)0., (FP)0., (FP)0.};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      FP n = nd[nd_off + 4 * s + c];
+      const FP* e = ed + (nd_off + 4 * s + c) * 3;
+      g[0] += n * e[0];
+      g[1] += n * e[1];
+      g[2] += n * e[2];
+    }
+    ci[0] += g[0];
+    ci[1] += g[1];
+    ci[2] += g[2];
+    int j = nlist[gi * nnei + s];
+    if (j >= 0) {
+      atomicAdd(&fr[3 * (int64_t)j + 0], g[0]);
+      atomicAdd(&fr[3 * (int64_t)j + 1], g[1]);
+      atomicAdd(&fr[3 * (int64_t)j + 2], g[2]);
+    }
+  }
+  ci[0] = wave_sum(ci[0]);
+  ci[1] = wave_sum(ci[1]);
+  ci[2] = wave_sum(ci[2]);
+  if (lane == 0) {
+    atomicAdd(&fr[3 * li + 0], -ci[0]);
+    atomicAdd(&fr[3 * li + 1], -ci[1]);
+    atomicAdd(&fr[3 * li + 2], -ci[2]);
+  }
+}
+
+// Block partial of nine virial components -> partial[block*9 + k] (double).
+template <typename FP>
+__device__ inline void block_virial_partial(double v[9], double* __restrict__ partial) {
+  __shared__ double red[kAtomsPerBlock][9];
+  const int w = threadIdx.x / kWave, lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < 9; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) red[w][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 9) {
+    double s = 0.;
+    for (int q = 0; q < kAtomsPerBlock; ++q) s += red[q][threadIdx.x];
+    partial[(int64_t)blockIdx.x * 9 + threadIdx.x] = s;
+  }
+}
+
+// prod_virial_a (source/lib/src/prod_virial.cc:22-70):
+// V[d0][d1] += nd * rij[d1] * ed[d0] over valid neighbor slots.
+template <typename FP>
+__global__ void __launch_bounds__(256)
+    k_prod_virial(double* __restrict__ partial, FP* __restrict__ atom_virial, const FP* __restrict__ nd,
+                  const FP* __restrict__ ed, const FP* __restrict__ rij, const int* __restrict__ nlist, int nloc,
+                  int nall, int nnei, int blocks_per_frame) {
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int f = blockIdx.x / blocks_per_frame;
+  const int64_t li = (int64_t)(blockIdx.x % blocks_per_frame) * kAtomsPerBlock + w;
+  double v[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) v[k] = 0.;
+  if (li < nloc) {
+    const int64_t gi = (int64_t)f * nloc + li;
+    for (int s = lane; s < nnei; s += kWave) {
+      int j = nlist[gi * nnei + s];
+      if (j < 0) continue;
+      const FP* r = rij + (gi * nnei + s) * 3;
+      FP g[3] = {(FP)0., (FP)0., (FP)0.};
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        FP n = nd[gi * nnei * 4 + 4 * s + c];
+        const FP* e = ed + (gi * nnei * 4 + 4 * s + c) * 3;
+        g[0] += n * e[0];
+        g[1] += n * e[1];
+        g[2] += n * e[2];
+      }
+#pragma unroll
+      for (int d0 = 0; d0 < 3; ++d0)
+#pragma unroll
+        for (int d1 = 0; d1 < 3; ++d1) {
+          FP tmp = g[d0] * r[d1];
+          v[d0 * 3 + d1] += (double)tmp;
+          if (atom_virial) atomicAdd(&atom_virial[((int64_t)f * nall + j) * 9 + d0 * 3 + d1], tmp);
+        }
+    }
+  }
+  block_virial_partial<FP>(v, partial);
+}
+
+// Compensated, deterministic sum of per-block partials: one workgroup per
+// frame, 9 x 64 lanes; lane q of component k sums blocks q, q+64, ...
+// (Neumaier), then a fixed-order tree combines the 64 (sum, carry) pairs.
+template <typename FP>
+__global__ void __launch_bounds__(576)
+    k_virial_reduce(FP* __restrict__ out, const double* __restrict__ partial, int blocks_per_frame) {
+  const int k = threadIdx.x / kWave, lane = lane_id();
+  const int f = blockIdx.x;
+  const double* p = partial + (int64_t)f * blocks_per_frame * 9;
+  double s = 0., c = 0.;
+  for (int b = lane; b < blocks_per_frame; b += kWave) {
+    double x = p[(int64_t)b * 9 + k];
+    double t = s + x;
+    c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+    s = t;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    double so = __shfl_xor(s, off, kWave), co = __shfl_xor(c, off, kWave);
+    double lo_s = lane & off ? so : s, hi_s = lane & off ? s : so;
+    double t = lo_s + hi_s;
+    double e = (fabs(lo_s) >= fabs(hi_s)) ? ((lo_s - t) + hi_s) : ((hi_s - t) + lo_s);
+    c = (lane & off ? co + c : c + co) + e;
+    s = t;
+  }
+  if (lane == 0) out[(int64_t)f * 9 + k] = (FP)(s + c);
+}
+
+// Fused backward of the env-mat for the model path: em_deriv is recomputed
+// from coordinates instead of being stored, forces on extended atoms are
+// folded onto their local owners through `mapping` (the ghost -> local
+// scatter of communicate_extended_output), and the virial partials are
+// formed in the same pass.  net_deriv is dE/d em (normalised em).
+template <typename FP, typename ACC>
+__global__ void __launch_bounds__(256)
+    k_force_virial_recompute(ACC* __restrict__ force, double* __restrict__ partial, const FP* __restrict__ nd,
+                             const int* __restrict__ nlist, const int* __restrict__ mapping,
+                             const double* __restrict__ ext_c, const int* __restrict__ ext_t, int nloc,
+                             const FP* __restrict__ std_, float rmin, float rmax, SecParams sp) {
+  __shared__ int lim_all[kAtomsPerBlock][kMaxTypes];
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int i = blockIdx.x * kAtomsPerBlock + w;
+  const bool active = i < nloc;
+  const int nnei = sp.nnei;
+  const int* nl = nlist + (int64_t)(active ? i : 0) * nnei;
+  section_limits(nl, sp, lim_all[w]);
+  double v[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) v[k] = 0.;
+  if (active) {
+    const int ti = ext_t[i] < sp.ntypes ? ext_t[i] : -1;
+    const double xi0 = ext_c[3 * i + 0], xi1 = ext_c[3 * i + 1], xi2 = ext_c[3 * i + 2];
+    ACC ci[3] = {0, 0, 0};
+    for (int s = lane; s < nnei && ti >= 0; s += kWave) {
+      int t = slot_type(sp, s);
+      if (s >= lim_all[w][t]) continue;
+      int j = nl[s];
+      double r64[3] = {ext_c[3 * j + 0] - xi0, ext_c[3 * j + 1] - xi1, ext_c[3 * j + 2] - xi2};
+      FP rr[3] = {(FP)r64[0], (FP)r64[1], (FP)r64[2]};
+      FP val[4], d[12];
+      env_one(rr, rmin, rmax, val, d);
+      const FP* sd = std_ + ((int64_t)ti * nnei + s) * 4;
+      const FP* n = nd + ((int64_t)i * nnei + s) * 4;
+      FP g[3] = {(FP)0., (FP)0., (FP)0.};
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        FP nc = n[c] / sd[c];
+        g[0] += nc * d[3 * c + 0];
+        g[1] += nc * d[3 * c + 1];
+        g[2] += nc * d[3 * c + 2];
+      }
+      ci[0] += g[0];
+      ci[1] += g[1];
+      ci[2] += g[2];
+      int o = mapping ? mapping[j] : j;
+      atomicAdd(&force[3 * (int64_t)o + 0], (ACC)g[0]);
+      atomicAdd(&force[3 * (int64_t)o + 1], (ACC)g[1]);
+      atomicAdd(&force[3 * (int64_t)o + 2], (ACC)g[2]);
+#pragma unroll
+      for (int d0 = 0; d0 < 3; ++d0)
+#pragma unroll
+        for (int d1 = 0; d1 < 3; ++d1) v[d0 * 3 + d1] += (double)g[d0] * r64[d1];
+    }
+    ci[0] = wave_sum(ci[0]);
+    ci[1] = wave_sum(ci[1]);
+    ci[2] = wave_sum(ci[2]);
+    if (lane == 0) {
+      atomicAdd(&force[3 * (int64_t)i + 0], -ci[0]);
+      atomicAdd(&force[3 * (int64_t)i + 1], -ci[1]);
+      atomicAdd(&force[3 * (int64_t)i + 2], -ci[2]);
+    }
+  }
+  block_virial_partial<FP>(v, partial);
+}
+
+template <typename FP>
+int env_mat_impl(FP* em, FP* em_deriv, FP* rij, const int* nlist, const FP* ext_coord, const int* ext_type,
+                 int64_t nloc, const FP* avg, const FP* std_, float rcut, float rcut_smth, const int* sel_host,
+                 int ntypes, void* stream) {
+  SecParams sp;
+  int rc = make_sec(&sp, sel_host, ntypes);
+  if (rc) return rc;
+  DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
+  DPMI_CHECK_ARG(em != nullptr, "em is NULL");
+  if (nloc == 0) return DPMI_OK;
+  hipLaunchKernelGGL(k_env_mat_a<FP>, dim3(ceil_div(nloc, kAtomsPerBlock)), dim3(256), 0, as_stream(stream), em,
+                     em_deriv, rij, nlist, ext_coord, ext_type, (int)nloc, avg, std_, rcut_smth, rcut, sp);
+  DPMI_LAUNCH_CHECK();
+  return DPMI_OK;
+}
+
+template <typename FP>
+int prod_force_impl(FP* force, const FP* nd, const FP* ed, const int* nlist, int64_t nloc, int64_t nall, int64_t nnei,
+                    int64_t nframes, void* stream) {
+  DPMI_CHECK_ARG(nloc >= 0 && nall >= nloc && nnei >= 0 && nframes >= 0, "bad sizes");
+  hipStream_t st = as_stream(stream);
+  int64_t nf = nframes * nall * 3;
+  if (nf > 0) hipLaunchKernelGGL(k_fill<FP>, dim3(ceil_div(nf, 256)), dim3(256), 0, st, force, nf, (FP)0.);
+  int64_t natoms = nloc * nframes;
+  if (natoms > 0)
+    hipLaunchKernelGGL(k_prod_force<FP>, dim3(ceil_div(natoms, kAtomsPerBlock)), dim3(256), 0, st, force, nd, ed,
+                       nlist, (int)nloc, (int)nall, (int)nnei, (int)nframes);
+  DPMI_LAUNCH_CHECK();
+  return DPMI_OK;
+}
+
+int64_t virial_ws(int64_t nloc, int64_t nframes) {
+  return std::max<int64_t>(1, nframes) * std::max<int64_t>(1, ceil_div(nloc, kAtomsPerBlock)) * 9 * 8;
+}
+
+template <typename FP>
+int prod_virial_impl(FP* virial, FP* atom_virial, const FP* nd, const FP* ed, const FP* rij, const int* nlist,
+                     int64_t nloc, int64_t nall, int64_t nnei, int64_t nframes, void* ws, int64_t ws_bytes,
+                     void* stream) {
+  DPMI_CHECK_ARG(nloc >= 0 && nall >= nloc && nnei >= 0 && nframes >= 0, "bad sizes");
+  DPMI_CHECK_ARG(ws != nullptr && ws_bytes >= virial_ws(nloc, nframes), "workspace too small");
+  hipStream_t st = as_stream(stream);
+  if (nframes == 0) return DPMI_OK;
+  if (atom_virial) {
+    int64_t na = nframes * nall * 9;
+    if (na > 0) hipLaunchKernelGGL(k_fill<FP>, dim3(ceil_div(na, 256)), dim3(256), 0, st, atom_virial, na, (FP)0.);
+  }
+  int bpf = (int)std::max<int64_t>(1, ceil_div(nloc, kAtomsPerBlock));
+  double* partial = static_cast<double*>(ws);
+  hipLaunchKernelGGL(k_prod_virial<FP>, dim3(bpf * nframes), dim3(256), 0, st, partial, atom_virial, nd, ed, rij,
+                     nlist, (int)nloc, (int)nall, (int)nnei, bpf);
+  hipLaunchKernelGGL(k_virial_reduce<FP>, dim3(nframes), dim3(576), 0, st, virial, partial, bpf);
+  DPMI_LAUNCH_CHECK();
+  return DPMI_OK;
+}
+
+}  // namespace
+}  // namespace dpmi
+
+using namespace dpmi;
+
+extern "C" int dpmi_prod_env_mat_a_f64(double* em, double* em_deriv, double* rij, const int* nlist,
+                                       const double* ext_coord, const int* ext_type, int64_t nloc, const double* avg,
+                                       const double* std_, float rcut, float rcut_smth, const int* sel_host,
+                                       int ntypes, void* stream) {
+  return env_mat_impl<double>(em, em_deriv, rij, nlist, ext_coord, ext_type, nloc, avg, std_, rcut, rcut_smth,
+                              sel_host, ntypes, stream);
+}
+
+extern "C" int dpmi_prod_env_mat_a_f32(float* em, float* em_deriv, float* rij, const int* nlist,
+                                       const float* ext_coord, const int* ext_type, int64_t nloc, const float* avg,
+                                       const float* std_, float rcut, float rcut_smth, const int* sel_host,
+                                       int ntypes, void* stream) {
+  return env_mat_impl<float>(em, em_deriv, rij, nlist, ext_coord, ext_type, nloc, avg, std_, rcut, rcut_smth,
+                             sel_host, ntypes, stream);
+}
+
+extern "C" int dpmi_prod_force_se_a_f64(double* force, const double* net_deriv, const double* in_deriv,
+                                        const int* nlist, int64_t nloc, int64_t nall, int64_t nnei, int64_t nframes,
+                                        void* stream) {
+  return prod_force_impl<double>(force, net_deriv, in_deriv, nlist, nloc, nall, nnei, nframes, stream);
+}
+
+extern "C" int dpmi_prod_force_se_a_f32(float* force, const float* net_deriv, const float* in_deriv,
+                                        const int* nlist, int64_t nloc, int64_t nall, int64_t nnei, int64_t nframes,
+                                        void* stream) {
+  return prod_force_impl<float>(force, net_deriv, in_deriv, nlist, nloc, nall, nnei, nframes, stream);
+}
+
+extern "C" int dpmi_prod_virial_workspace_bytes(int64_t* bytes, int64_t nloc, int64_t nframes) {
+  DPMI_CHECK_ARG(bytes != nullptr, "bytes is NULL");
+  *bytes = virial_ws(nloc, nframes);
+  return DPMI_OK;
+}
+
+extern "C" int dpmi_prod_virial_se_a_f64(double* virial, double* atom_virial, const double* net_deriv,
+                                         const double* in_deriv, const double* rij, const int* nlist, int64_t nloc,
+                                         int64_t nall, int64_t nnei, int64_t nframes, void* workspace,
+                                         int64_t workspace_bytes, void* stream) {
+  return prod_virial_impl<double>(virial, atom_virial, net_deriv, in_deriv, rij, nlist, nloc, nall, nnei, nframes,
+                                  workspace, workspace_bytes, stream);
+}
+
+extern "C" int dpmi_prod_virial_se_a_f32(float* virial, float* atom_virial, const float* net_deriv,
+                                         const float* in_deriv, const float* rij, const int* nlist, int64_t nloc,
+                                         int64_t nall, int64_t nnei, int64_t nframes, void* workspace,
+                                         int64_t workspace_bytes, void* stream) {
+  return prod_virial_impl<float>(virial, atom_virial, net_deriv, in_deriv, rij, nlist, nloc, nall, nnei, nframes,
+                                 workspace, workspace_bytes, stream);
+}
+
+template <typename FP>
+static int force_virial_recompute_impl(double* force, double* virial, const FP* net_deriv, const int* nlist,
+                                       const int* mapping, const double* ext_coord, const int* ext_type,
+                                       int64_t nloc, int64_t nall, const FP* std_, float rcut, float rcut_smth,
+                                       const int* sel_host, int ntypes, void* workspace, int64_t workspace_bytes,
+                                       void* stream) {
+  SecParams sp;
+  int rc = make_sec(&sp, sel_host, ntypes);
+  if (rc) return rc;
+  DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31) && nall >= nloc, "nloc/nall out of range");
+  DPMI_CHECK_ARG(workspace != nullptr && workspace_bytes >= virial_ws(nloc, 1), "workspace too small");
+  hipStream_t st = as_stream(stream);
+  const int64_t nout = (mapping ? nloc : nall) * 3;
+  if (nout > 0) hipLaunchKernelGGL(k_fill<double>, dim3(ceil_div(nout, 256)), dim3(256), 0, st, force, nout, 0.);
+  int bpf = (int)std::max<int64_t>(1, ceil_div(nloc, kAtomsPerBlock));
+  double* partial = static_cast<double*>(workspace);
+  hipLaunchKernelGGL((k_force_virial_recompute<FP, double>), dim3(bpf), dim3(256), 0, st, force, partial, net_deriv,
+                     nlist, mapping, ext_coord, ext_type, (int)nloc, std_, rcut_smth, rcut, sp);
+  if (virial) hipLaunchKernelGGL(k_virial_reduce<double>, dim3(1), dim3(576), 0, st, virial, partial, bpf);
+  DPMI_LAUNCH_CHECK();
+  return DPMI_OK;
+}
+
+extern "C" int dpmi_force_virial_recompute_f64(double* force, double* virial, const double* net_deriv,
+                                               const int* nlist, const int* mapping, const double* ext_coord,
+                                               const int* ext_type, int64_t nloc, int64_t nall, const double* std_,
+                                               float rcut, float rcut_smth, const int* sel_host, int ntypes,
+                                               void* workspace, int64_t workspace_bytes, void* stream) {
+  return force_virial_recompute_impl<double>(force, virial, net_deriv, nlist, mapping, ext_coord, ext_type, nloc,
+                                             nall, std_, rcut, rcut_smth, sel_host, ntypes, workspace,
+                                             workspace_bytes, stream);
+}
+
+extern "C" int dpmi_force_virial_recompute_f32(double* force, double* virial, const float* net_deriv,
+                                               const int* nlist, const int* mapping, const double* ext_coord,
+                                               const int* ext_type, int64_t nloc, int64_t nall, const float* std_,
+                                               float rcut, float rcut_smth, const int* sel_host, int ntypes,
+                                               void* workspace, int64_t workspace_bytes, void* stream) {
+  return force_virial_recompute_impl<float>(force, virial, net_deriv, nlist, mapping, ext_coord, ext_type, nloc,
+                                            nall, std_, rcut, rcut_smth, sel_host, ntypes, workspace,
+                                            workspace_bytes, stream);
+}

@@ -1,0 +1,681 @@
+// Periodic cell-list neighbor search for se_e2_a on gfx950.
+//
+// Pipeline (one stream, no host round trip except reading nall once):
+//   1. normalize + bin   one thread per local atom: wrap into the box exactly
+//                        as normalize_coord_cpu does, assign the build_clist
+//                        cell, count atoms per cell (atomics).
+//   2. scan              one workgroup: exclusive prefix sums of the cell
+//                        counts and of the ghost-cell counts in the
+//                        (x, y, z) loop order of copy_coord; gives nall.
+//   3. order             stable per-cell order by atom index (rank = number
+//                        of cell members with a smaller index), so the ghost
+//                        layout equals copy_coord's push_back order.
+//   4. ghosts            one wavefront per extended cell writes its images.
+//   5. search + format   one wavefront per local atom scans the 27 (or 125)
+//                        neighbouring cells, filters with the double test of
+//                        build_nlist_cpu and the float test of
+//                        format_nlist_i_cpu, compacts candidates into LDS with
+//                        ballot/mbcnt prefix sums, ranks them by the 64-bit key
+//                        (type, float r^2 bits, extended index) and writes the
+//                        type sections of the formatted row.
+//
+// Bit-exactness: every floating-point expression follows the reference
+// evaluation order and this file is compiled with -ffp-contract=off, so no
+// multiply-add is fused; divisions and sqrt are correctly rounded on gfx950.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "dpmi_internal.h"
+
+namespace dpmi {
+namespace {
+
+constexpr int kMaxTypes = 64;
+constexpr int kMaxGridNoPbc = 64;
+
+struct GridParams {
+  double box[9];
+  double rec[9];
+  double lo[3];  // open systems: grid origin
+  double cs[3];  // open systems: cell width
+  int ncell[3];
+  int ngcell[3];
+  int ext[3];
+  int nsearch;
+  int pbc;
+  int nloc;
+  long long lay_nlc;   // cell counts the workspace layout was sized for
+  long long lay_next;
+};
+
+struct SelParams {
+  int ntypes;
+  int nnei;
+  int sel[kMaxTypes];
+  int sec[kMaxTypes + 1];
+};
+
+// ---------------------------------------------------------------- host geometry
+// Same formulas as the reference region code (source/lib/src/region.cc:35-122,
+// SimulationRegion_Impl.h:344-470); x86-64 host code has no FMA, so these
+// scalars are bit-identical to the reference's.
+double box_volume(const double* b) {
+  double v = b[0] * (b[4] * b[8] - b[7] * b[5]) - b[1] * (b[3] * b[8] - b[6] * b[5]) +
+             b[2] * (b[3] * b[7] - b[6] * b[4]);
+  return std::fabs(v);
+}
+
+void rec_box(const double* b, double* rec) {
+  double vi = 1. / box_volume(b);
+  rec[0] = (b[4] * b[8] - b[7] * b[5]) * vi;
+  rec[4] = (b[0] * b[8] - b[6] * b[2]) * vi;
+  rec[8] = (b[0] * b[4] - b[3] * b[1]) * vi;
+  rec[1] = (-b[3] * b[8] + b[6] * b[5]) * vi;
+  rec[2] = (b[3] * b[7] - b[6] * b[4]) * vi;
+  rec[3] = (-b[1] * b[8] + b[7] * b[2]) * vi;
+  rec[5] = (-b[0] * b[7] + b[6] * b[1]) * vi;
+  rec[6] = (b[1] * b[5] - b[4] * b[2]) * vi;
+  rec[7] = (-b[0] * b[5] + b[3] * b[2]) * vi;
+}
+
+void face_distance(const double* b, double* dd) {
+  double vol = box_volume(b);
+  auto cp = [](const double* a, const double* c, double* o) {
+    o[0] = a[1] * c[2] - a[2] * c[1];
+    o[1] = a[2] * c[0] - a[0] * c[2];
+    o[2] = a[0] * c[1] - a[1] * c[0];
+  };
+  double t[3];
+  cp(b + 3, b + 6, t);
+  dd[0] = vol * (1. / std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]));
+  cp(b + 6, b + 0, t);
+  dd[1] = vol * (1. / std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]));
+  cp(b + 0, b + 3, t);
+  dd[2] = vol * (1. / std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]));
+}
+
+// Cell grid of copy_coord (neighbor_list.cc:756-770): ncell = int(face / rc),
+// ngcell = int(rc / cell_size) + 1, rc being the float cutoff as a double.
+GridParams pbc_grid(const double* box, float rcut, int nloc) {
+  GridParams g;
+  std::memset(&g, 0, sizeof(g));
+  std::memcpy(g.box, box, sizeof(double) * 9);
+  rec_box(box, g.rec);
+  double to_face[3], rc = (double)rcut;
+  face_distance(box, to_face);
+  g.nsearch = 1;
+  for (int d = 0; d < 3; ++d) {
+    g.ncell[d] = (int)(to_face[d] / rc);
+    if (g.ncell[d] == 0) g.ncell[d] = 1;
+    double cell_size = to_face[d] / g.ncell[d];
+    g.ngcell[d] = (int)(rc / cell_size) + 1;
+    g.ext[d] = g.ncell[d] + 2 * g.ngcell[d];
+    // a cell only just wider than rc leaves no margin for rounding at a cell
+    // boundary: search two cells out there (never needed for C0-C4).
+    if (cell_size - rc < 1e-6 * to_face[d]) g.nsearch = 2;
+  }
+  g.pbc = 1;
+  g.nloc = nloc;
+  return g;
+}
+
+struct WsLayout {
+  int64_t params, misc, bbox, cell_of, cell_count, cell_start, cell_cursor, cell_unordered, cell_atoms, ghost_off,
+      total;
+};
+
+__host__ __device__ inline int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
+
+// Header (grid parameters, status words, bounding box) at fixed offsets, then
+// the per-atom and per-cell arrays.  Host and device derive the same offsets
+// from (nloc, nlc, next), so kernels need only the workspace base pointer.
+__host__ __device__ inline WsLayout ws_layout(int64_t nloc, int64_t nlc, int64_t next) {
+  WsLayout w;
+  int64_t na = nloc > 0 ? nloc : 1;
+  int64_t o = 0;
+  w.params = o;
+  o = align256(o + (int64_t)sizeof(GridParams));
+  w.misc = o;
+  o = align256(o + 4 * 16);
+  w.bbox = o;
+  o = align256(o + 8 * 8);
+  w.cell_of = o;
+  o = align256(o + 4 * na);
+  w.cell_count = o;
+  o = align256(o + 4 * nlc);
+  w.cell_start = o;
+  o = align256(o + 4 * (nlc + 1));
+  w.cell_cursor = o;
+  o = align256(o + 4 * nlc);
+  w.cell_unordered = o;
+  o = align256(o + 4 * na);
+  w.cell_atoms = o;
+  o = align256(o + 4 * na);
+  w.ghost_off = o;
+  o = align256(o + 4 * (next + 1));
+  w.total = o;
+  return w;
+}
+
+bool grid_sizes(const double* box_host, int64_t nloc, float rcut, int64_t* nlc, int64_t* next) {
+  if (box_host) {
+    GridParams g = pbc_grid(box_host, rcut, (int)nloc);
+    *nlc = (int64_t)g.ncell[0] * g.ncell[1] * g.ncell[2];
+    *next = (int64_t)g.ext[0] * g.ext[1] * g.ext[2];
+  } else {
+    *nlc = (int64_t)kMaxGridNoPbc * kMaxGridNoPbc * kMaxGridNoPbc;
+    *next = *nlc;
+  }
+  return *nlc > 0 && *next < (int64_t(1) << 31);
+}
+
+// ---------------------------------------------------------------- device helpers
+__device__ inline void to_inter(double* ri, const double* rec, const double* rp) {
+  ri[0] = rp[0] * rec[0] + rp[1] * rec[1] + rp[2] * rec[2];
+  ri[1] = rp[0] * rec[3] + rp[1] * rec[4] + rp[2] * rec[5];
+  ri[2] = rp[0] * rec[6] + rp[1] * rec[7] + rp[2] * rec[8];
+}
+
+__device__ inline void to_phys(double* rp, const double* b, const double* ri) {
+  rp[0] = ri[0] * b[0] + ri[1] * b[3] + ri[2] * b[6];
+  rp[1] = ri[0] * b[1] + ri[1] * b[4] + ri[2] * b[7];
+  rp[2] = ri[0] * b[2] + ri[1] * b[5] + ri[2] * b[8];
+}
+
+// compute_pbc_shift (neighbor_list.cc:727-743)
+__device__ inline int pbc_shift(int idx, int n) {
+  int s = 0;
+  if (idx < 0) {
+    s = 1;
+    while (idx + s * n < 0) s++;
+  } else if (idx >= n) {
+    s = -1;
+    while (idx + s * n >= n) s--;
+  }
+  return s;
+}
+
+__global__ void k_fill_int(int* p, int64_t n, int v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_write_params(GridParams p, GridParams* dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = p;
+}
+
+// Bounding box of an open system (one workgroup).
+__global__ void __launch_bounds__(1024) k_bbox(const double* c, int n, double* out) {
+  __shared__ double red[6][1024 / kWave];
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    for (int d = 0; d < 3; ++d) {
+      double v = c[3 * i + d];
+      lo[d] = fmin(lo[d], v);
+      hi[d] = fmax(hi[d], v);
+    }
+  for (int off = 32; off > 0; off >>= 1)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fmin(lo[d], __shfl_xor(lo[d], off, kWave));
+      hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, kWave));
+    }
+  int w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  if (lane_id() == 0)
+    for (int d = 0; d < 3; ++d) {
+      red[d][w] = lo[d];
+      red[3 + d][w] = hi[d];
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int d = 0; d < 3; ++d) {
+      double a = red[d][0], b = red[3 + d][0];
+      for (int k = 1; k < nw; ++k) {
+        a = fmin(a, red[d][k]);
+        b = fmax(b, red[3 + d][k]);
+      }
+      out[d] = a;
+      out[3 + d] = b;
+    }
+  }
+}
+
+// Step 1: normalize_coord_cpu (coord.cc:13-28) + build_clist cell index
+// (neighbor_list.cc:61-111 with nat_stt = ext_stt = 0, grid = ncell).
+__global__ void k_normalize_bin(const double* __restrict__ in_c, const int* __restrict__ in_t,
+                                double* __restrict__ out_c, int* __restrict__ out_t, int* __restrict__ mapping,
+                                int* __restrict__ cell_of, int* __restrict__ cell_count, GridParams p) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.nloc) return;
+  double x[3] = {in_c[3 * i + 0], in_c[3 * i + 1], in_c[3 * i + 2]};
+  int idx[3];
+  if (p.pbc) {
+    double ri[3];
+    to_inter(ri, p.rec, x);
+    for (int d = 0; d < 3; ++d) {
+      ri[d] = fmod(ri[d], 1.);
+      if (ri[d] < 0.) ri[d] += 1.;
+    }
+    to_phys(x, p.box, ri);
+    double inter[3];
+    to_inter(inter, p.rec, x);
+    for (int d = 0; d < 3; ++d) {
+      double cell_size = 1. / p.ncell[d];
+      idx[d] = (int)((inter[d] - 0.) / cell_size);
+      if (inter[d] - 0. < 0.) idx[d]--;
+      if (idx[d] < 0) idx[d] = 0;
+      else if (idx[d] >= p.ncell[d]) idx[d] = p.ncell[d] - 1;
+    }
+  } else {
+    for (int d = 0; d < 3; ++d) {
+      idx[d] = (int)floor((x[d] - p.lo[d]) / p.cs[d]);
+      if (idx[d] < 0) idx[d] = 0;
+      else if (idx[d] >= p.ncell[d]) idx[d] = p.ncell[d] - 1;
+    }
+  }
+  int c = (idx[0] * p.ncell[1] + idx[1]) * p.ncell[2] + idx[2];
+  out_c[3 * i + 0] = x[0];
+  out_c[3 * i + 1] = x[1];
+  out_c[3 * i + 2] = x[2];
+  out_t[i] = in_t[i];
+  mapping[i] = i;
+  cell_of[i] = c;
+  atomicAdd(&cell_count[c], 1);
+}
+
+// Block-wide exclusive scan helper (blockDim = 1024 = 16 waves).
+__device__ int block_exclusive_scan(int v, int* total, int* smem) {
+  int lane = lane_id(), w = threadIdx.x / kWave;
+  int incl = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    int t = __shfl_up(incl, off, kWave);
+    if (lane >= off) incl += t;
+  }
+  if (lane == kWave - 1) smem[w] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int k = 0; k < (int)(blockDim.x / kWave); ++k) {
+      int t = smem[k];
+      smem[k] = run;
+      run += t;
+    }
+    smem[16] = run;
+  }
+  __syncthreads();
+  int excl = smem[w] + incl - v;
+  *total = smem[16];
+  __syncthreads();
+  return excl;
+}
+
+// Step 2: prefix sums of local cell counts (cell_start) and of ghost counts in
+// the ii[0], ii[1], ii[2] loop order of copy_coord (ghost_off); misc[0] = nall.
+__global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ cell_count, int* __restrict__ cell_start,
+                                               int* __restrict__ ghost_off, int* __restrict__ misc, GridParams p) {
+  __shared__ int smem[32];
+  int nlc = p.ncell[0] * p.ncell[1] * p.ncell[2];
+  int carry = 0;
+  for (int base = 0; base < nlc; base += blockDim.x) {
+    int c = base + threadIdx.x;
+    int v = c < nlc ? cell_count[c] : 0;
+    int tot;
+    int ex = block_exclusive_scan(v, &tot, smem);
+    if (c < nlc) cell_start[c] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) cell_start[nlc] = carry;
+  int next = p.ext[0] * p.ext[1] * p.ext[2];
+  int gcarry = 0;
+  for (int base = 0; base < next; base += blockDim.x) {
+    int e = base + threadIdx.x;
+    int v = 0;
+    if (e < next && p.pbc) {
+      int e2 = e % p.ext[2], e1 = (e / p.ext[2]) % p.ext[1], e0 = e / (p.ext[1] * p.ext[2]);
+      int ii[3] = {e0 - p.ngcell[0], e1 - p.ngcell[1], e2 - p.ngcell[2]};
+      bool local = true;
+      int jj[3];
+      for (int d = 0; d < 3; ++d) {
+        if (ii[d] < 0 || ii[d] >= p.ncell[d]) local = false;
+        jj[d] = ii[d] + pbc_shift(ii[d], p.ncell[d]) * p.ncell[d];
+      }
+      if (!local) v = cell_count[(jj[0] * p.ncell[1] + jj[1]) * p.ncell[2] + jj[2]];
+    }
+    int tot;
+    int ex = block_exclusive_scan(v, &tot, smem);
+    if (e < next) ghost_off[e] = gcarry + ex;
+    gcarry += tot;
+  }
+  if (threadIdx.x == 0) {
+    ghost_off[next] = gcarry;
+    misc[0] = p.nloc + gcarry;
+  }
+}
+
+// Step 3a: unordered bucket fill.
+__global__ void k_cell_fill(const int* __restrict__ cell_of, const int* __restrict__ cell_start,
+                            int* __restrict__ cursor, int* __restrict__ unordered, int nloc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nloc) return;
+  int c = cell_of[i];
+  int slot = atomicAdd(&cursor[c], 1);
+  unordered[cell_start[c] + slot] = i;
+}
+
+// Step 3b: stable order inside each cell (index order, as build_clist pushes).
+__global__ void k_cell_order(const int* __restrict__ cell_of, const int* __restrict__ cell_start,
+                             const int* __restrict__ unordered, int* __restrict__ cell_atoms, int nloc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nloc) return;
+  int c = cell_of[i];
+  int b = cell_start[c], e = cell_start[c + 1];
+  int rank = 0;
+  for (int k = b; k < e; ++k) rank += unordered[k] < i;
+  cell_atoms[b + rank] = i;
+}
+
+// Step 4: ghost images, one wavefront per extended cell (copy_coord,
+// neighbor_list.cc:796-824: coordinate minus inter2Phys(shift)).
+__global__ void __launch_bounds__(64) k_ghosts(double* __restrict__ out_c, int* __restrict__ out_t,
+                                               int* __restrict__ mapping, const int* __restrict__ cell_start,
+                                               const int* __restrict__ cell_atoms, const int* __restrict__ ghost_off,
+                                               GridParams p) {
+  int e = blockIdx.x;
+  int e2 = e % p.ext[2], e1 = (e / p.ext[2]) % p.ext[1], e0 = e / (p.ext[1] * p.ext[2]);
+  int ii[3] = {e0 - p.ngcell[0], e1 - p.ngcell[1], e2 - p.ngcell[2]};
+  bool local = true;
+  int jj[3];
+  double sd[3];
+  for (int d = 0; d < 3; ++d) {
+    if (ii[d] < 0 || ii[d] >= p.ncell[d]) local = false;
+    int s = pbc_shift(ii[d], p.ncell[d]);
+    sd[d] = (double)s;
+    jj[d] = ii[d] + s * p.ncell[d];
+  }
+  if (local) return;
+  double sv[3];
+  to_phys(sv, p.box, sd);
+  int c = (jj[0] * p.ncell[1] + jj[1]) * p.ncell[2] + jj[2];
+  int b = cell_start[c], cnt = cell_start[c + 1] - b;
+  int64_t base = (int64_t)p.nloc + ghost_off[e];
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+    int a = cell_atoms[b + k];
+    int64_t q = base + k;
+    out_c[3 * q + 0] = out_c[3 * a + 0] - sv[0];
+    out_c[3 * q + 1] = out_c[3 * a + 1] - sv[1];
+    out_c[3 * q + 2] = out_c[3 * a + 2] - sv[2];
+    out_t[q] = out_t[a];
+    mapping[q] = a;
+  }
+}
+
+// Step 5: candidates + format, one wavefront per local atom.
+// key = type << 56 | float-bits(r^2) << 24 | extended index (nall < 2^24).
+template <int CAP>
+__global__ void __launch_bounds__(64) k_search_format(
+    int* __restrict__ nlist, int* __restrict__ overflow, int* __restrict__ nlist_mapped,
+    const double* __restrict__ ext_c, const int* __restrict__ ext_t, const int* __restrict__ mapping,
+    char* __restrict__ ws, SelParams sp, float rcut) {
+  extern __shared__ unsigned char smem_raw[];
+  uint64_t* keys = reinterpret_cast<uint64_t*>(smem_raw);
+  int* row = reinterpret_cast<int*>(keys + CAP);
+  const GridParams p = *reinterpret_cast<const GridParams*>(ws);
+  const WsLayout L = ws_layout(p.nloc, p.lay_nlc, p.lay_next);
+  const int* __restrict__ cell_of = reinterpret_cast<const int*>(ws + L.cell_of);
+  const int* __restrict__ cell_start = reinterpret_cast<const int*>(ws + L.cell_start);
+  const int* __restrict__ cell_atoms = reinterpret_cast<const int*>(ws + L.cell_atoms);
+  const int* __restrict__ ghost_off = reinterpret_cast<const int*>(ws + L.ghost_off);
+  int* misc = reinterpret_cast<int*>(ws + L.misc);
+  const int i = blockIdx.x;
+  const int lane = lane_id();
+  const double rcut2d = (double)(rcut * rcut);
+  const float rcut2f = rcut * rcut;
+  const double xi0 = ext_c[3 * i + 0], xi1 = ext_c[3 * i + 1], xi2 = ext_c[3 * i + 2];
+  const float fi0 = (float)xi0, fi1 = (float)xi1, fi2 = (float)xi2;
+  int c = cell_of[i];
+  int c2 = c % p.ncell[2], c1 = (c / p.ncell[2]) % p.ncell[1], c0 = c / (p.ncell[1] * p.ncell[2]);
+  int n = 0;  // wave-uniform candidate count
+  const int ns = p.nsearch;
+  for (int d0 = -ns; d0 <= ns; ++d0) {
+    for (int d1 = -ns; d1 <= ns; ++d1) {
+      for (int d2 = -ns; d2 <= ns; ++d2) {
+        int ii0 = c0 + d0, ii1 = c1 + d1, ii2 = c2 + d2;
+        bool local;
+        int b, cnt;
+        int64_t gbase = 0;
+        if (p.pbc) {
+          if (ii0 < -p.ngcell[0] || ii0 >= p.ncell[0] + p.ngcell[0] || ii1 < -p.ngcell[1] ||
+              ii1 >= p.ncell[1] + p.ngcell[1] || ii2 < -p.ngcell[2] || ii2 >= p.ncell[2] + p.ngcell[2])
+            continue;
+          local = ii0 >= 0 && ii0 < p.ncell[0] && ii1 >= 0 && ii1 < p.ncell[1] && ii2 >= 0 && ii2 < p.ncell[2];
+          int j0 = ii0 + pbc_shift(ii0, p.ncell[0]) * p.ncell[0];
+          int j1 = ii1 + pbc_shift(ii1, p.ncell[1]) * p.ncell[1];
+          int j2 = ii2 + pbc_shift(ii2, p.ncell[2]) * p.ncell[2];
+          int cc = (j0 * p.ncell[1] + j1) * p.ncell[2] + j2;
+          b = cell_start[cc];
+          cnt = cell_start[cc + 1] - b;
+          if (!local) {
+            int e = ((ii0 + p.ngcell[0]) * p.ext[1] + (ii1 + p.ngcell[1])) * p.ext[2] + (ii2 + p.ngcell[2]);
+            gbase = (int64_t)p.nloc + ghost_off[e];
+          }
+        } else {
+          if (ii0 < 0 || ii0 >= p.ncell[0] || ii1 < 0 || ii1 >= p.ncell[1] || ii2 < 0 || ii2 >= p.ncell[2])
+            continue;
+          local = true;
+          int cc = (ii0 * p.ncell[1] + ii1) * p.ncell[2] + ii2;
+          b = cell_start[cc];
+          cnt = cell_start[cc + 1] - b;
+        }
+        for (int k0 = 0; k0 < cnt; k0 += kWave) {
+          int k = k0 + lane;
+          bool pass = false;
+          uint64_t key = 0;
+          if (k < cnt) {
+            int j = local ? cell_atoms[b + k] : (int)(gbase + k);
+            if (j != i) {
+              double xj0 = ext_c[3 * j + 0], xj1 = ext_c[3 * j + 1], xj2 = ext_c[3 * j + 2];
+              double e0 = xi0 - xj0, e1 = xi1 - xj1, e2 = xi2 - xj2;
+              double r2 = e0 * e0 + e1 * e1 + e2 * e2;
+              if (r2 < rcut2d) {
+                int tj = ext_t[j];
+                float f0 = (float)xj0 - fi0, f1 = (float)xj1 - fi1, f2 = (float)xj2 - fi2;
+                float rr2 = f0 * f0 + f1 * f1 + f2 * f2;
+                // types outside [0, ntypes) are not neighbors of any section
+                if (tj >= 0 && tj < sp.ntypes && rr2 <= rcut2f) {
+                  pass = true;
+                  key = ((uint64_t)tj << 56) | ((uint64_t)__float_as_uint(rr2) << 24) | (uint64_t)j;
+                }
+              }
+            }
+          }
+          uint64_t m = __ballot(pass);
+          int pos = n + mask_prefix(m);
+          if (pass && pos < CAP) keys[pos] = key;
+          n += __popcll(m);
+        }
+      }
+    }
+  }
+  if (n > CAP) {
+    if (lane == 0) atomicOr(&misc[1], 1);
+    n = CAP;
+  }
+  for (int s = lane; s < sp.nnei; s += kWave) row[s] = -1;
+  __syncthreads();
+  int ovf = -1;
+  for (int k = lane; k < n; k += kWave) {
+    uint64_t kk = keys[k];
+    int t = (int)(kk >> 56);
+    uint64_t tbase = (uint64_t)t << 56;
+    uint64_t rel = kk - tbase;
+    int rank = 0;
+    for (int m = 0; m < n; ++m) rank += (keys[m] - tbase) < rel;
+    if (rank < sp.sel[t])
+      row[sp.sec[t] + rank] = (int)(kk & 0xFFFFFFull);
+    else
+      ovf = max(ovf, t);
+  }
+  __syncthreads();
+  int64_t ro = (int64_t)i * sp.nnei;
+  for (int s = lane; s < sp.nnei; s += kWave) {
+    int j = row[s];
+    nlist[ro + s] = j;
+    if (nlist_mapped) nlist_mapped[ro + s] = j >= 0 ? mapping[j] : -1;
+  }
+  for (int off = 32; off > 0; off >>= 1) ovf = max(ovf, __shfl_xor(ovf, off, kWave));
+  if (overflow && lane == 0) overflow[i] = ovf;
+}
+
+}  // namespace
+
+int nlist_workspace(int64_t* bytes, const double* box_host, int64_t nloc, float rcut) {
+  int64_t nlc, next;
+  DPMI_CHECK_ARG(nloc >= 0, "nloc must be >= 0");
+  DPMI_CHECK_ARG(rcut > 0.f, "rcut must be positive");
+  DPMI_CHECK_ARG(grid_sizes(box_host, nloc, rcut, &nlc, &next), "degenerate box");
+  *bytes = ws_layout(nloc, nlc, next).total;
+  return DPMI_OK;
+}
+
+}  // namespace dpmi
+
+using namespace dpmi;
+
+extern "C" int dpmi_nlist_workspace_bytes(int64_t* bytes, const double* box_host, int64_t nloc, float rcut) {
+  DPMI_CHECK_ARG(bytes != nullptr, "bytes is NULL");
+  return nlist_workspace(bytes, box_host, nloc, rcut);
+}
+
+extern "C" int dpmi_copy_coord_f64(double* out_coord, int* out_type, int* mapping, int64_t* nall, int64_t mem_nall,
+                                   const double* in_coord, const int* in_type, int64_t nloc,
+                                   const double* box_host, float rcut, void* workspace, int64_t workspace_bytes,
+                                   void* stream) {
+  DPMI_CHECK_ARG(nall != nullptr, "nall is NULL");
+  DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 24), "nloc out of range [0, 2^24)");
+  DPMI_CHECK_ARG(mem_nall >= nloc, "mem_nall must be >= nloc");
+  int64_t need;
+  int rc = nlist_workspace(&need, box_host, nloc, rcut);
+  if (rc) return rc;
+  DPMI_CHECK_ARG(workspace != nullptr && workspace_bytes >= need, "workspace too small");
+  hipStream_t st = as_stream(stream);
+  int64_t nlc, next;
+  grid_sizes(box_host, nloc, rcut, &nlc, &next);
+  WsLayout L = ws_layout(nloc, nlc, next);
+  char* ws = static_cast<char*>(workspace);
+  int* cell_of = reinterpret_cast<int*>(ws + L.cell_of);
+  int* cell_count = reinterpret_cast<int*>(ws + L.cell_count);
+  int* cell_start = reinterpret_cast<int*>(ws + L.cell_start);
+  int* cursor = reinterpret_cast<int*>(ws + L.cell_cursor);
+  int* unordered = reinterpret_cast<int*>(ws + L.cell_unordered);
+  int* cell_atoms = reinterpret_cast<int*>(ws + L.cell_atoms);
+  int* ghost_off = reinterpret_cast<int*>(ws + L.ghost_off);
+  int* misc = reinterpret_cast<int*>(ws + L.misc);
+  GridParams* gp = reinterpret_cast<GridParams*>(ws + L.params);
+  double* bbox = reinterpret_cast<double*>(ws + L.bbox);
+
+  GridParams g;
+  if (box_host) {
+    g = pbc_grid(box_host, rcut, (int)nloc);
+  } else {
+    std::memset(&g, 0, sizeof(g));
+    g.pbc = 0;
+    g.nloc = (int)nloc;
+    g.nsearch = 1;
+    double h[6] = {0, 0, 0, 0, 0, 0};
+    if (nloc > 0) {
+      hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, st, in_coord, (int)nloc, bbox);
+      DPMI_LAUNCH_CHECK();
+      DPMI_HIP(hipMemcpyAsync(h, bbox, sizeof(h), hipMemcpyDeviceToHost, st));
+      DPMI_HIP(hipStreamSynchronize(st));
+    }
+    for (int d = 0; d < 3; ++d) {
+      double extent = h[3 + d] - h[d];
+      int nc = (int)(extent / (double)rcut);
+      nc = std::max(1, std::min(nc, kMaxGridNoPbc));
+      g.ncell[d] = nc;
+      g.ext[d] = nc;
+      g.lo[d] = h[d];
+      g.cs[d] = std::max(extent / nc, (double)rcut) * (1.0 + 1e-12) + 1e-300;
+    }
+  }
+  g.lay_nlc = nlc;
+  g.lay_next = next;
+  const int64_t nlc_use = (int64_t)g.ncell[0] * g.ncell[1] * g.ncell[2];
+  const int64_t next_use = (int64_t)g.ext[0] * g.ext[1] * g.ext[2];
+  hipLaunchKernelGGL(k_write_params, dim3(1), dim3(1), 0, st, g, gp);
+  DPMI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nlc_use, 256)), dim3(256), 0, st, cell_count, nlc_use, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nlc_use, 256)), dim3(256), 0, st, cursor, nlc_use, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(16), 0, st, misc, (int64_t)16, 0);
+  DPMI_LAUNCH_CHECK();
+  if (nloc > 0) {
+    hipLaunchKernelGGL(k_normalize_bin, dim3(ceil_div(nloc, 256)), dim3(256), 0, st, in_coord, in_type, out_coord,
+                       out_type, mapping, cell_of, cell_count, g);
+    DPMI_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, cell_count, cell_start, ghost_off, misc, g);
+  DPMI_LAUNCH_CHECK();
+  if (nloc > 0) {
+    hipLaunchKernelGGL(k_cell_fill, dim3(ceil_div(nloc, 256)), dim3(256), 0, st, cell_of, cell_start, cursor,
+                       unordered, (int)nloc);
+    hipLaunchKernelGGL(k_cell_order, dim3(ceil_div(nloc, 256)), dim3(256), 0, st, cell_of, cell_start, unordered,
+                       cell_atoms, (int)nloc);
+    DPMI_LAUNCH_CHECK();
+  }
+  int h_nall = 0;
+  DPMI_HIP(hipMemcpyAsync(&h_nall, misc, sizeof(int), hipMemcpyDeviceToHost, st));
+  DPMI_HIP(hipStreamSynchronize(st));
+  *nall = h_nall;
+  if (h_nall > mem_nall) {
+    set_error("nall exceeds mem_nall");
+    return DPMI_ERR_CAPACITY;
+  }
+  DPMI_CHECK_ARG(h_nall < (1 << 24), "nall >= 2^24 is not supported by the 64-bit neighbor key");
+  if (g.pbc && next_use > 0 && h_nall > nloc) {
+    hipLaunchKernelGGL(k_ghosts, dim3(next_use), dim3(64), 0, st, out_coord, out_type, mapping, cell_start,
+                       cell_atoms, ghost_off, g);
+    DPMI_LAUNCH_CHECK();
+  }
+  return DPMI_OK;
+}
+
+extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped, const double* ext_coord,
+                                    const int* ext_type, const int* mapping, int64_t nloc, int64_t nall, float rcut,
+                                    const int* sel_host, int ntypes, const void* workspace, int64_t workspace_bytes,
+                                    void* stream) {
+  DPMI_CHECK_ARG(ntypes > 0 && ntypes <= kMaxTypes, "ntypes out of range [1, 64]");
+  DPMI_CHECK_ARG(sel_host != nullptr, "sel is NULL");
+  DPMI_CHECK_ARG(nall >= nloc && nall < (int64_t(1) << 24), "nall out of range");
+  DPMI_CHECK_ARG(nlist_mapped == nullptr || mapping != nullptr, "nlist_mapped needs mapping");
+  SelParams sp;
+  std::memset(&sp, 0, sizeof(sp));
+  sp.ntypes = ntypes;
+  sp.sec[0] = 0;
+  for (int t = 0; t < ntypes; ++t) {
+    DPMI_CHECK_ARG(sel_host[t] >= 0, "sel must be >= 0");
+    sp.sel[t] = sel_host[t];
+    sp.sec[t + 1] = sp.sec[t] + sel_host[t];
+  }
+  sp.nnei = sp.sec[ntypes];
+  DPMI_CHECK_ARG(sp.nnei <= 4096, "sum(sel) must be <= 4096");
+  if (nloc == 0) return DPMI_OK;
+  DPMI_CHECK_ARG(workspace != nullptr && workspace_bytes >= (int64_t)(512 + 64), "workspace too small");
+  hipStream_t st = as_stream(stream);
+  char* ws = const_cast<char*>(static_cast<const char*>(workspace));
+  int* misc = reinterpret_cast<int*>(ws + ws_layout(0, 0, 0).misc);
+  size_t shm = 1024 * sizeof(uint64_t) + sizeof(int) * sp.nnei;
+  hipLaunchKernelGGL(k_search_format<1024>, dim3(nloc), dim3(64), shm, st, nlist, overflow, nlist_mapped, ext_coord,
+                     ext_type, mapping, ws, sp, rcut);
+  DPMI_LAUNCH_CHECK();
+  int err = 0;
+  DPMI_HIP(hipMemcpyAsync(&err, misc + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  DPMI_HIP(hipStreamSynchronize(st));
+  if (err) {
+    set_error("more than 1024 candidates within rcut for some atom");
+    return DPMI_ERR_CAPACITY;
+  }
+  return DPMI_OK;
+}

@@ -1,0 +1,382 @@
+"""Kernel wrappers and the ``torch.ops.deepmd`` op surface.
+
+Registers, in namespace ``deepmd`` (the namespace deepmd/pt calls through):
+
+* ``tabulate_fusion_se_a(table, table_info, em_x, em, last_layer_size) -> Tensor[]``
+  — same schema, dtype dispatch (double/float by ``table.dtype``) and autograd
+  structure as source/op/pt/tabulate_multi_device.cc:605-830,1498-1552:
+  forward -> grad (w.r.t. em_x, em) -> grad_grad (w.r.t. dy only).
+* ``prod_env_mat_a``, ``prod_force_se_a``, ``prod_virial_se_a`` — the C++/TF
+  ops of source/op/tf/prod_env_mat_multi_device.cc:11-28,
+  prod_force_multi_device.cc:6-14, prod_virial_multi_device.cc:5-15, which
+  deepmd/pt lacks; argument meaning follows the TF ops (natoms[0] = nloc,
+  natoms[1] = nall), with coordinates per frame.
+
+Every op runs the HIP kernels of libdpmi; tensors must live on the GPU.
+There is no CPU path: a CPU tensor raises, exactly as the reference raises
+for a GPU tensor when its library was built without GPU support.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, ptr
+
+# --------------------------------------------------------------------------- helpers
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _gpu(*ts):
+    for t in ts:
+        if t is not None and not t.is_cuda:
+            raise RuntimeError(
+                "dpmi implements the GPU path only: the input tensor is on the CPU "
+                "(move it to the MI355X device)"
+            )
+
+
+def _c(t, dtype=None):
+    if dtype is not None and t.dtype != dtype:
+        t = t.to(dtype)
+    return t.contiguous()
+
+
+def _sfx(dtype):
+    if dtype == torch.float64:
+        return "f64"
+    if dtype == torch.float32:
+        return "f32"
+    raise TypeError(f"unsupported dtype {dtype}: only float64 and float32")
+
+
+def _sel_host(sel):
+    return np.ascontiguousarray(np.asarray(sel, dtype=np.int32))
+
+
+def _box_host(box):
+    if box is None:
+        return None
+    if isinstance(box, torch.Tensor):
+        box = box.detach().to("cpu", torch.float64).numpy()
+    return np.ascontiguousarray(np.asarray(box, dtype=np.float64).reshape(9))
+
+
+# --------------------------------------------------------------------------- neighbor list
+
+
+def nlist_workspace_bytes(box, nloc: int, rcut: float) -> int:
+    out = ctypes.c_int64(0)
+    b = _box_host(box)
+    call("dpmi_nlist_workspace_bytes", ctypes.byref(out), ptr(b) if b is not None else None, nloc, rcut)
+    return int(out.value)
+
+
+def build_nlist(coord: torch.Tensor, atype: torch.Tensor, box, rcut: float, sel, mapped: bool = False,
+                mem_nall: int | None = None):
+    """Periodic (box given) or open neighbor list of one frame.
+
+    coord [nloc, 3] float64, atype [nloc] int.  Returns (ext_coord [nall, 3],
+    ext_type [nall] int32, mapping [nall] int32, nlist [nloc, nnei] int32 in
+    extended indices, overflow [nloc] int32, nlist_mapped or None).
+    """
+    _gpu(coord, atype)
+    dev = coord.device
+    coord = _c(coord, torch.float64).view(-1, 3)
+    atype = _c(atype, torch.int32).view(-1)
+    nloc = coord.shape[0]
+    b = _box_host(box)
+    ws_bytes = nlist_workspace_bytes(b, nloc, rcut)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    cap = mem_nall if mem_nall is not None else (nloc * 4 + 4096 if b is not None else nloc)
+    nall = ctypes.c_int64(0)
+    for _ in range(8):
+        ext_c = torch.empty((max(cap, 1), 3), dtype=torch.float64, device=dev)
+        ext_t = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+        mapping = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+        try:
+            call("dpmi_copy_coord_f64", ptr(ext_c), ptr(ext_t), ptr(mapping), ctypes.byref(nall), cap,
+                 ptr(coord), ptr(atype), nloc, ptr(b) if b is not None else None, rcut, ptr(ws), ws_bytes,
+                 _stream())
+            break
+        except _lib.DpmiCapacityError:
+            cap = int(nall.value)
+    n = int(nall.value)
+    ext_c, ext_t, mapping = ext_c[:n], ext_t[:n], mapping[:n]
+    s = _sel_host(sel)
+    nnei = int(s.sum())
+    nlist = torch.empty((nloc, nnei), dtype=torch.int32, device=dev)
+    overflow = torch.empty(nloc, dtype=torch.int32, device=dev)
+    nl_map = torch.empty((nloc, nnei), dtype=torch.int32, device=dev) if mapped else None
+    call("dpmi_build_nlist_f64", ptr(nlist), ptr(overflow), ptr(nl_map), ptr(ext_c), ptr(ext_t), ptr(mapping),
+         nloc, n, rcut, ptr(s), len(s), ptr(ws), ws_bytes, _stream())
+    return ext_c, ext_t, mapping, nlist, overflow, nl_map
+
+
+# --------------------------------------------------------------------------- env-mat
+
+
+def env_mat_a(nlist, ext_coord, ext_type, avg, std, rcut, rcut_smth, sel, with_deriv=True, with_rij=True):
+    """prod_env_mat_a on a formatted list of one frame -> (em, em_deriv, rij)."""
+    _gpu(nlist, ext_coord, ext_type, avg, std)
+    dt = ext_coord.dtype
+    sfx = _sfx(dt)
+    nloc, nnei = nlist.shape
+    dev = ext_coord.device
+    em = torch.empty((nloc, nnei, 4), dtype=dt, device=dev)
+    ed = torch.empty((nloc, nnei, 4, 3), dtype=dt, device=dev) if with_deriv else None
+    rij = torch.empty((nloc, nnei, 3), dtype=dt, device=dev) if with_rij else None
+    s = _sel_host(sel)
+    call(f"dpmi_prod_env_mat_a_{sfx}", ptr(em), ptr(ed), ptr(rij), ptr(_c(nlist, torch.int32)), ptr(_c(ext_coord)),
+         ptr(_c(ext_type, torch.int32)), nloc, ptr(_c(avg, dt)), ptr(_c(std, dt)), rcut, rcut_smth, ptr(s), len(s),
+         _stream())
+    return em, ed, rij
+
+
+def force_virial_recompute(net_deriv, nlist, mapping, ext_coord, ext_type, std, rcut, rcut_smth, sel,
+                           want_virial=True):
+    """Adjoint of the env-mat: forces (-dE/dx) from dE/d em, with the env-mat
+    derivative recomputed.  mapping given -> forces folded onto local atoms."""
+    _gpu(net_deriv, nlist, ext_coord, ext_type, std)
+    sfx = _sfx(net_deriv.dtype)
+    nloc = nlist.shape[0]
+    nall = ext_coord.shape[0]
+    dev = ext_coord.device
+    force = torch.empty((nloc if mapping is not None else nall, 3), dtype=torch.float64, device=dev)
+    virial = torch.empty(9, dtype=torch.float64, device=dev) if want_virial else None
+    wsb = ctypes.c_int64(0)
+    call("dpmi_prod_virial_workspace_bytes", ctypes.byref(wsb), nloc, 1)
+    ws = torch.empty(int(wsb.value), dtype=torch.uint8, device=dev)
+    s = _sel_host(sel)
+    call(f"dpmi_force_virial_recompute_{sfx}", ptr(force), ptr(virial), ptr(_c(net_deriv)),
+         ptr(_c(nlist, torch.int32)), ptr(_c(mapping, torch.int32)) if mapping is not None else None,
+         ptr(_c(ext_coord, torch.float64)), ptr(_c(ext_type, torch.int32)), nloc, nall,
+         ptr(_c(std, net_deriv.dtype)), rcut, rcut_smth, ptr(s), len(s), ptr(ws), int(wsb.value), _stream())
+    return force, virial
+
+
+def prod_force(net_deriv, in_deriv, nlist, nloc: int, nall: int, nframes: int = 1):
+    _gpu(net_deriv, in_deriv, nlist)
+    dt = net_deriv.dtype
+    nnei = nlist.numel() // (nloc * nframes) if nloc * nframes > 0 else 0
+    force = torch.empty((nframes, nall * 3), dtype=dt, device=net_deriv.device)
+    call(f"dpmi_prod_force_se_a_{_sfx(dt)}", ptr(force), ptr(_c(net_deriv)), ptr(_c(in_deriv, dt)),
+         ptr(_c(nlist, torch.int32)), nloc, nall, nnei, nframes, _stream())
+    return force
+
+
+def prod_virial(net_deriv, in_deriv, rij, nlist, nloc: int, nall: int, nframes: int = 1, atomic: bool = True):
+    _gpu(net_deriv, in_deriv, rij, nlist)
+    dt = net_deriv.dtype
+    dev = net_deriv.device
+    nnei = nlist.numel() // (nloc * nframes) if nloc * nframes > 0 else 0
+    virial = torch.empty((nframes, 9), dtype=dt, device=dev)
+    atom_virial = torch.empty((nframes, nall * 9), dtype=dt, device=dev) if atomic else None
+    wsb = ctypes.c_int64(0)
+    call("dpmi_prod_virial_workspace_bytes", ctypes.byref(wsb), nloc, nframes)
+    ws = torch.empty(int(wsb.value), dtype=torch.uint8, device=dev)
+    call(f"dpmi_prod_virial_se_a_{_sfx(dt)}", ptr(virial), ptr(atom_virial), ptr(_c(net_deriv)),
+         ptr(_c(in_deriv, dt)), ptr(_c(rij, dt)), ptr(_c(nlist, torch.int32)), nloc, nall, nnei, nframes, ptr(ws),
+         int(wsb.value), _stream())
+    return virial, atom_virial
+
+
+# --------------------------------------------------------------------------- tabulate
+
+
+def _info_host(table_info, dtype):
+    # The reference reads table_info on the host (source/op/pt/tabulate_multi_device.cc,
+    # tabulate.cu:999-1001); the caller passes a CPU tensor.
+    arr = table_info.detach().to("cpu").numpy().astype(np.float64 if dtype == torch.float64 else np.float32)
+    return np.ascontiguousarray(arr.reshape(-1))
+
+
+def _tab_dims(em_x, em):
+    if em_x.dim() != 2:
+        raise ValueError("Dim of input should be 2")
+    if em.dim() != 3:
+        raise ValueError("Dim of input should be 3")
+    return em.shape[0], em.shape[1]
+
+
+def tabulate_forward(table, table_info, em_x, em, last_layer_size: int, is_sorted: bool = True):
+    if table.dim() != 2:
+        raise ValueError("Dim of table should be 2")
+    _gpu(table, em_x, em)
+    dt = table.dtype
+    nloc, nnei = _tab_dims(em_x, em)
+    out = torch.empty((nloc, 4, last_layer_size), dtype=dt, device=table.device)
+    info = _info_host(table_info, dt)
+    call(f"dpmi_tabulate_fusion_se_a_{_sfx(dt)}", ptr(out), ptr(_c(table)), ptr(info), ptr(_c(em_x, dt)),
+         ptr(_c(em, dt)), nloc, nnei, last_layer_size, int(is_sorted), _stream())
+    return out
+
+
+def tabulate_grad(table, table_info, em_x, em, dy, is_sorted: bool = True):
+    if dy.dim() != 3:
+        raise ValueError("Dim of dy_tensor should be 3")
+    _gpu(table, em_x, em, dy)
+    dt = table.dtype
+    nloc, nnei = _tab_dims(em_x, em)
+    M = dy.shape[2]
+    dx = torch.empty_like(em_x, dtype=dt)
+    dem = torch.empty_like(em, dtype=dt)
+    info = _info_host(table_info, dt)
+    call(f"dpmi_tabulate_fusion_se_a_grad_{_sfx(dt)}", ptr(dx), ptr(dem), ptr(_c(table)), ptr(info),
+         ptr(_c(em_x, dt)), ptr(_c(em, dt)), ptr(_c(dy, dt)), nloc, nnei, M, int(is_sorted), _stream())
+    return dx, dem
+
+
+def tabulate_grad_grad(table, table_info, em_x, em, dz_dy_dem_x, dz_dy_dem, M: int, is_sorted: bool = True):
+    if dz_dy_dem_x.dim() != 2:
+        raise ValueError("Dim of dz_dy_dem_x should be 2")
+    if dz_dy_dem.dim() != 3:
+        raise ValueError("Dim of dz_dy_dem should be 3")
+    _gpu(table, em_x, em, dz_dy_dem_x, dz_dy_dem)
+    dt = table.dtype
+    nloc, nnei = _tab_dims(em_x, em)
+    out = torch.empty((nloc, 4, M), dtype=dt, device=table.device)
+    info = _info_host(table_info, dt)
+    call(f"dpmi_tabulate_fusion_se_a_grad_grad_{_sfx(dt)}", ptr(out), ptr(_c(table)), ptr(info), ptr(_c(em_x, dt)),
+         ptr(_c(em, dt)), ptr(_c(dz_dy_dem_x, dt)), ptr(_c(dz_dy_dem, dt)), nloc, nnei, M, int(is_sorted), _stream())
+    return out
+
+
+class TabulateFusionSeAGradOp(torch.autograd.Function):
+    """(dy_dem_x, dy_dem) = grad(dy); differentiable w.r.t. dy only
+    (TabulateFusionSeAGradOp, tabulate_multi_device.cc:605-700)."""
+
+    @staticmethod
+    def forward(ctx, table, table_info, em_x, em, dy, descriptor):
+        dx, dem = tabulate_grad(table, table_info, em_x, em, dy)
+        ctx.save_for_backward(table, table_info, em_x, em, descriptor)
+        return dx, dem
+
+    @staticmethod
+    def backward(ctx, g_dx, g_dem):
+        table, table_info, em_x, em, descriptor = ctx.saved_tensors
+        g_dx = g_dx.contiguous() if g_dx is not None else torch.zeros_like(em_x)
+        g_dem = g_dem.contiguous() if g_dem is not None else torch.zeros_like(em)
+        dz_dy = tabulate_grad_grad(table, table_info, em_x, em, g_dx, g_dem, descriptor.shape[2], True)
+        return None, None, None, None, dz_dy, None
+
+
+class TabulateFusionSeAOp(torch.autograd.Function):
+    """descriptor = tabulate(em_x, em) (TabulateFusionSeAOp,
+    tabulate_multi_device.cc:702-830)."""
+
+    @staticmethod
+    def forward(ctx, table, table_info, em_x, em, last_layer_size):
+        out = tabulate_forward(table, table_info, em_x, em, last_layer_size)
+        ctx.save_for_backward(table, table_info, em_x, em, out)
+        return out
+
+    @staticmethod
+    def backward(ctx, dy):
+        table, table_info, em_x, em, out = ctx.saved_tensors
+        dx, dem = TabulateFusionSeAGradOp.apply(table, table_info, em_x, em, dy.contiguous(), out)
+        return None, None, dx, dem, None
+
+
+def tabulate_fusion_se_a(table, table_info, em_x, em, last_layer_size: int):
+    return [TabulateFusionSeAOp.apply(table, table_info, em_x, em, last_layer_size)]
+
+
+# --------------------------------------------------------------------------- prod_env_mat_a op
+
+
+class ProdEnvMatAOp(torch.autograd.Function):
+    """Env-mat of every frame with the neighbor list built on the GPU; the
+    backward is prod_force with the returned derivative (forces on local
+    atoms, the list being mapped to local indices as ProdEnvMatAOp returns it)."""
+
+    @staticmethod
+    def forward(ctx, coord, atype, box, davg, dstd, rcut, rcut_smth, sel):
+        nf = coord.shape[0]
+        coord = coord.reshape(nf, -1, 3)
+        atype = atype.reshape(nf, -1)
+        nloc = coord.shape[1]
+        ems, eds, rijs, nls = [], [], [], []
+        for f in range(nf):
+            b = None if box is None else box.reshape(nf, 9)[f]
+            ext_c, ext_t, mapping, nlist, _ovf, nl_map = build_nlist(coord[f].detach(), atype[f], b, rcut, sel,
+                                                                      mapped=True)
+            em, ed, rij = env_mat_a(nlist, ext_c.to(coord.dtype), ext_t, davg, dstd, rcut, rcut_smth, sel)
+            ems.append(em.reshape(nloc, -1))
+            eds.append(ed.reshape(nloc, -1))
+            rijs.append(rij.reshape(nloc, -1))
+            nls.append(nl_map)
+        em = torch.stack(ems).reshape(nf, -1)
+        ed = torch.stack(eds).reshape(nf, -1)
+        rij = torch.stack(rijs).reshape(nf, -1)
+        nl = torch.stack(nls).reshape(nf, -1)
+        ctx.save_for_backward(ed, nl)
+        ctx.dims = (nf, nloc)
+        ctx.mark_non_differentiable(ed, rij, nl)
+        return em, ed, rij, nl
+
+    @staticmethod
+    def backward(ctx, g_em, g_ed, g_rij, g_nl):
+        ed, nl = ctx.saved_tensors
+        nf, nloc = ctx.dims
+        force = prod_force(g_em.contiguous(), ed, nl, nloc, nloc, nf)
+        return -force.reshape(nf, nloc, 3), None, None, None, None, None, None, None
+
+
+def prod_env_mat_a(coord, atype, box, davg, dstd, rcut: float, rcut_smth: float, sel):
+    return list(ProdEnvMatAOp.apply(coord, atype, box, davg, dstd, float(rcut), float(rcut_smth), list(sel)))
+
+
+def prod_force_se_a(net_deriv, in_deriv, nlist, natoms, n_a_sel: int, n_r_sel: int):
+    nf = net_deriv.shape[0]
+    na = natoms.detach().to("cpu").tolist()
+    return prod_force(net_deriv, in_deriv, nlist, int(na[0]), int(na[1]), nf)
+
+
+def prod_virial_se_a(net_deriv, in_deriv, rij, nlist, natoms, n_a_sel: int, n_r_sel: int):
+    nf = net_deriv.shape[0]
+    na = natoms.detach().to("cpu").tolist()
+    v, av = prod_virial(net_deriv, in_deriv, rij, nlist, int(na[0]), int(na[1]), nf)
+    return [v, av]
+
+
+# --------------------------------------------------------------------------- registration
+
+_REGISTERED = False
+_LIBS = []
+
+
+def register() -> None:
+    """Define the deepmd:: ops (once per process)."""
+    global _REGISTERED
+    if _REGISTERED:
+        return
+    existing = getattr(torch.ops, "deepmd", None)
+    if existing is not None and hasattr(existing, "tabulate_fusion_se_a"):
+        try:
+            existing.tabulate_fusion_se_a.default  # noqa: B018
+            raise RuntimeError("torch.ops.deepmd.tabulate_fusion_se_a is already defined by another library")
+        except AttributeError:
+            pass
+    lib = torch.library.Library("deepmd", "FRAGMENT")
+    lib.define("tabulate_fusion_se_a(Tensor table, Tensor table_info, Tensor em_x, Tensor em, "
+               "int last_layer_size) -> Tensor[]")
+    lib.impl("tabulate_fusion_se_a", tabulate_fusion_se_a, "CompositeImplicitAutograd")
+    lib.define("prod_env_mat_a(Tensor coord, Tensor atype, Tensor? box, Tensor davg, Tensor dstd, "
+               "float rcut, float rcut_smth, int[] sel) -> Tensor[]")
+    lib.impl("prod_env_mat_a", prod_env_mat_a, "CompositeImplicitAutograd")
+    lib.define("prod_force_se_a(Tensor net_deriv, Tensor in_deriv, Tensor nlist, Tensor natoms, "
+               "int n_a_sel, int n_r_sel) -> Tensor")
+    lib.impl("prod_force_se_a", prod_force_se_a, "CompositeImplicitAutograd")
+    lib.define("prod_virial_se_a(Tensor net_deriv, Tensor in_deriv, Tensor rij, Tensor nlist, Tensor natoms, "
+               "int n_a_sel, int n_r_sel) -> Tensor[]")
+    lib.impl("prod_virial_se_a", prod_virial_se_a, "CompositeImplicitAutograd")
+    _LIBS.append(lib)
+    _REGISTERED = True

@@ -1,0 +1,179 @@
+/*
+ * libdpmi — MI355X (gfx950) kernels for the DeepPot-SE se_e2_a
+ * descriptor-to-force path, behind a plain C ABI.
+ *
+ * Conventions
+ *  - Every pointer argument is DEVICE memory unless its name ends in _host.
+ *  - The caller allocates every output and every workspace; the library
+ *    never allocates or frees device memory.
+ *  - Sizes are int64_t; `stream` is a hipStream_t passed as void* (NULL is the
+ *    null stream).  Work is queued on `stream`; functions return without
+ *    waiting unless documented otherwise.
+ *  - Return value: DPMI_OK (0) or an error code; dpmi_last_error() gives the
+ *    message of the last failure on the calling thread.
+ *  - Layouts are contiguous row-major, as in the reference op contracts.
+ *
+ * Each entry point names the reference function it replaces (paths relative
+ * to the reference repository root).
+ */
+#ifndef DPMI_H_
+#define DPMI_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DPMI_OK 0
+#define DPMI_ERR_ARG 1      /* invalid argument (std::invalid_argument in the reference) */
+#define DPMI_ERR_HIP 2      /* HIP runtime error (deepmd_exception from DPErrcheck) */
+#define DPMI_ERR_CAPACITY 3 /* output capacity too small; the needed size is returned */
+
+const char* dpmi_last_error(void);
+int dpmi_abi_version(void);
+
+/* ------------------------------------------------------------------------
+ * Neighbor list.  Replaces, as one pipeline, normalize_coord_gpu,
+ * copy_coord_gpu and build_nlist_gpu (source/lib/include/coord.h:36-80,
+ * neighbor_list.h:145-240) plus format_nbor_list_gpu
+ * (source/lib/include/fmt_nlist.h:20-32).  The output is the formatted
+ * list of format_nlist_cpu (source/lib/src/fmt_nlist.cc:98-177): per type
+ * section, neighbors sorted by (float r^2, extended index), -1 padding.
+ * ------------------------------------------------------------------------ */
+
+/* Workspace bytes for dpmi_copy_coord_f64 + dpmi_build_nlist_f64.
+ * box_host: 9 doubles (rows are the cell vectors), or NULL for an open
+ * (non-periodic) system. */
+int dpmi_nlist_workspace_bytes(int64_t* bytes, const double* box_host, int64_t nloc, float rcut);
+
+/* Normalize local coordinates into the box and append periodic ghost atoms
+ * in the layout of copy_coord_cpu (source/lib/src/coord.cc:30-66,
+ * neighbor_list.cc:745-845): locals first, then ghost cells in (x, y, z) loop
+ * order.  Also bins atoms into the cell grid kept in the workspace for
+ * dpmi_build_nlist_f64.  Waits for the stream once to learn nall.
+ * If nall > mem_nall, returns DPMI_ERR_CAPACITY with *nall set (the caller
+ * grows the buffers and calls again, as _norm_copy_coord_cpu does).
+ * With box_host == NULL nothing is copied and nall = nloc. */
+int dpmi_copy_coord_f64(double* out_coord, int* out_type, int* mapping, int64_t* nall, int64_t mem_nall,
+                        const double* in_coord, const int* in_type, int64_t nloc,
+                        const double* box_host, float rcut, void* workspace, int64_t workspace_bytes,
+                        void* stream);
+
+/* Formatted neighbor list [nloc, sum(sel)] in extended indices.  Candidates
+ * are |x_i - x_j|^2 < rcut^2 in double (build_nlist_cpu,
+ * neighbor_list.cc:873-916); kept when the float r^2 of float-rounded
+ * coordinates is <= rcut^2 (fmt_nlist.cc:114-128).  overflow[i] (optional)
+ * receives -1 or the largest type whose neighbors exceeded sel.
+ * nlist_mapped (optional) receives mapping[nlist] (the local-index list
+ * that ProdEnvMatAOp returns, source/op/tf/prod_env_mat_multi_device.cc:
+ * 619-622). */
+int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped, const double* ext_coord,
+                         const int* ext_type, const int* mapping, int64_t nloc, int64_t nall, float rcut,
+                         const int* sel_host, int ntypes, const void* workspace, int64_t workspace_bytes,
+                         void* stream);
+
+/* ------------------------------------------------------------------------
+ * Environment matrix.  Replaces prod_env_mat_a_gpu
+ * (source/lib/include/prod_env_mat.h:45-62) on an already formatted list
+ * (env_mat_a_cpu, source/lib/src/env_mat.cc:123-221, and the davg/dstd
+ * normalisation of prod_env_mat.cc:66-107).  em [nloc, nnei, 4],
+ * em_deriv [nloc, nnei, 4, 3] (derivative w.r.t. the centre atom, already
+ * divided by std; may be NULL), rij [nloc, nnei, 3] (may be NULL).
+ * avg/std: [ntypes, nnei, 4].
+ * ------------------------------------------------------------------------ */
+int dpmi_prod_env_mat_a_f64(double* em, double* em_deriv, double* rij, const int* nlist,
+                            const double* ext_coord, const int* ext_type, int64_t nloc, const double* avg,
+                            const double* std, float rcut, float rcut_smth, const int* sel_host, int ntypes,
+                            void* stream);
+int dpmi_prod_env_mat_a_f32(float* em, float* em_deriv, float* rij, const int* nlist,
+                            const float* ext_coord, const int* ext_type, int64_t nloc, const float* avg,
+                            const float* std, float rcut, float rcut_smth, const int* sel_host, int ntypes,
+                            void* stream);
+
+/* ------------------------------------------------------------------------
+ * Force and virial.  prod_force_a_gpu / prod_virial_a_gpu
+ * (source/lib/include/prod_force.h, prod_virial.h; CPU semantics
+ * source/lib/src/prod_force.cc:23-76, prod_virial.cc:22-70).
+ * force [nframes, nall, 3]; virial [nframes, 9]; atom_virial
+ * [nframes, nall, 9] (may be NULL).  The virial reduction is deterministic
+ * (per-block partials, then one compensated sum); its workspace size comes
+ * from dpmi_prod_virial_workspace_bytes.
+ * ------------------------------------------------------------------------ */
+int dpmi_prod_force_se_a_f64(double* force, const double* net_deriv, const double* in_deriv,
+                             const int* nlist, int64_t nloc, int64_t nall, int64_t nnei, int64_t nframes,
+                             void* stream);
+int dpmi_prod_force_se_a_f32(float* force, const float* net_deriv, const float* in_deriv,
+                             const int* nlist, int64_t nloc, int64_t nall, int64_t nnei, int64_t nframes,
+                             void* stream);
+int dpmi_prod_virial_workspace_bytes(int64_t* bytes, int64_t nloc, int64_t nframes);
+int dpmi_prod_virial_se_a_f64(double* virial, double* atom_virial, const double* net_deriv,
+                              const double* in_deriv, const double* rij, const int* nlist, int64_t nloc,
+                              int64_t nall, int64_t nnei, int64_t nframes, void* workspace,
+                              int64_t workspace_bytes, void* stream);
+int dpmi_prod_virial_se_a_f32(float* virial, float* atom_virial, const float* net_deriv,
+                              const float* in_deriv, const float* rij, const int* nlist, int64_t nloc,
+                              int64_t nall, int64_t nnei, int64_t nframes, void* workspace,
+                              int64_t workspace_bytes, void* stream);
+
+/* Backward of the env-mat for the model path (the reverse-mode derivative
+ * that deepmd/pt obtains from autograd through prod_env_mat,
+ * deepmd/pt/model/descriptor/env_mat.py:11-92, followed by the ghost ->
+ * local scatter of communicate_extended_output,
+ * deepmd/pt/model/model/transform_output.py:317-383, and the virial of
+ * transform_output.py:85-87).  net_deriv = dE/d em [nloc, nnei, 4]; the env-mat
+ * derivative is recomputed from coordinates, never stored.  With mapping
+ * non-NULL, force is [nloc, 3] (ghost contributions folded onto owners);
+ * with mapping NULL, force is [nall, 3] on extended atoms.  force holds
+ * -dE/dx (the sign of a force).  virial [9] (may be NULL) is the
+ * deterministic compensated sum over pairs of g (x) r_ij.  Coordinates are
+ * double; the _f32 variant evaluates the env-mat derivative in float.
+ * Workspace: dpmi_prod_virial_workspace_bytes(nloc, 1). */
+int dpmi_force_virial_recompute_f64(double* force, double* virial, const double* net_deriv, const int* nlist,
+                                    const int* mapping, const double* ext_coord, const int* ext_type, int64_t nloc,
+                                    int64_t nall, const double* std, float rcut, float rcut_smth,
+                                    const int* sel_host, int ntypes, void* workspace, int64_t workspace_bytes,
+                                    void* stream);
+int dpmi_force_virial_recompute_f32(double* force, double* virial, const float* net_deriv, const int* nlist,
+                                    const int* mapping, const double* ext_coord, const int* ext_type, int64_t nloc,
+                                    int64_t nall, const float* std, float rcut, float rcut_smth,
+                                    const int* sel_host, int ntypes, void* workspace, int64_t workspace_bytes,
+                                    void* stream);
+
+/* ------------------------------------------------------------------------
+ * Tabulated embedding fused with G^T R.  tabulate_fusion_se_a_gpu and its
+ * _grad/_grad_grad (source/lib/include/tabulate.h:6-46; CPU semantics
+ * source/lib/src/tabulate.cc:80-383).  table [nspline, 6*M] (coefficients
+ * of channel k at 6k..6k+5), table_info_host: 6 values on the HOST as in
+ * the reference op (source/op/pt/tabulate_multi_device.cc), em_x [nloc*nnei],
+ * em [nloc, nnei, 4], out/dy/dz_dy [nloc, 4, M].  is_sorted enables the
+ * reference's early exit at the first padded neighbor.
+ * ------------------------------------------------------------------------ */
+int dpmi_tabulate_fusion_se_a_f64(double* out, const double* table, const double* table_info_host,
+                                  const double* em_x, const double* em, int64_t nloc, int64_t nnei,
+                                  int64_t last_layer_size, int is_sorted, void* stream);
+int dpmi_tabulate_fusion_se_a_f32(float* out, const float* table, const float* table_info_host,
+                                  const float* em_x, const float* em, int64_t nloc, int64_t nnei,
+                                  int64_t last_layer_size, int is_sorted, void* stream);
+int dpmi_tabulate_fusion_se_a_grad_f64(double* dy_dem_x, double* dy_dem, const double* table,
+                                       const double* table_info_host, const double* em_x, const double* em,
+                                       const double* dy, int64_t nloc, int64_t nnei, int64_t last_layer_size,
+                                       int is_sorted, void* stream);
+int dpmi_tabulate_fusion_se_a_grad_f32(float* dy_dem_x, float* dy_dem, const float* table,
+                                       const float* table_info_host, const float* em_x, const float* em,
+                                       const float* dy, int64_t nloc, int64_t nnei, int64_t last_layer_size,
+                                       int is_sorted, void* stream);
+int dpmi_tabulate_fusion_se_a_grad_grad_f64(double* dz_dy, const double* table, const double* table_info_host,
+                                            const double* em_x, const double* em, const double* dz_dy_dem_x,
+                                            const double* dz_dy_dem, int64_t nloc, int64_t nnei,
+                                            int64_t last_layer_size, int is_sorted, void* stream);
+int dpmi_tabulate_fusion_se_a_grad_grad_f32(float* dz_dy, const float* table, const float* table_info_host,
+                                            const float* em_x, const float* em, const float* dz_dy_dem_x,
+                                            const float* dz_dy_dem, int64_t nloc, int64_t nnei,
+                                            int64_t last_layer_size, int is_sorted, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DPMI_H_ */

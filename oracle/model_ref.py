@@ -1,0 +1,162 @@
+"""CPU model oracle for se_e2_a energy models (test infrastructure only).
+
+A float64 torch-CPU restatement of the reference deepmd/pt model path:
+
+* env-mat: deepmd/pt/model/descriptor/env_mat.py:11-92 and
+  deepmd/pt/utils/preprocess.py:9-19 (smooth weight);
+* embedding/fitting layers: deepmd/pt/model/network/mlp.py:188-219
+  (linear, tanh, timestep, resnet with x or [x, x]);
+* descriptor: deepmd/pt/model/descriptor/se_a.py:750-866 (type_one_side,
+  G^T R / nnei, D = (G^T R)^T (G^T R)[:, :axis]);
+* fitting: deepmd/pt/model/task/fitting.py:734-900 (own-type net + bias_atom_e)
+  and the out_bias of base_atomic_model.py:571-592;
+* energy/force/virial: deepmd/pt/model/model/transform_output.py:65-206,
+  317-383 (autograd force, F (x) x virial, ghost -> local fold).
+
+The neighbor list comes from the C oracle (reference C++ semantics).  Pinned
+by source/tests/infer/deeppot-testcase.yaml (reference tolerance: 12 decimal
+places, source/tests/infer/test_models.py:30-32) and by finite differences.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import oracle
+
+
+def _arr(x):
+    return None if x is None else torch.as_tensor(np.asarray(x, dtype=np.float64))
+
+
+class Layer:
+    def __init__(self, d: dict, activation: bool = True):
+        d = d.get("@variables", d)  # serialized layers keep arrays under @variables
+        self.w = _arr(d["w"])
+        self.b = _arr(d["b"])
+        self.idt = _arr(d.get("idt"))
+        self.act = activation
+
+    def __call__(self, x):
+        y = x @ self.w + self.b
+        if self.act:
+            y = torch.tanh(y)
+        if self.idt is not None:
+            y = y * self.idt
+        nin, nout = self.w.shape
+        if self.act:  # hidden layers carry resnet=True
+            if nin == nout:
+                y = y + x
+            elif 2 * nin == nout:
+                y = y + torch.cat([x, x], dim=-1)
+        return y
+
+
+class Model:
+    """se_e2_a + energy fitting, from a dict in the reference serialization
+    layout (the 'model' block of deeppot_sea.yaml) or from dpmi.systems
+    parameters (see from_params)."""
+
+    def __init__(self, rcut, rcut_smth, sel, axis, davg, dstd, emb, fit, bias_atom_e, out_bias=None):
+        self.rcut = float(rcut)
+        self.rcut_smth = float(rcut_smth)
+        self.sel = list(sel)
+        self.nnei = int(sum(sel))
+        self.sec = np.concatenate([[0], np.cumsum(sel)])
+        self.axis = int(axis)
+        self.davg = _arr(davg)
+        self.dstd = _arr(dstd)
+        self.emb = [[Layer(l) for l in net] for net in emb]
+        self.fit = [[Layer(l, activation=(k < len(net) - 1)) for k, l in enumerate(net)] for net in fit]
+        self.bias_atom_e = _arr(bias_atom_e).reshape(-1)
+        nt = len(sel)
+        self.out_bias = _arr(out_bias).reshape(nt) if out_bias is not None else torch.zeros(nt, dtype=torch.float64)
+
+    @classmethod
+    def from_serialized(cls, m: dict) -> "Model":
+        d = m["descriptor"]
+        f = m["fitting"]
+        emb = [net["layers"] for net in d["embeddings"]["networks"]]
+        fit = [net["layers"] for net in f["nets"]["networks"]]
+        return cls(d["rcut"], d["rcut_smth"], d["sel"], d["axis_neuron"], d["@variables"]["davg"],
+                   d["@variables"]["dstd"], emb, fit, f["@variables"]["bias_atom_e"],
+                   m["@variables"]["out_bias"])
+
+    @classmethod
+    def from_params(cls, spec, params: dict) -> "Model":
+        return cls(spec.rcut, spec.rcut_smth, spec.sel, spec.axis_neuron, params["davg"], params["dstd"],
+                   params["embedding"], params["fitting"], params["bias_atom_e"])
+
+    # ------------------------------------------------------------------ pieces
+    def env_mat(self, ext, nlist, atype):
+        nloc, nnei = nlist.shape
+        nl = torch.as_tensor(nlist, dtype=torch.long)
+        mask = nl >= 0
+        idx = torch.where(mask, nl, torch.zeros_like(nl))
+        diff = ext[idx] - ext[:nloc].unsqueeze(1)
+        length = torch.linalg.norm(diff, dim=-1, keepdim=True)
+        length = length + (~mask).unsqueeze(-1)
+        t0 = 1.0 / length
+        t1 = diff / length**2
+        u = (torch.clamp(length, self.rcut_smth, self.rcut) - self.rcut_smth) / (self.rcut - self.rcut_smth)
+        sw = u * u * u * (-6 * u * u + 15 * u - 10) + 1
+        sw = sw * mask.unsqueeze(-1)
+        em = torch.cat([t0, t1], dim=-1) * sw
+        at = torch.as_tensor(atype[:nloc], dtype=torch.long)
+        return (em - self.davg[at]) / self.dstd[at]
+
+    def descriptor(self, em):
+        nloc = em.shape[0]
+        M = self.emb[0][-1].w.shape[1]
+        gr = torch.zeros((nloc, 4, M), dtype=torch.float64)
+        for t in range(len(self.sel)):
+            rr = em[:, self.sec[t]:self.sec[t + 1], :]
+            g = rr[:, :, :1]
+            for layer in self.emb[t]:
+                g = layer(g)
+            gr = gr + rr.transpose(1, 2) @ g
+        gr = gr / self.nnei
+        d = gr.transpose(1, 2) @ gr[:, :, : self.axis]
+        return d.reshape(nloc, -1)
+
+    def atom_energy(self, desc, atype):
+        at = torch.as_tensor(atype, dtype=torch.long)
+        e = torch.zeros(desc.shape[0], dtype=torch.float64)
+        for t, net in enumerate(self.fit):
+            y = desc
+            for layer in net:
+                y = layer(y)
+            y = y[:, 0] + self.bias_atom_e[t]
+            e = torch.where(at == t, y, e)
+        return e + self.out_bias[at]
+
+    # ------------------------------------------------------------------ E, F, V
+    def evaluate(self, coord, atype, box):
+        coord = np.asarray(coord, dtype=np.float64).reshape(-1, 3)
+        atype = np.asarray(atype, dtype=np.int32)
+        nloc = len(atype)
+        if box is None:
+            ec, et, mp, nlist, _ = oracle.build_nlist_nopbc(coord, atype, self.rcut, self.sel)
+        else:
+            ec, et, mp, nlist, _ = oracle.build_nlist_pbc(coord, atype, box, self.rcut, self.sel)
+        ext = torch.tensor(ec, requires_grad=True)
+        em = self.env_mat(ext, nlist, et)
+        desc = self.descriptor(em)
+        ae = self.atom_energy(desc, atype)
+        energy = ae.sum()
+        (g,) = torch.autograd.grad(energy, ext)
+        fext = -g.detach().numpy()
+        virial = np.einsum("ik,ij->kj", fext, ec).reshape(9)
+        force = np.zeros((nloc, 3))
+        np.add.at(force, mp, fext)
+        return {
+            "energy": float(energy.item()),
+            "atom_energy": ae.detach().numpy(),
+            "force": force,
+            "virial": virial,
+            "descriptor": desc.detach().numpy(),
+            "nlist": nlist,
+            "ext_coord": ec,
+            "mapping": mp,
+        }
