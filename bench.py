@@ -1,0 +1,189 @@
+#!/usr/bin/env python
+"""E+F+V atom-steps/s of the se_e2_a path on one MI355X.
+
+A step = neighbor-list rebuild + energy + forces + virial of one frame, all on
+the GPU, inputs resident in HBM.  Both forms run on the same GPU and the same
+coordinates: the plain FP64 path and the BF16-MFMA form (embedding/fitting
+GEMM operands bf16 with fp32 accumulation, env-mat fp32, energy and virial
+reductions fp64).  `value` is the BF16 form's throughput; the FP64 figure,
+the speedup and the BF16 error against FP64 ride along.
+
+    python bench.py [--steps K] [--warmup W] [--workload water110k]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _p in (str(ROOT), str(ROOT / "dpmd-public_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "E+F+V atom-steps/s/GPU: FP64 plain vs BF16-MFMA form at force RMSE≤1e-3 eV/Å"
+
+WORKLOADS = {
+    "water110k": dict(desc="110,592-atom water-like box (36,864 type-0 / 73,728 type-1), L=99.36 A, jittered "
+                           "48^3 lattice, seed 3; se_e2_a rcut 6.0, rcut_smth 0.5, sel [46,92], embedding "
+                           "[25,50,100] tanh resnet_dt, axis 16, fitting [240,240,240]",
+                      spec="water", stat="water192"),
+    "water12k": dict(desc="12,288-atom water-like box, L=49.68 A, seed 1; same model", spec="water",
+                     stat="water192"),
+    "water192": dict(desc="192-atom water-like box, L=12.42 A, seed 0; same model", spec="water", stat="water192"),
+    "alloy262k": dict(desc="262,144-atom four-type FCC alloy (64^3 primitive cells, a=3.6 A), seed 4; rcut 8.0, "
+                           "rcut_smth 2.0, sel [60]*4, embedding [32,64,128], axis 16, fitting [240]*3",
+                      spec="alloy", stat="alloy_small"),
+}
+
+# algorithmic HBM bytes per local atom of each libdpmi kernel (recompute design):
+# nlist row (4*nnei) + em or dE/dem row (4*nnei*elem) + centre and neighbor coordinates
+# counted once per atom (2 * 3 * 8).
+def _bytes_per_atom(name: str, nnei: int) -> float:
+    elem = 8 if name.endswith("f64") else 4
+    if name.startswith("dpmi_force_virial_recompute") or name.startswith("dpmi_prod_env_mat_a"):
+        return 4 * nnei + 4 * nnei * elem + 48
+    if name.startswith("dpmi_build_nlist"):
+        return 4 * nnei + 4 * nnei + 24 + 4  # nlist + mapped/overflow, own coordinate
+    return 0.0
+
+
+HBM_PEAK_GBS = 8000.0
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="water110k", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="bf16", choices=["bf16", "fp16"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X, so --gpus "
+              f"{max(args.gpus, world)} is not run (the multi-GPU sharding design is described in DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from dpmi import ops, systems
+    from dpmi.model import SeAEnergyModel
+    from dpmi.stat import gpu_model_params
+
+    dev = torch.device("cuda:0")
+    wl = WORKLOADS[args.workload]
+    spec = systems.WATER_SPEC if wl["spec"] == "water" else systems.ALLOY_SPEC
+    stat_sys = systems.water192(0) if wl["stat"] == "water192" else systems.alloy262k(4, ncell=8)
+    params = gpu_model_params(spec, stat_sys)
+    sysm = systems.SYSTEMS[args.workload]()
+    coord = torch.tensor(sysm.coord, device=dev)
+    atype = torch.tensor(sysm.atype, device=dev)
+    nloc = sysm.nloc
+    nnei = int(sum(spec.sel))
+
+    def run(model, names):
+        for _ in range(args.warmup):
+            model.compute(coord, atype, sysm.box)
+        torch.cuda.synchronize()
+        with ops.KernelTimer(*names) as kt:
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                out = model.compute(coord, atype, sysm.box)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+        return (t1 - t0) / args.steps, out, kt.summary()
+
+    names64 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f64",
+               "dpmi_force_virial_recompute_f64"]
+    names32 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f32",
+               "dpmi_force_virial_recompute_f32"]
+    m64 = SeAEnergyModel.from_params(spec, params, device=dev, precision="fp64")
+    t64, out64, k64 = run(m64, names64)
+    del m64
+    mlo = SeAEnergyModel.from_params(spec, params, device=dev, precision=args.precision)
+    tlo, outlo, klo = run(mlo, names32)
+
+    frms = torch.sqrt((out64["force"] ** 2).mean()).item()
+    rmse = torch.sqrt(((out64["force"] - outlo["force"]) ** 2).mean()).item()
+    de_atom = abs(out64["energy"].item() - outlo["energy"].item()) / nloc
+    vscale = torch.abs(out64["virial"]).max().item()
+    dv = torch.abs(out64["virial"] - outlo["virial"]).max().item() / max(vscale, 1e-300)
+
+    # dominant libdpmi kernel of the BF16 step and its HBM roofline
+    dom = max(klo, key=lambda n: klo[n]["total_ms"])
+    algo = _bytes_per_atom(dom, nnei) * nloc
+    avg_s = klo[dom]["avg_ms"] / 1e3
+    achieved = algo / avg_s / 1e9 if avg_s > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": int(algo), "avg_launch_ms": round(klo[dom]["avg_ms"], 4)}
+
+    line = {
+        "metric": METRIC,
+        "value": round(nloc / tlo, 1),
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(tlo * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic (seeded SplitMix64 lattice, random-init weights, davg/dstd from the 192-atom frame)",
+        "config": {"workload": args.workload, "description": wl["desc"], "atoms": nloc,
+                   "form": f"{args.precision} GEMM operands / fp32 accumulate, fp32 env-mat, fp64 E and virial"},
+        "fp64": {"value": round(nloc / t64, 1), "ms_per_step": round(t64 * 1e3, 3)},
+        "speedup_vs_fp64": round(t64 / tlo, 3),
+        "accuracy_vs_fp64": {"force_rmse_eV_per_A": rmse, "force_rms_eV_per_A": frms,
+                             "force_rmse_relative": rmse / frms if frms else None,
+                             "energy_per_atom_abs_eV": de_atom, "virial_max_rel": dv,
+                             "force_rmse_ok": rmse <= 1e-3},
+        "kernels_ms_per_step": {"fp64": {k: round(v["total_ms"] / args.steps, 4) for k, v in k64.items()},
+                                args.precision: {k: round(v["total_ms"] / args.steps, 4) for k, v in klo.items()}},
+        "roofline": roofline,
+    }
+
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(spec, params, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_baseline(spec, params, budget_s: float) -> dict:
+    """The CPU oracle (C neighbor list + float64 torch-CPU model with autograd
+    forces, the reference PT algorithm) on the 12,288-atom water box: whole
+    E+F+V steps until budget_s of CPU time has elapsed (at least one)."""
+    import torch
+
+    from dpmi import systems
+    from oracle.model_ref import Model
+
+    threads = torch.get_num_threads()
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    m = Model.from_params(spec, params)
+    s = systems.water12k(1)
+    m.evaluate(s.coord[:600], s.atype[:600], None)  # warm up allocators
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        m.evaluate(s.coord, s.atype, s.box)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 8:
+            break
+    return {"value": round(s.nloc * steps / el, 1), "unit": "atom-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} FP64 E+F+V step(s) of the 12,288-atom water box (same model and density) with the "
+                      f"CPU oracle: C neighbor list + torch-CPU float64 model, autograd forces; {el:.1f} s"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

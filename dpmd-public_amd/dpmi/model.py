@@ -1,0 +1,206 @@
+"""se_e2_a energy model on the GPU: energy, forces and virial per step.
+
+Mirrors the composition of the reference deepmd/pt model
+(deepmd/pt/model/descriptor/se_a.py:750-866, network/mlp.py:188-219,
+task/fitting.py:734-900, model/transform_output.py:65-206) with the data
+path moved onto libdpmi:
+
+  coordinates --nlist kernels--> formatted list (bit-exact C++ semantics)
+              --env-mat kernel--> em [nloc, nnei, 4]
+              --embedding + G^T R, D, fitting--> atomic energies, E (FP64 sum)
+  dE/d em     --recompute adjoint kernel--> forces (ghosts folded), virial
+
+Forces are the hand-written adjoint of the env-mat (the derivative is
+recomputed, never stored); the network part of dE/d em comes from the
+embedding/fitting adjoint.  Precision forms:
+
+* ``fp64``  — everything in double ("plain FP64").
+* ``bf16`` / ``fp16`` — embedding, descriptor and fitting GEMM operands in
+  bf16/fp16 with fp32 accumulation, env-mat and its adjoint in fp32, energy
+  and virial reductions in fp64.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import ops
+
+_ACT = torch.tanh
+
+
+class _LowPrecisionMM(torch.autograd.Function):
+    """y = x @ w with bf16/fp16 operands, fp32 accumulation and an fp32
+    result; the backward products use the same operand precision."""
+
+    @staticmethod
+    def forward(ctx, x, w, gemm_dtype):
+        xl, wl = x.to(gemm_dtype), w.to(gemm_dtype)
+        ctx.save_for_backward(xl, wl)
+        ctx.out = x.dtype
+        return torch.mm(xl, wl, out_dtype=x.dtype)
+
+    @staticmethod
+    def backward(ctx, gy):
+        xl, wl = ctx.saved_tensors
+        gl = gy.to(xl.dtype)
+        gx = torch.mm(gl, wl.t().contiguous(), out_dtype=ctx.out) if ctx.needs_input_grad[0] else None
+        gw = torch.mm(xl.t().contiguous(), gl, out_dtype=ctx.out) if ctx.needs_input_grad[1] else None
+        return gx, gw, None
+
+
+class _LowPrecisionBMM(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, a, b, gemm_dtype):
+        al, bl = a.to(gemm_dtype).contiguous(), b.to(gemm_dtype).contiguous()
+        ctx.save_for_backward(al, bl)
+        ctx.out = a.dtype
+        return torch.bmm(al, bl, out_dtype=a.dtype)
+
+    @staticmethod
+    def backward(ctx, gy):
+        al, bl = ctx.saved_tensors
+        gl = gy.to(al.dtype).contiguous()
+        ga = torch.bmm(gl, bl.transpose(1, 2).contiguous(), out_dtype=ctx.out) if ctx.needs_input_grad[0] else None
+        gb = torch.bmm(al.transpose(1, 2).contiguous(), gl, out_dtype=ctx.out) if ctx.needs_input_grad[1] else None
+        return ga, gb, None
+
+
+class Net:
+    """Stack of reference MLP layers (MLPLayer.forward, mlp.py:188-219): y =
+    tanh(x W + b) [* idt] [+ x or + (x, x)]; the last layer of a fitting net is
+    linear without skip."""
+
+    def __init__(self, layers, device, dtype, last_linear: bool):
+        self.w, self.b, self.idt, self.act = [], [], [], []
+        for k, l in enumerate(layers):
+            v = l.get("@variables", l)
+            self.w.append(torch.as_tensor(np.asarray(v["w"], dtype=np.float64), device=device, dtype=dtype))
+            self.b.append(torch.as_tensor(np.asarray(v["b"], dtype=np.float64), device=device, dtype=dtype))
+            idt = v.get("idt")
+            self.idt.append(None if idt is None else torch.as_tensor(np.asarray(idt, dtype=np.float64),
+                                                                     device=device, dtype=dtype))
+            self.act.append(not (last_linear and k == len(layers) - 1))
+
+    def __call__(self, x, gemm_dtype=None):
+        for w, b, idt, act in zip(self.w, self.b, self.idt, self.act):
+            if gemm_dtype is None:
+                y = torch.addmm(b, x, w) if x.dim() == 2 else x @ w + b
+            else:
+                # low-precision operands, fp32 accumulation and fp32 result
+                y = _LowPrecisionMM.apply(x, w, gemm_dtype) + b
+            if act:
+                y = _ACT(y)
+            if idt is not None:
+                y = y * idt
+            if act:
+                nin, nout = w.shape
+                if nin == nout:
+                    y = y + x
+                elif 2 * nin == nout:
+                    y = y + torch.cat([x, x], dim=-1)
+            x = y
+        return x
+
+
+class SeAEnergyModel:
+    def __init__(self, rcut, rcut_smth, sel, axis, davg, dstd, emb, fit, bias_atom_e, out_bias=None,
+                 device="cuda", precision: str = "fp64"):
+        if precision not in ("fp64", "bf16", "fp16"):
+            raise ValueError(f"precision must be fp64, bf16 or fp16, not {precision}")
+        self.device = torch.device(device)
+        self.precision = precision
+        self.dtype = torch.float64 if precision == "fp64" else torch.float32
+        self.gemm_dtype = {"fp64": None, "bf16": torch.bfloat16, "fp16": torch.float16}[precision]
+        self.rcut = float(rcut)
+        self.rcut_smth = float(rcut_smth)
+        self.sel = [int(s) for s in sel]
+        self.sec = np.concatenate([[0], np.cumsum(self.sel)]).astype(int)
+        self.nnei = int(sum(self.sel))
+        self.ntypes = len(self.sel)
+        self.axis = int(axis)
+        dt = self.dtype
+        self.davg = torch.as_tensor(np.asarray(davg, dtype=np.float64), device=self.device, dtype=dt)
+        self.dstd = torch.as_tensor(np.asarray(dstd, dtype=np.float64), device=self.device, dtype=dt)
+        self.emb = [Net(layers, self.device, dt, last_linear=False) for layers in emb]
+        self.fit = [Net(layers, self.device, dt, last_linear=True) for layers in fit]
+        self.M = self.emb[0].w[-1].shape[1]
+        self.bias_atom_e = torch.as_tensor(np.asarray(bias_atom_e, dtype=np.float64).reshape(-1),
+                                           device=self.device)
+        ob = np.zeros(self.ntypes) if out_bias is None else np.asarray(out_bias, dtype=np.float64).reshape(-1)
+        self.out_bias = torch.as_tensor(ob, device=self.device)
+
+    # ----------------------------------------------------------------- constructors
+    @classmethod
+    def from_params(cls, spec, params, device="cuda", precision="fp64"):
+        return cls(spec.rcut, spec.rcut_smth, spec.sel, spec.axis_neuron, params["davg"], params["dstd"],
+                   params["embedding"], params["fitting"], params["bias_atom_e"], device=device,
+                   precision=precision)
+
+    @classmethod
+    def from_serialized(cls, m: dict, device="cuda", precision="fp64"):
+        """From the 'model' block of the reference serialization
+        (deepmd/pt/model/descriptor/se_a.py:392-443 layout)."""
+        d = m["descriptor"]
+        f = m["fitting"]
+        emb = [net["layers"] for net in d["embeddings"]["networks"]]
+        fit = [net["layers"] for net in f["nets"]["networks"]]
+        return cls(d["rcut"], d["rcut_smth"], d["sel"], d["axis_neuron"], d["@variables"]["davg"],
+                   d["@variables"]["dstd"], emb, fit, f["@variables"]["bias_atom_e"],
+                   m["@variables"]["out_bias"], device=device, precision=precision)
+
+    # ----------------------------------------------------------------- pieces
+    def neighbor_list(self, coord, atype, box):
+        return ops.build_nlist(coord, atype, box, self.rcut, self.sel)
+
+    def env_mat(self, nlist, ext_c, ext_t):
+        em, _, _ = ops.env_mat_a(nlist, ext_c.to(self.dtype), ext_t, self.davg, self.dstd, self.rcut,
+                                 self.rcut_smth, self.sel, with_deriv=False, with_rij=False)
+        return em
+
+    def descriptor(self, em):
+        """G^T R / nnei and D = (G^T R)^T (G^T R)[:, :axis] (se_a.py:802-858)."""
+        nloc = em.shape[0]
+        gr = None
+        for t in range(self.ntypes):
+            rr = em[:, self.sec[t]:self.sec[t + 1], :]
+            g = self.emb[t](rr[:, :, :1].reshape(-1, 1), self.gemm_dtype).reshape(nloc, -1, self.M)
+            if self.gemm_dtype is None:
+                c = torch.bmm(rr.transpose(1, 2), g)
+            else:
+                c = _LowPrecisionBMM.apply(rr.transpose(1, 2), g, self.gemm_dtype)
+            gr = c if gr is None else gr + c
+        gr = gr / self.nnei
+        d = torch.bmm(gr.transpose(1, 2), gr[:, :, : self.axis])
+        return d.reshape(nloc, -1)
+
+    def atom_energy(self, desc, atype):
+        e = torch.empty(desc.shape[0], dtype=torch.float64, device=desc.device)
+        for t in range(self.ntypes):
+            idx = (atype == t).nonzero().squeeze(1)
+            if idx.numel() == 0:
+                continue
+            y = self.fit[t](desc.index_select(0, idx), self.gemm_dtype)[:, 0].to(torch.float64)
+            e.index_copy_(0, idx, y + self.bias_atom_e[t] + self.out_bias[t])
+        return e
+
+    # ----------------------------------------------------------------- E + F + V
+    def compute(self, coord: torch.Tensor, atype: torch.Tensor, box, need_atom_energy: bool = False):
+        """One E+F+V evaluation of one frame.  coord [nloc, 3] float64 on the
+        GPU, atype [nloc] int; box: 9 values or None.  The neighbor list is
+        rebuilt every call."""
+        atype = atype.to(torch.int32)
+        ext_c, ext_t, mapping, nlist, _ovf, _ = self.neighbor_list(coord, atype, box)
+        em = self.env_mat(nlist, ext_c, ext_t).requires_grad_(True)
+        with torch.enable_grad():
+            desc = self.descriptor(em)
+            ae = self.atom_energy(desc, atype)
+            energy = ae.sum()
+            (net_deriv,) = torch.autograd.grad(energy, em)
+        force, virial = ops.force_virial_recompute(net_deriv, nlist, mapping, ext_c, ext_t, self.dstd, self.rcut,
+                                                   self.rcut_smth, self.sel)
+        out = {"energy": energy.detach(), "force": force, "virial": virial}
+        if need_atom_energy:
+            out["atom_energy"] = ae.detach()
+        return out

@@ -29,6 +29,48 @@ from ._lib import call, ptr
 
 # --------------------------------------------------------------------------- helpers
 
+_TIMED: dict | None = None
+
+
+class KernelTimer:
+    """Records HIP events on the launching stream around the named libdpmi
+    entry points while active (used by bench.py for live per-kernel time)."""
+
+    def __init__(self, *names):
+        self.names = names
+        self.events = {n: [] for n in names}
+
+    def __enter__(self):
+        global _TIMED
+        _TIMED = self.events
+        return self
+
+    def __exit__(self, *exc):
+        global _TIMED
+        _TIMED = None
+
+    def summary(self) -> dict:
+        torch.cuda.synchronize()
+        out = {}
+        for n, evs in self.events.items():
+            ms = [a.elapsed_time(b) for a, b in evs]
+            out[n] = {"launches": len(ms), "total_ms": float(sum(ms)),
+                      "avg_ms": float(sum(ms) / len(ms)) if ms else 0.0}
+        return out
+
+
+def _launch(name, *args):
+    if _TIMED is not None and name in _TIMED:
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        call(name, *args)
+        b.record()
+        _TIMED[name].append((a, b))
+    else:
+        call(name, *args)
+
+
 
 def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
@@ -102,7 +144,7 @@ def build_nlist(coord: torch.Tensor, atype: torch.Tensor, box, rcut: float, sel,
         ext_t = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
         mapping = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
         try:
-            call("dpmi_copy_coord_f64", ptr(ext_c), ptr(ext_t), ptr(mapping), ctypes.byref(nall), cap,
+            _launch("dpmi_copy_coord_f64", ptr(ext_c), ptr(ext_t), ptr(mapping), ctypes.byref(nall), cap,
                  ptr(coord), ptr(atype), nloc, ptr(b) if b is not None else None, rcut, ptr(ws), ws_bytes,
                  _stream())
             break
@@ -115,7 +157,7 @@ def build_nlist(coord: torch.Tensor, atype: torch.Tensor, box, rcut: float, sel,
     nlist = torch.empty((nloc, nnei), dtype=torch.int32, device=dev)
     overflow = torch.empty(nloc, dtype=torch.int32, device=dev)
     nl_map = torch.empty((nloc, nnei), dtype=torch.int32, device=dev) if mapped else None
-    call("dpmi_build_nlist_f64", ptr(nlist), ptr(overflow), ptr(nl_map), ptr(ext_c), ptr(ext_t), ptr(mapping),
+    _launch("dpmi_build_nlist_f64", ptr(nlist), ptr(overflow), ptr(nl_map), ptr(ext_c), ptr(ext_t), ptr(mapping),
          nloc, n, rcut, ptr(s), len(s), ptr(ws), ws_bytes, _stream())
     return ext_c, ext_t, mapping, nlist, overflow, nl_map
 
@@ -134,7 +176,7 @@ def env_mat_a(nlist, ext_coord, ext_type, avg, std, rcut, rcut_smth, sel, with_d
     ed = torch.empty((nloc, nnei, 4, 3), dtype=dt, device=dev) if with_deriv else None
     rij = torch.empty((nloc, nnei, 3), dtype=dt, device=dev) if with_rij else None
     s = _sel_host(sel)
-    call(f"dpmi_prod_env_mat_a_{sfx}", ptr(em), ptr(ed), ptr(rij), ptr(_c(nlist, torch.int32)), ptr(_c(ext_coord)),
+    _launch(f"dpmi_prod_env_mat_a_{sfx}", ptr(em), ptr(ed), ptr(rij), ptr(_c(nlist, torch.int32)), ptr(_c(ext_coord)),
          ptr(_c(ext_type, torch.int32)), nloc, ptr(_c(avg, dt)), ptr(_c(std, dt)), rcut, rcut_smth, ptr(s), len(s),
          _stream())
     return em, ed, rij
@@ -155,7 +197,7 @@ def force_virial_recompute(net_deriv, nlist, mapping, ext_coord, ext_type, std, 
     call("dpmi_prod_virial_workspace_bytes", ctypes.byref(wsb), nloc, 1)
     ws = torch.empty(int(wsb.value), dtype=torch.uint8, device=dev)
     s = _sel_host(sel)
-    call(f"dpmi_force_virial_recompute_{sfx}", ptr(force), ptr(virial), ptr(_c(net_deriv)),
+    _launch(f"dpmi_force_virial_recompute_{sfx}", ptr(force), ptr(virial), ptr(_c(net_deriv)),
          ptr(_c(nlist, torch.int32)), ptr(_c(mapping, torch.int32)) if mapping is not None else None,
          ptr(_c(ext_coord, torch.float64)), ptr(_c(ext_type, torch.int32)), nloc, nall,
          ptr(_c(std, net_deriv.dtype)), rcut, rcut_smth, ptr(s), len(s), ptr(ws), int(wsb.value), _stream())
@@ -167,7 +209,7 @@ def prod_force(net_deriv, in_deriv, nlist, nloc: int, nall: int, nframes: int = 
     dt = net_deriv.dtype
     nnei = nlist.numel() // (nloc * nframes) if nloc * nframes > 0 else 0
     force = torch.empty((nframes, nall * 3), dtype=dt, device=net_deriv.device)
-    call(f"dpmi_prod_force_se_a_{_sfx(dt)}", ptr(force), ptr(_c(net_deriv)), ptr(_c(in_deriv, dt)),
+    _launch(f"dpmi_prod_force_se_a_{_sfx(dt)}", ptr(force), ptr(_c(net_deriv)), ptr(_c(in_deriv, dt)),
          ptr(_c(nlist, torch.int32)), nloc, nall, nnei, nframes, _stream())
     return force
 
@@ -182,7 +224,7 @@ def prod_virial(net_deriv, in_deriv, rij, nlist, nloc: int, nall: int, nframes: 
     wsb = ctypes.c_int64(0)
     call("dpmi_prod_virial_workspace_bytes", ctypes.byref(wsb), nloc, nframes)
     ws = torch.empty(int(wsb.value), dtype=torch.uint8, device=dev)
-    call(f"dpmi_prod_virial_se_a_{_sfx(dt)}", ptr(virial), ptr(atom_virial), ptr(_c(net_deriv)),
+    _launch(f"dpmi_prod_virial_se_a_{_sfx(dt)}", ptr(virial), ptr(atom_virial), ptr(_c(net_deriv)),
          ptr(_c(in_deriv, dt)), ptr(_c(rij, dt)), ptr(_c(nlist, torch.int32)), nloc, nall, nnei, nframes, ptr(ws),
          int(wsb.value), _stream())
     return virial, atom_virial
@@ -214,7 +256,7 @@ def tabulate_forward(table, table_info, em_x, em, last_layer_size: int, is_sorte
     nloc, nnei = _tab_dims(em_x, em)
     out = torch.empty((nloc, 4, last_layer_size), dtype=dt, device=table.device)
     info = _info_host(table_info, dt)
-    call(f"dpmi_tabulate_fusion_se_a_{_sfx(dt)}", ptr(out), ptr(_c(table)), ptr(info), ptr(_c(em_x, dt)),
+    _launch(f"dpmi_tabulate_fusion_se_a_{_sfx(dt)}", ptr(out), ptr(_c(table)), ptr(info), ptr(_c(em_x, dt)),
          ptr(_c(em, dt)), nloc, nnei, last_layer_size, int(is_sorted), _stream())
     return out
 
@@ -229,7 +271,7 @@ def tabulate_grad(table, table_info, em_x, em, dy, is_sorted: bool = True):
     dx = torch.empty_like(em_x, dtype=dt)
     dem = torch.empty_like(em, dtype=dt)
     info = _info_host(table_info, dt)
-    call(f"dpmi_tabulate_fusion_se_a_grad_{_sfx(dt)}", ptr(dx), ptr(dem), ptr(_c(table)), ptr(info),
+    _launch(f"dpmi_tabulate_fusion_se_a_grad_{_sfx(dt)}", ptr(dx), ptr(dem), ptr(_c(table)), ptr(info),
          ptr(_c(em_x, dt)), ptr(_c(em, dt)), ptr(_c(dy, dt)), nloc, nnei, M, int(is_sorted), _stream())
     return dx, dem
 
@@ -244,7 +286,7 @@ def tabulate_grad_grad(table, table_info, em_x, em, dz_dy_dem_x, dz_dy_dem, M: i
     nloc, nnei = _tab_dims(em_x, em)
     out = torch.empty((nloc, 4, M), dtype=dt, device=table.device)
     info = _info_host(table_info, dt)
-    call(f"dpmi_tabulate_fusion_se_a_grad_grad_{_sfx(dt)}", ptr(out), ptr(_c(table)), ptr(info), ptr(_c(em_x, dt)),
+    _launch(f"dpmi_tabulate_fusion_se_a_grad_grad_{_sfx(dt)}", ptr(out), ptr(_c(table)), ptr(info), ptr(_c(em_x, dt)),
          ptr(_c(em, dt)), ptr(_c(dz_dy_dem_x, dt)), ptr(_c(dz_dy_dem, dt)), nloc, nnei, M, int(is_sorted), _stream())
     return out
 
