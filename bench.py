@@ -104,8 +104,9 @@ def main() -> int:
 
     names64 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f64",
                "dpmi_force_virial_recompute_f64"]
-    names32 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f32",
-               "dpmi_force_virial_recompute_f32"]
+    names32 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f32c64",
+               "dpmi_force_virial_recompute_f32", "dpmi_se_a_embed_fwd_bf16", "dpmi_se_a_descriptor_fwd",
+               "dpmi_se_a_descriptor_bwd", "dpmi_se_a_embed_bwd_bf16"]
     m64 = SeAEnergyModel.from_params(spec, params, device=dev, precision="fp64")
     t64, out64, k64 = run(m64, names64)
     del m64

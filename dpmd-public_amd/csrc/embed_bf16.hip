@@ -1,0 +1,441 @@
+// Fused se_e2_a embedding net + G^T R on gfx950 MFMA (bf16 operands, fp32
+// accumulation), its recompute-based adjoint, and the (G^T R)(G^T R<)^T
+// descriptor with its adjoint.
+//
+// Reference semantics: DescrptBlockSeA.forward (deepmd/pt/model/descriptor/
+// se_a.py:802-858) with the embedding layers of MLPLayer.forward
+// (deepmd/pt/model/network/mlp.py:188-219), type_one_side: for neighbor type t
+// the rows are the sel_t slots of every atom, s = em[..., 0],
+//   h1 = tanh(s w1 + b1) * idt1,           (1 -> n1)
+//   h2 = tanh(h1 W2 + b2) * idt2 + [h1,h1], (n1 -> 2 n1)
+//   G  = tanh(h2 W3 + b3) * idt3 + [h2,h2], (2 n1 -> 4 n1 = M)
+//   GR = sum_t em_t^T G_t / nnei,   D = GR^T GR[:, :axis].
+//
+// MI355X mapping.  One 64-lane wavefront owns one atom's section of one
+// neighbor type (four atoms of the same type per workgroup); rows are
+// processed 16 at a time.  With v_mfma_f32_16x16x32_bf16 (A: lane -> M row
+// lane%16, K 8*(lane/16)+u; B: lane -> N col lane%16, same K; D: lane -> col
+// lane%16, rows 4*(lane/16)+v) each layer is computed transposed,
+// y^T = W^T x^T, so the neighbor row is the N (lane) index everywhere and the
+// D fragment of one layer is directly the B fragment of the next: the K order
+// inside a 32-wide chunk is the permutation phi(g,u) = 16*(u/4) + 4g + u%4,
+// baked into the pre-arranged weight fragments.  Features use an internal
+// order in which the [x, x] skip connections land on the same lane and
+// register: q1 = f (f < n1), q2 = q1(f mod n1) + 32*[f >= n1],
+// q3 = q2(f mod 2n1) + 64*[f >= 2n1]; padded features are exact zeros.
+// G^T R contracts over rows, so G goes once through LDS (transposed to
+// [feature][row]) and is reduced with MFMA against em^T.  The backward pass
+// recomputes the forward per tile and runs the adjoint chain (dG = em dGR,
+// dh2 = dz3 W3^T + skip, dh1 = dz2 W2^T + skip, ds = dz1 . w1) in the same
+// register layout, so it needs no transposes at all.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "dpmi_internal.h"
+
+namespace dpmi {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMaxTypes = 16;
+constexpr int kFragFwd = 20;   // 4 (layer 2) + 16 (layer 3)
+constexpr int kFragBwd = 20;   // 16 (dh2) + 4 (dh1)
+constexpr int kConsts = 480;   // w1,b1,idt1 [32]; b2,idt2 [64]; b3,idt3 [128]
+constexpr int kQ3 = 128;       // internal width of G
+constexpr int kGStride = 40;   // LDS row stride (bf16) of the transposed G tile
+constexpr int kAtoms = 4;      // atoms (wavefronts) per workgroup
+
+struct EmbedParams {
+  int ntypes;
+  int nnei;
+  int sec[kMaxTypes + 1];
+};
+
+__device__ inline float fast_tanh(float x) {
+  // tanh(x) = 1 - 2 / (exp(2x) + 1); exact limits at +-inf
+  float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
+  return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + e);
+}
+
+__device__ inline f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Layer-1 feature owned by lane group g at B-slot u (also the K order phi).
+__device__ inline int phi(int g, int u) { return 16 * (u >> 2) + 4 * g + (u & 3); }
+
+// Recompute of the forward chain for one 16-row tile.  Outputs the tanh values
+// of every layer (for the backward) and G; all in the lane's layout.
+struct TileFwd {
+  float t1[8], h1[8];
+  float t2[16], h2[16];
+  float t3[32], g[32];
+};
+
+__device__ inline void forward_tile(float s, const bf16x8* __restrict__ frag, const float* __restrict__ cst, int g,
+                                    int lane, TileFwd& f) {
+  bf16x8 b1;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    int q = phi(g, u);
+    float t = fast_tanh(s * cst[q] + cst[32 + q]);
+    f.t1[u] = t;
+    f.h1[u] = t * cst[64 + q];
+    b1[u] = (__bf16)f.h1[u];
+  }
+  f32x4 z2[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) z2[mt] = mfma(frag[mt * 64 + lane], b1, f32x4{0.f, 0.f, 0.f, 0.f});
+  bf16x8 b2[2];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    const int q = 16 * mt + 4 * g;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      float t = fast_tanh(z2[mt][v] + cst[96 + q + v]);
+      f.t2[mt * 4 + v] = t;
+      f.h2[mt * 4 + v] = t * cst[160 + q + v] + f.h1[(mt & 1) * 4 + v];
+      b2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)f.h2[mt * 4 + v];
+    }
+  }
+#pragma unroll
+  for (int mt = 0; mt < 8; ++mt) {
+    f32x4 z = mfma(frag[(4 + 2 * mt) * 64 + lane], b2[0], f32x4{0.f, 0.f, 0.f, 0.f});
+    z = mfma(frag[(5 + 2 * mt) * 64 + lane], b2[1], z);
+    const int q = 16 * mt + 4 * g;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      float t = fast_tanh(z[v] + cst[224 + q + v]);
+      f.t3[mt * 4 + v] = t;
+      f.g[mt * 4 + v] = t * cst[352 + q + v] + f.h2[(mt & 3) * 4 + v];
+    }
+  }
+}
+
+// Forward: per (atom, neighbor type) wavefront, GR partial [t][i][c][q3].
+__global__ void __launch_bounds__(256)
+    k_embed_fwd(float* __restrict__ gr_part, const float* __restrict__ em, const bf16x8* __restrict__ frag_all,
+                const float* __restrict__ consts, int nloc, EmbedParams p) {
+  __shared__ bf16x8 s_frag[kFragFwd * 64];
+  __shared__ __attribute__((aligned(16))) __bf16 s_g[kAtoms][kQ3 * kGStride];
+  __shared__ __attribute__((aligned(16))) __bf16 s_em[kAtoms][4 * kGStride];
+  const int t = blockIdx.y;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  const bf16x8* frag_t = frag_all + (int64_t)t * (kFragFwd + kFragBwd) * 64;
+  for (int k = threadIdx.x; k < kFragFwd * 64; k += blockDim.x) s_frag[k] = frag_t[k];
+  const float* cst = consts + (int64_t)t * kConsts;
+  const int i = blockIdx.x * kAtoms + w;
+  const bool active = i < nloc;
+  const int sel = p.sec[t + 1] - p.sec[t];
+  const int nchunk = (sel + 31) / 32;
+  __syncthreads();
+  f32x4 acc[8];
+#pragma unroll
+  for (int nt = 0; nt < 8; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const float* em_i = em + ((int64_t)(active ? i : 0) * p.nnei + p.sec[t]) * 4;
+  for (int ch = 0; ch < nchunk; ++ch) {
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt) {
+      const int row = ch * 32 + rt * 16 + r;
+      float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
+      if (active && row < sel) {
+        const float4 ev = *reinterpret_cast<const float4*>(em_i + 4 * row);
+        e0 = ev.x;
+        e1 = ev.y;
+        e2 = ev.z;
+        e3 = ev.w;
+      }
+      if (g == 0) {
+        s_em[w][0 * kGStride + rt * 16 + r] = (__bf16)e0;
+        s_em[w][1 * kGStride + rt * 16 + r] = (__bf16)e1;
+        s_em[w][2 * kGStride + rt * 16 + r] = (__bf16)e2;
+        s_em[w][3 * kGStride + rt * 16 + r] = (__bf16)e3;
+      }
+      TileFwd f;
+      forward_tile(e0, s_frag, cst, g, lane, f);
+#pragma unroll
+      for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s_g[w][(16 * mt + 4 * g + v) * kGStride + rt * 16 + r] = (__bf16)f.g[mt * 4 + v];
+    }
+    __syncthreads();
+    // GR^T tile: D(M = c, N = q3) += em^T (M = c, K = 32 rows) * G (K = rows, N = q3)
+    bf16x8 a;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = (__bf16)0.f;
+    if (r < 4) a = *reinterpret_cast<const bf16x8*>(&s_em[w][r * kGStride + 8 * g]);
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt) {
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(&s_g[w][(16 * nt + r) * kGStride + 8 * g]);
+      acc[nt] = mfma(a, b, acc[nt]);
+    }
+    __syncthreads();
+  }
+  if (active && g == 0) {
+    float* out = gr_part + ((int64_t)t * nloc + i) * 4 * kQ3;
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) out[v * kQ3 + 16 * nt + r] = acc[nt][v];
+  }
+}
+
+// internal G index of original feature f (n1 = width of layer 1)
+__device__ inline int q3_of(int f, int n1) {
+  int n2 = 2 * n1;
+  int base = f >= n2 ? 64 : 0;
+  int f2 = f >= n2 ? f - n2 : f;
+  return base + (f2 >= n1 ? 32 + (f2 - n1) : f2);
+}
+
+// GR = sum_t GR_t / nnei (internal order, saved for the backward) and
+// D[m][a] = sum_c GR[c][m] GR[c][a] in the original order (se_a.py:851-858).
+__global__ void __launch_bounds__(256)
+    k_descrpt_fwd(float* __restrict__ gr, float* __restrict__ desc, const float* __restrict__ gr_part, int nloc,
+                  int ntypes, float inv_nnei, int n1, int axis) {
+  __shared__ float s[kAtoms][4 * kQ3];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * kAtoms + w;
+  if (i >= nloc) return;
+  for (int k = lane; k < 4 * kQ3; k += 64) {
+    float v = 0.f;
+    for (int t = 0; t < ntypes; ++t) v += gr_part[((int64_t)t * nloc + i) * 4 * kQ3 + k];
+    v *= inv_nnei;
+    s[w][k] = v;
+    gr[(int64_t)i * 4 * kQ3 + k] = v;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const int M = 4 * n1;
+  for (int k = lane; k < M * axis; k += 64) {
+    const int m = k / axis, a = k - m * axis;
+    const int qm = q3_of(m, n1), qa = q3_of(a, n1);
+    float d = 0.f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) d += s[w][c * kQ3 + qm] * s[w][c * kQ3 + qa];
+    desc[(int64_t)i * M * axis + k] = d;
+  }
+}
+
+// dGR_raw[c][q3(k)] = (sum_a dD[k][a] GR[c][a] + [k < axis] sum_m dD[m][k] GR[c][m]) / nnei
+__global__ void __launch_bounds__(256)
+    k_descrpt_bwd(float* __restrict__ dgr, const float* __restrict__ ddesc, const float* __restrict__ gr, int nloc,
+                  float inv_nnei, int n1, int axis) {
+  __shared__ float s[kAtoms][4 * kQ3];
+  __shared__ float sd[kAtoms][128 * 16];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * kAtoms + w;
+  if (i >= nloc) return;
+  const int M = 4 * n1;
+  for (int k = lane; k < 4 * kQ3; k += 64) s[w][k] = gr[(int64_t)i * 4 * kQ3 + k];
+  for (int k = lane; k < M * axis; k += 64) sd[w][k] = ddesc[(int64_t)i * M * axis + k];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int q = lane; q < kQ3; q += 64) {
+    // original feature of internal q (or -1 for padding)
+    int base = q >= 64 ? 2 * n1 : 0, q2 = q & 63;
+    int f2 = q2 >= 32 ? (q2 - 32 < n1 ? n1 + (q2 - 32) : -1) : (q2 < n1 ? q2 : -1);
+    int k = f2 < 0 ? -1 : base + f2;
+    float out[4] = {0.f, 0.f, 0.f, 0.f};
+    if (k >= 0) {
+      for (int a = 0; a < axis; ++a) {
+        float dd = sd[w][k * axis + a];
+        int qa = q3_of(a, n1);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) out[c] += dd * s[w][c * kQ3 + qa];
+      }
+      if (k < axis) {
+        for (int m = 0; m < M; ++m) {
+          float dd = sd[w][m * axis + k];
+          int qm = q3_of(m, n1);
+#pragma unroll
+          for (int c = 0; c < 4; ++c) out[c] += dd * s[w][c * kQ3 + qm];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) dgr[(int64_t)i * 4 * kQ3 + c * kQ3 + q] = out[c] * inv_nnei;
+  }
+}
+
+// Backward: net_deriv[i][sec_t + row][c] = dE/d em, recomputing the forward.
+__global__ void __launch_bounds__(256)
+    k_embed_bwd(float* __restrict__ net_deriv, const float* __restrict__ em, const float* __restrict__ dgr,
+                const bf16x8* __restrict__ frag_all, const float* __restrict__ consts, int nloc, EmbedParams p) {
+  __shared__ bf16x8 s_frag[(kFragFwd + kFragBwd) * 64];
+  const int t = blockIdx.y;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  const bf16x8* frag_t = frag_all + (int64_t)t * (kFragFwd + kFragBwd) * 64;
+  for (int k = threadIdx.x; k < (kFragFwd + kFragBwd) * 64; k += blockDim.x) s_frag[k] = frag_t[k];
+  __syncthreads();
+  const bf16x8* fb = s_frag + kFragFwd * 64;
+  const float* cst = consts + (int64_t)t * kConsts;
+  const int i = blockIdx.x * kAtoms + w;
+  if (i >= nloc) return;
+  const int sel = p.sec[t + 1] - p.sec[t];
+  const float* dg = dgr + (int64_t)i * 4 * kQ3;
+  // dGR^T as A (M = q3 tile, K = c): only K slots 0..3 (lane group 0) are non-zero
+  bf16x8 a_dgt[8];
+#pragma unroll
+  for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a_dgt[mt][u] = (__bf16)((g == 0 && u < 4) ? dg[u * kQ3 + 16 * mt + r] : 0.f);
+  // dGR as A (M = c, K = q3 chunk in phi order): rows c >= 4 are zero
+  bf16x8 a_dg[4];
+#pragma unroll
+  for (int kc = 0; kc < 4; ++kc)
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a_dg[kc][u] = (__bf16)(r < 4 ? dg[r * kQ3 + 32 * kc + phi(g, u)] : 0.f);
+  const float* em_i = em + ((int64_t)i * p.nnei + p.sec[t]) * 4;
+  float* nd_i = net_deriv + ((int64_t)i * p.nnei + p.sec[t]) * 4;
+  const int ntile = (sel + 15) / 16;
+  for (int tl = 0; tl < ntile; ++tl) {
+    const int row = tl * 16 + r;
+    float4 ev = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (row < sel) ev = *reinterpret_cast<const float4*>(em_i + 4 * row);
+    TileFwd f;
+    forward_tile(ev.x, s_frag, cst, g, lane, f);
+    // em as B (N = rows, K = c)
+    bf16x8 b_em;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) b_em[u] = (__bf16)0.f;
+    if (g == 0) {
+      b_em[0] = (__bf16)ev.x;
+      b_em[1] = (__bf16)ev.y;
+      b_em[2] = (__bf16)ev.z;
+      b_em[3] = (__bf16)ev.w;
+    }
+    // direct term dem[c] = sum_q3 G dGR[c]  -> D(M = c, N = rows)
+    f32x4 dem = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc) {
+      bf16x8 bg;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) bg[u] = (__bf16)f.g[(2 * kc + (u >> 2)) * 4 + (u & 3)];
+      dem = mfma(a_dg[kc], bg, dem);
+    }
+    // dG = em dGR  -> D(M = q3, N = rows); dz3 = dG * idt3 * (1 - t3^2)
+    float dgv[32];
+    bf16x8 b_dz3[4];
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt) {
+      f32x4 d = mfma(a_dgt[mt], b_em, f32x4{0.f, 0.f, 0.f, 0.f});
+      const int q = 16 * mt + 4 * g;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        dgv[mt * 4 + v] = d[v];
+        float tt = f.t3[mt * 4 + v];
+        float dz = d[v] * cst[352 + q + v] * (1.f - tt * tt);
+        b_dz3[mt >> 1][(mt & 1) * 4 + v] = (__bf16)dz;
+      }
+    }
+    // dh2 = dz3 W3^T + dG[q2] + dG[q2 + 64];  dz2 = dh2 * idt2 * (1 - t2^2)
+    float dh2[16];
+    bf16x8 b_dz2[2];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      f32x4 d = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kc = 0; kc < 4; ++kc) d = mfma(fb[(mt * 4 + kc) * 64 + lane], b_dz3[kc], d);
+      const int q = 16 * mt + 4 * g;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        float x = d[v] + dgv[mt * 4 + v] + dgv[(mt + 4) * 4 + v];
+        dh2[mt * 4 + v] = x;
+        float tt = f.t2[mt * 4 + v];
+        b_dz2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(x * cst[160 + q + v] * (1.f - tt * tt));
+      }
+    }
+    // dh1 = dz2 W2^T + dh2[q1] + dh2[q1 + 32];  ds = sum_f1 dh1 * idt1 * (1 - t1^2) * w1
+    float ds = 0.f;
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      f32x4 d = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kc = 0; kc < 2; ++kc) d = mfma(fb[(16 + mt * 2 + kc) * 64 + lane], b_dz2[kc], d);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int u = mt * 4 + v, q = phi(g, u);
+        float x = d[v] + dh2[mt * 4 + v] + dh2[(mt + 2) * 4 + v];
+        float tt = f.t1[u];
+        ds += x * cst[64 + q] * (1.f - tt * tt) * cst[q];
+      }
+    }
+    ds += __shfl_xor(ds, 16, 64);
+    ds += __shfl_xor(ds, 32, 64);
+    if (g == 0 && row < sel) {
+      float4 o = make_float4(dem[0] + ds, dem[1], dem[2], dem[3]);
+      *reinterpret_cast<float4*>(nd_i + 4 * row) = o;
+    }
+  }
+}
+
+int make_params(EmbedParams* p, const int* sel_host, int ntypes) {
+  DPMI_CHECK_ARG(sel_host != nullptr, "sel is NULL");
+  DPMI_CHECK_ARG(ntypes > 0 && ntypes <= kMaxTypes, "ntypes out of range [1, 16]");
+  std::memset(p, 0, sizeof(*p));
+  p->ntypes = ntypes;
+  for (int t = 0; t < ntypes; ++t) {
+    DPMI_CHECK_ARG(sel_host[t] >= 0, "sel must be >= 0");
+    p->sec[t + 1] = p->sec[t] + sel_host[t];
+  }
+  p->nnei = p->sec[ntypes];
+  return DPMI_OK;
+}
+
+}  // namespace
+}  // namespace dpmi
+
+using namespace dpmi;
+
+extern "C" int dpmi_se_a_embed_fwd_bf16(float* gr_part, const float* em, const void* frags, const float* consts,
+                                        int64_t nloc, const int* sel_host, int ntypes, void* stream) {
+  EmbedParams p;
+  int rc = make_params(&p, sel_host, ntypes);
+  if (rc) return rc;
+  DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
+  if (nloc == 0) return DPMI_OK;
+  dim3 grid((unsigned)ceil_div(nloc, kAtoms), (unsigned)ntypes);
+  hipLaunchKernelGGL(k_embed_fwd, grid, dim3(256), 0, as_stream(stream), gr_part, em,
+                     static_cast<const bf16x8*>(frags), consts, (int)nloc, p);
+  DPMI_LAUNCH_CHECK();
+  return DPMI_OK;
+}
+
+extern "C" int dpmi_se_a_descriptor_fwd(float* gr, float* desc, const float* gr_part, int64_t nloc, int ntypes,
+                                        int nnei, int n1, int axis, void* stream) {
+  DPMI_CHECK_ARG(n1 > 0 && n1 <= 32 && axis > 0 && axis <= 4 * n1 && axis <= 16, "unsupported widths");
+  if (nloc == 0) return DPMI_OK;
+  hipLaunchKernelGGL(k_descrpt_fwd, dim3(ceil_div(nloc, kAtoms)), dim3(256), 0, as_stream(stream), gr, desc, gr_part,
+                     (int)nloc, ntypes, 1.f / (float)nnei, n1, axis);
+  DPMI_LAUNCH_CHECK();
+  return DPMI_OK;
+}
+
+extern "C" int dpmi_se_a_descriptor_bwd(float* dgr, const float* ddesc, const float* gr, int64_t nloc, int nnei,
+                                        int n1, int axis, void* stream) {
+  DPMI_CHECK_ARG(n1 > 0 && n1 <= 32 && axis > 0 && axis <= 4 * n1 && axis <= 16, "unsupported widths");
+  if (nloc == 0) return DPMI_OK;
+  hipLaunchKernelGGL(k_descrpt_bwd, dim3(ceil_div(nloc, kAtoms)), dim3(256), 0, as_stream(stream), dgr, ddesc, gr,
+                     (int)nloc, 1.f / (float)nnei, n1, axis);
+  DPMI_LAUNCH_CHECK();
+  return DPMI_OK;
+}
+
+extern "C" int dpmi_se_a_embed_bwd_bf16(float* net_deriv, const float* em, const float* dgr, const void* frags,
+                                        const float* consts, int64_t nloc, const int* sel_host, int ntypes,
+                                        void* stream) {
+  EmbedParams p;
+  int rc = make_params(&p, sel_host, ntypes);
+  if (rc) return rc;
+  DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
+  if (nloc == 0) return DPMI_OK;
+  dim3 grid((unsigned)ceil_div(nloc, kAtoms), (unsigned)ntypes);
+  hipLaunchKernelGGL(k_embed_bwd, grid, dim3(256), 0, as_stream(stream), net_deriv, em, dgr,
+                     static_cast<const bf16x8*>(frags), consts, (int)nloc, p);
+  DPMI_LAUNCH_CHECK();
+  return DPMI_OK;
+}

@@ -108,10 +108,12 @@ __device__ inline void section_limits(const int* __restrict__ nl, const SecParam
   __syncthreads();
 }
 
-template <typename FP>
+// FP: arithmetic and output type; CT: coordinate type (the difference
+// x_j - x_i is formed in CT, then rounded to FP).
+template <typename FP, typename CT>
 __global__ void __launch_bounds__(256)
     k_env_mat_a(FP* __restrict__ em, FP* __restrict__ em_deriv, FP* __restrict__ rij, const int* __restrict__ nlist,
-                const FP* __restrict__ ext_c, const int* __restrict__ ext_t, int nloc, const FP* __restrict__ avg,
+                const CT* __restrict__ ext_c, const int* __restrict__ ext_t, int nloc, const FP* __restrict__ avg,
                 const FP* __restrict__ std_, float rmin, float rmax, SecParams sp) {
   __shared__ int lim_all[kAtomsPerBlock][kMaxTypes];
   const int w = threadIdx.x / kWave, lane = lane_id();
@@ -124,7 +126,7 @@ __global__ void __launch_bounds__(256)
   if (!active) return;
   int ti = ext_t[i];
   if (ti >= sp.ntypes) ti = -1;  // no statistics for such a type: treated as virtual
-  const FP xi0 = ext_c[3 * i + 0], xi1 = ext_c[3 * i + 1], xi2 = ext_c[3 * i + 2];
+  const CT xi0 = ext_c[3 * i + 0], xi1 = ext_c[3 * i + 1], xi2 = ext_c[3 * i + 2];
   const int nem = nnei * 4;
   for (int s = lane; s < nnei; s += kWave) {
     int t = slot_type(sp, s);
@@ -136,9 +138,9 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int k = 0; k < 12; ++k) d[k] = (FP)0.;
     if (valid) {
-      rr[0] = ext_c[3 * j + 0] - xi0;
-      rr[1] = ext_c[3 * j + 1] - xi1;
-      rr[2] = ext_c[3 * j + 2] - xi2;
+      rr[0] = (FP)(ext_c[3 * j + 0] - xi0);
+      rr[1] = (FP)(ext_c[3 * j + 1] - xi1);
+      rr[2] = (FP)(ext_c[3 * j + 2] - xi2);
       env_one(rr, rmin, rmax, v, d);
     }
     int64_t o = (int64_t)i * nem + 4 * s;
@@ -370,8 +372,8 @@ __global__ void __launch_bounds__(256)
   block_virial_partial<FP>(v, partial);
 }
 
-template <typename FP>
-int env_mat_impl(FP* em, FP* em_deriv, FP* rij, const int* nlist, const FP* ext_coord, const int* ext_type,
+template <typename FP, typename CT>
+int env_mat_impl(FP* em, FP* em_deriv, FP* rij, const int* nlist, const CT* ext_coord, const int* ext_type,
                  int64_t nloc, const FP* avg, const FP* std_, float rcut, float rcut_smth, const int* sel_host,
                  int ntypes, void* stream) {
   SecParams sp;
@@ -380,7 +382,7 @@ int env_mat_impl(FP* em, FP* em_deriv, FP* rij, const int* nlist, const FP* ext_
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
   DPMI_CHECK_ARG(em != nullptr, "em is NULL");
   if (nloc == 0) return DPMI_OK;
-  hipLaunchKernelGGL(k_env_mat_a<FP>, dim3(ceil_div(nloc, kAtomsPerBlock)), dim3(256), 0, as_stream(stream), em,
+  hipLaunchKernelGGL((k_env_mat_a<FP, CT>), dim3(ceil_div(nloc, kAtomsPerBlock)), dim3(256), 0, as_stream(stream), em,
                      em_deriv, rij, nlist, ext_coord, ext_type, (int)nloc, avg, std_, rcut_smth, rcut, sp);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
@@ -435,16 +437,24 @@ extern "C" int dpmi_prod_env_mat_a_f64(double* em, double* em_deriv, double* rij
                                        const double* ext_coord, const int* ext_type, int64_t nloc, const double* avg,
                                        const double* std_, float rcut, float rcut_smth, const int* sel_host,
                                        int ntypes, void* stream) {
-  return env_mat_impl<double>(em, em_deriv, rij, nlist, ext_coord, ext_type, nloc, avg, std_, rcut, rcut_smth,
-                              sel_host, ntypes, stream);
+  return env_mat_impl<double, double>(em, em_deriv, rij, nlist, ext_coord, ext_type, nloc, avg, std_, rcut,
+                                      rcut_smth, sel_host, ntypes, stream);
 }
 
 extern "C" int dpmi_prod_env_mat_a_f32(float* em, float* em_deriv, float* rij, const int* nlist,
                                        const float* ext_coord, const int* ext_type, int64_t nloc, const float* avg,
                                        const float* std_, float rcut, float rcut_smth, const int* sel_host,
                                        int ntypes, void* stream) {
-  return env_mat_impl<float>(em, em_deriv, rij, nlist, ext_coord, ext_type, nloc, avg, std_, rcut, rcut_smth,
-                             sel_host, ntypes, stream);
+  return env_mat_impl<float, float>(em, em_deriv, rij, nlist, ext_coord, ext_type, nloc, avg, std_, rcut,
+                                    rcut_smth, sel_host, ntypes, stream);
+}
+
+extern "C" int dpmi_prod_env_mat_a_f32c64(float* em, float* em_deriv, float* rij, const int* nlist,
+                                          const double* ext_coord, const int* ext_type, int64_t nloc,
+                                          const float* avg, const float* std_, float rcut, float rcut_smth,
+                                          const int* sel_host, int ntypes, void* stream) {
+  return env_mat_impl<float, double>(em, em_deriv, rij, nlist, ext_coord, ext_type, nloc, avg, std_, rcut,
+                                     rcut_smth, sel_host, ntypes, stream);
 }
 
 extern "C" int dpmi_prod_force_se_a_f64(double* force, const double* net_deriv, const double* in_deriv,

@@ -1,0 +1,147 @@
+"""Host side of the fused se_e2_a embedding kernels (csrc/embed_bf16.hip).
+
+pack_embedding() lays the embedding-net weights of each neighbor type out
+as MFMA A-operand fragments (bf16) in the order the kernels read them, plus a
+small table of per-feature constants (fp32), both in the kernels' internal
+feature order:
+
+    q1(f) = f                                  layer 1, width n1 <= 32
+    q2(f) = f            (f <  n1)             layer 2, width 2 n1
+          = 32 + f - n1  (f >= n1)
+    q3(f) = q2(f)        (f <  2 n1)           layer 3, width 4 n1 = M
+          = 64 + q2(f - 2 n1)
+
+so that the [x, x] skip connections of MLPLayer (deepmd/pt/model/network/
+mlp.py:188-219) pair features on the same lane.  A fragment is 64 lanes x 8
+bf16; lane l holds output feature 16*mt + l%16 and the K slots
+phi(g, u) = 16*(u//4) + 4*g + u%4 of its 32-wide chunk (g = l//16).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import ops
+from ._lib import ptr
+
+N_FRAG = 40  # 20 forward + 20 backward fragments per type
+N_CONST = 480
+
+
+def _phi():
+    g = np.arange(4)[:, None]
+    u = np.arange(8)[None, :]
+    return 16 * (u // 4) + 4 * g + (u % 4)  # [4, 8]
+
+
+def internal_maps(n1: int):
+    f1 = np.arange(n1)
+    q1 = f1
+    f2 = np.arange(2 * n1)
+    q2 = np.where(f2 < n1, f2, 32 + f2 - n1)
+    f3 = np.arange(4 * n1)
+    q3 = np.where(f3 < 2 * n1, q2[np.minimum(f3, 2 * n1 - 1)], 64 + q2[np.clip(f3 - 2 * n1, 0, 2 * n1 - 1)])
+    return q1, q2, q3
+
+
+def supported(neuron) -> bool:
+    n1 = neuron[0]
+    return len(neuron) == 3 and n1 <= 32 and neuron[1] == 2 * n1 and neuron[2] == 4 * n1
+
+
+def _layer(l):
+    v = l.get("@variables", l)
+    w = np.asarray(v["w"], dtype=np.float64)
+    b = np.asarray(v["b"], dtype=np.float64)
+    idt = v.get("idt")
+    idt = np.ones_like(b) if idt is None else np.asarray(idt, dtype=np.float64)
+    return w, b, idt
+
+
+def pack_embedding(emb_layers: list) -> tuple[np.ndarray, np.ndarray, int]:
+    """emb_layers[t] = the three layer dicts of neighbor type t.
+    Returns (frags uint16 [ntypes, 40, 64, 8] holding bf16 bits,
+    consts float32 [ntypes, 480], n1)."""
+    ntypes = len(emb_layers)
+    (w1, _, _) = _layer(emb_layers[0][0])
+    n1 = w1.shape[1]
+    q1, q2, q3 = internal_maps(n1)
+    phi = _phi()  # [g, u]
+    lane = np.arange(64)
+    lg, lr = lane // 16, lane % 16
+    frags = np.zeros((ntypes, N_FRAG, 64, 8), dtype=np.float32)
+    consts = np.zeros((ntypes, N_CONST), dtype=np.float32)
+    for t, layers in enumerate(emb_layers):
+        (w1, b1, i1), (w2, b2, i2), (w3, b3, i3) = (_layer(x) for x in layers)
+        if w1.shape != (1, n1) or w2.shape != (n1, 2 * n1) or w3.shape != (2 * n1, 4 * n1):
+            raise ValueError("fused embedding needs widths (n1, 2 n1, 4 n1) with n1 <= 32")
+        W2 = np.zeros((32, 64))
+        W2[np.ix_(q1, q2)] = w2
+        W3 = np.zeros((64, 128))
+        W3[np.ix_(q2, q3)] = w3
+        c = consts[t]
+        c[q1] = w1[0]
+        c[32 + q1] = b1
+        c[64 + q1] = i1
+        c[96 + q2] = b2
+        c[160 + q2] = i2
+        c[224 + q3] = b3
+        c[352 + q3] = i3
+        K = phi[lg]  # [64, 8] K slot -> feature offset within a 32-chunk
+        f = frags[t]
+        for mt in range(4):  # layer 2, A = W2^T (M = q2, K = q1)
+            f[mt] = W2[K, (16 * mt + lr)[:, None]]
+        for mt in range(8):  # layer 3, A = W3^T (M = q3, K = q2)
+            for kc in range(2):
+                f[4 + 2 * mt + kc] = W3[32 * kc + K, (16 * mt + lr)[:, None]]
+        for mt in range(4):  # dh2 = W3 dz3 (M = q2, K = q3)
+            for kc in range(4):
+                f[20 + mt * 4 + kc] = W3[(16 * mt + lr)[:, None], 32 * kc + K]
+        for mt in range(2):  # dh1 = W2 dz2 (M = q1, K = q2)
+            for kc in range(2):
+                f[20 + 16 + mt * 2 + kc] = W2[(16 * mt + lr)[:, None], 32 * kc + K]
+    bits = torch.from_numpy(frags).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    return bits, consts, n1
+
+
+class FusedEmbedding:
+    """Device-resident packed weights + the four fused launches."""
+
+    def __init__(self, emb_layers, sel, axis: int, device):
+        bits, consts, n1 = pack_embedding(emb_layers)
+        self.n1 = n1
+        self.M = 4 * n1
+        self.axis = int(axis)
+        self.sel = np.ascontiguousarray(np.asarray(sel, dtype=np.int32))
+        self.ntypes = len(sel)
+        self.nnei = int(self.sel.sum())
+        self.frags = torch.from_numpy(bits.astype(np.int16)).to(device)
+        self.consts = torch.from_numpy(consts).to(device)
+
+    def forward(self, em: torch.Tensor):
+        """em [nloc, nnei, 4] fp32 -> (gr [nloc, 4, 128] internal, desc [nloc, M*axis])."""
+        nloc = em.shape[0]
+        dev = em.device
+        em = em.contiguous()
+        gr_part = torch.empty((self.ntypes, nloc, 4, 128), dtype=torch.float32, device=dev)
+        ops._launch("dpmi_se_a_embed_fwd_bf16", ptr(gr_part), ptr(em), ptr(self.frags), ptr(self.consts), nloc,
+                    ptr(self.sel), self.ntypes, ops._stream())
+        gr = torch.empty((nloc, 4, 128), dtype=torch.float32, device=dev)
+        desc = torch.empty((nloc, self.M * self.axis), dtype=torch.float32, device=dev)
+        ops._launch("dpmi_se_a_descriptor_fwd", ptr(gr), ptr(desc), ptr(gr_part), nloc, self.ntypes, self.nnei,
+                    self.n1, self.axis, ops._stream())
+        return gr, desc
+
+    def backward(self, em: torch.Tensor, gr: torch.Tensor, ddesc: torch.Tensor) -> torch.Tensor:
+        """dE/d desc -> dE/d em [nloc, nnei, 4] fp32."""
+        nloc = em.shape[0]
+        dgr = torch.empty_like(gr)
+        ops._launch("dpmi_se_a_descriptor_bwd", ptr(dgr), ptr(ddesc.contiguous()), ptr(gr), nloc, self.nnei, self.n1,
+                    self.axis, ops._stream())
+        nd = torch.empty((nloc, self.nnei, 4), dtype=torch.float32, device=em.device)
+        ops._launch("dpmi_se_a_embed_bwd_bf16", ptr(nd), ptr(em.contiguous()), ptr(dgr), ptr(self.frags),
+                    ptr(self.consts), nloc, ptr(self.sel), self.ntypes, ops._stream())
+        return nd

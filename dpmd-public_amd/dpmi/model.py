@@ -25,7 +25,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from . import ops
+from . import embed, ops
 
 _ACT = torch.tanh
 
@@ -124,6 +124,11 @@ class SeAEnergyModel:
         self.davg = torch.as_tensor(np.asarray(davg, dtype=np.float64), device=self.device, dtype=dt)
         self.dstd = torch.as_tensor(np.asarray(dstd, dtype=np.float64), device=self.device, dtype=dt)
         self.emb = [Net(layers, self.device, dt, last_linear=False) for layers in emb]
+        neuron = [w.shape[1] for w in self.emb[0].w]
+        # bf16 form: fused MFMA embedding kernels (else the torch layers above)
+        self.fused = None
+        if precision == "bf16" and embed.supported(neuron) and self.axis <= 16:
+            self.fused = embed.FusedEmbedding(emb, self.sel, self.axis, self.device)
         self.fit = [Net(layers, self.device, dt, last_linear=True) for layers in fit]
         self.M = self.emb[0].w[-1].shape[1]
         self.bias_atom_e = torch.as_tensor(np.asarray(bias_atom_e, dtype=np.float64).reshape(-1),
@@ -155,8 +160,8 @@ class SeAEnergyModel:
         return ops.build_nlist(coord, atype, box, self.rcut, self.sel)
 
     def env_mat(self, nlist, ext_c, ext_t):
-        em, _, _ = ops.env_mat_a(nlist, ext_c.to(self.dtype), ext_t, self.davg, self.dstd, self.rcut,
-                                 self.rcut_smth, self.sel, with_deriv=False, with_rij=False)
+        em, _, _ = ops.env_mat_a(nlist, ext_c, ext_t, self.davg, self.dstd, self.rcut, self.rcut_smth, self.sel,
+                                 with_deriv=False, with_rij=False, dtype=self.dtype)
         return em
 
     def descriptor(self, em):
@@ -192,6 +197,8 @@ class SeAEnergyModel:
         rebuilt every call."""
         atype = atype.to(torch.int32)
         ext_c, ext_t, mapping, nlist, _ovf, _ = self.neighbor_list(coord, atype, box)
+        if self.fused is not None:
+            return self._compute_fused(atype, ext_c, ext_t, mapping, nlist, need_atom_energy)
         em = self.env_mat(nlist, ext_c, ext_t).requires_grad_(True)
         with torch.enable_grad():
             desc = self.descriptor(em)
@@ -199,6 +206,26 @@ class SeAEnergyModel:
             energy = ae.sum()
             (net_deriv,) = torch.autograd.grad(energy, em)
         force, virial = ops.force_virial_recompute(net_deriv, nlist, mapping, ext_c, ext_t, self.dstd, self.rcut,
+                                                   self.rcut_smth, self.sel)
+        out = {"energy": energy.detach(), "force": force, "virial": virial}
+        if need_atom_energy:
+            out["atom_energy"] = ae.detach()
+        return out
+
+    def _compute_fused(self, atype, ext_c, ext_t, mapping, nlist, need_atom_energy):
+        """bf16 form with hand-written adjoints end to end: fused embedding +
+        G^T R forward, fitting (hipBLASLt GEMMs, autograd only through the
+        fitting net), descriptor and embedding adjoints recomputing the
+        forward, env-mat adjoint for forces and virial."""
+        em = self.env_mat(nlist, ext_c, ext_t)
+        gr, desc = self.fused.forward(em)
+        desc.requires_grad_(True)
+        with torch.enable_grad():
+            ae = self.atom_energy(desc, atype)
+            energy = ae.sum()
+            (ddesc,) = torch.autograd.grad(energy, desc)
+        nd = self.fused.backward(em, gr, ddesc)
+        force, virial = ops.force_virial_recompute(nd, nlist, mapping, ext_c, ext_t, self.dstd, self.rcut,
                                                    self.rcut_smth, self.sel)
         out = {"energy": energy.detach(), "force": force, "virial": virial}
         if need_atom_energy:

@@ -165,11 +165,16 @@ def build_nlist(coord: torch.Tensor, atype: torch.Tensor, box, rcut: float, sel,
 # --------------------------------------------------------------------------- env-mat
 
 
-def env_mat_a(nlist, ext_coord, ext_type, avg, std, rcut, rcut_smth, sel, with_deriv=True, with_rij=True):
-    """prod_env_mat_a on a formatted list of one frame -> (em, em_deriv, rij)."""
+def env_mat_a(nlist, ext_coord, ext_type, avg, std, rcut, rcut_smth, sel, with_deriv=True, with_rij=True,
+              dtype=None):
+    """prod_env_mat_a on a formatted list of one frame -> (em, em_deriv, rij).
+    dtype=float32 with float64 coordinates selects the mixed kernel (pair
+    differences in double, the rest in float)."""
     _gpu(nlist, ext_coord, ext_type, avg, std)
-    dt = ext_coord.dtype
+    dt = ext_coord.dtype if dtype is None else dtype
     sfx = _sfx(dt)
+    if dt == torch.float32 and ext_coord.dtype == torch.float64:
+        sfx = "f32c64"
     nloc, nnei = nlist.shape
     dev = ext_coord.device
     em = torch.empty((nloc, nnei, 4), dtype=dt, device=dev)

@@ -90,6 +90,11 @@ int dpmi_prod_env_mat_a_f32(float* em, float* em_deriv, float* rij, const int* n
                             const float* ext_coord, const int* ext_type, int64_t nloc, const float* avg,
                             const float* std, float rcut, float rcut_smth, const int* sel_host, int ntypes,
                             void* stream);
+/* Precision form: double coordinates, the difference x_j - x_i formed in
+ * double and everything after it in float. */
+int dpmi_prod_env_mat_a_f32c64(float* em, float* em_deriv, float* rij, const int* nlist, const double* ext_coord,
+                               const int* ext_type, int64_t nloc, const float* avg, const float* std, float rcut,
+                               float rcut_smth, const int* sel_host, int ntypes, void* stream);
 
 /* ------------------------------------------------------------------------
  * Force and virial.  prod_force_a_gpu / prod_virial_a_gpu
@@ -171,6 +176,31 @@ int dpmi_tabulate_fusion_se_a_grad_grad_f32(float* dz_dy, const float* table, co
                                             const float* em_x, const float* em, const float* dz_dy_dem_x,
                                             const float* dz_dy_dem, int64_t nloc, int64_t nnei,
                                             int64_t last_layer_size, int is_sorted, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Fused embedding net + G^T R and the descriptor, bf16 MFMA with fp32
+ * accumulation (the precision form of DescrptBlockSeA.forward,
+ * deepmd/pt/model/descriptor/se_a.py:802-858, embedding layers of
+ * deepmd/pt/model/network/mlp.py:188-219; type_one_side, widths
+ * (n1, 2 n1, 4 n1), n1 <= 32, resnet with timestep).  The reference has no
+ * single function for this: it chains torch MLP layers and matmuls.
+ * frags: [ntypes, 40, 64, 8] bf16 weight fragments and consts
+ * [ntypes, 480] fp32, both laid out by dpmi.embed.pack_embedding (internal
+ * feature order q3; see that module).  em [nloc, nnei, 4] fp32.
+ * ------------------------------------------------------------------------ */
+/* gr_part [ntypes, nloc, 4, 128]: sum over each type's neighbors of em^T G. */
+int dpmi_se_a_embed_fwd_bf16(float* gr_part, const float* em, const void* frags, const float* consts, int64_t nloc,
+                             const int* sel_host, int ntypes, void* stream);
+/* gr [nloc, 4, 128] = sum_t gr_part / nnei (internal order); desc
+ * [nloc, 4 n1 * axis] = GR^T GR[:, :axis] in the original order (se_a.py:851-858). */
+int dpmi_se_a_descriptor_fwd(float* gr, float* desc, const float* gr_part, int64_t nloc, int ntypes, int nnei,
+                             int n1, int axis, void* stream);
+/* dgr [nloc, 4, 128] = dE/d(sum_t gr_part) from ddesc = dE/d desc. */
+int dpmi_se_a_descriptor_bwd(float* dgr, const float* ddesc, const float* gr, int64_t nloc, int nnei, int n1,
+                             int axis, void* stream);
+/* net_deriv [nloc, nnei, 4] = dE/d em, recomputing the embedding forward. */
+int dpmi_se_a_embed_bwd_bf16(float* net_deriv, const float* em, const float* dgr, const void* frags,
+                             const float* consts, int64_t nloc, const int* sel_host, int ntypes, void* stream);
 
 #ifdef __cplusplus
 }

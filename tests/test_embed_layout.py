@@ -1,0 +1,193 @@
+"""CPU check of the fused-embedding register layout: a numpy emulation of the
+kernels' per-lane dataflow (MFMA 16x16x32 with the lane layout verified on
+gfx950: A lane -> row l%16, K 8*(l//16)+u; B lane -> col l%16, same K;
+D lane -> col l%16, rows 4*(l//16)+v) using the packed fragments, against
+the plain MLP / descriptor math in float64."""
+
+import numpy as np
+import pytest
+
+from dpmi import embed, systems
+
+L = np.arange(64)
+LG, LR = L // 16, L % 16
+
+
+def mfma(a, b, c=None):
+    A = np.zeros((16, 32))
+    B = np.zeros((32, 16))
+    for u in range(8):
+        A[LR, 8 * LG + u] = a[:, u]
+        B[8 * LG + u, LR] = b[:, u]
+    D = A @ B
+    out = np.stack([D[4 * LG + v, LR] for v in range(4)], axis=1)
+    return out if c is None else out + c
+
+
+def phi(g, u):
+    return 16 * (u // 4) + 4 * g + (u % 4)
+
+
+def forward_tile(s, frag, cst):
+    """s: [64] radial input of lane's row (row = l%16)."""
+    t1 = np.zeros((64, 8))
+    h1 = np.zeros((64, 8))
+    for u in range(8):
+        q = phi(LG, u)
+        t1[:, u] = np.tanh(s * cst[q] + cst[32 + q])
+        h1[:, u] = t1[:, u] * cst[64 + q]
+    z2 = [mfma(frag[mt], h1) for mt in range(4)]
+    t2 = np.zeros((64, 16))
+    h2 = np.zeros((64, 16))
+    for mt in range(4):
+        q = 16 * mt + 4 * LG
+        for v in range(4):
+            t2[:, mt * 4 + v] = np.tanh(z2[mt][:, v] + cst[96 + q + v])
+            h2[:, mt * 4 + v] = t2[:, mt * 4 + v] * cst[160 + q + v] + h1[:, (mt & 1) * 4 + v]
+    b2 = [np.concatenate([h2[:, (2 * kc) * 4:(2 * kc) * 4 + 4], h2[:, (2 * kc + 1) * 4:(2 * kc + 1) * 4 + 4]], axis=1)
+          for kc in range(2)]
+    t3 = np.zeros((64, 32))
+    G = np.zeros((64, 32))
+    for mt in range(8):
+        z = mfma(frag[4 + 2 * mt], b2[0]) + mfma(frag[5 + 2 * mt], b2[1])
+        q = 16 * mt + 4 * LG
+        for v in range(4):
+            t3[:, mt * 4 + v] = np.tanh(z[:, v] + cst[224 + q + v])
+            G[:, mt * 4 + v] = t3[:, mt * 4 + v] * cst[352 + q + v] + h2[:, (mt & 3) * 4 + v]
+    return t1, h1, t2, h2, t3, G
+
+
+def lanes_to_rows(G):
+    """[64, 32] lane layout -> [16 rows, 128 q3]."""
+    out = np.zeros((16, 128))
+    for mt in range(8):
+        for v in range(4):
+            out[LR, 16 * mt + 4 * LG + v] = G[:, mt * 4 + v]
+    return out
+
+
+def mlp_ref(s, layers):
+    x = s[:, None]
+    for l in layers:
+        w, b, idt = embed._layer(l)
+        y = np.tanh(x @ w + b) * idt
+        nin, nout = w.shape
+        if nin == nout:
+            y = y + x
+        elif 2 * nin == nout:
+            y = y + np.concatenate([x, x], axis=1)
+        x = y
+    return x
+
+
+@pytest.fixture(scope="module")
+def packed():
+    spec = systems.WATER_SPEC
+    params = systems.make_params(spec, seed=7)
+    bits, consts, n1 = embed.pack_embedding(params["embedding"])
+    import torch
+
+    frags = torch.from_numpy(bits.astype(np.int16)).view(torch.bfloat16).float().numpy().astype(np.float64)
+    return params, frags, consts.astype(np.float64), n1
+
+
+def test_forward_tile_matches_mlp(packed):
+    params, frags, consts, n1 = packed
+    rng = np.random.default_rng(0)
+    q1, q2, q3 = embed.internal_maps(n1)
+    for t in range(2):
+        s = rng.normal(size=16) * 2
+        _, _, _, _, _, G = forward_tile(s[LR], frags[t], consts[t])
+        got = lanes_to_rows(G)
+        ref = mlp_ref(s, params["embedding"][t])
+        # weights are bf16 in the fragments: compare against the same rounding
+        # of W2/W3 through a float64 reference with rounded weights
+        import torch
+
+        lay = []
+        for l in params["embedding"][t]:
+            d = dict(l)
+            d["w"] = torch.tensor(l["w"]).to(torch.bfloat16).double().numpy() if l["w"].shape[0] > 1 else l["w"]
+            lay.append(d)
+        ref = mlp_ref(s, lay)
+        # biases / timesteps / w1 are stored in fp32 in the constant table
+        np.testing.assert_allclose(got[:, q3], ref, rtol=0, atol=1e-6)
+        pad = np.setdiff1d(np.arange(128), q3)
+        assert np.all(got[:, pad] == 0)
+
+
+def test_backward_tile_matches_autograd(packed):
+    """dE/ds and dE/d em of one tile for E = sum dGR . (em^T G)."""
+    import torch
+
+    params, frags, consts, n1 = packed
+    q1, q2, q3 = embed.internal_maps(n1)
+    rng = np.random.default_rng(1)
+    t = 1
+    em = rng.normal(size=(16, 4))
+    dgr = np.zeros((4, 128))
+    dgr[:, q3] = rng.normal(size=(4, 4 * n1))
+    f = frags[t]
+    c = consts[t]
+    t1, h1, t2, h2, t3, G = forward_tile(em[LR, 0], f, c)
+    # the kernel's backward dataflow
+    a_dgt = [np.where(((LG == 0)[:, None]) & (np.arange(8)[None, :] < 4),
+                      dgr[np.minimum(np.arange(8), 3)[None, :], (16 * mt + LR)[:, None]], 0.0) for mt in range(8)]
+    a_dg = [np.where((LR < 4)[:, None], dgr[np.minimum(LR, 3)[:, None], 32 * kc + phi(LG[:, None], np.arange(8)[None, :])],
+                     0.0) for kc in range(4)]
+    b_em = np.zeros((64, 8))
+    b_em[LG == 0, :4] = em[LR[LG == 0]]
+    dem = np.zeros((64, 4))
+    for kc in range(4):
+        bg = np.stack([G[:, (2 * kc + (u >> 2)) * 4 + (u & 3)] for u in range(8)], axis=1)
+        dem = mfma(a_dg[kc], bg, dem)
+    dgv = np.zeros((64, 32))
+    bdz3 = [np.zeros((64, 8)) for _ in range(4)]
+    for mt in range(8):
+        d = mfma(a_dgt[mt], b_em)
+        q = 16 * mt + 4 * LG
+        for v in range(4):
+            dgv[:, mt * 4 + v] = d[:, v]
+            bdz3[mt >> 1][:, (mt & 1) * 4 + v] = d[:, v] * c[352 + q + v] * (1 - t3[:, mt * 4 + v] ** 2)
+    dh2 = np.zeros((64, 16))
+    bdz2 = [np.zeros((64, 8)) for _ in range(2)]
+    for mt in range(4):
+        d = np.zeros((64, 4))
+        for kc in range(4):
+            d = mfma(f[20 + mt * 4 + kc], bdz3[kc], d)
+        q = 16 * mt + 4 * LG
+        for v in range(4):
+            x = d[:, v] + dgv[:, mt * 4 + v] + dgv[:, (mt + 4) * 4 + v]
+            dh2[:, mt * 4 + v] = x
+            bdz2[mt >> 1][:, (mt & 1) * 4 + v] = x * c[160 + q + v] * (1 - t2[:, mt * 4 + v] ** 2)
+    ds = np.zeros(64)
+    for mt in range(2):
+        d = np.zeros((64, 4))
+        for kc in range(2):
+            d = mfma(f[36 + mt * 2 + kc], bdz2[kc], d)
+        for v in range(4):
+            u = mt * 4 + v
+            q = phi(LG, u)
+            x = d[:, v] + dh2[:, mt * 4 + v] + dh2[:, (mt + 2) * 4 + v]
+            ds += x * c[64 + q] * (1 - t1[:, u] ** 2) * c[q]
+    ds_tot = np.array([ds[LR == r].sum() for r in range(16)])
+    nd = np.zeros((16, 4))
+    nd[:, 0] = dem[LG == 0][:, 0][np.argsort(LR[LG == 0])] + ds_tot
+    nd[:, 1:] = dem[LG == 0][:, 1:][np.argsort(LR[LG == 0])]
+    # autograd reference with the same (bf16-rounded) weights
+    emt = torch.tensor(em, requires_grad=True)
+    x = emt[:, :1]
+    for l in params["embedding"][t]:
+        w, b, idt = embed._layer(l)
+        wt = torch.tensor(w)
+        if w.shape[0] > 1:
+            wt = wt.to(torch.bfloat16).double()
+        y = torch.tanh(x @ wt + torch.tensor(b)) * torch.tensor(idt)
+        if w.shape[0] == w.shape[1]:
+            y = y + x
+        elif 2 * w.shape[0] == w.shape[1]:
+            y = y + torch.cat([x, x], dim=1)
+        x = y
+    e = (torch.tensor(dgr[:, q3]) * (emt.t() @ x)).sum()
+    e.backward()
+    np.testing.assert_allclose(nd, emt.grad.numpy(), rtol=1e-5, atol=1e-5)  # fp32 constants

@@ -1,0 +1,78 @@
+"""Fused bf16 embedding kernels against the float64 reference math."""
+
+import numpy as np
+import pytest
+import torch
+
+from dpmi import systems
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def _say(msg):
+    print(f"[gpu case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def setup():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dpmi.model import SeAEnergyModel
+    from dpmi.stat import gpu_model_params
+
+    spec = systems.WATER_SPEC
+    params = gpu_model_params(spec, systems.water192(0))
+    m64 = SeAEnergyModel.from_params(spec, params, device=DEV)
+    mbf = SeAEnergyModel.from_params(spec, params, device=DEV, precision="bf16")
+    assert mbf.fused is not None
+    s = systems.water192(0)
+    c = torch.tensor(s.coord, device=DEV)
+    t = torch.tensor(s.atype, device=DEV)
+    return spec, params, m64, mbf, s, c, t
+
+
+def _rel_rms(a, b):
+    return (torch.sqrt(((a - b) ** 2).mean()) / torch.sqrt((b**2).mean())).item()
+
+
+def test_fused_descriptor_forward(setup):
+    spec, params, m64, mbf, s, c, t = setup
+    _say("fused descriptor forward water192")
+    ext_c, ext_t, mapping, nlist, _, _ = m64.neighbor_list(c, t.to(torch.int32), s.box)
+    em64 = m64.env_mat(nlist, ext_c, ext_t)
+    d64 = m64.descriptor(em64)
+    gr, d = mbf.fused.forward(em64.float())
+    err = _rel_rms(d.double(), d64)
+    print(f"descriptor rel RMS error {err:.2e}", flush=True)
+    assert err < 1e-2
+
+
+def test_fused_embedding_backward(setup):
+    spec, params, m64, mbf, s, c, t = setup
+    _say("fused descriptor backward water192")
+    ext_c, ext_t, mapping, nlist, _, _ = m64.neighbor_list(c, t.to(torch.int32), s.box)
+    em64 = m64.env_mat(nlist, ext_c, ext_t).requires_grad_(True)
+    d64 = m64.descriptor(em64)
+    w = torch.linspace(-1, 1, d64.numel(), dtype=torch.float64, device=DEV).reshape(d64.shape)
+    (g64,) = torch.autograd.grad((d64 * w).sum(), em64)
+    gr, d = mbf.fused.forward(em64.detach().float())
+    nd = mbf.fused.backward(em64.detach().float(), gr, w.float())
+    err = _rel_rms(nd.double(), g64)
+    print(f"dE/d em rel RMS error {err:.2e}", flush=True)
+    assert err < 2e-2
+
+
+def test_fused_model_force_rmse(setup):
+    spec, params, m64, mbf, s, c, t = setup
+    _say("fused bf16 model water192 E+F+V")
+    a = m64.compute(c, t, s.box)
+    b = mbf.compute(c, t, s.box)
+    rmse = torch.sqrt(((a["force"] - b["force"]) ** 2).mean()).item()
+    frms = torch.sqrt((a["force"] ** 2).mean()).item()
+    de = abs(a["energy"].item() - b["energy"].item()) / s.nloc
+    dv = (torch.abs(a["virial"] - b["virial"]).max() / torch.abs(a["virial"]).max()).item()
+    print(f"fused bf16: force RMSE {rmse:.3e} ({rmse / frms:.2e} rel), |dE|/atom {de:.3e}, virial {dv:.2e}",
+          flush=True)
+    assert rmse <= 1e-3
+    assert de <= 5e-3
