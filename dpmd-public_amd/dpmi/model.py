@@ -85,7 +85,9 @@ class Net:
 
     def __call__(self, x, gemm_dtype=None):
         for w, b, idt, act in zip(self.w, self.b, self.idt, self.act):
-            if gemm_dtype is None:
+            if gemm_dtype is None or w.shape[1] < 16:
+                # fp64 path, or the 240 -> 1 output layer: a per-atom dot product
+                # (the low-precision N=1 GEMM path of the BLAS is pathologically slow)
                 y = torch.addmm(b, x, w) if x.dim() == 2 else x @ w + b
             else:
                 # low-precision operands, fp32 accumulation and fp32 result
