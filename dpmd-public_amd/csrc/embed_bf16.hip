@@ -30,6 +30,7 @@
 // register layout, so it needs no transposes at all.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 
 #include "dpmi_internal.h"
@@ -116,70 +117,85 @@ __device__ inline void forward_tile(float s, const bf16x8* __restrict__ frag, co
 }
 
 // Forward: per (atom, neighbor type) wavefront, GR partial [t][i][c][q3].
-__global__ void __launch_bounds__(256)
+// Persistent grid: a workgroup loads its type's weight fragments once and
+// walks atoms in groups of four (one per wavefront); every wavefront of a
+// workgroup runs the same number of iterations, so the block barriers that
+// guard the LDS transpose are uniform.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_embed_fwd(float* __restrict__ gr_part, const float* __restrict__ em, const bf16x8* __restrict__ frag_all,
                 const float* __restrict__ consts, int nloc, EmbedParams p) {
   __shared__ bf16x8 s_frag[kFragFwd * 64];
   __shared__ __attribute__((aligned(16))) __bf16 s_g[kAtoms][kQ3 * kGStride];
   __shared__ __attribute__((aligned(16))) __bf16 s_em[kAtoms][4 * kGStride];
+  __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
   const int t = blockIdx.y;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   const bf16x8* frag_t = frag_all + (int64_t)t * (kFragFwd + kFragBwd) * 64;
   for (int k = threadIdx.x; k < kFragFwd * 64; k += blockDim.x) s_frag[k] = frag_t[k];
-  const float* cst = consts + (int64_t)t * kConsts;
-  const int i = blockIdx.x * kAtoms + w;
-  const bool active = i < nloc;
+  for (int k = threadIdx.x; k < kConsts; k += blockDim.x) s_cst[k] = consts[(int64_t)t * kConsts + k];
+  const float* cst = s_cst;
   const int sel = p.sec[t + 1] - p.sec[t];
   const int nchunk = (sel + 31) / 32;
   __syncthreads();
-  f32x4 acc[8];
+  const int stride = gridDim.x * kAtoms;
+  // the two em row tiles of the next chunk (or of the next atom's first
+  // chunk) are loaded while the current chunk is computed
+  auto load_em = [&](int ai, int row) {
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ai < nloc && row < sel) v = *reinterpret_cast<const float4*>(em + ((int64_t)ai * p.nnei + p.sec[t] + row) * 4);
+    return v;
+  };
+  float4 ev_next[2] = {load_em(blockIdx.x * kAtoms + w, r), load_em(blockIdx.x * kAtoms + w, 16 + r)};
+  for (int base = blockIdx.x * kAtoms; base < nloc; base += stride) {
+    const int i = base + w;
+    const bool active = i < nloc;
+    f32x4 acc[8];
 #pragma unroll
-  for (int nt = 0; nt < 8; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const float* em_i = em + ((int64_t)(active ? i : 0) * p.nnei + p.sec[t]) * 4;
-  for (int ch = 0; ch < nchunk; ++ch) {
+    for (int nt = 0; nt < 8; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int ch = 0; ch < nchunk; ++ch) {
+      const float4 ev_cur[2] = {ev_next[0], ev_next[1]};
+      const int ni = ch + 1 < nchunk ? i : i + stride, nrow = ch + 1 < nchunk ? (ch + 1) * 32 : 0;
+      ev_next[0] = load_em(ni, nrow + r);
+      ev_next[1] = load_em(ni, nrow + 16 + r);
+      // weights and constants stay in LDS (re-read per chunk), not in registers
+      asm volatile("" ::: "memory");
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt) {
-      const int row = ch * 32 + rt * 16 + r;
-      float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
-      if (active && row < sel) {
-        const float4 ev = *reinterpret_cast<const float4*>(em_i + 4 * row);
-        e0 = ev.x;
-        e1 = ev.y;
-        e2 = ev.z;
-        e3 = ev.w;
+      for (int rt = 0; rt < 2; ++rt) {
+        const float4 ev = ev_cur[rt];
+        if (g == 0) {
+          s_em[w][0 * kGStride + rt * 16 + r] = (__bf16)ev.x;
+          s_em[w][1 * kGStride + rt * 16 + r] = (__bf16)ev.y;
+          s_em[w][2 * kGStride + rt * 16 + r] = (__bf16)ev.z;
+          s_em[w][3 * kGStride + rt * 16 + r] = (__bf16)ev.w;
+        }
+        TileFwd f;
+        forward_tile(ev.x, s_frag, cst, g, lane, f);
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            s_g[w][(16 * mt + 4 * g + v) * kGStride + rt * 16 + r] = (__bf16)f.g[mt * 4 + v];
       }
-      if (g == 0) {
-        s_em[w][0 * kGStride + rt * 16 + r] = (__bf16)e0;
-        s_em[w][1 * kGStride + rt * 16 + r] = (__bf16)e1;
-        s_em[w][2 * kGStride + rt * 16 + r] = (__bf16)e2;
-        s_em[w][3 * kGStride + rt * 16 + r] = (__bf16)e3;
+      __syncthreads();
+      // GR^T tile: D(M = c, N = q3) += em^T (M = c, K = 32 rows) * G (K = rows, N = q3)
+      bf16x8 a;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = (__bf16)0.f;
+      if (r < 4) a = *reinterpret_cast<const bf16x8*>(&s_em[w][r * kGStride + 8 * g]);
+#pragma unroll
+      for (int nt = 0; nt < 8; ++nt) {
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(&s_g[w][(16 * nt + r) * kGStride + 8 * g]);
+        acc[nt] = mfma(a, b, acc[nt]);
       }
-      TileFwd f;
-      forward_tile(e0, s_frag, cst, g, lane, f);
-#pragma unroll
-      for (int mt = 0; mt < 8; ++mt)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) s_g[w][(16 * mt + 4 * g + v) * kGStride + rt * 16 + r] = (__bf16)f.g[mt * 4 + v];
+      __syncthreads();
     }
-    __syncthreads();
-    // GR^T tile: D(M = c, N = q3) += em^T (M = c, K = 32 rows) * G (K = rows, N = q3)
-    bf16x8 a;
+    if (active && g == 0) {
+      float* out = gr_part + ((int64_t)t * nloc + i) * 4 * kQ3;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) a[u] = (__bf16)0.f;
-    if (r < 4) a = *reinterpret_cast<const bf16x8*>(&s_em[w][r * kGStride + 8 * g]);
+      for (int nt = 0; nt < 8; ++nt)
 #pragma unroll
-    for (int nt = 0; nt < 8; ++nt) {
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(&s_g[w][(16 * nt + r) * kGStride + 8 * g]);
-      acc[nt] = mfma(a, b, acc[nt]);
+        for (int v = 0; v < 4; ++v) out[v * kQ3 + 16 * nt + r] = acc[nt][v];
     }
-    __syncthreads();
-  }
-  if (active && g == 0) {
-    float* out = gr_part + ((int64_t)t * nloc + i) * 4 * kQ3;
-#pragma unroll
-    for (int nt = 0; nt < 8; ++nt)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) out[v * kQ3 + 16 * nt + r] = acc[nt][v];
   }
 }
 
@@ -262,114 +278,148 @@ __global__ void __launch_bounds__(256)
 }
 
 // Backward: net_deriv[i][sec_t + row][c] = dE/d em, recomputing the forward.
-__global__ void __launch_bounds__(256)
+// Persistent grid (weight fragments loaded into LDS once per workgroup);
+// each wavefront keeps dGR of its atom in LDS as [q3][c] so the mostly-zero
+// dGR^T A fragments are built per tile from one 16-byte read.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_embed_bwd(float* __restrict__ net_deriv, const float* __restrict__ em, const float* __restrict__ dgr,
                 const bf16x8* __restrict__ frag_all, const float* __restrict__ consts, int nloc, EmbedParams p) {
   __shared__ bf16x8 s_frag[(kFragFwd + kFragBwd) * 64];
+  __shared__ __attribute__((aligned(16))) float s_dg[kAtoms][kQ3 * 4];
+  __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
   const int t = blockIdx.y;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   const bf16x8* frag_t = frag_all + (int64_t)t * (kFragFwd + kFragBwd) * 64;
   for (int k = threadIdx.x; k < (kFragFwd + kFragBwd) * 64; k += blockDim.x) s_frag[k] = frag_t[k];
   __syncthreads();
   const bf16x8* fb = s_frag + kFragFwd * 64;
-  const float* cst = consts + (int64_t)t * kConsts;
-  const int i = blockIdx.x * kAtoms + w;
-  if (i >= nloc) return;
+  for (int k = threadIdx.x; k < kConsts; k += blockDim.x) s_cst[k] = consts[(int64_t)t * kConsts + k];
+  __syncthreads();
+  const float* cst = s_cst;
   const int sel = p.sec[t + 1] - p.sec[t];
-  const float* dg = dgr + (int64_t)i * 4 * kQ3;
-  // dGR^T as A (M = q3 tile, K = c): only K slots 0..3 (lane group 0) are non-zero
-  bf16x8 a_dgt[8];
-#pragma unroll
-  for (int mt = 0; mt < 8; ++mt)
-#pragma unroll
-    for (int u = 0; u < 8; ++u) a_dgt[mt][u] = (__bf16)((g == 0 && u < 4) ? dg[u * kQ3 + 16 * mt + r] : 0.f);
-  // dGR as A (M = c, K = q3 chunk in phi order): rows c >= 4 are zero
-  bf16x8 a_dg[4];
-#pragma unroll
-  for (int kc = 0; kc < 4; ++kc)
-#pragma unroll
-    for (int u = 0; u < 8; ++u) a_dg[kc][u] = (__bf16)(r < 4 ? dg[r * kQ3 + 32 * kc + phi(g, u)] : 0.f);
-  const float* em_i = em + ((int64_t)i * p.nnei + p.sec[t]) * 4;
-  float* nd_i = net_deriv + ((int64_t)i * p.nnei + p.sec[t]) * 4;
   const int ntile = (sel + 15) / 16;
-  for (int tl = 0; tl < ntile; ++tl) {
+  float* sdg = s_dg[w];
+  const int stride = gridDim.x * kAtoms;
+  // em rows are prefetched one tile ahead (across atoms too) so the HBM
+  // latency overlaps the current tile's work
+  auto load_em = [&](int ai, int tl) {
     const int row = tl * 16 + r;
-    float4 ev = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (row < sel) ev = *reinterpret_cast<const float4*>(em_i + 4 * row);
-    TileFwd f;
-    forward_tile(ev.x, s_frag, cst, g, lane, f);
-    // em as B (N = rows, K = c)
-    bf16x8 b_em;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ai < nloc && row < sel) v = *reinterpret_cast<const float4*>(em + ((int64_t)ai * p.nnei + p.sec[t] + row) * 4);
+    return v;
+  };
+  float4 ev_next = load_em(blockIdx.x * kAtoms + w, 0);
+  for (int i = blockIdx.x * kAtoms + w; i < nloc; i += stride) {
+    const float* dg = dgr + (int64_t)i * 4 * kQ3;
+    // stage dGR transposed: sdg[q3 * 4 + c]
+    for (int k = lane; k < 4 * kQ3; k += 64) sdg[(k & (kQ3 - 1)) * 4 + (k >> 7)] = dg[k];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    // dGR as A (M = c, K = q3 chunk in phi order): rows c >= 4 are zero
+    bf16x8 a_dg[4];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) b_em[u] = (__bf16)0.f;
-    if (g == 0) {
-      b_em[0] = (__bf16)ev.x;
-      b_em[1] = (__bf16)ev.y;
-      b_em[2] = (__bf16)ev.z;
-      b_em[3] = (__bf16)ev.w;
-    }
-    // direct term dem[c] = sum_q3 G dGR[c]  -> D(M = c, N = rows)
-    f32x4 dem = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < 4; ++kc)
 #pragma unroll
-    for (int kc = 0; kc < 4; ++kc) {
-      bf16x8 bg;
+      for (int u = 0; u < 8; ++u) {
+        const float x = sdg[(32 * kc + phi(g, u)) * 4 + (r & 3)];
+        a_dg[kc][u] = (__bf16)(r < 4 ? x : 0.f);
+      }
+    float* nd_i = net_deriv + ((int64_t)i * p.nnei + p.sec[t]) * 4;
+    for (int tl = 0; tl < ntile; ++tl) {
+      const int row = tl * 16 + r;
+      const float4 ev = ev_next;
+      ev_next = tl + 1 < ntile ? load_em(i, tl + 1) : load_em(i + stride, 0);
+      // weights and constants are re-read from LDS every tile instead of being
+      // held in registers across the loop: that keeps the kernel at two
+      // wavefronts per SIMD
+      asm volatile("" ::: "memory");
+      TileFwd f;
+      forward_tile(ev.x, s_frag, cst, g, lane, f);
+      // direct term dem[c] = sum_q3 G dGR[c]  -> D(M = c, N = rows)
+      f32x4 dem = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int u = 0; u < 8; ++u) bg[u] = (__bf16)f.g[(2 * kc + (u >> 2)) * 4 + (u & 3)];
-      dem = mfma(a_dg[kc], bg, dem);
-    }
-    // dG = em dGR  -> D(M = q3, N = rows); dz3 = dG * idt3 * (1 - t3^2)
-    float dgv[32];
-    bf16x8 b_dz3[4];
+      for (int kc = 0; kc < 4; ++kc) {
+        bf16x8 bg;
 #pragma unroll
-    for (int mt = 0; mt < 8; ++mt) {
-      f32x4 d = mfma(a_dgt[mt], b_em, f32x4{0.f, 0.f, 0.f, 0.f});
-      const int q = 16 * mt + 4 * g;
+        for (int u = 0; u < 8; ++u) bg[u] = (__bf16)f.g[(2 * kc + (u >> 2)) * 4 + (u & 3)];
+        dem = mfma(a_dg[kc], bg, dem);
+      }
+      // em as B (N = rows, K = c): K slots 0..3 in lane group 0
+      bf16x8 b_em;
+      b_em[0] = (__bf16)(g == 0 ? ev.x : 0.f);
+      b_em[1] = (__bf16)(g == 0 ? ev.y : 0.f);
+      b_em[2] = (__bf16)(g == 0 ? ev.z : 0.f);
+      b_em[3] = (__bf16)(g == 0 ? ev.w : 0.f);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        dgv[mt * 4 + v] = d[v];
-        float tt = f.t3[mt * 4 + v];
-        float dz = d[v] * cst[352 + q + v] * (1.f - tt * tt);
-        b_dz3[mt >> 1][(mt & 1) * 4 + v] = (__bf16)dz;
+      for (int u = 4; u < 8; ++u) b_em[u] = (__bf16)0.f;
+      // dG = em dGR -> D(M = q3, N = rows); dz3 = dG * idt3 * (1 - t3^2); skip sums dG[q] + dG[q + 64]
+      float skip[16];
+      bf16x8 b_dz3[4];
+#pragma unroll
+      for (int mt = 0; mt < 8; ++mt) {
+        bf16x8 a;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = (__bf16)0.f;
+        if (g == 0) {
+          const float4 d4 = *reinterpret_cast<const float4*>(&sdg[(16 * mt + r) * 4]);
+          a[0] = (__bf16)d4.x;
+          a[1] = (__bf16)d4.y;
+          a[2] = (__bf16)d4.z;
+          a[3] = (__bf16)d4.w;
+        }
+        f32x4 d = mfma(a, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
+        const int q = 16 * mt + 4 * g;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          if (mt < 4)
+            skip[mt * 4 + v] = d[v];
+          else
+            skip[(mt - 4) * 4 + v] += d[v];
+          float tt = f.t3[mt * 4 + v];
+          b_dz3[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(d[v] * cst[352 + q + v] * (1.f - tt * tt));
+        }
+      }
+      // dh2 = dz3 W3^T + skip;  dz2 = dh2 * idt2 * (1 - t2^2)
+      float dh2[16];
+      bf16x8 b_dz2[2];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        f32x4 d = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc) d = mfma(fb[(mt * 4 + kc) * 64 + lane], b_dz3[kc], d);
+        const int q = 16 * mt + 4 * g;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          float x = d[v] + skip[mt * 4 + v];
+          dh2[mt * 4 + v] = x;
+          float tt = f.t2[mt * 4 + v];
+          b_dz2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(x * cst[160 + q + v] * (1.f - tt * tt));
+        }
+      }
+      // dh1 = dz2 W2^T + dh2[q1] + dh2[q1 + 32];  ds = sum_f1 dh1 * idt1 * (1 - t1^2) * w1
+      float ds = 0.f;
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt) {
+        f32x4 d = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < 2; ++kc) d = mfma(fb[(16 + mt * 2 + kc) * 64 + lane], b_dz2[kc], d);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int u = mt * 4 + v, q = phi(g, u);
+          float x = d[v] + dh2[mt * 4 + v] + dh2[(mt + 2) * 4 + v];
+          float tt = f.t1[u];
+          ds += x * cst[64 + q] * (1.f - tt * tt) * cst[q];
+        }
+      }
+      ds += __shfl_xor(ds, 16, 64);
+      ds += __shfl_xor(ds, 32, 64);
+      if (g == 0 && row < sel) {
+        float4 o = make_float4(dem[0] + ds, dem[1], dem[2], dem[3]);
+        *reinterpret_cast<float4*>(nd_i + 4 * row) = o;
       }
     }
-    // dh2 = dz3 W3^T + dG[q2] + dG[q2 + 64];  dz2 = dh2 * idt2 * (1 - t2^2)
-    float dh2[16];
-    bf16x8 b_dz2[2];
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-      f32x4 d = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kc = 0; kc < 4; ++kc) d = mfma(fb[(mt * 4 + kc) * 64 + lane], b_dz3[kc], d);
-      const int q = 16 * mt + 4 * g;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        float x = d[v] + dgv[mt * 4 + v] + dgv[(mt + 4) * 4 + v];
-        dh2[mt * 4 + v] = x;
-        float tt = f.t2[mt * 4 + v];
-        b_dz2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(x * cst[160 + q + v] * (1.f - tt * tt));
-      }
-    }
-    // dh1 = dz2 W2^T + dh2[q1] + dh2[q1 + 32];  ds = sum_f1 dh1 * idt1 * (1 - t1^2) * w1
-    float ds = 0.f;
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
-      f32x4 d = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kc = 0; kc < 2; ++kc) d = mfma(fb[(16 + mt * 2 + kc) * 64 + lane], b_dz2[kc], d);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int u = mt * 4 + v, q = phi(g, u);
-        float x = d[v] + dh2[mt * 4 + v] + dh2[(mt + 2) * 4 + v];
-        float tt = f.t1[u];
-        ds += x * cst[64 + q] * (1.f - tt * tt) * cst[q];
-      }
-    }
-    ds += __shfl_xor(ds, 16, 64);
-    ds += __shfl_xor(ds, 32, 64);
-    if (g == 0 && row < sel) {
-      float4 o = make_float4(dem[0] + ds, dem[1], dem[2], dem[3]);
-      *reinterpret_cast<float4*>(nd_i + 4 * row) = o;
-    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -398,7 +448,8 @@ extern "C" int dpmi_se_a_embed_fwd_bf16(float* gr_part, const float* em, const v
   if (rc) return rc;
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
   if (nloc == 0) return DPMI_OK;
-  dim3 grid((unsigned)ceil_div(nloc, kAtoms), (unsigned)ntypes);
+  // persistent grid: enough workgroups per type to fill 256 CUs a few times over
+  dim3 grid((unsigned)std::min<int64_t>(ceil_div(nloc, kAtoms), 1024), (unsigned)ntypes);
   hipLaunchKernelGGL(k_embed_fwd, grid, dim3(256), 0, as_stream(stream), gr_part, em,
                      static_cast<const bf16x8*>(frags), consts, (int)nloc, p);
   DPMI_LAUNCH_CHECK();
@@ -433,7 +484,7 @@ extern "C" int dpmi_se_a_embed_bwd_bf16(float* net_deriv, const float* em, const
   if (rc) return rc;
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
   if (nloc == 0) return DPMI_OK;
-  dim3 grid((unsigned)ceil_div(nloc, kAtoms), (unsigned)ntypes);
+  dim3 grid((unsigned)std::min<int64_t>(ceil_div(nloc, kAtoms), 1024), (unsigned)ntypes);
   hipLaunchKernelGGL(k_embed_bwd, grid, dim3(256), 0, as_stream(stream), net_deriv, em, dgr,
                      static_cast<const bf16x8*>(frags), consts, (int)nloc, p);
   DPMI_LAUNCH_CHECK();
