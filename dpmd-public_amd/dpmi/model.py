@@ -25,7 +25,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from . import embed, ops
+from . import embed, fitting, ops
 
 _ACT = torch.tanh
 
@@ -137,6 +137,10 @@ class SeAEnergyModel:
                                            device=self.device)
         ob = np.zeros(self.ntypes) if out_bias is None else np.asarray(out_bias, dtype=np.float64).reshape(-1)
         self.out_bias = torch.as_tensor(ob, device=self.device)
+        # bf16 form: fused MFMA fitting kernel (else the torch layers above)
+        self.fused_fit = None
+        if self.fused is not None and fitting.supported(fit):
+            self.fused_fit = fitting.FusedFitting(fit, bias_atom_e, ob, self.device)
 
     # ----------------------------------------------------------------- constructors
     @classmethod
@@ -216,16 +220,21 @@ class SeAEnergyModel:
 
     def _compute_fused(self, atype, ext_c, ext_t, mapping, nlist, need_atom_energy):
         """bf16 form with hand-written adjoints end to end: fused embedding +
-        G^T R forward, fitting (hipBLASLt GEMMs, autograd only through the
-        fitting net), descriptor and embedding adjoints recomputing the
-        forward, env-mat adjoint for forces and virial."""
+        G^T R forward, fused fitting forward + adjoint (or hipBLASLt GEMMs with
+        autograd through the fitting net when its shape is not supported),
+        descriptor and embedding adjoints recomputing the forward, env-mat
+        adjoint for forces and virial."""
         em = self.env_mat(nlist, ext_c, ext_t)
         gr, desc = self.fused.forward(em)
-        desc.requires_grad_(True)
-        with torch.enable_grad():
-            ae = self.atom_energy(desc, atype)
+        if self.fused_fit is not None:
+            ae, ddesc = self.fused_fit(desc, atype)
             energy = ae.sum()
-            (ddesc,) = torch.autograd.grad(energy, desc)
+        else:
+            desc.requires_grad_(True)
+            with torch.enable_grad():
+                ae = self.atom_energy(desc, atype)
+                energy = ae.sum()
+                (ddesc,) = torch.autograd.grad(energy, desc)
         nd = self.fused.backward(em, gr, ddesc)
         force, virial = ops.force_virial_recompute(nd, nlist, mapping, ext_c, ext_t, self.dstd, self.rcut,
                                                    self.rcut_smth, self.sel)

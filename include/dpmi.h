@@ -202,6 +202,24 @@ int dpmi_se_a_descriptor_bwd(float* dgr, const float* ddesc, const float* gr, in
 int dpmi_se_a_embed_bwd_bf16(float* net_deriv, const float* em, const float* dgr, const void* frags,
                              const float* consts, int64_t nloc, const int* sel_host, int ntypes, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Fused energy fitting net (bf16 MFMA operands, fp32 accumulation).
+ * Replaces GeneralFitting._forward_common + its autograd backward
+ * (deepmd/pt/model/task/fitting.py:734-900; FittingNet layers of
+ * deepmd/dpmodel/utils/network.py:1031-1090): per type, three hidden layers
+ * of width H <= 256 (the second and third with the skip connection) and a
+ * linear H -> 1 output.  frags / consts / ebias are laid out by
+ * dpmi.fitting.pack_fitting; perm [nloc] lists the local atoms ordered by
+ * type and type_start [ntypes + 1] (device) delimits each type's run.
+ * ------------------------------------------------------------------------ */
+/* atom_energy [nloc] fp64 = network output + bias_atom_e + out_bias;
+ * ddesc [nloc, k1] fp32 = dE/d desc for E = sum atom_energy. */
+int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const float* desc, const int* perm,
+                           const int* type_start, const void* frags, const float* consts, const double* ebias,
+                           int64_t nloc, int ntypes, int k1, void* stream);
+/* Number of bf16x8 lanes (64 per fragment) of one type's packed fitting net. */
+int dpmi_se_a_fitting_frag_count(int64_t* count, int k1);
+
 #ifdef __cplusplus
 }
 #endif

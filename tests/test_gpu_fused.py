@@ -76,3 +76,29 @@ def test_fused_model_force_rmse(setup):
           flush=True)
     assert rmse <= 1e-3
     assert de <= 5e-3
+
+
+@pytest.mark.parametrize("nloc", [192, 1000])
+def test_fused_fitting(setup, nloc):
+    """Fused fitting kernel: atomic energies and dE/d desc against the float64
+    torch fitting net (autograd) on the same descriptor; atom counts that
+    leave partial 64-atom tiles in both types."""
+    spec, params, m64, mbf, s, c, t = setup
+    assert mbf.fused_fit is not None
+    _say(f"fused fitting nloc={nloc}")
+    g = torch.Generator(device="cpu").manual_seed(nloc)
+    k1 = mbf.fused_fit.k1
+    desc = (torch.randn(nloc, k1, generator=g, dtype=torch.float64) * 0.3).to(DEV)
+    atype = (torch.rand(nloc, generator=g) < 0.35).to(torch.int32).to(DEV)
+    d = desc.clone().requires_grad_(True)
+    with torch.enable_grad():
+        ae64 = m64.atom_energy(d, atype)
+        (dd64,) = torch.autograd.grad(ae64.sum(), d)
+    ae, dd = mbf.fused_fit(desc.float(), atype)
+    torch.cuda.synchronize()
+    e_err = (ae - ae64).abs().max().item()
+    d_err = _rel_rms(dd.double(), dd64)
+    print(f"fitting: max |de| {e_err:.3e} eV, dE/ddesc rel RMS {d_err:.2e}", flush=True)
+    assert torch.isfinite(ae).all() and torch.isfinite(dd).all()
+    assert e_err <= 2e-2
+    assert d_err <= 1e-2
