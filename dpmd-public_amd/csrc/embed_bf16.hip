@@ -44,7 +44,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kMaxTypes = 16;
 constexpr int kFragFwd = 20;   // 4 (layer 2) + 16 (layer 3)
 constexpr int kFragBwd = 20;   // 16 (dh2) + 4 (dh1)
-constexpr int kConsts = 480;   // w1,b1,idt1 [32]; b2,idt2 [64]; b3,idt3 [128]
+constexpr int kConsts = 704;   // k w1, k b1, idt1 [32]; k b2, idt2 [64]; k b3, idt3 [128]; w1 [32];
+                               // idt2 / k [64]; idt3 / k [128]   (k = 2 log2 e)
 constexpr int kQ3 = 128;       // internal width of G
 constexpr int kGStride = 40;   // LDS row stride (bf16) of the transposed G tile
 constexpr int kAtoms = 4;      // atoms (wavefronts) per workgroup
@@ -55,10 +56,11 @@ struct EmbedParams {
   int sec[kMaxTypes + 1];
 };
 
-__device__ inline float fast_tanh(float x) {
-  // tanh(x) = 1 - 2 / (exp(2x) + 1); exact limits at +-inf
-  float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
-  return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + e);
+// tanh(x) from z = 2 log2(e) x: 1 - 2 / (2^z + 1), exact limits at +-inf.
+// The forward weights, biases and layer-1 constants are stored pre-scaled by
+// 2 log2(e), so the MFMA (with the bias as its accumulator input) yields z.
+__device__ inline float tanh_scaled(float z) {
+  return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(z));
 }
 
 __device__ inline f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
@@ -82,21 +84,22 @@ __device__ inline void forward_tile(float s, const bf16x8* __restrict__ frag, co
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
     int q = phi(g, u);
-    float t = fast_tanh(s * cst[q] + cst[32 + q]);
+    float t = tanh_scaled(s * cst[q] + cst[32 + q]);
     f.t1[u] = t;
     f.h1[u] = t * cst[64 + q];
     b1[u] = (__bf16)f.h1[u];
   }
   f32x4 z2[4];
 #pragma unroll
-  for (int mt = 0; mt < 4; ++mt) z2[mt] = mfma(frag[mt * 64 + lane], b1, f32x4{0.f, 0.f, 0.f, 0.f});
+  for (int mt = 0; mt < 4; ++mt)
+    z2[mt] = mfma(frag[mt * 64 + lane], b1, *reinterpret_cast<const f32x4*>(cst + 96 + 16 * mt + 4 * g));
   bf16x8 b2[2];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     const int q = 16 * mt + 4 * g;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      float t = fast_tanh(z2[mt][v] + cst[96 + q + v]);
+      float t = tanh_scaled(z2[mt][v]);
       f.t2[mt * 4 + v] = t;
       f.h2[mt * 4 + v] = t * cst[160 + q + v] + f.h1[(mt & 1) * 4 + v];
       b2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)f.h2[mt * 4 + v];
@@ -104,12 +107,12 @@ __device__ inline void forward_tile(float s, const bf16x8* __restrict__ frag, co
   }
 #pragma unroll
   for (int mt = 0; mt < 8; ++mt) {
-    f32x4 z = mfma(frag[(4 + 2 * mt) * 64 + lane], b2[0], f32x4{0.f, 0.f, 0.f, 0.f});
+    f32x4 z = mfma(frag[(4 + 2 * mt) * 64 + lane], b2[0], *reinterpret_cast<const f32x4*>(cst + 224 + 16 * mt + 4 * g));
     z = mfma(frag[(5 + 2 * mt) * 64 + lane], b2[1], z);
     const int q = 16 * mt + 4 * g;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      float t = fast_tanh(z[v] + cst[224 + q + v]);
+      float t = tanh_scaled(z[v]);
       f.t3[mt * 4 + v] = t;
       f.g[mt * 4 + v] = t * cst[352 + q + v] + f.h2[(mt & 3) * 4 + v];
     }
@@ -117,17 +120,20 @@ __device__ inline void forward_tile(float s, const bf16x8* __restrict__ frag, co
 }
 
 // Forward: per (atom, neighbor type) wavefront, GR partial [t][i][c][q3].
-// Persistent grid: a workgroup loads its type's weight fragments once and
-// walks atoms in groups of four (one per wavefront); every wavefront of a
-// workgroup runs the same number of iterations, so the block barriers that
-// guard the LDS transpose are uniform.
+// Persistent grid: a workgroup loads its type's weight fragments once and its
+// four wavefronts walk atoms independently (no block barrier after set-up).
+// G^T R contracts over neighbor rows while G comes out of the layer-3 product
+// with the row as the lane index, so each 16-row tile of G is transposed by
+// MFMA against an identity fragment (exact for the bf16 operand): then the
+// lane index is the feature and every lane group holds four rows, and two
+// tiles make the 32-row K operand (row order psi(g, u) = 4g + u for u < 4,
+// 16 + 4g + u - 4 otherwise) of GR^T += em^T G.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_embed_fwd(float* __restrict__ gr_part, const float* __restrict__ em, const bf16x8* __restrict__ frag_all,
                 const float* __restrict__ consts, int nloc, EmbedParams p) {
   __shared__ bf16x8 s_frag[kFragFwd * 64];
-  __shared__ __attribute__((aligned(16))) __bf16 s_g[kAtoms][kQ3 * kGStride];
-  __shared__ __attribute__((aligned(16))) __bf16 s_em[kAtoms][4 * kGStride];
   __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
+  __shared__ __attribute__((aligned(16))) float s_em[kAtoms][32 * 4];
   const int t = blockIdx.y;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   const bf16x8* frag_t = frag_all + (int64_t)t * (kFragFwd + kFragBwd) * 64;
@@ -137,18 +143,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   const int sel = p.sec[t + 1] - p.sec[t];
   const int nchunk = (sel + 31) / 32;
   __syncthreads();
+  // identity fragments: B (K = phi slot of a 32-chunk, N = n), one per 16-wide half
+  bf16x8 eye[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int u = 0; u < 8; ++u) eye[j][u] = (__bf16)(((u >> 2) == j && 4 * g + (u & 3) == r) ? 1.f : 0.f);
+  float* sem = s_em[w];
   const int stride = gridDim.x * kAtoms;
-  // the two em row tiles of the next chunk (or of the next atom's first
-  // chunk) are loaded while the current chunk is computed
   auto load_em = [&](int ai, int row) {
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (ai < nloc && row < sel) v = *reinterpret_cast<const float4*>(em + ((int64_t)ai * p.nnei + p.sec[t] + row) * 4);
     return v;
   };
   float4 ev_next[2] = {load_em(blockIdx.x * kAtoms + w, r), load_em(blockIdx.x * kAtoms + w, 16 + r)};
-  for (int base = blockIdx.x * kAtoms; base < nloc; base += stride) {
-    const int i = base + w;
-    const bool active = i < nloc;
+  for (int i = blockIdx.x * kAtoms + w; i < nloc; i += stride) {
     f32x4 acc[8];
 #pragma unroll
     for (int nt = 0; nt < 8; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -159,37 +168,50 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       ev_next[1] = load_em(ni, nrow + 16 + r);
       // weights and constants stay in LDS (re-read per chunk), not in registers
       asm volatile("" ::: "memory");
+      bf16x8 gt[2][4];  // transposed G: [tile][pair of 16-feature tiles], 4 rows each
 #pragma unroll
       for (int rt = 0; rt < 2; ++rt) {
         const float4 ev = ev_cur[rt];
-        if (g == 0) {
-          s_em[w][0 * kGStride + rt * 16 + r] = (__bf16)ev.x;
-          s_em[w][1 * kGStride + rt * 16 + r] = (__bf16)ev.y;
-          s_em[w][2 * kGStride + rt * 16 + r] = (__bf16)ev.z;
-          s_em[w][3 * kGStride + rt * 16 + r] = (__bf16)ev.w;
-        }
+        if (g == 0) *reinterpret_cast<float4*>(sem + 4 * (16 * rt + r)) = ev;
         TileFwd f;
         forward_tile(ev.x, s_frag, cst, g, lane, f);
 #pragma unroll
-        for (int mt = 0; mt < 8; ++mt)
+        for (int kc = 0; kc < 4; ++kc) {
+          bf16x8 gb;
 #pragma unroll
-          for (int v = 0; v < 4; ++v)
-            s_g[w][(16 * mt + 4 * g + v) * kGStride + rt * 16 + r] = (__bf16)f.g[mt * 4 + v];
+          for (int u = 0; u < 8; ++u) gb[u] = (__bf16)f.g[(2 * kc + (u >> 2)) * 4 + (u & 3)];
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            const f32x4 d = mfma(gb, eye[j], f32x4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+            for (int v = 0; v < 4; ++v) gt[rt][kc][4 * j + v] = (__bf16)d[v];
+          }
+        }
       }
-      __syncthreads();
-      // GR^T tile: D(M = c, N = q3) += em^T (M = c, K = 32 rows) * G (K = rows, N = q3)
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      // em^T as A (M = c < 4, K = rows in psi order)
       bf16x8 a;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) a[u] = (__bf16)0.f;
-      if (r < 4) a = *reinterpret_cast<const bf16x8*>(&s_em[w][r * kGStride + 8 * g]);
+      for (int u = 0; u < 8; ++u) {
+        const int row = u < 4 ? 4 * g + u : 16 + 4 * g + (u - 4);
+        const float x = sem[4 * row + (r & 3)];
+        a[u] = (__bf16)(r < 4 ? x : 0.f);
+      }
 #pragma unroll
       for (int nt = 0; nt < 8; ++nt) {
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(&s_g[w][(16 * nt + r) * kGStride + 8 * g]);
+        bf16x8 b;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          b[v] = gt[0][nt >> 1][4 * (nt & 1) + v];
+          b[4 + v] = gt[1][nt >> 1][4 * (nt & 1) + v];
+        }
         acc[nt] = mfma(a, b, acc[nt]);
       }
-      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
     }
-    if (active && g == 0) {
+    if (g == 0) {
       float* out = gr_part + ((int64_t)t * nloc + i) * 4 * kQ3;
 #pragma unroll
       for (int nt = 0; nt < 8; ++nt)
@@ -376,7 +398,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
           else
             skip[(mt - 4) * 4 + v] += d[v];
           float tt = f.t3[mt * 4 + v];
-          b_dz3[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(d[v] * cst[352 + q + v] * (1.f - tt * tt));
+          b_dz3[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(d[v] * cst[576 + q + v] * (1.f - tt * tt));
         }
       }
       // dh2 = dz3 W3^T + skip;  dz2 = dh2 * idt2 * (1 - t2^2)
@@ -393,7 +415,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
           float x = d[v] + skip[mt * 4 + v];
           dh2[mt * 4 + v] = x;
           float tt = f.t2[mt * 4 + v];
-          b_dz2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(x * cst[160 + q + v] * (1.f - tt * tt));
+          b_dz2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(x * cst[512 + q + v] * (1.f - tt * tt));
         }
       }
       // dh1 = dz2 W2^T + dh2[q1] + dh2[q1 + 32];  ds = sum_f1 dh1 * idt1 * (1 - t1^2) * w1
@@ -408,7 +430,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
           const int u = mt * 4 + v, q = phi(g, u);
           float x = d[v] + dh2[mt * 4 + v] + dh2[(mt + 2) * 4 + v];
           float tt = f.t1[u];
-          ds += x * cst[64 + q] * (1.f - tt * tt) * cst[q];
+          ds += x * cst[64 + q] * (1.f - tt * tt) * cst[480 + q];
         }
       }
       ds += __shfl_xor(ds, 16, 64);

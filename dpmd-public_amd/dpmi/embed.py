@@ -28,7 +28,10 @@ from . import ops
 from ._lib import ptr
 
 N_FRAG = 40  # 20 forward + 20 backward fragments per type
-N_CONST = 480
+N_CONST = 704
+# tanh(x) = 1 - 2 / (2^(K2 x) + 1): the forward weights and biases are stored
+# multiplied by K2 so the kernels' MFMA results feed exp2 directly
+K2 = 2.0 / np.log(2.0)
 
 
 def _phi():
@@ -64,7 +67,10 @@ def _layer(l):
 def pack_embedding(emb_layers: list) -> tuple[np.ndarray, np.ndarray, int]:
     """emb_layers[t] = the three layer dicts of neighbor type t.
     Returns (frags uint16 [ntypes, 40, 64, 8] holding bf16 bits,
-    consts float32 [ntypes, 480], n1)."""
+    consts float32 [ntypes, 704], n1).  consts: K2 w1 [0:32], K2 b1 [32:64],
+    idt1 [64:96], K2 b2 [96:160], idt2 [160:224], K2 b3 [224:352],
+    idt3 [352:480], w1 [480:512], idt2 / K2 [512:576], idt3 / K2 [576:704];
+    all 40 fragments hold bf16(K2 W)."""
     ntypes = len(emb_layers)
     (w1, _, _) = _layer(emb_layers[0][0])
     n1 = w1.shape[1]
@@ -83,26 +89,31 @@ def pack_embedding(emb_layers: list) -> tuple[np.ndarray, np.ndarray, int]:
         W3 = np.zeros((64, 128))
         W3[np.ix_(q2, q3)] = w3
         c = consts[t]
-        c[q1] = w1[0]
-        c[32 + q1] = b1
+        c[q1] = K2 * w1[0]
+        c[32 + q1] = K2 * b1
         c[64 + q1] = i1
-        c[96 + q2] = b2
+        c[96 + q2] = K2 * b2
         c[160 + q2] = i2
-        c[224 + q3] = b3
+        c[224 + q3] = K2 * b3
         c[352 + q3] = i3
+        c[480 + q1] = w1[0]
+        c[512 + q2] = i2 / K2
+        c[576 + q3] = i3 / K2
         K = phi[lg]  # [64, 8] K slot -> feature offset within a 32-chunk
         f = frags[t]
-        for mt in range(4):  # layer 2, A = W2^T (M = q2, K = q1)
-            f[mt] = W2[K, (16 * mt + lr)[:, None]]
-        for mt in range(8):  # layer 3, A = W3^T (M = q3, K = q2)
+        for mt in range(4):  # layer 2, A = K2 W2^T (M = q2, K = q1)
+            f[mt] = K2 * W2[K, (16 * mt + lr)[:, None]]
+        for mt in range(8):  # layer 3, A = K2 W3^T (M = q3, K = q2)
             for kc in range(2):
-                f[4 + 2 * mt + kc] = W3[32 * kc + K, (16 * mt + lr)[:, None]]
+                f[4 + 2 * mt + kc] = K2 * W3[32 * kc + K, (16 * mt + lr)[:, None]]
+        # the adjoint uses the same bf16(K2 W) as the forward (with dz / K2 as
+        # its B operand), so forces are the gradient of the evaluated energy
         for mt in range(4):  # dh2 = W3 dz3 (M = q2, K = q3)
             for kc in range(4):
-                f[20 + mt * 4 + kc] = W3[(16 * mt + lr)[:, None], 32 * kc + K]
+                f[20 + mt * 4 + kc] = K2 * W3[(16 * mt + lr)[:, None], 32 * kc + K]
         for mt in range(2):  # dh1 = W2 dz2 (M = q1, K = q2)
             for kc in range(2):
-                f[20 + 16 + mt * 2 + kc] = W2[(16 * mt + lr)[:, None], 32 * kc + K]
+                f[20 + 16 + mt * 2 + kc] = K2 * W2[(16 * mt + lr)[:, None], 32 * kc + K]
     bits = torch.from_numpy(frags).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
     return bits, consts, n1
 

@@ -185,7 +185,7 @@ int dpmi_tabulate_fusion_se_a_grad_grad_f32(float* dz_dy, const float* table, co
  * (n1, 2 n1, 4 n1), n1 <= 32, resnet with timestep).  The reference has no
  * single function for this: it chains torch MLP layers and matmuls.
  * frags: [ntypes, 40, 64, 8] bf16 weight fragments and consts
- * [ntypes, 480] fp32, both laid out by dpmi.embed.pack_embedding (internal
+ * [ntypes, 704] fp32, both laid out by dpmi.embed.pack_embedding (internal
  * feature order q3; see that module).  em [nloc, nnei, 4] fp32.
  * ------------------------------------------------------------------------ */
 /* gr_part [ntypes, nloc, 4, 128]: sum over each type's neighbors of em^T G. */
