@@ -299,6 +299,95 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Same contraction for axis <= 16, one wavefront per atom, without cross-lane
+// reductions in the loops: the atom's dD row is staged in LDS (row stride
+// axis + 1, conflict-free column reads), the first term runs with one lane
+// per m over a, the second with lane (m0, a) over m = m0 + j (64 / AXIS),
+// and only its final sum crosses lanes.
+template <int AXIS>
+__global__ void __launch_bounds__(256)
+    k_descrpt_bwd_lanes(float* __restrict__ dgr, const float* __restrict__ ddesc, const float* __restrict__ gr,
+                        int nloc, float inv_nnei, int n1) {
+  constexpr int kGroups = 64 / AXIS;
+  constexpr int kS = AXIS + 1;
+  __shared__ float s_gr[kAtoms][4 * kQ3];
+  __shared__ float s_out[kAtoms][4 * kQ3];
+  __shared__ __attribute__((aligned(16))) float s_ga[kAtoms][AXIS * 4];
+  __shared__ float s_dd[kAtoms][kQ3 * kS];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * kAtoms + w;
+  if (i >= nloc) return;
+  const int M = 4 * n1;
+  float* sg = s_gr[w];
+  float* so = s_out[w];
+  float* sga = s_ga[w];
+  float* sdd = s_dd[w];
+  const float* gri = gr + (int64_t)i * 4 * kQ3;
+  const float4* dd4 = reinterpret_cast<const float4*>(ddesc + (int64_t)i * M * AXIS);
+  const int n4 = M * AXIS / 4;
+#pragma unroll 4
+  for (int k = lane; k < n4; k += 64) {
+    const float4 x = dd4[k];
+    const int m = (4 * k) / AXIS, a = (4 * k) % AXIS;
+    sdd[m * kS + a + 0] = x.x;
+    sdd[m * kS + a + 1] = x.y;
+    sdd[m * kS + a + 2] = x.z;
+    sdd[m * kS + a + 3] = x.w;
+  }
+  for (int k = lane; k < 4 * kQ3; k += 64) {
+    sg[k] = gri[k];
+    so[k] = 0.f;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane < AXIS * 4) {
+    const int a = lane >> 2, c = lane & 3;
+    sga[4 * a + c] = sg[c * kQ3 + q3_of(a, n1)];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  // first term: so[c][q3(m)] = sum_a dD[m][a] GR[c][a]
+  for (int m = lane; m < M; m += 64) {
+    float t[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < AXIS; ++a) {
+      const float x = sdd[m * kS + a];
+      const float4 g4 = *reinterpret_cast<const float4*>(sga + 4 * a);
+      t[0] += x * g4.x;
+      t[1] += x * g4.y;
+      t[2] += x * g4.z;
+      t[3] += x * g4.w;
+    }
+    const int qm = q3_of(m, n1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) so[c * kQ3 + qm] = t[c];
+  }
+  // second term: sum_m dD[m][a] GR[c][m] for a < axis
+  const int a = lane % AXIS, m0 = lane / AXIS;
+  float t2[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int m = m0; m < M; m += kGroups) {
+    const float x = sdd[m * kS + a];
+    const int qm = q3_of(m, n1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) t2[c] += x * sg[c * kQ3 + qm];
+  }
+#pragma unroll
+  for (int off = AXIS; off < 64; off <<= 1)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) t2[c] += __shfl_xor(t2[c], off, 64);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (m0 == 0) {
+    const int qa = q3_of(a, n1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) so[c * kQ3 + qa] += t2[c];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  float* out = dgr + (int64_t)i * 4 * kQ3;
+  for (int k = lane; k < 4 * kQ3; k += 64) out[k] = so[k] * inv_nnei;
+}
+
 // Backward: net_deriv[i][sec_t + row][c] = dE/d em, recomputing the forward.
 // Persistent grid (weight fragments loaded into LDS once per workgroup);
 // each wavefront keeps dGR of its atom in LDS as [q3][c] so the mostly-zero
@@ -492,8 +581,16 @@ extern "C" int dpmi_se_a_descriptor_bwd(float* dgr, const float* ddesc, const fl
                                         int n1, int axis, void* stream) {
   DPMI_CHECK_ARG(n1 > 0 && n1 <= 32 && axis > 0 && axis <= 4 * n1 && axis <= 16, "unsupported widths");
   if (nloc == 0) return DPMI_OK;
-  hipLaunchKernelGGL(k_descrpt_bwd, dim3(ceil_div(nloc, kAtoms)), dim3(256), 0, as_stream(stream), dgr, ddesc, gr,
-                     (int)nloc, 1.f / (float)nnei, n1, axis);
+  const dim3 grid((unsigned)ceil_div(nloc, kAtoms));
+  const float inv = 1.f / (float)nnei;
+  if (axis == 16)
+    hipLaunchKernelGGL(k_descrpt_bwd_lanes<16>, grid, dim3(256), 0, as_stream(stream), dgr, ddesc, gr, (int)nloc, inv, n1);
+  else if (axis == 8)
+    hipLaunchKernelGGL(k_descrpt_bwd_lanes<8>, grid, dim3(256), 0, as_stream(stream), dgr, ddesc, gr, (int)nloc, inv, n1);
+  else if (axis == 4)
+    hipLaunchKernelGGL(k_descrpt_bwd_lanes<4>, grid, dim3(256), 0, as_stream(stream), dgr, ddesc, gr, (int)nloc, inv, n1);
+  else
+    hipLaunchKernelGGL(k_descrpt_bwd, grid, dim3(256), 0, as_stream(stream), dgr, ddesc, gr, (int)nloc, inv, n1, axis);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }

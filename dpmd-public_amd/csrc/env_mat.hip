@@ -108,6 +108,18 @@ __device__ inline void section_limits(const int* __restrict__ nl, const SecParam
   __syncthreads();
 }
 
+// The same per wavefront (lim is the wavefront's own LDS row).
+__device__ inline void section_limits_wave(const int* __restrict__ nl, const SecParams& sp, int* lim) {
+  const int lane = lane_id();
+  for (int t = lane; t < sp.ntypes; t += kWave) lim[t] = sp.sec[t + 1];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int s = lane; s < sp.nnei; s += kWave)
+    if (nl[s] < 0) atomicMin(&lim[slot_type(sp, s)], s);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // FP: arithmetic and output type; CT: coordinate type (the difference
 // x_j - x_i is formed in CT, then rounded to FP).
 template <typename FP, typename CT>
@@ -318,15 +330,14 @@ __global__ void __launch_bounds__(256)
                              const FP* __restrict__ std_, float rmin, float rmax, SecParams sp) {
   __shared__ int lim_all[kAtomsPerBlock][kMaxTypes];
   const int w = threadIdx.x / kWave, lane = lane_id();
-  const int i = blockIdx.x * kAtomsPerBlock + w;
-  const bool active = i < nloc;
   const int nnei = sp.nnei;
-  const int* nl = nlist + (int64_t)(active ? i : 0) * nnei;
-  section_limits(nl, sp, lim_all[w]);
   double v[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) v[k] = 0.;
-  if (active) {
+  // persistent grid: each wavefront walks atoms; one virial partial per workgroup
+  for (int i = blockIdx.x * kAtomsPerBlock + w; i < nloc; i += gridDim.x * kAtomsPerBlock) {
+    const int* nl = nlist + (int64_t)i * nnei;
+    section_limits_wave(nl, sp, lim_all[w]);
     const int ti = ext_t[i] < sp.ntypes ? ext_t[i] : -1;
     const double xi0 = ext_c[3 * i + 0], xi1 = ext_c[3 * i + 1], xi2 = ext_c[3 * i + 2];
     ACC ci[3] = {0, 0, 0};
@@ -491,6 +502,8 @@ extern "C" int dpmi_prod_virial_se_a_f32(float* virial, float* atom_virial, cons
                                  workspace, workspace_bytes, stream);
 }
 
+constexpr int64_t kForceBlocks = 2048;
+
 template <typename FP>
 static int force_virial_recompute_impl(double* force, double* virial, const FP* net_deriv, const int* nlist,
                                        const int* mapping, const double* ext_coord, const int* ext_type,
@@ -505,7 +518,8 @@ static int force_virial_recompute_impl(double* force, double* virial, const FP* 
   hipStream_t st = as_stream(stream);
   const int64_t nout = (mapping ? nloc : nall) * 3;
   if (nout > 0) hipLaunchKernelGGL(k_fill<double>, dim3(ceil_div(nout, 256)), dim3(256), 0, st, force, nout, 0.);
-  int bpf = (int)std::max<int64_t>(1, ceil_div(nloc, kAtomsPerBlock));
+  // persistent grid (a few workgroups per CU): few virial partials to reduce
+  int bpf = (int)std::min<int64_t>(kForceBlocks, std::max<int64_t>(1, ceil_div(nloc, kAtomsPerBlock)));
   double* partial = static_cast<double*>(workspace);
   hipLaunchKernelGGL((k_force_virial_recompute<FP, double>), dim3(bpf), dim3(256), 0, st, force, partial, net_deriv,
                      nlist, mapping, ext_coord, ext_type, (int)nloc, std_, rcut_smth, rcut, sp);

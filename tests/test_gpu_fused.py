@@ -102,3 +102,35 @@ def test_fused_fitting(setup, nloc):
     assert torch.isfinite(ae).all() and torch.isfinite(dd).all()
     assert e_err <= 2e-2
     assert d_err <= 1e-2
+
+
+@pytest.mark.parametrize("axis", [4, 8, 12, 16])
+def test_descriptor_backward_widths(axis):
+    """dpmi_se_a_descriptor_bwd (lane-parallel kernels for axis 4/8/16, the
+    generic one otherwise) against float64 autograd of D = GR^T GR[:, :axis]."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dpmi import embed, ops
+    from dpmi._lib import ptr
+
+    _say(f"descriptor backward axis={axis}")
+    n1, nloc, nnei = 25, 300, 138
+    q1, q2, q3 = embed.internal_maps(n1)
+    g = torch.Generator(device="cpu").manual_seed(axis)
+    gr_o = torch.randn(nloc, 4, 4 * n1, generator=g, dtype=torch.float64)
+    gr = torch.zeros(nloc, 4, 128, dtype=torch.float64)
+    gr[:, :, torch.as_tensor(q3)] = gr_o
+    dd = torch.randn(nloc, 4 * n1 * axis, generator=g, dtype=torch.float64)
+    x = gr_o.clone().requires_grad_(True)
+    d = torch.bmm(x.transpose(1, 2), x[:, :, :axis]).reshape(nloc, -1)
+    (ref_o,) = torch.autograd.grad((d * dd).sum(), x)
+    ref = torch.zeros(nloc, 4, 128, dtype=torch.float64)
+    ref[:, :, torch.as_tensor(q3)] = ref_o / nnei
+    gr_d = gr.float().to(DEV)
+    dd_d = dd.float().to(DEV)
+    out = torch.full((nloc, 4, 128), float("nan"), dtype=torch.float32, device=DEV)
+    ops._launch("dpmi_se_a_descriptor_bwd", ptr(out), ptr(dd_d), ptr(gr_d), nloc, nnei, n1, axis, ops._stream())
+    torch.cuda.synchronize()
+    out = out.double().cpu()
+    assert torch.isfinite(out).all()
+    assert torch.allclose(out, ref, rtol=1e-4, atol=1e-5 * ref.abs().max().item())
