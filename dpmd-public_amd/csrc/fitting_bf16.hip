@@ -176,25 +176,39 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   const bf16x8* w1f = frag + p.w1f;
+  // weight fragments of the next K block are loaded one block ahead
+  bf16x8 a_cur[2][4];
+#pragma unroll
+  for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) a_cur[kc][mt] = w1f[((4 * w + mt) * p.nk1 + kc) * 64 + lane];
 #pragma unroll 1
   for (int kb = 0; kb < nkb; ++kb) {
     float4 r[4];
+    bf16x8 a_nxt[2][4];
     const bool more = kb + 1 < nkb;
+    const int kn = more ? kb + 1 : kb;
     if (more) load_block(kb + 1, r);
 #pragma unroll
-    for (int kc = 0; kc < 2; ++kc) {
-      bf16x8 a[4], b[4];
+    for (int kc = 0; kc < 2; ++kc)
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt) a[mt] = w1f[((4 * w + mt) * p.nk1 + 2 * kb + kc) * 64 + lane];
+      for (int mt = 0; mt < 4; ++mt) a_nxt[kc][mt] = w1f[((4 * w + mt) * p.nk1 + 2 * kn + kc) * 64 + lane];
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc) {
+      bf16x8 b[4];
 #pragma unroll
       for (int s = 0; s < 4; ++s) b[s] = lds_b(s_x[kb & 1], kXS, s, n, 32 * kc, g);
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) acc[mt][s] = mfma(a[mt], b[s], acc[mt][s]);
+        for (int s = 0; s < 4; ++s) acc[mt][s] = mfma(a_cur[kc][mt], b[s], acc[mt][s]);
     }
     if (more) store_block((kb + 1) & 1, r);
     __syncthreads();
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) a_cur[kc][mt] = a_nxt[kc][mt];
   }
 
   // ---- layer-1 epilogue: h (fp32, skip of layer 2), tanh factors f1 (fp32)
@@ -308,19 +322,27 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int s = 0; s < 4; ++s) atom_s[s] = s_atom[16 * s + n];
   const int mt_per_wave = (p.nm1 + 3) / 4;
+  const int mt0 = w * mt_per_wave, mt1 = min(p.nm1, mt0 + mt_per_wave);
+  // the next M tile's eight weight fragments are loaded one tile ahead
+  bf16x8 wa[8];
+  if (mt0 < mt1)
+#pragma unroll
+    for (int kc = 0; kc < 8; ++kc) wa[kc] = w1b[((int64_t)mt0 * 8 + kc) * 64 + lane];
 #pragma unroll 1
-  for (int m = 0; m < mt_per_wave; ++m) {
-    const int mt = w * mt_per_wave + m;
-    if (mt >= p.nm1) break;
+  for (int mt = mt0; mt < mt1; ++mt) {
+    bf16x8 wn[8];
+    const int mn = mt + 1 < mt1 ? mt + 1 : mt;
+#pragma unroll
+    for (int kc = 0; kc < 8; ++kc) wn[kc] = w1b[((int64_t)mn * 8 + kc) * 64 + lane];
     f32x4 d[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) d[s] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kc = 0; kc < 8; ++kc) {
-      const bf16x8 a = w1b[((int64_t)mt * 8 + kc) * 64 + lane];
+    for (int kc = 0; kc < 8; ++kc)
 #pragma unroll
-      for (int s = 0; s < 4; ++s) d[s] = mfma(a, bz[kc][s], d[s]);
-    }
+      for (int s = 0; s < 4; ++s) d[s] = mfma(wa[kc], bz[kc][s], d[s]);
+#pragma unroll
+    for (int kc = 0; kc < 8; ++kc) wa[kc] = wn[kc];
     const int f = 16 * mt + 4 * g;
 #pragma unroll
     for (int s = 0; s < 4; ++s)
