@@ -38,6 +38,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kH = 256;         // padded hidden width (4 wavefronts x 64 features)
 constexpr int kTile = 64;       // atoms per workgroup
@@ -112,10 +113,25 @@ __device__ inline void hidden_product(f32x4 (&acc)[4][4], const bf16x8* __restri
   }
 }
 
-__global__ void __launch_bounds__(256)
+// The tanh factors f_l = idt_l (1 - t_l^2) of the forward are parked in a
+// per-workgroup scratch block (fp16, each lane re-reads only what it wrote)
+// rather than in registers, so two workgroups fit on a CU.
+constexpr int64_t kScratchPerTile = 4 * 3 * 16 * 64;  // half4 entries: waves x layers x (mt, s) x lanes
+
+__device__ inline void put_f(f16x4* scr, int layer, int mt, int s, int lane, const float* v4) {
+  f16x4 q;
+  q[0] = (_Float16)v4[0];
+  q[1] = (_Float16)v4[1];
+  q[2] = (_Float16)v4[2];
+  q[3] = (_Float16)v4[3];
+  scr[(layer * 16 + mt * 4 + s) * 64 + lane] = q;
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_fit(double* __restrict__ atom_energy, float* __restrict__ ddesc, const float* __restrict__ desc,
           const int* __restrict__ perm, const int* __restrict__ type_start, const bf16x8* __restrict__ frag_all,
-          const float* __restrict__ consts, const double* __restrict__ ebias, FitParams p) {
+          const float* __restrict__ consts, const double* __restrict__ ebias, f16x4* __restrict__ scratch,
+          FitParams p) {
   __shared__ __attribute__((aligned(16))) __bf16 s_x[2][kTile * kXS];
   __shared__ __attribute__((aligned(16))) __bf16 s_h[kTile * kHS];
   __shared__ float s_e[4][kTile];
@@ -140,6 +156,7 @@ __global__ void __launch_bounds__(256)
   if (tid < kTile) s_atom[tid] = tid < cnt ? perm[start + tid] : -1;
   const bf16x8* frag = frag_all + (int64_t)t * p.type_frags;
   const float* cst = consts + (int64_t)t * kCst;
+  f16x4* scr = scratch + ((int64_t)blockIdx.x * 4 + w) * (3 * 16 * 64);
   __syncthreads();
 
   // ---- layer 1: D block loads (thread -> atom tid/4, features 16*(tid%4) .. +15 of the block)
@@ -150,7 +167,10 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int f = kb * kKB + 16 * xq + 4 * j;
-      r[j] = (xa >= 0 && f < p.k1) ? *reinterpret_cast<const float4*>(xrow + f) : make_float4(0.f, 0.f, 0.f, 0.f);
+      // streamed once: non-temporal, so it does not evict the weight fragments from L2
+      f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
+      if (xa >= 0 && f < p.k1) v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(xrow + f));
+      r[j] = make_float4(v[0], v[1], v[2], v[3]);
     }
   };
   auto store_block = [&](int buf, const float4 (&r)[4]) {
@@ -211,20 +231,22 @@ __global__ void __launch_bounds__(256)
       for (int mt = 0; mt < 4; ++mt) a_cur[kc][mt] = a_nxt[kc][mt];
   }
 
-  // ---- layer-1 epilogue: h (fp32, skip of layer 2), tanh factors f1 (fp32)
-  float h[4][4][4], f1[4][4][4], f2[4][4][4], f3[4][4][4];
+  // ---- layer-1 epilogue: h (fp32, skip of layer 2), tanh factor f1 to scratch
+  float h[4][4][4];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     const int j0 = 16 * (4 * w + mt) + 4 * g;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
+      float f[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const float tt = fast_tanh(acc[mt][s][v] + cst[0 * kH + j0 + v]);
         const float idt = cst[1 * kH + j0 + v];
         h[mt][s][v] = tt * idt;
-        f1[mt][s][v] = idt * (1.f - tt * tt);
+        f[v] = idt * (1.f - tt * tt);
       }
+      put_f(scr, 0, mt, s, lane, f);
       lds_put(s_h, w, mt, s, n, g, h[mt][s]);
     }
   }
@@ -242,20 +264,30 @@ __global__ void __launch_bounds__(256)
       const int j0 = 16 * (4 * w + mt) + 4 * g;
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
+        float f[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const float tt = fast_tanh(acc[mt][s][v] + bl[j0 + v]);
           const float idt = il[j0 + v];
           h[mt][s][v] += tt * idt;
-          (layer == 0 ? f2 : f3)[mt][s][v] = idt * (1.f - tt * tt);
+          f[v] = idt * (1.f - tt * tt);
         }
-        if (layer == 0) lds_put(s_h, w, mt, s, n, g, h[mt][s]);
+        if (layer == 0) {
+          put_f(scr, 1, mt, s, lane, f);
+          lds_put(s_h, w, mt, s, n, g, h[mt][s]);
+        } else {
+          // dz3 = w4 * f3 (dE/dh3 = w4)
+          float dz[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) dz[v] = cst[6 * kH + j0 + v] * f[v];
+          put_f(scr, 2, mt, s, lane, dz);
+        }
       }
     }
     if (layer == 0) __syncthreads();
   }
 
-  // ---- output layer and atomic energy; dz3 = w4 * f3 into the LDS buffer
+  // ---- output layer and atomic energy; dz3 into the LDS buffer
   const float* w4 = cst + 6 * kH;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
@@ -263,11 +295,12 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
       const int j0 = 16 * (4 * w + mt) + 4 * g;
+      const f16x4 q = scr[(2 * 16 + mt * 4 + s) * 64 + lane];
       float dz[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         e += h[mt][s][v] * w4[j0 + v];
-        dz[v] = w4[j0 + v] * f3[mt][s][v];
+        dz[v] = (float)q[v];
       }
       lds_put(s_h, w, mt, s, n, g, dz);
     }
@@ -292,6 +325,7 @@ __global__ void __launch_bounds__(256)
       const int j0 = 16 * (4 * w + mt) + 4 * g;
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
+        const f16x4 q = scr[((1 - layer) * 16 + mt * 4 + s) * 64 + lane];
         float dz[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -299,11 +333,10 @@ __global__ void __launch_bounds__(256)
           if (layer == 0) {
             dh = acc[mt][s][v] + w4[j0 + v];
             h[mt][s][v] = dh;  // h now holds dh2
-            dz[v] = dh * f2[mt][s][v];
           } else {
             dh = acc[mt][s][v] + h[mt][s][v];
-            dz[v] = dh * f1[mt][s][v];
           }
+          dz[v] = dh * (float)q[v];
         }
         lds_put(s_h, w, mt, s, n, g, dz);
       }
@@ -347,7 +380,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int s = 0; s < 4; ++s)
       if (atom_s[s] >= 0)
-        *reinterpret_cast<float4*>(ddesc + (int64_t)atom_s[s] * p.k1 + f) = make_float4(d[s][0], d[s][1], d[s][2], d[s][3]);
+        __builtin_nontemporal_store(d[s], reinterpret_cast<f32x4*>(ddesc + (int64_t)atom_s[s] * p.k1 + f));
   }
 }
 
@@ -356,13 +389,25 @@ __global__ void __launch_bounds__(256)
 
 using namespace dpmi;
 
+static int64_t fit_grid(int64_t nloc, int ntypes) { return ceil_div(nloc, kTile) + ntypes; }
+
+extern "C" int dpmi_se_a_fitting_workspace_bytes(int64_t* bytes, int64_t nloc, int ntypes) {
+  DPMI_CHECK_ARG(bytes != nullptr && nloc >= 0 && ntypes > 0 && ntypes <= kMaxTypes, "bad arguments");
+  *bytes = fit_grid(nloc, ntypes) * kScratchPerTile * (int64_t)sizeof(f16x4);
+  return DPMI_OK;
+}
+
 extern "C" int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const float* desc, const int* perm,
                                       const int* type_start, const void* frags, const float* consts,
-                                      const double* ebias, int64_t nloc, int ntypes, int k1, void* stream) {
+                                      const double* ebias, int64_t nloc, int ntypes, int k1, void* workspace,
+                                      int64_t workspace_bytes, void* stream) {
   DPMI_CHECK_ARG(ntypes > 0 && ntypes <= kMaxTypes, "ntypes out of range [1, 16]");
   DPMI_CHECK_ARG(k1 > 0 && k1 % 16 == 0, "descriptor width must be a positive multiple of 16");
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
   if (nloc == 0) return DPMI_OK;
+  DPMI_CHECK_ARG(workspace != nullptr &&
+                     workspace_bytes >= fit_grid(nloc, ntypes) * kScratchPerTile * (int64_t)sizeof(f16x4),
+                 "workspace too small (dpmi_se_a_fitting_workspace_bytes)");
   FitParams p;
   std::memset(&p, 0, sizeof(p));
   p.ntypes = ntypes;
@@ -378,9 +423,9 @@ extern "C" int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const f
   p.w1b = p.w2b + 16 * 8 * 64;
   p.type_frags = p.w1b + (int64_t)p.nm1 * 8 * 64;
   // one workgroup per 64-atom tile of each type (upper bound; extra ones exit)
-  const int64_t grid = ceil_div(nloc, kTile) + ntypes;
+  const int64_t grid = fit_grid(nloc, ntypes);
   hipLaunchKernelGGL(k_fit, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), atom_energy, ddesc, desc, perm,
-                     type_start, static_cast<const bf16x8*>(frags), consts, ebias, p);
+                     type_start, static_cast<const bf16x8*>(frags), consts, ebias, static_cast<f16x4*>(workspace), p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }

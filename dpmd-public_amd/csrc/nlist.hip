@@ -124,7 +124,7 @@ GridParams pbc_grid(const double* box, float rcut, int nloc) {
 
 struct WsLayout {
   int64_t params, misc, bbox, cell_of, cell_count, cell_start, cell_cursor, cell_unordered, cell_atoms, ghost_off,
-      total;
+      redo, total;
 };
 
 __host__ __device__ inline int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
@@ -142,6 +142,8 @@ __host__ __device__ inline WsLayout ws_layout(int64_t nloc, int64_t nlc, int64_t
   o = align256(o + 4 * 16);
   w.bbox = o;
   o = align256(o + 8 * 8);
+  w.redo = o;  // offset independent of the cell counts (the host reads it)
+  o = align256(o + 4 * na);
   w.cell_of = o;
   o = align256(o + 4 * na);
   w.cell_count = o;
@@ -418,7 +420,7 @@ template <int CAP>
 __global__ void __launch_bounds__(64) k_search_format(
     int* __restrict__ nlist, int* __restrict__ overflow, int* __restrict__ nlist_mapped,
     const double* __restrict__ ext_c, const int* __restrict__ ext_t, const int* __restrict__ mapping,
-    char* __restrict__ ws, SelParams sp, float rcut) {
+    char* __restrict__ ws, const int* __restrict__ atoms, SelParams sp, float rcut) {
   extern __shared__ unsigned char smem_raw[];
   uint64_t* keys = reinterpret_cast<uint64_t*>(smem_raw);
   int* row = reinterpret_cast<int*>(keys + CAP);
@@ -429,7 +431,7 @@ __global__ void __launch_bounds__(64) k_search_format(
   const int* __restrict__ cell_atoms = reinterpret_cast<const int*>(ws + L.cell_atoms);
   const int* __restrict__ ghost_off = reinterpret_cast<const int*>(ws + L.ghost_off);
   int* misc = reinterpret_cast<int*>(ws + L.misc);
-  const int i = blockIdx.x;
+  const int i = atoms ? atoms[blockIdx.x] : (int)blockIdx.x;
   const int lane = lane_id();
   const double rcut2d = (double)(rcut * rcut);
   const float rcut2f = rcut * rcut;
@@ -527,6 +529,193 @@ __global__ void __launch_bounds__(64) k_search_format(
   }
   for (int off = 32; off > 0; off >>= 1) ovf = max(ovf, __shfl_xor(ovf, off, kWave));
   if (overflow && lane == 0) overflow[i] = ovf;
+}
+
+// Step 5, cell-wise: one workgroup per local cell (grid-stride over cells).
+// The candidate atoms of the cell's (2 nsearch + 1)^3 neighborhood are
+// gathered into LDS once and every atom of the cell tests them from there
+// (one wavefront per atom), instead of each atom re-walking the cell lists.
+// Same candidate tests, keys and ranking as k_search_format; an atom with
+// more than kKeyCap candidates, or a cell whose neighborhood exceeds
+// kCandCap atoms, is appended to the redo list for k_search_format.
+constexpr int kCandCap = 1536;
+constexpr int kKeyCap = 512;  // plus 8 padding slots per wavefront
+constexpr int kMaxNb = 128;  // (2 nsearch + 1)^3 <= 125
+
+__global__ void __launch_bounds__(256) k_search_cells(
+    int* __restrict__ nlist, int* __restrict__ overflow, int* __restrict__ nlist_mapped,
+    const double* __restrict__ ext_c, const int* __restrict__ ext_t, const int* __restrict__ mapping,
+    char* __restrict__ ws, SelParams sp, float rcut) {
+  extern __shared__ unsigned char smem_raw[];
+  double* cx = reinterpret_cast<double*>(smem_raw);
+  double* cy = cx + kCandCap;
+  double* cz = cy + kCandCap;
+  int* cj = reinterpret_cast<int*>(cz + kCandCap);
+  int* ct = cj + kCandCap;
+  uint64_t* keys_all = reinterpret_cast<uint64_t*>(ct + kCandCap);
+  int* rows_all = reinterpret_cast<int*>(keys_all + 4 * (kKeyCap + 8));
+  __shared__ int s_base[kMaxNb], s_local[kMaxNb], s_pref[kMaxNb + 1];
+
+  const GridParams p = *reinterpret_cast<const GridParams*>(ws);
+  const WsLayout L = ws_layout(p.nloc, p.lay_nlc, p.lay_next);
+  const int* __restrict__ cell_start = reinterpret_cast<const int*>(ws + L.cell_start);
+  const int* __restrict__ cell_atoms = reinterpret_cast<const int*>(ws + L.cell_atoms);
+  const int* __restrict__ ghost_off = reinterpret_cast<const int*>(ws + L.ghost_off);
+  int* misc = reinterpret_cast<int*>(ws + L.misc);
+  int* redo = reinterpret_cast<int*>(ws + L.redo);
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const double rcut2d = (double)(rcut * rcut);
+  const float rcut2f = rcut * rcut;
+  const int ns = p.nsearch, side = 2 * ns + 1, nnb = side * side * side;
+  const int nlc = p.ncell[0] * p.ncell[1] * p.ncell[2];
+  uint64_t* keys = keys_all + w * (kKeyCap + 8);
+  int* row = rows_all + w * sp.nnei;
+
+  for (int cc = blockIdx.x; cc < nlc; cc += gridDim.x) {
+    const int a0 = cell_start[cc], na = cell_start[cc + 1] - a0;
+    if (na == 0) continue;  // uniform in the workgroup
+    const int c2 = cc % p.ncell[2], c1 = (cc / p.ncell[2]) % p.ncell[1], c0 = cc / (p.ncell[1] * p.ncell[2]);
+    // neighborhood table: atom range of each neighbor cell
+    if (tid < nnb) {
+      const int ii0 = c0 + tid / (side * side) - ns, ii1 = c1 + (tid / side) % side - ns, ii2 = c2 + tid % side - ns;
+      int base = 0, cnt = 0, local = 1;
+      if (p.pbc) {
+        if (!(ii0 < -p.ngcell[0] || ii0 >= p.ncell[0] + p.ngcell[0] || ii1 < -p.ngcell[1] ||
+              ii1 >= p.ncell[1] + p.ngcell[1] || ii2 < -p.ngcell[2] || ii2 >= p.ncell[2] + p.ngcell[2])) {
+          local = ii0 >= 0 && ii0 < p.ncell[0] && ii1 >= 0 && ii1 < p.ncell[1] && ii2 >= 0 && ii2 < p.ncell[2];
+          const int j0 = ii0 + pbc_shift(ii0, p.ncell[0]) * p.ncell[0];
+          const int j1 = ii1 + pbc_shift(ii1, p.ncell[1]) * p.ncell[1];
+          const int j2 = ii2 + pbc_shift(ii2, p.ncell[2]) * p.ncell[2];
+          const int q = (j0 * p.ncell[1] + j1) * p.ncell[2] + j2;
+          base = cell_start[q];
+          cnt = cell_start[q + 1] - base;
+          if (!local) {
+            const int e = ((ii0 + p.ngcell[0]) * p.ext[1] + (ii1 + p.ngcell[1])) * p.ext[2] + (ii2 + p.ngcell[2]);
+            base = p.nloc + ghost_off[e];
+          }
+        }
+      } else if (!(ii0 < 0 || ii0 >= p.ncell[0] || ii1 < 0 || ii1 >= p.ncell[1] || ii2 < 0 || ii2 >= p.ncell[2])) {
+        const int q = (ii0 * p.ncell[1] + ii1) * p.ncell[2] + ii2;
+        base = cell_start[q];
+        cnt = cell_start[q + 1] - base;
+      }
+      s_base[tid] = base;
+      s_local[tid] = local;
+      s_pref[tid + 1] = cnt;
+    }
+    if (tid == 0) s_pref[0] = 0;
+    __syncthreads();
+    if (tid == 0)
+      for (int t = 0; t < nnb; ++t) s_pref[t + 1] += s_pref[t];
+    __syncthreads();
+    const int T = s_pref[nnb];
+    if (T > kCandCap) {
+      for (int k = tid; k < na; k += blockDim.x) redo[atomicAdd(&misc[2], 1)] = cell_atoms[a0 + k];
+      __syncthreads();
+      continue;
+    }
+    for (int g = tid; g < T; g += blockDim.x) {
+      int lo = 0, hi = nnb;  // last t with s_pref[t] <= g
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_pref[mid] <= g) lo = mid; else hi = mid;
+      }
+      const int k = g - s_pref[lo];
+      const int j = s_local[lo] ? cell_atoms[s_base[lo] + k] : s_base[lo] + k;
+      cj[g] = j;
+      ct[g] = ext_t[j];
+      cx[g] = ext_c[3 * j + 0];
+      cy[g] = ext_c[3 * j + 1];
+      cz[g] = ext_c[3 * j + 2];
+    }
+    __syncthreads();
+    for (int ai = w; ai < na; ai += 4) {
+      const int i = cell_atoms[a0 + ai];
+      const double xi0 = ext_c[3 * i + 0], xi1 = ext_c[3 * i + 1], xi2 = ext_c[3 * i + 2];
+      const float fi0 = (float)xi0, fi1 = (float)xi1, fi2 = (float)xi2;
+      int n = 0;
+      for (int g0 = 0; g0 < T; g0 += kWave) {
+        const int g = g0 + lane;
+        bool pass = false;
+        uint64_t key = 0;
+        if (g < T) {
+          const int j = cj[g];
+          if (j != i) {
+            const double xj0 = cx[g], xj1 = cy[g], xj2 = cz[g];
+            const double e0 = xi0 - xj0, e1 = xi1 - xj1, e2 = xi2 - xj2;
+            const double r2 = e0 * e0 + e1 * e1 + e2 * e2;
+            if (r2 < rcut2d) {
+              const int tj = ct[g];
+              const float f0 = (float)xj0 - fi0, f1 = (float)xj1 - fi1, f2 = (float)xj2 - fi2;
+              const float rr2 = f0 * f0 + f1 * f1 + f2 * f2;
+              if (tj >= 0 && tj < sp.ntypes && rr2 <= rcut2f) {
+                pass = true;
+                key = ((uint64_t)tj << 56) | ((uint64_t)__float_as_uint(rr2) << 24) | (uint64_t)j;
+              }
+            }
+          }
+        }
+        const uint64_t m = __ballot(pass);
+        const int pos = n + mask_prefix(m);
+        if (pass && pos < kKeyCap) keys[pos] = key;
+        n += __popcll(m);
+      }
+      if (n > kKeyCap) {  // wave-uniform: leave this atom to the per-atom kernel
+        if (lane == 0) redo[atomicAdd(&misc[2], 1)] = i;
+        continue;
+      }
+      for (int s = lane; s < sp.nnei; s += kWave) row[s] = -1;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      // rank of each key within its type: two keys per lane per pass over the
+      // key list, read 8 at a time (independent LDS reads)
+      if (n & 7) {  // pad to a multiple of 8 with keys that rank after everything
+        if (lane < 8 - (n & 7)) keys[n + lane] = ~0ull;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+      const int n8 = (n + 7) & ~7;
+      int ovf = -1;
+      for (int k0 = 0; k0 < n; k0 += 2 * kWave) {
+        const int ka = k0 + lane, kb = k0 + kWave + lane;
+        const uint64_t kka = ka < n ? keys[ka] : ~0ull, kkb = kb < n ? keys[kb] : ~0ull;
+        const uint64_t ta = kka & (0xFFull << 56), tb = kkb & (0xFFull << 56);
+        const uint64_t rela = kka - ta, relb = kkb - tb;
+        int ra = 0, rb = 0;
+        for (int q = 0; q < n8; q += 8) {
+          uint64_t kq[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) kq[u] = keys[q + u];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            ra += (kq[u] - ta) < rela;
+            rb += (kq[u] - tb) < relb;
+          }
+        }
+        if (ka < n) {
+          const int t = (int)(kka >> 56);
+          if (ra < sp.sel[t]) row[sp.sec[t] + ra] = (int)(kka & 0xFFFFFFull); else ovf = max(ovf, t);
+        }
+        if (kb < n) {
+          const int t = (int)(kkb >> 56);
+          if (rb < sp.sel[t]) row[sp.sec[t] + rb] = (int)(kkb & 0xFFFFFFull); else ovf = max(ovf, t);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      const int64_t ro = (int64_t)i * sp.nnei;
+      for (int s = lane; s < sp.nnei; s += kWave) {
+        const int j = row[s];
+        nlist[ro + s] = j;
+        if (nlist_mapped) nlist_mapped[ro + s] = j >= 0 ? mapping[j] : -1;
+      }
+      for (int off = 32; off > 0; off >>= 1) ovf = max(ovf, __shfl_xor(ovf, off, kWave));
+      if (overflow && lane == 0) overflow[i] = ovf;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();  // the next cell overwrites the candidate arrays
+  }
 }
 
 }  // namespace
@@ -666,13 +855,30 @@ extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped
   hipStream_t st = as_stream(stream);
   char* ws = const_cast<char*>(static_cast<const char*>(workspace));
   int* misc = reinterpret_cast<int*>(ws + ws_layout(0, 0, 0).misc);
-  size_t shm = 1024 * sizeof(uint64_t) + sizeof(int) * sp.nnei;
-  hipLaunchKernelGGL(k_search_format<1024>, dim3(nloc), dim3(64), shm, st, nlist, overflow, nlist_mapped, ext_coord,
-                     ext_type, mapping, ws, sp, rcut);
+  // status words: misc[1] capacity error, misc[2] redo count
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, misc + 1, (int64_t)2, 0);
+  const size_t shm_cells = (size_t)kCandCap * (3 * sizeof(double) + 2 * sizeof(int)) +
+                           4 * ((size_t)(kKeyCap + 8) * sizeof(uint64_t) + sizeof(int) * sp.nnei);
+  if (shm_cells > 65536)
+    DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_search_cells),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_cells));
+  hipLaunchKernelGGL(k_search_cells, dim3((unsigned)std::min<int64_t>(nloc, 2048)), dim3(256), shm_cells, st, nlist,
+                     overflow, nlist_mapped, ext_coord, ext_type, mapping, ws, sp, rcut);
   DPMI_LAUNCH_CHECK();
-  int err = 0;
-  DPMI_HIP(hipMemcpyAsync(&err, misc + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  int stat[2] = {0, 0};
+  DPMI_HIP(hipMemcpyAsync(stat, misc + 1, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
   DPMI_HIP(hipStreamSynchronize(st));
+  if (stat[1] > 0) {
+    // atoms left by the cell kernel: one wavefront each, up to 1024 candidates
+    const int* redo = reinterpret_cast<const int*>(ws + ws_layout(nloc, 0, 0).redo);
+    size_t shm = 1024 * sizeof(uint64_t) + sizeof(int) * sp.nnei;
+    hipLaunchKernelGGL(k_search_format<1024>, dim3(stat[1]), dim3(64), shm, st, nlist, overflow, nlist_mapped,
+                       ext_coord, ext_type, mapping, ws, redo, sp, rcut);
+    DPMI_LAUNCH_CHECK();
+    DPMI_HIP(hipMemcpyAsync(stat, misc + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    DPMI_HIP(hipStreamSynchronize(st));
+  }
+  int err = stat[0];
   if (err) {
     set_error("more than 1024 candidates within rcut for some atom");
     return DPMI_ERR_CAPACITY;

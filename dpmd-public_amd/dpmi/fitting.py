@@ -18,10 +18,12 @@ where a layer has no timestep), and ebias (fp64) = bias_atom_e + out_bias.
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
-from . import ops
+from . import _lib, ops
 from ._lib import ptr
 
 KH = 256
@@ -127,6 +129,10 @@ class FusedFitting:
         desc = desc.contiguous()
         ae = torch.empty(nloc, dtype=torch.float64, device=desc.device)
         dd = torch.empty_like(desc)
+        wsb = ctypes.c_int64(0)
+        _lib.call("dpmi_se_a_fitting_workspace_bytes", ctypes.byref(wsb), nloc, self.ntypes)
+        ws = torch.empty(max(int(wsb.value), 1), dtype=torch.uint8, device=desc.device)
         ops._launch("dpmi_se_a_fitting_bf16", ptr(ae), ptr(dd), ptr(desc), ptr(perm), ptr(start), ptr(self.frags),
-                    ptr(self.consts), ptr(self.ebias), nloc, self.ntypes, self.k1, ops._stream())
+                    ptr(self.consts), ptr(self.ebias), nloc, self.ntypes, self.k1, ptr(ws), int(wsb.value),
+                    ops._stream())
         return ae, dd

@@ -216,7 +216,10 @@ int dpmi_se_a_embed_bwd_bf16(float* net_deriv, const float* em, const float* dgr
  * ddesc [nloc, k1] fp32 = dE/d desc for E = sum atom_energy. */
 int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const float* desc, const int* perm,
                            const int* type_start, const void* frags, const float* consts, const double* ebias,
-                           int64_t nloc, int ntypes, int k1, void* stream);
+                           int64_t nloc, int ntypes, int k1, void* workspace, int64_t workspace_bytes,
+                           void* stream);
+/* Workspace of dpmi_se_a_fitting_bf16 (per-tile activation derivatives). */
+int dpmi_se_a_fitting_workspace_bytes(int64_t* bytes, int64_t nloc, int ntypes);
 /* Number of bf16x8 lanes (64 per fragment) of one type's packed fitting net. */
 int dpmi_se_a_fitting_frag_count(int64_t* count, int k1);
 

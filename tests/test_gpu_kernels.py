@@ -28,8 +28,24 @@ def T(a, dtype=None):
     return torch.as_tensor(np.ascontiguousarray(a), device=DEV, dtype=dtype)
 
 
+def _clustered():
+    """3000 atoms spread over a 40 A box plus 600 packed into a 2.5 A ball:
+    the ball's atoms have more candidates than the cell-wise search keeps
+    per atom, so they go through the per-atom path."""
+    rng = np.random.default_rng(11)
+    spread = rng.uniform(0, 40, (3000, 3))
+    v = rng.normal(size=(600, 3))
+    ball = 20 + v / np.linalg.norm(v, axis=1, keepdims=True) * 2.5 * rng.uniform(0, 1, (600, 1)) ** (1 / 3)
+    coord = np.concatenate([spread, ball])
+    atype = rng.integers(0, 2, coord.shape[0]).astype(np.int32)
+    return systems.System("clustered", coord, atype, np.diag([40.0, 40.0, 40.0]).ravel())
+
+
 NLIST_CASES = [
     ("water192", lambda: systems.water192(0), 6.0, [46, 92]),
+    # neighborhoods larger than the cell-wise search's LDS candidate buffer
+    ("water12k_rcut9", lambda: systems.water12k(1), 9.0, [110, 220]),
+    ("clustered", _clustered, 6.0, [200, 200]),
     ("water192_tight_sel", lambda: systems.water192(2), 6.0, [20, 40]),
     ("water12k", lambda: systems.water12k(1), 6.0, [46, 92]),
     ("alloy_triclinic_4k", lambda: systems.alloy262k(4, ncell=16), 8.0, [60, 60, 60, 60]),
