@@ -53,8 +53,16 @@ constexpr int kAtoms = 4;      // atoms (wavefronts) per workgroup
 struct EmbedParams {
   int ntypes;
   int nnei;
+  int n1;
   int sec[kMaxTypes + 1];
 };
+
+// original feature (0 .. 4 n1 - 1) of internal G index q, or -1 for padding
+__host__ __device__ inline int orig_of_q3(int q, int n1) {
+  const int base = q >= 64 ? 2 * n1 : 0, q2 = q & 63;
+  const int f2 = q2 >= 32 ? (q2 - 32 < n1 ? n1 + (q2 - 32) : -1) : (q2 < n1 ? q2 : -1);
+  return f2 < 0 ? -1 : base + f2;
+}
 
 // tanh(x) from z = 2 log2(e) x: 1 - 2 / (2^z + 1), exact limits at +-inf.
 // The forward weights, biases and layer-1 constants are stored pre-scaled by
@@ -119,7 +127,8 @@ __device__ inline void forward_tile(float s, const bf16x8* __restrict__ frag, co
   }
 }
 
-// Forward: per (atom, neighbor type) wavefront, GR partial [t][i][c][q3].
+// Forward: per (atom, neighbor type) wavefront, GR partial [t][i][c][m]
+// (original feature order m < 4 n1).
 // Persistent grid: a workgroup loads its type's weight fragments once and its
 // four wavefronts walk atoms independently (no block barrier after set-up).
 // G^T R contracts over neighbor rows while G comes out of the layer-3 product
@@ -212,11 +221,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       __builtin_amdgcn_wave_barrier();
     }
     if (g == 0) {
-      float* out = gr_part + ((int64_t)t * nloc + i) * 4 * kQ3;
+      // GR_t^T rows in the original feature order: gr_part[t][i][c][m]
+      const int M = 4 * p.n1;
+      float* out = gr_part + ((int64_t)t * nloc + i) * 4 * M;
 #pragma unroll
-      for (int nt = 0; nt < 8; ++nt)
+      for (int nt = 0; nt < 8; ++nt) {
+        const int m = orig_of_q3(16 * nt + r, p.n1);
+        if (m >= 0)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) out[v * kQ3 + 16 * nt + r] = acc[nt][v];
+          for (int v = 0; v < 4; ++v) out[v * M + m] = acc[nt][v];
+      }
     }
   }
 }
@@ -229,36 +243,37 @@ __device__ inline int q3_of(int f, int n1) {
   return base + (f2 >= n1 ? 32 + (f2 - n1) : f2);
 }
 
-// GR = sum_t GR_t / nnei (internal order, saved for the backward) and
-// D[m][a] = sum_c GR[c][m] GR[c][a] in the original order (se_a.py:851-858).
+// GR = sum_t GR_t / nnei (original order, saved for the backward) and
+// D[m][a] = sum_c GR[c][m] GR[c][a] (se_a.py:851-858).
+template <typename DT>
 __global__ void __launch_bounds__(256)
-    k_descrpt_fwd(float* __restrict__ gr, float* __restrict__ desc, const float* __restrict__ gr_part, int nloc,
+    k_descrpt_fwd(float* __restrict__ gr, DT* __restrict__ desc, const float* __restrict__ gr_part, int nloc,
                   int ntypes, float inv_nnei, int n1, int axis) {
   __shared__ float s[kAtoms][4 * kQ3];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = blockIdx.x * kAtoms + w;
   if (i >= nloc) return;
-  for (int k = lane; k < 4 * kQ3; k += 64) {
+  const int M = 4 * n1;
+  for (int k = lane; k < 4 * M; k += 64) {
     float v = 0.f;
-    for (int t = 0; t < ntypes; ++t) v += gr_part[((int64_t)t * nloc + i) * 4 * kQ3 + k];
+    for (int t = 0; t < ntypes; ++t) v += gr_part[((int64_t)t * nloc + i) * 4 * M + k];
     v *= inv_nnei;
     s[w][k] = v;
-    gr[(int64_t)i * 4 * kQ3 + k] = v;
+    gr[(int64_t)i * 4 * M + k] = v;
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  const int M = 4 * n1;
   for (int k = lane; k < M * axis; k += 64) {
     const int m = k / axis, a = k - m * axis;
-    const int qm = q3_of(m, n1), qa = q3_of(a, n1);
     float d = 0.f;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) d += s[w][c * kQ3 + qm] * s[w][c * kQ3 + qa];
-    desc[(int64_t)i * M * axis + k] = d;
+    for (int c = 0; c < 4; ++c) d += s[w][c * M + m] * s[w][c * M + a];
+    desc[(int64_t)i * M * axis + k] = (DT)d;
   }
 }
 
-// dGR_raw[c][q3(k)] = (sum_a dD[k][a] GR[c][a] + [k < axis] sum_m dD[m][k] GR[c][m]) / nnei
+// dGR[c][q3(k)] = (sum_a dD[k][a] GR[c][a] + [k < axis] sum_m dD[m][k] GR[c][m]) / nnei
+// (GR in the original order, dGR in the internal order of the embedding kernels)
 __global__ void __launch_bounds__(256)
     k_descrpt_bwd(float* __restrict__ dgr, const float* __restrict__ ddesc, const float* __restrict__ gr, int nloc,
                   float inv_nnei, int n1, int axis) {
@@ -268,29 +283,24 @@ __global__ void __launch_bounds__(256)
   const int i = blockIdx.x * kAtoms + w;
   if (i >= nloc) return;
   const int M = 4 * n1;
-  for (int k = lane; k < 4 * kQ3; k += 64) s[w][k] = gr[(int64_t)i * 4 * kQ3 + k];
+  for (int k = lane; k < 4 * M; k += 64) s[w][k] = gr[(int64_t)i * 4 * M + k];
   for (int k = lane; k < M * axis; k += 64) sd[w][k] = ddesc[(int64_t)i * M * axis + k];
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   for (int q = lane; q < kQ3; q += 64) {
-    // original feature of internal q (or -1 for padding)
-    int base = q >= 64 ? 2 * n1 : 0, q2 = q & 63;
-    int f2 = q2 >= 32 ? (q2 - 32 < n1 ? n1 + (q2 - 32) : -1) : (q2 < n1 ? q2 : -1);
-    int k = f2 < 0 ? -1 : base + f2;
+    const int k = orig_of_q3(q, n1);
     float out[4] = {0.f, 0.f, 0.f, 0.f};
     if (k >= 0) {
       for (int a = 0; a < axis; ++a) {
         float dd = sd[w][k * axis + a];
-        int qa = q3_of(a, n1);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) out[c] += dd * s[w][c * kQ3 + qa];
+        for (int c = 0; c < 4; ++c) out[c] += dd * s[w][c * M + a];
       }
       if (k < axis) {
         for (int m = 0; m < M; ++m) {
           float dd = sd[w][m * axis + k];
-          int qm = q3_of(m, n1);
 #pragma unroll
-          for (int c = 0; c < 4; ++c) out[c] += dd * s[w][c * kQ3 + qm];
+          for (int c = 0; c < 4; ++c) out[c] += dd * s[w][c * M + m];
         }
       }
     }
@@ -322,7 +332,7 @@ __global__ void __launch_bounds__(256)
   float* so = s_out[w];
   float* sga = s_ga[w];
   float* sdd = s_dd[w];
-  const float* gri = gr + (int64_t)i * 4 * kQ3;
+  const float* gri = gr + (int64_t)i * 4 * M;
   const float4* dd4 = reinterpret_cast<const float4*>(ddesc + (int64_t)i * M * AXIS);
   const int n4 = M * AXIS / 4;
 #pragma unroll 4
@@ -335,14 +345,14 @@ __global__ void __launch_bounds__(256)
     sdd[m * kS + a + 3] = x.w;
   }
   for (int k = lane; k < 4 * kQ3; k += 64) {
-    sg[k] = gri[k];
+    if (k < 4 * M) sg[k] = gri[k];
     so[k] = 0.f;
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   if (lane < AXIS * 4) {
     const int a = lane >> 2, c = lane & 3;
-    sga[4 * a + c] = sg[c * kQ3 + q3_of(a, n1)];
+    sga[4 * a + c] = sg[c * M + a];
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -367,9 +377,8 @@ __global__ void __launch_bounds__(256)
   float t2[4] = {0.f, 0.f, 0.f, 0.f};
   for (int m = m0; m < M; m += kGroups) {
     const float x = sdd[m * kS + a];
-    const int qm = q3_of(m, n1);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) t2[c] += x * sg[c * kQ3 + qm];
+    for (int c = 0; c < 4; ++c) t2[c] += x * sg[c * M + m];
   }
 #pragma unroll
   for (int off = AXIS; off < 64; off <<= 1)
@@ -553,10 +562,12 @@ int make_params(EmbedParams* p, const int* sel_host, int ntypes) {
 using namespace dpmi;
 
 extern "C" int dpmi_se_a_embed_fwd_bf16(float* gr_part, const float* em, const void* frags, const float* consts,
-                                        int64_t nloc, const int* sel_host, int ntypes, void* stream) {
+                                        int64_t nloc, const int* sel_host, int ntypes, int n1, void* stream) {
   EmbedParams p;
   int rc = make_params(&p, sel_host, ntypes);
   if (rc) return rc;
+  DPMI_CHECK_ARG(n1 > 0 && n1 <= 32, "n1 out of range [1, 32]");
+  p.n1 = n1;
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
   if (nloc == 0) return DPMI_OK;
   // persistent grid: enough workgroups per type to fill 256 CUs a few times over
@@ -567,12 +578,16 @@ extern "C" int dpmi_se_a_embed_fwd_bf16(float* gr_part, const float* em, const v
   return DPMI_OK;
 }
 
-extern "C" int dpmi_se_a_descriptor_fwd(float* gr, float* desc, const float* gr_part, int64_t nloc, int ntypes,
-                                        int nnei, int n1, int axis, void* stream) {
+extern "C" int dpmi_se_a_descriptor_fwd(float* gr, void* desc, const float* gr_part, int64_t nloc, int ntypes,
+                                        int nnei, int n1, int axis, int desc_bf16, void* stream) {
   DPMI_CHECK_ARG(n1 > 0 && n1 <= 32 && axis > 0 && axis <= 4 * n1 && axis <= 16, "unsupported widths");
   if (nloc == 0) return DPMI_OK;
-  hipLaunchKernelGGL(k_descrpt_fwd, dim3(ceil_div(nloc, kAtoms)), dim3(256), 0, as_stream(stream), gr, desc, gr_part,
-                     (int)nloc, ntypes, 1.f / (float)nnei, n1, axis);
+  if (desc_bf16)
+    hipLaunchKernelGGL(k_descrpt_fwd<__bf16>, dim3(ceil_div(nloc, kAtoms)), dim3(256), 0, as_stream(stream), gr,
+                       static_cast<__bf16*>(desc), gr_part, (int)nloc, ntypes, 1.f / (float)nnei, n1, axis);
+  else
+    hipLaunchKernelGGL(k_descrpt_fwd<float>, dim3(ceil_div(nloc, kAtoms)), dim3(256), 0, as_stream(stream), gr,
+                       static_cast<float*>(desc), gr_part, (int)nloc, ntypes, 1.f / (float)nnei, n1, axis);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }

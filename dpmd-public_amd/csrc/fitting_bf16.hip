@@ -20,7 +20,7 @@
 // output feature), the layer input is the B operand (K = input feature,
 // N = atom), read from an LDS buffer [atom][feature] that every layer's
 // output is written back to.  The D matrix streams through LDS in 64-wide K
-// blocks, converted to bf16 on the way and prefetched one block ahead.  The
+// blocks (bf16, as the descriptor kernel writes it), prefetched one block ahead.  The
 // backward pass runs the transposed products (dh_{l-1} = dz_l W_l^T + skip)
 // through the same LDS buffer and ends with dD = dz1 W1^T, whose B operands
 // (dz1 for all 64 atoms) stay in registers while W1's fragments stream from
@@ -38,7 +38,6 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kH = 256;         // padded hidden width (4 wavefronts x 64 features)
 constexpr int kTile = 64;       // atoms per workgroup
@@ -113,25 +112,10 @@ __device__ inline void hidden_product(f32x4 (&acc)[4][4], const bf16x8* __restri
   }
 }
 
-// The tanh factors f_l = idt_l (1 - t_l^2) of the forward are parked in a
-// per-workgroup scratch block (fp16, each lane re-reads only what it wrote)
-// rather than in registers, so two workgroups fit on a CU.
-constexpr int64_t kScratchPerTile = 4 * 3 * 16 * 64;  // half4 entries: waves x layers x (mt, s) x lanes
-
-__device__ inline void put_f(f16x4* scr, int layer, int mt, int s, int lane, const float* v4) {
-  f16x4 q;
-  q[0] = (_Float16)v4[0];
-  q[1] = (_Float16)v4[1];
-  q[2] = (_Float16)v4[2];
-  q[3] = (_Float16)v4[3];
-  scr[(layer * 16 + mt * 4 + s) * 64 + lane] = q;
-}
-
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
-    k_fit(double* __restrict__ atom_energy, float* __restrict__ ddesc, const float* __restrict__ desc,
+__global__ void __launch_bounds__(256)
+    k_fit(double* __restrict__ atom_energy, float* __restrict__ ddesc, const __bf16* __restrict__ desc,
           const int* __restrict__ perm, const int* __restrict__ type_start, const bf16x8* __restrict__ frag_all,
-          const float* __restrict__ consts, const double* __restrict__ ebias, f16x4* __restrict__ scratch,
-          FitParams p) {
+          const float* __restrict__ consts, const double* __restrict__ ebias, FitParams p) {
   __shared__ __attribute__((aligned(16))) __bf16 s_x[2][kTile * kXS];
   __shared__ __attribute__((aligned(16))) __bf16 s_h[kTile * kHS];
   __shared__ float s_e[4][kTile];
@@ -156,32 +140,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   if (tid < kTile) s_atom[tid] = tid < cnt ? perm[start + tid] : -1;
   const bf16x8* frag = frag_all + (int64_t)t * p.type_frags;
   const float* cst = consts + (int64_t)t * kCst;
-  f16x4* scr = scratch + ((int64_t)blockIdx.x * 4 + w) * (3 * 16 * 64);
   __syncthreads();
 
-  // ---- layer 1: D block loads (thread -> atom tid/4, features 16*(tid%4) .. +15 of the block)
+  // ---- layer 1: D block loads (thread -> atom tid/4, features 16*(tid%4) .. +15 of the block, bf16)
   const int xa = s_atom[tid >> 2], xq = tid & 3;
-  const float* xrow = desc + (int64_t)(xa < 0 ? 0 : xa) * p.k1;
+  const __bf16* xrow = desc + (int64_t)(xa < 0 ? 0 : xa) * p.k1;
   const int nkb = p.nk1 / 2;
-  auto load_block = [&](int kb, float4 (&r)[4]) {
+  auto load_block = [&](int kb, bf16x8 (&r)[2]) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int f = kb * kKB + 16 * xq + 4 * j;
+    for (int j = 0; j < 2; ++j) {
+      const int f = kb * kKB + 16 * xq + 8 * j;
       // streamed once: non-temporal, so it does not evict the weight fragments from L2
-      f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
-      if (xa >= 0 && f < p.k1) v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(xrow + f));
-      r[j] = make_float4(v[0], v[1], v[2], v[3]);
+      r[j] = (xa >= 0 && f < p.k1) ? __builtin_nontemporal_load(reinterpret_cast<const bf16x8*>(xrow + f))
+                                   : bf16x8{};
     }
   };
-  auto store_block = [&](int buf, const float4 (&r)[4]) {
-    bf16x8 lo, hi;
-    lo[0] = (__bf16)r[0].x; lo[1] = (__bf16)r[0].y; lo[2] = (__bf16)r[0].z; lo[3] = (__bf16)r[0].w;
-    lo[4] = (__bf16)r[1].x; lo[5] = (__bf16)r[1].y; lo[6] = (__bf16)r[1].z; lo[7] = (__bf16)r[1].w;
-    hi[0] = (__bf16)r[2].x; hi[1] = (__bf16)r[2].y; hi[2] = (__bf16)r[2].z; hi[3] = (__bf16)r[2].w;
-    hi[4] = (__bf16)r[3].x; hi[5] = (__bf16)r[3].y; hi[6] = (__bf16)r[3].z; hi[7] = (__bf16)r[3].w;
+  auto store_block = [&](int buf, const bf16x8 (&r)[2]) {
     __bf16* dst = s_x[buf] + (tid >> 2) * kXS + 16 * xq;
-    *reinterpret_cast<bf16x8*>(dst) = lo;
-    *reinterpret_cast<bf16x8*>(dst + 8) = hi;
+    *reinterpret_cast<bf16x8*>(dst) = r[0];
+    *reinterpret_cast<bf16x8*>(dst + 8) = r[1];
   };
 
   f32x4 acc[4][4];
@@ -190,7 +167,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
     for (int s = 0; s < 4; ++s) acc[mt][s] = f32x4{0.f, 0.f, 0.f, 0.f};
   {
-    float4 r[4];
+    bf16x8 r[2];
     load_block(0, r);
     store_block(0, r);
   }
@@ -204,7 +181,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     for (int mt = 0; mt < 4; ++mt) a_cur[kc][mt] = w1f[((4 * w + mt) * p.nk1 + kc) * 64 + lane];
 #pragma unroll 1
   for (int kb = 0; kb < nkb; ++kb) {
-    float4 r[4];
+    bf16x8 r[2];
     bf16x8 a_nxt[2][4];
     const bool more = kb + 1 < nkb;
     const int kn = more ? kb + 1 : kb;
@@ -231,22 +208,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       for (int mt = 0; mt < 4; ++mt) a_cur[kc][mt] = a_nxt[kc][mt];
   }
 
-  // ---- layer-1 epilogue: h (fp32, skip of layer 2), tanh factor f1 to scratch
-  float h[4][4][4];
+  // ---- layer-1 epilogue: h (fp32, skip of layer 2), tanh factors f1 (fp32)
+  float h[4][4][4], f1[4][4][4], f2[4][4][4], f3[4][4][4];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     const int j0 = 16 * (4 * w + mt) + 4 * g;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      float f[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const float tt = fast_tanh(acc[mt][s][v] + cst[0 * kH + j0 + v]);
         const float idt = cst[1 * kH + j0 + v];
         h[mt][s][v] = tt * idt;
-        f[v] = idt * (1.f - tt * tt);
+        f1[mt][s][v] = idt * (1.f - tt * tt);
       }
-      put_f(scr, 0, mt, s, lane, f);
       lds_put(s_h, w, mt, s, n, g, h[mt][s]);
     }
   }
@@ -264,30 +239,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       const int j0 = 16 * (4 * w + mt) + 4 * g;
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        float f[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const float tt = fast_tanh(acc[mt][s][v] + bl[j0 + v]);
           const float idt = il[j0 + v];
           h[mt][s][v] += tt * idt;
-          f[v] = idt * (1.f - tt * tt);
+          (layer == 0 ? f2 : f3)[mt][s][v] = idt * (1.f - tt * tt);
         }
-        if (layer == 0) {
-          put_f(scr, 1, mt, s, lane, f);
-          lds_put(s_h, w, mt, s, n, g, h[mt][s]);
-        } else {
-          // dz3 = w4 * f3 (dE/dh3 = w4)
-          float dz[4];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) dz[v] = cst[6 * kH + j0 + v] * f[v];
-          put_f(scr, 2, mt, s, lane, dz);
-        }
+        if (layer == 0) lds_put(s_h, w, mt, s, n, g, h[mt][s]);
       }
     }
     if (layer == 0) __syncthreads();
   }
 
-  // ---- output layer and atomic energy; dz3 into the LDS buffer
+  // ---- output layer and atomic energy; dz3 = w4 * f3 into the LDS buffer
   const float* w4 = cst + 6 * kH;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
@@ -295,12 +260,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
       const int j0 = 16 * (4 * w + mt) + 4 * g;
-      const f16x4 q = scr[(2 * 16 + mt * 4 + s) * 64 + lane];
       float dz[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         e += h[mt][s][v] * w4[j0 + v];
-        dz[v] = (float)q[v];
+        dz[v] = w4[j0 + v] * f3[mt][s][v];
       }
       lds_put(s_h, w, mt, s, n, g, dz);
     }
@@ -325,7 +289,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       const int j0 = 16 * (4 * w + mt) + 4 * g;
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        const f16x4 q = scr[((1 - layer) * 16 + mt * 4 + s) * 64 + lane];
         float dz[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -333,10 +296,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
           if (layer == 0) {
             dh = acc[mt][s][v] + w4[j0 + v];
             h[mt][s][v] = dh;  // h now holds dh2
+            dz[v] = dh * f2[mt][s][v];
           } else {
             dh = acc[mt][s][v] + h[mt][s][v];
+            dz[v] = dh * f1[mt][s][v];
           }
-          dz[v] = dh * (float)q[v];
         }
         lds_put(s_h, w, mt, s, n, g, dz);
       }
@@ -389,25 +353,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 
 using namespace dpmi;
 
-static int64_t fit_grid(int64_t nloc, int ntypes) { return ceil_div(nloc, kTile) + ntypes; }
-
-extern "C" int dpmi_se_a_fitting_workspace_bytes(int64_t* bytes, int64_t nloc, int ntypes) {
-  DPMI_CHECK_ARG(bytes != nullptr && nloc >= 0 && ntypes > 0 && ntypes <= kMaxTypes, "bad arguments");
-  *bytes = fit_grid(nloc, ntypes) * kScratchPerTile * (int64_t)sizeof(f16x4);
-  return DPMI_OK;
-}
-
-extern "C" int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const float* desc, const int* perm,
+extern "C" int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const void* desc, const int* perm,
                                       const int* type_start, const void* frags, const float* consts,
-                                      const double* ebias, int64_t nloc, int ntypes, int k1, void* workspace,
-                                      int64_t workspace_bytes, void* stream) {
+                                      const double* ebias, int64_t nloc, int ntypes, int k1, void* stream) {
   DPMI_CHECK_ARG(ntypes > 0 && ntypes <= kMaxTypes, "ntypes out of range [1, 16]");
   DPMI_CHECK_ARG(k1 > 0 && k1 % 16 == 0, "descriptor width must be a positive multiple of 16");
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
   if (nloc == 0) return DPMI_OK;
-  DPMI_CHECK_ARG(workspace != nullptr &&
-                     workspace_bytes >= fit_grid(nloc, ntypes) * kScratchPerTile * (int64_t)sizeof(f16x4),
-                 "workspace too small (dpmi_se_a_fitting_workspace_bytes)");
   FitParams p;
   std::memset(&p, 0, sizeof(p));
   p.ntypes = ntypes;
@@ -423,9 +375,10 @@ extern "C" int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const f
   p.w1b = p.w2b + 16 * 8 * 64;
   p.type_frags = p.w1b + (int64_t)p.nm1 * 8 * 64;
   // one workgroup per 64-atom tile of each type (upper bound; extra ones exit)
-  const int64_t grid = fit_grid(nloc, ntypes);
-  hipLaunchKernelGGL(k_fit, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), atom_energy, ddesc, desc, perm,
-                     type_start, static_cast<const bf16x8*>(frags), consts, ebias, static_cast<f16x4*>(workspace), p);
+  const int64_t grid = ceil_div(nloc, kTile) + ntypes;
+  hipLaunchKernelGGL(k_fit, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), atom_energy, ddesc,
+                     static_cast<const __bf16*>(desc), perm,
+                     type_start, static_cast<const bf16x8*>(frags), consts, ebias, p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }

@@ -32,12 +32,11 @@ SIGNATURES = {
     "dpmi_prod_virial_se_a_f32": [_P, _P, _P, _P, _P, _P, _L, _L, _L, _L, _P, _L, _P],
     "dpmi_force_virial_recompute_f64": [_P, _P, _P, _P, _P, _P, _P, _L, _L, _P, _F, _F, _P, _I, _P, _L, _P],
     "dpmi_force_virial_recompute_f32": [_P, _P, _P, _P, _P, _P, _P, _L, _L, _P, _F, _F, _P, _I, _P, _L, _P],
-    "dpmi_se_a_embed_fwd_bf16": [_P, _P, _P, _P, _L, _P, _I, _P],
-    "dpmi_se_a_descriptor_fwd": [_P, _P, _P, _L, _I, _I, _I, _I, _P],
+    "dpmi_se_a_embed_fwd_bf16": [_P, _P, _P, _P, _L, _P, _I, _I, _P],
+    "dpmi_se_a_descriptor_fwd": [_P, _P, _P, _L, _I, _I, _I, _I, _I, _P],
     "dpmi_se_a_descriptor_bwd": [_P, _P, _P, _L, _I, _I, _I, _P],
     "dpmi_se_a_embed_bwd_bf16": [_P, _P, _P, _P, _P, _L, _P, _I, _P],
-    "dpmi_se_a_fitting_bf16": [_P, _P, _P, _P, _P, _P, _P, _P, _L, _I, _I, _P, _L, _P],
-    "dpmi_se_a_fitting_workspace_bytes": [_P, _L, _I],
+    "dpmi_se_a_fitting_bf16": [_P, _P, _P, _P, _P, _P, _P, _P, _L, _I, _I, _P],
     "dpmi_se_a_fitting_frag_count": [_P, _I],
     "dpmi_tabulate_fusion_se_a_f64": [_P, _P, _P, _P, _P, _L, _L, _L, _I, _P],
     "dpmi_tabulate_fusion_se_a_f32": [_P, _P, _P, _P, _P, _L, _L, _L, _I, _P],

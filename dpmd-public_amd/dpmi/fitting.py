@@ -18,12 +18,10 @@ where a layer has no timestep), and ebias (fp64) = bias_atom_e + out_bias.
 
 from __future__ import annotations
 
-import ctypes
-
 import numpy as np
 import torch
 
-from . import _lib, ops
+from . import ops
 from ._lib import ptr
 
 KH = 256
@@ -121,18 +119,16 @@ class FusedFitting:
         return perm, start
 
     def __call__(self, desc: torch.Tensor, atype: torch.Tensor, order=None):
-        """desc [nloc, k1] fp32 -> (atom_energy float64 [nloc], d E / d desc fp32 [nloc, k1])."""
+        """desc [nloc, k1] bf16 -> (atom_energy float64 [nloc], d E / d desc fp32 [nloc, k1])."""
         nloc = desc.shape[0]
         if desc.shape[1] != self.k1:
             raise ValueError(f"descriptor width {desc.shape[1]} != fitting input {self.k1}")
+        if desc.dtype != torch.bfloat16:
+            raise TypeError("the fused fitting kernel takes the descriptor in bfloat16")
         perm, start = self.sort_by_type(atype) if order is None else order
         desc = desc.contiguous()
         ae = torch.empty(nloc, dtype=torch.float64, device=desc.device)
-        dd = torch.empty_like(desc)
-        wsb = ctypes.c_int64(0)
-        _lib.call("dpmi_se_a_fitting_workspace_bytes", ctypes.byref(wsb), nloc, self.ntypes)
-        ws = torch.empty(max(int(wsb.value), 1), dtype=torch.uint8, device=desc.device)
+        dd = torch.empty((nloc, self.k1), dtype=torch.float32, device=desc.device)
         ops._launch("dpmi_se_a_fitting_bf16", ptr(ae), ptr(dd), ptr(desc), ptr(perm), ptr(start), ptr(self.frags),
-                    ptr(self.consts), ptr(self.ebias), nloc, self.ntypes, self.k1, ptr(ws), int(wsb.value),
-                    ops._stream())
+                    ptr(self.consts), ptr(self.ebias), nloc, self.ntypes, self.k1, ops._stream())
         return ae, dd

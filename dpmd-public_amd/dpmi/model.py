@@ -225,7 +225,7 @@ class SeAEnergyModel:
         descriptor and embedding adjoints recomputing the forward, env-mat
         adjoint for forces and virial."""
         em = self.env_mat(nlist, ext_c, ext_t)
-        gr, desc = self.fused.forward(em)
+        gr, desc = self.fused.forward(em, torch.bfloat16 if self.fused_fit is not None else torch.float32)
         if self.fused_fit is not None:
             ae, ddesc = self.fused_fit(desc, atype)
             energy = ae.sum()

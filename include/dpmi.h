@@ -188,14 +188,16 @@ int dpmi_tabulate_fusion_se_a_grad_grad_f32(float* dz_dy, const float* table, co
  * [ntypes, 704] fp32, both laid out by dpmi.embed.pack_embedding (internal
  * feature order q3; see that module).  em [nloc, nnei, 4] fp32.
  * ------------------------------------------------------------------------ */
-/* gr_part [ntypes, nloc, 4, 128]: sum over each type's neighbors of em^T G. */
+/* gr_part [ntypes, nloc, 4, 4 n1]: sum over each type's neighbors of em^T G
+ * (original feature order). */
 int dpmi_se_a_embed_fwd_bf16(float* gr_part, const float* em, const void* frags, const float* consts, int64_t nloc,
-                             const int* sel_host, int ntypes, void* stream);
-/* gr [nloc, 4, 128] = sum_t gr_part / nnei (internal order); desc
- * [nloc, 4 n1 * axis] = GR^T GR[:, :axis] in the original order (se_a.py:851-858). */
-int dpmi_se_a_descriptor_fwd(float* gr, float* desc, const float* gr_part, int64_t nloc, int ntypes, int nnei,
-                             int n1, int axis, void* stream);
-/* dgr [nloc, 4, 128] = dE/d(sum_t gr_part) from ddesc = dE/d desc. */
+                             const int* sel_host, int ntypes, int n1, void* stream);
+/* gr [nloc, 4, 4 n1] = sum_t gr_part / nnei; desc [nloc, 4 n1 * axis] =
+ * GR^T GR[:, :axis] (se_a.py:851-858), fp32 or (desc_bf16 != 0) bf16. */
+int dpmi_se_a_descriptor_fwd(float* gr, void* desc, const float* gr_part, int64_t nloc, int ntypes, int nnei,
+                             int n1, int axis, int desc_bf16, void* stream);
+/* dgr [nloc, 4, 128] = dE/d(sum_t gr_part) from ddesc = dE/d desc, in the
+ * embedding kernels' internal feature order (gr as written above). */
 int dpmi_se_a_descriptor_bwd(float* dgr, const float* ddesc, const float* gr, int64_t nloc, int nnei, int n1,
                              int axis, void* stream);
 /* net_deriv [nloc, nnei, 4] = dE/d em, recomputing the embedding forward. */
@@ -208,18 +210,16 @@ int dpmi_se_a_embed_bwd_bf16(float* net_deriv, const float* em, const float* dgr
  * (deepmd/pt/model/task/fitting.py:734-900; FittingNet layers of
  * deepmd/dpmodel/utils/network.py:1031-1090): per type, three hidden layers
  * of width H <= 256 (the second and third with the skip connection) and a
- * linear H -> 1 output.  frags / consts / ebias are laid out by
+ * linear H -> 1 output.  desc_bf16 [nloc, k1] is the descriptor in bf16 (as
+ * dpmi_se_a_descriptor_fwd writes it with desc_bf16 = 1).  frags / consts / ebias are laid out by
  * dpmi.fitting.pack_fitting; perm [nloc] lists the local atoms ordered by
  * type and type_start [ntypes + 1] (device) delimits each type's run.
  * ------------------------------------------------------------------------ */
 /* atom_energy [nloc] fp64 = network output + bias_atom_e + out_bias;
  * ddesc [nloc, k1] fp32 = dE/d desc for E = sum atom_energy. */
-int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const float* desc, const int* perm,
+int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const void* desc_bf16, const int* perm,
                            const int* type_start, const void* frags, const float* consts, const double* ebias,
-                           int64_t nloc, int ntypes, int k1, void* workspace, int64_t workspace_bytes,
-                           void* stream);
-/* Workspace of dpmi_se_a_fitting_bf16 (per-tile activation derivatives). */
-int dpmi_se_a_fitting_workspace_bytes(int64_t* bytes, int64_t nloc, int ntypes);
+                           int64_t nloc, int ntypes, int k1, void* stream);
 /* Number of bf16x8 lanes (64 per fragment) of one type's packed fitting net. */
 int dpmi_se_a_fitting_frag_count(int64_t* count, int k1);
 
