@@ -222,11 +222,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     }
     if (g == 0) {
       // GR_t^T rows in the original feature order: gr_part[t][i][c][m]
-      const int M = 4 * p.n1;
+      // (n1 passes through an opaque copy so the index arithmetic below is
+      // not hoisted out of the atom loop, where registers are scarce)
+      int n1 = p.n1;
+      asm volatile("" : "+s"(n1));
+      const int M = 4 * n1;
       float* out = gr_part + ((int64_t)t * nloc + i) * 4 * M;
 #pragma unroll
       for (int nt = 0; nt < 8; ++nt) {
-        const int m = orig_of_q3(16 * nt + r, p.n1);
+        const int m = orig_of_q3(16 * nt + r, n1);
         if (m >= 0)
 #pragma unroll
           for (int v = 0; v < 4; ++v) out[v * M + m] = acc[nt][v];
@@ -344,15 +348,16 @@ __global__ void __launch_bounds__(256)
     sdd[m * kS + a + 2] = x.z;
     sdd[m * kS + a + 3] = x.w;
   }
-  for (int k = lane; k < 4 * kQ3; k += 64) {
-    if (k < 4 * M) sg[k] = gri[k];
-    so[k] = 0.f;
-  }
+  // GR rows staged at the fixed stride kQ3 (constant LDS offsets per c)
+  for (int k = lane; k < 4 * kQ3; k += 64) so[k] = 0.f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    for (int m = lane; m < M; m += 64) sg[c * kQ3 + m] = gri[c * M + m];
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   if (lane < AXIS * 4) {
     const int a = lane >> 2, c = lane & 3;
-    sga[4 * a + c] = sg[c * M + a];
+    sga[4 * a + c] = sg[c * kQ3 + a];
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -378,7 +383,7 @@ __global__ void __launch_bounds__(256)
   for (int m = m0; m < M; m += kGroups) {
     const float x = sdd[m * kS + a];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) t2[c] += x * sg[c * M + m];
+    for (int c = 0; c < 4; ++c) t2[c] += x * sg[c * kQ3 + m];
   }
 #pragma unroll
   for (int off = AXIS; off < 64; off <<= 1)
