@@ -53,6 +53,25 @@ def _bytes_per_atom(name: str, nnei: int) -> float:
 
 
 HBM_PEAK_GBS = 8000.0
+# entry point -> kernel it launches (for the measured-traffic table)
+KERNEL_OF = {"dpmi_se_a_embed_bwd_bf16": "k_embed_bwd", "dpmi_se_a_embed_fwd_bf16": "k_embed_fwd",
+             "dpmi_se_a_fitting_bf16": "k_fit", "dpmi_force_virial_recompute_f32": "k_force_virial_recompute<float, double>"}
+
+
+def _measured_traffic(workload: str, entry: str):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the
+    kernel behind `entry`, from the committed rocprofv3 counter summary of
+    this workload, or None."""
+    f = ROOT / "profiles" / f"hbm_traffic_{workload}_bf16.txt"
+    k = KERNEL_OF.get(entry)
+    if not f.exists() or k is None:
+        return None, None
+    for line in f.read_text().splitlines():
+        # table rows: kernel name, read MB, write MB, algorithmic read/write MB, ratio
+        if line.startswith(k + " "):
+            cols = line[len(k):].split()
+            return int((float(cols[0]) + float(cols[1])) * 1e6), str(f.relative_to(ROOT))
+    return None, None
 MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "fp64": 78.6}
 
 
@@ -94,11 +113,15 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true",
                     help="skip the FP64 leg (profiling runs of the low-precision step only)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="calibration for HBM counters: only run 5 device-to-device copies of this many bytes")
     args = ap.parse_args()
+    if args.copy_bytes:
+        return copy_calibration(args.copy_bytes)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X, so --gpus "
-              f"{max(args.gpus, world)} is not run (the multi-GPU sharding design is described in DESIGN.md)",
+              f"{max(args.gpus, world)} is not run",
               file=sys.stderr)
         return 2
 
@@ -160,17 +183,20 @@ def main() -> int:
     dom = max(klo, key=lambda n: klo[n]["total_ms"])
     avg_s = klo[dom]["avg_ms"] / 1e3
     flops = _mfma_flops(dom, spec, nloc)
+    traffic, traffic_src = _measured_traffic(args.workload, dom)
     if flops > 0:
         achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
         peak = MFMA_PEAK_TFLOPS[args.precision]
         roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 2), "peak": peak,
-                    "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                    "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
+                    "traffic_source": traffic_src,
                     "algorithmic_flops_per_launch": flops, "avg_launch_ms": round(klo[dom]["avg_ms"], 4)}
     else:
         algo = _bytes_per_atom(dom, nnei) * nloc
         achieved = algo / avg_s / 1e9 if avg_s > 0 else 0.0
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "traffic_source": traffic_src,
                     "algorithmic_bytes_per_launch": int(algo), "avg_launch_ms": round(klo[dom]["avg_ms"], 4)}
     # the HBM-bound kernels of the same step, for reference
     line_hbm = {}
@@ -206,6 +232,24 @@ def main() -> int:
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(spec, params, args.cpu_seconds)
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def copy_calibration(nbytes: int) -> int:
+    """5 plain device-to-device copies of nbytes (reads nbytes, writes
+    nbytes each): the reference for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 5
+    print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": round(dt * 1e3, 4),
+                      "GB_per_s_read_plus_write": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
     return 0
 
 
