@@ -45,7 +45,7 @@ WORKLOADS = {
 # counted once per atom (2 * 3 * 8).
 def _bytes_per_atom(name: str, nnei: int) -> float:
     elem = 8 if name.endswith("f64") else 4
-    if name.startswith("dpmi_force_virial_recompute") or name.startswith("dpmi_prod_env_mat_a"):
+    if name.startswith("dpmi_force_virial") or name.startswith("dpmi_prod_env_mat_a"):
         return 4 * nnei + 4 * nnei * elem + 48
     if name.startswith("dpmi_build_nlist"):
         return 4 * nnei + 4 * nnei + 24 + 4  # nlist + mapped/overflow, own coordinate
@@ -55,7 +55,7 @@ def _bytes_per_atom(name: str, nnei: int) -> float:
 HBM_PEAK_GBS = 8000.0
 # entry point -> kernel it launches (for the measured-traffic table)
 KERNEL_OF = {"dpmi_se_a_embed_bwd_bf16": "k_embed_bwd", "dpmi_se_a_embed_fwd_bf16": "k_embed_fwd",
-             "dpmi_se_a_fitting_bf16": "k_fit", "dpmi_force_virial_recompute_f32": "k_force_virial_recompute<float, double>"}
+             "dpmi_se_a_fitting_bf16": "k_fit", "dpmi_force_virial_cells_f32": "k_force_cells<float, 1536, false>"}
 
 
 def _measured_traffic(workload: str, entry: str):
@@ -156,9 +156,9 @@ def main() -> int:
         return (t1 - t0) / args.steps, out, kt.summary()
 
     names64 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f64",
-               "dpmi_force_virial_recompute_f64"]
+               "dpmi_force_virial_cells_f64"]
     names32 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f32c64",
-               "dpmi_force_virial_recompute_f32", "dpmi_se_a_embed_fwd_bf16", "dpmi_se_a_descriptor_fwd",
+               "dpmi_force_virial_cells_f32", "dpmi_se_a_embed_fwd_bf16", "dpmi_se_a_descriptor_fwd",
                "dpmi_se_a_descriptor_bwd", "dpmi_se_a_embed_bwd_bf16", "dpmi_se_a_fitting_bf16"]
     t64, out64, k64 = None, None, {}
     if not args.no_fp64:

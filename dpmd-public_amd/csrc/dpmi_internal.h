@@ -14,6 +14,10 @@ namespace dpmi {
 // Thread-local last error text, read through dpmi_last_error().
 void set_error(const std::string& msg);
 
+// Cell grid of the neighbor search for this box (nlist.hip): number of local
+// cells the workspace is sized for and the search radius in cells.
+int cell_grid_info(const double* box_host, int64_t nloc, float rcut, int64_t* nlc, int* nsearch);
+
 #define DPMI_CHECK_ARG(cond, msg)       \
   do {                                  \
     if (!(cond)) {                      \

@@ -12,6 +12,7 @@
 
 #include <cstring>
 
+#include "cells.h"
 #include "dpmi_internal.h"
 
 namespace dpmi {
@@ -383,6 +384,168 @@ __global__ void __launch_bounds__(256)
   block_virial_partial<FP>(v, partial);
 }
 
+// Cell-wise form of the recompute adjoint, without memory atomics for the
+// neighbor forces.  Pass 1: one workgroup per local cell (grid-stride); the
+// cell's neighbor terms g(i, s) accumulate in LDS at the neighbor's position in
+// the cell's candidate table (cand_slot from the neighbor search), then each
+// table entry is written once to partial[target][neighbor-cell index].  The
+// centre terms and the virial are as in k_force_virial_recompute.  Atoms or
+// cells the search left to its per-atom path (cand_slot < 0, or a table
+// over the table capacity) add their neighbor terms with atomics instead.
+// Each wavefront accumulates into its own LDS copy of the table (a wave's
+// lanes never share a slot, so plain adds in program order suffice) and the
+// copies are summed in a fixed order: the result does not depend on timing.
+// The first pass takes every cell with tables up to kCandCap; cells the
+// search moved to its second pass (bigger tables) are left to a second force
+// pass (kBigPass, fp32: 4 x 3072 x 12 B of LDS) or, in FP64 (4 x 1536 x 24 B
+// already), processed with neighbor atomics.
+template <typename FP, int CAP, bool kBigPass>
+__global__ void __launch_bounds__(256)
+    k_force_cells(double* __restrict__ force, FP* __restrict__ partial, int* __restrict__ cell_ok,
+                  double* __restrict__ vpartial, const FP* __restrict__ nd, const int* __restrict__ nlist,
+                  const int16_t* __restrict__ cand_slot, const int* __restrict__ mapping,
+                  const double* __restrict__ ext_c, const int* __restrict__ ext_t, const FP* __restrict__ std_,
+                  const char* __restrict__ nlws, float rmin, float rmax, SecParams sp) {
+  __shared__ int s_base[kMaxNb], s_local[kMaxNb], s_pref[kMaxNb + 1];
+  __shared__ int lim_all[kAtomsPerBlock][kMaxTypes];
+  extern __shared__ unsigned char smem_raw[];
+  constexpr int kCap = CAP;
+  constexpr bool kLeaveBig = !kBigPass && sizeof(FP) == 4;  // a second pass takes the big cells
+  FP* acc = reinterpret_cast<FP*>(smem_raw);  // [4 wavefronts][kCap][3]
+  const GridParams p = *reinterpret_cast<const GridParams*>(nlws);
+  const WsLayout L = ws_layout(p.nloc, p.lay_nlc, p.lay_next);
+  const int* __restrict__ cell_start = reinterpret_cast<const int*>(nlws + L.cell_start);
+  const int* __restrict__ cell_atoms = reinterpret_cast<const int*>(nlws + L.cell_atoms);
+  const int* __restrict__ ghost_off = reinterpret_cast<const int*>(nlws + L.ghost_off);
+  const int* __restrict__ misc = reinterpret_cast<const int*>(nlws + L.misc);
+  const int* __restrict__ bigcells = reinterpret_cast<const int*>(nlws + L.bigcells);
+  const int tid = threadIdx.x, w = tid / kWave, lane = lane_id();
+  const int side = 2 * p.nsearch + 1, nnb = side * side * side;
+  const int ncells = kBigPass ? misc[3] : p.ncell[0] * p.ncell[1] * p.ncell[2];
+  const int nnei = sp.nnei;
+  double v[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) v[k] = 0.;
+  for (int ci = blockIdx.x; ci < ncells; ci += gridDim.x) {
+    const int cc = kBigPass ? bigcells[ci] : ci;
+    const int a0 = cell_start[cc], na = cell_start[cc + 1] - a0;
+    if (na == 0) {  // uniform in the workgroup
+      if (tid == 0) cell_ok[cc] = 0;
+      continue;
+    }
+    const int T = nb_table(p, cell_start, ghost_off, cc, s_base, s_local, s_pref);
+    const bool ok = T <= kCap;
+    if (kLeaveBig && !ok) continue;  // uniform; the second pass writes cell_ok
+    if (tid == 0) cell_ok[cc] = ok ? 1 : 0;
+    if (ok)
+      for (int k = tid; k < kAtomsPerBlock * 3 * kCap; k += blockDim.x) acc[k] = (FP)0.;
+    __syncthreads();
+    FP* accw = acc + w * 3 * kCap;
+    for (int ai = w; ai < na; ai += kAtomsPerBlock) {
+      const int i = cell_atoms[a0 + ai];
+      const int* nl = nlist + (int64_t)i * nnei;
+      section_limits_wave(nl, sp, lim_all[w]);
+      const int ti = ext_t[i] < sp.ntypes ? ext_t[i] : -1;
+      const double xi0 = ext_c[3 * i + 0], xi1 = ext_c[3 * i + 1], xi2 = ext_c[3 * i + 2];
+      double ci[3] = {0., 0., 0.};
+      for (int s = lane; s < nnei && ti >= 0; s += kWave) {
+        const int t = slot_type(sp, s);
+        if (s >= lim_all[w][t]) continue;
+        const int j = nl[s];
+        const double r64[3] = {ext_c[3 * j + 0] - xi0, ext_c[3 * j + 1] - xi1, ext_c[3 * j + 2] - xi2};
+        const FP rr[3] = {(FP)r64[0], (FP)r64[1], (FP)r64[2]};
+        FP val[4], d[12];
+        env_one(rr, rmin, rmax, val, d);
+        const FP* sd = std_ + ((int64_t)ti * nnei + s) * 4;
+        const FP* n = nd + ((int64_t)i * nnei + s) * 4;
+        FP g[3] = {(FP)0., (FP)0., (FP)0.};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const FP nc = n[c] / sd[c];
+          g[0] += nc * d[3 * c + 0];
+          g[1] += nc * d[3 * c + 1];
+          g[2] += nc * d[3 * c + 2];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) ci[k] += (double)g[k];
+#pragma unroll
+        for (int d0 = 0; d0 < 3; ++d0)
+#pragma unroll
+          for (int d1 = 0; d1 < 3; ++d1) v[d0 * 3 + d1] += (double)g[d0] * r64[d1];
+        const int cs = ok ? (int)cand_slot[(int64_t)i * nnei + s] : -1;
+        if (cs >= 0 && cs < T) {
+          accw[3 * cs + 0] += g[0];
+          accw[3 * cs + 1] += g[1];
+          accw[3 * cs + 2] += g[2];
+        } else {
+          const int o = mapping ? mapping[j] : j;
+          atomicAdd(&force[3 * (int64_t)o + 0], (double)g[0]);
+          atomicAdd(&force[3 * (int64_t)o + 1], (double)g[1]);
+          atomicAdd(&force[3 * (int64_t)o + 2], (double)g[2]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) ci[k] = wave_sum(ci[k]);
+      if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) atomicAdd(&force[3 * (int64_t)i + k], -ci[k]);
+    }
+    __syncthreads();
+    if (ok)
+      for (int g = tid; g < T; g += blockDim.x) {
+        int t;
+        const int j = nb_candidate(g, nnb, s_base, s_local, s_pref, cell_atoms, &t);
+        const int o = mapping ? mapping[j] : j;
+        FP* dst = partial + ((int64_t)o * nnb + t) * 3;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          FP x = acc[3 * g + k];
+#pragma unroll
+          for (int ww = 1; ww < kAtomsPerBlock; ++ww) x += acc[ww * 3 * kCap + 3 * g + k];
+          dst[k] = x;
+        }
+      }
+    __syncthreads();
+  }
+  block_virial_partial<FP>(v, vpartial);
+}
+
+// Pass 2: thread per local atom; the neighbor-cell index t of a cell's table
+// holding this atom's image identifies that cell (c - offset(t), wrapped), so
+// the atom sums its partial[o][t] over t in a fixed order.
+template <typename FP>
+__global__ void __launch_bounds__(256)
+    k_force_gather(double* __restrict__ force, const FP* __restrict__ partial, const int* __restrict__ cell_ok,
+                   const char* __restrict__ nlws) {
+  const GridParams p = *reinterpret_cast<const GridParams*>(nlws);
+  const WsLayout L = ws_layout(p.nloc, p.lay_nlc, p.lay_next);
+  const int* __restrict__ cell_of = reinterpret_cast<const int*>(nlws + L.cell_of);
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= p.nloc) return;
+  const int ns = p.nsearch, side = 2 * ns + 1, nnb = side * side * side;
+  const int c = cell_of[o];
+  const int c2 = c % p.ncell[2], c1 = (c / p.ncell[2]) % p.ncell[1], c0 = c / (p.ncell[1] * p.ncell[2]);
+  double f[3] = {0., 0., 0.};
+  for (int t = 0; t < nnb; ++t) {
+    int q0 = c0 - (t / (side * side) - ns), q1 = c1 - ((t / side) % side - ns), q2 = c2 - (t % side - ns);
+    if (p.pbc) {
+      q0 += pbc_shift(q0, p.ncell[0]) * p.ncell[0];
+      q1 += pbc_shift(q1, p.ncell[1]) * p.ncell[1];
+      q2 += pbc_shift(q2, p.ncell[2]) * p.ncell[2];
+    } else if (q0 < 0 || q0 >= p.ncell[0] || q1 < 0 || q1 >= p.ncell[1] || q2 < 0 || q2 >= p.ncell[2]) {
+      continue;
+    }
+    if (!cell_ok[(q0 * p.ncell[1] + q1) * p.ncell[2] + q2]) continue;
+    const FP* src = partial + ((int64_t)o * nnb + t) * 3;
+    f[0] += (double)src[0];
+    f[1] += (double)src[1];
+    f[2] += (double)src[2];
+  }
+  force[3 * (int64_t)o + 0] += f[0];
+  force[3 * (int64_t)o + 1] += f[1];
+  force[3 * (int64_t)o + 2] += f[2];
+}
+
 template <typename FP, typename CT>
 int env_mat_impl(FP* em, FP* em_deriv, FP* rij, const int* nlist, const CT* ext_coord, const int* ext_type,
                  int64_t nloc, const FP* avg, const FP* std_, float rcut, float rcut_smth, const int* sel_host,
@@ -546,4 +709,107 @@ extern "C" int dpmi_force_virial_recompute_f32(double* force, double* virial, co
   return force_virial_recompute_impl<float>(force, virial, net_deriv, nlist, mapping, ext_coord, ext_type, nloc,
                                             nall, std_, rcut, rcut_smth, sel_host, ntypes, workspace,
                                             workspace_bytes, stream);
+}
+
+namespace dpmi {
+namespace {
+
+constexpr int kBigBlocks = 256;  // workgroups of the second (big-cell) force pass
+
+int64_t force_cells_ws(const double* box_host, int64_t nloc, float rcut, int64_t elem, int64_t* nnb_out) {
+  int64_t nlc;
+  int ns;
+  if (cell_grid_info(box_host, nloc, rcut, &nlc, &ns) != DPMI_OK) return -1;
+  const int64_t side = 2 * ns + 1, nnb = side * side * side;
+  if (nnb_out) *nnb_out = nnb;
+  const int64_t na = std::max<int64_t>(nloc, 1);
+  return align256(na * nnb * 3 * elem) + align256(4 * nlc) + align256((kForceBlocks + kBigBlocks) * 9 * 8);
+}
+
+template <typename FP>
+int force_cells_impl(double* force, double* virial, const FP* net_deriv, const int* nlist, const int16_t* cand_slot,
+                     const int* mapping, const double* ext_coord, const int* ext_type, int64_t nloc, int64_t nall,
+                     const FP* std_, float rcut, float rcut_smth, const int* sel_host, int ntypes,
+                     const double* box_host, const void* nlist_workspace, void* workspace, int64_t workspace_bytes,
+                     void* stream) {
+  SecParams sp;
+  int rc = make_sec(&sp, sel_host, ntypes);
+  if (rc) return rc;
+  DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31) && nall >= nloc, "nloc/nall out of range");
+  DPMI_CHECK_ARG(mapping != nullptr || nall == nloc, "ghost atoms need mapping");
+  DPMI_CHECK_ARG(cand_slot != nullptr && nlist_workspace != nullptr, "cand_slot and the neighbor-list workspace are required");
+  int64_t nnb = 0;
+  const int64_t need = force_cells_ws(box_host, nloc, rcut, sizeof(FP), &nnb);
+  DPMI_CHECK_ARG(need > 0 && workspace != nullptr && workspace_bytes >= need,
+                 "workspace too small (dpmi_force_virial_cells_workspace_bytes)");
+  hipStream_t st = as_stream(stream);
+  if (nloc > 0) hipLaunchKernelGGL(k_fill<double>, dim3(ceil_div(nloc * 3, 256)), dim3(256), 0, st, force, nloc * 3, 0.);
+  char* ws = static_cast<char*>(workspace);
+  FP* partial = reinterpret_cast<FP*>(ws);
+  int* cell_ok = reinterpret_cast<int*>(ws + align256(std::max<int64_t>(nloc, 1) * nnb * 3 * sizeof(FP)));
+  int64_t nlc;
+  int ns;
+  cell_grid_info(box_host, nloc, rcut, &nlc, &ns);
+  double* vpartial = reinterpret_cast<double*>(reinterpret_cast<char*>(cell_ok) + align256(4 * nlc));
+  const int bpf = (int)std::min<int64_t>(kForceBlocks, std::max<int64_t>(1, nloc));
+  const size_t shm = (size_t)kAtomsPerBlock * kCandCap * 3 * sizeof(FP);
+  const size_t shm_big = (size_t)kAtomsPerBlock * kCandCapBig * 3 * sizeof(FP);
+  if (shm > 65536)
+    DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_force_cells<FP, kCandCap, false>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  if (nloc > 0) {
+    hipLaunchKernelGGL((k_force_cells<FP, kCandCap, false>), dim3(bpf), dim3(256), shm, st, force, partial, cell_ok,
+                       vpartial, net_deriv, nlist, cand_slot, mapping, ext_coord, ext_type, std_,
+                       static_cast<const char*>(nlist_workspace), rcut_smth, rcut, sp);
+    if constexpr (sizeof(FP) == 4) {
+      // the search's big cells (second table size); its virial partials follow the first pass's
+      DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_force_cells<FP, kCandCapBig, true>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_big));
+      hipLaunchKernelGGL((k_force_cells<FP, kCandCapBig, true>), dim3(kBigBlocks), dim3(256), shm_big, st, force,
+                         partial, cell_ok, vpartial + 9 * bpf, net_deriv, nlist, cand_slot, mapping, ext_coord,
+                         ext_type, std_, static_cast<const char*>(nlist_workspace), rcut_smth, rcut, sp);
+    }
+    hipLaunchKernelGGL((k_force_gather<FP>), dim3(ceil_div(nloc, 256)), dim3(256), 0, st, force, partial, cell_ok,
+                       static_cast<const char*>(nlist_workspace));
+  } else {
+    hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(64), 0, st, vpartial, (int64_t)9, 0.);
+  }
+  const int nparts = nloc > 0 ? bpf + (sizeof(FP) == 4 ? kBigBlocks : 0) : 1;
+  if (virial) hipLaunchKernelGGL(k_virial_reduce<double>, dim3(1), dim3(576), 0, st, virial, vpartial, nparts);
+  DPMI_LAUNCH_CHECK();
+  return DPMI_OK;
+}
+
+}  // namespace
+}  // namespace dpmi
+
+extern "C" int dpmi_force_virial_cells_workspace_bytes(int64_t* bytes, const double* box_host, int64_t nloc,
+                                                       float rcut, int fp64) {
+  DPMI_CHECK_ARG(bytes != nullptr && nloc >= 0, "bad arguments");
+  const int64_t b = force_cells_ws(box_host, nloc, rcut, fp64 ? 8 : 4, nullptr);
+  DPMI_CHECK_ARG(b > 0, "degenerate box");
+  *bytes = b;
+  return DPMI_OK;
+}
+
+extern "C" int dpmi_force_virial_cells_f64(double* force, double* virial, const double* net_deriv, const int* nlist,
+                                           const int16_t* cand_slot, const int* mapping, const double* ext_coord,
+                                           const int* ext_type, int64_t nloc, int64_t nall, const double* std_,
+                                           float rcut, float rcut_smth, const int* sel_host, int ntypes,
+                                           const double* box_host, const void* nlist_workspace, void* workspace,
+                                           int64_t workspace_bytes, void* stream) {
+  return force_cells_impl<double>(force, virial, net_deriv, nlist, cand_slot, mapping, ext_coord, ext_type, nloc,
+                                  nall, std_, rcut, rcut_smth, sel_host, ntypes, box_host, nlist_workspace,
+                                  workspace, workspace_bytes, stream);
+}
+
+extern "C" int dpmi_force_virial_cells_f32(double* force, double* virial, const float* net_deriv, const int* nlist,
+                                           const int16_t* cand_slot, const int* mapping, const double* ext_coord,
+                                           const int* ext_type, int64_t nloc, int64_t nall, const float* std_,
+                                           float rcut, float rcut_smth, const int* sel_host, int ntypes,
+                                           const double* box_host, const void* nlist_workspace, void* workspace,
+                                           int64_t workspace_bytes, void* stream) {
+  return force_cells_impl<float>(force, virial, net_deriv, nlist, cand_slot, mapping, ext_coord, ext_type, nloc,
+                                 nall, std_, rcut, rcut_smth, sel_host, ntypes, box_host, nlist_workspace, workspace,
+                                 workspace_bytes, stream);
 }

@@ -28,28 +28,15 @@
 #include <cstring>
 #include <vector>
 
+#include "cells.h"
 #include "dpmi_internal.h"
 
 namespace dpmi {
 namespace {
 
 constexpr int kMaxTypes = 64;
-constexpr int kMaxGridNoPbc = 64;
 
-struct GridParams {
-  double box[9];
-  double rec[9];
-  double lo[3];  // open systems: grid origin
-  double cs[3];  // open systems: cell width
-  int ncell[3];
-  int ngcell[3];
-  int ext[3];
-  int nsearch;
-  int pbc;
-  int nloc;
-  long long lay_nlc;   // cell counts the workspace layout was sized for
-  long long lay_next;
-};
+
 
 struct SelParams {
   int ntypes;
@@ -122,46 +109,6 @@ GridParams pbc_grid(const double* box, float rcut, int nloc) {
   return g;
 }
 
-struct WsLayout {
-  int64_t params, misc, bbox, cell_of, cell_count, cell_start, cell_cursor, cell_unordered, cell_atoms, ghost_off,
-      redo, total;
-};
-
-__host__ __device__ inline int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
-
-// Header (grid parameters, status words, bounding box) at fixed offsets, then
-// the per-atom and per-cell arrays.  Host and device derive the same offsets
-// from (nloc, nlc, next), so kernels need only the workspace base pointer.
-__host__ __device__ inline WsLayout ws_layout(int64_t nloc, int64_t nlc, int64_t next) {
-  WsLayout w;
-  int64_t na = nloc > 0 ? nloc : 1;
-  int64_t o = 0;
-  w.params = o;
-  o = align256(o + (int64_t)sizeof(GridParams));
-  w.misc = o;
-  o = align256(o + 4 * 16);
-  w.bbox = o;
-  o = align256(o + 8 * 8);
-  w.redo = o;  // offset independent of the cell counts (the host reads it)
-  o = align256(o + 4 * na);
-  w.cell_of = o;
-  o = align256(o + 4 * na);
-  w.cell_count = o;
-  o = align256(o + 4 * nlc);
-  w.cell_start = o;
-  o = align256(o + 4 * (nlc + 1));
-  w.cell_cursor = o;
-  o = align256(o + 4 * nlc);
-  w.cell_unordered = o;
-  o = align256(o + 4 * na);
-  w.cell_atoms = o;
-  o = align256(o + 4 * na);
-  w.ghost_off = o;
-  o = align256(o + 4 * (next + 1));
-  w.total = o;
-  return w;
-}
-
 bool grid_sizes(const double* box_host, int64_t nloc, float rcut, int64_t* nlc, int64_t* next) {
   if (box_host) {
     GridParams g = pbc_grid(box_host, rcut, (int)nloc);
@@ -187,18 +134,6 @@ __device__ inline void to_phys(double* rp, const double* b, const double* ri) {
   rp[2] = ri[0] * b[2] + ri[1] * b[5] + ri[2] * b[8];
 }
 
-// compute_pbc_shift (neighbor_list.cc:727-743)
-__device__ inline int pbc_shift(int idx, int n) {
-  int s = 0;
-  if (idx < 0) {
-    s = 1;
-    while (idx + s * n < 0) s++;
-  } else if (idx >= n) {
-    s = -1;
-    while (idx + s * n >= n) s--;
-  }
-  return s;
-}
 
 __global__ void k_fill_int(int* p, int64_t n, int v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -420,7 +355,8 @@ template <int CAP>
 __global__ void __launch_bounds__(64) k_search_format(
     int* __restrict__ nlist, int* __restrict__ overflow, int* __restrict__ nlist_mapped,
     const double* __restrict__ ext_c, const int* __restrict__ ext_t, const int* __restrict__ mapping,
-    char* __restrict__ ws, const int* __restrict__ atoms, SelParams sp, float rcut) {
+    char* __restrict__ ws, const int* __restrict__ atoms, int16_t* __restrict__ cand_slot, SelParams sp,
+    float rcut) {
   extern __shared__ unsigned char smem_raw[];
   uint64_t* keys = reinterpret_cast<uint64_t*>(smem_raw);
   int* row = reinterpret_cast<int*>(keys + CAP);
@@ -526,6 +462,7 @@ __global__ void __launch_bounds__(64) k_search_format(
     int j = row[s];
     nlist[ro + s] = j;
     if (nlist_mapped) nlist_mapped[ro + s] = j >= 0 ? mapping[j] : -1;
+    if (cand_slot) cand_slot[ro + s] = -1;  // not in a cell's candidate table
   }
   for (int off = 32; off > 0; off >>= 1) ovf = max(ovf, __shfl_xor(ovf, off, kWave));
   if (overflow && lane == 0) overflow[i] = ovf;
@@ -538,22 +475,24 @@ __global__ void __launch_bounds__(64) k_search_format(
 // Same candidate tests, keys and ranking as k_search_format; an atom with
 // more than kKeyCap candidates, or a cell whose neighborhood exceeds
 // kCandCap atoms, is appended to the redo list for k_search_format.
-constexpr int kCandCap = 1536;
 constexpr int kKeyCap = 512;  // plus 8 padding slots per wavefront
-constexpr int kMaxNb = 128;  // (2 nsearch + 1)^3 <= 125
+constexpr size_t kMaxCellsLds = 150 * 1024;  // dynamic LDS the cell-wise search may take
 
+template <int CAP, bool kBigPass>
 __global__ void __launch_bounds__(256) k_search_cells(
     int* __restrict__ nlist, int* __restrict__ overflow, int* __restrict__ nlist_mapped,
-    const double* __restrict__ ext_c, const int* __restrict__ ext_t, const int* __restrict__ mapping,
-    char* __restrict__ ws, SelParams sp, float rcut) {
+    int16_t* __restrict__ cand_slot, const double* __restrict__ ext_c, const int* __restrict__ ext_t,
+    const int* __restrict__ mapping, char* __restrict__ ws, SelParams sp, float rcut) {
   extern __shared__ unsigned char smem_raw[];
   double* cx = reinterpret_cast<double*>(smem_raw);
-  double* cy = cx + kCandCap;
-  double* cz = cy + kCandCap;
-  int* cj = reinterpret_cast<int*>(cz + kCandCap);
-  int* ct = cj + kCandCap;
-  uint64_t* keys_all = reinterpret_cast<uint64_t*>(ct + kCandCap);
+  double* cy = cx + CAP;
+  double* cz = cy + CAP;
+  int* cj = reinterpret_cast<int*>(cz + CAP);
+  int* ct = cj + CAP;
+  uint64_t* keys_all = reinterpret_cast<uint64_t*>(ct + CAP);
   int* rows_all = reinterpret_cast<int*>(keys_all + 4 * (kKeyCap + 8));
+  int16_t* kslot_all = reinterpret_cast<int16_t*>(rows_all + 4 * sp.nnei);
+  int16_t* rslot_all = kslot_all + 4 * (kKeyCap + 8);
   __shared__ int s_base[kMaxNb], s_local[kMaxNb], s_pref[kMaxNb + 1];
 
   const GridParams p = *reinterpret_cast<const GridParams*>(ws);
@@ -563,65 +502,34 @@ __global__ void __launch_bounds__(256) k_search_cells(
   const int* __restrict__ ghost_off = reinterpret_cast<const int*>(ws + L.ghost_off);
   int* misc = reinterpret_cast<int*>(ws + L.misc);
   int* redo = reinterpret_cast<int*>(ws + L.redo);
+  int* bigcells = reinterpret_cast<int*>(ws + L.bigcells);
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const double rcut2d = (double)(rcut * rcut);
   const float rcut2f = rcut * rcut;
   const int ns = p.nsearch, side = 2 * ns + 1, nnb = side * side * side;
-  const int nlc = p.ncell[0] * p.ncell[1] * p.ncell[2];
+  const int ncells = kBigPass ? misc[3] : p.ncell[0] * p.ncell[1] * p.ncell[2];
   uint64_t* keys = keys_all + w * (kKeyCap + 8);
   int* row = rows_all + w * sp.nnei;
+  int16_t* kslot = kslot_all + w * (kKeyCap + 8);
+  int16_t* rslot = rslot_all + w * sp.nnei;
 
-  for (int cc = blockIdx.x; cc < nlc; cc += gridDim.x) {
+  for (int ci = blockIdx.x; ci < ncells; ci += gridDim.x) {
+    const int cc = kBigPass ? bigcells[ci] : ci;
     const int a0 = cell_start[cc], na = cell_start[cc + 1] - a0;
     if (na == 0) continue;  // uniform in the workgroup
-    const int c2 = cc % p.ncell[2], c1 = (cc / p.ncell[2]) % p.ncell[1], c0 = cc / (p.ncell[1] * p.ncell[2]);
-    // neighborhood table: atom range of each neighbor cell
-    if (tid < nnb) {
-      const int ii0 = c0 + tid / (side * side) - ns, ii1 = c1 + (tid / side) % side - ns, ii2 = c2 + tid % side - ns;
-      int base = 0, cnt = 0, local = 1;
-      if (p.pbc) {
-        if (!(ii0 < -p.ngcell[0] || ii0 >= p.ncell[0] + p.ngcell[0] || ii1 < -p.ngcell[1] ||
-              ii1 >= p.ncell[1] + p.ngcell[1] || ii2 < -p.ngcell[2] || ii2 >= p.ncell[2] + p.ngcell[2])) {
-          local = ii0 >= 0 && ii0 < p.ncell[0] && ii1 >= 0 && ii1 < p.ncell[1] && ii2 >= 0 && ii2 < p.ncell[2];
-          const int j0 = ii0 + pbc_shift(ii0, p.ncell[0]) * p.ncell[0];
-          const int j1 = ii1 + pbc_shift(ii1, p.ncell[1]) * p.ncell[1];
-          const int j2 = ii2 + pbc_shift(ii2, p.ncell[2]) * p.ncell[2];
-          const int q = (j0 * p.ncell[1] + j1) * p.ncell[2] + j2;
-          base = cell_start[q];
-          cnt = cell_start[q + 1] - base;
-          if (!local) {
-            const int e = ((ii0 + p.ngcell[0]) * p.ext[1] + (ii1 + p.ngcell[1])) * p.ext[2] + (ii2 + p.ngcell[2]);
-            base = p.nloc + ghost_off[e];
-          }
-        }
-      } else if (!(ii0 < 0 || ii0 >= p.ncell[0] || ii1 < 0 || ii1 >= p.ncell[1] || ii2 < 0 || ii2 >= p.ncell[2])) {
-        const int q = (ii0 * p.ncell[1] + ii1) * p.ncell[2] + ii2;
-        base = cell_start[q];
-        cnt = cell_start[q + 1] - base;
+    const int T = nb_table(p, cell_start, ghost_off, cc, s_base, s_local, s_pref);
+    if (T > CAP) {
+      if (!kBigPass) {
+        if (tid == 0) bigcells[atomicAdd(&misc[3], 1)] = cc;  // second pass, larger table
+      } else {
+        for (int k = tid; k < na; k += blockDim.x) redo[atomicAdd(&misc[2], 1)] = cell_atoms[a0 + k];
       }
-      s_base[tid] = base;
-      s_local[tid] = local;
-      s_pref[tid + 1] = cnt;
-    }
-    if (tid == 0) s_pref[0] = 0;
-    __syncthreads();
-    if (tid == 0)
-      for (int t = 0; t < nnb; ++t) s_pref[t + 1] += s_pref[t];
-    __syncthreads();
-    const int T = s_pref[nnb];
-    if (T > kCandCap) {
-      for (int k = tid; k < na; k += blockDim.x) redo[atomicAdd(&misc[2], 1)] = cell_atoms[a0 + k];
       __syncthreads();
       continue;
     }
     for (int g = tid; g < T; g += blockDim.x) {
-      int lo = 0, hi = nnb;  // last t with s_pref[t] <= g
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (s_pref[mid] <= g) lo = mid; else hi = mid;
-      }
-      const int k = g - s_pref[lo];
-      const int j = s_local[lo] ? cell_atoms[s_base[lo] + k] : s_base[lo] + k;
+      int t;
+      const int j = nb_candidate(g, nnb, s_base, s_local, s_pref, cell_atoms, &t);
       cj[g] = j;
       ct[g] = ext_t[j];
       cx[g] = ext_c[3 * j + 0];
@@ -657,23 +565,27 @@ __global__ void __launch_bounds__(256) k_search_cells(
         }
         const uint64_t m = __ballot(pass);
         const int pos = n + mask_prefix(m);
-        if (pass && pos < kKeyCap) keys[pos] = key;
+        if (pass && pos < kKeyCap) {
+          keys[pos] = key;
+          kslot[pos] = (int16_t)g;
+        }
         n += __popcll(m);
       }
       if (n > kKeyCap) {  // wave-uniform: leave this atom to the per-atom kernel
         if (lane == 0) redo[atomicAdd(&misc[2], 1)] = i;
         continue;
       }
-      for (int s = lane; s < sp.nnei; s += kWave) row[s] = -1;
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
+      for (int s = lane; s < sp.nnei; s += kWave) {
+        row[s] = -1;
+        rslot[s] = -1;
+      }
       // rank of each key within its type: two keys per lane per pass over the
       // key list, read 8 at a time (independent LDS reads)
       if (n & 7) {  // pad to a multiple of 8 with keys that rank after everything
         if (lane < 8 - (n & 7)) keys[n + lane] = ~0ull;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
       }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
       const int n8 = (n + 7) & ~7;
       int ovf = -1;
       for (int k0 = 0; k0 < n; k0 += 2 * kWave) {
@@ -694,11 +606,21 @@ __global__ void __launch_bounds__(256) k_search_cells(
         }
         if (ka < n) {
           const int t = (int)(kka >> 56);
-          if (ra < sp.sel[t]) row[sp.sec[t] + ra] = (int)(kka & 0xFFFFFFull); else ovf = max(ovf, t);
+          if (ra < sp.sel[t]) {
+            row[sp.sec[t] + ra] = (int)(kka & 0xFFFFFFull);
+            rslot[sp.sec[t] + ra] = kslot[ka];
+          } else {
+            ovf = max(ovf, t);
+          }
         }
         if (kb < n) {
           const int t = (int)(kkb >> 56);
-          if (rb < sp.sel[t]) row[sp.sec[t] + rb] = (int)(kkb & 0xFFFFFFull); else ovf = max(ovf, t);
+          if (rb < sp.sel[t]) {
+            row[sp.sec[t] + rb] = (int)(kkb & 0xFFFFFFull);
+            rslot[sp.sec[t] + rb] = kslot[kb];
+          } else {
+            ovf = max(ovf, t);
+          }
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -708,6 +630,7 @@ __global__ void __launch_bounds__(256) k_search_cells(
         const int j = row[s];
         nlist[ro + s] = j;
         if (nlist_mapped) nlist_mapped[ro + s] = j >= 0 ? mapping[j] : -1;
+        if (cand_slot) cand_slot[ro + s] = rslot[s];
       }
       for (int off = 32; off > 0; off >>= 1) ovf = max(ovf, __shfl_xor(ovf, off, kWave));
       if (overflow && lane == 0) overflow[i] = ovf;
@@ -719,6 +642,14 @@ __global__ void __launch_bounds__(256) k_search_cells(
 }
 
 }  // namespace
+
+int cell_grid_info(const double* box_host, int64_t nloc, float rcut, int64_t* nlc, int* nsearch) {
+  int64_t next;
+  DPMI_CHECK_ARG(rcut > 0.f, "rcut must be positive");
+  DPMI_CHECK_ARG(grid_sizes(box_host, nloc, rcut, nlc, &next), "degenerate box");
+  *nsearch = box_host ? pbc_grid(box_host, rcut, (int)nloc).nsearch : 1;
+  return DPMI_OK;
+}
 
 int nlist_workspace(int64_t* bytes, const double* box_host, int64_t nloc, float rcut) {
   int64_t nlc, next;
@@ -831,10 +762,10 @@ extern "C" int dpmi_copy_coord_f64(double* out_coord, int* out_type, int* mappin
   return DPMI_OK;
 }
 
-extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped, const double* ext_coord,
-                                    const int* ext_type, const int* mapping, int64_t nloc, int64_t nall, float rcut,
-                                    const int* sel_host, int ntypes, const void* workspace, int64_t workspace_bytes,
-                                    void* stream) {
+extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped, int16_t* cand_slot,
+                                    const double* ext_coord, const int* ext_type, const int* mapping, int64_t nloc,
+                                    int64_t nall, float rcut, const int* sel_host, int ntypes, const void* workspace,
+                                    int64_t workspace_bytes, void* stream) {
   DPMI_CHECK_ARG(ntypes > 0 && ntypes <= kMaxTypes, "ntypes out of range [1, 64]");
   DPMI_CHECK_ARG(sel_host != nullptr, "sel is NULL");
   DPMI_CHECK_ARG(nall >= nloc && nall < (int64_t(1) << 24), "nall out of range");
@@ -855,15 +786,33 @@ extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped
   hipStream_t st = as_stream(stream);
   char* ws = const_cast<char*>(static_cast<const char*>(workspace));
   int* misc = reinterpret_cast<int*>(ws + ws_layout(0, 0, 0).misc);
-  // status words: misc[1] capacity error, misc[2] redo count
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, misc + 1, (int64_t)2, 0);
-  const size_t shm_cells = (size_t)kCandCap * (3 * sizeof(double) + 2 * sizeof(int)) +
-                           4 * ((size_t)(kKeyCap + 8) * sizeof(uint64_t) + sizeof(int) * sp.nnei);
-  if (shm_cells > 65536)
-    DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_search_cells),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_cells));
-  hipLaunchKernelGGL(k_search_cells, dim3((unsigned)std::min<int64_t>(nloc, 2048)), dim3(256), shm_cells, st, nlist,
-                     overflow, nlist_mapped, ext_coord, ext_type, mapping, ws, sp, rcut);
+  // status words misc[1..3] (capacity error, redo count, big-cell count)
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, misc + 1, (int64_t)3, 0);
+  auto shm_for = [&](int cap) {
+    return (size_t)cap * (3 * sizeof(double) + 2 * sizeof(int)) +
+           4 * ((size_t)(kKeyCap + 8) * (sizeof(uint64_t) + sizeof(int16_t)) + (sizeof(int) + sizeof(int16_t)) * sp.nnei);
+  };
+  const size_t shm_cells = shm_for(kCandCap), shm_big = shm_for(kCandCapBig);
+  if (shm_big <= kMaxCellsLds) {
+    // first pass over every cell; cells whose neighborhood exceeds its table
+    // are listed for the second pass (larger table, fewer workgroups per CU)
+    if (shm_cells > 65536)
+      DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_search_cells<kCandCap, false>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_cells));
+    DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_search_cells<kCandCapBig, true>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_big));
+    hipLaunchKernelGGL((k_search_cells<kCandCap, false>), dim3((unsigned)std::min<int64_t>(nloc, 2048)), dim3(256),
+                       shm_cells, st, nlist, overflow, nlist_mapped, cand_slot, ext_coord, ext_type, mapping, ws, sp,
+                       rcut);
+    hipLaunchKernelGGL((k_search_cells<kCandCapBig, true>), dim3((unsigned)std::min<int64_t>(nloc, 1024)), dim3(256),
+                       shm_big, st, nlist, overflow, nlist_mapped, cand_slot, ext_coord, ext_type, mapping, ws, sp,
+                       rcut);
+  } else {
+    // very long rows (sum(sel)): every atom through the per-atom kernel
+    size_t shm = 1024 * sizeof(uint64_t) + sizeof(int) * sp.nnei;
+    hipLaunchKernelGGL(k_search_format<1024>, dim3((unsigned)nloc), dim3(64), shm, st, nlist, overflow, nlist_mapped,
+                       ext_coord, ext_type, mapping, ws, nullptr, cand_slot, sp, rcut);
+  }
   DPMI_LAUNCH_CHECK();
   int stat[2] = {0, 0};
   DPMI_HIP(hipMemcpyAsync(stat, misc + 1, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -873,7 +822,7 @@ extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped
     const int* redo = reinterpret_cast<const int*>(ws + ws_layout(nloc, 0, 0).redo);
     size_t shm = 1024 * sizeof(uint64_t) + sizeof(int) * sp.nnei;
     hipLaunchKernelGGL(k_search_format<1024>, dim3(stat[1]), dim3(64), shm, st, nlist, overflow, nlist_mapped,
-                       ext_coord, ext_type, mapping, ws, redo, sp, rcut);
+                       ext_coord, ext_type, mapping, ws, redo, cand_slot, sp, rcut);
     DPMI_LAUNCH_CHECK();
     DPMI_HIP(hipMemcpyAsync(stat, misc + 1, sizeof(int), hipMemcpyDeviceToHost, st));
     DPMI_HIP(hipStreamSynchronize(st));

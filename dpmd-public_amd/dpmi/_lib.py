@@ -21,7 +21,7 @@ _F = ctypes.c_float
 SIGNATURES = {
     "dpmi_nlist_workspace_bytes": [_P, _P, _L, _F],
     "dpmi_copy_coord_f64": [_P, _P, _P, _P, _L, _P, _P, _L, _P, _F, _P, _L, _P],
-    "dpmi_build_nlist_f64": [_P, _P, _P, _P, _P, _P, _L, _L, _F, _P, _I, _P, _L, _P],
+    "dpmi_build_nlist_f64": [_P, _P, _P, _P, _P, _P, _P, _L, _L, _F, _P, _I, _P, _L, _P],
     "dpmi_prod_env_mat_a_f64": [_P, _P, _P, _P, _P, _P, _L, _P, _P, _F, _F, _P, _I, _P],
     "dpmi_prod_env_mat_a_f32": [_P, _P, _P, _P, _P, _P, _L, _P, _P, _F, _F, _P, _I, _P],
     "dpmi_prod_env_mat_a_f32c64": [_P, _P, _P, _P, _P, _P, _L, _P, _P, _F, _F, _P, _I, _P],
@@ -32,6 +32,9 @@ SIGNATURES = {
     "dpmi_prod_virial_se_a_f32": [_P, _P, _P, _P, _P, _P, _L, _L, _L, _L, _P, _L, _P],
     "dpmi_force_virial_recompute_f64": [_P, _P, _P, _P, _P, _P, _P, _L, _L, _P, _F, _F, _P, _I, _P, _L, _P],
     "dpmi_force_virial_recompute_f32": [_P, _P, _P, _P, _P, _P, _P, _L, _L, _P, _F, _F, _P, _I, _P, _L, _P],
+    "dpmi_force_virial_cells_workspace_bytes": [_P, _P, _L, _F, _I],
+    "dpmi_force_virial_cells_f64": [_P] * 8 + [_L, _L, _P, _F, _F, _P, _I, _P, _P, _P, _L, _P],
+    "dpmi_force_virial_cells_f32": [_P] * 8 + [_L, _L, _P, _F, _F, _P, _I, _P, _P, _P, _L, _P],
     "dpmi_se_a_embed_fwd_bf16": [_P, _P, _P, _P, _L, _P, _I, _I, _P],
     "dpmi_se_a_descriptor_fwd": [_P, _P, _P, _L, _I, _I, _I, _I, _I, _P],
     "dpmi_se_a_descriptor_bwd": [_P, _P, _P, _L, _I, _I, _I, _P],

@@ -163,7 +163,13 @@ class SeAEnergyModel:
 
     # ----------------------------------------------------------------- pieces
     def neighbor_list(self, coord, atype, box):
-        return ops.build_nlist(coord, atype, box, self.rcut, self.sel)
+        """(ext_coord, ext_type, mapping, nlist, overflow, None, cells); cells
+        = (candidate slots, workspace) for the cell-wise force kernel."""
+        return ops.build_nlist(coord, atype, box, self.rcut, self.sel, cells=True)
+
+    def forces(self, net_deriv, nlist, cells, mapping, ext_c, ext_t, box):
+        return ops.force_virial_cells(net_deriv, nlist, cells, mapping, ext_c, ext_t, self.dstd, self.rcut,
+                                      self.rcut_smth, self.sel, box)
 
     def env_mat(self, nlist, ext_c, ext_t):
         em, _, _ = ops.env_mat_a(nlist, ext_c, ext_t, self.davg, self.dstd, self.rcut, self.rcut_smth, self.sel,
@@ -202,23 +208,22 @@ class SeAEnergyModel:
         GPU, atype [nloc] int; box: 9 values or None.  The neighbor list is
         rebuilt every call."""
         atype = atype.to(torch.int32)
-        ext_c, ext_t, mapping, nlist, _ovf, _ = self.neighbor_list(coord, atype, box)
+        ext_c, ext_t, mapping, nlist, _ovf, _, cells = self.neighbor_list(coord, atype, box)
         if self.fused is not None:
-            return self._compute_fused(atype, ext_c, ext_t, mapping, nlist, need_atom_energy)
+            return self._compute_fused(atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy)
         em = self.env_mat(nlist, ext_c, ext_t).requires_grad_(True)
         with torch.enable_grad():
             desc = self.descriptor(em)
             ae = self.atom_energy(desc, atype)
             energy = ae.sum()
             (net_deriv,) = torch.autograd.grad(energy, em)
-        force, virial = ops.force_virial_recompute(net_deriv, nlist, mapping, ext_c, ext_t, self.dstd, self.rcut,
-                                                   self.rcut_smth, self.sel)
+        force, virial = self.forces(net_deriv, nlist, cells, mapping, ext_c, ext_t, box)
         out = {"energy": energy.detach(), "force": force, "virial": virial}
         if need_atom_energy:
             out["atom_energy"] = ae.detach()
         return out
 
-    def _compute_fused(self, atype, ext_c, ext_t, mapping, nlist, need_atom_energy):
+    def _compute_fused(self, atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy):
         """bf16 form with hand-written adjoints end to end: fused embedding +
         G^T R forward, fused fitting forward + adjoint (or hipBLASLt GEMMs with
         autograd through the fitting net when its shape is not supported),
@@ -236,8 +241,7 @@ class SeAEnergyModel:
                 energy = ae.sum()
                 (ddesc,) = torch.autograd.grad(energy, desc)
         nd = self.fused.backward(em, gr, ddesc)
-        force, virial = ops.force_virial_recompute(nd, nlist, mapping, ext_c, ext_t, self.dstd, self.rcut,
-                                                   self.rcut_smth, self.sel)
+        force, virial = self.forces(nd, nlist, cells, mapping, ext_c, ext_t, box)
         out = {"energy": energy.detach(), "force": force, "virial": virial}
         if need_atom_energy:
             out["atom_energy"] = ae.detach()

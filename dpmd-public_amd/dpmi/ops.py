@@ -122,12 +122,14 @@ def nlist_workspace_bytes(box, nloc: int, rcut: float) -> int:
 
 
 def build_nlist(coord: torch.Tensor, atype: torch.Tensor, box, rcut: float, sel, mapped: bool = False,
-                mem_nall: int | None = None):
+                mem_nall: int | None = None, cells: bool = False):
     """Periodic (box given) or open neighbor list of one frame.
 
     coord [nloc, 3] float64, atype [nloc] int.  Returns (ext_coord [nall, 3],
     ext_type [nall] int32, mapping [nall] int32, nlist [nloc, nnei] int32 in
-    extended indices, overflow [nloc] int32, nlist_mapped or None).
+    extended indices, overflow [nloc] int32, nlist_mapped or None).  With
+    cells=True a seventh item (cand_slot [nloc, nnei] int16, workspace) is
+    returned for force_virial_cells.
     """
     _gpu(coord, atype)
     dev = coord.device
@@ -157,8 +159,11 @@ def build_nlist(coord: torch.Tensor, atype: torch.Tensor, box, rcut: float, sel,
     nlist = torch.empty((nloc, nnei), dtype=torch.int32, device=dev)
     overflow = torch.empty(nloc, dtype=torch.int32, device=dev)
     nl_map = torch.empty((nloc, nnei), dtype=torch.int32, device=dev) if mapped else None
-    _launch("dpmi_build_nlist_f64", ptr(nlist), ptr(overflow), ptr(nl_map), ptr(ext_c), ptr(ext_t), ptr(mapping),
-         nloc, n, rcut, ptr(s), len(s), ptr(ws), ws_bytes, _stream())
+    cslot = torch.empty((nloc, nnei), dtype=torch.int16, device=dev) if cells else None
+    _launch("dpmi_build_nlist_f64", ptr(nlist), ptr(overflow), ptr(nl_map), ptr(cslot), ptr(ext_c), ptr(ext_t),
+         ptr(mapping), nloc, n, rcut, ptr(s), len(s), ptr(ws), ws_bytes, _stream())
+    if cells:
+        return ext_c, ext_t, mapping, nlist, overflow, nl_map, (cslot, ws)
     return ext_c, ext_t, mapping, nlist, overflow, nl_map
 
 
@@ -206,6 +211,33 @@ def force_virial_recompute(net_deriv, nlist, mapping, ext_coord, ext_type, std, 
          ptr(_c(nlist, torch.int32)), ptr(_c(mapping, torch.int32)) if mapping is not None else None,
          ptr(_c(ext_coord, torch.float64)), ptr(_c(ext_type, torch.int32)), nloc, nall,
          ptr(_c(std, net_deriv.dtype)), rcut, rcut_smth, ptr(s), len(s), ptr(ws), int(wsb.value), _stream())
+    return force, virial
+
+
+def force_virial_cells(net_deriv, nlist, cells, mapping, ext_coord, ext_type, std, rcut, rcut_smth, sel, box,
+                       want_virial=True):
+    """force_virial_recompute for a list from build_nlist(..., cells=True)
+    (cells = its (cand_slot, workspace)): no memory atomics for the neighbor
+    forces, deterministic.  Forces folded onto local atoms."""
+    _gpu(net_deriv, nlist, ext_coord, ext_type, std)
+    sfx = _sfx(net_deriv.dtype)
+    cslot, nlws = cells
+    nloc = nlist.shape[0]
+    nall = ext_coord.shape[0]
+    dev = ext_coord.device
+    b = _box_host(box)
+    force = torch.empty((nloc, 3), dtype=torch.float64, device=dev)
+    virial = torch.empty(9, dtype=torch.float64, device=dev) if want_virial else None
+    wsb = ctypes.c_int64(0)
+    call("dpmi_force_virial_cells_workspace_bytes", ctypes.byref(wsb), ptr(b) if b is not None else None, nloc,
+         rcut, int(net_deriv.dtype == torch.float64))
+    ws = torch.empty(int(wsb.value), dtype=torch.uint8, device=dev)
+    s = _sel_host(sel)
+    _launch(f"dpmi_force_virial_cells_{sfx}", ptr(force), ptr(virial), ptr(_c(net_deriv)),
+         ptr(_c(nlist, torch.int32)), ptr(cslot), ptr(_c(mapping, torch.int32)) if mapping is not None else None,
+         ptr(_c(ext_coord, torch.float64)), ptr(_c(ext_type, torch.int32)), nloc, nall,
+         ptr(_c(std, net_deriv.dtype)), rcut, rcut_smth, ptr(s), len(s), ptr(b) if b is not None else None,
+         ptr(nlws), ptr(ws), int(wsb.value), _stream())
     return force, virial
 
 

@@ -67,11 +67,14 @@ int dpmi_copy_coord_f64(double* out_coord, int* out_type, int* mapping, int64_t*
  * receives -1 or the largest type whose neighbors exceeded sel.
  * nlist_mapped (optional) receives mapping[nlist] (the local-index list
  * that ProdEnvMatAOp returns, source/op/tf/prod_env_mat_multi_device.cc:
- * 619-622). */
-int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped, const double* ext_coord,
-                         const int* ext_type, const int* mapping, int64_t nloc, int64_t nall, float rcut,
-                         const int* sel_host, int ntypes, const void* workspace, int64_t workspace_bytes,
-                         void* stream);
+ * 619-622).  cand_slot (optional, [nloc, sum(sel)] int16) receives each
+ * neighbor's position in the candidate table of its centre's cell (-1 where
+ * none), which dpmi_force_virial_cells_* uses; the workspace then keeps the
+ * cell grid for that call. */
+int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped, int16_t* cand_slot,
+                         const double* ext_coord, const int* ext_type, const int* mapping, int64_t nloc,
+                         int64_t nall, float rcut, const int* sel_host, int ntypes, const void* workspace,
+                         int64_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
  * Environment matrix.  Replaces prod_env_mat_a_gpu
@@ -144,6 +147,30 @@ int dpmi_force_virial_recompute_f32(double* force, double* virial, const float* 
                                     int64_t nall, const float* std, float rcut, float rcut_smth,
                                     const int* sel_host, int ntypes, void* workspace, int64_t workspace_bytes,
                                     void* stream);
+
+/* Cell-wise form of dpmi_force_virial_recompute_* for a list built with
+ * cand_slot (same results up to summation order; deterministic).  Each
+ * cell's neighbor terms accumulate in on-chip memory at the neighbor's slot
+ * of the cell's candidate table and every local atom then sums its per-cell
+ * partials, so the neighbor forces need no memory atomics (atoms the search
+ * handled one by one fall back to atomics).  Replaces the same reference
+ * functions as dpmi_force_virial_recompute_*; box_host and rcut as given to
+ * dpmi_copy_coord_f64, nlist_workspace = that call's workspace.  Forces are
+ * folded onto local atoms (mapping may be NULL only without ghosts). */
+int dpmi_force_virial_cells_workspace_bytes(int64_t* bytes, const double* box_host, int64_t nloc, float rcut,
+                                            int fp64);
+int dpmi_force_virial_cells_f64(double* force, double* virial, const double* net_deriv, const int* nlist,
+                                const int16_t* cand_slot, const int* mapping, const double* ext_coord,
+                                const int* ext_type, int64_t nloc, int64_t nall, const double* std, float rcut,
+                                float rcut_smth, const int* sel_host, int ntypes, const double* box_host,
+                                const void* nlist_workspace, void* workspace, int64_t workspace_bytes,
+                                void* stream);
+int dpmi_force_virial_cells_f32(double* force, double* virial, const float* net_deriv, const int* nlist,
+                                const int16_t* cand_slot, const int* mapping, const double* ext_coord,
+                                const int* ext_type, int64_t nloc, int64_t nall, const float* std, float rcut,
+                                float rcut_smth, const int* sel_host, int ntypes, const double* box_host,
+                                const void* nlist_workspace, void* workspace, int64_t workspace_bytes,
+                                void* stream);
 
 /* ------------------------------------------------------------------------
  * Tabulated embedding fused with G^T R.  tabulate_fusion_se_a_gpu and its

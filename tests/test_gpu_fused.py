@@ -39,7 +39,7 @@ def _rel_rms(a, b):
 def test_fused_descriptor_forward(setup):
     spec, params, m64, mbf, s, c, t = setup
     _say("fused descriptor forward water192")
-    ext_c, ext_t, mapping, nlist, _, _ = m64.neighbor_list(c, t.to(torch.int32), s.box)
+    ext_c, ext_t, mapping, nlist, _, _, _ = m64.neighbor_list(c, t.to(torch.int32), s.box)
     em64 = m64.env_mat(nlist, ext_c, ext_t)
     d64 = m64.descriptor(em64)
     gr, d = mbf.fused.forward(em64.float())
@@ -51,7 +51,7 @@ def test_fused_descriptor_forward(setup):
 def test_fused_embedding_backward(setup):
     spec, params, m64, mbf, s, c, t = setup
     _say("fused descriptor backward water192")
-    ext_c, ext_t, mapping, nlist, _, _ = m64.neighbor_list(c, t.to(torch.int32), s.box)
+    ext_c, ext_t, mapping, nlist, _, _, _ = m64.neighbor_list(c, t.to(torch.int32), s.box)
     em64 = m64.env_mat(nlist, ext_c, ext_t).requires_grad_(True)
     d64 = m64.descriptor(em64)
     w = torch.linspace(-1, 1, d64.numel(), dtype=torch.float64, device=DEV).reshape(d64.shape)

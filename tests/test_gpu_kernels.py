@@ -193,6 +193,72 @@ def test_force_virial_recompute(oracle, dtype):
     np.testing.assert_allclose(fe.cpu().numpy(), of, rtol=tol, atol=tol * scale)
 
 
+CELLS_CASES = [
+    # ncell = 2 per side: several images of one atom in a cell's table
+    ("water192", lambda: systems.water192(0), 6.0, [46, 92]),
+    ("water12k", lambda: systems.water12k(1), 6.0, [46, 92]),
+    ("alloy_triclinic_4k", lambda: systems.alloy262k(4, ncell=16), 8.0, [60, 60, 60, 60]),
+    # atoms and cells the search handles one by one: neighbor terms by atomics
+    ("clustered", _clustered, 6.0, [200, 200]),
+    ("water12k_rcut9", lambda: systems.water12k(1), 9.0, [110, 220]),
+]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("name,make,rcut,sel", CELLS_CASES, ids=[c[0] for c in CELLS_CASES])
+def test_force_virial_cells(oracle, name, make, rcut, sel, dtype):
+    """Cell-wise adjoint (no neighbor atomics) against the per-atom recompute
+    adjoint (itself pinned to the oracle above) on the GPU's own list, and for
+    FP64 also against the oracle's prod_force/prod_virial."""
+    s = make()
+    _say(f"force/virial cells {name} {dtype}")
+    ext_c, ext_t, mapping, nlist, ovf, _, cells = ops.build_nlist(T(s.coord), T(s.atype), s.box, rcut, sel,
+                                                                  cells=True)
+    nnei = int(sum(sel))
+    rng = np.random.default_rng(7)
+    std = 0.5 + rng.uniform(size=(len(sel), nnei, 4))
+    nd = rng.normal(size=(s.nloc, nnei, 4))
+    f0, v0 = ops.force_virial_recompute(T(nd, dtype), nlist, mapping, ext_c, ext_t, T(std, dtype), rcut, 0.5, sel)
+    f1, v1 = ops.force_virial_cells(T(nd, dtype), nlist, cells, mapping, ext_c, ext_t, T(std, dtype), rcut, 0.5,
+                                    sel, s.box)
+    f0, v0, f1, v1 = (x.cpu().numpy() for x in (f0, v0, f1, v1))
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    np.testing.assert_allclose(f1, f0, rtol=tol, atol=tol * np.abs(f0).max())
+    np.testing.assert_allclose(v1, v0, rtol=tol, atol=tol * np.abs(v0).max())
+    if dtype == torch.float64 and name == "water12k":
+        oc, ot, om, onl, _ = oracle.build_nlist_pbc(s.coord, s.atype, s.box, rcut, sel)
+        avg = np.zeros_like(std)
+        _, oed, orij = oracle.env_mat_a(onl, oc, ot, avg, std, rcut, 0.5, sel)
+        of = oracle.prod_force_a(nd.reshape(s.nloc, -1), oed, onl, oc.shape[0])[0]
+        of_local = np.zeros((s.nloc, 3))
+        np.add.at(of_local, om, of)
+        np.testing.assert_allclose(f1, of_local, rtol=1e-10, atol=1e-10 * np.abs(of_local).max())
+    if name in ("clustered", "water12k_rcut9"):
+        return  # atoms left to the per-atom search add their neighbor terms with atomics
+    if name.startswith("alloy") and dtype == torch.float64:
+        return  # FP64 cells over the first table size add their neighbor terms with atomics
+    # deterministic: a second call gives the same bits
+    f2, v2 = ops.force_virial_cells(T(nd, dtype), nlist, cells, mapping, ext_c, ext_t, T(std, dtype), rcut, 0.5,
+                                    sel, s.box)
+    np.testing.assert_array_equal(f2.cpu().numpy(), f1)
+    np.testing.assert_array_equal(v2.cpu().numpy(), v1)
+
+
+def test_force_virial_cells_open_system():
+    _say("force/virial cells open system")
+    rng = np.random.default_rng(2)
+    pos = rng.uniform(0, 9, (300, 3))
+    atype = rng.integers(0, 2, 300).astype(np.int32)
+    sel = [30, 30]
+    ext_c, ext_t, mapping, nlist, ovf, _, cells = ops.build_nlist(T(pos), T(atype), None, 3.0, sel, cells=True)
+    std = 0.5 + rng.uniform(size=(2, 60, 4))
+    nd = rng.normal(size=(300, 60, 4))
+    f0, v0 = ops.force_virial_recompute(T(nd), nlist, None, ext_c, ext_t, T(std), 3.0, 0.5, sel)
+    f1, v1 = ops.force_virial_cells(T(nd), nlist, cells, None, ext_c, ext_t, T(std), 3.0, 0.5, sel, None)
+    np.testing.assert_allclose(f1.cpu().numpy(), f0.cpu().numpy(), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(v1.cpu().numpy(), v0.cpu().numpy(), rtol=1e-12, atol=1e-12)
+
+
 def _tab(golden, pt=True):
     if pt:
         g = golden("tabulate_fusion_se_a_pt.json")
