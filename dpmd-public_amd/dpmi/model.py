@@ -108,7 +108,7 @@ class Net:
 
 class SeAEnergyModel:
     def __init__(self, rcut, rcut_smth, sel, axis, davg, dstd, emb, fit, bias_atom_e, out_bias=None,
-                 device="cuda", precision: str = "fp64"):
+                 device="cuda", precision: str = "fp64", tables=None):
         if precision not in ("fp64", "bf16", "fp16"):
             raise ValueError(f"precision must be fp64, bf16 or fp16, not {precision}")
         self.device = torch.device(device)
@@ -127,6 +127,14 @@ class SeAEnergyModel:
         self.dstd = torch.as_tensor(np.asarray(dstd, dtype=np.float64), device=self.device, dtype=dt)
         self.emb = [Net(layers, self.device, dt, last_linear=False) for layers in emb]
         neuron = [w.shape[1] for w in self.emb[0].w]
+        # compressed form: per neighbor type (table, table_info) of the
+        # tabulated embedding (dpmi.table.build_tables), fp64
+        self.tables = None
+        if tables is not None:
+            if precision == "bf16" or precision == "fp16":
+                raise ValueError("the tabulated embedding runs in fp64")
+            self.tables = [(torch.as_tensor(np.asarray(t), device=self.device, dtype=dt),
+                            torch.as_tensor(np.asarray(i), dtype=dt)) for t, i in tables]
         # bf16 form: fused MFMA embedding kernels (else the torch layers above)
         self.fused = None
         if precision == "bf16" and embed.supported(neuron) and self.axis <= 16:
@@ -144,10 +152,10 @@ class SeAEnergyModel:
 
     # ----------------------------------------------------------------- constructors
     @classmethod
-    def from_params(cls, spec, params, device="cuda", precision="fp64"):
+    def from_params(cls, spec, params, device="cuda", precision="fp64", tables=None):
         return cls(spec.rcut, spec.rcut_smth, spec.sel, spec.axis_neuron, params["davg"], params["dstd"],
                    params["embedding"], params["fitting"], params["bias_atom_e"], device=device,
-                   precision=precision)
+                   precision=precision, tables=tables)
 
     @classmethod
     def from_serialized(cls, m: dict, device="cuda", precision="fp64"):
@@ -182,6 +190,13 @@ class SeAEnergyModel:
         gr = None
         for t in range(self.ntypes):
             rr = em[:, self.sec[t]:self.sec[t + 1], :]
+            if self.tables is not None:
+                # tabulate_fusion_se_a in place of embedding + G^T R (se_a.py:833-839)
+                table, info = self.tables[t]
+                c = ops.TabulateFusionSeAOp.apply(table, info, rr[:, :, :1].reshape(-1, 1), rr.contiguous(),
+                                                  self.M)
+                gr = c if gr is None else gr + c
+                continue
             g = self.emb[t](rr[:, :, :1].reshape(-1, 1), self.gemm_dtype).reshape(nloc, -1, self.M)
             if self.gemm_dtype is None:
                 c = torch.bmm(rr.transpose(1, 2), g)

@@ -7,7 +7,8 @@ A float64 torch-CPU restatement of the reference deepmd/pt model path:
 * embedding/fitting layers: deepmd/pt/model/network/mlp.py:188-219
   (linear, tanh, timestep, resnet with x or [x, x]);
 * descriptor: deepmd/pt/model/descriptor/se_a.py:750-866 (type_one_side,
-  G^T R / nnei, D = (G^T R)^T (G^T R)[:, :axis]);
+  G^T R / nnei, D = (G^T R)^T (G^T R)[:, :axis]); compressed: the C
+  oracle's tabulate_fusion_se_a in place of the embedding + G^T R per type;
 * fitting: deepmd/pt/model/task/fitting.py:734-900 (own-type net + bias_atom_e)
   and the out_bias of base_atomic_model.py:571-592;
 * energy/force/virial: deepmd/pt/model/model/transform_output.py:65-206,
@@ -53,12 +54,34 @@ class Layer:
         return y
 
 
+class _Tabulate(torch.autograd.Function):
+    """Compressed embedding of one neighbor-type section on the CPU: the C
+    oracle's tabulate_fusion_se_a and its grad (the op and its backward,
+    tabulate_multi_device.cc:702-830; called as in se_a.py:826-839)."""
+
+    @staticmethod
+    def forward(ctx, em_x, em, table, info, M):
+        ctx.save_for_backward(em_x, em)
+        ctx.table, ctx.info = table, info
+        out = oracle.tabulate_fusion_se_a(table, info, em_x.detach().numpy(), em.detach().numpy(), M)
+        return torch.as_tensor(out)
+
+    @staticmethod
+    def backward(ctx, dy):
+        em_x, em = ctx.saved_tensors
+        dx, dem = oracle.tabulate_fusion_se_a_grad(ctx.table, ctx.info, em_x.detach().numpy(),
+                                                   em.detach().numpy(), dy.detach().numpy())
+        return torch.as_tensor(dx).reshape(em_x.shape), torch.as_tensor(dem), None, None, None
+
+
 class Model:
     """se_e2_a + energy fitting, from a dict in the reference serialization
     layout (the 'model' block of deeppot_sea.yaml) or from dpmi.systems
-    parameters (see from_params)."""
+    parameters (see from_params).  tables (optional): per neighbor type
+    (table, table_info) of the compressed embedding."""
 
-    def __init__(self, rcut, rcut_smth, sel, axis, davg, dstd, emb, fit, bias_atom_e, out_bias=None):
+    def __init__(self, rcut, rcut_smth, sel, axis, davg, dstd, emb, fit, bias_atom_e, out_bias=None, tables=None):
+        self.tables = tables
         self.rcut = float(rcut)
         self.rcut_smth = float(rcut_smth)
         self.sel = list(sel)
@@ -84,9 +107,9 @@ class Model:
                    m["@variables"]["out_bias"])
 
     @classmethod
-    def from_params(cls, spec, params: dict) -> "Model":
+    def from_params(cls, spec, params: dict, tables=None) -> "Model":
         return cls(spec.rcut, spec.rcut_smth, spec.sel, spec.axis_neuron, params["davg"], params["dstd"],
-                   params["embedding"], params["fitting"], params["bias_atom_e"])
+                   params["embedding"], params["fitting"], params["bias_atom_e"], tables=tables)
 
     # ------------------------------------------------------------------ pieces
     def env_mat(self, ext, nlist, atype):
@@ -112,6 +135,10 @@ class Model:
         gr = torch.zeros((nloc, 4, M), dtype=torch.float64)
         for t in range(len(self.sel)):
             rr = em[:, self.sec[t]:self.sec[t + 1], :]
+            if self.tables is not None:
+                table, info = self.tables[t]
+                gr = gr + _Tabulate.apply(rr[:, :, :1].reshape(-1, 1), rr.contiguous(), table, info, M)
+                continue
             g = rr[:, :, :1]
             for layer in self.emb[t]:
                 g = layer(g)

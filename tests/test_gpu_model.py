@@ -88,3 +88,39 @@ def test_low_precision_forms_error_budget(water_model, precision):
     # 1600-wide fitting input to 8 mantissa bits, ~1 meV/atom here)
     assert rmse <= 1e-3
     assert de <= 5e-3
+
+
+def test_tabulated_fp64_model_matches_oracle():
+    """Compressed model (tabulated embedding, grid [0, 2) / 0.01 + [2, 10) /
+    0.1, 280 intervals) at 12,288 water atoms, FP64: E, F, V within 1e-10
+    relative of the oracle running the same tables on the CPU."""
+    from oracle.model_ref import Model
+
+    from dpmi import table
+    from dpmi.model import SeAEnergyModel
+
+    spec = systems.WATER_SPEC
+    s = systems.water12k()
+    params = systems.make_params(spec, seed=7)  # davg 0, dstd 1: table range [0, 2)
+    tabs = table.build_tables(params["embedding"], params["davg"], params["dstd"], spec.rcut, spec.rcut_smth,
+                              0.9, spec.sel)
+    assert tabs[0][0].shape == (280, 600)
+    _say("tabulated fp64 model water12k vs oracle")
+    m = SeAEnergyModel.from_params(spec, params, device=DEV, tables=tabs)
+    out = m.compute(torch.tensor(s.coord, device=DEV), torch.tensor(s.atype, device=DEV), s.box,
+                    need_atom_energy=True)
+    o = Model.from_params(spec, params, tables=tabs).evaluate(s.coord, s.atype, s.box)
+    assert abs(out["energy"].item() - o["energy"]) <= 1e-10 * abs(o["energy"])
+    np.testing.assert_allclose(out["atom_energy"].cpu().numpy(), o["atom_energy"], rtol=1e-10,
+                               atol=1e-10 * np.abs(o["atom_energy"]).max())
+    f = out["force"].cpu().numpy()
+    np.testing.assert_allclose(f, o["force"], rtol=1e-10, atol=1e-10 * np.abs(o["force"]).max())
+    np.testing.assert_allclose(out["virial"].cpu().numpy(), o["virial"], rtol=1e-10,
+                               atol=1e-10 * np.abs(o["virial"]).max())
+    # the table stands in for the network: report how far the compressed
+    # model is from the uncompressed one
+    full = SeAEnergyModel.from_params(spec, params, device=DEV).compute(
+        torch.tensor(s.coord, device=DEV), torch.tensor(s.atype, device=DEV), s.box)
+    df = torch.sqrt(((full["force"] - out["force"]) ** 2).mean()).item()
+    de = abs(full["energy"].item() - out["energy"].item()) / s.nloc
+    print(f"table vs network: force RMSE {df:.3e} eV/A, |dE|/atom {de:.3e} eV", flush=True)

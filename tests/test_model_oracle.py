@@ -62,3 +62,35 @@ def test_virial_is_cell_derivative(model):
 
         fd = -(e(h) - e(-h)) / (2 * h)
         assert abs(fd - base["virial"].reshape(3, 3)[b, a]) < 1e-5 * max(1, abs(fd))
+
+
+def test_tabulated_oracle_tracks_network():
+    """The compressed oracle (C tabulate_fusion_se_a + its grad in place of
+    embedding + G^T R) with a table of the network as evaluated
+    (timestep folded in) agrees with the uncompressed oracle to the table's
+    interpolation error, and its forces are minus its energy gradient."""
+    from oracle.model_ref import Model
+
+    from dpmi import systems, table
+
+    spec = systems.WATER_SPEC
+    params = systems.make_params(spec, seed=3)
+    tabs = table.build_tables(params["embedding"], params["davg"], params["dstd"], spec.rcut, spec.rcut_smth,
+                              0.9, spec.sel, fold_idt=True)
+    s = systems.water192(2)
+    s.coord = s.coord[:48]
+    s.atype = s.atype[:48]
+    net = Model.from_params(spec, params).evaluate(s.coord, s.atype, s.box)
+    tm = Model.from_params(spec, params, tables=tabs)
+    tab = tm.evaluate(s.coord, s.atype, s.box)
+    assert abs(tab["energy"] - net["energy"]) < 1e-7 * s.nloc
+    np.testing.assert_allclose(tab["force"], net["force"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(tab["virial"], net["virial"], rtol=0, atol=1e-5)
+    h = 1e-5
+    for i, d in [(3, 0), (30, 2)]:
+        cp = s.coord.copy()
+        cm = s.coord.copy()
+        cp[i, d] += h
+        cm[i, d] -= h
+        fd = -(tm.evaluate(cp, s.atype, s.box)["energy"] - tm.evaluate(cm, s.atype, s.box)["energy"]) / (2 * h)
+        assert abs(fd - tab["force"][i, d]) < 1e-6 * max(1, abs(fd))
