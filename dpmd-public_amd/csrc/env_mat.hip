@@ -291,6 +291,72 @@ __global__ void __launch_bounds__(256)
   block_virial_partial<FP>(v, partial);
 }
 
+// prod_force_grad_a (source/lib/src/prod_force_grad.cc:22-79), the transpose
+// of prod_force: gn[i, 4s+c] = -<g_i, ed[i,4s+c]> + <g_j, ed[i,4s+c]> for a
+// valid neighbor j of slot s.  grad holds nall rows per frame; an index
+// j >= nall reads row j % nloc (with nall = nloc: the reference's rule).
+// One lane per (atom, slot): 12 contiguous ed values in, 4 gn values out.
+template <typename FP>
+__global__ void __launch_bounds__(256)
+    k_prod_force_grad(FP* __restrict__ gn, const FP* __restrict__ grad, const FP* __restrict__ ed,
+                      const int* __restrict__ nlist, int64_t nslots, int nloc, int nall, int nnei) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nslots) return;
+  const int64_t gi = t / nnei;  // frame * nloc + atom
+  const int64_t f = gi / nloc;
+  const FP* e = ed + t * 12;
+  const FP* g = grad + (f * nall + (gi - f * nloc)) * 3;
+  FP o[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    FP a = (FP)0.;
+    a -= g[0] * e[3 * c + 0];
+    a -= g[1] * e[3 * c + 1];
+    a -= g[2] * e[3 * c + 2];
+    o[c] = a;
+  }
+  int j = nlist[t];
+  if (j >= nall) j = j % nloc;
+  if (j >= 0) {
+    const FP* gj = grad + (f * nall + j) * 3;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      o[c] += gj[0] * e[3 * c + 0];
+      o[c] += gj[1] * e[3 * c + 1];
+      o[c] += gj[2] * e[3 * c + 2];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) gn[t * 4 + c] = o[c];
+}
+
+// prod_virial_grad_a (source/lib/src/prod_virial_grad.cc:20-62):
+// gn[i, 4s+c] = sum_{d0,d1} g[d0,d1] rij[i,s,d1] ed[i,4s+c,d0] for a valid
+// neighbor, 0 for an empty slot; g is the frame's [9] virial gradient.
+template <typename FP>
+__global__ void __launch_bounds__(256)
+    k_prod_virial_grad(FP* __restrict__ gn, const FP* __restrict__ grad, const FP* __restrict__ ed,
+                       const FP* __restrict__ rij, const int* __restrict__ nlist, int64_t nslots, int nloc,
+                       int nnei) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nslots) return;
+  const int64_t f = t / ((int64_t)nloc * nnei);
+  FP o[4] = {(FP)0., (FP)0., (FP)0., (FP)0.};
+  if (nlist[t] >= 0) {
+    const FP* e = ed + t * 12;
+    const FP* r = rij + t * 3;
+    const FP* g = grad + f * 9;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int d0 = 0; d0 < 3; ++d0)
+#pragma unroll
+        for (int d1 = 0; d1 < 3; ++d1) o[c] += g[d0 * 3 + d1] * r[d1] * e[3 * c + d0];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) gn[t * 4 + c] = o[c];
+}
+
 // Compensated, deterministic sum of per-block partials: one workgroup per
 // frame, 9 x 64 lanes; lane q of component k sums blocks q, q+64, ...
 // (Neumaier), then a fixed-order tree combines the 64 (sum, carry) pairs.
@@ -602,10 +668,61 @@ int prod_virial_impl(FP* virial, FP* atom_virial, const FP* nd, const FP* ed, co
   return DPMI_OK;
 }
 
+template <typename FP>
+int prod_force_grad_impl(FP* gn, const FP* grad, const FP* ed, const int* nlist, int64_t nloc, int64_t nall,
+                         int64_t nnei, int64_t nframes, void* stream) {
+  DPMI_CHECK_ARG(nloc >= 0 && nall >= nloc && nall < (int64_t(1) << 31) && nnei >= 0 && nframes >= 0,
+                 "bad sizes");
+  const int64_t n = nframes * nloc * nnei;
+  if (n == 0) return DPMI_OK;
+  DPMI_CHECK_ARG(gn && grad && ed && nlist, "NULL argument");
+  hipLaunchKernelGGL(k_prod_force_grad<FP>, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), gn, grad, ed,
+                     nlist, n, (int)nloc, (int)nall, (int)nnei);
+  DPMI_LAUNCH_CHECK();
+  return DPMI_OK;
+}
+
+template <typename FP>
+int prod_virial_grad_impl(FP* gn, const FP* grad, const FP* ed, const FP* rij, const int* nlist, int64_t nloc,
+                          int64_t nnei, int64_t nframes, void* stream) {
+  DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31) && nnei >= 0 && nframes >= 0, "bad sizes");
+  const int64_t n = nframes * nloc * nnei;
+  if (n == 0) return DPMI_OK;
+  DPMI_CHECK_ARG(gn && grad && ed && rij && nlist, "NULL argument");
+  hipLaunchKernelGGL(k_prod_virial_grad<FP>, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), gn, grad, ed,
+                     rij, nlist, n, (int)nloc, (int)nnei);
+  DPMI_LAUNCH_CHECK();
+  return DPMI_OK;
+}
+
 }  // namespace
 }  // namespace dpmi
 
 using namespace dpmi;
+
+extern "C" int dpmi_prod_force_grad_a_f64(double* grad_net, const double* grad, const double* in_deriv,
+                                          const int* nlist, int64_t nloc, int64_t nall, int64_t nnei,
+                                          int64_t nframes, void* stream) {
+  return prod_force_grad_impl<double>(grad_net, grad, in_deriv, nlist, nloc, nall, nnei, nframes, stream);
+}
+
+extern "C" int dpmi_prod_force_grad_a_f32(float* grad_net, const float* grad, const float* in_deriv,
+                                          const int* nlist, int64_t nloc, int64_t nall, int64_t nnei,
+                                          int64_t nframes, void* stream) {
+  return prod_force_grad_impl<float>(grad_net, grad, in_deriv, nlist, nloc, nall, nnei, nframes, stream);
+}
+
+extern "C" int dpmi_prod_virial_grad_a_f64(double* grad_net, const double* grad, const double* in_deriv,
+                                           const double* rij, const int* nlist, int64_t nloc, int64_t nnei,
+                                           int64_t nframes, void* stream) {
+  return prod_virial_grad_impl<double>(grad_net, grad, in_deriv, rij, nlist, nloc, nnei, nframes, stream);
+}
+
+extern "C" int dpmi_prod_virial_grad_a_f32(float* grad_net, const float* grad, const float* in_deriv,
+                                           const float* rij, const int* nlist, int64_t nloc, int64_t nnei,
+                                           int64_t nframes, void* stream) {
+  return prod_virial_grad_impl<float>(grad_net, grad, in_deriv, rij, nlist, nloc, nnei, nframes, stream);
+}
 
 extern "C" int dpmi_prod_env_mat_a_f64(double* em, double* em_deriv, double* rij, const int* nlist,
                                        const double* ext_coord, const int* ext_type, int64_t nloc, const double* avg,

@@ -15,6 +15,8 @@ recomputed, never stored); the network part of dE/d em comes from the
 embedding/fitting adjoint.  Precision forms:
 
 * ``fp64``  — everything in double ("plain FP64").
+* ``fp32``  — the same path in float (pair differences in double), energy
+  reduction in fp64.
 * ``bf16`` / ``fp16`` — embedding, descriptor and fitting GEMM operands in
   bf16/fp16 with fp32 accumulation, env-mat and its adjoint in fp32, energy
   and virial reductions in fp64.
@@ -109,12 +111,12 @@ class Net:
 class SeAEnergyModel:
     def __init__(self, rcut, rcut_smth, sel, axis, davg, dstd, emb, fit, bias_atom_e, out_bias=None,
                  device="cuda", precision: str = "fp64", tables=None):
-        if precision not in ("fp64", "bf16", "fp16"):
-            raise ValueError(f"precision must be fp64, bf16 or fp16, not {precision}")
+        if precision not in ("fp64", "fp32", "bf16", "fp16"):
+            raise ValueError(f"precision must be fp64, fp32, bf16 or fp16, not {precision}")
         self.device = torch.device(device)
         self.precision = precision
         self.dtype = torch.float64 if precision == "fp64" else torch.float32
-        self.gemm_dtype = {"fp64": None, "bf16": torch.bfloat16, "fp16": torch.float16}[precision]
+        self.gemm_dtype = {"fp64": None, "fp32": None, "bf16": torch.bfloat16, "fp16": torch.float16}[precision]
         self.rcut = float(rcut)
         self.rcut_smth = float(rcut_smth)
         self.sel = [int(s) for s in sel]
@@ -128,11 +130,11 @@ class SeAEnergyModel:
         self.emb = [Net(layers, self.device, dt, last_linear=False) for layers in emb]
         neuron = [w.shape[1] for w in self.emb[0].w]
         # compressed form: per neighbor type (table, table_info) of the
-        # tabulated embedding (dpmi.table.build_tables), fp64
+        # tabulated embedding (dpmi.table.build_tables), fp64 or fp32
         self.tables = None
         if tables is not None:
             if precision == "bf16" or precision == "fp16":
-                raise ValueError("the tabulated embedding runs in fp64")
+                raise ValueError("the tabulated embedding runs in fp64 or fp32")
             self.tables = [(torch.as_tensor(np.asarray(t), device=self.device, dtype=dt),
                             torch.as_tensor(np.asarray(i), dtype=dt)) for t, i in tables]
         # bf16 form: fused MFMA embedding kernels (else the torch layers above)
@@ -149,6 +151,27 @@ class SeAEnergyModel:
         self.fused_fit = None
         if self.fused is not None and fitting.supported(fit):
             self.fused_fit = fitting.FusedFitting(fit, bias_atom_e, ob, self.device)
+
+    def named_parameters(self):
+        """[((kind, net, layer, name), tensor)] of the trainable tensors, made
+        leaves that require grad: w, b and idt of every embedding layer (not
+        with tables: the compressed embedding is fixed, as in the reference)
+        and every fitting layer."""
+        groups = [("fitting", self.fit)]
+        if self.tables is None:
+            groups.insert(0, ("embedding", self.emb))
+        out = []
+        for kind, nets in groups:
+            for n, net in enumerate(nets):
+                for k in range(len(net.w)):
+                    for name in ("w", "b", "idt"):
+                        t = getattr(net, name)[k]
+                        if t is not None:
+                            out.append(((kind, n, k, name), t.requires_grad_(True)))
+        return out
+
+    def parameters(self):
+        return [t for _, t in self.named_parameters()]
 
     # ----------------------------------------------------------------- constructors
     @classmethod

@@ -267,6 +267,71 @@ def prod_virial(net_deriv, in_deriv, rij, nlist, nloc: int, nall: int, nframes: 
     return virial, atom_virial
 
 
+def prod_force_grad(grad, in_deriv, nlist, nloc: int, nall: int, nframes: int = 1):
+    """grad [nframes, nall * 3] -> grad_net [nframes, nloc * 4 nnei]
+    (ProdForceSeAGrad; nall = nloc is the reference op)."""
+    _gpu(grad, in_deriv, nlist)
+    dt = in_deriv.dtype
+    nnei = nlist.numel() // (nloc * nframes) if nloc * nframes > 0 else 0
+    gn = torch.empty((nframes, nloc * nnei * 4), dtype=dt, device=grad.device)
+    _launch(f"dpmi_prod_force_grad_a_{_sfx(dt)}", ptr(gn), ptr(_c(grad, dt)), ptr(_c(in_deriv)),
+            ptr(_c(nlist, torch.int32)), nloc, nall, nnei, nframes, _stream())
+    return gn
+
+
+def prod_virial_grad(grad, in_deriv, rij, nlist, nloc: int, nframes: int = 1):
+    """grad [nframes, 9] -> grad_net [nframes, nloc * 4 nnei] (ProdVirialSeAGrad)."""
+    _gpu(grad, in_deriv, rij, nlist)
+    dt = in_deriv.dtype
+    nnei = nlist.numel() // (nloc * nframes) if nloc * nframes > 0 else 0
+    gn = torch.empty((nframes, nloc * nnei * 4), dtype=dt, device=grad.device)
+    _launch(f"dpmi_prod_virial_grad_a_{_sfx(dt)}", ptr(gn), ptr(_c(grad, dt)), ptr(_c(in_deriv)),
+            ptr(_c(rij, dt)), ptr(_c(nlist, torch.int32)), nloc, nnei, nframes, _stream())
+    return gn
+
+
+class ProdForceOp(torch.autograd.Function):
+    """force = prod_force(net_deriv) with its gradient w.r.t. net_deriv from
+    the ProdForceSeAGrad kernel (the reference's registered gradient,
+    deepmd/tf/op/_prod_force_se_a_grad.py:16-28).  The backward is itself
+    linear in the incoming gradient, so a second differentiation needs none
+    of its own."""
+
+    @staticmethod
+    def forward(ctx, net_deriv, in_deriv, nlist, nloc, nall, nframes):
+        ctx.save_for_backward(in_deriv, nlist)
+        ctx.dims = (nloc, nall, nframes)
+        return prod_force(net_deriv, in_deriv, nlist, nloc, nall, nframes)
+
+    @staticmethod
+    def backward(ctx, g):
+        in_deriv, nlist = ctx.saved_tensors
+        nloc, nall, nframes = ctx.dims
+        gn = prod_force_grad(g.contiguous(), in_deriv, nlist, nloc, nall, nframes)
+        return gn, None, None, None, None, None
+
+
+class ProdVirialOp(torch.autograd.Function):
+    """(virial, atom_virial) = prod_virial(net_deriv); the gradient w.r.t.
+    net_deriv takes the virial's gradient only, as the reference's registered
+    gradient does (deepmd/tf/op/_prod_virial_se_a_grad.py:16-30)."""
+
+    @staticmethod
+    def forward(ctx, net_deriv, in_deriv, rij, nlist, nloc, nall, nframes):
+        ctx.save_for_backward(in_deriv, rij, nlist)
+        ctx.dims = (nloc, nframes)
+        return prod_virial(net_deriv, in_deriv, rij, nlist, nloc, nall, nframes)
+
+    @staticmethod
+    def backward(ctx, g, g_atom):
+        in_deriv, rij, nlist = ctx.saved_tensors
+        nloc, nframes = ctx.dims
+        if g is None:
+            return None, None, None, None, None, None, None
+        gn = prod_virial_grad(g.contiguous(), in_deriv, rij, nlist, nloc, nframes)
+        return gn, None, None, None, None, None, None
+
+
 # --------------------------------------------------------------------------- tabulate
 
 
@@ -416,13 +481,13 @@ def prod_env_mat_a(coord, atype, box, davg, dstd, rcut: float, rcut_smth: float,
 def prod_force_se_a(net_deriv, in_deriv, nlist, natoms, n_a_sel: int, n_r_sel: int):
     nf = net_deriv.shape[0]
     na = natoms.detach().to("cpu").tolist()
-    return prod_force(net_deriv, in_deriv, nlist, int(na[0]), int(na[1]), nf)
+    return ProdForceOp.apply(net_deriv, in_deriv, nlist, int(na[0]), int(na[1]), nf)
 
 
 def prod_virial_se_a(net_deriv, in_deriv, rij, nlist, natoms, n_a_sel: int, n_r_sel: int):
     nf = net_deriv.shape[0]
     na = natoms.detach().to("cpu").tolist()
-    v, av = prod_virial(net_deriv, in_deriv, rij, nlist, int(na[0]), int(na[1]), nf)
+    v, av = ProdVirialOp.apply(net_deriv, in_deriv, rij, nlist, int(na[0]), int(na[1]), nf)
     return [v, av]
 
 

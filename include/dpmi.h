@@ -124,6 +124,30 @@ int dpmi_prod_virial_se_a_f32(float* virial, float* atom_virial, const float* ne
                               int64_t nall, int64_t nnei, int64_t nframes, void* workspace,
                               int64_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Gradients of force and virial w.r.t. net_deriv (the training step's
+ * double backward).  prod_force_grad_a_cpu (source/lib/src/
+ * prod_force_grad.cc:22-79; TF op ProdForceSeAGrad,
+ * source/op/tf/prod_force_grad_multi_device.cc) and prod_virial_grad_a_cpu
+ * (source/lib/src/prod_virial_grad.cc:20-62; ProdVirialSeAGrad).
+ * grad [nframes, nall, 3] (force) or [nframes, 9] (virial) -> grad_net
+ * [nframes, nloc, 4 nnei]; in_deriv [nframes, nloc, 4 nnei, 3]; rij
+ * [nframes, nloc, nnei, 3]; nlist [nframes, nloc, nnei].  The force
+ * gradient has nall rows per frame and a neighbor index j >= nall reads
+ * row j % nloc: with nall = nloc this is the reference op exactly (its
+ * grad is [nframes, nloc * 3]); with nall = the extended count it is the
+ * exact transpose of dpmi_prod_force_se_a.  As the reference's registered
+ * gradients, the virial gradient takes the total virial's gradient only.
+ * ------------------------------------------------------------------------ */
+int dpmi_prod_force_grad_a_f64(double* grad_net, const double* grad, const double* in_deriv, const int* nlist,
+                               int64_t nloc, int64_t nall, int64_t nnei, int64_t nframes, void* stream);
+int dpmi_prod_force_grad_a_f32(float* grad_net, const float* grad, const float* in_deriv, const int* nlist,
+                               int64_t nloc, int64_t nall, int64_t nnei, int64_t nframes, void* stream);
+int dpmi_prod_virial_grad_a_f64(double* grad_net, const double* grad, const double* in_deriv, const double* rij,
+                                const int* nlist, int64_t nloc, int64_t nnei, int64_t nframes, void* stream);
+int dpmi_prod_virial_grad_a_f32(float* grad_net, const float* grad, const float* in_deriv, const float* rij,
+                                const int* nlist, int64_t nloc, int64_t nnei, int64_t nframes, void* stream);
+
 /* Backward of the env-mat for the model path (the reverse-mode derivative
  * that deepmd/pt obtains from autograd through prod_env_mat,
  * deepmd/pt/model/descriptor/env_mat.py:11-92, followed by the ghost ->

@@ -21,6 +21,9 @@
  *       test_tabulate_fusion_se_a.py arrays, finite differences, reference.
  *   prod_force_a / prod_virial_a      : test_prod_force_a.cc /
  *       test_prod_virial_a.cc expected arrays, sum-of-forces identity, ref.
+ *   prod_force_grad_a / prod_virial_grad_a : test_prod_force_grad_a.cc /
+ *       test_prod_virial_grad_a.cc expected arrays, adjoint identity with
+ *       prod_force_a / prod_virial_a, ref.
  */
 #include <math.h>
 #include <stdint.h>
@@ -747,6 +750,59 @@ void oracle_prod_virial_a(double *virial, double *atom_virial,
             virial[d0 * 3 + d1] -= tmp;
             atom_virial[(long)j_idx * 9 + d0 * 3 + d1] -= tmp;
           }
+      }
+    }
+  }
+}
+
+/* prod_force_grad_a_cpu: source/lib/src/prod_force_grad.cc:22-79 (the
+ * transpose of prod_force_a with the neighbor's force gradient read from the
+ * frame's local atoms, j -> j % nloc).  grad: [nframes, nloc, 3];
+ * grad_net: [nframes, nloc, 4 nnei]. */
+void oracle_prod_force_grad_a(double *grad_net, const double *grad,
+                              const double *env_deriv, const int *nlist,
+                              int nloc, int nnei, int nframes) {
+  const int ndescrpt = 4 * nnei;
+  for (long ii = 0; ii < (long)nframes * nloc; ++ii) {
+    const long kk = ii / nloc;
+    double *gn = grad_net + ii * ndescrpt;
+    const double *gi = grad + ii * 3;
+    for (int aa = 0; aa < ndescrpt; ++aa) {
+      const double *ed = env_deriv + ii * ndescrpt * 3 + aa * 3;
+      double acc = 0.;
+      for (int dd = 0; dd < 3; ++dd) acc -= gi[dd] * ed[dd];
+      gn[aa] = acc;
+    }
+    for (int jj = 0; jj < nnei; ++jj) {
+      int j_idx = nlist[ii * nnei + jj];
+      if (j_idx >= nloc) j_idx = j_idx % nloc;
+      if (j_idx < 0) continue;
+      const double *gj = grad + (kk * nloc + j_idx) * 3;
+      for (int aa = jj * 4; aa < jj * 4 + 4; ++aa) {
+        const double *ed = env_deriv + ii * ndescrpt * 3 + aa * 3;
+        for (int dd = 0; dd < 3; ++dd) gn[aa] += gj[dd] * ed[dd];
+      }
+    }
+  }
+}
+
+/* prod_virial_grad_a_cpu: source/lib/src/prod_virial_grad.cc:20-62 (one
+ * frame): grad_net[i, 4j+a] = sum_{d0,d1} grad[d0,d1] rij[i,j,d1]
+ * env_deriv[i,4j+a,d0] over valid neighbors, 0 elsewhere. */
+void oracle_prod_virial_grad_a(double *grad_net, const double *grad,
+                               const double *env_deriv, const double *rij,
+                               const int *nlist, int nloc, int nnei) {
+  const int ndescrpt = 4 * nnei;
+  memset(grad_net, 0, sizeof(double) * (size_t)nloc * ndescrpt);
+  for (long ii = 0; ii < nloc; ++ii) {
+    for (int jj = 0; jj < nnei; ++jj) {
+      if (nlist[ii * nnei + jj] < 0) continue;
+      const double *r = rij + ii * nnei * 3 + jj * 3;
+      for (int aa = jj * 4; aa < jj * 4 + 4; ++aa) {
+        const double *ed = env_deriv + ii * ndescrpt * 3 + aa * 3;
+        for (int d0 = 0; d0 < 3; ++d0)
+          for (int d1 = 0; d1 < 3; ++d1)
+            grad_net[ii * ndescrpt + aa] -= -1.0 * grad[d0 * 3 + d1] * r[d1] * ed[d0];
       }
     }
   }

@@ -12,7 +12,10 @@ A float64 torch-CPU restatement of the reference deepmd/pt model path:
 * fitting: deepmd/pt/model/task/fitting.py:734-900 (own-type net + bias_atom_e)
   and the out_bias of base_atomic_model.py:571-592;
 * energy/force/virial: deepmd/pt/model/model/transform_output.py:65-206,
-  317-383 (autograd force, F (x) x virial, ghost -> local fold).
+  317-383 (autograd force, F (x) x virial, ghost -> local fold);
+* training loss: deepmd/pt/loss/ener.py:243-271,302-328 (E + F mse) and its
+  parameter gradient by torch double backward; compressed: the C
+  tabulate_fusion_se_a_grad_grad as the backward of the tabulate grad.
 
 The neighbor list comes from the C oracle (reference C++ semantics).  Pinned
 by source/tests/infer/deeppot-testcase.yaml (reference tolerance: 12 decimal
@@ -69,9 +72,33 @@ class _Tabulate(torch.autograd.Function):
     @staticmethod
     def backward(ctx, dy):
         em_x, em = ctx.saved_tensors
-        dx, dem = oracle.tabulate_fusion_se_a_grad(ctx.table, ctx.info, em_x.detach().numpy(),
-                                                   em.detach().numpy(), dy.detach().numpy())
-        return torch.as_tensor(dx).reshape(em_x.shape), torch.as_tensor(dem), None, None, None
+        dx, dem = _TabulateGrad.apply(em_x, em, dy, ctx.table, ctx.info)
+        return dx, dem, None, None, None
+
+
+class _TabulateGrad(torch.autograd.Function):
+    """(dy_dem_x, dy_dem) of the compressed embedding; its own backward is
+    tabulate_fusion_se_a_grad_grad w.r.t. dy only, as the reference's
+    TabulateFusionSeAGradOp::backward (tabulate_multi_device.cc:669-700)."""
+
+    @staticmethod
+    def forward(ctx, em_x, em, dy, table, info):
+        ctx.save_for_backward(em_x, em)
+        ctx.table, ctx.info, ctx.M = table, info, dy.shape[-1]
+        dx, dem = oracle.tabulate_fusion_se_a_grad(table, info, em_x.detach().numpy(), em.detach().numpy(),
+                                                   dy.detach().numpy())
+        return torch.as_tensor(dx).reshape(em_x.shape), torch.as_tensor(dem)
+
+    @staticmethod
+    def backward(ctx, g_dx, g_dem):
+        em_x, em = ctx.saved_tensors
+        g_dx = torch.zeros_like(em_x) if g_dx is None else g_dx
+        g_dem = torch.zeros_like(em) if g_dem is None else g_dem
+        dz = oracle.tabulate_fusion_se_a_grad_grad(ctx.table, ctx.info, em_x.detach().numpy(),
+                                                   em.detach().numpy(),
+                                                   g_dx.detach().numpy().reshape(em.shape[0], em.shape[1]),
+                                                   g_dem.detach().numpy(), ctx.M)
+        return None, None, torch.as_tensor(dz), None, None
 
 
 class Model:
@@ -157,6 +184,64 @@ class Model:
             y = y[:, 0] + self.bias_atom_e[t]
             e = torch.where(at == t, y, e)
         return e + self.out_bias[at]
+
+    # ------------------------------------------------------------------ training
+    def named_parameters(self):
+        """[((kind, net, layer, name), tensor)] for w, b, idt of every
+        embedding layer (none with tables: the compressed embedding is fixed)
+        and every fitting layer, made leaves that require grad."""
+        groups = [("fitting", self.fit)]
+        if self.tables is None:
+            groups.insert(0, ("embedding", self.emb))
+        out = []
+        for kind, nets in groups:
+            for n, net in enumerate(nets):
+                for k, l in enumerate(net):
+                    for name in ("w", "b", "idt"):
+                        t = getattr(l, name)
+                        if t is not None:
+                            t = t.detach().clone().requires_grad_(True)
+                            setattr(l, name, t)
+                            out.append(((kind, n, k, name), t))
+        return out
+
+    def parameters(self):
+        return [t for _, t in self.named_parameters()]
+
+    def energy_force(self, coord, atype, box, create_graph=False):
+        """E (fp64 scalar tensor) and F [nloc, 3] = -dE/dx with ghost forces
+        folded to their owners (transform_output.py:152-206,317-383)."""
+        coord = np.asarray(coord, dtype=np.float64).reshape(-1, 3)
+        atype = np.asarray(atype, dtype=np.int32)
+        if box is None:
+            ec, et, mp, nlist, _ = oracle.build_nlist_nopbc(coord, atype, self.rcut, self.sel)
+        else:
+            ec, et, mp, nlist, _ = oracle.build_nlist_pbc(coord, atype, box, self.rcut, self.sel)
+        ext = torch.tensor(ec, requires_grad=True)
+        energy = self.atom_energy(self.descriptor(self.env_mat(ext, nlist, et)), atype).sum()
+        (g,) = torch.autograd.grad(energy, ext, create_graph=create_graph)
+        f = torch.zeros((len(atype), 3), dtype=torch.float64).index_add(0, torch.as_tensor(mp, dtype=torch.long), -g)
+        return energy, f
+
+    def train_loss(self, frames, energy_label, force_label, pref_e=0.02, pref_f=1000.0, params=None):
+        """EnergyStdLoss (deepmd/pt/loss/ener.py:243-271,302-328; mse, 1/N
+        energy normalisation) over frames [(coord, atype, box)], and with
+        params (from parameters()) its gradient by a double backward."""
+        es, fs = [], []
+        for c, t, b in frames:
+            e, f = self.energy_force(c, t, b, create_graph=params is not None)
+            es.append(e)
+            fs.append(f)
+        energy = torch.stack(es)
+        force = torch.stack(fs)
+        natoms = force.shape[1]
+        l2_e = torch.mean(torch.square(energy - torch.as_tensor(np.asarray(energy_label, dtype=np.float64))))
+        l2_f = torch.mean(torch.square((torch.as_tensor(np.asarray(force_label, dtype=np.float64)) - force).reshape(-1)))
+        loss = (1.0 / natoms) * (pref_e * l2_e) + pref_f * l2_f
+        if params is None:
+            return float(loss.item()), None
+        grads = torch.autograd.grad(loss, params)
+        return float(loss.item()), [g.detach().numpy() for g in grads]
 
     # ------------------------------------------------------------------ E, F, V
     def evaluate(self, coord, atype, box):

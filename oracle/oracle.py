@@ -63,6 +63,8 @@ def lib():
         L.oracle_tabulate_fusion_se_a_grad_grad.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _i, _i, _i]
         L.oracle_prod_force_a.argtypes = [_dp, _dp, _dp, _ip, _i, _i, _i, _i]
         L.oracle_prod_virial_a.argtypes = [_dp, _dp, _dp, _dp, _dp, _ip, _i, _i, _i]
+        L.oracle_prod_force_grad_a.argtypes = [_dp, _dp, _dp, _ip, _i, _i, _i]
+        L.oracle_prod_virial_grad_a.argtypes = [_dp, _dp, _dp, _dp, _ip, _i, _i]
         L.oracle_num_threads.restype = _i
         _lib = L
     return _lib
@@ -251,6 +253,28 @@ def prod_force_a(net_deriv, env_deriv, nlist, nall, nframes=1):
                               _p(_c(env_deriv, np.float64), _dp), _p(nlist, _ip), nloc, nall,
                               nnei, nframes)
     return f
+
+
+def prod_force_grad_a(grad, env_deriv, nlist, nframes=1):
+    """grad [nframes, nloc, 3] -> grad_net [nframes * nloc, 4 nnei]."""
+    nlist = _c(nlist, np.int32)
+    nnei = nlist.shape[-1]
+    nloc = nlist.size // nnei // nframes
+    gn = np.zeros((nframes * nloc, 4 * nnei), np.float64)
+    lib().oracle_prod_force_grad_a(_p(gn, _dp), _p(_c(grad, np.float64), _dp), _p(_c(env_deriv, np.float64), _dp),
+                                   _p(nlist, _ip), nloc, nnei, nframes)
+    return gn
+
+
+def prod_virial_grad_a(grad, env_deriv, rij, nlist):
+    """grad [9] -> grad_net [nloc, 4 nnei] (one frame)."""
+    nlist = _c(nlist, np.int32)
+    nnei = nlist.shape[-1]
+    nloc = nlist.size // nnei
+    gn = np.zeros((nloc, 4 * nnei), np.float64)
+    lib().oracle_prod_virial_grad_a(_p(gn, _dp), _p(_c(grad, np.float64), _dp), _p(_c(env_deriv, np.float64), _dp),
+                                    _p(_c(rij, np.float64), _dp), _p(nlist, _ip), nloc, nnei)
+    return gn
 
 
 def prod_virial_a(net_deriv, env_deriv, rij, nlist, nall):

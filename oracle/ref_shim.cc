@@ -14,7 +14,9 @@
 #include "neighbor_list.h"
 #include "prod_env_mat.h"
 #include "prod_force.h"
+#include "prod_force_grad.h"
 #include "prod_virial.h"
+#include "prod_virial_grad.h"
 #include "region.h"
 #include "tabulate.h"
 
@@ -124,5 +126,21 @@ void ref_prod_virial_a(double* virial, double* atom_virial,
                        int nnei) {
   deepmd::prod_virial_a_cpu(virial, atom_virial, net_deriv, env_deriv, rij,
                             nlist, nloc, nall, nnei);
+}
+
+// deepmd::prod_force_grad_a_cpu (source/lib/src/prod_force_grad.cc:22-79)
+void ref_prod_force_grad_a(double* grad_net, const double* grad,
+                           const double* env_deriv, const int* nlist, int nloc,
+                           int nnei, int nframes) {
+  deepmd::prod_force_grad_a_cpu(grad_net, grad, env_deriv, nlist, nloc, nnei,
+                                nframes);
+}
+
+// deepmd::prod_virial_grad_a_cpu (source/lib/src/prod_virial_grad.cc:20-62)
+void ref_prod_virial_grad_a(double* grad_net, const double* grad,
+                            const double* env_deriv, const double* rij,
+                            const int* nlist, int nloc, int nnei) {
+  deepmd::prod_virial_grad_a_cpu(grad_net, grad, env_deriv, rij, nlist, nloc,
+                                 nnei);
 }
 }

@@ -8,6 +8,8 @@ only data: the input arrays and expected outputs its tests assert against.
     source/lib/tests/test_tabulate_se_a.cc  -> tabulate_se_a.json
     source/lib/tests/test_prod_force_a.cc   -> prod_force_a.json
     source/lib/tests/test_prod_virial_a.cc  -> prod_virial_a.json
+    source/lib/tests/test_prod_force_grad_a.cc  -> prod_force_grad_a.json
+    source/lib/tests/test_prod_virial_grad_a.cc -> prod_virial_grad_a.json
     source/lib/tests/test_coord.cc          -> coord.json
     source/tests/pt/test_tabulate_fusion_se_a.py -> tabulate_fusion_se_a_pt.json
     source/tests/infer/deeppot_sea.yaml     -> deeppot_sea_model.json
@@ -90,6 +92,8 @@ def main(ref: Path) -> None:
         "tabulate_se_a.json": cpp_classes(t / "test_tabulate_se_a.cc"),
         "prod_force_a.json": cpp_classes(t / "test_prod_force_a.cc"),
         "prod_virial_a.json": cpp_classes(t / "test_prod_virial_a.cc"),
+        "prod_force_grad_a.json": cpp_classes(t / "test_prod_force_grad_a.cc"),
+        "prod_virial_grad_a.json": cpp_classes(t / "test_prod_virial_grad_a.cc"),
         "coord.json": cpp_classes(t / "test_coord.cc"),
         "tabulate_fusion_se_a_pt.json": py_tensors(ref / "source/tests/pt/test_tabulate_fusion_se_a.py"),
     }

@@ -226,6 +226,60 @@ def test_prod_virial_golden(oracle, golden):
     np.testing.assert_allclose(av.sum(axis=0), v, atol=1e-10)
 
 
+def _grad_case(oracle, golden, name):
+    cls = "TestProdForceGradA" if name == "prod_force_grad_a" else "TestProdVirialGradA"
+    g = golden(f"{name}.json")[cls]
+    posi = np.array(g["posi"]).reshape(-1, 3)
+    atype = np.array(g["atype"], np.int32)
+    box = np.diag([13.0, 13.0, 13.0]).ravel()
+    sel = [5, 5]
+    ec, et, _, nlist, _ = oracle.build_nlist_pbc(posi, atype, box, g["rc"], sel)
+    em, ed, rij = oracle.env_mat_a(nlist, ec, et, np.zeros((2, 10, 4)), np.ones((2, 10, 4)), g["rc"],
+                                   g["rc_smth"], sel)
+    return g, ec, nlist, ed, rij
+
+
+def test_prod_force_grad_golden(oracle, golden):
+    g, ec, nlist, ed, rij = _grad_case(oracle, golden, "prod_force_grad_a")
+    nloc = 6
+    # grad[i] = 10 - 0.1 i, two identical frames (test_prod_force_grad_a.cc:116-124)
+    grad = 10 - np.arange(nloc * 3) * 0.1
+    gn = oracle.prod_force_grad_a(np.concatenate([grad, grad]), np.concatenate([ed.ravel(), ed.ravel()]),
+                                  np.concatenate([nlist, nlist]), nframes=2)
+    np.testing.assert_allclose(gn.ravel(), g["expected_grad_net"] * 2, atol=1e-5, rtol=0)
+
+
+def test_prod_virial_grad_golden(oracle, golden):
+    g, ec, nlist, ed, rij = _grad_case(oracle, golden, "prod_virial_grad_a")
+    grad = 10 - np.arange(9) * 1.0  # test_prod_virial_grad_a.cc:116-119
+    gn = oracle.prod_virial_grad_a(grad, ed, rij, nlist)
+    np.testing.assert_allclose(gn.ravel(), g["expected_grad_net"], atol=1e-5, rtol=0)
+
+
+def test_prod_grads_are_adjoints(oracle):
+    """<g, prod_force(nd)> = <prod_force_grad(g), nd> and likewise for the
+    virial, on a mapped (local-index) list as the training step uses."""
+    from dpmi import systems
+
+    s = systems.water192(3)
+    sel = [46, 92]
+    ec, et, mp, nlist, _ = oracle.build_nlist_pbc(s.coord, s.atype, s.box, 6.0, sel)
+    em, ed, rij = oracle.env_mat_a(nlist, ec, et, np.zeros((2, 138, 4)), np.ones((2, 138, 4)), 6.0, 0.5, sel)
+    nlm = np.where(nlist >= 0, mp[np.maximum(nlist, 0)], -1).astype(np.int32)
+    rng = np.random.default_rng(2)
+    nd = rng.normal(size=(s.nloc, 4 * 138))
+    gf = rng.normal(size=(s.nloc, 3))
+    f = oracle.prod_force_a(nd, ed, nlm, s.nloc)[0]
+    lhs = np.sum(gf * f)
+    rhs = np.sum(oracle.prod_force_grad_a(gf, ed, nlm) * nd)
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+    gv = rng.normal(size=9)
+    v, _ = oracle.prod_virial_a(nd, ed, rij, nlm, s.nloc)
+    lhs = np.sum(gv * v)
+    rhs = np.sum(oracle.prod_virial_grad_a(gv, ed, rij, nlm) * nd)
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
 def test_cell_candidates_equal_brute(oracle):
     """The O(N) candidate search finds exactly the brute-force set."""
     from dpmi import systems

@@ -106,6 +106,15 @@ def test_oracle_equals_reference_bitwise(oracle, ref, name, make, rcut, rs, sel)
     np.testing.assert_allclose(v, v_ref, rtol=1e-12, atol=1e-10)
     np.testing.assert_allclose(av, av_ref, rtol=1e-12, atol=1e-10)
 
+    # their transposes (the training-step ops), same order of operations
+    gf = rng.normal(size=(s.nloc, 3))
+    gn_ref = np.zeros((s.nloc, sum(sel) * 4))
+    ref.ref_prod_force_grad_a(P(gn_ref), P(gf), P(ed), PI(nlist), s.nloc, sum(sel), 1)
+    np.testing.assert_array_equal(oracle.prod_force_grad_a(gf, ed, nlist), gn_ref)
+    gv = rng.normal(size=9)
+    ref.ref_prod_virial_grad_a(P(gn_ref), P(gv), P(ed), P(rij), PI(nlist), s.nloc, sum(sel))
+    np.testing.assert_array_equal(oracle.prod_virial_grad_a(gv, ed, rij, nlist), gn_ref)
+
 
 def _random_table(rng, nspline, M):
     return rng.normal(size=(nspline, 6 * M)) * np.array([1, 0.5, 0.2, 0.1, 0.05, 0.02] * M)
