@@ -109,6 +109,9 @@ class Net:
 
 
 class SeAEnergyModel:
+    # neighbor rows per block of the FP64/FP32 network evaluation
+    ROWS_PER_BLOCK = 1 << 22
+
     def __init__(self, rcut, rcut_smth, sel, axis, davg, dstd, emb, fit, bias_atom_e, out_bias=None,
                  device="cuda", precision: str = "fp64", tables=None):
         if precision not in ("fp64", "fp32", "bf16", "fp16"):
@@ -249,16 +252,26 @@ class SeAEnergyModel:
         ext_c, ext_t, mapping, nlist, _ovf, _, cells = self.neighbor_list(coord, atype, box)
         if self.fused is not None:
             return self._compute_fused(atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy)
-        em = self.env_mat(nlist, ext_c, ext_t).requires_grad_(True)
-        with torch.enable_grad():
-            desc = self.descriptor(em)
-            ae = self.atom_energy(desc, atype)
-            energy = ae.sum()
-            (net_deriv,) = torch.autograd.grad(energy, em)
+        em = self.env_mat(nlist, ext_c, ext_t)
+        # atoms in blocks: an atom's energy and dE/d em depend on its own em
+        # rows only, so the network's working set stays bounded (about
+        # ROWS_PER_BLOCK neighbor rows of every layer width)
+        nloc = em.shape[0]
+        blk = max(256, self.ROWS_PER_BLOCK // max(self.nnei, 1))
+        net_deriv = torch.empty_like(em)
+        ae = torch.empty(nloc, dtype=torch.float64, device=em.device)
+        for a in range(0, nloc, blk):
+            b = min(nloc, a + blk)
+            em_b = em[a:b].detach().requires_grad_(True)
+            with torch.enable_grad():
+                ae_b = self.atom_energy(self.descriptor(em_b), atype[a:b])
+                (net_deriv[a:b],) = torch.autograd.grad(ae_b.sum(), em_b)
+            ae[a:b] = ae_b.detach()
+        energy = ae.sum()
         force, virial = self.forces(net_deriv, nlist, cells, mapping, ext_c, ext_t, box)
-        out = {"energy": energy.detach(), "force": force, "virial": virial}
+        out = {"energy": energy, "force": force, "virial": virial}
         if need_atom_energy:
-            out["atom_energy"] = ae.detach()
+            out["atom_energy"] = ae
         return out
 
     def _compute_fused(self, atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy):

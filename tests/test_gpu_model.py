@@ -124,3 +124,52 @@ def test_tabulated_fp64_model_matches_oracle():
     df = torch.sqrt(((full["force"] - out["force"]) ** 2).mean()).item()
     de = abs(full["energy"].item() - out["energy"].item()) / s.nloc
     print(f"table vs network: force RMSE {df:.3e} eV/A, |dE|/atom {de:.3e} eV", flush=True)
+
+
+@pytest.fixture(scope="module")
+def alloy_model():
+    from dpmi.stat import gpu_model_params
+
+    spec = systems.ALLOY_SPEC
+    return spec, gpu_model_params(spec, systems.alloy262k(4, ncell=8))
+
+
+def test_alloy_fp64_matches_oracle(alloy_model):
+    """Four-type FCC alloy (triclinic box, rcut 8, sel [60]*4, embedding
+    [32, 64, 128]) at 1,000 atoms, FP64: E, F, V within 1e-10 relative."""
+    from oracle.model_ref import Model
+
+    from dpmi.model import SeAEnergyModel
+
+    spec, params = alloy_model
+    s = systems.alloy262k(5, ncell=10)
+    _say("fp64 model alloy1000 vs oracle")
+    out = SeAEnergyModel.from_params(spec, params, device=DEV).compute(
+        torch.tensor(s.coord, device=DEV), torch.tensor(s.atype, device=DEV), s.box)
+    o = Model.from_params(spec, params).evaluate(s.coord, s.atype, s.box)
+    assert abs(out["energy"].item() - o["energy"]) <= 1e-10 * abs(o["energy"])
+    np.testing.assert_allclose(out["force"].cpu().numpy(), o["force"], rtol=1e-10,
+                               atol=1e-10 * np.abs(o["force"]).max())
+    np.testing.assert_allclose(out["virial"].cpu().numpy(), o["virial"], rtol=1e-10,
+                               atol=1e-10 * np.abs(o["virial"]).max())
+
+
+def test_alloy_bf16_error_budget(alloy_model):
+    """BF16-MFMA form on the alloy (4,096 atoms): force RMSE <= 1e-3 eV/A
+    against FP64; energy/atom and virial errors reported."""
+    from dpmi.model import SeAEnergyModel
+
+    spec, params = alloy_model
+    s = systems.alloy262k(4, ncell=16)
+    _say("bf16 model alloy4096 error")
+    c = torch.tensor(s.coord, device=DEV)
+    t = torch.tensor(s.atype, device=DEV)
+    a = SeAEnergyModel.from_params(spec, params, device=DEV).compute(c, t, s.box)
+    m = SeAEnergyModel.from_params(spec, params, device=DEV, precision="bf16")
+    assert m.fused is not None and m.fused_fit is not None
+    b = m.compute(c, t, s.box)
+    rmse = torch.sqrt(((a["force"] - b["force"]) ** 2).mean()).item()
+    de = abs(a["energy"].item() - b["energy"].item()) / s.nloc
+    dv = (torch.abs(a["virial"] - b["virial"]).max() / torch.abs(a["virial"]).max()).item()
+    print(f"alloy bf16: force RMSE {rmse:.3e} eV/A, |dE|/atom {de:.3e} eV, virial max rel {dv:.3e}", flush=True)
+    assert rmse <= 1e-3
