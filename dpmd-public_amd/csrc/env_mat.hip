@@ -458,13 +458,15 @@ __global__ void __launch_bounds__(256)
 // centre terms and the virial are as in k_force_virial_recompute.  Atoms or
 // cells the search left to its per-atom path (cand_slot < 0, or a table
 // over the table capacity) add their neighbor terms with atomics instead.
-// Each wavefront accumulates into its own LDS copy of the table (a wave's
-// lanes never share a slot, so plain adds in program order suffice) and the
-// copies are summed in a fixed order: the result does not depend on timing.
-// The first pass takes every cell with tables up to kCandCap; cells the
-// search moved to its second pass (bigger tables) are left to a second force
-// pass (kBigPass, fp32: 4 x 3072 x 12 B of LDS) or, in FP64 (4 x 1536 x 24 B
-// already), processed with neighbor atomics.
+// The table sums do not depend on timing: in fp32 one table of 64-bit
+// fixed-point sums (2^-32 resolution, |term| < 2^31) takes integer LDS atomics,
+// which are exact and so order-free; in fp64 each wavefront accumulates into
+// its own copy (a wave's lanes never share a slot, so plain adds in program
+// order suffice) and the copies are summed in a fixed order.  The first pass
+// takes every cell with tables up to kCandCap; cells the search moved to its
+// second pass (bigger tables) are left to a second force pass (kBigPass, fp32:
+// 3072 x 24 B of LDS) or, in FP64 (4 x 1536 x 24 B already), processed with
+// neighbor atomics.
 template <typename FP, int CAP, bool kBigPass>
 __global__ void __launch_bounds__(256)
     k_force_cells(double* __restrict__ force, FP* __restrict__ partial, int* __restrict__ cell_ok,
@@ -476,8 +478,13 @@ __global__ void __launch_bounds__(256)
   __shared__ int lim_all[kAtomsPerBlock][kMaxTypes];
   extern __shared__ unsigned char smem_raw[];
   constexpr int kCap = CAP;
-  constexpr bool kLeaveBig = !kBigPass && sizeof(FP) == 4;  // a second pass takes the big cells
-  FP* acc = reinterpret_cast<FP*>(smem_raw);  // [4 wavefronts][kCap][3]
+  constexpr bool kFixed = sizeof(FP) == 4;   // fixed-point table (fp32) or per-wave copies (fp64)
+  constexpr bool kLeaveBig = !kBigPass && kFixed;  // a second pass takes the big cells
+  constexpr float kFix = 4294967296.0f;      // 2^32
+  constexpr double kUnfix = 1.0 / 4294967296.0;
+  FP* acc = reinterpret_cast<FP*>(smem_raw);  // fp64: [4 wavefronts][kCap][3]
+  unsigned long long* accq = reinterpret_cast<unsigned long long*>(smem_raw);  // fp32: [kCap][3]
+  constexpr int kAccWords = kFixed ? 3 * kCap : kAtomsPerBlock * 3 * kCap;
   const GridParams p = *reinterpret_cast<const GridParams*>(nlws);
   const WsLayout L = ws_layout(p.nloc, p.lay_nlc, p.lay_next);
   const int* __restrict__ cell_start = reinterpret_cast<const int*>(nlws + L.cell_start);
@@ -503,8 +510,12 @@ __global__ void __launch_bounds__(256)
     const bool ok = T <= kCap;
     if (kLeaveBig && !ok) continue;  // uniform; the second pass writes cell_ok
     if (tid == 0) cell_ok[cc] = ok ? 1 : 0;
-    if (ok)
-      for (int k = tid; k < kAtomsPerBlock * 3 * kCap; k += blockDim.x) acc[k] = (FP)0.;
+    if (ok) {
+      if constexpr (kFixed)
+        for (int k = tid; k < kAccWords; k += blockDim.x) accq[k] = 0ull;
+      else
+        for (int k = tid; k < kAccWords; k += blockDim.x) acc[k] = (FP)0.;
+    }
     __syncthreads();
     FP* accw = acc + w * 3 * kCap;
     for (int ai = w; ai < na; ai += kAtomsPerBlock) {
@@ -540,9 +551,15 @@ __global__ void __launch_bounds__(256)
           for (int d1 = 0; d1 < 3; ++d1) v[d0 * 3 + d1] += (double)g[d0] * r64[d1];
         const int cs = ok ? (int)cand_slot[(int64_t)i * nnei + s] : -1;
         if (cs >= 0 && cs < T) {
-          accw[3 * cs + 0] += g[0];
-          accw[3 * cs + 1] += g[1];
-          accw[3 * cs + 2] += g[2];
+          if constexpr (kFixed) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+              atomicAdd(&accq[3 * cs + k], (unsigned long long)__float2ll_rn((float)g[k] * kFix));
+          } else {
+            accw[3 * cs + 0] += g[0];
+            accw[3 * cs + 1] += g[1];
+            accw[3 * cs + 2] += g[2];
+          }
         } else {
           const int o = mapping ? mapping[j] : j;
           atomicAdd(&force[3 * (int64_t)o + 0], (double)g[0]);
@@ -565,9 +582,14 @@ __global__ void __launch_bounds__(256)
         FP* dst = partial + ((int64_t)o * nnb + t) * 3;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          FP x = acc[3 * g + k];
+          FP x;
+          if constexpr (kFixed) {
+            x = (FP)((double)(long long)accq[3 * g + k] * kUnfix);
+          } else {
+            x = acc[3 * g + k];
 #pragma unroll
-          for (int ww = 1; ww < kAtomsPerBlock; ++ww) x += acc[ww * 3 * kCap + 3 * g + k];
+            for (int ww = 1; ww < kAtomsPerBlock; ++ww) x += acc[ww * 3 * kCap + 3 * g + k];
+          }
           dst[k] = x;
         }
       }
@@ -831,7 +853,7 @@ extern "C" int dpmi_force_virial_recompute_f32(double* force, double* virial, co
 namespace dpmi {
 namespace {
 
-constexpr int kBigBlocks = 256;  // workgroups of the second (big-cell) force pass
+constexpr int kBigBlocks = 512;  // workgroups of the second (big-cell) force pass
 
 int64_t force_cells_ws(const double* box_host, int64_t nloc, float rcut, int64_t elem, int64_t* nnb_out) {
   int64_t nlc;
@@ -869,8 +891,10 @@ int force_cells_impl(double* force, double* virial, const FP* net_deriv, const i
   cell_grid_info(box_host, nloc, rcut, &nlc, &ns);
   double* vpartial = reinterpret_cast<double*>(reinterpret_cast<char*>(cell_ok) + align256(4 * nlc));
   const int bpf = (int)std::min<int64_t>(kForceBlocks, std::max<int64_t>(1, nloc));
-  const size_t shm = (size_t)kAtomsPerBlock * kCandCap * 3 * sizeof(FP);
-  const size_t shm_big = (size_t)kAtomsPerBlock * kCandCapBig * 3 * sizeof(FP);
+  // LDS table: fp32 one fixed-point copy (8 B per component), fp64 one copy per wavefront
+  const size_t per_slot = sizeof(FP) == 4 ? 3 * sizeof(unsigned long long) : kAtomsPerBlock * 3 * sizeof(FP);
+  const size_t shm = (size_t)kCandCap * per_slot;
+  const size_t shm_big = (size_t)kCandCapBig * per_slot;
   if (shm > 65536)
     DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_force_cells<FP, kCandCap, false>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));

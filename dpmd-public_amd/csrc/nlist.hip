@@ -477,6 +477,36 @@ __global__ void __launch_bounds__(64) k_search_format(
 // kCandCap atoms, is appended to the redo list for k_search_format.
 constexpr int kKeyCap = 512;  // plus 8 padding slots per wavefront
 constexpr size_t kMaxCellsLds = 150 * 1024;  // dynamic LDS the cell-wise search may take
+constexpr int kBadType = 127;                // packed type code of a candidate that never passes
+
+// Bound on |x| of any extended coordinate: the box grown by the ghost cells
+// (periodic), or the grid of an open system.
+__device__ inline double coord_bound(const GridParams& p) {
+  double xm = 0.;
+  for (int d = 0; d < 3; ++d) {
+    double x = 0.;
+    if (p.pbc) {
+      for (int i = 0; i < 3; ++i) x += (1.0 + (double)p.ngcell[i] / p.ncell[i]) * fabs(p.box[3 * i + d]);
+    } else {
+      x = fmax(fabs(p.lo[d]), fabs(p.lo[d] + p.ncell[d] * p.cs[d]));
+    }
+    xm = fmax(xm, x);
+  }
+  return xm * 1.001 + 1e-30;
+}
+
+// Width of the band below rc^2 in which the float r^2 of float-rounded
+// coordinates cannot decide the double test |xi - xj|^2 < rc^2.  Each
+// float coordinate is within ulp(X)/2 of the double one and the float
+// difference adds at most ulp(rc)/2, so every component differs by at most
+// dl = ulp(X) + ulp(rc)/2; then |rr2 - r2| <= 2 sqrt(3) rc dl + 3 dl^2 plus
+// the rounding of the float squares and sums (<= 3 ulp(rc^2)).  Twice that.
+__device__ inline double screen_margin(double xmax, float rcut) {
+  const double ux = ldexp(1.0, ilogb(xmax) - 23), urc = ldexp(1.0, ilogb((double)rcut) - 23);
+  const double url = ldexp(1.0, ilogb((double)rcut * rcut) - 23);
+  const double dl = ux + 0.5 * urc;
+  return 2.0 * (3.5 * (double)rcut * dl + 3.0 * dl * dl + 3.0 * url);
+}
 
 template <int CAP, bool kBigPass>
 __global__ void __launch_bounds__(256) k_search_cells(
@@ -484,12 +514,9 @@ __global__ void __launch_bounds__(256) k_search_cells(
     int16_t* __restrict__ cand_slot, const double* __restrict__ ext_c, const int* __restrict__ ext_t,
     const int* __restrict__ mapping, char* __restrict__ ws, SelParams sp, float rcut) {
   extern __shared__ unsigned char smem_raw[];
-  double* cx = reinterpret_cast<double*>(smem_raw);
-  double* cy = cx + CAP;
-  double* cz = cy + CAP;
-  int* cj = reinterpret_cast<int*>(cz + CAP);
-  int* ct = cj + CAP;
-  uint64_t* keys_all = reinterpret_cast<uint64_t*>(ct + CAP);
+  // candidate g: float coordinates and (type code << 24 | extended index)
+  float4* cand = reinterpret_cast<float4*>(smem_raw);
+  uint64_t* keys_all = reinterpret_cast<uint64_t*>(cand + CAP);
   int* rows_all = reinterpret_cast<int*>(keys_all + 4 * (kKeyCap + 8));
   int16_t* kslot_all = reinterpret_cast<int16_t*>(rows_all + 4 * sp.nnei);
   int16_t* rslot_all = kslot_all + 4 * (kKeyCap + 8);
@@ -506,6 +533,7 @@ __global__ void __launch_bounds__(256) k_search_cells(
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const double rcut2d = (double)(rcut * rcut);
   const float rcut2f = rcut * rcut;
+  const double rsure = rcut2d - screen_margin(coord_bound(p), rcut);
   const int ns = p.nsearch, side = 2 * ns + 1, nnb = side * side * side;
   const int ncells = kBigPass ? misc[3] : p.ncell[0] * p.ncell[1] * p.ncell[2];
   uint64_t* keys = keys_all + w * (kKeyCap + 8);
@@ -530,11 +558,10 @@ __global__ void __launch_bounds__(256) k_search_cells(
     for (int g = tid; g < T; g += blockDim.x) {
       int t;
       const int j = nb_candidate(g, nnb, s_base, s_local, s_pref, cell_atoms, &t);
-      cj[g] = j;
-      ct[g] = ext_t[j];
-      cx[g] = ext_c[3 * j + 0];
-      cy[g] = ext_c[3 * j + 1];
-      cz[g] = ext_c[3 * j + 2];
+      const int tj = ext_t[j];
+      const int code = (tj >= 0 && tj < sp.ntypes) ? tj : kBadType;
+      cand[g] = make_float4((float)ext_c[3 * j + 0], (float)ext_c[3 * j + 1], (float)ext_c[3 * j + 2],
+                            __int_as_float((code << 24) | j));
     }
     __syncthreads();
     for (int ai = w; ai < na; ai += 4) {
@@ -547,19 +574,23 @@ __global__ void __launch_bounds__(256) k_search_cells(
         bool pass = false;
         uint64_t key = 0;
         if (g < T) {
-          const int j = cj[g];
-          if (j != i) {
-            const double xj0 = cx[g], xj1 = cy[g], xj2 = cz[g];
-            const double e0 = xi0 - xj0, e1 = xi1 - xj1, e2 = xi2 - xj2;
-            const double r2 = e0 * e0 + e1 * e1 + e2 * e2;
-            if (r2 < rcut2d) {
-              const int tj = ct[g];
-              const float f0 = (float)xj0 - fi0, f1 = (float)xj1 - fi1, f2 = (float)xj2 - fi2;
-              const float rr2 = f0 * f0 + f1 * f1 + f2 * f2;
-              if (tj >= 0 && tj < sp.ntypes && rr2 <= rcut2f) {
-                pass = true;
-                key = ((uint64_t)tj << 56) | ((uint64_t)__float_as_uint(rr2) << 24) | (uint64_t)j;
-              }
+          const float4 c = cand[g];
+          const int packed = __float_as_int(c.w);
+          const int j = packed & 0xFFFFFF, tj = packed >> 24;
+          // the float test of format_nlist_i_cpu (exact: the same float
+          // coordinates), then the double test of build_nlist_cpu, which the
+          // float r^2 decides outside a narrow band below rc^2
+          const float f0 = c.x - fi0, f1 = c.y - fi1, f2 = c.z - fi2;
+          const float rr2 = f0 * f0 + f1 * f1 + f2 * f2;
+          if (j != i && tj != kBadType && rr2 <= rcut2f) {
+            bool in = (double)rr2 < rsure;
+            if (!in) {
+              const double e0 = xi0 - ext_c[3 * j + 0], e1 = xi1 - ext_c[3 * j + 1], e2 = xi2 - ext_c[3 * j + 2];
+              in = e0 * e0 + e1 * e1 + e2 * e2 < rcut2d;
+            }
+            if (in) {
+              pass = true;
+              key = ((uint64_t)tj << 56) | ((uint64_t)__float_as_uint(rr2) << 24) | (uint64_t)j;
             }
           }
         }
@@ -789,7 +820,7 @@ extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped
   // status words misc[1..3] (capacity error, redo count, big-cell count)
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, misc + 1, (int64_t)3, 0);
   auto shm_for = [&](int cap) {
-    return (size_t)cap * (3 * sizeof(double) + 2 * sizeof(int)) +
+    return (size_t)cap * sizeof(float4) +
            4 * ((size_t)(kKeyCap + 8) * (sizeof(uint64_t) + sizeof(int16_t)) + (sizeof(int) + sizeof(int16_t)) * sp.nnei);
   };
   const size_t shm_cells = shm_for(kCandCap), shm_big = shm_for(kCandCapBig);
@@ -804,7 +835,7 @@ extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped
     hipLaunchKernelGGL((k_search_cells<kCandCap, false>), dim3((unsigned)std::min<int64_t>(nloc, 2048)), dim3(256),
                        shm_cells, st, nlist, overflow, nlist_mapped, cand_slot, ext_coord, ext_type, mapping, ws, sp,
                        rcut);
-    hipLaunchKernelGGL((k_search_cells<kCandCapBig, true>), dim3((unsigned)std::min<int64_t>(nloc, 1024)), dim3(256),
+    hipLaunchKernelGGL((k_search_cells<kCandCapBig, true>), dim3((unsigned)std::min<int64_t>(nloc, 2048)), dim3(256),
                        shm_big, st, nlist, overflow, nlist_mapped, cand_slot, ext_coord, ext_type, mapping, ws, sp,
                        rcut);
   } else {

@@ -41,8 +41,23 @@ def _clustered():
     return systems.System("clustered", coord, atype, np.diag([40.0, 40.0, 40.0]).ravel())
 
 
+def _lattice_at_rc(jitter: float):
+    """Simple cubic lattice, spacing 3 A, 16^3 sites, rc = 6 A: every atom has
+    six neighbors at exactly rc (excluded by the strict double test), and with
+    a 1e-6 A jitter pairs fall on both sides of rc within float rounding."""
+    rng = np.random.default_rng(13)
+    g = np.arange(16) * 3.0
+    c = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    c = c + jitter * rng.uniform(-1, 1, c.shape)
+    atype = rng.integers(0, 2, c.shape[0]).astype(np.int32)
+    return systems.System("lattice", c, atype, np.diag([48.0, 48.0, 48.0]).ravel())
+
+
 NLIST_CASES = [
     ("water192", lambda: systems.water192(0), 6.0, [46, 92]),
+    # neighbors at exactly rc, and within float rounding of it
+    ("lattice_at_rc", lambda: _lattice_at_rc(0.0), 6.0, [40, 40]),
+    ("lattice_jitter", lambda: _lattice_at_rc(1e-6), 6.0, [40, 40]),
     # neighborhoods larger than the cell-wise search's LDS candidate buffer
     ("water12k_rcut9", lambda: systems.water12k(1), 9.0, [110, 220]),
     ("clustered", _clustered, 6.0, [200, 200]),
