@@ -115,6 +115,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="calibration for HBM counters: only run 5 device-to-device copies of this many bytes")
+    ap.add_argument("--form", default="precision", choices=["precision", "tabulated", "train"],
+                    help="precision: FP64 plain vs the bf16/fp16 form (default); tabulated: the compressed "
+                         "FP64 model (use with --workload water12k); train: the 8 x 192 E+F training step")
     args = ap.parse_args()
     if args.copy_bytes:
         return copy_calibration(args.copy_bytes)
@@ -124,6 +127,11 @@ def main() -> int:
               f"{max(args.gpus, world)} is not run",
               file=sys.stderr)
         return 2
+
+    if args.form == "tabulated":
+        return bench_tabulated(args)
+    if args.form == "train":
+        return bench_train(args)
 
     import numpy as np
     import torch
@@ -242,6 +250,113 @@ def main() -> int:
 
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(spec, params, args.cpu_seconds, wl["spec"])
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def _line(args, value, ms, dtype, config, **extra) -> dict:
+    return {"metric": METRIC, "value": round(value, 1), "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded SplitMix64 lattice, random-init weights)",
+            "config": config, **extra}
+
+
+def bench_tabulated(args) -> int:
+    """The compressed FP64 model: E+F+V steps with the tabulated embedding
+    (tables from the model, grid [0, 2) / 0.01 + [2, 10) / 0.1), beside the
+    same model with the network embedding."""
+    import torch
+
+    from dpmi import ops, systems, table
+    from dpmi.model import SeAEnergyModel
+
+    dev = torch.device("cuda:0")
+    spec = systems.WATER_SPEC
+    params = systems.make_params(spec, seed=7)  # davg 0, dstd 1
+    tabs = table.build_tables(params["embedding"], params["davg"], params["dstd"], spec.rcut, spec.rcut_smth,
+                              0.9, spec.sel)
+    sysm = systems.SYSTEMS[args.workload]()
+    coord = torch.tensor(sysm.coord, device=dev)
+    atype = torch.tensor(sysm.atype, device=dev)
+    names = ["dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f64", "dpmi_tabulate_fusion_se_a_f64",
+             "dpmi_tabulate_fusion_se_a_grad_f64", "dpmi_force_virial_cells_f64"]
+
+    def run(model):
+        for _ in range(args.warmup):
+            model.compute(coord, atype, sysm.box)
+        torch.cuda.synchronize()
+        with ops.KernelTimer(*names) as kt:
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                out = model.compute(coord, atype, sysm.box)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / args.steps
+        return dt, out, kt.summary()
+
+    tt, ot, kt = run(SeAEnergyModel.from_params(spec, params, device=dev, tables=tabs))
+    tn, on, _ = run(SeAEnergyModel.from_params(spec, params, device=dev))
+    frms = torch.sqrt((on["force"] ** 2).mean()).item()
+    line = _line(args, sysm.nloc / tt, tt * 1e3, "f64",
+                 {"workload": args.workload, "atoms": sysm.nloc, "form": "FP64, tabulated embedding "
+                  "(info [0, 2, 10, 0.01, 0.1, -1], 280 intervals)"},
+                 network_fp64={"value": round(sysm.nloc / tn, 1), "ms_per_step": round(tn * 1e3, 3)},
+                 table_vs_network={"force_rmse_eV_per_A": torch.sqrt(((ot["force"] - on["force"]) ** 2).mean()).item(),
+                                   "force_rms_eV_per_A": frms,
+                                   "energy_per_atom_abs_eV": abs(ot["energy"].item() - on["energy"].item()) / sysm.nloc},
+                 kernels_ms_per_step={k: round(v["total_ms"] / args.steps, 4) for k, v in kt.items()})
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_train(args) -> int:
+    """Training steps on 8 frames x 192 water atoms (E+F loss, pref_e 0.02,
+    pref_f 1000, labels from the same architecture with weight seed 8): the
+    neighbor lists and env-mat of the batch, forward with the force graph,
+    loss, double backward and an Adam step, in each of the four forms."""
+    import numpy as np
+    import torch
+
+    from dpmi import systems, table, train
+    from dpmi.model import SeAEnergyModel
+
+    dev = torch.device("cuda:0")
+    spec = systems.WATER_SPEC
+    params = systems.make_params(spec, seed=7)
+    lab = SeAEnergyModel.from_params(spec, systems.make_params(spec, seed=8), device=dev)
+    frames = [systems.water192(k) for k in range(8)]
+    coords = [torch.tensor(f.coord, device=dev) for f in frames]
+    atypes = [torch.tensor(f.atype, device=dev) for f in frames]
+    boxes = [f.box for f in frames]
+    outs = [lab.compute(c, t, b) for c, t, b in zip(coords, atypes, boxes)]
+    e_l = torch.stack([o["energy"] for o in outs])
+    f_l = torch.stack([o["force"] for o in outs])
+    tabs = table.build_tables(params["embedding"], params["davg"], params["dstd"], spec.rcut, spec.rcut_smth,
+                              0.9, spec.sel)
+    res = {}
+    for name, prec, tb in (("fp64_network", "fp64", None), ("fp64_table", "fp64", tabs),
+                           ("fp32_network", "fp32", None),
+                           ("fp32_table", "fp32", [(t.astype(np.float32), i) for t, i in tabs])):
+        m = SeAEnergyModel.from_params(spec, params, device=dev, precision=prec, tables=tb)
+        tr = train.Trainer(m)
+
+        def step():
+            batch = train.prepare(m, coords, atypes, boxes)
+            return tr.step(batch, e_l, f_l)[0]
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            loss = step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        res[name] = {"atom_steps_per_s": round(8 * 192 / dt, 1), "ms_per_step": round(dt * 1e3, 3),
+                     "last_loss": float(loss)}
+    main_ = res["fp64_network"]
+    line = _line(args, main_["atom_steps_per_s"], main_["ms_per_step"], "f64",
+                 {"workload": "train8x192", "atoms": 8 * 192, "form": "training step (E+F loss, double backward, "
+                  "Adam) FP64 network embedding; the other forms below"}, forms=res)
     print(json.dumps(line), flush=True)
     return 0
 

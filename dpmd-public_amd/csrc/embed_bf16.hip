@@ -57,11 +57,29 @@ struct EmbedParams {
   int sec[kMaxTypes + 1];
 };
 
+// Layer-1 slots.  With n1 <= 28 the slots 19, 23, 27, 31 stay empty: they are
+// B-slot u = 7 of every lane group (phi(g, 7) = 19 + 4 g), and through the
+// skip connections the (mt odd, v = 3) accumulator rows of layers 2 and 3, so
+// the kernels skip those tanh evaluations outright.  Wider layers use slots
+// 0 .. n1 - 1.
+__host__ __device__ inline bool skip_layout(int n1) { return n1 <= 28; }
+
+__host__ __device__ inline int q1_of(int f, int n1) {
+  return (!skip_layout(n1) || f < 16) ? f : 16 + 4 * ((f - 16) / 3) + (f - 16) % 3;
+}
+
+// original layer-1 feature in slot q, or -1 for an empty slot
+__host__ __device__ inline int f1_of(int q, int n1) {
+  int f = q;
+  if (skip_layout(n1) && q >= 16) f = (q & 3) == 3 ? -1 : q - ((q - 16) >> 2);
+  return f < n1 ? f : -1;
+}
+
 // original feature (0 .. 4 n1 - 1) of internal G index q, or -1 for padding
 __host__ __device__ inline int orig_of_q3(int q, int n1) {
   const int base = q >= 64 ? 2 * n1 : 0, q2 = q & 63;
-  const int f2 = q2 >= 32 ? (q2 - 32 < n1 ? n1 + (q2 - 32) : -1) : (q2 < n1 ? q2 : -1);
-  return f2 < 0 ? -1 : base + f2;
+  const int f1 = f1_of(q2 & 31, n1);
+  return f1 < 0 ? -1 : base + (q2 >= 32 ? n1 : 0) + f1;
 }
 
 // tanh(x) from z = 2 log2(e) x: 1 - 2 / (2^z + 1), exact limits at +-inf.
@@ -86,11 +104,18 @@ struct TileFwd {
   float t3[32], g[32];
 };
 
+template <bool kSkip>
 __device__ inline void forward_tile(float s, const bf16x8* __restrict__ frag, const float* __restrict__ cst, int g,
                                     int lane, TileFwd& f) {
   bf16x8 b1;
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
+    if (kSkip && u == 7) {  // empty slot in every lane group
+      f.t1[u] = 0.f;
+      f.h1[u] = 0.f;
+      b1[u] = (__bf16)0.f;
+      continue;
+    }
     int q = phi(g, u);
     float t = tanh_scaled(s * cst[q] + cst[32 + q]);
     f.t1[u] = t;
@@ -107,6 +132,12 @@ __device__ inline void forward_tile(float s, const bf16x8* __restrict__ frag, co
     const int q = 16 * mt + 4 * g;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
+      if (kSkip && (mt & 1) && v == 3) {
+        f.t2[mt * 4 + v] = 0.f;
+        f.h2[mt * 4 + v] = 0.f;
+        b2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)0.f;
+        continue;
+      }
       float t = tanh_scaled(z2[mt][v]);
       f.t2[mt * 4 + v] = t;
       f.h2[mt * 4 + v] = t * cst[160 + q + v] + f.h1[(mt & 1) * 4 + v];
@@ -120,6 +151,11 @@ __device__ inline void forward_tile(float s, const bf16x8* __restrict__ frag, co
     const int q = 16 * mt + 4 * g;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
+      if (kSkip && (mt & 1) && v == 3) {
+        f.t3[mt * 4 + v] = 0.f;
+        f.g[mt * 4 + v] = 0.f;
+        continue;
+      }
       float t = tanh_scaled(z[v]);
       f.t3[mt * 4 + v] = t;
       f.g[mt * 4 + v] = t * cst[352 + q + v] + f.h2[(mt & 3) * 4 + v];
@@ -137,6 +173,7 @@ __device__ inline void forward_tile(float s, const bf16x8* __restrict__ frag, co
 // lane index is the feature and every lane group holds four rows, and two
 // tiles make the 32-row K operand (row order psi(g, u) = 4g + u for u < 4,
 // 16 + 4g + u - 4 otherwise) of GR^T += em^T G.
+template <bool kSkip>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_embed_fwd(float* __restrict__ gr_part, const float* __restrict__ em, const bf16x8* __restrict__ frag_all,
                 const float* __restrict__ consts, int nloc, EmbedParams p) {
@@ -182,8 +219,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       for (int rt = 0; rt < 2; ++rt) {
         const float4 ev = ev_cur[rt];
         if (g == 0) *reinterpret_cast<float4*>(sem + 4 * (16 * rt + r)) = ev;
+        if (32 * ch + 16 * rt >= sel) {  // wave-uniform: a tile of padding rows only (its em rows are 0)
+#pragma unroll
+          for (int kc = 0; kc < 4; ++kc)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) gt[rt][kc][u] = (__bf16)0.f;
+          continue;
+        }
         TileFwd f;
-        forward_tile(ev.x, s_frag, cst, g, lane, f);
+        forward_tile<kSkip>(ev.x, s_frag, cst, g, lane, f);
 #pragma unroll
         for (int kc = 0; kc < 4; ++kc) {
           bf16x8 gb;
@@ -244,7 +288,7 @@ __device__ inline int q3_of(int f, int n1) {
   int n2 = 2 * n1;
   int base = f >= n2 ? 64 : 0;
   int f2 = f >= n2 ? f - n2 : f;
-  return base + (f2 >= n1 ? 32 + (f2 - n1) : f2);
+  return base + (f2 >= n1 ? 32 + q1_of(f2 - n1, n1) : q1_of(f2, n1));
 }
 
 // GR = sum_t GR_t / nnei (original order, saved for the backward) and
@@ -406,6 +450,7 @@ __global__ void __launch_bounds__(256)
 // Persistent grid (weight fragments loaded into LDS once per workgroup);
 // each wavefront keeps dGR of its atom in LDS as [q3][c] so the mostly-zero
 // dGR^T A fragments are built per tile from one 16-byte read.
+template <bool kSkip>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_embed_bwd(float* __restrict__ net_deriv, const float* __restrict__ em, const float* __restrict__ dgr,
                 const bf16x8* __restrict__ frag_all, const float* __restrict__ consts, int nloc, EmbedParams p) {
@@ -459,7 +504,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       // wavefronts per SIMD
       asm volatile("" ::: "memory");
       TileFwd f;
-      forward_tile(ev.x, s_frag, cst, g, lane, f);
+      forward_tile<kSkip>(ev.x, s_frag, cst, g, lane, f);
       // direct term dem[c] = sum_q3 G dGR[c]  -> D(M = c, N = rows)
       f32x4 dem = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -500,6 +545,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
             skip[mt * 4 + v] = d[v];
           else
             skip[(mt - 4) * 4 + v] += d[v];
+          if (kSkip && (mt & 1) && v == 3) {  // empty slot: no weights read it
+            b_dz3[mt >> 1][(mt & 1) * 4 + v] = (__bf16)0.f;
+            continue;
+          }
           float tt = f.t3[mt * 4 + v];
           b_dz3[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(d[v] * cst[576 + q + v] * (1.f - tt * tt));
         }
@@ -517,6 +566,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
         for (int v = 0; v < 4; ++v) {
           float x = d[v] + skip[mt * 4 + v];
           dh2[mt * 4 + v] = x;
+          if (kSkip && (mt & 1) && v == 3) {
+            b_dz2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)0.f;
+            continue;
+          }
           float tt = f.t2[mt * 4 + v];
           b_dz2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(x * cst[512 + q + v] * (1.f - tt * tt));
         }
@@ -531,6 +584,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int u = mt * 4 + v, q = phi(g, u);
+          if (kSkip && u == 7) continue;
           float x = d[v] + dh2[mt * 4 + v] + dh2[(mt + 2) * 4 + v];
           float tt = f.t1[u];
           ds += x * cst[64 + q] * (1.f - tt * tt) * cst[480 + q];
@@ -577,8 +631,12 @@ extern "C" int dpmi_se_a_embed_fwd_bf16(float* gr_part, const float* em, const v
   if (nloc == 0) return DPMI_OK;
   // persistent grid: enough workgroups per type to fill 256 CUs a few times over
   dim3 grid((unsigned)std::min<int64_t>(ceil_div(nloc, kAtoms), 1024), (unsigned)ntypes);
-  hipLaunchKernelGGL(k_embed_fwd, grid, dim3(256), 0, as_stream(stream), gr_part, em,
-                     static_cast<const bf16x8*>(frags), consts, (int)nloc, p);
+  if (skip_layout(n1))
+    hipLaunchKernelGGL(k_embed_fwd<true>, grid, dim3(256), 0, as_stream(stream), gr_part, em,
+                       static_cast<const bf16x8*>(frags), consts, (int)nloc, p);
+  else
+    hipLaunchKernelGGL(k_embed_fwd<false>, grid, dim3(256), 0, as_stream(stream), gr_part, em,
+                       static_cast<const bf16x8*>(frags), consts, (int)nloc, p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }
@@ -616,16 +674,22 @@ extern "C" int dpmi_se_a_descriptor_bwd(float* dgr, const float* ddesc, const fl
 }
 
 extern "C" int dpmi_se_a_embed_bwd_bf16(float* net_deriv, const float* em, const float* dgr, const void* frags,
-                                        const float* consts, int64_t nloc, const int* sel_host, int ntypes,
+                                        const float* consts, int64_t nloc, const int* sel_host, int ntypes, int n1,
                                         void* stream) {
   EmbedParams p;
   int rc = make_params(&p, sel_host, ntypes);
   if (rc) return rc;
+  DPMI_CHECK_ARG(n1 > 0 && n1 <= 32, "n1 out of range [1, 32]");
+  p.n1 = n1;
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
   if (nloc == 0) return DPMI_OK;
   dim3 grid((unsigned)std::min<int64_t>(ceil_div(nloc, kAtoms), 1024), (unsigned)ntypes);
-  hipLaunchKernelGGL(k_embed_bwd, grid, dim3(256), 0, as_stream(stream), net_deriv, em, dgr,
-                     static_cast<const bf16x8*>(frags), consts, (int)nloc, p);
+  if (skip_layout(n1))
+    hipLaunchKernelGGL(k_embed_bwd<true>, grid, dim3(256), 0, as_stream(stream), net_deriv, em, dgr,
+                       static_cast<const bf16x8*>(frags), consts, (int)nloc, p);
+  else
+    hipLaunchKernelGGL(k_embed_bwd<false>, grid, dim3(256), 0, as_stream(stream), net_deriv, em, dgr,
+                       static_cast<const bf16x8*>(frags), consts, (int)nloc, p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }

@@ -5,11 +5,15 @@ as MFMA A-operand fragments (bf16) in the order the kernels read them, plus a
 small table of per-feature constants (fp32), both in the kernels' internal
 feature order:
 
-    q1(f) = f                                  layer 1, width n1 <= 32
-    q2(f) = f            (f <  n1)             layer 2, width 2 n1
-          = 32 + f - n1  (f >= n1)
+    q1(f) = f (f < 16 or n1 > 28), else 16 + 4 ((f-16) // 3) + (f-16) % 3
+                                               layer 1, width n1 <= 32
+    q2(f) = q1(f)            (f <  n1)         layer 2, width 2 n1
+          = 32 + q1(f - n1)  (f >= n1)
     q3(f) = q2(f)        (f <  2 n1)           layer 3, width 4 n1 = M
           = 64 + q2(f - 2 n1)
+
+(for n1 <= 28 the layer-1 slots 19, 23, 27, 31 stay empty: one B-slot
+column of every lane group, whose tanh the kernels skip in all layers)
 
 so that the [x, x] skip connections of MLPLayer (deepmd/pt/model/network/
 mlp.py:188-219) pair features on the same lane.  A fragment is 64 lanes x 8
@@ -40,11 +44,20 @@ def _phi():
     return 16 * (u // 4) + 4 * g + (u % 4)  # [4, 8]
 
 
+def skip_layout(n1: int) -> bool:
+    """n1 <= 28: layer-1 slots 19, 23, 27, 31 (B-slot u = 7 of every lane
+    group) stay empty, so the kernels skip those tanh columns in all layers."""
+    return n1 <= 28
+
+
 def internal_maps(n1: int):
     f1 = np.arange(n1)
-    q1 = f1
+    if skip_layout(n1):
+        q1 = np.where(f1 < 16, f1, 16 + 4 * ((f1 - 16) // 3) + (f1 - 16) % 3)
+    else:
+        q1 = f1
     f2 = np.arange(2 * n1)
-    q2 = np.where(f2 < n1, f2, 32 + f2 - n1)
+    q2 = np.where(f2 < n1, q1[np.minimum(f2, n1 - 1)], 32 + q1[np.clip(f2 - n1, 0, n1 - 1)])
     f3 = np.arange(4 * n1)
     q3 = np.where(f3 < 2 * n1, q2[np.minimum(f3, 2 * n1 - 1)], 64 + q2[np.clip(f3 - 2 * n1, 0, 2 * n1 - 1)])
     return q1, q2, q3
@@ -155,5 +168,5 @@ class FusedEmbedding:
                     self.axis, ops._stream())
         nd = torch.empty((nloc, self.nnei, 4), dtype=torch.float32, device=em.device)
         ops._launch("dpmi_se_a_embed_bwd_bf16", ptr(nd), ptr(em.contiguous()), ptr(dgr), ptr(self.frags),
-                    ptr(self.consts), nloc, ptr(self.sel), self.ntypes, ops._stream())
+                    ptr(self.consts), nloc, ptr(self.sel), self.ntypes, self.n1, ops._stream())
         return nd

@@ -251,9 +251,11 @@ int dpmi_se_a_descriptor_fwd(float* gr, void* desc, const float* gr_part, int64_
  * embedding kernels' internal feature order (gr as written above). */
 int dpmi_se_a_descriptor_bwd(float* dgr, const float* ddesc, const float* gr, int64_t nloc, int nnei, int n1,
                              int axis, void* stream);
-/* net_deriv [nloc, nnei, 4] = dE/d em, recomputing the embedding forward. */
+/* net_deriv [nloc, nnei, 4] = dE/d em, recomputing the embedding forward
+ * (n1 = width of the first embedding layer, as packed). */
 int dpmi_se_a_embed_bwd_bf16(float* net_deriv, const float* em, const float* dgr, const void* frags,
-                             const float* consts, int64_t nloc, const int* sel_host, int ntypes, void* stream);
+                             const float* consts, int64_t nloc, const int* sel_host, int ntypes, int n1,
+                             void* stream);
 
 /* ------------------------------------------------------------------------
  * Fused energy fitting net (bf16 MFMA operands, fp32 accumulation).

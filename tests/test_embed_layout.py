@@ -235,3 +235,22 @@ def test_backward_tile_matches_autograd(packed):
     e = (torch.tensor(dgr[:, q3]) * (emt.t() @ x)).sum()
     e.backward()
     np.testing.assert_allclose(nd, emt.grad.numpy(), rtol=1e-5, atol=1e-5)  # fp32 constants
+
+
+@pytest.mark.parametrize("n1", [25, 28, 29, 32, 8])
+def test_skip_columns_are_empty(n1):
+    """With n1 <= 28 the kernels skip the tanh of layer-1 slot column u = 7
+    (slots 19, 23, 27, 31) and of the (mt odd, v = 3) rows of layers 2 and 3:
+    no feature may live there.  Every map is a bijection onto its slots."""
+    q1, q2, q3 = embed.internal_maps(n1)
+    assert len(set(q1)) == n1 and len(set(q2)) == 2 * n1 and len(set(q3)) == 4 * n1
+    assert q1.max() < 32 and q2.max() < 64 and q3.max() < 128
+    skipped1 = {phi(g, 7) for g in range(4)}
+    skipped3 = {16 * mt + 4 * g + 3 for mt in (1, 3, 5, 7) for g in range(4)}
+    if embed.skip_layout(n1):
+        assert skipped1 == {19, 23, 27, 31}
+        assert not skipped1 & set(q1.tolist())
+        assert not {q for q in skipped3 if q < 64} & set(q2.tolist())
+        assert not skipped3 & set(q3.tolist())
+    else:
+        assert n1 > 28
