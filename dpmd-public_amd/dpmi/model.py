@@ -69,6 +69,22 @@ class _LowPrecisionBMM(torch.autograd.Function):
         return ga, gb, None
 
 
+_SPLIT_ROWS = 1 << 15
+
+
+def _split_k_linear(x, w, b):
+    """x @ w + b as a batched product over row blocks with the weight
+    broadcast, so the weight gradient is a batched product summed over the
+    blocks (split K) and stays differentiable for the double backward."""
+    rows = x.shape[0]
+    nb = min(256, rows // 1024)
+    pad = (-rows) % nb
+    if pad:
+        x = torch.cat([x, x.new_zeros((pad, x.shape[1]))])
+    y = torch.bmm(x.view(nb, -1, x.shape[1]), w.unsqueeze(0).expand(nb, -1, -1)).reshape(-1, w.shape[1]) + b
+    return y[:rows] if pad else y
+
+
 class Net:
     """Stack of reference MLP layers (MLPLayer.forward, mlp.py:188-219): y =
     tanh(x W + b) [* idt] [+ x or + (x, x)]; the last layer of a fitting net is
@@ -87,7 +103,11 @@ class Net:
 
     def __call__(self, x, gemm_dtype=None):
         for w, b, idt, act in zip(self.w, self.b, self.idt, self.act):
-            if gemm_dtype is None or w.shape[1] < 16:
+            if gemm_dtype is None and w.requires_grad and x.dim() == 2 and x.shape[0] >= _SPLIT_ROWS:
+                # training on neighbor rows: dL/dW would be one GEMM with a tiny
+                # output and K = all rows (a single workgroup); split K
+                y = _split_k_linear(x, w, b)
+            elif gemm_dtype is None or w.shape[1] < 16:
                 # fp64 path, or the 240 -> 1 output layer: a per-atom dot product
                 # (the low-precision N=1 GEMM path of the BLAS is pathologically slow)
                 y = torch.addmm(b, x, w) if x.dim() == 2 else x @ w + b
