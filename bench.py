@@ -169,7 +169,7 @@ def main() -> int:
         return (t1 - t0) / args.steps, out, kt.summary()
 
     names64 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f64",
-               "dpmi_force_virial_cells_f64"]
+               "dpmi_force_virial_cells_f64", "dpmi_se_a_embed_fwd_f64", "dpmi_se_a_embed_bwd_f64"]
     names32 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f32c64",
                "dpmi_force_virial_cells_f32", "dpmi_se_a_embed_fwd_bf16", "dpmi_se_a_descriptor_fwd",
                "dpmi_se_a_descriptor_bwd", "dpmi_se_a_embed_bwd_bf16", "dpmi_se_a_fitting_bf16"]

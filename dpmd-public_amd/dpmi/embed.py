@@ -170,3 +170,77 @@ class FusedEmbedding:
         ops._launch("dpmi_se_a_embed_bwd_bf16", ptr(nd), ptr(em.contiguous()), ptr(dgr), ptr(self.frags),
                     ptr(self.consts), nloc, ptr(self.sel), self.ntypes, self.n1, ops._stream())
         return nd
+
+
+# --------------------------------------------------------------------------- FP64 form
+
+
+def supported_f64(neuron) -> bool:
+    return supported(neuron)
+
+
+def pack_embedding_f64(emb_layers: list):
+    """FP64 kernels (csrc/embed_f64.hip): per type W2 [32, 64] and W3 [64, 128]
+    in the internal order q1 = f, q2 = q1(f mod n1) + 32 [f >= n1],
+    q3 = q2(f mod 2 n1) + 64 [f >= 2 n1] (zero padded), and consts [480]:
+    w1 [0:32], b1 [32:64], idt1 [64:96], b2 [96:160], idt2 [160:224],
+    b3 [224:352], idt3 [352:480].  Returns (w2, w3, consts, n1), float64."""
+    ntypes = len(emb_layers)
+    n1 = _layer(emb_layers[0][0])[0].shape[1]
+    f1 = np.arange(n1)
+    q1 = f1
+    f2 = np.arange(2 * n1)
+    q2 = np.where(f2 < n1, f2, 32 + f2 - n1)
+    f3 = np.arange(4 * n1)
+    q3 = np.where(f3 < 2 * n1, q2[np.minimum(f3, 2 * n1 - 1)], 64 + q2[np.clip(f3 - 2 * n1, 0, 2 * n1 - 1)])
+    w2 = np.zeros((ntypes, 32, 64))
+    w3 = np.zeros((ntypes, 64, 128))
+    consts = np.zeros((ntypes, 480))
+    for t, layers in enumerate(emb_layers):
+        (a1, b1, i1), (a2, b2, i2), (a3, b3, i3) = (_layer(x) for x in layers)
+        if a1.shape != (1, n1) or a2.shape != (n1, 2 * n1) or a3.shape != (2 * n1, 4 * n1):
+            raise ValueError("fused embedding needs widths (n1, 2 n1, 4 n1) with n1 <= 32")
+        w2[t][np.ix_(q1, q2)] = a2
+        w3[t][np.ix_(q2, q3)] = a3
+        c = consts[t]
+        c[q1] = a1[0]
+        c[32 + q1] = b1
+        c[64 + q1] = i1
+        c[96 + q2] = b2
+        c[160 + q2] = i2
+        c[224 + q3] = b3
+        c[352 + q3] = i3
+    return w2, w3, consts, n1
+
+
+class FusedEmbeddingF64:
+    """FP64 fused embedding + G^T R and its recompute adjoint (MFMA f64)."""
+
+    def __init__(self, emb_layers, sel, device):
+        w2, w3, consts, n1 = pack_embedding_f64(emb_layers)
+        self.n1 = n1
+        self.M = 4 * n1
+        self.sel = np.ascontiguousarray(np.asarray(sel, dtype=np.int32))
+        self.ntypes = len(sel)
+        self.nnei = int(self.sel.sum())
+        self.w2 = torch.from_numpy(w2).to(device)
+        self.w3 = torch.from_numpy(w3).to(device)
+        self.consts = torch.from_numpy(consts).to(device)
+
+    def forward(self, em: torch.Tensor) -> torch.Tensor:
+        """em [nloc, nnei, 4] float64 -> GR [nloc, 4, M] = sum_t em_t^T G_t / nnei."""
+        nloc = em.shape[0]
+        em = em.contiguous()
+        part = torch.empty((self.ntypes, nloc, 4, self.M), dtype=torch.float64, device=em.device)
+        ops._launch("dpmi_se_a_embed_fwd_f64", ptr(part), ptr(em), ptr(self.w2), ptr(self.w3), ptr(self.consts),
+                    nloc, ptr(self.sel), self.ntypes, self.n1, ops._stream())
+        return part.sum(dim=0) / self.nnei
+
+    def backward(self, em: torch.Tensor, dgr: torch.Tensor) -> torch.Tensor:
+        """dE/dGR [nloc, 4, M] -> dE/d em [nloc, nnei, 4] float64."""
+        nloc = em.shape[0]
+        dgr_t = (dgr / self.nnei).contiguous()
+        nd = torch.empty((nloc, self.nnei, 4), dtype=torch.float64, device=em.device)
+        ops._launch("dpmi_se_a_embed_bwd_f64", ptr(nd), ptr(em.contiguous()), ptr(dgr_t), ptr(self.w2),
+                    ptr(self.w3), ptr(self.consts), nloc, ptr(self.sel), self.ntypes, self.n1, ops._stream())
+        return nd

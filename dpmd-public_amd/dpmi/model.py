@@ -160,6 +160,11 @@ class SeAEnergyModel:
                 raise ValueError("the tabulated embedding runs in fp64 or fp32")
             self.tables = [(torch.as_tensor(np.asarray(t), device=self.device, dtype=dt),
                             torch.as_tensor(np.asarray(i), dtype=dt)) for t, i in tables]
+        # FP64 form: fused FP64-MFMA embedding kernels for E+F+V (the torch
+        # layers above stay for training, which differentiates the forces)
+        self.fused64 = None
+        if precision == "fp64" and tables is None and embed.supported_f64(neuron):
+            self.fused64 = embed.FusedEmbeddingF64(emb, self.sel, self.device)
         # bf16 form: fused MFMA embedding kernels (else the torch layers above)
         self.fused = None
         if precision == "bf16" and embed.supported(neuron) and self.axis <= 16:
@@ -272,6 +277,8 @@ class SeAEnergyModel:
         ext_c, ext_t, mapping, nlist, _ovf, _, cells = self.neighbor_list(coord, atype, box)
         if self.fused is not None:
             return self._compute_fused(atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy)
+        if self.fused64 is not None:
+            return self._compute_fused64(atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy)
         em = self.env_mat(nlist, ext_c, ext_t)
         # atoms in blocks: an atom's energy and dE/d em depend on its own em
         # rows only, so the network's working set stays bounded (about
@@ -290,6 +297,32 @@ class SeAEnergyModel:
         energy = ae.sum()
         force, virial = self.forces(net_deriv, nlist, cells, mapping, ext_c, ext_t, box)
         out = {"energy": energy, "force": force, "virial": virial}
+        if need_atom_energy:
+            out["atom_energy"] = ae
+        return out
+
+    def _compute_fused64(self, atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy):
+        """FP64 plain form on hand-written kernels: fused FP64 embedding + G^T R
+        and its recompute adjoint; descriptor and fitting (per-type nets,
+        rocBLAS) through torch autograd in atom blocks; env-mat adjoint for
+        forces and virial."""
+        em = self.env_mat(nlist, ext_c, ext_t)
+        gr = self.fused64.forward(em)
+        nloc = em.shape[0]
+        blk = max(1024, self.ROWS_PER_BLOCK // 64)
+        dgr = torch.empty_like(gr)
+        ae = torch.empty(nloc, dtype=torch.float64, device=em.device)
+        for a in range(0, nloc, blk):
+            b = min(nloc, a + blk)
+            g = gr[a:b].detach().requires_grad_(True)
+            with torch.enable_grad():
+                d = torch.bmm(g.transpose(1, 2), g[:, :, : self.axis]).reshape(b - a, -1)
+                ae_b = self.atom_energy(d, atype[a:b])
+                (dgr[a:b],) = torch.autograd.grad(ae_b.sum(), g)
+            ae[a:b] = ae_b.detach()
+        nd = self.fused64.backward(em, dgr)
+        force, virial = self.forces(nd, nlist, cells, mapping, ext_c, ext_t, box)
+        out = {"energy": ae.sum(), "force": force, "virial": virial}
         if need_atom_energy:
             out["atom_energy"] = ae
         return out
