@@ -138,8 +138,7 @@ __device__ inline void layers12(double s, const double* __restrict__ sW2, const 
     if (kKeep) t1[c] = t;
     h1[c] = t * sC[64 + q];
   }
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt) {
+  auto z2 = [&](int mt) {
     asm volatile("" ::: "memory");
     f64x4 d;
 #pragma unroll
@@ -150,6 +149,13 @@ __device__ inline void layers12(double s, const double* __restrict__ sW2, const 
       if (!live_chunk<NL>(s1)) continue;
       d = mfma64(sW2[(s1 + g) * kP2 + 16 * mt + r], h1[c], d);  // A: W2[q1 = s1 + k][q2 = 16 mt + m]
     }
+    return d;
+  };
+  f64x4 d_next = z2(0);
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    const f64x4 d = d_next;
+    if (mt < 3) d_next = z2(mt + 1);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int c = 4 * mt + v, s1 = 16 * (mt & 1) + 4 * v;
@@ -165,10 +171,10 @@ __device__ inline void layers12(double s, const double* __restrict__ sW2, const 
   }
 }
 
-// Layer-3 feature tile mt: G (4 slots) and its tanh values.
+// Layer-3 feature tile mt: the MFMA chain (z3 of the tile) ...
 template <int NL>
-__device__ inline void layer3_tile(int mt, const double (&h2)[16], const double* __restrict__ sW3,
-                                   const double* __restrict__ sC, int g, int r, double (&G)[4], double (&t3)[4]) {
+__device__ inline f64x4 layer3_z(int mt, const double (&h2)[16], const double* __restrict__ sW3,
+                                 const double* __restrict__ sC, int g, int r) {
   // keep the weight reads of this tile here: hoisting all tiles' LDS reads
   // ahead would need hundreds of registers
   asm volatile("" ::: "memory");
@@ -181,6 +187,15 @@ __device__ inline void layer3_tile(int mt, const double (&h2)[16], const double*
     if (!live_chunk<NL>(s1)) continue;
     d = mfma64(sW3[(16 * (c >> 2) + 4 * (c & 3) + g) * kP3 + 16 * mt + r], h2[c], d);
   }
+  return d;
+}
+
+// ... and its activation: G (4 slots) and the tanh values.  The loops issue
+// the next tile's MFMA chain before this, so the matrix core and the VALU
+// work concurrently.
+template <int NL>
+__device__ inline void layer3_act(int mt, const f64x4& d, const double (&h2)[16], const double* __restrict__ sC,
+                                  int g, double (&G)[4], double (&t3)[4]) {
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int s1 = 16 * (mt & 1) + 4 * v;
@@ -230,10 +245,13 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
       }
       double h1[8], h2[16], t1[8], t2[16];
       layers12<NL, false>(s, sW2, sC, g, r, h1, h2, t1, t2);
+      f64x4 z_next = layer3_z<NL>(0, h2, sW3, sC, g, r);
 #pragma unroll
       for (int mt = 0; mt < 8; ++mt) {
+        const f64x4 z = z_next;
+        if (mt < 7) z_next = layer3_z<NL>(mt + 1, h2, sW3, sC, g, r);
         double G[4], t3[4];
-        layer3_tile<NL>(mt, h2, sW3, sC, g, r, G, t3);
+        layer3_act<NL>(mt, z, h2, sC, g, G, t3);
         // transpose: T[m = row][n] = G[row][16 mt + n]; T slot v' holds rows 4 v' + g, column r
         f64x4 T = f64x4{0., 0., 0., 0.};
 #pragma unroll
@@ -300,10 +318,13 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
       double sk[16];
 #pragma unroll
       for (int c = 0; c < 16; ++c) sk[c] = 0.;
+      f64x4 z_next = layer3_z<NL>(0, h2, sW3, sC, g, r);
 #pragma unroll
       for (int mt = 0; mt < 8; ++mt) {
+        const f64x4 z = z_next;
+        if (mt < 7) z_next = layer3_z<NL>(mt + 1, h2, sW3, sC, g, r);
         double G[4], t3[4];
-        layer3_tile<NL>(mt, h2, sW3, sC, g, r, G, t3);
+        layer3_act<NL>(mt, z, h2, sC, g, G, t3);
         // direct term dem[c][row] += sum_q dGR[c][q] G[row][q]: A (M = c = r < 4, K = q3 chunk)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
