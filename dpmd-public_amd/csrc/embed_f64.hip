@@ -39,7 +39,7 @@ constexpr int kMaxTypes64 = 16;
 constexpr int kP2 = 65;     // LDS row pitch (doubles) of W2 [32][64]
 constexpr int kP3 = 129;    // LDS row pitch (doubles) of W3 [64][128]
 constexpr int kC64 = 480;   // w1[32] b1[32] idt1[32] b2[64] idt2[64] b3[128] idt3[128]
-constexpr int kWaves64 = 4;  // wavefronts per workgroup: one per SIMD with the full register file
+constexpr int kWaves64 = 8;  // wavefronts per workgroup: two per SIMD, so FP64 MFMA and VALU of different waves overlap
 constexpr size_t kLdsW = (size_t)(32 * kP2 + 64 * kP3 + kC64) * sizeof(double);
 
 struct Embed64Params {
@@ -214,7 +214,7 @@ __device__ inline void layer3_act(int mt, const f64x4& d, const double (&h2)[16]
 // (original feature order).  Workgroups are split over neighbor types in
 // proportion to sel; each wavefront walks atoms, 16 rows per tile.
 template <int NL>
-__global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_embed64_fwd(double* __restrict__ gr_part, const double* __restrict__ em, const double* __restrict__ w2,
                   const double* __restrict__ w3, const double* __restrict__ consts, int nloc, Embed64Params p) {
   extern __shared__ double smem64[];
@@ -230,9 +230,12 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
   const int nb = p.blk[t + 1] - p.blk[t], b0 = blockIdx.x - p.blk[t];
   for (int i = b0 * kWaves64 + w; i < nloc; i += nb * kWaves64) {
     const double* emi = em + ((int64_t)i * p.nnei + p.sec[t]) * 4;
-    f64x4 acc[8];
+    // GR[c = g][16 mt + r]: slot 0 of each tile's GR^T product (the other
+    // slots are zero rows c >= 4), accumulated here so that no full
+    // accumulator stays live across tiles
+    double acc[8];
 #pragma unroll
-    for (int mt = 0; mt < 8; ++mt) acc[mt] = f64x4{0., 0., 0., 0.};
+    for (int mt = 0; mt < 8; ++mt) acc[mt] = 0.;
     for (int tl = 0; tl < ntile; ++tl) {
       const int row = 16 * tl + r;
       const double s = row < sel ? emi[4 * row] : 0.;
@@ -260,8 +263,12 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
           T = mfma64(G[v], r == 4 * v + g ? 1. : 0., T);
         }
         // GR^T[c][16 mt + n] += sum_rows em[row][c] G[row][16 mt + n]
+        {
+          f64x4 grt = f64x4{0., 0., 0., 0.};
 #pragma unroll
-        for (int kc = 0; kc < 4; ++kc) acc[mt] = mfma64(a_em[kc], T[kc], acc[mt]);
+          for (int kc = 0; kc < 4; ++kc) grt = mfma64(a_em[kc], T[kc], grt);
+          acc[mt] += grt[0];
+        }
       }
     }
     // acc[mt] slot 0 of lane group g = c holds GR[c][q3 = 16 mt + r]
@@ -269,7 +276,7 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
 #pragma unroll
     for (int mt = 0; mt < 8; ++mt) {
       const int m = orig_of_q3_64(16 * mt + r, n1);
-      if (m >= 0) out[g * M + m] = acc[mt][0];
+      if (m >= 0) out[g * M + m] = acc[mt];
     }
   }
 }
@@ -279,7 +286,7 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
 // forward per tile.  dG = em dGR, direct term dem = G . dGR, then the
 // adjoint chain dh2 = W3 dz3 + skip, dh1 = W2 dz2 + skip, ds = dz1 . w1.
 template <int NL>
-__global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_embed64_bwd(double* __restrict__ net_deriv, const double* __restrict__ em, const double* __restrict__ dgr,
                   const double* __restrict__ w2, const double* __restrict__ w3, const double* __restrict__ consts,
                   int nloc, Embed64Params p) {
@@ -315,9 +322,6 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
       f64x4 dh2[4];
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt) dh2[mt] = f64x4{0., 0., 0., 0.};
-      double sk[16];
-#pragma unroll
-      for (int c = 0; c < 16; ++c) sk[c] = 0.;
       f64x4 z_next = layer3_z<NL>(0, h2, sW3, sC, g, r);
 #pragma unroll
       for (int mt = 0; mt < 8; ++mt) {
@@ -337,7 +341,7 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           if (!live_chunk<NL>(16 * (mt & 1) + 4 * v)) continue;
-          sk[4 * (mt & 3) + v] += dG[v];
+          dh2[mt & 3][v] += dG[v];  // skip: G = ... + [h2, h2] carries dG to dh2[q3 mod 64]
           const double dz3 = dG[v] * sC[352 + 16 * mt + 4 * v + g] * fma(-t3[v], t3[v], 1.);
           asm volatile("" ::: "memory");
           // dh2[q2] += sum_q3 W3[q2][q3] dz3[q3]: A (M = q2 = 16 mt2 + r, K = q3 = 16 mt + 4 v + g)
@@ -356,7 +360,7 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int c = 4 * mt + v;
-          dh2v[c] = dh2[mt][v] + sk[c];
+          dh2v[c] = dh2[mt][v];
           if (!live_chunk<NL>(16 * (mt & 1) + 4 * v)) continue;
           const double dz2 = dh2v[c] * sC[160 + 16 * mt + 4 * v + g] * fma(-t2[c], t2[c], 1.);
           asm volatile("" ::: "memory");
