@@ -54,8 +54,23 @@ def _bytes_per_atom(name: str, nnei: int) -> float:
 
 HBM_PEAK_GBS = 8000.0
 # entry point -> kernel it launches (for the measured-traffic table)
-KERNEL_OF = {"dpmi_se_a_embed_bwd_bf16": "k_embed_bwd", "dpmi_se_a_embed_fwd_bf16": "k_embed_fwd",
+KERNEL_OF = {"dpmi_se_a_embed_bwd_bf16": "k_embed_bwd<skip>", "dpmi_se_a_embed_fwd_bf16": "k_embed_fwd<skip>",
              "dpmi_se_a_fitting_bf16": "k_fit", "dpmi_force_virial_cells_f32": "k_force_cells<float, 1536, false>"}
+
+
+def _issue_bound(workload: str, entry: str):
+    """SIMD-cycles per 16-row tile that the kernel's VALU instruction mix
+    needs (4 per VALU, 16 per transcendental instruction), from the committed
+    SQ counter summary of this workload, or None."""
+    f = ROOT / "profiles" / f"sq_mfma_{workload}_bf16.txt"
+    k = KERNEL_OF.get(entry)
+    if not f.exists() or k is None:
+        return None, None
+    for line in f.read_text().splitlines():
+        cols = line.split()
+        if len(cols) > 3 and cols[0] == k and cols[1] == "bound":
+            return float(cols[2]), str(f.relative_to(ROOT))
+    return None, None
 
 
 def _measured_traffic(workload: str, entry: str):
@@ -204,6 +219,14 @@ def main() -> int:
                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
                     "traffic_source": traffic_src,
                     "algorithmic_flops_per_launch": flops, "avg_launch_ms": round(klo[dom]["avg_ms"], 4)}
+        # what actually binds the embedding kernels: VALU issue (tanh = two
+        # quarter-rate transcendentals per element), from the counted mix
+        cyc, cyc_src = _issue_bound(args.workload, dom)
+        if cyc is not None and avg_s > 0:
+            tiles = nloc * sum((s + 15) // 16 for s in spec.sel)
+            roofline["valu_issue"] = {"frac": round(cyc * tiles / (avg_s * 2.4e9 * 1024), 4),
+                                      "cycles_per_tile_bound": cyc, "tiles_per_launch": tiles,
+                                      "source": cyc_src}
     else:
         algo = _bytes_per_atom(dom, nnei) * nloc
         achieved = algo / avg_s / 1e9 if avg_s > 0 else 0.0

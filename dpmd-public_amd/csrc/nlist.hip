@@ -672,6 +672,82 @@ __global__ void __launch_bounds__(256) k_search_cells(
   }
 }
 
+// format_nlist_cpu (source/lib/src/fmt_nlist.cc:145-177) of an external
+// list: row i of cand (width ncand, extended indices) holds numneigh[i]
+// candidates (or, without counts, entries until the row ends, -1 skipped).
+// One wavefront per local atom keeps the candidates of type >= 0 (and <
+// ntypes) with float r^2 <= rc^2 (format_nlist_i_cpu, fmt_nlist.cc:98-143)
+// and ranks them by the same (type, float r^2, index) key as the search.
+template <int CAP>
+__global__ void __launch_bounds__(64) k_format_external(
+    int* __restrict__ nlist, int* __restrict__ overflow, int* __restrict__ nlist_mapped, int* __restrict__ status,
+    const int* __restrict__ cand, const int* __restrict__ numneigh, int64_t ncand, const double* __restrict__ ext_c,
+    const int* __restrict__ ext_t, const int* __restrict__ mapping, int nall, SelParams sp, float rcut) {
+  extern __shared__ unsigned char smem_raw[];
+  uint64_t* keys = reinterpret_cast<uint64_t*>(smem_raw);
+  int* row = reinterpret_cast<int*>(keys + CAP);
+  const int i = blockIdx.x;
+  const int lane = lane_id();
+  const float rcut2f = rcut * rcut;
+  const float fi0 = (float)ext_c[3 * i + 0], fi1 = (float)ext_c[3 * i + 1], fi2 = (float)ext_c[3 * i + 2];
+  const int64_t nn = numneigh ? (int64_t)numneigh[i] : ncand;
+  const int64_t cnt = nn < ncand ? nn : ncand;
+  const int* ci = cand + (int64_t)i * ncand;
+  int n = 0;  // wave-uniform count of kept neighbors
+  for (int64_t k0 = 0; k0 < cnt; k0 += kWave) {
+    const int64_t k = k0 + lane;
+    bool pass = false;
+    uint64_t key = 0;
+    if (k < cnt) {
+      const int j = ci[k];
+      if (j >= 0 && j < nall) {
+        const int tj = ext_t[j];
+        if (tj >= 0 && tj < sp.ntypes) {
+          const float f0 = (float)ext_c[3 * j + 0] - fi0, f1 = (float)ext_c[3 * j + 1] - fi1,
+                      f2 = (float)ext_c[3 * j + 2] - fi2;
+          const float rr2 = f0 * f0 + f1 * f1 + f2 * f2;
+          if (rr2 <= rcut2f) {
+            pass = true;
+            key = ((uint64_t)tj << 56) | ((uint64_t)__float_as_uint(rr2) << 24) | (uint64_t)j;
+          }
+        }
+      }
+    }
+    const uint64_t m = __ballot(pass);
+    const int pos = n + mask_prefix(m);
+    if (pass && pos < CAP) keys[pos] = key;
+    n += __popcll(m);
+  }
+  if (n > CAP) {
+    if (lane == 0) atomicOr(status, 1);
+    n = CAP;
+  }
+  for (int s = lane; s < sp.nnei; s += kWave) row[s] = -1;
+  __syncthreads();
+  int ovf = -1;
+  for (int k = lane; k < n; k += kWave) {
+    const uint64_t kk = keys[k];
+    const int t = (int)(kk >> 56);
+    const uint64_t tbase = (uint64_t)t << 56;
+    const uint64_t rel = kk - tbase;
+    int rank = 0;
+    for (int m2 = 0; m2 < n; ++m2) rank += (keys[m2] - tbase) < rel;
+    if (rank < sp.sel[t])
+      row[sp.sec[t] + rank] = (int)(kk & 0xFFFFFFull);
+    else
+      ovf = max(ovf, t);
+  }
+  __syncthreads();
+  const int64_t ro = (int64_t)i * sp.nnei;
+  for (int s = lane; s < sp.nnei; s += kWave) {
+    const int j = row[s];
+    nlist[ro + s] = j;
+    if (nlist_mapped) nlist_mapped[ro + s] = j >= 0 ? mapping[j] : -1;
+  }
+  for (int off = 32; off > 0; off >>= 1) ovf = max(ovf, __shfl_xor(ovf, off, kWave));
+  if (overflow && lane == 0) overflow[i] = ovf;
+}
+
 }  // namespace
 
 int cell_grid_info(const double* box_host, int64_t nloc, float rcut, int64_t* nlc, int* nsearch) {
@@ -861,6 +937,44 @@ extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped
   int err = stat[0];
   if (err) {
     set_error("more than 1024 candidates within rcut for some atom");
+    return DPMI_ERR_CAPACITY;
+  }
+  return DPMI_OK;
+}
+
+extern "C" int dpmi_format_nlist_f64(int* nlist, int* overflow, int* nlist_mapped, const int* cand,
+                                     const int* numneigh, int64_t ncand, const double* ext_coord, const int* ext_type,
+                                     const int* mapping, int64_t nloc, int64_t nall, float rcut, const int* sel_host,
+                                     int ntypes, void* workspace, int64_t workspace_bytes, void* stream) {
+  DPMI_CHECK_ARG(ntypes > 0 && ntypes <= kMaxTypes, "ntypes out of range [1, 64]");
+  DPMI_CHECK_ARG(sel_host != nullptr, "sel is NULL");
+  DPMI_CHECK_ARG(nloc >= 0 && nall >= nloc && nall < (int64_t(1) << 24), "nloc/nall out of range");
+  DPMI_CHECK_ARG(ncand >= 0, "ncand must be >= 0");
+  DPMI_CHECK_ARG(nlist_mapped == nullptr || mapping != nullptr, "nlist_mapped needs mapping");
+  DPMI_CHECK_ARG(workspace != nullptr && workspace_bytes >= (int64_t)sizeof(int), "workspace too small (4 bytes)");
+  SelParams sp;
+  std::memset(&sp, 0, sizeof(sp));
+  sp.ntypes = ntypes;
+  for (int t = 0; t < ntypes; ++t) {
+    DPMI_CHECK_ARG(sel_host[t] >= 0, "sel must be >= 0");
+    sp.sel[t] = sel_host[t];
+    sp.sec[t + 1] = sp.sec[t] + sel_host[t];
+  }
+  sp.nnei = sp.sec[ntypes];
+  DPMI_CHECK_ARG(sp.nnei <= 4096, "sum(sel) must be <= 4096");
+  if (nloc == 0) return DPMI_OK;
+  hipStream_t st = as_stream(stream);
+  int* status = static_cast<int*>(workspace);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, status, (int64_t)1, 0);
+  const size_t shm = 1024 * sizeof(uint64_t) + sizeof(int) * sp.nnei;
+  hipLaunchKernelGGL(k_format_external<1024>, dim3((unsigned)nloc), dim3(64), shm, st, nlist, overflow, nlist_mapped,
+                     status, cand, numneigh, ncand, ext_coord, ext_type, mapping, (int)nall, sp, rcut);
+  DPMI_LAUNCH_CHECK();
+  int h = 0;
+  DPMI_HIP(hipMemcpyAsync(&h, status, sizeof(int), hipMemcpyDeviceToHost, st));
+  DPMI_HIP(hipStreamSynchronize(st));
+  if (h) {
+    set_error("more than 1024 neighbors within rcut for some atom");
     return DPMI_ERR_CAPACITY;
   }
   return DPMI_OK;

@@ -301,6 +301,47 @@ class SeAEnergyModel:
             out["atom_energy"] = ae
         return out
 
+    def compute_lower(self, ext_coord: torch.Tensor, ext_type: torch.Tensor, cand: torch.Tensor, nloc: int,
+                      numneigh=None, mapping=None):
+        """E+F+V from extended coordinates and an external neighbor list, as
+        forward_lower (deepmd/pt/model/model/make_model.py:334-415) and the
+        LAMMPS path (source/api_cc/src/DeepPotPT.cc:180-330) take them: the
+        list is formatted on the GPU (format_nlist_cpu semantics).  Returns
+        energy, atom_energy [nloc], force [nall, 3] on every extended atom
+        (ghost forces left for the caller to fold or communicate; folded onto
+        local atoms when mapping is given) and virial [9]."""
+        ext_c = ext_coord.to(torch.float64).reshape(-1, 3).contiguous()
+        ext_t = ext_type.to(torch.int32).reshape(-1).contiguous()
+        atype = ext_t[:nloc]
+        nlist, _ovf, _ = ops.format_nlist(cand, ext_c, ext_t, nloc, self.rcut, self.sel, numneigh=numneigh)
+        em = self.env_mat(nlist, ext_c, ext_t)
+        if self.fused is not None:
+            gr, desc = self.fused.forward(em, torch.bfloat16 if self.fused_fit is not None else torch.float32)
+            if self.fused_fit is not None:
+                ae, ddesc = self.fused_fit(desc, atype)
+            else:
+                desc.requires_grad_(True)
+                with torch.enable_grad():
+                    ae = self.atom_energy(desc, atype)
+                    (ddesc,) = torch.autograd.grad(ae.sum(), desc)
+            nd = self.fused.backward(em, gr, ddesc)
+        elif self.fused64 is not None:
+            g = self.fused64.forward(em).requires_grad_(True)
+            with torch.enable_grad():
+                d = torch.bmm(g.transpose(1, 2), g[:, :, : self.axis]).reshape(nloc, -1)
+                ae = self.atom_energy(d, atype)
+                (dgr,) = torch.autograd.grad(ae.sum(), g)
+            nd = self.fused64.backward(em, dgr)
+        else:
+            em = em.detach().requires_grad_(True)
+            with torch.enable_grad():
+                ae = self.atom_energy(self.descriptor(em), atype)
+                (nd,) = torch.autograd.grad(ae.sum(), em)
+        force, virial = ops.force_virial_recompute(nd, nlist, mapping, ext_c, ext_t, self.dstd, self.rcut,
+                                                   self.rcut_smth, self.sel)
+        ae = ae.detach()
+        return {"energy": ae.sum(), "atom_energy": ae, "force": force, "virial": virial}
+
     def _compute_fused64(self, atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy):
         """FP64 plain form on hand-written kernels: fused FP64 embedding + G^T R
         and its recompute adjoint; descriptor and fitting (per-type nets,

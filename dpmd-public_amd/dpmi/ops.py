@@ -167,6 +167,28 @@ def build_nlist(coord: torch.Tensor, atype: torch.Tensor, box, rcut: float, sel,
     return ext_c, ext_t, mapping, nlist, overflow, nl_map
 
 
+def format_nlist(cand, ext_coord, ext_type, nloc: int, rcut: float, sel, numneigh=None, mapping=None):
+    """format_nlist_cpu of an external list (the LAMMPS / forward_lower input):
+    cand [nloc, ncand] extended indices (-1 padded, or numneigh [nloc] valid
+    entries per row) -> (nlist [nloc, nnei], overflow [nloc], nlist mapped to
+    local atoms or None)."""
+    _gpu(cand, ext_coord, ext_type)
+    dev = ext_coord.device
+    cand = _c(cand, torch.int32).view(nloc, -1)
+    s = _sel_host(sel)
+    nnei = int(s.sum())
+    nlist = torch.empty((nloc, nnei), dtype=torch.int32, device=dev)
+    overflow = torch.empty(nloc, dtype=torch.int32, device=dev)
+    nl_map = torch.empty((nloc, nnei), dtype=torch.int32, device=dev) if mapping is not None else None
+    status = torch.empty(1, dtype=torch.int32, device=dev)
+    nn = _c(numneigh, torch.int32) if numneigh is not None else None
+    _launch("dpmi_format_nlist_f64", ptr(nlist), ptr(overflow), ptr(nl_map), ptr(cand), ptr(nn), cand.shape[1],
+            ptr(_c(ext_coord, torch.float64)), ptr(_c(ext_type, torch.int32)),
+            ptr(_c(mapping, torch.int32)) if mapping is not None else None, nloc, ext_coord.shape[0], rcut,
+            ptr(s), len(s), ptr(status), 4, _stream())
+    return nlist, overflow, nl_map
+
+
 # --------------------------------------------------------------------------- env-mat
 
 

@@ -99,6 +99,19 @@ int dpmi_prod_env_mat_a_f32c64(float* em, float* em_deriv, float* rij, const int
                                const int* ext_type, int64_t nloc, const float* avg, const float* std, float rcut,
                                float rcut_smth, const int* sel_host, int ntypes, void* stream);
 
+/* format_nlist_cpu (source/lib/src/fmt_nlist.cc:145-177; the formatting step
+ * of forward_lower with an external list, make_model.py:334-415, and of the
+ * LAMMPS / C++ path, DeepPotPT.cc:180-330): row i of cand (width ncand,
+ * extended indices; numneigh[i] entries, or NULL to scan the whole row
+ * skipping -1) is formatted like the search output: neighbors of type in
+ * [0, ntypes) with float r^2 <= rc^2, by type section, sorted by (float r^2,
+ * index), -1 padding, overflow as above.  Candidates must be distinct.
+ * workspace: 4 bytes of device memory (a status word). */
+int dpmi_format_nlist_f64(int* nlist, int* overflow, int* nlist_mapped, const int* cand, const int* numneigh,
+                          int64_t ncand, const double* ext_coord, const int* ext_type, const int* mapping,
+                          int64_t nloc, int64_t nall, float rcut, const int* sel_host, int ntypes, void* workspace,
+                          int64_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------------
  * Force and virial.  prod_force_a_gpu / prod_virial_a_gpu
  * (source/lib/include/prod_force.h, prod_virial.h; CPU semantics

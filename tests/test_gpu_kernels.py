@@ -411,3 +411,34 @@ def test_prod_env_mat_op_and_backward(oracle):
                                               sel)[0]
         fd = ((ep - emm) * w).sum().item() / (2 * h)
         assert abs(fd - grad[i, d]) < 1e-6 * max(1.0, abs(fd))
+
+
+@pytest.mark.parametrize("name,make,rcut,sel", [NLIST_CASES[0], NLIST_CASES[3], NLIST_CASES[-2]],
+                         ids=[NLIST_CASES[i][0] for i in (0, 3, -2)])
+@pytest.mark.parametrize("counts", [True, False], ids=["numneigh", "padded"])
+def test_format_external_list_matches_oracle(oracle, name, make, rcut, sel, counts):
+    """An external neighbor list (the LAMMPS / forward_lower input: the
+    reference's candidate lists, each row shuffled, plus extra far atoms)
+    formatted on the GPU equals format_nlist_cpu of the oracle, bitwise."""
+    s = make()
+    _say(f"format external list {name} counts={counts}")
+    oc, ot, om, onl, _ = oracle.build_nlist_pbc(s.coord, s.atype, s.box, rcut, sel)
+    nn, cand, mem = oracle.candidates(oc, s.nloc, rcut, box=s.box)
+    rng = np.random.default_rng(17)
+    nall = oc.shape[0]
+    extra = 6
+    rows = np.full((s.nloc, mem + extra), -1, np.int32)
+    counts_v = np.zeros(s.nloc, np.int32)
+    for i in range(s.nloc):
+        r = np.concatenate([cand[i, :nn[i]], rng.integers(0, nall, extra)])
+        r = r[r != i]
+        r = np.unique(r)  # distinct candidates
+        rng.shuffle(r)
+        rows[i, :len(r)] = r
+        counts_v[i] = len(r)
+    exp, eovf = oracle.format_nlist(oc, ot, s.nloc, counts_v, rows, rows.shape[1], rcut, sel)
+    got, ovf, nmap = ops.format_nlist(T(rows), T(oc), T(ot), s.nloc, rcut, sel,
+                                      numneigh=T(counts_v) if counts else None, mapping=T(om))
+    np.testing.assert_array_equal(got.cpu().numpy(), exp)
+    np.testing.assert_array_equal(ovf.cpu().numpy(), eovf)
+    np.testing.assert_array_equal(nmap.cpu().numpy(), np.where(exp >= 0, om[np.maximum(exp, 0)], -1))

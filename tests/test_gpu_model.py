@@ -173,3 +173,40 @@ def test_alloy_bf16_error_budget(alloy_model):
     dv = (torch.abs(a["virial"] - b["virial"]).max() / torch.abs(a["virial"]).max()).item()
     print(f"alloy bf16: force RMSE {rmse:.3e} eV/A, |dE|/atom {de:.3e} eV, virial max rel {dv:.3e}", flush=True)
     assert rmse <= 1e-3
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16"])
+def test_compute_lower_external_list(oracle, water_model, precision):
+    """forward_lower semantics: extended coordinates + an external (shuffled,
+    unsorted) neighbor list give the same E, F, V as compute(); without a
+    mapping the forces stay on the extended atoms and fold to the same."""
+    from dpmi.model import SeAEnergyModel
+
+    spec, params, _ = water_model
+    s = systems.water192(1)
+    _say(f"compute_lower external list water192 {precision}")
+    m = SeAEnergyModel.from_params(spec, params, device=DEV, precision=precision)
+    c = torch.tensor(s.coord, device=DEV)
+    t = torch.tensor(s.atype, device=DEV)
+    ref = m.compute(c, t, s.box, need_atom_energy=True)
+    oc, ot, om, _, _ = oracle.build_nlist_pbc(s.coord, s.atype, s.box, spec.rcut, spec.sel)
+    nn, cand, mem = oracle.candidates(oc, s.nloc, spec.rcut, box=s.box)
+    rng = np.random.default_rng(3)
+    for i in range(s.nloc):
+        rng.shuffle(cand[i, :nn[i]])
+    ext_c = torch.tensor(oc, device=DEV)
+    ext_t = torch.tensor(ot, device=DEV)
+    out = m.compute_lower(ext_c, ext_t, torch.tensor(cand, device=DEV), s.nloc, numneigh=torch.tensor(nn, device=DEV),
+                          mapping=torch.tensor(om, device=DEV))
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    assert abs(out["energy"].item() - ref["energy"].item()) <= tol * abs(ref["energy"].item())
+    np.testing.assert_allclose(out["force"].cpu().numpy(), ref["force"].cpu().numpy(), rtol=tol,
+                               atol=tol * ref["force"].abs().max().item())
+    np.testing.assert_allclose(out["virial"].cpu().numpy(), ref["virial"].cpu().numpy(), rtol=tol,
+                               atol=tol * ref["virial"].abs().max().item())
+    ext = m.compute_lower(ext_c, ext_t, torch.tensor(cand, device=DEV), s.nloc, numneigh=torch.tensor(nn, device=DEV))
+    fext = ext["force"].cpu().numpy()
+    assert fext.shape == (oc.shape[0], 3)
+    folded = np.zeros((s.nloc, 3))
+    np.add.at(folded, om, fext)
+    np.testing.assert_allclose(folded, ref["force"].cpu().numpy(), rtol=tol, atol=tol * ref["force"].abs().max().item())
