@@ -35,32 +35,49 @@ def _layer(l):
     return w, b, idt
 
 
-def mlp_value_derivs(x: np.ndarray, layers, fold_idt: bool = False):
+def mlp_value_derivs(x: np.ndarray, layers, fold_idt: bool = False, device=None):
     """Embedding net G(x) for scalar inputs x [n] with dG/dx and d2G/dx2,
-    each [n, M] (float64, tanh activation, MLPLayer skip rules)."""
-    y = np.asarray(x, dtype=np.float64).reshape(-1, 1)
-    dy = np.ones_like(y)
-    d2y = np.zeros_like(y)
+    each [n, M] (float64, tanh activation, MLPLayer skip rules).  With a
+    device (e.g. "cuda") the chain runs there in torch float64; the result
+    is returned as numpy either way."""
+    if device is not None:
+        import torch
+
+        dev = torch.device(device)
+        xp = torch
+        y = torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev).reshape(-1, 1)
+        cat = lambda a: torch.cat([a, a], dim=1)  # noqa: E731
+        arr = lambda a: torch.as_tensor(a, device=dev)  # noqa: E731
+    else:
+        xp = np
+        y = np.asarray(x, dtype=np.float64).reshape(-1, 1)
+        cat = lambda a: np.concatenate([a, a], axis=1)  # noqa: E731
+        arr = lambda a: a  # noqa: E731
+    dy = xp.ones_like(y)
+    d2y = xp.zeros_like(y)
     for l in layers:
         w, b, idt = _layer(l)
         nin, nout = w.shape
+        w, b = arr(w), arr(b)
         z = y @ w + b
         dz = dy @ w
         d2z = d2y @ w
-        t = np.tanh(z)
+        t = xp.tanh(z)
         g1 = 1.0 - t * t
         g2 = -2.0 * t * g1
-        scale = idt if (fold_idt and idt is not None) else 1.0
+        scale = arr(idt) if (fold_idt and idt is not None) else 1.0
         out = t * scale
         dout = g1 * dz * scale
         d2out = (g2 * dz * dz + g1 * d2z) * scale
         if nout == nin:
             out, dout, d2out = out + y, dout + dy, d2out + d2y
         elif nout == 2 * nin:
-            out = out + np.concatenate([y, y], axis=1)
-            dout = dout + np.concatenate([dy, dy], axis=1)
-            d2out = d2out + np.concatenate([d2y, d2y], axis=1)
+            out = out + cat(y)
+            dout = dout + cat(dy)
+            d2out = d2out + cat(d2y)
         y, dy, d2y = out, dout, d2out
+    if device is not None:
+        return y.cpu().numpy(), dy.cpu().numpy(), d2y.cpu().numpy()
     return y, dy, d2y
 
 
@@ -107,9 +124,10 @@ def env_mat_range(davg, dstd, rcut: float, rcut_smth: float, min_nbor_dist: floa
 
 def build_tables(emb_layers, davg, dstd, rcut: float, rcut_smth: float, min_nbor_dist: float, sel,
                  extrapolate: float = 5.0, stride0: float = 0.01, stride1: float = 0.1,
-                 check_frequency: int = -1, fold_idt: bool = False):
+                 check_frequency: int = -1, fold_idt: bool = False, device=None):
     """Tables of a type_one_side se_e2_a model: a list (one per neighbor type)
-    of (table [nspline, 6 M] float64, table_info [6] float64)."""
+    of (table [nspline, 6 M] float64, table_info [6] float64).  device: where
+    the value/slope/curvature chain of the nets runs (None = numpy)."""
     lower, upper = env_mat_range(davg, dstd, rcut, rcut_smth, min_nbor_dist)
     live = np.asarray(sel) > 0
     ll = float(np.min(lower[live]))
@@ -122,7 +140,7 @@ def build_tables(emb_layers, davg, dstd, rcut: float, rcut_smth: float, min_nbor
     tt[: int((uu - ll) / stride0)] = stride0
     out = []
     for layers in emb_layers:
-        vv, dd, d2 = mlp_value_derivs(xx, layers, fold_idt)
+        vv, dd, d2 = mlp_value_derivs(xx, layers, fold_idt, device=device)
         table = hermite_table(vv, dd, d2, tt)
         info = np.array([int(ll), int(uu), int(uu) * extrapolate, stride0, stride1, check_frequency],
                         dtype=np.float64)

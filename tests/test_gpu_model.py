@@ -210,3 +210,28 @@ def test_compute_lower_external_list(oracle, water_model, precision):
     folded = np.zeros((s.nloc, 3))
     np.add.at(folded, om, fext)
     np.testing.assert_allclose(folded, ref["force"].cpu().numpy(), rtol=tol, atol=tol * ref["force"].abs().max().item())
+
+
+def test_table_builder_on_gpu():
+    """The table builder's net chain on the GPU gives the CPU tables: every
+    coefficient's contribution over its interval, a_k t^k, to 1e-12 of the
+    largest (a_3..a_5 themselves divide last-bit differences of the node
+    values by t^3..t^5, so they are compared at that scale)."""
+    from dpmi import table
+
+    spec = systems.WATER_SPEC
+    params = systems.make_params(spec, seed=7)
+    _say("table builder on the GPU")
+    for fold in (False, True):
+        a = table.build_tables(params["embedding"], params["davg"], params["dstd"], spec.rcut, spec.rcut_smth, 0.9,
+                               spec.sel, fold_idt=fold)
+        b = table.build_tables(params["embedding"], params["davg"], params["dstd"], spec.rcut, spec.rcut_smth, 0.9,
+                               spec.sel, fold_idt=fold, device=DEV)
+        for (ta, ia), (tb, ib) in zip(a, b):
+            np.testing.assert_array_equal(ia, ib)
+            n = ta.shape[0]
+            tt = np.where(np.arange(n) < round((ia[1] - ia[0]) / ia[3]), ia[3], ia[4])
+            scale = tt[:, None, None] ** np.arange(6)[None, None, :]
+            xa = ta.reshape(n, -1, 6) * scale
+            xb = tb.reshape(n, -1, 6) * scale
+            np.testing.assert_allclose(xb, xa, rtol=0, atol=1e-12 * np.abs(xa).max())
