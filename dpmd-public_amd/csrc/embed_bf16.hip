@@ -23,8 +23,10 @@
 // order in which the [x, x] skip connections land on the same lane and
 // register: q1 = f (f < n1), q2 = q1(f mod n1) + 32*[f >= n1],
 // q3 = q2(f mod 2n1) + 64*[f >= 2n1]; padded features are exact zeros.
-// G^T R contracts over rows, so G goes once through LDS (transposed to
-// [feature][row]) and is reduced with MFMA against em^T.  The backward pass
+// G itself is never formed: with r3 = 1 / (1 + exp(2 x3)), tanh = 1 - 2 r3 and
+// G = idt3 (1 - 2 r3) + [h2, h2], so G^T R = idt3 (em^T 1) - 2 idt3 (em^T r3)
+// + em^T [h2, h2]; r3 and h2 are transposed per 16-row tile by MFMA against
+// an identity fragment and reduced with MFMA against em^T.  The backward pass
 // recomputes the forward per tile and runs the adjoint chain (dG = em dGR,
 // dh2 = dz3 W3^T + skip, dh1 = dz2 W2^T + skip, ds = dz1 . w1) in the same
 // register layout, so it needs no transposes at all.
@@ -45,7 +47,7 @@ constexpr int kMaxTypes = 16;
 constexpr int kFragFwd = 20;   // 4 (layer 2) + 16 (layer 3)
 constexpr int kFragBwd = 20;   // 16 (dh2) + 4 (dh1)
 constexpr int kConsts = 704;   // k w1, k b1, idt1 [32]; k b2, idt2 [64]; k b3, idt3 [128]; w1 [32];
-                               // idt2 / k [64]; idt3 / k [128]   (k = 2 log2 e)
+                               // idt2 / k [64]; 4 idt3 / k [128]   (k = 2 log2 e)
 constexpr int kQ3 = 128;       // internal width of G
 constexpr int kGStride = 40;   // LDS row stride (bf16) of the transposed G tile
 constexpr int kAtoms = 4;      // atoms (wavefronts) per workgroup
@@ -96,12 +98,16 @@ __device__ inline f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
 // Layer-1 feature owned by lane group g at B-slot u (also the K order phi).
 __device__ inline int phi(int g, int u) { return 16 * (u >> 2) + 4 * g + (u & 3); }
 
-// Recompute of the forward chain for one 16-row tile.  Outputs the tanh values
-// of every layer (for the backward) and G; all in the lane's layout.
+// Recompute of the forward chain for one 16-row tile, all in the lane's
+// layout.  Layer 3 is returned as r3 = 1 / (1 + 2^z3) only: with
+// tanh = 1 - 2 r3 the embedding output is G = idt3 (1 - 2 r3) + [h2, h2], and
+// the kernels contract r3 and h2 separately (G^T R = idt3 (sum_rows em)
+// - 2 idt3 (em^T r3) + em^T [h2, h2]), so G itself is never formed.
 struct TileFwd {
   float t1[8], h1[8];
-  float t2[16], h2[16];
-  float t3[32], g[32];
+  float t2[16];
+  bf16x8 b2[2];  // h2 in bf16: layer 3's B operand (features 32 kc + phi(g, u))
+  float r3[32];  // slot mt * 4 + v: internal feature 16 mt + 4 g + v
 };
 
 template <bool kSkip>
@@ -126,7 +132,6 @@ __device__ inline void forward_tile(float s, const bf16x8* __restrict__ frag, co
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
     z2[mt] = mfma(frag[mt * 64 + lane], b1, *reinterpret_cast<const f32x4*>(cst + 96 + 16 * mt + 4 * g));
-  bf16x8 b2[2];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     const int q = 16 * mt + 4 * g;
@@ -134,45 +139,43 @@ __device__ inline void forward_tile(float s, const bf16x8* __restrict__ frag, co
     for (int v = 0; v < 4; ++v) {
       if (kSkip && (mt & 1) && v == 3) {
         f.t2[mt * 4 + v] = 0.f;
-        f.h2[mt * 4 + v] = 0.f;
-        b2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)0.f;
+        f.b2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)0.f;
         continue;
       }
       float t = tanh_scaled(z2[mt][v]);
       f.t2[mt * 4 + v] = t;
-      f.h2[mt * 4 + v] = t * cst[160 + q + v] + f.h1[(mt & 1) * 4 + v];
-      b2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)f.h2[mt * 4 + v];
+      f.b2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(t * cst[160 + q + v] + f.h1[(mt & 1) * 4 + v]);
     }
   }
 #pragma unroll
   for (int mt = 0; mt < 8; ++mt) {
-    f32x4 z = mfma(frag[(4 + 2 * mt) * 64 + lane], b2[0], *reinterpret_cast<const f32x4*>(cst + 224 + 16 * mt + 4 * g));
-    z = mfma(frag[(5 + 2 * mt) * 64 + lane], b2[1], z);
-    const int q = 16 * mt + 4 * g;
+    f32x4 z = mfma(frag[(4 + 2 * mt) * 64 + lane], f.b2[0], *reinterpret_cast<const f32x4*>(cst + 224 + 16 * mt + 4 * g));
+    z = mfma(frag[(5 + 2 * mt) * 64 + lane], f.b2[1], z);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      if (kSkip && (mt & 1) && v == 3) {
-        f.t3[mt * 4 + v] = 0.f;
-        f.g[mt * 4 + v] = 0.f;
-        continue;
-      }
-      float t = tanh_scaled(z[v]);
-      f.t3[mt * 4 + v] = t;
-      f.g[mt * 4 + v] = t * cst[352 + q + v] + f.h2[(mt & 3) * 4 + v];
-    }
+    for (int v = 0; v < 4; ++v)
+      f.r3[mt * 4 + v] = (kSkip && (mt & 1) && v == 3) ? 0.f
+                                                        : __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(z[v]));
   }
+}
+
+// r3 of the 32-feature chunk kc as an MFMA operand (K = feature slot phi(g, u))
+__device__ inline bf16x8 r3_frag(const TileFwd& f, int kc) {
+  bf16x8 b;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) b[u] = (__bf16)f.r3[(2 * kc + (u >> 2)) * 4 + (u & 3)];
+  return b;
 }
 
 // Forward: per (atom, neighbor type) wavefront, GR partial [t][i][c][m]
 // (original feature order m < 4 n1).
 // Persistent grid: a workgroup loads its type's weight fragments once and its
 // four wavefronts walk atoms independently (no block barrier after set-up).
-// G^T R contracts over neighbor rows while G comes out of the layer-3 product
-// with the row as the lane index, so each 16-row tile of G is transposed by
-// MFMA against an identity fragment (exact for the bf16 operand): then the
+// G^T R contracts over neighbor rows while r3 and h2 come out of the layer
+// products with the row as the lane index, so each 16-row tile is transposed
+// by MFMA against an identity fragment (exact for the bf16 operand): then the
 // lane index is the feature and every lane group holds four rows, and two
 // tiles make the 32-row K operand (row order psi(g, u) = 4g + u for u < 4,
-// 16 + 4g + u - 4 otherwise) of GR^T += em^T G.
+// 16 + 4g + u - 4 otherwise) of the products em^T r3, em^T h2 and em^T 1.
 template <bool kSkip>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_embed_fwd(float* __restrict__ gr_part, const float* __restrict__ em, const bf16x8* __restrict__ frag_all,
@@ -195,6 +198,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   for (int j = 0; j < 2; ++j)
 #pragma unroll
     for (int u = 0; u < 8; ++u) eye[j][u] = (__bf16)(((u >> 2) == j && 4 * g + (u & 3) == r) ? 1.f : 0.f);
+  bf16x8 ones;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) ones[u] = (__bf16)1.f;
   float* sem = s_em[w];
   const int stride = gridDim.x * kAtoms;
   auto load_em = [&](int ai, int row) {
@@ -204,9 +210,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   };
   float4 ev_next[2] = {load_em(blockIdx.x * kAtoms + w, r), load_em(blockIdx.x * kAtoms + w, 16 + r)};
   for (int i = blockIdx.x * kAtoms + w; i < nloc; i += stride) {
-    f32x4 acc[8];
+    // D(M = c, N = internal feature 16 nt + r): acc = em^T r3, acc_h = em^T h2,
+    // acc_s = sum over rows of em (the same in every column)
+    f32x4 acc[8], acc_h[4], acc_s = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int nt = 0; nt < 8; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) acc_h[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int ch = 0; ch < nchunk; ++ch) {
       const float4 ev_cur[2] = {ev_next[0], ev_next[1]};
       const int ni = ch + 1 < nchunk ? i : i + stride, nrow = ch + 1 < nchunk ? (ch + 1) * 32 : 0;
@@ -214,7 +224,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       ev_next[1] = load_em(ni, nrow + 16 + r);
       // weights and constants stay in LDS (re-read per chunk), not in registers
       asm volatile("" ::: "memory");
-      bf16x8 gt[2][4];  // transposed G: [tile][pair of 16-feature tiles], 4 rows each
+      // transposed tiles: [tile][pair of 16-feature tiles], 4 rows each
+      bf16x8 rt3[2][4], ht2[2][2];
 #pragma unroll
       for (int rt = 0; rt < 2; ++rt) {
         const float4 ev = ev_cur[rt];
@@ -223,44 +234,65 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
           for (int kc = 0; kc < 4; ++kc)
 #pragma unroll
-            for (int u = 0; u < 8; ++u) gt[rt][kc][u] = (__bf16)0.f;
+            for (int u = 0; u < 8; ++u) rt3[rt][kc][u] = (__bf16)0.f;
+#pragma unroll
+          for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) ht2[rt][kc][u] = (__bf16)0.f;
           continue;
         }
         TileFwd f;
         forward_tile<kSkip>(ev.x, s_frag, cst, g, lane, f);
 #pragma unroll
         for (int kc = 0; kc < 4; ++kc) {
-          bf16x8 gb;
-#pragma unroll
-          for (int u = 0; u < 8; ++u) gb[u] = (__bf16)f.g[(2 * kc + (u >> 2)) * 4 + (u & 3)];
+          const bf16x8 rb = r3_frag(f, kc);
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
-            const f32x4 d = mfma(gb, eye[j], f32x4{0.f, 0.f, 0.f, 0.f});
+            const f32x4 d = mfma(rb, eye[j], f32x4{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
-            for (int v = 0; v < 4; ++v) gt[rt][kc][4 * j + v] = (__bf16)d[v];
+            for (int v = 0; v < 4; ++v) rt3[rt][kc][4 * j + v] = (__bf16)d[v];
           }
         }
+#pragma unroll
+        for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            const f32x4 d = mfma(f.b2[kc], eye[j], f32x4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+            for (int v = 0; v < 4; ++v) ht2[rt][kc][4 * j + v] = (__bf16)d[v];
+          }
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      // em^T as A (M = c < 4, K = rows in psi order)
+      // em^T as A (M = c, K = rows in psi order); rows M >= 4 of the products
+      // are never read, so their A values are left as they come
       bf16x8 a;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int row = u < 4 ? 4 * g + u : 16 + 4 * g + (u - 4);
-        const float x = sem[4 * row + (r & 3)];
-        a[u] = (__bf16)(r < 4 ? x : 0.f);
+        a[u] = (__bf16)sem[4 * row + (r & 3)];
       }
 #pragma unroll
       for (int nt = 0; nt < 8; ++nt) {
         bf16x8 b;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          b[v] = gt[0][nt >> 1][4 * (nt & 1) + v];
-          b[4 + v] = gt[1][nt >> 1][4 * (nt & 1) + v];
+          b[v] = rt3[0][nt >> 1][4 * (nt & 1) + v];
+          b[4 + v] = rt3[1][nt >> 1][4 * (nt & 1) + v];
         }
         acc[nt] = mfma(a, b, acc[nt]);
       }
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        bf16x8 b;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          b[v] = ht2[0][nt >> 1][4 * (nt & 1) + v];
+          b[4 + v] = ht2[1][nt >> 1][4 * (nt & 1) + v];
+        }
+        acc_h[nt] = mfma(a, b, acc_h[nt]);
+      }
+      acc_s = mfma(a, ones, acc_s);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
@@ -274,10 +306,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       float* out = gr_part + ((int64_t)t * nloc + i) * 4 * M;
 #pragma unroll
       for (int nt = 0; nt < 8; ++nt) {
-        const int m = orig_of_q3(16 * nt + r, n1);
+        const int q = 16 * nt + r;
+        const int m = orig_of_q3(q, n1);
+        const float idt3 = cst[352 + q];
         if (m >= 0)
 #pragma unroll
-          for (int v = 0; v < 4; ++v) out[v * M + m] = acc[nt][v];
+          for (int v = 0; v < 4; ++v) out[v * M + m] = idt3 * (acc_s[v] - 2.f * acc[nt][v]) + acc_h[nt & 3][v];
       }
     }
   }
@@ -448,14 +482,18 @@ __global__ void __launch_bounds__(256)
 
 // Backward: net_deriv[i][sec_t + row][c] = dE/d em, recomputing the forward.
 // Persistent grid (weight fragments loaded into LDS once per workgroup);
-// each wavefront keeps dGR of its atom in LDS as [q3][c] so the mostly-zero
-// dGR^T A fragments are built per tile from one 16-byte read.
+// each wavefront keeps dGR of its atom in LDS twice: as float [q3][c] for the
+// per-atom operands, and as 16-byte bf16 rows [q3] = (dGR[0..3][q3], 0 x 4),
+// the A operand of dG = em dGR read with one 16-byte load per 16 features.
 template <bool kSkip>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_embed_bwd(float* __restrict__ net_deriv, const float* __restrict__ em, const float* __restrict__ dgr,
                 const bf16x8* __restrict__ frag_all, const float* __restrict__ consts, int nloc, EmbedParams p) {
   __shared__ bf16x8 s_frag[(kFragFwd + kFragBwd) * 64];
   __shared__ __attribute__((aligned(16))) float s_dg[kAtoms][kQ3 * 4];
+  __shared__ bf16x8 s_dgb[kAtoms][kQ3];
+  __shared__ bf16x8 s_dop[kAtoms][6 * 16];
+  __shared__ __attribute__((aligned(16))) float s_kin[kAtoms][4];
   __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
   const int t = blockIdx.y;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
@@ -469,6 +507,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   const int sel = p.sec[t + 1] - p.sec[t];
   const int ntile = (sel + 15) / 16;
   float* sdg = s_dg[w];
+  bf16x8* sdgb = s_dgb[w];
+  bf16x8* sdop = s_dop[w];
+  float* skin = s_kin[w];
   const int stride = gridDim.x * kAtoms;
   // em rows are prefetched one tile ahead (across atoms too) so the HBM
   // latency overlaps the current tile's work
@@ -485,15 +526,43 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     for (int k = lane; k < 4 * kQ3; k += 64) sdg[(k & (kQ3 - 1)) * 4 + (k >> 7)] = dg[k];
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    // dGR as A (M = c, K = q3 chunk in phi order): rows c >= 4 are zero
-    bf16x8 a_dg[4];
+    for (int q = lane; q < kQ3; q += 64) {
+      const float4 x = *reinterpret_cast<const float4*>(sdg + 4 * q);
+      bf16x8 v;
+      v[0] = (__bf16)x.x;
+      v[1] = (__bf16)x.y;
+      v[2] = (__bf16)x.z;
+      v[3] = (__bf16)x.w;
 #pragma unroll
-    for (int kc = 0; kc < 4; ++kc)
+      for (int u = 4; u < 8; ++u) v[u] = (__bf16)0.f;
+      sdgb[q] = v;
+    }
+    // Operands of the direct term dem[c] = sum_q3 G[q3] dGR[c][q3] with
+    // G = idt3 (1 - 2 r3) + [h2, h2]:  dem = K_c + sum_q3 r3 (-2 idt3 dGR)
+    // + sum_q2 h2 (dGR[q2] + dGR[q2 + 64]), A (M = c, K = feature slot); rows
+    // c >= 4 of the product are never read, so an A fragment depends on the
+    // lane only through (g, c = r % 4): sdop[f][4 g + c], f = 0..3 the r3
+    // chunks, 4..5 the h2 chunks, read per tile (registers are scarce here).
+    for (int k = lane; k < 96; k += 64) {
+      const int fi = k >> 4, gg = (k >> 2) & 3, cc = k & 3;
+      bf16x8 v;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const float x = sdg[(32 * kc + phi(g, u)) * 4 + (r & 3)];
-        a_dg[kc][u] = (__bf16)(r < 4 ? x : 0.f);
+        const int q = 32 * (fi & 3) + phi(gg, u);
+        v[u] = fi < 4 ? (__bf16)(-2.f * cst[352 + q] * sdg[q * 4 + cc])
+                      : (__bf16)(sdg[q * 4 + cc] + sdg[(q + 64) * 4 + cc]);
       }
+      sdop[k] = v;
+    }
+    // K_c = sum_q3 idt3 dGR[c]: lane (q0 = lane / 4, c = lane % 4) sums q3 = q0 + 16 j
+    float kpart = 0.f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) kpart += cst[352 + (lane >> 2) + 16 * j] * sdg[lane + 64 * j];
+#pragma unroll
+    for (int off = 4; off < 64; off <<= 1) kpart += __shfl_xor(kpart, off, 64);
+    if (lane < 4) skin[lane] = kpart;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
     float* nd_i = net_deriv + ((int64_t)i * p.nnei + p.sec[t]) * 4;
     for (int tl = 0; tl < ntile; ++tl) {
       const int row = tl * 16 + r;
@@ -505,16 +574,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       asm volatile("" ::: "memory");
       TileFwd f;
       forward_tile<kSkip>(ev.x, s_frag, cst, g, lane, f);
-      // direct term dem[c] = sum_q3 G dGR[c]  -> D(M = c, N = rows)
-      f32x4 dem = f32x4{0.f, 0.f, 0.f, 0.f};
+      // direct term -> D(M = c, N = rows)
+      const bf16x8* dop = sdop + 4 * g + (r & 3);
+      f32x4 dem = *reinterpret_cast<const f32x4*>(skin);
 #pragma unroll
-      for (int kc = 0; kc < 4; ++kc) {
-        bf16x8 bg;
+      for (int kc = 0; kc < 4; ++kc) dem = mfma(dop[16 * kc], r3_frag(f, kc), dem);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) bg[u] = (__bf16)f.g[(2 * kc + (u >> 2)) * 4 + (u & 3)];
-        dem = mfma(a_dg[kc], bg, dem);
-      }
-      // em as B (N = rows, K = c): K slots 0..3 in lane group 0
+      for (int kc = 0; kc < 2; ++kc) dem = mfma(dop[16 * (4 + kc)], f.b2[kc], dem);
+      // em as B (N = rows, K = c): K slots 0..3 in lane group 0, all others zero
       bf16x8 b_em;
       b_em[0] = (__bf16)(g == 0 ? ev.x : 0.f);
       b_em[1] = (__bf16)(g == 0 ? ev.y : 0.f);
@@ -522,35 +589,31 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       b_em[3] = (__bf16)(g == 0 ? ev.w : 0.f);
 #pragma unroll
       for (int u = 4; u < 8; ++u) b_em[u] = (__bf16)0.f;
-      // dG = em dGR -> D(M = q3, N = rows); dz3 = dG * idt3 * (1 - t3^2); skip sums dG[q] + dG[q + 64]
-      float skip[16];
+      // dG = em dGR -> D(M = q3, N = rows), A = row q3 of sdgb (its K slots
+      // 4..7 are zero; those of lane groups 1..3 meet the zero K slots of b_em);
+      // dz3 = dG idt3 (1 - t3^2) = dG 4 idt3 r3 (1 - r3); skip = dG[q] + dG[q + 64]
+      f32x4 skip[4];
       bf16x8 b_dz3[4];
+      // keep the sdgb reads below the recompute (hoisted, they would hold 32
+      // registers across it)
+      asm volatile("" ::: "memory");
 #pragma unroll
-      for (int mt = 0; mt < 8; ++mt) {
-        bf16x8 a;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] = (__bf16)0.f;
-        if (g == 0) {
-          const float4 d4 = *reinterpret_cast<const float4*>(&sdg[(16 * mt + r) * 4]);
-          a[0] = (__bf16)d4.x;
-          a[1] = (__bf16)d4.y;
-          a[2] = (__bf16)d4.z;
-          a[3] = (__bf16)d4.w;
-        }
-        f32x4 d = mfma(a, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
+      for (int mt = 0; mt < 4; ++mt) {
+        const bf16x8 a_lo = sdgb[16 * mt + r], a_hi = sdgb[16 * (mt + 4) + r];
+        const f32x4 d_lo = mfma(a_lo, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
+        const f32x4 d_hi = mfma(a_hi, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
+        skip[mt] = mfma(a_hi, b_em, d_lo);
         const int q = 16 * mt + 4 * g;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          if (mt < 4)
-            skip[mt * 4 + v] = d[v];
-          else
-            skip[(mt - 4) * 4 + v] += d[v];
           if (kSkip && (mt & 1) && v == 3) {  // empty slot: no weights read it
             b_dz3[mt >> 1][(mt & 1) * 4 + v] = (__bf16)0.f;
+            b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (__bf16)0.f;
             continue;
           }
-          float tt = f.t3[mt * 4 + v];
-          b_dz3[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(d[v] * cst[576 + q + v] * (1.f - tt * tt));
+          const float rl = f.r3[mt * 4 + v], rh = f.r3[(mt + 4) * 4 + v];
+          b_dz3[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(d_lo[v] * cst[576 + q + v] * (rl - rl * rl));
+          b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (__bf16)(d_hi[v] * cst[640 + q + v] * (rh - rh * rh));
         }
       }
       // dh2 = dz3 W3^T + skip;  dz2 = dh2 * idt2 * (1 - t2^2)
@@ -558,13 +621,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       bf16x8 b_dz2[2];
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt) {
-        f32x4 d = f32x4{0.f, 0.f, 0.f, 0.f};
+        f32x4 d = skip[mt];
 #pragma unroll
         for (int kc = 0; kc < 4; ++kc) d = mfma(fb[(mt * 4 + kc) * 64 + lane], b_dz3[kc], d);
         const int q = 16 * mt + 4 * g;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          float x = d[v] + skip[mt * 4 + v];
+          float x = d[v];
           dh2[mt * 4 + v] = x;
           if (kSkip && (mt & 1) && v == 3) {
             b_dz2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)0.f;

@@ -82,7 +82,8 @@ def pack_embedding(emb_layers: list) -> tuple[np.ndarray, np.ndarray, int]:
     Returns (frags uint16 [ntypes, 40, 64, 8] holding bf16 bits,
     consts float32 [ntypes, 704], n1).  consts: K2 w1 [0:32], K2 b1 [32:64],
     idt1 [64:96], K2 b2 [96:160], idt2 [160:224], K2 b3 [224:352],
-    idt3 [352:480], w1 [480:512], idt2 / K2 [512:576], idt3 / K2 [576:704];
+    idt3 [352:480], w1 [480:512], idt2 / K2 [512:576], 4 idt3 / K2 [576:704]
+    (the backward's dz3 = dG idt3 (1 - t3^2) is formed as dG (4 idt3) r3 (1 - r3));
     all 40 fragments hold bf16(K2 W)."""
     ntypes = len(emb_layers)
     (w1, _, _) = _layer(emb_layers[0][0])
@@ -111,7 +112,7 @@ def pack_embedding(emb_layers: list) -> tuple[np.ndarray, np.ndarray, int]:
         c[352 + q3] = i3
         c[480 + q1] = w1[0]
         c[512 + q2] = i2 / K2
-        c[576 + q3] = i3 / K2
+        c[576 + q3] = 4.0 * i3 / K2
         K = phi[lg]  # [64, 8] K slot -> feature offset within a 32-chunk
         f = frags[t]
         for mt in range(4):  # layer 2, A = K2 W2^T (M = q2, K = q1)

@@ -57,16 +57,22 @@ def forward_tile(s, frag, cst):
             h2[:, mt * 4 + v] = t2[:, mt * 4 + v] * cst[160 + q + v] + h1[:, (mt & 1) * 4 + v]
     b2 = [np.concatenate([h2[:, (2 * kc) * 4:(2 * kc) * 4 + 4], h2[:, (2 * kc + 1) * 4:(2 * kc + 1) * 4 + 4]], axis=1)
           for kc in range(2)]
-    t3 = np.zeros((64, 32))
+    r3 = np.zeros((64, 32))
     G = np.zeros((64, 32))
     for mt in range(8):
         c0 = np.stack([cst[224 + 16 * mt + 4 * LG + v] for v in range(4)], axis=1)
         z = mfma(frag[5 + 2 * mt], b2[1], mfma(frag[4 + 2 * mt], b2[0], c0))
         q = 16 * mt + 4 * LG
         for v in range(4):
-            t3[:, mt * 4 + v] = tanh_scaled(z[:, v])
-            G[:, mt * 4 + v] = t3[:, mt * 4 + v] * cst[352 + q + v] + h2[:, (mt & 3) * 4 + v]
-    return t1, h1, t2, h2, t3, G
+            r3[:, mt * 4 + v] = 1.0 / (1.0 + np.exp2(z[:, v]))
+            # what the kernels contract: idt3 (1 - 2 r3) + [h2, h2] (never formed there)
+            G[:, mt * 4 + v] = (1.0 - 2.0 * r3[:, mt * 4 + v]) * cst[352 + q + v] + h2[:, (mt & 3) * 4 + v]
+    return t1, h1, t2, h2, r3, G
+
+
+def frag_of(x, kc):
+    """[64, 32] lane values -> the MFMA operand of 32-feature chunk kc (K = phi slot)."""
+    return np.stack([x[:, (2 * kc + (u >> 2)) * 4 + (u & 3)] for u in range(8)], axis=1)
 
 
 def lanes_to_rows(G):
@@ -129,35 +135,49 @@ def test_forward_tile_matches_mlp(packed):
 
 
 def test_forward_gr_transpose(packed):
-    """k_embed_fwd's G^T R for one 32-row chunk: G of each 16-row tile
-    transposed by MFMA against identity fragments, rows in psi order, em^T as
-    the A operand -> GR^T (M = c, N = feature), against em^T G in float64
-    with G rounded to bf16."""
+    """k_embed_fwd's G^T R for one 32-row chunk: r3 and h2 of each 16-row
+    tile transposed by MFMA against identity fragments, rows in psi order,
+    em^T as the A operand (rows c >= 4 left as they come) ->
+    GR^T = idt3 (em^T 1 - 2 em^T r3) + em^T [h2, h2] (M = c, N = feature),
+    against em^T G in float64 with r3 and h2 rounded to bf16."""
     params, frags, consts, n1 = packed
+    c = consts[1]
     rng = np.random.default_rng(5)
     em = rng.normal(size=(32, 4))
     eye = [np.array([[1.0 if (u >> 2) == j and 4 * LG[l] + (u & 3) == LR[l] else 0.0 for u in range(8)]
                      for l in range(64)]) for j in range(2)]
-    gt = np.zeros((2, 4, 64, 8))
+    rt3 = np.zeros((2, 4, 64, 8))
+    ht2 = np.zeros((2, 2, 64, 8))
     Gfull = np.zeros((32, 128))
     for rt in range(2):
-        _, _, _, _, _, G = forward_tile(em[16 * rt + LR, 0], frags[1], consts[1])
-        Gb = bf16(G)
+        _, _, _, h2, r3, _ = forward_tile(em[16 * rt + LR, 0], frags[1], c)
+        r3b, h2b = bf16(r3), bf16(h2)
+        Gb = np.zeros((64, 32))
+        for mt in range(8):
+            q = 16 * mt + 4 * LG
+            for v in range(4):
+                Gb[:, mt * 4 + v] = (1.0 - 2.0 * r3b[:, mt * 4 + v]) * c[352 + q + v] + h2b[:, (mt & 3) * 4 + v]
         Gfull[16 * rt:16 * rt + 16] = lanes_to_rows(Gb)
         for kc in range(4):
-            gb = np.stack([Gb[:, (2 * kc + (u >> 2)) * 4 + (u & 3)] for u in range(8)], axis=1)
             for j in range(2):
-                gt[rt, kc][:, 4 * j:4 * j + 4] = mfma(gb, eye[j])
+                rt3[rt, kc][:, 4 * j:4 * j + 4] = mfma(frag_of(r3b, kc), eye[j])
+        for kc in range(2):
+            for j in range(2):
+                ht2[rt, kc][:, 4 * j:4 * j + 4] = mfma(frag_of(h2b, kc), eye[j])
     psi = np.array([[4 * g + u if u < 4 else 16 + 4 * g + u - 4 for u in range(8)] for g in range(4)])
     emb = bf16(em)
-    a = np.where((LR < 4)[:, None], emb[psi[LG], np.minimum(LR, 3)[:, None]], 0.0)
+    a = emb[psi[LG], (LR & 3)[:, None]]
+    acc = [mfma(a, np.concatenate([rt3[0, nt >> 1][:, 4 * (nt & 1):4 * (nt & 1) + 4],
+                                   rt3[1, nt >> 1][:, 4 * (nt & 1):4 * (nt & 1) + 4]], axis=1)) for nt in range(8)]
+    acc_h = [mfma(a, np.concatenate([ht2[0, nt >> 1][:, 4 * (nt & 1):4 * (nt & 1) + 4],
+                                     ht2[1, nt >> 1][:, 4 * (nt & 1):4 * (nt & 1) + 4]], axis=1)) for nt in range(4)]
+    acc_s = mfma(a, np.ones((64, 8)))
     got = np.zeros((4, 128))
+    l0 = LG == 0
     for nt in range(8):
-        b = np.concatenate([gt[0, nt >> 1][:, 4 * (nt & 1):4 * (nt & 1) + 4],
-                            gt[1, nt >> 1][:, 4 * (nt & 1):4 * (nt & 1) + 4]], axis=1)
-        d = mfma(a, b)
+        q = 16 * nt + LR[l0]
         for v in range(4):
-            got[v, 16 * nt + LR[LG == 0]] = d[LG == 0][:, v]
+            got[v, q] = c[352 + q] * (acc_s[l0][:, v] - 2.0 * acc[nt][l0][:, v]) + acc_h[nt & 3][l0][:, v]
     ref = emb.T @ Gfull
     np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
 
@@ -175,35 +195,45 @@ def test_backward_tile_matches_autograd(packed):
     dgr[:, q3] = rng.normal(size=(4, 4 * n1))
     f = frags[t]
     c = consts[t]
-    t1, h1, t2, h2, t3, G = forward_tile(em[LR, 0], f, c)
-    # the kernel's backward dataflow
-    a_dgt = [np.where(((LG == 0)[:, None]) & (np.arange(8)[None, :] < 4),
-                      dgr[np.minimum(np.arange(8), 3)[None, :], (16 * mt + LR)[:, None]], 0.0) for mt in range(8)]
-    a_dg = [np.where((LR < 4)[:, None], dgr[np.minimum(LR, 3)[:, None], 32 * kc + phi(LG[:, None], np.arange(8)[None, :])],
-                     0.0) for kc in range(4)]
+    t1, h1, t2, h2, r3, G = forward_tile(em[LR, 0], f, c)
+    # activations unrounded here: this checks the dataflow, not the bf16 error
+    r3b, h2b = r3, h2
+    # the kernel's backward dataflow: dem = K_c + sum r3 (-2 idt3 dGR) + sum h2 (dGR[q2] + dGR[q2 + 64])
+    kslot = 32 * np.arange(4)[:, None, None] + phi(LG[:, None], np.arange(8)[None, :])[None]  # [kc, 64, 8]
+    crow = (LR & 3)[:, None]
+    a_r = [-2.0 * c[352 + kslot[kc]] * dgr[crow, kslot[kc]] for kc in range(4)]
+    a_h = [dgr[crow, kslot[kc]] + dgr[crow, kslot[kc] + 64] for kc in range(2)]
+    kc_all = (c[352:480][None, :] * dgr).sum(axis=1)  # K_c
+    dem = np.tile(kc_all, (64, 1))
+    for kc in range(4):
+        dem = mfma(a_r[kc], frag_of(r3b, kc), dem)
+    for kc in range(2):
+        dem = mfma(a_h[kc], frag_of(h2b, kc), dem)
+    # dG A operand: row q3 of the bf16 table (dGR[0..3][q3], 0 x 4) in every lane group
+    sdgb = np.zeros((128, 8))
+    sdgb[:, :4] = dgr.T
     b_em = np.zeros((64, 8))
     b_em[LG == 0, :4] = em[LR[LG == 0]]
-    dem = np.zeros((64, 4))
-    for kc in range(4):
-        bg = np.stack([G[:, (2 * kc + (u >> 2)) * 4 + (u & 3)] for u in range(8)], axis=1)
-        dem = mfma(a_dg[kc], bg, dem)
-    dgv = np.zeros((64, 32))
     bdz3 = [np.zeros((64, 8)) for _ in range(4)]
-    for mt in range(8):
-        d = mfma(a_dgt[mt], b_em)
+    skip = []
+    for mt in range(4):
+        a_lo, a_hi = sdgb[16 * mt + LR], sdgb[16 * (mt + 4) + LR]
+        d_lo, d_hi = mfma(a_lo, b_em), mfma(a_hi, b_em)
+        skip.append(mfma(a_hi, b_em, d_lo))
         q = 16 * mt + 4 * LG
         for v in range(4):
-            dgv[:, mt * 4 + v] = d[:, v]
-            bdz3[mt >> 1][:, (mt & 1) * 4 + v] = d[:, v] * c[576 + q + v] * (1 - t3[:, mt * 4 + v] ** 2)
+            rl, rh = r3[:, mt * 4 + v], r3[:, (mt + 4) * 4 + v]
+            bdz3[mt >> 1][:, (mt & 1) * 4 + v] = d_lo[:, v] * c[576 + q + v] * (rl - rl * rl)
+            bdz3[(mt + 4) >> 1][:, (mt & 1) * 4 + v] = d_hi[:, v] * c[640 + q + v] * (rh - rh * rh)
     dh2 = np.zeros((64, 16))
     bdz2 = [np.zeros((64, 8)) for _ in range(2)]
     for mt in range(4):
-        d = np.zeros((64, 4))
+        d = skip[mt]
         for kc in range(4):
             d = mfma(f[20 + mt * 4 + kc], bdz3[kc], d)
         q = 16 * mt + 4 * LG
         for v in range(4):
-            x = d[:, v] + dgv[:, mt * 4 + v] + dgv[:, (mt + 4) * 4 + v]
+            x = d[:, v]
             dh2[:, mt * 4 + v] = x
             bdz2[mt >> 1][:, (mt & 1) * 4 + v] = x * c[512 + q + v] * (1 - t2[:, mt * 4 + v] ** 2)
     ds = np.zeros(64)
