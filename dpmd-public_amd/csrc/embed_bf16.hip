@@ -36,6 +36,7 @@
 #include <cstring>
 
 #include "dpmi_internal.h"
+#include "embed_layout.h"
 
 namespace dpmi {
 namespace {
@@ -58,31 +59,6 @@ struct EmbedParams {
   int n1;
   int sec[kMaxTypes + 1];
 };
-
-// Layer-1 slots.  With n1 <= 28 the slots 19, 23, 27, 31 stay empty: they are
-// B-slot u = 7 of every lane group (phi(g, 7) = 19 + 4 g), and through the
-// skip connections the (mt odd, v = 3) accumulator rows of layers 2 and 3, so
-// the kernels skip those tanh evaluations outright.  Wider layers use slots
-// 0 .. n1 - 1.
-__host__ __device__ inline bool skip_layout(int n1) { return n1 <= 28; }
-
-__host__ __device__ inline int q1_of(int f, int n1) {
-  return (!skip_layout(n1) || f < 16) ? f : 16 + 4 * ((f - 16) / 3) + (f - 16) % 3;
-}
-
-// original layer-1 feature in slot q, or -1 for an empty slot
-__host__ __device__ inline int f1_of(int q, int n1) {
-  int f = q;
-  if (skip_layout(n1) && q >= 16) f = (q & 3) == 3 ? -1 : q - ((q - 16) >> 2);
-  return f < n1 ? f : -1;
-}
-
-// original feature (0 .. 4 n1 - 1) of internal G index q, or -1 for padding
-__host__ __device__ inline int orig_of_q3(int q, int n1) {
-  const int base = q >= 64 ? 2 * n1 : 0, q2 = q & 63;
-  const int f1 = f1_of(q2 & 31, n1);
-  return f1 < 0 ? -1 : base + (q2 >= 32 ? n1 : 0) + f1;
-}
 
 // tanh(x) from z = 2 log2(e) x: 1 - 2 / (2^z + 1), exact limits at +-inf.
 // The forward weights, biases and layer-1 constants are stored pre-scaled by
@@ -317,14 +293,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   }
 }
 
-// internal G index of original feature f (n1 = width of layer 1)
-__device__ inline int q3_of(int f, int n1) {
-  int n2 = 2 * n1;
-  int base = f >= n2 ? 64 : 0;
-  int f2 = f >= n2 ? f - n2 : f;
-  return base + (f2 >= n1 ? 32 + q1_of(f2 - n1, n1) : q1_of(f2, n1));
-}
-
 // GR = sum_t GR_t / nnei (original order, saved for the backward) and
 // D[m][a] = sum_c GR[c][m] GR[c][a] (se_a.py:851-858).
 template <typename DT>
@@ -386,8 +354,8 @@ __global__ void __launch_bounds__(256)
         }
       }
     }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) dgr[(int64_t)i * 4 * kQ3 + c * kQ3 + q] = out[c] * inv_nnei;
+    *reinterpret_cast<float4*>(dgr + ((int64_t)i * kQ3 + q) * 4) =
+        make_float4(out[0] * inv_nnei, out[1] * inv_nnei, out[2] * inv_nnei, out[3] * inv_nnei);
   }
 }
 
@@ -476,8 +444,8 @@ __global__ void __launch_bounds__(256)
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  float* out = dgr + (int64_t)i * 4 * kQ3;
-  for (int k = lane; k < 4 * kQ3; k += 64) out[k] = so[k] * inv_nnei;
+  float* out = dgr + (int64_t)i * 4 * kQ3;  // [q3][c]
+  for (int k = lane; k < 4 * kQ3; k += 64) out[k] = so[(k & 3) * kQ3 + (k >> 2)] * inv_nnei;
 }
 
 // Backward: net_deriv[i][sec_t + row][c] = dE/d em, recomputing the forward.
@@ -521,9 +489,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   };
   float4 ev_next = load_em(blockIdx.x * kAtoms + w, 0);
   for (int i = blockIdx.x * kAtoms + w; i < nloc; i += stride) {
-    const float* dg = dgr + (int64_t)i * 4 * kQ3;
-    // stage dGR transposed: sdg[q3 * 4 + c]
-    for (int k = lane; k < 4 * kQ3; k += 64) sdg[(k & (kQ3 - 1)) * 4 + (k >> 7)] = dg[k];
+    // stage dGR [q3][c] of the atom: sdg[q3 * 4 + c]
+    {
+      // (the lane index passes through an opaque copy: hoisted address
+      // arithmetic here made the allocator spill inside the atom loop)
+      const float4* dg = reinterpret_cast<const float4*>(dgr + (int64_t)i * 4 * kQ3);
+      int l = lane;
+      asm volatile("" : "+v"(l));
+      const float4 x0 = dg[l], x1 = dg[l + 64];
+      reinterpret_cast<float4*>(sdg)[l] = x0;
+      reinterpret_cast<float4*>(sdg)[l + 64] = x1;
+    }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     for (int q = lane; q < kQ3; q += 64) {

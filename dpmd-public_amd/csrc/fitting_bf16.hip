@@ -31,6 +31,7 @@
 #include <cstring>
 
 #include "dpmi_internal.h"
+#include "embed_layout.h"
 
 namespace dpmi {
 namespace {
@@ -46,10 +47,17 @@ constexpr int kXS = kKB + 8;    // LDS row stride (bf16) of the staged D block
 constexpr int kHS = kH + 8;     // LDS row stride (bf16) of the hidden-layer buffer
 constexpr int kMaxTypes = 16;
 constexpr int kCst = 7 * kH + 8;  // b1 idt1 b2 idt2 b3 idt3 w4 [kH each], b4
+constexpr int kMaxGR = 104;       // widest G^T R (M) the fused descriptor form stages
+constexpr int kGRS = 4 * kMaxGR + 4;  // LDS floats per atom of the staged GR ([m][c], padded)
+constexpr int kMaxTypesGR = 4;        // neighbor types whose partials the fused form sums
 
 // Fragment offsets (in 64-lane fragments of bf16x8) inside one type's block.
 struct FitParams {
   int ntypes;
+  int nloc;
+  int M;       // fused descriptor form: G^T R width (4 n1), axis 16
+  int n1;
+  float inv_nnei;
   int k1;    // width of D (layer-1 input)
   int nk1;   // 32-wide K chunks of layer 1 (k1 padded to a multiple of 64, / 32)
   int nm1;   // 16-row M tiles of the dD product (k1 / 16)
@@ -64,6 +72,15 @@ __device__ inline float fast_tanh(float x) {
 
 __device__ inline f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Sum over the four 16-lane groups, the same value in every lane: the gfx950
+// row swaps (lanes l <-> l ^ 16, then l <-> l ^ 32) stay in the VALU.
+__device__ inline float sum_lane_groups(float x) {
+  const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  const float y = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+  const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(y), __float_as_uint(y), false, false);
+  return __uint_as_float(b[0]) + __uint_as_float(b[1]);
 }
 
 // B fragment of 16 atoms (rows 16s..16s+15 of an LDS [atom][stride] buffer),
@@ -112,14 +129,25 @@ __device__ inline void hidden_product(f32x4 (&acc)[4][4], const bf16x8* __restri
   }
 }
 
+// With kGR the descriptor is formed inside the kernel (the fused descriptor
+// form, axis 16): the workgroup stages GR = sum_t gr_part / nnei of its 64
+// atoms in LDS ([atom][m][c], fp32), builds each 64-wide block of
+// D[m][a] = sum_c GR[c][m] GR[c][a] (se_a.py:851-858) into the layer-1
+// staging buffer, and ends with dE/dGR instead of dE/dD:
+//   dGR[c][m] = sum_a dD[m][a] GR[c][a] + [m < 16] sum_m' dD[m'][m] GR[c][m'],
+// written / nnei in the embedding kernels' internal order as dgr
+// [nloc, 128, 4] (component fastest: one 16-byte store per atom and feature).
+template <bool kGR>
 __global__ void __launch_bounds__(256)
-    k_fit(double* __restrict__ atom_energy, float* __restrict__ ddesc, const __bf16* __restrict__ desc,
-          const int* __restrict__ perm, const int* __restrict__ type_start, const bf16x8* __restrict__ frag_all,
-          const float* __restrict__ consts, const double* __restrict__ ebias, FitParams p) {
+    k_fit(double* __restrict__ atom_energy, float* __restrict__ dout, const __bf16* __restrict__ desc,
+          const float* __restrict__ gr_part, const int* __restrict__ perm, const int* __restrict__ type_start,
+          const bf16x8* __restrict__ frag_all, const float* __restrict__ consts, const double* __restrict__ ebias,
+          FitParams p) {
   __shared__ __attribute__((aligned(16))) __bf16 s_x[2][kTile * kXS];
   __shared__ __attribute__((aligned(16))) __bf16 s_h[kTile * kHS];
   __shared__ float s_e[4][kTile];
   __shared__ int s_atom[kTile];
+  __shared__ __attribute__((aligned(16))) float s_gr[kGR ? kTile * kGRS : 4];
 
   // tile -> (type, first sorted index); every thread of the workgroup
   // computes the same answer, so the early exit is uniform
@@ -143,7 +171,7 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
 
   // ---- layer 1: D block loads (thread -> atom tid/4, features 16*(tid%4) .. +15 of the block, bf16)
-  const int xa = s_atom[tid >> 2], xq = tid & 3;
+  const int xl = tid >> 2, xa = s_atom[xl], xq = tid & 3;
   const __bf16* xrow = desc + (int64_t)(xa < 0 ? 0 : xa) * p.k1;
   const int nkb = p.nk1 / 2;
   auto load_block = [&](int kb, bf16x8 (&r)[2]) {
@@ -156,9 +184,33 @@ __global__ void __launch_bounds__(256)
     }
   };
   auto store_block = [&](int buf, const bf16x8 (&r)[2]) {
-    __bf16* dst = s_x[buf] + (tid >> 2) * kXS + 16 * xq;
+    __bf16* dst = s_x[buf] + xl * kXS + 16 * xq;
     *reinterpret_cast<bf16x8*>(dst) = r[0];
     *reinterpret_cast<bf16x8*>(dst + 8) = r[1];
+  };
+  float4* sgr4 = reinterpret_cast<float4*>(s_gr);
+  // D block kb of atom xl, features (m = 4 kb + ml, a = 4 xq .. 4 xq + 3):
+  // the thread's GR[.][a] (ga) stay in registers through layer 1
+  float4 ga[4];
+  auto gen_block = [&](int kb, bf16x8 (&r)[2]) {
+#pragma unroll
+    for (int ml = 0; ml < 4; ++ml) {
+      const float4 gm = sgr4[xl * (kGRS / 4) + 4 * kb + ml];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        r[ml >> 1][4 * (ml & 1) + j] = (__bf16)(gm.x * ga[j].x + gm.y * ga[j].y + gm.z * ga[j].z + gm.w * ga[j].w);
+    }
+  };
+  auto store_gen = [&](int buf, const bf16x8 (&r)[2]) {
+    // r[ml / 2][4 (ml % 2) + j] -> feature 16 ml + 4 xq + j
+    __bf16* dst = s_x[buf] + xl * kXS + 4 * xq;
+#pragma unroll
+    for (int ml = 0; ml < 4; ++ml) {
+      bf16x4 q;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) q[j] = r[ml >> 1][4 * (ml & 1) + j];
+      *reinterpret_cast<bf16x4*>(dst + 16 * ml) = q;
+    }
   };
 
   f32x4 acc[4][4];
@@ -166,7 +218,49 @@ __global__ void __launch_bounds__(256)
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
     for (int s = 0; s < 4; ++s) acc[mt][s] = f32x4{0.f, 0.f, 0.f, 0.f};
-  {
+  if constexpr (kGR) {
+    // all of the tile's GR at once, kBatch float4 of every type in flight per
+    // thread (row-major over [atom][c][m] so each load instruction reads
+    // contiguous memory); GR[c][m] -> s_gr[atom][m][c]
+    constexpr int kBatch = 8;
+    const int m4n = p.M >> 2, n4 = kTile * p.M;  // float4 per (atom, c) row / per tile
+    const float inv_m = 1.f / (float)p.M, inv_m4 = 1.f / (float)m4n;
+#pragma unroll 1
+    for (int k0 = tid; k0 < n4; k0 += 256 * kBatch) {
+      f32x4 x[kBatch][kMaxTypesGR];
+#pragma unroll
+      for (int b = 0; b < kBatch; ++b) {
+        const int k = k0 + 256 * b;
+        const int la = k < n4 ? __float2int_rz(((float)k + 0.5f) * inv_m) : 0;
+        const int ai = k < n4 ? s_atom[la] : -1;
+#pragma unroll
+        for (int tt = 0; tt < kMaxTypesGR; ++tt)
+          x[b][tt] = (ai >= 0 && tt < p.ntypes)
+                         ? __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(
+                               gr_part + ((int64_t)tt * p.nloc + ai) * 4 * p.M) + (k - la * p.M))
+                         : f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+#pragma unroll
+      for (int b = 0; b < kBatch; ++b) {
+        const int k = k0 + 256 * b;
+        if (k >= n4) break;
+        const int la = __float2int_rz(((float)k + 0.5f) * inv_m), f4 = k - la * p.M;
+        const int c = __float2int_rz(((float)f4 + 0.5f) * inv_m4), m = 4 * (f4 - c * m4n);
+        f32x4 v = x[b][0];
+#pragma unroll
+        for (int tt = 1; tt < kMaxTypesGR; ++tt) v += x[b][tt];
+        float* dst = s_gr + la * kGRS + 4 * m + c;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) dst[4 * j] = v[j] * p.inv_nnei;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ga[j] = sgr4[xl * (kGRS / 4) + 4 * xq + j];
+    bf16x8 r[2];
+    gen_block(0, r);
+    store_gen(0, r);
+  } else {
     bf16x8 r[2];
     load_block(0, r);
     store_block(0, r);
@@ -184,8 +278,13 @@ __global__ void __launch_bounds__(256)
     bf16x8 r[2];
     bf16x8 a_nxt[2][4];
     const bool more = kb + 1 < nkb;
+    // fused form: D block kb + 1 is built from the staged GR
     const int kn = more ? kb + 1 : kb;
-    if (more) load_block(kb + 1, r);
+    if constexpr (kGR) {
+      if (more) gen_block(kb + 1, r);
+    } else {
+      if (more) load_block(kb + 1, r);
+    }
 #pragma unroll
     for (int kc = 0; kc < 2; ++kc)
 #pragma unroll
@@ -200,14 +299,17 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int s = 0; s < 4; ++s) acc[mt][s] = mfma(a_cur[kc][mt], b[s], acc[mt][s]);
     }
-    if (more) store_block((kb + 1) & 1, r);
+    if constexpr (kGR) {
+      if (more) store_gen((kb + 1) & 1, r);
+    } else {
+      if (more) store_block((kb + 1) & 1, r);
+    }
     __syncthreads();
 #pragma unroll
     for (int kc = 0; kc < 2; ++kc)
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt) a_cur[kc][mt] = a_nxt[kc][mt];
   }
-
   // ---- layer-1 epilogue: h (fp32, skip of layer 2), tanh factors f1 (fp32)
   float h[4][4][4], f1[4][4][4], f2[4][4][4], f3[4][4][4];
 #pragma unroll
@@ -318,8 +420,25 @@ __global__ void __launch_bounds__(256)
   int atom_s[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) atom_s[s] = s_atom[16 * s + n];
+  const int atom_g = s_atom[16 * g + n];
   const int mt_per_wave = (p.nm1 + 3) / 4;
   const int mt0 = w * mt_per_wave, mt1 = min(p.nm1, mt0 + mt_per_wave);
+  // fused form: GR[.][a] of the lane's atoms (a = 4 g + v) in registers,
+  // acc2 = this wavefront's part of sum_m dD[m][a] GR[.][m]; s_h is reused
+  // (after the barrier) for the first term of m < 16, s_x for the reduction
+  float4 gra[4][4], acc2[4][4];
+  float4* t1buf = reinterpret_cast<float4*>(s_h);
+  float4* red = reinterpret_cast<float4*>(&s_x[0][0]);
+  if constexpr (kGR) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        gra[s][v] = sgr4[(16 * s + n) * (kGRS / 4) + 4 * g + v];
+        acc2[s][v] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    __syncthreads();  // every wavefront has read dz1 from s_h
+  }
   // the next M tile's eight weight fragments are loaded one tile ahead
   bf16x8 wa[8];
   if (mt0 < mt1)
@@ -340,12 +459,99 @@ __global__ void __launch_bounds__(256)
       for (int s = 0; s < 4; ++s) d[s] = mfma(wa[kc], bz[kc][s], d[s]);
 #pragma unroll
     for (int kc = 0; kc < 8; ++kc) wa[kc] = wn[kc];
-    const int f = 16 * mt + 4 * g;
+    if constexpr (kGR) {
+      // M tile mt is m = mt, its rows a = 4 g + v (axis 16)
+      const int q3 = mt >= 16 ? q3_of(mt, p.n1) : 0;
+      float4 p1[4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
-      if (atom_s[s] >= 0)
-        __builtin_nontemporal_store(d[s], reinterpret_cast<f32x4*>(ddesc + (int64_t)atom_s[s] * p.k1 + f));
+      for (int s = 0; s < 4; ++s) {
+        const float4 gm = sgr4[(16 * s + n) * (kGRS / 4) + mt];
+        p1[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float x = d[s][v];
+          p1[s].x += x * gra[s][v].x;
+          p1[s].y += x * gra[s][v].y;
+          p1[s].z += x * gra[s][v].z;
+          p1[s].w += x * gra[s][v].w;
+          acc2[s][v].x += x * gm.x;
+          acc2[s][v].y += x * gm.y;
+          acc2[s][v].z += x * gm.z;
+          acc2[s][v].w += x * gm.w;
+        }
+        // first term: sum over a = the four lane groups
+        p1[s].x = sum_lane_groups(p1[s].x);
+        p1[s].y = sum_lane_groups(p1[s].y);
+        p1[s].z = sum_lane_groups(p1[s].z);
+        p1[s].w = sum_lane_groups(p1[s].w);
+      }
+      // lane group g holds atom 16 g + n: one 16-byte (four-component) store per lane
+      const float4 x = g == 0 ? p1[0] : g == 1 ? p1[1] : g == 2 ? p1[2] : p1[3];
+      if (atom_g >= 0) {
+        if (mt >= 16)
+          *reinterpret_cast<float4*>(dout + ((int64_t)atom_g * 128 + q3) * 4) =
+              make_float4(x.x * p.inv_nnei, x.y * p.inv_nnei, x.z * p.inv_nnei, x.w * p.inv_nnei);
+        else
+          t1buf[(16 * g + n) * 16 + mt] = x;
+      }
+    } else {
+      const int f = 16 * mt + 4 * g;
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        if (atom_s[s] >= 0)
+          __builtin_nontemporal_store(d[s], reinterpret_cast<f32x4*>(dout + (int64_t)atom_s[s] * p.k1 + f));
+    }
   }
+  if constexpr (kGR) {
+    // second term summed over the four wavefronts in a fixed order
+#pragma unroll 1
+    for (int ph = 0; ph < 4; ++ph) {
+      if (w == ph)
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            float4* r4 = red + (16 * s + n) * 16 + 4 * g + v;
+            float4 x = acc2[s][v];
+            if (ph > 0) {
+              const float4 y = *r4;
+              x.x += y.x;
+              x.y += y.y;
+              x.z += y.z;
+              x.w += y.w;
+            }
+            *r4 = x;
+          }
+      __syncthreads();
+    }
+    // dGR for m < 16 (both terms) and zeros in the padding slots
+    for (int k = tid; k < kTile * 128; k += 256) {
+      const int la = k >> 7, q = k & 127, ai = s_atom[la];
+      if (ai < 0) continue;
+      const int m = orig_of_q3(q, p.n1);
+      if (m >= 16) continue;
+      float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (m >= 0) {
+        const float4 y = red[la * 16 + m], z = t1buf[la * 16 + m];
+        x = make_float4((y.x + z.x) * p.inv_nnei, (y.y + z.y) * p.inv_nnei, (y.z + z.z) * p.inv_nnei,
+                        (y.w + z.w) * p.inv_nnei);
+      }
+      *reinterpret_cast<float4*>(dout + ((int64_t)ai * 128 + q) * 4) = x;
+    }
+  }
+}
+
+// Fragment blocks per type: W1^T (16 x nk1), W2^T, W3^T (16 x 8), W3, W2 (16 x 8), W1 (nm1 x 8).
+void set_offsets(FitParams& p) {
+  p.nk1 = (int)(2 * ceil_div(p.k1, kKB));
+  p.nm1 = p.k1 / 16;
+  p.w1f = 0;
+  p.w2f = p.w1f + (int64_t)16 * p.nk1 * 64;
+  p.w3f = p.w2f + 16 * 8 * 64;
+  p.w3b = p.w3f + 16 * 8 * 64;
+  p.w2b = p.w3b + 16 * 8 * 64;
+  p.w1b = p.w2b + 16 * 8 * 64;
+  p.type_frags = p.w1b + (int64_t)p.nm1 * 8 * 64;
 }
 
 }  // namespace
@@ -364,21 +570,37 @@ extern "C" int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const v
   std::memset(&p, 0, sizeof(p));
   p.ntypes = ntypes;
   p.k1 = k1;
-  p.nk1 = (int)(2 * ceil_div(k1, kKB));
-  p.nm1 = k1 / 16;
-  // fragment blocks per type: W1^T (16 x nk1), W2^T, W3^T (16 x 8), W3, W2 (16 x 8), W1 (nm1 x 8)
-  p.w1f = 0;
-  p.w2f = p.w1f + (int64_t)16 * p.nk1 * 64;
-  p.w3f = p.w2f + 16 * 8 * 64;
-  p.w3b = p.w3f + 16 * 8 * 64;
-  p.w2b = p.w3b + 16 * 8 * 64;
-  p.w1b = p.w2b + 16 * 8 * 64;
-  p.type_frags = p.w1b + (int64_t)p.nm1 * 8 * 64;
+  set_offsets(p);
   // one workgroup per 64-atom tile of each type (upper bound; extra ones exit)
   const int64_t grid = ceil_div(nloc, kTile) + ntypes;
-  hipLaunchKernelGGL(k_fit, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), atom_energy, ddesc,
-                     static_cast<const __bf16*>(desc), perm,
-                     type_start, static_cast<const bf16x8*>(frags), consts, ebias, p);
+  hipLaunchKernelGGL(k_fit<false>, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), atom_energy, ddesc,
+                     static_cast<const __bf16*>(desc), nullptr, perm, type_start, static_cast<const bf16x8*>(frags),
+                     consts, ebias, p);
+  DPMI_LAUNCH_CHECK();
+  return DPMI_OK;
+}
+
+extern "C" int dpmi_se_a_fitting_gr_bf16(double* atom_energy, float* dgr, const float* gr_part, const int* perm,
+                                         const int* type_start, const void* frags, const float* consts,
+                                         const double* ebias, int64_t nloc, int ntypes, int n1, int nnei,
+                                         void* stream) {
+  DPMI_CHECK_ARG(ntypes > 0 && ntypes <= kMaxTypesGR, "fused descriptor form: ntypes out of range [1, 4]");
+  DPMI_CHECK_ARG(n1 >= 4 && 4 * n1 <= kMaxGR, "fused descriptor form needs 16 <= 4 n1 <= 104");
+  DPMI_CHECK_ARG(nnei > 0, "nnei must be positive");
+  DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
+  if (nloc == 0) return DPMI_OK;
+  FitParams p;
+  std::memset(&p, 0, sizeof(p));
+  p.ntypes = ntypes;
+  p.nloc = (int)nloc;
+  p.M = 4 * n1;
+  p.n1 = n1;
+  p.inv_nnei = 1.f / (float)nnei;
+  p.k1 = 16 * p.M;
+  set_offsets(p);
+  const int64_t grid = ceil_div(nloc, kTile) + ntypes;
+  hipLaunchKernelGGL(k_fit<true>, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), atom_energy, dgr, nullptr,
+                     gr_part, perm, type_start, static_cast<const bf16x8*>(frags), consts, ebias, p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }

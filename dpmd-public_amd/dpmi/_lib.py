@@ -47,6 +47,7 @@ SIGNATURES = {
     "dpmi_se_a_embed_fwd_f64": [_P, _P, _P, _P, _P, _L, _P, _I, _I, _P],
     "dpmi_se_a_embed_bwd_f64": [_P, _P, _P, _P, _P, _P, _L, _P, _I, _I, _P],
     "dpmi_se_a_fitting_bf16": [_P, _P, _P, _P, _P, _P, _P, _P, _L, _I, _I, _P],
+    "dpmi_se_a_fitting_gr_bf16": [_P, _P, _P, _P, _P, _P, _P, _P, _L, _I, _I, _I, _P],
     "dpmi_se_a_fitting_frag_count": [_P, _I],
     "dpmi_tabulate_fusion_se_a_f64": [_P, _P, _P, _P, _P, _L, _L, _L, _I, _P],
     "dpmi_tabulate_fusion_se_a_f32": [_P, _P, _P, _P, _P, _L, _L, _L, _I, _P],

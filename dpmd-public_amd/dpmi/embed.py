@@ -146,15 +146,21 @@ class FusedEmbedding:
         self.frags = torch.from_numpy(bits.astype(np.int16)).to(device)
         self.consts = torch.from_numpy(consts).to(device)
 
+    def forward_part(self, em: torch.Tensor) -> torch.Tensor:
+        """em [nloc, nnei, 4] fp32 -> gr_part [ntypes, nloc, 4, M]: each
+        neighbor type's em^T G (original feature order, not yet / nnei)."""
+        nloc = em.shape[0]
+        gr_part = torch.empty((self.ntypes, nloc, 4, self.M), dtype=torch.float32, device=em.device)
+        ops._launch("dpmi_se_a_embed_fwd_bf16", ptr(gr_part), ptr(em.contiguous()), ptr(self.frags), ptr(self.consts),
+                    nloc, ptr(self.sel), self.ntypes, self.n1, ops._stream())
+        return gr_part
+
     def forward(self, em: torch.Tensor, desc_dtype=torch.float32):
         """em [nloc, nnei, 4] fp32 -> (gr [nloc, 4, M], desc [nloc, M*axis] in
         desc_dtype: float32, or bfloat16 for the fused fitting kernel)."""
         nloc = em.shape[0]
         dev = em.device
-        em = em.contiguous()
-        gr_part = torch.empty((self.ntypes, nloc, 4, self.M), dtype=torch.float32, device=dev)
-        ops._launch("dpmi_se_a_embed_fwd_bf16", ptr(gr_part), ptr(em), ptr(self.frags), ptr(self.consts), nloc,
-                    ptr(self.sel), self.ntypes, self.n1, ops._stream())
+        gr_part = self.forward_part(em)
         gr = torch.empty((nloc, 4, self.M), dtype=torch.float32, device=dev)
         desc = torch.empty((nloc, self.M * self.axis), dtype=desc_dtype, device=dev)
         ops._launch("dpmi_se_a_descriptor_fwd", ptr(gr), ptr(desc), ptr(gr_part), nloc, self.ntypes, self.nnei,
@@ -164,12 +170,18 @@ class FusedEmbedding:
     def backward(self, em: torch.Tensor, gr: torch.Tensor, ddesc: torch.Tensor) -> torch.Tensor:
         """dE/d desc -> dE/d em [nloc, nnei, 4] fp32 (gr as returned by forward)."""
         nloc = em.shape[0]
-        dgr = torch.empty((nloc, 4, 128), dtype=torch.float32, device=em.device)
+        dgr = torch.empty((nloc, 128, 4), dtype=torch.float32, device=em.device)
         ops._launch("dpmi_se_a_descriptor_bwd", ptr(dgr), ptr(ddesc.contiguous()), ptr(gr), nloc, self.nnei, self.n1,
                     self.axis, ops._stream())
+        return self.backward_dgr(em, dgr)
+
+    def backward_dgr(self, em: torch.Tensor, dgr: torch.Tensor) -> torch.Tensor:
+        """dgr [nloc, 128, 4] = dE/d(sum_t gr_part) in the internal order ->
+        dE/d em [nloc, nnei, 4] fp32 (recomputing the embedding forward)."""
+        nloc = em.shape[0]
         nd = torch.empty((nloc, self.nnei, 4), dtype=torch.float32, device=em.device)
-        ops._launch("dpmi_se_a_embed_bwd_bf16", ptr(nd), ptr(em.contiguous()), ptr(dgr), ptr(self.frags),
-                    ptr(self.consts), nloc, ptr(self.sel), self.ntypes, self.n1, ops._stream())
+        ops._launch("dpmi_se_a_embed_bwd_bf16", ptr(nd), ptr(em.contiguous()), ptr(dgr.contiguous()),
+                    ptr(self.frags), ptr(self.consts), nloc, ptr(self.sel), self.ntypes, self.n1, ops._stream())
         return nd
 
 

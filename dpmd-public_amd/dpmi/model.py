@@ -179,6 +179,8 @@ class SeAEnergyModel:
         self.fused_fit = None
         if self.fused is not None and fitting.supported(fit):
             self.fused_fit = fitting.FusedFitting(fit, bias_atom_e, ob, self.device)
+        # the descriptor fused into the fitting kernel (axis 16, M <= 104)
+        self.fused_gr = self.fused_fit is not None and self.fused_fit.supports_gr(self.fused.n1, self.axis)
 
     def named_parameters(self):
         """[((kind, net, layer, name), tensor)] of the trainable tensors, made
@@ -316,15 +318,7 @@ class SeAEnergyModel:
         nlist, _ovf, _ = ops.format_nlist(cand, ext_c, ext_t, nloc, self.rcut, self.sel, numneigh=numneigh)
         em = self.env_mat(nlist, ext_c, ext_t)
         if self.fused is not None:
-            gr, desc = self.fused.forward(em, torch.bfloat16 if self.fused_fit is not None else torch.float32)
-            if self.fused_fit is not None:
-                ae, ddesc = self.fused_fit(desc, atype)
-            else:
-                desc.requires_grad_(True)
-                with torch.enable_grad():
-                    ae = self.atom_energy(desc, atype)
-                    (ddesc,) = torch.autograd.grad(ae.sum(), desc)
-            nd = self.fused.backward(em, gr, ddesc)
+            nd, ae = self._fused_chain(em, atype)
         elif self.fused64 is not None:
             g = self.fused64.forward(em).requires_grad_(True)
             with torch.enable_grad():
@@ -368,6 +362,27 @@ class SeAEnergyModel:
             out["atom_energy"] = ae
         return out
 
+    def _fused_chain(self, em, atype, use_gr=None):
+        """bf16 embedding -> descriptor -> fitting -> adjoints: (dE/d em, atom
+        energies).  With the descriptor fused into the fitting kernel
+        (fused_gr) the G^T R partials go straight to it and dE/dGR comes back;
+        otherwise the descriptor kernels run between (and torch runs the
+        fitting net when its shape has no fused kernel)."""
+        use_gr = self.fused_gr if use_gr is None else use_gr
+        if use_gr:
+            gr_part = self.fused.forward_part(em)
+            ae, dgr = self.fused_fit.from_gr(gr_part, atype, self.fused.n1, self.fused.nnei)
+            return self.fused.backward_dgr(em, dgr), ae
+        gr, desc = self.fused.forward(em, torch.bfloat16 if self.fused_fit is not None else torch.float32)
+        if self.fused_fit is not None:
+            ae, ddesc = self.fused_fit(desc, atype)
+        else:
+            desc.requires_grad_(True)
+            with torch.enable_grad():
+                ae = self.atom_energy(desc, atype)
+                (ddesc,) = torch.autograd.grad(ae.sum(), desc)
+        return self.fused.backward(em, gr, ddesc), ae
+
     def _compute_fused(self, atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy):
         """bf16 form with hand-written adjoints end to end: fused embedding +
         G^T R forward, fused fitting forward + adjoint (or hipBLASLt GEMMs with
@@ -375,17 +390,8 @@ class SeAEnergyModel:
         descriptor and embedding adjoints recomputing the forward, env-mat
         adjoint for forces and virial."""
         em = self.env_mat(nlist, ext_c, ext_t)
-        gr, desc = self.fused.forward(em, torch.bfloat16 if self.fused_fit is not None else torch.float32)
-        if self.fused_fit is not None:
-            ae, ddesc = self.fused_fit(desc, atype)
-            energy = ae.sum()
-        else:
-            desc.requires_grad_(True)
-            with torch.enable_grad():
-                ae = self.atom_energy(desc, atype)
-                energy = ae.sum()
-                (ddesc,) = torch.autograd.grad(energy, desc)
-        nd = self.fused.backward(em, gr, ddesc)
+        nd, ae = self._fused_chain(em, atype)
+        energy = ae.sum()
         force, virial = self.forces(nd, nlist, cells, mapping, ext_c, ext_t, box)
         out = {"energy": energy.detach(), "force": force, "virial": virial}
         if need_atom_energy:

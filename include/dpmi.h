@@ -260,8 +260,9 @@ int dpmi_se_a_embed_fwd_bf16(float* gr_part, const float* em, const void* frags,
  * GR^T GR[:, :axis] (se_a.py:851-858), fp32 or (desc_bf16 != 0) bf16. */
 int dpmi_se_a_descriptor_fwd(float* gr, void* desc, const float* gr_part, int64_t nloc, int ntypes, int nnei,
                              int n1, int axis, int desc_bf16, void* stream);
-/* dgr [nloc, 4, 128] = dE/d(sum_t gr_part) from ddesc = dE/d desc, in the
- * embedding kernels' internal feature order (gr as written above). */
+/* dgr [nloc, 128, 4] = dE/d(sum_t gr_part) from ddesc = dE/d desc, in the
+ * embedding kernels' internal feature order, the four components of a
+ * feature contiguous (gr as written above). */
 int dpmi_se_a_descriptor_bwd(float* dgr, const float* ddesc, const float* gr, int64_t nloc, int nnei, int n1,
                              int axis, void* stream);
 /* net_deriv [nloc, nnei, 4] = dE/d em, recomputing the embedding forward
@@ -299,6 +300,18 @@ int dpmi_se_a_embed_bwd_f64(double* net_deriv, const double* em, const double* d
 int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const void* desc_bf16, const int* perm,
                            const int* type_start, const void* frags, const float* consts, const double* ebias,
                            int64_t nloc, int ntypes, int k1, void* stream);
+/* The same fitting net with the descriptor fused in (axis 16, M = 4 n1 with
+ * 16 <= M <= 104): gr_part [ntypes, nloc, 4, M] as dpmi_se_a_embed_fwd_bf16
+ * writes it; each 64-atom workgroup forms GR = sum_t gr_part / nnei and
+ * D = GR^T GR[:, :16] (se_a.py:851-858) in LDS and, instead of dE/dD, writes
+ * dgr [nloc, 128, 4] = dE/d(sum_t gr_part) in the embedding kernels' internal
+ * feature order (what dpmi_se_a_descriptor_bwd produces, the input of
+ * dpmi_se_a_embed_bwd_bf16).  Replaces the descriptor kernels and
+ * dpmi_se_a_fitting_bf16 on that path; frags / consts / ebias as above with
+ * k1 = 16 M. */
+int dpmi_se_a_fitting_gr_bf16(double* atom_energy, float* dgr, const float* gr_part, const int* perm,
+                              const int* type_start, const void* frags, const float* consts, const double* ebias,
+                              int64_t nloc, int ntypes, int n1, int nnei, void* stream);
 /* Number of bf16x8 lanes (64 per fragment) of one type's packed fitting net. */
 int dpmi_se_a_fitting_frag_count(int64_t* count, int k1);
 

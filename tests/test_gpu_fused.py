@@ -104,6 +104,52 @@ def test_fused_fitting(setup, nloc):
     assert d_err <= 1e-2
 
 
+@pytest.mark.parametrize("nloc", [192, 1000])
+def test_fused_descriptor_fitting(setup, nloc):
+    """Descriptor fused into the fitting kernel (dpmi_se_a_fitting_gr_bf16)
+    against the two-kernel chain on the same G^T R partials (descriptor
+    forward -> fitting on the bf16 descriptor -> descriptor backward): the
+    same D and fitting arithmetic, dE/dGR summed in another order; padding
+    slots of dgr exactly zero.  nloc = 192: partials of the water box;
+    1000: random partials and types (partial 64-atom tiles in both types)."""
+    from dpmi import embed, ops
+    from dpmi._lib import ptr
+
+    spec, params, m64, mbf, s, c, t = setup
+    assert mbf.fused_gr
+    fe, ff = mbf.fused, mbf.fused_fit
+    _say(f"fused descriptor + fitting nloc={nloc}")
+    if nloc == s.nloc:
+        ext_c, ext_t, mapping, nlist, _, _, _ = m64.neighbor_list(c, t.to(torch.int32), s.box)
+        gr_part = fe.forward_part(m64.env_mat(nlist, ext_c, ext_t).float())
+        atype = t.to(torch.int32)
+    else:
+        g = torch.Generator(device="cpu").manual_seed(nloc)
+        gr_part = (torch.randn(fe.ntypes, nloc, 4, fe.M, generator=g) * 15.0).to(DEV)
+        atype = (torch.rand(nloc, generator=g) < 0.35).to(torch.int32).to(DEV)
+    gr = torch.empty((nloc, 4, fe.M), dtype=torch.float32, device=DEV)
+    desc = torch.empty((nloc, fe.M * 16), dtype=torch.bfloat16, device=DEV)
+    ops._launch("dpmi_se_a_descriptor_fwd", ptr(gr), ptr(desc), ptr(gr_part), nloc, fe.ntypes, fe.nnei, fe.n1, 16, 1,
+                ops._stream())
+    ae_ref, ddesc = ff(desc, atype)
+    dgr_ref = torch.empty((nloc, 128, 4), dtype=torch.float32, device=DEV)
+    ops._launch("dpmi_se_a_descriptor_bwd", ptr(dgr_ref), ptr(ddesc), ptr(gr), nloc, fe.nnei, fe.n1, 16,
+                ops._stream())
+    ae, dgr_out = ff.from_gr(gr_part, atype, fe.n1, fe.nnei)
+    torch.cuda.synchronize()
+    dgr = dgr_out.double().cpu()
+    ref = dgr_ref.double().cpu()
+    q1, q2, q3 = embed.internal_maps(fe.n1)
+    pad = np.setdiff1d(np.arange(128), q3)
+    assert torch.isfinite(dgr).all() and torch.isfinite(ae).all()
+    assert (dgr[:, torch.as_tensor(pad), :] == 0).all()
+    e_err = (ae - ae_ref).abs().max().item()
+    d_err = ((dgr - ref).abs().max() / ref.abs().max()).item()
+    print(f"fused descriptor: max |de| {e_err:.3e} eV, dgr max rel {d_err:.2e}", flush=True)
+    assert e_err <= 1e-6 * max(1.0, ae_ref.abs().max().item())
+    assert d_err <= 1e-4
+
+
 @pytest.mark.parametrize("axis", [4, 8, 12, 16])
 def test_descriptor_backward_widths(axis):
     """dpmi_se_a_descriptor_bwd (lane-parallel kernels for axis 4/8/16, the
@@ -127,10 +173,10 @@ def test_descriptor_backward_widths(axis):
     ref[:, :, torch.as_tensor(q3)] = ref_o / nnei
     gr_d = gr.float().to(DEV)
     dd_d = dd.float().to(DEV)
-    out = torch.full((nloc, 4, 128), float("nan"), dtype=torch.float32, device=DEV)
+    out = torch.full((nloc, 128, 4), float("nan"), dtype=torch.float32, device=DEV)
     ops._launch("dpmi_se_a_descriptor_bwd", ptr(out), ptr(dd_d), ptr(gr_d), nloc, nnei, n1, axis, ops._stream())
     torch.cuda.synchronize()
-    out = out.double().cpu()
+    out = out.double().cpu().transpose(1, 2)  # [nloc, 4, 128] like ref
     assert torch.isfinite(out).all()
     assert torch.allclose(out, ref, rtol=1e-4, atol=1e-5 * ref.abs().max().item())
 
