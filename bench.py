@@ -55,7 +55,8 @@ def _bytes_per_atom(name: str, nnei: int) -> float:
 HBM_PEAK_GBS = 8000.0
 # entry point -> kernel it launches (for the measured-traffic table)
 KERNEL_OF = {"dpmi_se_a_embed_bwd_bf16": "k_embed_bwd<skip>", "dpmi_se_a_embed_fwd_bf16": "k_embed_fwd<skip>",
-             "dpmi_se_a_fitting_bf16": "k_fit", "dpmi_force_virial_cells_f32": "k_force_cells<float, 1536, false>"}
+             "dpmi_se_a_fitting_bf16": "k_fit<false>", "dpmi_se_a_fitting_gr_bf16": "k_fit<true>",
+             "dpmi_force_virial_cells_f32": "k_force_cells<float, 1536, false>"}
 
 
 def _issue_bound(workload: str, entry: str):
@@ -187,7 +188,8 @@ def main() -> int:
                "dpmi_force_virial_cells_f64", "dpmi_se_a_embed_fwd_f64", "dpmi_se_a_embed_bwd_f64"]
     names32 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f32c64",
                "dpmi_force_virial_cells_f32", "dpmi_se_a_embed_fwd_bf16", "dpmi_se_a_descriptor_fwd",
-               "dpmi_se_a_descriptor_bwd", "dpmi_se_a_embed_bwd_bf16", "dpmi_se_a_fitting_bf16"]
+               "dpmi_se_a_descriptor_bwd", "dpmi_se_a_embed_bwd_bf16", "dpmi_se_a_fitting_bf16",
+               "dpmi_se_a_fitting_gr_bf16"]
     t64, out64, k64 = None, None, {}
     if not args.no_fp64:
         m64 = SeAEnergyModel.from_params(spec, params, device=dev, precision="fp64")

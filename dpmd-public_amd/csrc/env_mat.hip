@@ -10,6 +10,8 @@
 // compensated (Neumaier) sum per component.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cstring>
 
 #include "cells.h"
@@ -467,7 +469,7 @@ __global__ void __launch_bounds__(256)
 // second pass (bigger tables) are left to a second force pass (kBigPass, fp32:
 // 3072 x 24 B of LDS) or, in FP64 (4 x 1536 x 24 B already), processed with
 // neighbor atomics.
-template <typename FP, int CAP, bool kBigPass>
+template <typename FP, int CAP, bool kBigPass, int kChunks>
 __global__ void __launch_bounds__(256)
     k_force_cells(double* __restrict__ force, FP* __restrict__ partial, int* __restrict__ cell_ok,
                   double* __restrict__ vpartial, const FP* __restrict__ nd, const int* __restrict__ nlist,
@@ -521,20 +523,16 @@ __global__ void __launch_bounds__(256)
     for (int ai = w; ai < na; ai += kAtomsPerBlock) {
       const int i = cell_atoms[a0 + ai];
       const int* nl = nlist + (int64_t)i * nnei;
-      section_limits_wave(nl, sp, lim_all[w]);
       const int ti = ext_t[i] < sp.ntypes ? ext_t[i] : -1;
       const double xi0 = ext_c[3 * i + 0], xi1 = ext_c[3 * i + 1], xi2 = ext_c[3 * i + 2];
       double ci[3] = {0., 0., 0.};
-      for (int s = lane; s < nnei && ti >= 0; s += kWave) {
-        const int t = slot_type(sp, s);
-        if (s >= lim_all[w][t]) continue;
-        const int j = nl[s];
-        const double r64[3] = {ext_c[3 * j + 0] - xi0, ext_c[3 * j + 1] - xi1, ext_c[3 * j + 2] - xi2};
+      // neighbor term of slot s (neighbor j, its coordinates, dE/d em row,
+      // std row and table slot already loaded)
+      auto term = [&](int s, int j, const double xj[3], const FP n[4], const FP sd[4], int cs) {
+        const double r64[3] = {xj[0] - xi0, xj[1] - xi1, xj[2] - xi2};
         const FP rr[3] = {(FP)r64[0], (FP)r64[1], (FP)r64[2]};
         FP val[4], d[12];
         env_one(rr, rmin, rmax, val, d);
-        const FP* sd = std_ + ((int64_t)ti * nnei + s) * 4;
-        const FP* n = nd + ((int64_t)i * nnei + s) * 4;
         FP g[3] = {(FP)0., (FP)0., (FP)0.};
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -549,7 +547,6 @@ __global__ void __launch_bounds__(256)
         for (int d0 = 0; d0 < 3; ++d0)
 #pragma unroll
           for (int d1 = 0; d1 < 3; ++d1) v[d0 * 3 + d1] += (double)g[d0] * r64[d1];
-        const int cs = ok ? (int)cand_slot[(int64_t)i * nnei + s] : -1;
         if (cs >= 0 && cs < T) {
           if constexpr (kFixed) {
 #pragma unroll
@@ -565,6 +562,65 @@ __global__ void __launch_bounds__(256)
           atomicAdd(&force[3 * (int64_t)o + 0], (double)g[0]);
           atomicAdd(&force[3 * (int64_t)o + 1], (double)g[1]);
           atomicAdd(&force[3 * (int64_t)o + 2], (double)g[2]);
+        }
+      };
+      if (nnei <= kWave * kChunks) {
+        // the atom's whole row in registers: every load of every chunk is
+        // issued before any arithmetic, so the wave waits for two dependent
+        // round trips (list, then coordinates) instead of two per chunk
+        int jv[kChunks];
+#pragma unroll
+        for (int k = 0; k < kChunks; ++k) jv[k] = lane + kWave * k < nnei ? nl[lane + kWave * k] : 0;
+        FP nv[kChunks][4], sv[kChunks][4];
+        int csv[kChunks];
+#pragma unroll
+        for (int k = 0; k < kChunks; ++k) {
+          const int s = lane + kWave * k;
+          const bool in = s < nnei && ti >= 0;
+          const FP* n = nd + ((int64_t)i * nnei + (in ? s : 0)) * 4;
+          const FP* sd = std_ + ((int64_t)(ti < 0 ? 0 : ti) * nnei + (in ? s : 0)) * 4;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            nv[k][c] = in ? n[c] : (FP)0.;
+            sv[k][c] = in ? sd[c] : (FP)1.;
+          }
+          csv[k] = in && ok ? (int)cand_slot[(int64_t)i * nnei + s] : -1;
+        }
+        // section limits (first -1 of each type section) from the registers
+        for (int t = lane; t < sp.ntypes; t += kWave) lim_all[w][t] = sp.sec[t + 1];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int k = 0; k < kChunks; ++k) {
+          const int s = lane + kWave * k;
+          if (s < nnei && jv[k] < 0) atomicMin(&lim_all[w][slot_type(sp, s)], s);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        bool act[kChunks];
+        double xj[kChunks][3];
+#pragma unroll
+        for (int k = 0; k < kChunks; ++k) {
+          const int s = lane + kWave * k;
+          act[k] = ti >= 0 && s < nnei && s < lim_all[w][slot_type(sp, s)];
+          const int j = act[k] ? jv[k] : i;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) xj[k][c] = ext_c[3 * j + c];
+        }
+#pragma unroll
+        for (int k = 0; k < kChunks; ++k)
+          if (act[k]) term(lane + kWave * k, jv[k], xj[k], nv[k], sv[k], csv[k]);
+      } else {
+        section_limits_wave(nl, sp, lim_all[w]);
+        for (int s = lane; s < nnei && ti >= 0; s += kWave) {
+          const int t = slot_type(sp, s);
+          if (s >= lim_all[w][t]) continue;
+          const int j = nl[s];
+          const double xj[3] = {ext_c[3 * j + 0], ext_c[3 * j + 1], ext_c[3 * j + 2]};
+          const FP* sd = std_ + ((int64_t)ti * nnei + s) * 4;
+          const FP* n = nd + ((int64_t)i * nnei + s) * 4;
+          const FP nn[4] = {n[0], n[1], n[2], n[3]}, ss[4] = {sd[0], sd[1], sd[2], sd[3]};
+          term(s, j, xj, nn, ss, ok ? (int)cand_slot[(int64_t)i * nnei + s] : -1);
         }
       }
 #pragma unroll
@@ -895,21 +951,28 @@ int force_cells_impl(double* force, double* virial, const FP* net_deriv, const i
   const size_t per_slot = sizeof(FP) == 4 ? 3 * sizeof(unsigned long long) : kAtomsPerBlock * 3 * sizeof(FP);
   const size_t shm = (size_t)kCandCap * per_slot;
   const size_t shm_big = (size_t)kCandCapBig * per_slot;
-  if (shm > 65536)
-    DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_force_cells<FP, kCandCap, false>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
   if (nloc > 0) {
-    hipLaunchKernelGGL((k_force_cells<FP, kCandCap, false>), dim3(bpf), dim3(256), shm, st, force, partial, cell_ok,
-                       vpartial, net_deriv, nlist, cand_slot, mapping, ext_coord, ext_type, std_,
-                       static_cast<const char*>(nlist_workspace), rcut_smth, rcut, sp);
-    if constexpr (sizeof(FP) == 4) {
-      // the search's big cells (second table size); its virial partials follow the first pass's
-      DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_force_cells<FP, kCandCapBig, true>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_big));
-      hipLaunchKernelGGL((k_force_cells<FP, kCandCapBig, true>), dim3(kBigBlocks), dim3(256), shm_big, st, force,
-                         partial, cell_ok, vpartial + 9 * bpf, net_deriv, nlist, cand_slot, mapping, ext_coord,
-                         ext_type, std_, static_cast<const char*>(nlist_workspace), rcut_smth, rcut, sp);
-    }
+    // a row of up to 3 (4) chunks of 64 neighbors is loaded whole per atom
+    auto launch = [&](auto chunks) -> int {
+      constexpr int kCh = decltype(chunks)::value;
+      if (shm > 65536)
+        DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_force_cells<FP, kCandCap, false, kCh>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+      hipLaunchKernelGGL((k_force_cells<FP, kCandCap, false, kCh>), dim3(bpf), dim3(256), shm, st, force, partial,
+                         cell_ok, vpartial, net_deriv, nlist, cand_slot, mapping, ext_coord, ext_type, std_,
+                         static_cast<const char*>(nlist_workspace), rcut_smth, rcut, sp);
+      if constexpr (sizeof(FP) == 4) {
+        // the search's big cells (second table size); its virial partials follow the first pass's
+        DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_force_cells<FP, kCandCapBig, true, kCh>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_big));
+        hipLaunchKernelGGL((k_force_cells<FP, kCandCapBig, true, kCh>), dim3(kBigBlocks), dim3(256), shm_big, st,
+                           force, partial, cell_ok, vpartial + 9 * bpf, net_deriv, nlist, cand_slot, mapping,
+                           ext_coord, ext_type, std_, static_cast<const char*>(nlist_workspace), rcut_smth, rcut, sp);
+      }
+      return DPMI_OK;
+    };
+    rc = sp.nnei <= 3 * kWave ? launch(std::integral_constant<int, 3>{}) : launch(std::integral_constant<int, 4>{});
+    if (rc) return rc;
     hipLaunchKernelGGL((k_force_gather<FP>), dim3(ceil_div(nloc, 256)), dim3(256), 0, st, force, partial, cell_ok,
                        static_cast<const char*>(nlist_workspace));
   } else {
