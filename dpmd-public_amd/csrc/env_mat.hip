@@ -99,6 +99,48 @@ __device__ inline void env_one(const FP rr[3], float rmin, float rmax, FP v[4], 
   v[3] *= sw;
 }
 
+// The same in fp32 with hardware reciprocals instead of IEEE divisions (the
+// recompute adjoint of the fp32 forms: a few ulp, far below their bf16 / fp32
+// operand rounding): 1/r from v_rsq_f32, 1/(rmax - rmin) precomputed.
+__device__ inline void env_one_fast(const float rr[3], float rmin, float inv_range, float v[4], float d[12]) {
+  const float nr2 = rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
+  const float inr = __builtin_amdgcn_rsqf(nr2);
+  const float nr = nr2 * inr;
+  const float inr2 = inr * inr;
+  const float inr4 = inr2 * inr2;
+  const float inr3 = inr4 * nr;
+  float sw, dsw;
+  const float uu = (nr - rmin) * inv_range;
+  if (uu <= 0.f) {
+    sw = 1.f;
+    dsw = 0.f;
+  } else if (uu < 1.f) {
+    sw = uu * uu * uu * (-6.f * uu * uu + 15.f * uu - 10.f) + 1.f;
+    dsw = (3.f * uu * uu * (-6.f * uu * uu + 15.f * uu - 10.f) + uu * uu * uu * (-12.f * uu + 15.f)) * inv_range;
+  } else {
+    sw = 0.f;
+    dsw = 0.f;
+  }
+  v[0] = inr;
+  v[1] = rr[0] * inr2;
+  v[2] = rr[1] * inr2;
+  v[3] = rr[2] * inr2;
+#pragma unroll
+  for (int dd = 0; dd < 3; ++dd) d[dd] = rr[dd] * inr3 * sw - v[0] * dsw * rr[dd] * inr;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int dd = 0; dd < 3; ++dd) {
+      float t = 2.f * rr[c] * rr[dd] * inr4;
+      if (c == dd) t = t - inr2;
+      d[3 + 3 * c + dd] = t * sw - v[1 + c] * dsw * rr[dd] * inr;
+    }
+  v[0] *= sw;
+  v[1] *= sw;
+  v[2] *= sw;
+  v[3] *= sw;
+}
+
 // First -1 in each type section of the wave's row: slots at or after it are
 // empty (env_mat_a_cpu breaks at the first negative entry of a section).
 // lim must hold kMaxTypes ints for this wave.
@@ -498,6 +540,7 @@ __global__ void __launch_bounds__(256)
   const int side = 2 * p.nsearch + 1, nnb = side * side * side;
   const int ncells = kBigPass ? misc[3] : p.ncell[0] * p.ncell[1] * p.ncell[2];
   const int nnei = sp.nnei;
+  const float inv_range = 1.f / (rmax - rmin);
   double v[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) v[k] = 0.;
@@ -532,11 +575,18 @@ __global__ void __launch_bounds__(256)
         const double r64[3] = {xj[0] - xi0, xj[1] - xi1, xj[2] - xi2};
         const FP rr[3] = {(FP)r64[0], (FP)r64[1], (FP)r64[2]};
         FP val[4], d[12];
-        env_one(rr, rmin, rmax, val, d);
+        if constexpr (kFixed)
+          env_one_fast(rr, rmin, inv_range, val, d);
+        else
+          env_one(rr, rmin, rmax, val, d);
         FP g[3] = {(FP)0., (FP)0., (FP)0.};
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-          const FP nc = n[c] / sd[c];
+          FP nc;
+          if constexpr (kFixed)
+            nc = n[c] * __builtin_amdgcn_rcpf(sd[c]);
+          else
+            nc = n[c] / sd[c];
           g[0] += nc * d[3 * c + 0];
           g[1] += nc * d[3 * c + 1];
           g[2] += nc * d[3 * c + 2];
