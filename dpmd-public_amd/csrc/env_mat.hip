@@ -179,25 +179,21 @@ __global__ void __launch_bounds__(256)
   const int nnei = sp.nnei;
   const int ii = active ? i : 0;
   const int* nl = nlist + (int64_t)ii * nnei;
-  section_limits(nl, sp, lim_all[w]);
-  if (!active) return;
-  int ti = ext_t[i];
+  int ti = ext_t[ii];
   if (ti >= sp.ntypes) ti = -1;  // no statistics for such a type: treated as virtual
-  const CT xi0 = ext_c[3 * i + 0], xi1 = ext_c[3 * i + 1], xi2 = ext_c[3 * i + 2];
+  const CT xi0 = ext_c[3 * ii + 0], xi1 = ext_c[3 * ii + 1], xi2 = ext_c[3 * ii + 2];
   const int nem = nnei * 4;
-  for (int s = lane; s < nnei; s += kWave) {
-    int t = slot_type(sp, s);
-    int j = nl[s];
-    bool valid = s < lim_all[w][t];
+  // one neighbor slot: coordinates loaded, limits known
+  auto slot = [&](int s, int j, bool valid, const CT xj[3]) {
     FP rr[3] = {(FP)0., (FP)0., (FP)0.};
     FP v[4] = {(FP)0., (FP)0., (FP)0., (FP)0.};
     FP d[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) d[k] = (FP)0.;
     if (valid) {
-      rr[0] = (FP)(ext_c[3 * j + 0] - xi0);
-      rr[1] = (FP)(ext_c[3 * j + 1] - xi1);
-      rr[2] = (FP)(ext_c[3 * j + 2] - xi2);
+      rr[0] = (FP)(xj[0] - xi0);
+      rr[1] = (FP)(xj[1] - xi1);
+      rr[2] = (FP)(xj[2] - xi2);
       env_one(rr, rmin, rmax, v, d);
     }
     int64_t o = (int64_t)i * nem + 4 * s;
@@ -223,6 +219,44 @@ __global__ void __launch_bounds__(256)
       rij[r + 0] = rr[0];
       rij[r + 1] = rr[1];
       rij[r + 2] = rr[2];
+    }
+  };
+  if (nnei <= kWave * 4) {
+    // the row in registers, every coordinate load issued before any arithmetic
+    int jv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) jv[k] = lane + kWave * k < nnei ? nl[lane + kWave * k] : 0;
+    for (int t = lane; t < sp.ntypes; t += kWave) lim_all[w][t] = sp.sec[t + 1];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int s = lane + kWave * k;
+      if (s < nnei && jv[k] < 0) atomicMin(&lim_all[w][slot_type(sp, s)], s);
+    }
+    __syncthreads();
+    if (!active) return;
+    bool valid[4];
+    CT xj[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int s = lane + kWave * k;
+      valid[k] = s < nnei && s < lim_all[w][slot_type(sp, s)];
+      const int j = valid[k] ? jv[k] : i;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xj[k][c] = ext_c[3 * j + c];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (lane + kWave * k < nnei) slot(lane + kWave * k, jv[k], valid[k], xj[k]);
+  } else {
+    section_limits(nl, sp, lim_all[w]);
+    if (!active) return;
+    for (int s = lane; s < nnei; s += kWave) {
+      const int j = nl[s];
+      const bool valid = s < lim_all[w][slot_type(sp, s)];
+      const CT xj[3] = {valid ? ext_c[3 * j + 0] : xi0, valid ? ext_c[3 * j + 1] : xi1,
+                        valid ? ext_c[3 * j + 2] : xi2};
+      slot(s, j, valid, xj);
     }
   }
 }

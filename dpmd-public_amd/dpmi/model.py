@@ -181,6 +181,7 @@ class SeAEnergyModel:
             self.fused_fit = fitting.FusedFitting(fit, bias_atom_e, ob, self.device)
         # the descriptor fused into the fitting kernel (axis 16, M <= 104)
         self.fused_gr = self.fused_fit is not None and self.fused_fit.supports_gr(self.fused.n1, self.axis)
+        self._order_cache = None
 
     def named_parameters(self):
         """[((kind, net, layer, name), tensor)] of the trainable tensors, made
@@ -275,10 +276,12 @@ class SeAEnergyModel:
         """One E+F+V evaluation of one frame.  coord [nloc, 3] float64 on the
         GPU, atype [nloc] int; box: 9 values or None.  The neighbor list is
         rebuilt every call."""
+        atype_in = atype
         atype = atype.to(torch.int32)
         ext_c, ext_t, mapping, nlist, _ovf, _, cells = self.neighbor_list(coord, atype, box)
         if self.fused is not None:
-            return self._compute_fused(atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy)
+            return self._compute_fused(atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy,
+                                       order=self._type_order(atype_in, atype))
         if self.fused64 is not None:
             return self._compute_fused64(atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy)
         em = self.env_mat(nlist, ext_c, ext_t)
@@ -362,7 +365,21 @@ class SeAEnergyModel:
             out["atom_energy"] = ae
         return out
 
-    def _fused_chain(self, em, atype, use_gr=None):
+    def _type_order(self, atype_in, atype):
+        """(perm, type_start) of the fused fitting kernel, reused while the
+        caller passes the same, unmodified atype tensor (the cache holds the
+        tensor itself and its version counter, so neither a new tensor nor an
+        in-place change can reuse a stale order)."""
+        if self.fused_fit is None:
+            return None
+        c = self._order_cache
+        if c is not None and c[0] is atype_in and c[1] == atype_in._version:
+            return c[2]
+        order = self.fused_fit.sort_by_type(atype)
+        self._order_cache = (atype_in, atype_in._version, order)
+        return order
+
+    def _fused_chain(self, em, atype, use_gr=None, order=None):
         """bf16 embedding -> descriptor -> fitting -> adjoints: (dE/d em, atom
         energies).  With the descriptor fused into the fitting kernel
         (fused_gr) the G^T R partials go straight to it and dE/dGR comes back;
@@ -371,11 +388,11 @@ class SeAEnergyModel:
         use_gr = self.fused_gr if use_gr is None else use_gr
         if use_gr:
             gr_part = self.fused.forward_part(em)
-            ae, dgr = self.fused_fit.from_gr(gr_part, atype, self.fused.n1, self.fused.nnei)
+            ae, dgr = self.fused_fit.from_gr(gr_part, atype, self.fused.n1, self.fused.nnei, order=order)
             return self.fused.backward_dgr(em, dgr), ae
         gr, desc = self.fused.forward(em, torch.bfloat16 if self.fused_fit is not None else torch.float32)
         if self.fused_fit is not None:
-            ae, ddesc = self.fused_fit(desc, atype)
+            ae, ddesc = self.fused_fit(desc, atype, order=order)
         else:
             desc.requires_grad_(True)
             with torch.enable_grad():
@@ -383,14 +400,14 @@ class SeAEnergyModel:
                 (ddesc,) = torch.autograd.grad(ae.sum(), desc)
         return self.fused.backward(em, gr, ddesc), ae
 
-    def _compute_fused(self, atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy):
+    def _compute_fused(self, atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy, order=None):
         """bf16 form with hand-written adjoints end to end: fused embedding +
         G^T R forward, fused fitting forward + adjoint (or hipBLASLt GEMMs with
         autograd through the fitting net when its shape is not supported),
         descriptor and embedding adjoints recomputing the forward, env-mat
         adjoint for forces and virial."""
         em = self.env_mat(nlist, ext_c, ext_t)
-        nd, ae = self._fused_chain(em, atype)
+        nd, ae = self._fused_chain(em, atype, order=order)
         energy = ae.sum()
         force, virial = self.forces(nd, nlist, cells, mapping, ext_c, ext_t, box)
         out = {"energy": energy.detach(), "force": force, "virial": virial}

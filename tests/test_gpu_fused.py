@@ -104,6 +104,24 @@ def test_fused_fitting(setup, nloc):
     assert d_err <= 1e-2
 
 
+def test_type_order_cache_follows_atype(setup):
+    """The model reuses the type ordering of the fitting kernel while the
+    same atype tensor comes back unchanged: an in-place change of the types
+    must give the energy a fresh model computes for them."""
+    from dpmi.model import SeAEnergyModel
+
+    spec, params, m64, mbf, s, c, t = setup
+    _say("type-order cache water192")
+    e0 = mbf.compute(c, t, s.box)["energy"].item()
+    t2 = t.clone()
+    assert mbf.compute(c, t2, s.box)["energy"].item() == e0
+    t2[:8] = 1 - t2[:8]
+    e1 = mbf.compute(c, t2, s.box)["energy"].item()
+    fresh = SeAEnergyModel.from_params(spec, params, device=DEV, precision="bf16")
+    e_ref = fresh.compute(c, t2, s.box)["energy"].item()
+    assert e1 == e_ref and e1 != e0
+
+
 @pytest.mark.parametrize("nloc", [192, 1000])
 def test_fused_descriptor_fitting(setup, nloc):
     """Descriptor fused into the fitting kernel (dpmi_se_a_fitting_gr_bf16)
