@@ -24,6 +24,8 @@
 // multiply-add is fused; divisions and sqrt are correctly rounded on gfx950.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -537,6 +539,12 @@ __global__ void __launch_bounds__(256) k_search_cells(
   const int ns = p.nsearch, side = 2 * ns + 1, nnb = side * side * side;
   const int ncells = kBigPass ? misc[3] : p.ncell[0] * p.ncell[1] * p.ncell[2];
   uint64_t* keys = keys_all + w * (kKeyCap + 8);
+  // Two types: the key region holds instead 32-bit keys (type << 31 | float
+  // bits of r^2), which order the candidates like the 64-bit keys except for
+  // exact r^2 ties, followed by the candidates' extended indices.
+  const bool short_keys = sp.ntypes <= 2;
+  uint32_t* k32 = reinterpret_cast<uint32_t*>(keys);
+  int* kj = reinterpret_cast<int*>(k32 + (kKeyCap + 8));
   int* row = rows_all + w * sp.nnei;
   int16_t* kslot = kslot_all + w * (kKeyCap + 8);
   int16_t* rslot = rslot_all + w * sp.nnei;
@@ -568,11 +576,13 @@ __global__ void __launch_bounds__(256) k_search_cells(
       const int i = cell_atoms[a0 + ai];
       const double xi0 = ext_c[3 * i + 0], xi1 = ext_c[3 * i + 1], xi2 = ext_c[3 * i + 2];
       const float fi0 = (float)xi0, fi1 = (float)xi1, fi2 = (float)xi2;
-      int n = 0;
+      int n = 0, n0 = 0;
       for (int g0 = 0; g0 < T; g0 += kWave) {
         const int g = g0 + lane;
         bool pass = false;
         uint64_t key = 0;
+        uint32_t key32 = 0;
+        int jpass = 0;
         if (g < T) {
           const float4 c = cand[g];
           const int packed = __float_as_int(c.w);
@@ -591,16 +601,24 @@ __global__ void __launch_bounds__(256) k_search_cells(
             if (in) {
               pass = true;
               key = ((uint64_t)tj << 56) | ((uint64_t)__float_as_uint(rr2) << 24) | (uint64_t)j;
+              key32 = ((uint32_t)tj << 31) | __float_as_uint(rr2);
+              jpass = j;
             }
           }
         }
         const uint64_t m = __ballot(pass);
         const int pos = n + mask_prefix(m);
         if (pass && pos < kKeyCap) {
-          keys[pos] = key;
+          if (short_keys) {
+            k32[pos] = key32;
+            kj[pos] = jpass;
+          } else {
+            keys[pos] = key;
+          }
           kslot[pos] = (int16_t)g;
         }
         n += __popcll(m);
+        n0 += __popcll(__ballot(pass && (key >> 56) == 0));
       }
       if (n > kKeyCap) {  // wave-uniform: leave this atom to the per-atom kernel
         if (lane == 0) redo[atomicAdd(&misc[2], 1)] = i;
@@ -613,22 +631,88 @@ __global__ void __launch_bounds__(256) k_search_cells(
       // rank of each key within its type: two keys per lane per pass over the
       // key list, read 8 at a time (independent LDS reads)
       if (n & 7) {  // pad to a multiple of 8 with keys that rank after everything
-        if (lane < 8 - (n & 7)) keys[n + lane] = ~0ull;
+        if (lane < 8 - (n & 7)) {
+          if (short_keys) {
+            k32[n + lane] = ~0u;
+            kj[n + lane] = 0xFFFFFF;
+          } else {
+            keys[n + lane] = ~0ull;
+          }
+        }
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
       const int n8 = (n + 7) & ~7;
       int ovf = -1;
+      // Two types, at most 128 keys: rank by the 32-bit keys, counting the
+      // keys <= each key (itself included; the type bit puts type-0 keys
+      // first, so a type-1 key subtracts their count n0).  Without ties these
+      // counts are 1..n_t within each type, which their sum certifies; with
+      // a tie it exceeds sum n_t (n_t + 1) / 2 and the 64-bit ranking below
+      // decides (index order among equal r^2).
+      bool ranked = false;
+      if (short_keys && n <= 2 * kWave) {
+        const int ka = lane, kb = kWave + lane;
+        const uint32_t a32 = ka < n ? k32[ka] : ~0u, b32 = kb < n ? k32[kb] : ~0u;
+        int ra = 0, rb = 0;
+        for (int q = 0; q < n8; q += 8) {
+          uint32_t kq[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) kq[u] = k32[q + u];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            ra += kq[u] <= a32;
+            rb += kq[u] <= b32;
+          }
+        }
+        ra -= (a32 >> 31) ? n0 : 0;
+        rb -= (b32 >> 31) ? n0 : 0;
+        int sum = (ka < n ? ra : 0) + (kb < n ? rb : 0);
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, kWave);
+        const int n1 = n - n0;
+        if (sum == n0 * (n0 + 1) / 2 + n1 * (n1 + 1) / 2) {
+          ranked = true;
+          if (ka < n) {
+            const int t = (int)(a32 >> 31), r = ra - 1;
+            if (r < sp.sel[t]) {
+              row[sp.sec[t] + r] = kj[ka];
+              rslot[sp.sec[t] + r] = kslot[ka];
+            } else {
+              ovf = max(ovf, t);
+            }
+          }
+          if (kb < n) {
+            const int t = (int)(b32 >> 31), r = rb - 1;
+            if (r < sp.sel[t]) {
+              row[sp.sec[t] + r] = kj[kb];
+              rslot[sp.sec[t] + r] = kslot[kb];
+            } else {
+              ovf = max(ovf, t);
+            }
+          }
+        }
+      }
+      // the 64-bit key of entry q (rebuilt from the short layout for two types)
+      auto key_at = [&](int q, auto shortk) -> uint64_t {
+        if constexpr (decltype(shortk)::value) {
+          const uint32_t k = k32[q];
+          return k == ~0u ? ~0ull
+                          : ((uint64_t)(k >> 31) << 56) | ((uint64_t)(k & 0x7FFFFFFFu) << 24) | (uint64_t)kj[q];
+        } else {
+          return keys[q];
+        }
+      };
+      auto rank64 = [&](auto shortk) {
       for (int k0 = 0; k0 < n; k0 += 2 * kWave) {
         const int ka = k0 + lane, kb = k0 + kWave + lane;
-        const uint64_t kka = ka < n ? keys[ka] : ~0ull, kkb = kb < n ? keys[kb] : ~0ull;
+        const uint64_t kka = ka < n ? key_at(ka, shortk) : ~0ull, kkb = kb < n ? key_at(kb, shortk) : ~0ull;
         const uint64_t ta = kka & (0xFFull << 56), tb = kkb & (0xFFull << 56);
         const uint64_t rela = kka - ta, relb = kkb - tb;
         int ra = 0, rb = 0;
         for (int q = 0; q < n8; q += 8) {
           uint64_t kq[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) kq[u] = keys[q + u];
+          for (int u = 0; u < 8; ++u) kq[u] = key_at(q + u, shortk);
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             ra += (kq[u] - ta) < rela;
@@ -653,6 +737,13 @@ __global__ void __launch_bounds__(256) k_search_cells(
             ovf = max(ovf, t);
           }
         }
+      }
+      };
+      if (!ranked) {
+        if (short_keys)
+          rank64(std::true_type{});
+        else
+          rank64(std::false_type{});
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
