@@ -56,7 +56,7 @@ HBM_PEAK_GBS = 8000.0
 # entry point -> kernel it launches (for the measured-traffic table)
 KERNEL_OF = {"dpmi_se_a_embed_bwd_bf16": "k_embed_bwd<skip>", "dpmi_se_a_embed_fwd_bf16": "k_embed_fwd<skip>",
              "dpmi_se_a_fitting_bf16": "k_fit<false>", "dpmi_se_a_fitting_gr_bf16": "k_fit<true>",
-             "dpmi_force_virial_cells_f32": "k_force_cells<float, 1536, false>"}
+             "dpmi_force_virial_cells_f32": "k_force_cells<float, 1536, false, 3>"}
 
 
 def _issue_bound(workload: str, entry: str):
