@@ -122,29 +122,15 @@ def test_type_order_cache_follows_atype(setup):
     assert e1 == e_ref and e1 != e0
 
 
-@pytest.mark.parametrize("nloc", [192, 1000])
-def test_fused_descriptor_fitting(setup, nloc):
-    """Descriptor fused into the fitting kernel (dpmi_se_a_fitting_gr_bf16)
-    against the two-kernel chain on the same G^T R partials (descriptor
-    forward -> fitting on the bf16 descriptor -> descriptor backward): the
-    same D and fitting arithmetic, dE/dGR summed in another order; padding
-    slots of dgr exactly zero.  nloc = 192: partials of the water box;
-    1000: random partials and types (partial 64-atom tiles in both types)."""
+def _check_fused_descriptor(fe, ff, gr_part, atype):
+    """dpmi_se_a_fitting_gr_bf16 against the two-kernel chain on the same G^T R
+    partials (descriptor forward -> fitting on the bf16 descriptor ->
+    descriptor backward): the same D and fitting arithmetic, dE/dGR summed in
+    another order; padding slots of dgr exactly zero."""
     from dpmi import embed, ops
     from dpmi._lib import ptr
 
-    spec, params, m64, mbf, s, c, t = setup
-    assert mbf.fused_gr
-    fe, ff = mbf.fused, mbf.fused_fit
-    _say(f"fused descriptor + fitting nloc={nloc}")
-    if nloc == s.nloc:
-        ext_c, ext_t, mapping, nlist, _, _, _ = m64.neighbor_list(c, t.to(torch.int32), s.box)
-        gr_part = fe.forward_part(m64.env_mat(nlist, ext_c, ext_t).float())
-        atype = t.to(torch.int32)
-    else:
-        g = torch.Generator(device="cpu").manual_seed(nloc)
-        gr_part = (torch.randn(fe.ntypes, nloc, 4, fe.M, generator=g) * 15.0).to(DEV)
-        atype = (torch.rand(nloc, generator=g) < 0.35).to(torch.int32).to(DEV)
+    nloc = gr_part.shape[1]
     gr = torch.empty((nloc, 4, fe.M), dtype=torch.float32, device=DEV)
     desc = torch.empty((nloc, fe.M * 16), dtype=torch.bfloat16, device=DEV)
     ops._launch("dpmi_se_a_descriptor_fwd", ptr(gr), ptr(desc), ptr(gr_part), nloc, fe.ntypes, fe.nnei, fe.n1, 16, 1,
@@ -160,12 +146,54 @@ def test_fused_descriptor_fitting(setup, nloc):
     q1, q2, q3 = embed.internal_maps(fe.n1)
     pad = np.setdiff1d(np.arange(128), q3)
     assert torch.isfinite(dgr).all() and torch.isfinite(ae).all()
-    assert (dgr[:, torch.as_tensor(pad), :] == 0).all()
+    if len(pad):
+        assert (dgr[:, torch.as_tensor(pad), :] == 0).all()
     e_err = (ae - ae_ref).abs().max().item()
     d_err = ((dgr - ref).abs().max() / ref.abs().max()).item()
     print(f"fused descriptor: max |de| {e_err:.3e} eV, dgr max rel {d_err:.2e}", flush=True)
     assert e_err <= 1e-6 * max(1.0, ae_ref.abs().max().item())
     assert d_err <= 1e-4
+
+
+@pytest.mark.parametrize("nloc", [192, 1000])
+def test_fused_descriptor_fitting(setup, nloc):
+    """Water model: nloc = 192 the partials of the water box, 1000 random
+    partials and types (partial 64-atom tiles in both types)."""
+    spec, params, m64, mbf, s, c, t = setup
+    assert mbf.fused_gr
+    fe, ff = mbf.fused, mbf.fused_fit
+    _say(f"fused descriptor + fitting nloc={nloc}")
+    if nloc == s.nloc:
+        ext_c, ext_t, mapping, nlist, _, _, _ = m64.neighbor_list(c, t.to(torch.int32), s.box)
+        gr_part = fe.forward_part(m64.env_mat(nlist, ext_c, ext_t).float())
+        atype = t.to(torch.int32)
+    else:
+        g = torch.Generator(device="cpu").manual_seed(nloc)
+        gr_part = (torch.randn(fe.ntypes, nloc, 4, fe.M, generator=g) * 15.0).to(DEV)
+        atype = (torch.rand(nloc, generator=g) < 0.35).to(torch.int32).to(DEV)
+    _check_fused_descriptor(fe, ff, gr_part, atype)
+
+
+@pytest.mark.parametrize("n1,ntypes", [(16, 3), (26, 4), (4, 1)])
+def test_fused_descriptor_fitting_shapes(n1, ntypes):
+    """Other widths the fused form takes: M = 64 with three types, M = 104
+    (the widest) with four, M = 16 with one type; random partials and types,
+    700 atoms; fitting width 96."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dpmi.model import SeAEnergyModel
+
+    _say(f"fused descriptor shapes n1={n1} ntypes={ntypes}")
+    sel = tuple(10 + 7 * k for k in range(ntypes))
+    spec = systems.ModelSpec(ntypes=ntypes, sel=sel, neuron=(n1, 2 * n1, 4 * n1), fitting=(96, 96, 96))
+    params = systems.make_params(spec, seed=n1 + ntypes)
+    m = SeAEnergyModel.from_params(spec, params, device=DEV, precision="bf16")
+    assert m.fused_gr
+    g = torch.Generator(device="cpu").manual_seed(n1)
+    nloc = 700
+    gr_part = (torch.randn(ntypes, nloc, 4, 4 * n1, generator=g) * 10.0).to(DEV)
+    atype = torch.randint(0, ntypes, (nloc,), generator=g).to(torch.int32).to(DEV)
+    _check_fused_descriptor(m.fused, m.fused_fit, gr_part, atype)
 
 
 @pytest.mark.parametrize("axis", [4, 8, 12, 16])

@@ -130,6 +130,8 @@ def _env_inputs(oracle, s, rcut, sel, seed=0):
 @pytest.mark.parametrize("name,make,rcut,rs,sel", [
     ("water192", lambda: systems.water192(0), 6.0, 0.5, [46, 92]),
     ("alloy_triclinic_4k", lambda: systems.alloy262k(4, ncell=16), 8.0, 2.0, [60, 60, 60, 60]),
+    # rows longer than 256 slots: the kernel's per-chunk path
+    ("water192_rcut9", lambda: systems.water192(0), 9.0, 0.5, [110, 220]),
 ])
 def test_env_mat_f64(oracle, name, make, rcut, rs, sel):
     s = make()
