@@ -58,6 +58,12 @@ struct EmbedParams {
   int nnei;
   int n1;
   int sec[kMaxTypes + 1];
+  // optional padding shortcut: counts[i * ntypes + t] = listed neighbors of
+  // atom i in section t (the rows after them are padding slots whose em row
+  // is pad_em[(type(i) * ntypes + t) * 4 ..], the same in every such slot)
+  const int16_t* counts;
+  const int* atype;
+  const float* pad_em;
 };
 
 // tanh(x) from z = 2 log2(e) x: 1 - 2 / (2^z + 1), exact limits at +-inf.
@@ -166,8 +172,36 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   for (int k = threadIdx.x; k < kConsts; k += blockDim.x) s_cst[k] = consts[(int64_t)t * kConsts + k];
   const float* cst = s_cst;
   const int sel = p.sec[t + 1] - p.sec[t];
-  const int nchunk = (sel + 31) / 32;
+  const int ntile_sel = (sel + 15) / 16;
   __syncthreads();
+  // Padding shortcut: an atom's tiles after its last listed neighbor hold
+  // only padding slots, all with the same em row (its center type's
+  // -davg/dstd); they are not evaluated, their G^T R share is
+  // n_pad em_pad^T G_pad with G_pad evaluated once per center type here
+  // (with the bf16 roundings the tile path applies).
+  __shared__ float s_gpad[kMaxTypes][kQ3];
+  if (p.counts != nullptr) {
+    for (int ti = w; ti < p.ntypes; ti += kAtoms) {
+      TileFwd f;
+      forward_tile<kSkip>(p.pad_em[(ti * p.ntypes + t) * 4], s_frag, cst, g, lane, f);
+      if (r == 0)
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int q = 16 * mt + 4 * g + v;
+            s_gpad[ti][q] = cst[352 + q] * (1.f - 2.f * (float)(__bf16)f.r3[mt * 4 + v]) +
+                            (float)f.b2[(mt & 3) >> 1][((mt & 3) & 1) * 4 + v];
+          }
+    }
+    __syncthreads();
+  }
+  // tiles of atom ai that are evaluated: all of its section, or with counts
+  // those holding a listed neighbor
+  auto tiles_of = [&](int ai) -> int {
+    if (p.counts == nullptr || ai >= nloc) return ntile_sel;
+    return (p.counts[(int64_t)ai * p.ntypes + t] + 15) >> 4;
+  };
   // identity fragments: B (K = phi slot of a 32-chunk, N = n), one per 16-wide half
   bf16x8 eye[2];
 #pragma unroll
@@ -185,7 +219,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     return v;
   };
   float4 ev_next[2] = {load_em(blockIdx.x * kAtoms + w, r), load_em(blockIdx.x * kAtoms + w, 16 + r)};
+  int nt_i = tiles_of(blockIdx.x * kAtoms + w);
   for (int i = blockIdx.x * kAtoms + w; i < nloc; i += stride) {
+    const int nti = nt_i, nchunk = (nti + 1) >> 1, inext = i + stride;
+    nt_i = tiles_of(inext);  // read ahead of its use
+    const int ti = p.counts != nullptr ? p.atype[i] : -1;
+    if (nchunk == 0) {  // no evaluated tile: the prefetched rows belong to the next atom
+      ev_next[0] = load_em(inext, r);
+      ev_next[1] = load_em(inext, 16 + r);
+    }
     // D(M = c, N = internal feature 16 nt + r): acc = em^T r3, acc_h = em^T h2,
     // acc_s = sum over rows of em (the same in every column)
     f32x4 acc[8], acc_h[4], acc_s = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -195,7 +237,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     for (int nt = 0; nt < 4; ++nt) acc_h[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int ch = 0; ch < nchunk; ++ch) {
       const float4 ev_cur[2] = {ev_next[0], ev_next[1]};
-      const int ni = ch + 1 < nchunk ? i : i + stride, nrow = ch + 1 < nchunk ? (ch + 1) * 32 : 0;
+      const int ni = ch + 1 < nchunk ? i : inext, nrow = ch + 1 < nchunk ? (ch + 1) * 32 : 0;
       ev_next[0] = load_em(ni, nrow + r);
       ev_next[1] = load_em(ni, nrow + 16 + r);
       // weights and constants stay in LDS (re-read per chunk), not in registers
@@ -205,8 +247,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
       for (int rt = 0; rt < 2; ++rt) {
         const float4 ev = ev_cur[rt];
-        if (g == 0) *reinterpret_cast<float4*>(sem + 4 * (16 * rt + r)) = ev;
-        if (32 * ch + 16 * rt >= sel) {  // wave-uniform: a tile of padding rows only (its em rows are 0)
+        // a tile not evaluated contributes no em row either (its padding
+        // slots, if any, are counted by the n_pad term)
+        if (g == 0)
+          *reinterpret_cast<float4*>(sem + 4 * (16 * rt + r)) =
+              2 * ch + rt < nti ? ev : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (2 * ch + rt >= nti) {  // wave-uniform: a tile not evaluated (rows past sel or padding slots)
 #pragma unroll
           for (int kc = 0; kc < 4; ++kc)
 #pragma unroll
@@ -280,14 +326,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       asm volatile("" : "+s"(n1));
       const int M = 4 * n1;
       float* out = gr_part + ((int64_t)t * nloc + i) * 4 * M;
+      // padding rows not evaluated: n_pad em_pad^T G_pad
+      float padc[4] = {0.f, 0.f, 0.f, 0.f};
+      const int n_pad = sel - min(sel, 16 * nti);
+      if (n_pad > 0 && ti >= 0 && ti < p.ntypes)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) padc[v] = (float)n_pad * (float)(__bf16)p.pad_em[(ti * p.ntypes + t) * 4 + v];
+      const int tp = ti >= 0 && ti < p.ntypes ? ti : 0;
 #pragma unroll
       for (int nt = 0; nt < 8; ++nt) {
         const int q = 16 * nt + r;
         const int m = orig_of_q3(q, n1);
         const float idt3 = cst[352 + q];
+        const float gp = n_pad > 0 ? s_gpad[tp][q] : 0.f;
         if (m >= 0)
 #pragma unroll
-          for (int v = 0; v < 4; ++v) out[v * M + m] = idt3 * (acc_s[v] - 2.f * acc[nt][v]) + acc_h[nt & 3][v];
+          for (int v = 0; v < 4; ++v)
+            out[v * M + m] = idt3 * (acc_s[v] - 2.f * acc[nt][v]) + acc_h[nt & 3][v] + padc[v] * gp;
       }
     }
   }
@@ -487,8 +542,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     if (ai < nloc && row < sel) v = *reinterpret_cast<const float4*>(em + ((int64_t)ai * p.nnei + p.sec[t] + row) * 4);
     return v;
   };
+  // evaluated tiles: those holding listed neighbors (see the forward's
+  // padding shortcut); padding slots, whose dE/d em no force term reads,
+  // are written 0
+  auto listed = [&](int ai) -> int {
+    return (p.counts == nullptr || ai >= nloc) ? sel : (int)p.counts[(int64_t)ai * p.ntypes + t];
+  };
   float4 ev_next = load_em(blockIdx.x * kAtoms + w, 0);
+  int nv_next = listed(blockIdx.x * kAtoms + w);
   for (int i = blockIdx.x * kAtoms + w; i < nloc; i += stride) {
+    const int nv = nv_next, nti = (nv + 15) >> 4, inext = i + stride;
+    nv_next = listed(inext);
+    if (nti == 0) ev_next = load_em(inext, 0);
     // stage dGR [q3][c] of the atom: sdg[q3 * 4 + c]
     {
       // (the lane index passes through an opaque copy: hoisted address
@@ -540,10 +605,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     float* nd_i = net_deriv + ((int64_t)i * p.nnei + p.sec[t]) * 4;
-    for (int tl = 0; tl < ntile; ++tl) {
+    if (p.counts != nullptr)
+      for (int row = 16 * nti + lane; row < sel; row += 64)
+        *reinterpret_cast<float4*>(nd_i + 4 * row) = make_float4(0.f, 0.f, 0.f, 0.f);
+    // rows of evaluated tiles past the listed neighbors (padding slots) get 0 too
+    const int nrow_out = p.counts != nullptr ? nv : sel;
+    for (int tl = 0; tl < nti; ++tl) {
       const int row = tl * 16 + r;
       const float4 ev = ev_next;
-      ev_next = tl + 1 < ntile ? load_em(i, tl + 1) : load_em(i + stride, 0);
+      ev_next = tl + 1 < nti ? load_em(i, tl + 1) : load_em(inext, 0);
       // weights and constants are re-read from LDS every tile instead of being
       // held in registers across the loop: that keeps the kernel at two
       // wavefronts per SIMD
@@ -633,11 +703,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       ds += __shfl_xor(ds, 32, 64);
       if (g == 0 && row < sel) {
         float4 o = make_float4(dem[0] + ds, dem[1], dem[2], dem[3]);
+        if (row >= nrow_out) o = make_float4(0.f, 0.f, 0.f, 0.f);
         *reinterpret_cast<float4*>(nd_i + 4 * row) = o;
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// counts[i * ntypes + t] = leading entries >= 0 of section t of row i (the
+// slots env_mat_a_cpu treats as neighbors: it stops at the first -1).
+__global__ void __launch_bounds__(256)
+    k_section_counts(int16_t* __restrict__ counts, const int* __restrict__ nlist, int nloc, EmbedParams p) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * kAtoms + w;
+  if (i >= nloc) return;
+  const int* nl = nlist + (int64_t)i * p.nnei;
+  for (int t = 0; t < p.ntypes; ++t) {
+    int first = p.sec[t + 1];
+    for (int s = p.sec[t] + lane; s < p.sec[t + 1]; s += 64)
+      if (nl[s] < 0) first = min(first, s);
+    for (int off = 32; off > 0; off >>= 1) first = min(first, __shfl_xor(first, off, 64));
+    if (lane == 0) counts[(int64_t)i * p.ntypes + t] = (int16_t)(first - p.sec[t]);
   }
 }
 
@@ -660,10 +748,16 @@ int make_params(EmbedParams* p, const int* sel_host, int ntypes) {
 using namespace dpmi;
 
 extern "C" int dpmi_se_a_embed_fwd_bf16(float* gr_part, const float* em, const void* frags, const float* consts,
-                                        int64_t nloc, const int* sel_host, int ntypes, int n1, void* stream) {
+                                        int64_t nloc, const int* sel_host, int ntypes, int n1, const int16_t* counts,
+                                        const int* atype, const float* pad_em, void* stream) {
   EmbedParams p;
   int rc = make_params(&p, sel_host, ntypes);
   if (rc) return rc;
+  DPMI_CHECK_ARG(counts == nullptr || (atype != nullptr && pad_em != nullptr),
+                 "the padding shortcut needs atype and pad_em with counts");
+  p.counts = counts;
+  p.atype = atype;
+  p.pad_em = pad_em;
   DPMI_CHECK_ARG(n1 > 0 && n1 <= 32, "n1 out of range [1, 32]");
   p.n1 = n1;
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
@@ -676,6 +770,20 @@ extern "C" int dpmi_se_a_embed_fwd_bf16(float* gr_part, const float* em, const v
   else
     hipLaunchKernelGGL(k_embed_fwd<false>, grid, dim3(256), 0, as_stream(stream), gr_part, em,
                        static_cast<const bf16x8*>(frags), consts, (int)nloc, p);
+  DPMI_LAUNCH_CHECK();
+  return DPMI_OK;
+}
+
+extern "C" int dpmi_se_a_section_counts(int16_t* counts, const int* nlist, int64_t nloc, const int* sel_host,
+                                        int ntypes, void* stream) {
+  EmbedParams p;
+  int rc = make_params(&p, sel_host, ntypes);
+  if (rc) return rc;
+  DPMI_CHECK_ARG(p.nnei < 32768, "sum(sel) must be < 32768");
+  DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
+  if (nloc == 0) return DPMI_OK;
+  hipLaunchKernelGGL(k_section_counts, dim3((unsigned)ceil_div(nloc, kAtoms)), dim3(256), 0, as_stream(stream), counts,
+                     nlist, (int)nloc, p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }
@@ -714,10 +822,11 @@ extern "C" int dpmi_se_a_descriptor_bwd(float* dgr, const float* ddesc, const fl
 
 extern "C" int dpmi_se_a_embed_bwd_bf16(float* net_deriv, const float* em, const float* dgr, const void* frags,
                                         const float* consts, int64_t nloc, const int* sel_host, int ntypes, int n1,
-                                        void* stream) {
+                                        const int16_t* counts, void* stream) {
   EmbedParams p;
   int rc = make_params(&p, sel_host, ntypes);
   if (rc) return rc;
+  p.counts = counts;
   DPMI_CHECK_ARG(n1 > 0 && n1 <= 32, "n1 out of range [1, 32]");
   p.n1 = n1;
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");

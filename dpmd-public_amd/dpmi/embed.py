@@ -146,43 +146,58 @@ class FusedEmbedding:
         self.frags = torch.from_numpy(bits.astype(np.int16)).to(device)
         self.consts = torch.from_numpy(consts).to(device)
 
-    def forward_part(self, em: torch.Tensor) -> torch.Tensor:
+    def forward_part(self, em: torch.Tensor, pad=None) -> torch.Tensor:
         """em [nloc, nnei, 4] fp32 -> gr_part [ntypes, nloc, 4, M]: each
-        neighbor type's em^T G (original feature order, not yet / nnei)."""
+        neighbor type's em^T G (original feature order, not yet / nnei).
+        pad = (counts int16 [nloc, ntypes], atype int32 [nloc], pad_em fp32
+        [ntypes, ntypes, 4]) enables the padding shortcut: the slots after
+        each section's listed neighbors all hold pad_em[type(i), t]."""
         nloc = em.shape[0]
         gr_part = torch.empty((self.ntypes, nloc, 4, self.M), dtype=torch.float32, device=em.device)
+        counts, atype, pad_em = pad if pad is not None else (None, None, None)
         ops._launch("dpmi_se_a_embed_fwd_bf16", ptr(gr_part), ptr(em.contiguous()), ptr(self.frags), ptr(self.consts),
-                    nloc, ptr(self.sel), self.ntypes, self.n1, ops._stream())
+                    nloc, ptr(self.sel), self.ntypes, self.n1, ptr(counts), ptr(atype), ptr(pad_em), ops._stream())
         return gr_part
 
-    def forward(self, em: torch.Tensor, desc_dtype=torch.float32):
+    def forward(self, em: torch.Tensor, desc_dtype=torch.float32, pad=None):
         """em [nloc, nnei, 4] fp32 -> (gr [nloc, 4, M], desc [nloc, M*axis] in
         desc_dtype: float32, or bfloat16 for the fused fitting kernel)."""
         nloc = em.shape[0]
         dev = em.device
-        gr_part = self.forward_part(em)
+        gr_part = self.forward_part(em, pad)
         gr = torch.empty((nloc, 4, self.M), dtype=torch.float32, device=dev)
         desc = torch.empty((nloc, self.M * self.axis), dtype=desc_dtype, device=dev)
         ops._launch("dpmi_se_a_descriptor_fwd", ptr(gr), ptr(desc), ptr(gr_part), nloc, self.ntypes, self.nnei,
                     self.n1, self.axis, int(desc_dtype == torch.bfloat16), ops._stream())
         return gr, desc
 
-    def backward(self, em: torch.Tensor, gr: torch.Tensor, ddesc: torch.Tensor) -> torch.Tensor:
+    def backward(self, em: torch.Tensor, gr: torch.Tensor, ddesc: torch.Tensor, counts=None) -> torch.Tensor:
         """dE/d desc -> dE/d em [nloc, nnei, 4] fp32 (gr as returned by forward)."""
         nloc = em.shape[0]
         dgr = torch.empty((nloc, 128, 4), dtype=torch.float32, device=em.device)
         ops._launch("dpmi_se_a_descriptor_bwd", ptr(dgr), ptr(ddesc.contiguous()), ptr(gr), nloc, self.nnei, self.n1,
                     self.axis, ops._stream())
-        return self.backward_dgr(em, dgr)
+        return self.backward_dgr(em, dgr, counts)
 
-    def backward_dgr(self, em: torch.Tensor, dgr: torch.Tensor) -> torch.Tensor:
+    def backward_dgr(self, em: torch.Tensor, dgr: torch.Tensor, counts=None) -> torch.Tensor:
         """dgr [nloc, 128, 4] = dE/d(sum_t gr_part) in the internal order ->
-        dE/d em [nloc, nnei, 4] fp32 (recomputing the embedding forward)."""
+        dE/d em [nloc, nnei, 4] fp32 (recomputing the embedding forward).
+        With counts (the forward's padding shortcut) the padding slots after
+        each section's listed neighbors get 0."""
         nloc = em.shape[0]
         nd = torch.empty((nloc, self.nnei, 4), dtype=torch.float32, device=em.device)
         ops._launch("dpmi_se_a_embed_bwd_bf16", ptr(nd), ptr(em.contiguous()), ptr(dgr.contiguous()),
-                    ptr(self.frags), ptr(self.consts), nloc, ptr(self.sel), self.ntypes, self.n1, ops._stream())
+                    ptr(self.frags), ptr(self.consts), nloc, ptr(self.sel), self.ntypes, self.n1, ptr(counts),
+                    ops._stream())
         return nd
+
+    def section_counts(self, nlist: torch.Tensor) -> torch.Tensor:
+        """Listed neighbors per atom and type section: int16 [nloc, ntypes]."""
+        nloc = nlist.shape[0]
+        counts = torch.empty((nloc, self.ntypes), dtype=torch.int16, device=nlist.device)
+        ops._launch("dpmi_se_a_section_counts", ptr(counts), ptr(nlist.contiguous()), nloc, ptr(self.sel),
+                    self.ntypes, ops._stream())
+        return counts
 
 
 # --------------------------------------------------------------------------- FP64 form

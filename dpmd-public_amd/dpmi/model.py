@@ -182,6 +182,9 @@ class SeAEnergyModel:
         # the descriptor fused into the fitting kernel (axis 16, M <= 104)
         self.fused_gr = self.fused_fit is not None and self.fused_fit.supports_gr(self.fused.n1, self.axis)
         self._order_cache = None
+        # padding shortcut of the bf16 embedding kernels (each davg / dstd
+        # section constant, so all padding slots of a section share one em row)
+        self.pad_em = self._padding_rows() if self.fused is not None else None
 
     def named_parameters(self):
         """[((kind, net, layer, name), tensor)] of the trainable tensors, made
@@ -321,7 +324,7 @@ class SeAEnergyModel:
         nlist, _ovf, _ = ops.format_nlist(cand, ext_c, ext_t, nloc, self.rcut, self.sel, numneigh=numneigh)
         em = self.env_mat(nlist, ext_c, ext_t)
         if self.fused is not None:
-            nd, ae = self._fused_chain(em, atype)
+            nd, ae = self._fused_chain(em, atype, pad=self._pad_args(nlist, atype))
         elif self.fused64 is not None:
             g = self.fused64.forward(em).requires_grad_(True)
             with torch.enable_grad():
@@ -365,6 +368,30 @@ class SeAEnergyModel:
             out["atom_energy"] = ae
         return out
 
+    def _padding_rows(self):
+        """pad_em [ntypes (center), ntypes (neighbor), 4] float32 on the
+        device: the em row of every padding slot of a section, (0 - davg) /
+        dstd as the fp32 env-mat computes it (env_mat.cc keeps -davg/dstd in
+        empty slots); None when davg or dstd varies within a section."""
+        av = self.davg.detach().to(torch.float32).cpu().numpy().reshape(self.ntypes, self.nnei, 4)
+        sd = self.dstd.detach().to(torch.float32).cpu().numpy().reshape(self.ntypes, self.nnei, 4)
+        pad = np.zeros((self.ntypes, self.ntypes, 4), dtype=np.float32)
+        for ti in range(self.ntypes):
+            for t in range(self.ntypes):
+                a, d = av[ti, self.sec[t]:self.sec[t + 1]], sd[ti, self.sec[t]:self.sec[t + 1]]
+                if len(a) == 0:
+                    continue
+                if not (np.all(a == a[0]) and np.all(d == d[0])):
+                    return None
+                pad[ti, t] = (np.float32(0.0) - a[0]) / d[0]
+        return torch.from_numpy(pad).to(self.device)
+
+    def _pad_args(self, nlist, atype):
+        """(counts, atype, pad_em) of the padding shortcut for this list, or None."""
+        if self.pad_em is None:
+            return None
+        return self.fused.section_counts(nlist), atype.to(torch.int32).contiguous(), self.pad_em
+
     def _type_order(self, atype_in, atype):
         """(perm, type_start) of the fused fitting kernel, reused while the
         caller passes the same, unmodified atype tensor (the cache holds the
@@ -379,18 +406,19 @@ class SeAEnergyModel:
         self._order_cache = (atype_in, atype_in._version, order)
         return order
 
-    def _fused_chain(self, em, atype, use_gr=None, order=None):
+    def _fused_chain(self, em, atype, use_gr=None, order=None, pad=None):
         """bf16 embedding -> descriptor -> fitting -> adjoints: (dE/d em, atom
         energies).  With the descriptor fused into the fitting kernel
         (fused_gr) the G^T R partials go straight to it and dE/dGR comes back;
         otherwise the descriptor kernels run between (and torch runs the
         fitting net when its shape has no fused kernel)."""
         use_gr = self.fused_gr if use_gr is None else use_gr
+        counts = pad[0] if pad is not None else None
         if use_gr:
-            gr_part = self.fused.forward_part(em)
+            gr_part = self.fused.forward_part(em, pad)
             ae, dgr = self.fused_fit.from_gr(gr_part, atype, self.fused.n1, self.fused.nnei, order=order)
-            return self.fused.backward_dgr(em, dgr), ae
-        gr, desc = self.fused.forward(em, torch.bfloat16 if self.fused_fit is not None else torch.float32)
+            return self.fused.backward_dgr(em, dgr, counts), ae
+        gr, desc = self.fused.forward(em, torch.bfloat16 if self.fused_fit is not None else torch.float32, pad)
         if self.fused_fit is not None:
             ae, ddesc = self.fused_fit(desc, atype, order=order)
         else:
@@ -398,7 +426,7 @@ class SeAEnergyModel:
             with torch.enable_grad():
                 ae = self.atom_energy(desc, atype)
                 (ddesc,) = torch.autograd.grad(ae.sum(), desc)
-        return self.fused.backward(em, gr, ddesc), ae
+        return self.fused.backward(em, gr, ddesc, counts), ae
 
     def _compute_fused(self, atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy, order=None):
         """bf16 form with hand-written adjoints end to end: fused embedding +
@@ -407,7 +435,7 @@ class SeAEnergyModel:
         descriptor and embedding adjoints recomputing the forward, env-mat
         adjoint for forces and virial."""
         em = self.env_mat(nlist, ext_c, ext_t)
-        nd, ae = self._fused_chain(em, atype, order=order)
+        nd, ae = self._fused_chain(em, atype, order=order, pad=self._pad_args(nlist, atype))
         energy = ae.sum()
         force, virial = self.forces(nd, nlist, cells, mapping, ext_c, ext_t, box)
         out = {"energy": energy.detach(), "force": force, "virial": virial}

@@ -252,10 +252,22 @@ int dpmi_tabulate_fusion_se_a_grad_grad_f32(float* dz_dy, const float* table, co
  * [ntypes, 704] fp32, both laid out by dpmi.embed.pack_embedding (internal
  * feature order q3; see that module).  em [nloc, nnei, 4] fp32.
  * ------------------------------------------------------------------------ */
+/* counts [nloc, ntypes] int16 = listed neighbors of each atom per type
+ * section of nlist [nloc, sum(sel)] (entries before the section's first -1,
+ * where env_mat_a_cpu stops, source/lib/src/env_mat.cc:139-140). */
+int dpmi_se_a_section_counts(int16_t* counts, const int* nlist, int64_t nloc, const int* sel_host, int ntypes,
+                             void* stream);
 /* gr_part [ntypes, nloc, 4, 4 n1]: sum over each type's neighbors of em^T G
- * (original feature order). */
+ * (original feature order).  Optional padding shortcut (counts non-NULL, as
+ * dpmi_se_a_section_counts writes them): the slots after an atom's listed
+ * neighbors must all hold the same em row, pad_em [ntypes (center),
+ * ntypes (neighbor), 4] for the atom's type atype [nloc] (true when davg /
+ * dstd are constant over each section, the reference's trailing-padding
+ * assumption, source/lib/src/gpu/tabulate.cu:217-237); their share is then added as
+ * n_pad em_pad^T G(em_pad) instead of evaluating the net per slot. */
 int dpmi_se_a_embed_fwd_bf16(float* gr_part, const float* em, const void* frags, const float* consts, int64_t nloc,
-                             const int* sel_host, int ntypes, int n1, void* stream);
+                             const int* sel_host, int ntypes, int n1, const int16_t* counts, const int* atype,
+                             const float* pad_em, void* stream);
 /* gr [nloc, 4, 4 n1] = sum_t gr_part / nnei; desc [nloc, 4 n1 * axis] =
  * GR^T GR[:, :axis] (se_a.py:851-858), fp32 or (desc_bf16 != 0) bf16. */
 int dpmi_se_a_descriptor_fwd(float* gr, void* desc, const float* gr_part, int64_t nloc, int ntypes, int nnei,
@@ -266,10 +278,13 @@ int dpmi_se_a_descriptor_fwd(float* gr, void* desc, const float* gr_part, int64_
 int dpmi_se_a_descriptor_bwd(float* dgr, const float* ddesc, const float* gr, int64_t nloc, int nnei, int n1,
                              int axis, void* stream);
 /* net_deriv [nloc, nnei, 4] = dE/d em, recomputing the embedding forward
- * (n1 = width of the first embedding layer, as packed). */
+ * (n1 = width of the first embedding layer, as packed).  With counts (the
+ * padding shortcut of the forward) the rows after each section's listed
+ * neighbors are written 0: they are padding slots, which no force term
+ * reads. */
 int dpmi_se_a_embed_bwd_bf16(float* net_deriv, const float* em, const float* dgr, const void* frags,
                              const float* consts, int64_t nloc, const int* sel_host, int ntypes, int n1,
-                             void* stream);
+                             const int16_t* counts, void* stream);
 
 /* FP64 form of the same two kernels (v_mfma_f64_16x16x4f64, fp64
  * throughout).  Weights as packed by dpmi.embed.pack_embedding_f64: w2

@@ -104,6 +104,53 @@ def test_fused_fitting(setup, nloc):
     assert d_err <= 1e-2
 
 
+@pytest.mark.parametrize("system", ["water192", "water12k"])
+def test_padding_shortcut_matches_full_rows(setup, system):
+    """The embedding kernels' padding shortcut (tiles after an atom's listed
+    neighbors not evaluated; their padding slots' share added as
+    n_pad em_pad^T G(em_pad)) against evaluating every slot: the G^T R
+    partials agree to the fp32 summation order; from the same dE/dGR the
+    backward gives bitwise the same dE/d em on the listed slots and 0 on
+    the padding slots.  (Downstream of G^T R the bf16 roundings of D and of
+    the fitting activations amplify a last-bit difference to the form's
+    noise level, so energies are compared against FP64 elsewhere.)"""
+    from dpmi import embed
+
+    spec, params, m64, mbf, s, c, t = setup
+    assert mbf.pad_em is not None
+    _say(f"padding shortcut {system}")
+    if system != "water192":
+        s = systems.water12k(1)
+        c = torch.tensor(s.coord, device=DEV)
+        t = torch.tensor(s.atype, device=DEV)
+    at = t.to(torch.int32)
+    ext_c, ext_t, mapping, nlist, _, _, cells = mbf.neighbor_list(c, at, s.box)
+    em = mbf.env_mat(nlist, ext_c, ext_t)
+    pad = mbf._pad_args(nlist, at)
+    counts = pad[0]
+    # the counts are the listed neighbors of each section
+    ref_counts = torch.stack([(nlist[:, mbf.sec[k]:mbf.sec[k + 1]] >= 0).sum(1) for k in range(mbf.ntypes)], 1)
+    assert torch.equal(counts.to(torch.int64), ref_counts)
+    g1 = mbf.fused.forward_part(em, pad)
+    g0 = mbf.fused.forward_part(em, None)
+    gerr = ((g1 - g0).abs().max() / g0.abs().max()).item()
+    # one dE/dGR for both backward variants
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    dgr = (torch.randn(s.nloc, 128, 4, generator=gen) * 0.01).to(DEV)
+    q1, q2, q3 = embed.internal_maps(mbf.fused.n1)
+    mask = torch.zeros(128, dtype=torch.bool)
+    mask[torch.as_tensor(q3)] = True
+    dgr[:, ~mask.to(DEV), :] = 0
+    nd1 = mbf.fused.backward_dgr(em, dgr, counts)
+    nd0 = mbf.fused.backward_dgr(em, dgr, None)
+    torch.cuda.synchronize()
+    valid = nlist >= 0
+    print(f"G^T R max rel {gerr:.2e}; padded slots {(~valid).sum().item()} of {valid.numel()}", flush=True)
+    assert gerr <= 1e-5
+    assert torch.equal(nd1[valid], nd0[valid])
+    assert (nd1[~valid] == 0).all()
+
+
 def test_type_order_cache_follows_atype(setup):
     """The model reuses the type ordering of the fitting kernel while the
     same atype tensor comes back unchanged: an in-place change of the types
