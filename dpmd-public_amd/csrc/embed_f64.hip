@@ -41,6 +41,8 @@ constexpr int kP3 = 129;    // LDS row pitch (doubles) of W3 [64][128]
 constexpr int kC64 = 480;   // w1[32] b1[32] idt1[32] b2[64] idt2[64] b3[128] idt3[128]
 constexpr int kWaves64 = 8;  // wavefronts per workgroup: two per SIMD, so FP64 MFMA and VALU of different waves overlap
 constexpr size_t kLdsW = (size_t)(32 * kP2 + 64 * kP3 + kC64) * sizeof(double);
+// forward: G of the padding row of each center type after the weights
+constexpr size_t kLdsFwd = kLdsW + (size_t)kMaxTypes64 * 128 * sizeof(double);
 
 struct Embed64Params {
   int ntypes;
@@ -49,6 +51,11 @@ struct Embed64Params {
   int nblk_total;
   int sec[kMaxTypes64 + 1];
   int blk[kMaxTypes64 + 1];  // workgroup ranges per neighbor type
+  // optional padding shortcut (as in the bf16 kernels): counts[i * ntypes + t]
+  // listed neighbors, atype[i], pad_em[(type * ntypes + t) * 4 + c]
+  const int16_t* counts;
+  const int* atype;
+  const double* pad_em;
 };
 
 __device__ inline f64x4 mfma64(double a, double b, const f64x4& c) {
@@ -228,15 +235,34 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
   const int n1 = p.n1, M = 4 * n1;
   const int sel = p.sec[t + 1] - p.sec[t], ntile = (sel + 15) / 16;
   const int nb = p.blk[t + 1] - p.blk[t], b0 = blockIdx.x - p.blk[t];
+  // padding shortcut: G of each center type's padding row, once per workgroup
+  double* s_gpad = const_cast<double*>(sC) + kC64;  // [ntypes][128]
+  if (p.counts != nullptr) {
+    for (int ti = w; ti < p.ntypes; ti += kWaves64) {
+      double h1[8], h2[16], t1[8], t2[16];
+      layers12<NL, false>(p.pad_em[(ti * p.ntypes + t) * 4], sW2, sC, g, r, h1, h2, t1, t2);
+#pragma unroll
+      for (int mt = 0; mt < 8; ++mt) {
+        double G[4], t3[4];
+        layer3_act<NL>(mt, layer3_z<NL>(mt, h2, sW3, sC, g, r), h2, sC, g, G, t3);
+        if (r == 0)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) s_gpad[ti * 128 + 16 * mt + 4 * v + g] = G[v];
+      }
+    }
+    __syncthreads();
+  }
   for (int i = b0 * kWaves64 + w; i < nloc; i += nb * kWaves64) {
     const double* emi = em + ((int64_t)i * p.nnei + p.sec[t]) * 4;
+    // tiles holding listed neighbors (all of the section without counts)
+    const int nti = p.counts != nullptr ? (p.counts[(int64_t)i * p.ntypes + t] + 15) >> 4 : ntile;
     // GR[c = g][16 mt + r]: slot 0 of each tile's GR^T product (the other
     // slots are zero rows c >= 4), accumulated here so that no full
     // accumulator stays live across tiles
     double acc[8];
 #pragma unroll
     for (int mt = 0; mt < 8; ++mt) acc[mt] = 0.;
-    for (int tl = 0; tl < ntile; ++tl) {
+    for (int tl = 0; tl < nti; ++tl) {
       const int row = 16 * tl + r;
       const double s = row < sel ? emi[4 * row] : 0.;
       // em^T as A of GR^T: (M = c = r < 4, K = row 16 tl + 4 kc + g)
@@ -269,6 +295,16 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
           for (int kc = 0; kc < 4; ++kc) grt = mfma64(a_em[kc], T[kc], grt);
           acc[mt] += grt[0];
         }
+      }
+    }
+    // padding slots not evaluated: n_pad em_pad[c] G_pad
+    const int n_pad = sel - min(sel, 16 * nti);
+    if (n_pad > 0) {
+      const int ti = p.atype[i];
+      if (ti >= 0 && ti < p.ntypes) {
+        const double e = (double)n_pad * p.pad_em[(ti * p.ntypes + t) * 4 + g];
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt) acc[mt] += e * s_gpad[ti * 128 + 16 * mt + r];
       }
     }
     // acc[mt] slot 0 of lane group g = c holds GR[c][q3 = 16 mt + r]
@@ -311,7 +347,12 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     double* ndi = net_deriv + ((int64_t)i * p.nnei + p.sec[t]) * 4;
-    for (int tl = 0; tl < ntile; ++tl) {
+    // padding shortcut: tiles past the listed neighbors are not evaluated and
+    // every padding slot gets dE/d em = 0 (no force term reads it)
+    const int nv = p.counts != nullptr ? (int)p.counts[(int64_t)i * p.ntypes + t] : sel;
+    const int nti = (nv + 15) >> 4;
+    for (int row = 16 * nti + r; row < sel; row += 16) ndi[4 * row + g] = 0.;
+    for (int tl = 0; tl < nti; ++tl) {
       const int row = 16 * tl + r;
       const bool in = row < sel;
       const double s = in ? emi[4 * row] : 0.;
@@ -384,7 +425,7 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
       ds += __shfl_xor(ds, 16, 64);
       ds += __shfl_xor(ds, 32, 64);
       // dem slot 0 of lane group g = c holds dE/d em[row r][c] (direct part)
-      if (in) ndi[4 * row + g] = dem[0] + (g == 0 ? ds : 0.);
+      if (in) ndi[4 * row + g] = row < nv ? dem[0] + (g == 0 ? ds : 0.) : 0.;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -425,17 +466,23 @@ using namespace dpmi;
 
 extern "C" int dpmi_se_a_embed_fwd_f64(double* gr_part, const double* em, const double* w2, const double* w3,
                                        const double* consts, int64_t nloc, const int* sel_host, int ntypes, int n1,
+                                       const int16_t* counts, const int* atype, const double* pad_em,
                                        void* stream) {
   Embed64Params p;
   int rc = make_params64(&p, sel_host, ntypes, n1, nloc);
   if (rc) return rc;
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
+  DPMI_CHECK_ARG(counts == nullptr || (atype != nullptr && pad_em != nullptr),
+                 "the padding shortcut needs atype and pad_em with counts");
+  p.counts = counts;
+  p.atype = atype;
+  p.pad_em = pad_em;
   if (nloc == 0) return DPMI_OK;
   auto launch = [&](auto kern) -> int {
     DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)kLdsW));
-    hipLaunchKernelGGL(kern, dim3(p.nblk_total), dim3(64 * kWaves64), kLdsW, as_stream(stream), gr_part, em, w2, w3,
-                       consts, (int)nloc, p);
+                                 (int)kLdsFwd));
+    hipLaunchKernelGGL(kern, dim3(p.nblk_total), dim3(64 * kWaves64), kLdsFwd, as_stream(stream), gr_part, em, w2,
+                       w3, consts, (int)nloc, p);
     return DPMI_OK;
   };
   rc = n1 <= 24 ? launch(k_embed64_fwd<24>) : n1 <= 28 ? launch(k_embed64_fwd<28>) : launch(k_embed64_fwd<32>);
@@ -446,10 +493,11 @@ extern "C" int dpmi_se_a_embed_fwd_f64(double* gr_part, const double* em, const 
 
 extern "C" int dpmi_se_a_embed_bwd_f64(double* net_deriv, const double* em, const double* dgr, const double* w2,
                                        const double* w3, const double* consts, int64_t nloc, const int* sel_host,
-                                       int ntypes, int n1, void* stream) {
+                                       int ntypes, int n1, const int16_t* counts, void* stream) {
   Embed64Params p;
   int rc = make_params64(&p, sel_host, ntypes, n1, nloc);
   if (rc) return rc;
+  p.counts = counts;
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
   if (nloc == 0) return DPMI_OK;
   const size_t shm = lds_bwd();

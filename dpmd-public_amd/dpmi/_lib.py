@@ -45,8 +45,8 @@ SIGNATURES = {
     "dpmi_se_a_descriptor_bwd": [_P, _P, _P, _L, _I, _I, _I, _P],
     "dpmi_se_a_embed_bwd_bf16": [_P, _P, _P, _P, _P, _L, _P, _I, _I, _P, _P],
     "dpmi_se_a_section_counts": [_P, _P, _L, _P, _I, _P],
-    "dpmi_se_a_embed_fwd_f64": [_P, _P, _P, _P, _P, _L, _P, _I, _I, _P],
-    "dpmi_se_a_embed_bwd_f64": [_P, _P, _P, _P, _P, _P, _L, _P, _I, _I, _P],
+    "dpmi_se_a_embed_fwd_f64": [_P, _P, _P, _P, _P, _L, _P, _I, _I, _P, _P, _P, _P],
+    "dpmi_se_a_embed_bwd_f64": [_P, _P, _P, _P, _P, _P, _L, _P, _I, _I, _P, _P],
     "dpmi_se_a_fitting_bf16": [_P, _P, _P, _P, _P, _P, _P, _P, _L, _I, _I, _P],
     "dpmi_se_a_fitting_gr_bf16": [_P, _P, _P, _P, _P, _P, _P, _P, _L, _I, _I, _I, _P],
     "dpmi_se_a_fitting_frag_count": [_P, _I],

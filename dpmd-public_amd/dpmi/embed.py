@@ -255,20 +255,24 @@ class FusedEmbeddingF64:
         self.w3 = torch.from_numpy(w3).to(device)
         self.consts = torch.from_numpy(consts).to(device)
 
-    def forward(self, em: torch.Tensor) -> torch.Tensor:
-        """em [nloc, nnei, 4] float64 -> GR [nloc, 4, M] = sum_t em_t^T G_t / nnei."""
+    def forward(self, em: torch.Tensor, pad=None) -> torch.Tensor:
+        """em [nloc, nnei, 4] float64 -> GR [nloc, 4, M] = sum_t em_t^T G_t / nnei.
+        pad = (counts, atype, pad_em float64): the padding shortcut."""
         nloc = em.shape[0]
         em = em.contiguous()
         part = torch.empty((self.ntypes, nloc, 4, self.M), dtype=torch.float64, device=em.device)
+        counts, atype, pad_em = pad if pad is not None else (None, None, None)
         ops._launch("dpmi_se_a_embed_fwd_f64", ptr(part), ptr(em), ptr(self.w2), ptr(self.w3), ptr(self.consts),
-                    nloc, ptr(self.sel), self.ntypes, self.n1, ops._stream())
+                    nloc, ptr(self.sel), self.ntypes, self.n1, ptr(counts), ptr(atype), ptr(pad_em), ops._stream())
         return part.sum(dim=0) / self.nnei
 
-    def backward(self, em: torch.Tensor, dgr: torch.Tensor) -> torch.Tensor:
-        """dE/dGR [nloc, 4, M] -> dE/d em [nloc, nnei, 4] float64."""
+    def backward(self, em: torch.Tensor, dgr: torch.Tensor, counts=None) -> torch.Tensor:
+        """dE/dGR [nloc, 4, M] -> dE/d em [nloc, nnei, 4] float64 (padding slots
+        0 with counts)."""
         nloc = em.shape[0]
         dgr_t = (dgr / self.nnei).contiguous()
         nd = torch.empty((nloc, self.nnei, 4), dtype=torch.float64, device=em.device)
         ops._launch("dpmi_se_a_embed_bwd_f64", ptr(nd), ptr(em.contiguous()), ptr(dgr_t), ptr(self.w2),
-                    ptr(self.w3), ptr(self.consts), nloc, ptr(self.sel), self.ntypes, self.n1, ops._stream())
+                    ptr(self.w3), ptr(self.consts), nloc, ptr(self.sel), self.ntypes, self.n1, ptr(counts),
+                    ops._stream())
         return nd

@@ -27,7 +27,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from . import embed, fitting, ops
+from . import _lib, embed, fitting, ops
 
 _ACT = torch.tanh
 
@@ -144,6 +144,7 @@ class SeAEnergyModel:
         self.rcut_smth = float(rcut_smth)
         self.sel = [int(s) for s in sel]
         self.sec = np.concatenate([[0], np.cumsum(self.sel)]).astype(int)
+        self._sel_i32 = np.ascontiguousarray(np.asarray(self.sel, dtype=np.int32))
         self.nnei = int(sum(self.sel))
         self.ntypes = len(self.sel)
         self.axis = int(axis)
@@ -184,7 +185,7 @@ class SeAEnergyModel:
         self._order_cache = None
         # padding shortcut of the bf16 embedding kernels (each davg / dstd
         # section constant, so all padding slots of a section share one em row)
-        self.pad_em = self._padding_rows() if self.fused is not None else None
+        self.pad_em = self._padding_rows() if (self.fused is not None or self.fused64 is not None) else None
 
     def named_parameters(self):
         """[((kind, net, layer, name), tensor)] of the trainable tensors, made
@@ -326,12 +327,13 @@ class SeAEnergyModel:
         if self.fused is not None:
             nd, ae = self._fused_chain(em, atype, pad=self._pad_args(nlist, atype))
         elif self.fused64 is not None:
-            g = self.fused64.forward(em).requires_grad_(True)
+            pad = self._pad_args(nlist, atype)
+            g = self.fused64.forward(em, pad).requires_grad_(True)
             with torch.enable_grad():
                 d = torch.bmm(g.transpose(1, 2), g[:, :, : self.axis]).reshape(nloc, -1)
                 ae = self.atom_energy(d, atype)
                 (dgr,) = torch.autograd.grad(ae.sum(), g)
-            nd = self.fused64.backward(em, dgr)
+            nd = self.fused64.backward(em, dgr, pad[0] if pad is not None else None)
         else:
             em = em.detach().requires_grad_(True)
             with torch.enable_grad():
@@ -348,7 +350,8 @@ class SeAEnergyModel:
         rocBLAS) through torch autograd in atom blocks; env-mat adjoint for
         forces and virial."""
         em = self.env_mat(nlist, ext_c, ext_t)
-        gr = self.fused64.forward(em)
+        pad = self._pad_args(nlist, atype)
+        gr = self.fused64.forward(em, pad)
         nloc = em.shape[0]
         blk = max(1024, self.ROWS_PER_BLOCK // 64)
         dgr = torch.empty_like(gr)
@@ -361,7 +364,7 @@ class SeAEnergyModel:
                 ae_b = self.atom_energy(d, atype[a:b])
                 (dgr[a:b],) = torch.autograd.grad(ae_b.sum(), g)
             ae[a:b] = ae_b.detach()
-        nd = self.fused64.backward(em, dgr)
+        nd = self.fused64.backward(em, dgr, pad[0] if pad is not None else None)
         force, virial = self.forces(nd, nlist, cells, mapping, ext_c, ext_t, box)
         out = {"energy": ae.sum(), "force": force, "virial": virial}
         if need_atom_energy:
@@ -369,13 +372,15 @@ class SeAEnergyModel:
         return out
 
     def _padding_rows(self):
-        """pad_em [ntypes (center), ntypes (neighbor), 4] float32 on the
-        device: the em row of every padding slot of a section, (0 - davg) /
-        dstd as the fp32 env-mat computes it (env_mat.cc keeps -davg/dstd in
-        empty slots); None when davg or dstd varies within a section."""
-        av = self.davg.detach().to(torch.float32).cpu().numpy().reshape(self.ntypes, self.nnei, 4)
-        sd = self.dstd.detach().to(torch.float32).cpu().numpy().reshape(self.ntypes, self.nnei, 4)
-        pad = np.zeros((self.ntypes, self.ntypes, 4), dtype=np.float32)
+        """pad_em [ntypes (center), ntypes (neighbor), 4] on the device, in
+        the env-mat's dtype: the em row of every padding slot of a section,
+        (0 - davg) / dstd as the env-mat computes it (env_mat.cc keeps
+        -davg/dstd in empty slots); None when davg or dstd varies within a
+        section."""
+        npdt = np.float32 if self.dtype == torch.float32 else np.float64
+        av = self.davg.detach().cpu().numpy().astype(npdt).reshape(self.ntypes, self.nnei, 4)
+        sd = self.dstd.detach().cpu().numpy().astype(npdt).reshape(self.ntypes, self.nnei, 4)
+        pad = np.zeros((self.ntypes, self.ntypes, 4), dtype=npdt)
         for ti in range(self.ntypes):
             for t in range(self.ntypes):
                 a, d = av[ti, self.sec[t]:self.sec[t + 1]], sd[ti, self.sec[t]:self.sec[t + 1]]
@@ -383,14 +388,17 @@ class SeAEnergyModel:
                     continue
                 if not (np.all(a == a[0]) and np.all(d == d[0])):
                     return None
-                pad[ti, t] = (np.float32(0.0) - a[0]) / d[0]
+                pad[ti, t] = (npdt(0.0) - a[0]) / d[0]
         return torch.from_numpy(pad).to(self.device)
 
     def _pad_args(self, nlist, atype):
         """(counts, atype, pad_em) of the padding shortcut for this list, or None."""
         if self.pad_em is None:
             return None
-        return self.fused.section_counts(nlist), atype.to(torch.int32).contiguous(), self.pad_em
+        counts = torch.empty((nlist.shape[0], self.ntypes), dtype=torch.int16, device=nlist.device)
+        ops._launch("dpmi_se_a_section_counts", _lib.ptr(counts), _lib.ptr(nlist.contiguous()), nlist.shape[0],
+                    _lib.ptr(self._sel_i32), self.ntypes, ops._stream())
+        return counts, atype.to(torch.int32).contiguous(), self.pad_em
 
     def _type_order(self, atype_in, atype):
         """(perm, type_start) of the fused fitting kernel, reused while the

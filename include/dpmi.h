@@ -291,13 +291,14 @@ int dpmi_se_a_embed_bwd_bf16(float* net_deriv, const float* em, const float* dgr
  * [ntypes, 32, 64], w3 [ntypes, 64, 128] in the kernels' internal feature
  * order, consts [ntypes, 480].  gr_part [ntypes, nloc, 4, 4 n1] (original
  * feature order); dgr [nloc, 4, 4 n1] = dE / d(sum over types of gr_part),
- * the same for every type; net_deriv [nloc, nnei, 4]. */
+ * the same for every type; net_deriv [nloc, nnei, 4].  counts / atype /
+ * pad_em (fp64): the padding shortcut of the bf16 kernels above. */
 int dpmi_se_a_embed_fwd_f64(double* gr_part, const double* em, const double* w2, const double* w3,
                             const double* consts, int64_t nloc, const int* sel_host, int ntypes, int n1,
-                            void* stream);
+                            const int16_t* counts, const int* atype, const double* pad_em, void* stream);
 int dpmi_se_a_embed_bwd_f64(double* net_deriv, const double* em, const double* dgr, const double* w2,
                             const double* w3, const double* consts, int64_t nloc, const int* sel_host, int ntypes,
-                            int n1, void* stream);
+                            int n1, const int16_t* counts, void* stream);
 
 /* ------------------------------------------------------------------------
  * Fused energy fitting net (bf16 MFMA operands, fp32 accumulation).

@@ -151,6 +151,36 @@ def test_padding_shortcut_matches_full_rows(setup, system):
     assert (nd1[~valid] == 0).all()
 
 
+def test_padding_shortcut_fp64(setup):
+    """The FP64 embedding kernels' padding shortcut: G^T R to 1e-13 of the
+    all-slot evaluation, dE/d em of the listed slots identical, padding
+    slots 0."""
+    from dpmi.model import SeAEnergyModel
+
+    spec, params, m64, mbf, s, c, t = setup
+    assert m64.fused64 is not None and m64.pad_em is not None
+    _say("padding shortcut fp64 water12k")
+    s = systems.water12k(1)
+    c = torch.tensor(s.coord, device=DEV)
+    at = torch.tensor(s.atype, device=DEV).to(torch.int32)
+    ext_c, ext_t, mapping, nlist, _, _, cells = m64.neighbor_list(c, at, s.box)
+    em = m64.env_mat(nlist, ext_c, ext_t)
+    pad = m64._pad_args(nlist, at)
+    g1 = m64.fused64.forward(em, pad)
+    g0 = m64.fused64.forward(em, None)
+    gerr = ((g1 - g0).abs().max() / g0.abs().max()).item()
+    gen = torch.Generator(device="cpu").manual_seed(6)
+    dgr = torch.randn(s.nloc, 4, m64.fused64.M, generator=gen, dtype=torch.float64).to(DEV)
+    nd1 = m64.fused64.backward(em, dgr, pad[0])
+    nd0 = m64.fused64.backward(em, dgr, None)
+    torch.cuda.synchronize()
+    valid = nlist >= 0
+    print(f"fp64 G^T R max rel {gerr:.2e}", flush=True)
+    assert gerr <= 1e-13
+    assert torch.equal(nd1[valid], nd0[valid])
+    assert (nd1[~valid] == 0).all()
+
+
 def test_type_order_cache_follows_atype(setup):
     """The model reuses the type ordering of the fitting kernel while the
     same atype tensor comes back unchanged: an in-place change of the types
