@@ -98,14 +98,16 @@ def _embed_flops_per_row(neuron) -> float:
     return 2.0 * (n1 * n2 + n2 * n3)
 
 
-def _mfma_flops(name: str, spec, nloc: int) -> float:
+def _mfma_flops(name: str, spec, nloc: int, nrows: int) -> float:
     """Per-launch algorithmic FLOPs of the fused MFMA kernels.
     Embedding forward: layer GEMMs + em^T G (2*4*n3 per neighbor row);
     embedding backward: the recomputed forward, the two transposed GEMMs of
     the adjoint, dG = em dGR and the direct term G dGR^T (2 * 2*4*n3 per row).
+    ``nrows`` counts the rows the algorithm needs: the listed neighbors when
+    the padding rows (all equal within a type section) are folded into one
+    closed-form term, else every slot.
     Fitting: forward 2*(k1*H + 2*H*H + H) and adjoint 2*(2*H*H + H*k1) per
     atom (k1 = n3 * axis, H = hidden width)."""
-    nrows = nloc * int(sum(spec.sel))
     n3 = spec.neuron[2]
     f = _embed_flops_per_row(spec.neuron)
     if name.startswith("dpmi_se_a_embed_fwd"):
@@ -209,10 +211,25 @@ def main() -> int:
                     "force_rmse_relative": rmse / frms if frms else None,
                     "energy_per_atom_abs_eV": de_atom, "virial_max_rel": dv, "force_rmse_ok": rmse <= 1e-3}
 
+    # rows and 16-row tiles the embedding kernels evaluate: listed neighbors
+    # per (atom, type section) when the padding shortcut is on, else all slots
+    nlist = mlo.neighbor_list(coord, atype.to(torch.int32), sysm.box)[3]
+    if mlo.pad_em is not None:
+        offs = [0]
+        for s_t in spec.sel:
+            offs.append(offs[-1] + s_t)
+        listed = [(nlist[:, offs[t]:offs[t + 1]] >= 0).sum(1) for t in range(len(spec.sel))]
+        nrows = int(sum(c.sum().item() for c in listed))
+        tiles = int(sum(((c + 15) // 16).sum().item() for c in listed))
+    else:
+        nrows = nloc * nnei
+        tiles = nloc * sum((s_t + 15) // 16 for s_t in spec.sel)
+    del nlist
+
     # dominant libdpmi kernel of the low-precision step and its roofline
     dom = max(klo, key=lambda n: klo[n]["total_ms"])
     avg_s = klo[dom]["avg_ms"] / 1e3
-    flops = _mfma_flops(dom, spec, nloc)
+    flops = _mfma_flops(dom, spec, nloc, nrows)
     traffic, traffic_src = _measured_traffic(args.workload, dom)
     if flops > 0:
         achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
@@ -220,12 +237,13 @@ def main() -> int:
         roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 2), "peak": peak,
                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
                     "traffic_source": traffic_src,
-                    "algorithmic_flops_per_launch": flops, "avg_launch_ms": round(klo[dom]["avg_ms"], 4)}
+                    "algorithmic_flops_per_launch": flops, "embedding_rows_per_launch": nrows,
+                    "embedding_tiles_per_launch": tiles,
+                    "avg_launch_ms": round(klo[dom]["avg_ms"], 4)}
         # what actually binds the embedding kernels: VALU issue (tanh = two
         # quarter-rate transcendentals per element), from the counted mix
         cyc, cyc_src = _issue_bound(args.workload, dom)
         if cyc is not None and avg_s > 0:
-            tiles = nloc * sum((s + 15) // 16 for s in spec.sel)
             roofline["valu_issue"] = {"frac": round(cyc * tiles / (avg_s * 2.4e9 * 1024), 4),
                                       "cycles_per_tile_bound": cyc, "tiles_per_launch": tiles,
                                       "source": cyc_src}

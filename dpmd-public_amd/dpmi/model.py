@@ -374,9 +374,10 @@ class SeAEnergyModel:
     def _padding_rows(self):
         """pad_em [ntypes (center), ntypes (neighbor), 4] on the device, in
         the env-mat's dtype: the em row of every padding slot of a section,
-        (0 - davg) / dstd as the env-mat computes it (env_mat.cc keeps
-        -davg/dstd in empty slots); None when davg or dstd varies within a
-        section."""
+        (0 - davg) / dstd as the env-mat computes it (env_mat_a_cpu leaves
+        an empty slot's row at 0, source/lib/src/env_mat.cc:149-160, and the
+        normalization of prod_env_mat.cc:85-88 applies to every slot); None
+        when davg or dstd varies within a section."""
         npdt = np.float32 if self.dtype == torch.float32 else np.float64
         av = self.davg.detach().cpu().numpy().astype(npdt).reshape(self.ntypes, self.nnei, 4)
         sd = self.dstd.detach().cpu().numpy().astype(npdt).reshape(self.ntypes, self.nnei, 4)

@@ -254,7 +254,7 @@ int dpmi_tabulate_fusion_se_a_grad_grad_f32(float* dz_dy, const float* table, co
  * ------------------------------------------------------------------------ */
 /* counts [nloc, ntypes] int16 = listed neighbors of each atom per type
  * section of nlist [nloc, sum(sel)] (entries before the section's first -1,
- * where env_mat_a_cpu stops, source/lib/src/env_mat.cc:139-140). */
+ * where env_mat_a_cpu stops, source/lib/src/env_mat.cc:158-160). */
 int dpmi_se_a_section_counts(int16_t* counts, const int* nlist, int64_t nloc, const int* sel_host, int ntypes,
                              void* stream);
 /* gr_part [ntypes, nloc, 4, 4 n1]: sum over each type's neighbors of em^T G
