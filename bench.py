@@ -54,9 +54,12 @@ def _bytes_per_atom(name: str, nnei: int) -> float:
 
 HBM_PEAK_GBS = 8000.0
 # entry point -> kernel it launches (for the measured-traffic table)
-KERNEL_OF = {"dpmi_se_a_embed_bwd_bf16": "k_embed_bwd<skip>", "dpmi_se_a_embed_fwd_bf16": "k_embed_fwd<skip>",
-             "dpmi_se_a_fitting_bf16": "k_fit<false>", "dpmi_se_a_fitting_gr_bf16": "k_fit<true>",
-             "dpmi_force_virial_cells_f32": "k_force_cells<float, 1536, false, 3>"}
+KERNEL_OF = {"dpmi_force_virial_cells_f32": "k_force_cells<float, 1536, false, 3>"}
+for _h in ("bf16", "f16"):
+    KERNEL_OF.update({f"dpmi_se_a_embed_bwd_{_h}": f"k_embed_bwd<{_h}, skip>",
+                      f"dpmi_se_a_embed_fwd_{_h}": f"k_embed_fwd<{_h}, skip>",
+                      f"dpmi_se_a_fitting_{_h}": f"k_fit<{_h}, false>",
+                      f"dpmi_se_a_fitting_gr_{_h}": f"k_fit<{_h}, true>"})
 
 
 def _issue_bound(workload: str, entry: str):
@@ -188,10 +191,11 @@ def main() -> int:
 
     names64 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f64",
                "dpmi_force_virial_cells_f64", "dpmi_se_a_embed_fwd_f64", "dpmi_se_a_embed_bwd_f64"]
+    h = {"bf16": "bf16", "fp16": "f16"}[args.precision]
     names32 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f32c64",
-               "dpmi_force_virial_cells_f32", "dpmi_se_a_embed_fwd_bf16", "dpmi_se_a_descriptor_fwd",
-               "dpmi_se_a_descriptor_bwd", "dpmi_se_a_embed_bwd_bf16", "dpmi_se_a_fitting_bf16",
-               "dpmi_se_a_fitting_gr_bf16"]
+               "dpmi_force_virial_cells_f32", f"dpmi_se_a_embed_fwd_{h}", "dpmi_se_a_descriptor_fwd",
+               "dpmi_se_a_descriptor_bwd", f"dpmi_se_a_embed_bwd_{h}", f"dpmi_se_a_fitting_{h}",
+               f"dpmi_se_a_fitting_gr_{h}"]
     t64, out64, k64 = None, None, {}
     if not args.no_fp64:
         m64 = SeAEnergyModel.from_params(spec, params, device=dev, precision="fp64")

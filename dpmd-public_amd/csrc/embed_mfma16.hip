@@ -1,6 +1,8 @@
-// Fused se_e2_a embedding net + G^T R on gfx950 MFMA (bf16 operands, fp32
-// accumulation), its recompute-based adjoint, and the (G^T R)(G^T R<)^T
-// descriptor with its adjoint.
+// Fused se_e2_a embedding net + G^T R on gfx950 MFMA (bf16 or fp16 operands,
+// fp32 accumulation), its recompute-based adjoint, and the (G^T R)(G^T R<)^T
+// descriptor with its adjoint.  The kernels are templates on the 16-bit
+// operand type H (__bf16 or _Float16); both MFMA forms have the same lane
+// layout.
 //
 // Reference semantics: DescrptBlockSeA.forward (deepmd/pt/model/descriptor/
 // se_a.py:802-858) with the embedding layers of MLPLayer.forward
@@ -13,7 +15,7 @@
 //
 // MI355X mapping.  One 64-lane wavefront owns one atom's section of one
 // neighbor type (four atoms of the same type per workgroup); rows are
-// processed 16 at a time.  With v_mfma_f32_16x16x32_bf16 (A: lane -> M row
+// processed 16 at a time.  With v_mfma_f32_16x16x32_bf16 / _f16 (A: lane -> M row
 // lane%16, K 8*(lane/16)+u; B: lane -> N col lane%16, same K; D: lane -> col
 // lane%16, rows 4*(lane/16)+v) each layer is computed transposed,
 // y^T = W^T x^T, so the neighbor row is the N (lane) index everywhere and the
@@ -34,6 +36,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "dpmi_internal.h"
 #include "embed_layout.h"
@@ -41,7 +44,8 @@
 namespace dpmi {
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+template <typename H>
+using h8 = H __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxTypes = 16;
@@ -50,7 +54,6 @@ constexpr int kFragBwd = 20;   // 16 (dh2) + 4 (dh1)
 constexpr int kConsts = 704;   // k w1, k b1, idt1 [32]; k b2, idt2 [64]; k b3, idt3 [128]; w1 [32];
                                // idt2 / k [64]; 4 idt3 / k [128]   (k = 2 log2 e)
 constexpr int kQ3 = 128;       // internal width of G
-constexpr int kGStride = 40;   // LDS row stride (bf16) of the transposed G tile
 constexpr int kAtoms = 4;      // atoms (wavefronts) per workgroup
 
 struct EmbedParams {
@@ -73,8 +76,11 @@ __device__ inline float tanh_scaled(float z) {
   return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(z));
 }
 
-__device__ inline f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+__device__ inline f32x4 mfma(const h8<__bf16>& a, const h8<__bf16>& b, const f32x4& c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ inline f32x4 mfma(const h8<_Float16>& a, const h8<_Float16>& b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
 // Layer-1 feature owned by lane group g at B-slot u (also the K order phi).
@@ -85,30 +91,31 @@ __device__ inline int phi(int g, int u) { return 16 * (u >> 2) + 4 * g + (u & 3)
 // tanh = 1 - 2 r3 the embedding output is G = idt3 (1 - 2 r3) + [h2, h2], and
 // the kernels contract r3 and h2 separately (G^T R = idt3 (sum_rows em)
 // - 2 idt3 (em^T r3) + em^T [h2, h2]), so G itself is never formed.
+template <typename H>
 struct TileFwd {
   float t1[8], h1[8];
   float t2[16];
-  bf16x8 b2[2];  // h2 in bf16: layer 3's B operand (features 32 kc + phi(g, u))
+  h8<H> b2[2];  // h2 in H: layer 3's B operand (features 32 kc + phi(g, u))
   float r3[32];  // slot mt * 4 + v: internal feature 16 mt + 4 g + v
 };
 
-template <bool kSkip>
-__device__ inline void forward_tile(float s, const bf16x8* __restrict__ frag, const float* __restrict__ cst, int g,
-                                    int lane, TileFwd& f) {
-  bf16x8 b1;
+template <typename H, bool kSkip>
+__device__ inline void forward_tile(float s, const h8<H>* __restrict__ frag, const float* __restrict__ cst, int g,
+                                    int lane, TileFwd<H>& f) {
+  h8<H> b1;
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
     if (kSkip && u == 7) {  // empty slot in every lane group
       f.t1[u] = 0.f;
       f.h1[u] = 0.f;
-      b1[u] = (__bf16)0.f;
+      b1[u] = (H)0.f;
       continue;
     }
     int q = phi(g, u);
     float t = tanh_scaled(s * cst[q] + cst[32 + q]);
     f.t1[u] = t;
     f.h1[u] = t * cst[64 + q];
-    b1[u] = (__bf16)f.h1[u];
+    b1[u] = (H)f.h1[u];
   }
   f32x4 z2[4];
 #pragma unroll
@@ -121,12 +128,12 @@ __device__ inline void forward_tile(float s, const bf16x8* __restrict__ frag, co
     for (int v = 0; v < 4; ++v) {
       if (kSkip && (mt & 1) && v == 3) {
         f.t2[mt * 4 + v] = 0.f;
-        f.b2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)0.f;
+        f.b2[mt >> 1][(mt & 1) * 4 + v] = (H)0.f;
         continue;
       }
       float t = tanh_scaled(z2[mt][v]);
       f.t2[mt * 4 + v] = t;
-      f.b2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(t * cst[160 + q + v] + f.h1[(mt & 1) * 4 + v]);
+      f.b2[mt >> 1][(mt & 1) * 4 + v] = (H)(t * cst[160 + q + v] + f.h1[(mt & 1) * 4 + v]);
     }
   }
 #pragma unroll
@@ -141,10 +148,11 @@ __device__ inline void forward_tile(float s, const bf16x8* __restrict__ frag, co
 }
 
 // r3 of the 32-feature chunk kc as an MFMA operand (K = feature slot phi(g, u))
-__device__ inline bf16x8 r3_frag(const TileFwd& f, int kc) {
-  bf16x8 b;
+template <typename H>
+__device__ inline h8<H> r3_frag(const TileFwd<H>& f, int kc) {
+  h8<H> b;
 #pragma unroll
-  for (int u = 0; u < 8; ++u) b[u] = (__bf16)f.r3[(2 * kc + (u >> 2)) * 4 + (u & 3)];
+  for (int u = 0; u < 8; ++u) b[u] = (H)f.r3[(2 * kc + (u >> 2)) * 4 + (u & 3)];
   return b;
 }
 
@@ -154,20 +162,20 @@ __device__ inline bf16x8 r3_frag(const TileFwd& f, int kc) {
 // four wavefronts walk atoms independently (no block barrier after set-up).
 // G^T R contracts over neighbor rows while r3 and h2 come out of the layer
 // products with the row as the lane index, so each 16-row tile is transposed
-// by MFMA against an identity fragment (exact for the bf16 operand): then the
+// by MFMA against an identity fragment (exact for a 16-bit operand): then the
 // lane index is the feature and every lane group holds four rows, and two
 // tiles make the 32-row K operand (row order psi(g, u) = 4g + u for u < 4,
 // 16 + 4g + u - 4 otherwise) of the products em^T r3, em^T h2 and em^T 1.
-template <bool kSkip>
+template <typename H, bool kSkip>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
-    k_embed_fwd(float* __restrict__ gr_part, const float* __restrict__ em, const bf16x8* __restrict__ frag_all,
+    k_embed_fwd(float* __restrict__ gr_part, const float* __restrict__ em, const h8<H>* __restrict__ frag_all,
                 const float* __restrict__ consts, int nloc, EmbedParams p) {
-  __shared__ bf16x8 s_frag[kFragFwd * 64];
+  __shared__ h8<H> s_frag[kFragFwd * 64];
   __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
   __shared__ __attribute__((aligned(16))) float s_em[kAtoms][32 * 4];
   const int t = blockIdx.y;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-  const bf16x8* frag_t = frag_all + (int64_t)t * (kFragFwd + kFragBwd) * 64;
+  const h8<H>* frag_t = frag_all + (int64_t)t * (kFragFwd + kFragBwd) * 64;
   for (int k = threadIdx.x; k < kFragFwd * 64; k += blockDim.x) s_frag[k] = frag_t[k];
   for (int k = threadIdx.x; k < kConsts; k += blockDim.x) s_cst[k] = consts[(int64_t)t * kConsts + k];
   const float* cst = s_cst;
@@ -178,19 +186,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   // only padding slots, all with the same em row (its center type's
   // -davg/dstd); they are not evaluated, their G^T R share is
   // n_pad em_pad^T G_pad with G_pad evaluated once per center type here
-  // (with the bf16 roundings the tile path applies).
+  // (with the H roundings the tile path applies).
   __shared__ float s_gpad[kMaxTypes][kQ3];
   if (p.counts != nullptr) {
     for (int ti = w; ti < p.ntypes; ti += kAtoms) {
-      TileFwd f;
-      forward_tile<kSkip>(p.pad_em[(ti * p.ntypes + t) * 4], s_frag, cst, g, lane, f);
+      TileFwd<H> f;
+      forward_tile<H, kSkip>(p.pad_em[(ti * p.ntypes + t) * 4], s_frag, cst, g, lane, f);
       if (r == 0)
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int q = 16 * mt + 4 * g + v;
-            s_gpad[ti][q] = cst[352 + q] * (1.f - 2.f * (float)(__bf16)f.r3[mt * 4 + v]) +
+            s_gpad[ti][q] = cst[352 + q] * (1.f - 2.f * (float)(H)f.r3[mt * 4 + v]) +
                             (float)f.b2[(mt & 3) >> 1][((mt & 3) & 1) * 4 + v];
           }
     }
@@ -203,14 +211,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     return (p.counts[(int64_t)ai * p.ntypes + t] + 15) >> 4;
   };
   // identity fragments: B (K = phi slot of a 32-chunk, N = n), one per 16-wide half
-  bf16x8 eye[2];
+  h8<H> eye[2];
 #pragma unroll
   for (int j = 0; j < 2; ++j)
 #pragma unroll
-    for (int u = 0; u < 8; ++u) eye[j][u] = (__bf16)(((u >> 2) == j && 4 * g + (u & 3) == r) ? 1.f : 0.f);
-  bf16x8 ones;
+    for (int u = 0; u < 8; ++u) eye[j][u] = (H)(((u >> 2) == j && 4 * g + (u & 3) == r) ? 1.f : 0.f);
+  h8<H> ones;
 #pragma unroll
-  for (int u = 0; u < 8; ++u) ones[u] = (__bf16)1.f;
+  for (int u = 0; u < 8; ++u) ones[u] = (H)1.f;
   float* sem = s_em[w];
   const int stride = gridDim.x * kAtoms;
   auto load_em = [&](int ai, int row) {
@@ -243,7 +251,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       // weights and constants stay in LDS (re-read per chunk), not in registers
       asm volatile("" ::: "memory");
       // transposed tiles: [tile][pair of 16-feature tiles], 4 rows each
-      bf16x8 rt3[2][4], ht2[2][2];
+      h8<H> rt3[2][4], ht2[2][2];
 #pragma unroll
       for (int rt = 0; rt < 2; ++rt) {
         const float4 ev = ev_cur[rt];
@@ -256,23 +264,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
           for (int kc = 0; kc < 4; ++kc)
 #pragma unroll
-            for (int u = 0; u < 8; ++u) rt3[rt][kc][u] = (__bf16)0.f;
+            for (int u = 0; u < 8; ++u) rt3[rt][kc][u] = (H)0.f;
 #pragma unroll
           for (int kc = 0; kc < 2; ++kc)
 #pragma unroll
-            for (int u = 0; u < 8; ++u) ht2[rt][kc][u] = (__bf16)0.f;
+            for (int u = 0; u < 8; ++u) ht2[rt][kc][u] = (H)0.f;
           continue;
         }
-        TileFwd f;
-        forward_tile<kSkip>(ev.x, s_frag, cst, g, lane, f);
+        TileFwd<H> f;
+        forward_tile<H, kSkip>(ev.x, s_frag, cst, g, lane, f);
 #pragma unroll
         for (int kc = 0; kc < 4; ++kc) {
-          const bf16x8 rb = r3_frag(f, kc);
+          const h8<H> rb = r3_frag(f, kc);
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
             const f32x4 d = mfma(rb, eye[j], f32x4{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
-            for (int v = 0; v < 4; ++v) rt3[rt][kc][4 * j + v] = (__bf16)d[v];
+            for (int v = 0; v < 4; ++v) rt3[rt][kc][4 * j + v] = (H)d[v];
           }
         }
 #pragma unroll
@@ -281,22 +289,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
           for (int j = 0; j < 2; ++j) {
             const f32x4 d = mfma(f.b2[kc], eye[j], f32x4{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
-            for (int v = 0; v < 4; ++v) ht2[rt][kc][4 * j + v] = (__bf16)d[v];
+            for (int v = 0; v < 4; ++v) ht2[rt][kc][4 * j + v] = (H)d[v];
           }
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
       // em^T as A (M = c, K = rows in psi order); rows M >= 4 of the products
       // are never read, so their A values are left as they come
-      bf16x8 a;
+      h8<H> a;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int row = u < 4 ? 4 * g + u : 16 + 4 * g + (u - 4);
-        a[u] = (__bf16)sem[4 * row + (r & 3)];
+        a[u] = (H)sem[4 * row + (r & 3)];
       }
 #pragma unroll
       for (int nt = 0; nt < 8; ++nt) {
-        bf16x8 b;
+        h8<H> b;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           b[v] = rt3[0][nt >> 1][4 * (nt & 1) + v];
@@ -306,7 +314,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       }
 #pragma unroll
       for (int nt = 0; nt < 4; ++nt) {
-        bf16x8 b;
+        h8<H> b;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           b[v] = ht2[0][nt >> 1][4 * (nt & 1) + v];
@@ -331,7 +339,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       const int n_pad = sel - min(sel, 16 * nti);
       if (n_pad > 0 && ti >= 0 && ti < p.ntypes)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) padc[v] = (float)n_pad * (float)(__bf16)p.pad_em[(ti * p.ntypes + t) * 4 + v];
+        for (int v = 0; v < 4; ++v) padc[v] = (float)n_pad * (float)(H)p.pad_em[(ti * p.ntypes + t) * 4 + v];
       const int tp = ti >= 0 && ti < p.ntypes ? ti : 0;
 #pragma unroll
       for (int nt = 0; nt < 8; ++nt) {
@@ -506,32 +514,32 @@ __global__ void __launch_bounds__(256)
 // Backward: net_deriv[i][sec_t + row][c] = dE/d em, recomputing the forward.
 // Persistent grid (weight fragments loaded into LDS once per workgroup);
 // each wavefront keeps dGR of its atom in LDS twice: as float [q3][c] for the
-// per-atom operands, and as 16-byte bf16 rows [q3] = (dGR[0..3][q3], 0 x 4),
+// per-atom operands, and as 16-byte H rows [q3] = (dGR[0..3][q3], 0 x 4),
 // the A operand of dG = em dGR read with one 16-byte load per 16 features.
-template <bool kSkip>
+template <typename H, bool kSkip>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_embed_bwd(float* __restrict__ net_deriv, const float* __restrict__ em, const float* __restrict__ dgr,
-                const bf16x8* __restrict__ frag_all, const float* __restrict__ consts, int nloc, EmbedParams p) {
-  __shared__ bf16x8 s_frag[(kFragFwd + kFragBwd) * 64];
+                const h8<H>* __restrict__ frag_all, const float* __restrict__ consts, int nloc, EmbedParams p) {
+  __shared__ h8<H> s_frag[(kFragFwd + kFragBwd) * 64];
   __shared__ __attribute__((aligned(16))) float s_dg[kAtoms][kQ3 * 4];
-  __shared__ bf16x8 s_dgb[kAtoms][kQ3];
-  __shared__ bf16x8 s_dop[kAtoms][6 * 16];
+  __shared__ h8<H> s_dgb[kAtoms][kQ3];
+  __shared__ h8<H> s_dop[kAtoms][6 * 16];
   __shared__ __attribute__((aligned(16))) float s_kin[kAtoms][4];
   __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
   const int t = blockIdx.y;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-  const bf16x8* frag_t = frag_all + (int64_t)t * (kFragFwd + kFragBwd) * 64;
+  const h8<H>* frag_t = frag_all + (int64_t)t * (kFragFwd + kFragBwd) * 64;
   for (int k = threadIdx.x; k < (kFragFwd + kFragBwd) * 64; k += blockDim.x) s_frag[k] = frag_t[k];
   __syncthreads();
-  const bf16x8* fb = s_frag + kFragFwd * 64;
+  const h8<H>* fb = s_frag + kFragFwd * 64;
   for (int k = threadIdx.x; k < kConsts; k += blockDim.x) s_cst[k] = consts[(int64_t)t * kConsts + k];
   __syncthreads();
   const float* cst = s_cst;
   const int sel = p.sec[t + 1] - p.sec[t];
   const int ntile = (sel + 15) / 16;
   float* sdg = s_dg[w];
-  bf16x8* sdgb = s_dgb[w];
-  bf16x8* sdop = s_dop[w];
+  h8<H>* sdgb = s_dgb[w];
+  h8<H>* sdop = s_dop[w];
   float* skin = s_kin[w];
   const int stride = gridDim.x * kAtoms;
   // em rows are prefetched one tile ahead (across atoms too) so the HBM
@@ -555,13 +563,32 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     nv_next = listed(inext);
     if (nti == 0) ev_next = load_em(inext, 0);
     // stage dGR [q3][c] of the atom: sdg[q3 * 4 + c]
+    // fp16 operands: everything below is linear in dGR, so the atom's dGR is
+    // staged times 2^-e (2^e the power of two at or below its largest
+    // magnitude) and dE/d em multiplied by 2^e at the end, both exact; the
+    // dz operands then sit in fp16's normal range instead of underflowing
+    float unscale = 1.f;
     {
       // (the lane index passes through an opaque copy: hoisted address
       // arithmetic here made the allocator spill inside the atom loop)
       const float4* dg = reinterpret_cast<const float4*>(dgr + (int64_t)i * 4 * kQ3);
       int l = lane;
       asm volatile("" : "+v"(l));
-      const float4 x0 = dg[l], x1 = dg[l + 64];
+      float4 x0 = dg[l], x1 = dg[l + 64];
+      if constexpr (std::is_same<H, _Float16>::value) {
+        float mx = fmaxf(fmaxf(fmaxf(fabsf(x0.x), fabsf(x0.y)), fmaxf(fabsf(x0.z), fabsf(x0.w))),
+                         fmaxf(fmaxf(fabsf(x1.x), fabsf(x1.y)), fmaxf(fabsf(x1.z), fabsf(x1.w))));
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+        // exponent field of the maximum (0 for zero and subnormals), kept in
+        // [27, 227] so that both powers of two below are normal floats
+        int ef = (int)((__float_as_uint(mx) >> 23) & 255u);
+        ef = __builtin_amdgcn_readfirstlane(min(max(ef, 27), 227));
+        const float sc = __uint_as_float((uint32_t)(254 - ef) << 23);  // 2^-e
+        unscale = __uint_as_float((uint32_t)ef << 23);                  // 2^e
+        x0 = make_float4(x0.x * sc, x0.y * sc, x0.z * sc, x0.w * sc);
+        x1 = make_float4(x1.x * sc, x1.y * sc, x1.z * sc, x1.w * sc);
+      }
       reinterpret_cast<float4*>(sdg)[l] = x0;
       reinterpret_cast<float4*>(sdg)[l + 64] = x1;
     }
@@ -569,13 +596,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     __builtin_amdgcn_wave_barrier();
     for (int q = lane; q < kQ3; q += 64) {
       const float4 x = *reinterpret_cast<const float4*>(sdg + 4 * q);
-      bf16x8 v;
-      v[0] = (__bf16)x.x;
-      v[1] = (__bf16)x.y;
-      v[2] = (__bf16)x.z;
-      v[3] = (__bf16)x.w;
+      h8<H> v;
+      v[0] = (H)x.x;
+      v[1] = (H)x.y;
+      v[2] = (H)x.z;
+      v[3] = (H)x.w;
 #pragma unroll
-      for (int u = 4; u < 8; ++u) v[u] = (__bf16)0.f;
+      for (int u = 4; u < 8; ++u) v[u] = (H)0.f;
       sdgb[q] = v;
     }
     // Operands of the direct term dem[c] = sum_q3 G[q3] dGR[c][q3] with
@@ -586,12 +613,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     // chunks, 4..5 the h2 chunks, read per tile (registers are scarce here).
     for (int k = lane; k < 96; k += 64) {
       const int fi = k >> 4, gg = (k >> 2) & 3, cc = k & 3;
-      bf16x8 v;
+      h8<H> v;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int q = 32 * (fi & 3) + phi(gg, u);
-        v[u] = fi < 4 ? (__bf16)(-2.f * cst[352 + q] * sdg[q * 4 + cc])
-                      : (__bf16)(sdg[q * 4 + cc] + sdg[(q + 64) * 4 + cc]);
+        v[u] = fi < 4 ? (H)(-2.f * cst[352 + q] * sdg[q * 4 + cc])
+                      : (H)(sdg[q * 4 + cc] + sdg[(q + 64) * 4 + cc]);
       }
       sdop[k] = v;
     }
@@ -618,34 +645,34 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       // held in registers across the loop: that keeps the kernel at two
       // wavefronts per SIMD
       asm volatile("" ::: "memory");
-      TileFwd f;
-      forward_tile<kSkip>(ev.x, s_frag, cst, g, lane, f);
+      TileFwd<H> f;
+      forward_tile<H, kSkip>(ev.x, s_frag, cst, g, lane, f);
       // direct term -> D(M = c, N = rows)
-      const bf16x8* dop = sdop + 4 * g + (r & 3);
+      const h8<H>* dop = sdop + 4 * g + (r & 3);
       f32x4 dem = *reinterpret_cast<const f32x4*>(skin);
 #pragma unroll
       for (int kc = 0; kc < 4; ++kc) dem = mfma(dop[16 * kc], r3_frag(f, kc), dem);
 #pragma unroll
       for (int kc = 0; kc < 2; ++kc) dem = mfma(dop[16 * (4 + kc)], f.b2[kc], dem);
       // em as B (N = rows, K = c): K slots 0..3 in lane group 0, all others zero
-      bf16x8 b_em;
-      b_em[0] = (__bf16)(g == 0 ? ev.x : 0.f);
-      b_em[1] = (__bf16)(g == 0 ? ev.y : 0.f);
-      b_em[2] = (__bf16)(g == 0 ? ev.z : 0.f);
-      b_em[3] = (__bf16)(g == 0 ? ev.w : 0.f);
+      h8<H> b_em;
+      b_em[0] = (H)(g == 0 ? ev.x : 0.f);
+      b_em[1] = (H)(g == 0 ? ev.y : 0.f);
+      b_em[2] = (H)(g == 0 ? ev.z : 0.f);
+      b_em[3] = (H)(g == 0 ? ev.w : 0.f);
 #pragma unroll
-      for (int u = 4; u < 8; ++u) b_em[u] = (__bf16)0.f;
+      for (int u = 4; u < 8; ++u) b_em[u] = (H)0.f;
       // dG = em dGR -> D(M = q3, N = rows), A = row q3 of sdgb (its K slots
       // 4..7 are zero; those of lane groups 1..3 meet the zero K slots of b_em);
       // dz3 = dG idt3 (1 - t3^2) = dG 4 idt3 r3 (1 - r3); skip = dG[q] + dG[q + 64]
       f32x4 skip[4];
-      bf16x8 b_dz3[4];
+      h8<H> b_dz3[4];
       // keep the sdgb reads below the recompute (hoisted, they would hold 32
       // registers across it)
       asm volatile("" ::: "memory");
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt) {
-        const bf16x8 a_lo = sdgb[16 * mt + r], a_hi = sdgb[16 * (mt + 4) + r];
+        const h8<H> a_lo = sdgb[16 * mt + r], a_hi = sdgb[16 * (mt + 4) + r];
         const f32x4 d_lo = mfma(a_lo, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
         const f32x4 d_hi = mfma(a_hi, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
         skip[mt] = mfma(a_hi, b_em, d_lo);
@@ -653,18 +680,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           if (kSkip && (mt & 1) && v == 3) {  // empty slot: no weights read it
-            b_dz3[mt >> 1][(mt & 1) * 4 + v] = (__bf16)0.f;
-            b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (__bf16)0.f;
+            b_dz3[mt >> 1][(mt & 1) * 4 + v] = (H)0.f;
+            b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (H)0.f;
             continue;
           }
           const float rl = f.r3[mt * 4 + v], rh = f.r3[(mt + 4) * 4 + v];
-          b_dz3[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(d_lo[v] * cst[576 + q + v] * (rl - rl * rl));
-          b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (__bf16)(d_hi[v] * cst[640 + q + v] * (rh - rh * rh));
+          b_dz3[mt >> 1][(mt & 1) * 4 + v] = (H)(d_lo[v] * cst[576 + q + v] * (rl - rl * rl));
+          b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (H)(d_hi[v] * cst[640 + q + v] * (rh - rh * rh));
         }
       }
       // dh2 = dz3 W3^T + skip;  dz2 = dh2 * idt2 * (1 - t2^2)
       float dh2[16];
-      bf16x8 b_dz2[2];
+      h8<H> b_dz2[2];
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt) {
         f32x4 d = skip[mt];
@@ -676,11 +703,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
           float x = d[v];
           dh2[mt * 4 + v] = x;
           if (kSkip && (mt & 1) && v == 3) {
-            b_dz2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)0.f;
+            b_dz2[mt >> 1][(mt & 1) * 4 + v] = (H)0.f;
             continue;
           }
           float tt = f.t2[mt * 4 + v];
-          b_dz2[mt >> 1][(mt & 1) * 4 + v] = (__bf16)(x * cst[512 + q + v] * (1.f - tt * tt));
+          b_dz2[mt >> 1][(mt & 1) * 4 + v] = (H)(x * cst[512 + q + v] * (1.f - tt * tt));
         }
       }
       // dh1 = dz2 W2^T + dh2[q1] + dh2[q1 + 32];  ds = sum_f1 dh1 * idt1 * (1 - t1^2) * w1
@@ -702,7 +729,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       ds += __shfl_xor(ds, 16, 64);
       ds += __shfl_xor(ds, 32, 64);
       if (g == 0 && row < sel) {
-        float4 o = make_float4(dem[0] + ds, dem[1], dem[2], dem[3]);
+        float4 o = make_float4((dem[0] + ds) * unscale, dem[1] * unscale, dem[2] * unscale, dem[3] * unscale);
         if (row >= nrow_out) o = make_float4(0.f, 0.f, 0.f, 0.f);
         *reinterpret_cast<float4*>(nd_i + 4 * row) = o;
       }
@@ -742,14 +769,10 @@ int make_params(EmbedParams* p, const int* sel_host, int ntypes) {
   return DPMI_OK;
 }
 
-}  // namespace
-}  // namespace dpmi
-
-using namespace dpmi;
-
-extern "C" int dpmi_se_a_embed_fwd_bf16(float* gr_part, const float* em, const void* frags, const float* consts,
-                                        int64_t nloc, const int* sel_host, int ntypes, int n1, const int16_t* counts,
-                                        const int* atype, const float* pad_em, void* stream) {
+template <typename H>
+int embed_fwd(float* gr_part, const float* em, const void* frags, const float* consts, int64_t nloc,
+              const int* sel_host, int ntypes, int n1, const int16_t* counts, const int* atype, const float* pad_em,
+              void* stream) {
   EmbedParams p;
   int rc = make_params(&p, sel_host, ntypes);
   if (rc) return rc;
@@ -765,13 +788,52 @@ extern "C" int dpmi_se_a_embed_fwd_bf16(float* gr_part, const float* em, const v
   // persistent grid: enough workgroups per type to fill 256 CUs a few times over
   dim3 grid((unsigned)std::min<int64_t>(ceil_div(nloc, kAtoms), 1024), (unsigned)ntypes);
   if (skip_layout(n1))
-    hipLaunchKernelGGL(k_embed_fwd<true>, grid, dim3(256), 0, as_stream(stream), gr_part, em,
-                       static_cast<const bf16x8*>(frags), consts, (int)nloc, p);
+    hipLaunchKernelGGL((k_embed_fwd<H, true>), grid, dim3(256), 0, as_stream(stream), gr_part, em,
+                       static_cast<const h8<H>*>(frags), consts, (int)nloc, p);
   else
-    hipLaunchKernelGGL(k_embed_fwd<false>, grid, dim3(256), 0, as_stream(stream), gr_part, em,
-                       static_cast<const bf16x8*>(frags), consts, (int)nloc, p);
+    hipLaunchKernelGGL((k_embed_fwd<H, false>), grid, dim3(256), 0, as_stream(stream), gr_part, em,
+                       static_cast<const h8<H>*>(frags), consts, (int)nloc, p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
+}
+
+template <typename H>
+int embed_bwd(float* net_deriv, const float* em, const float* dgr, const void* frags, const float* consts,
+              int64_t nloc, const int* sel_host, int ntypes, int n1, const int16_t* counts, void* stream) {
+  EmbedParams p;
+  int rc = make_params(&p, sel_host, ntypes);
+  if (rc) return rc;
+  p.counts = counts;
+  DPMI_CHECK_ARG(n1 > 0 && n1 <= 32, "n1 out of range [1, 32]");
+  p.n1 = n1;
+  DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
+  if (nloc == 0) return DPMI_OK;
+  dim3 grid((unsigned)std::min<int64_t>(ceil_div(nloc, kAtoms), 1024), (unsigned)ntypes);
+  if (skip_layout(n1))
+    hipLaunchKernelGGL((k_embed_bwd<H, true>), grid, dim3(256), 0, as_stream(stream), net_deriv, em, dgr,
+                       static_cast<const h8<H>*>(frags), consts, (int)nloc, p);
+  else
+    hipLaunchKernelGGL((k_embed_bwd<H, false>), grid, dim3(256), 0, as_stream(stream), net_deriv, em, dgr,
+                       static_cast<const h8<H>*>(frags), consts, (int)nloc, p);
+  DPMI_LAUNCH_CHECK();
+  return DPMI_OK;
+}
+
+}  // namespace
+}  // namespace dpmi
+
+using namespace dpmi;
+
+extern "C" int dpmi_se_a_embed_fwd_bf16(float* gr_part, const float* em, const void* frags, const float* consts,
+                                        int64_t nloc, const int* sel_host, int ntypes, int n1, const int16_t* counts,
+                                        const int* atype, const float* pad_em, void* stream) {
+  return embed_fwd<__bf16>(gr_part, em, frags, consts, nloc, sel_host, ntypes, n1, counts, atype, pad_em, stream);
+}
+
+extern "C" int dpmi_se_a_embed_fwd_f16(float* gr_part, const float* em, const void* frags, const float* consts,
+                                       int64_t nloc, const int* sel_host, int ntypes, int n1, const int16_t* counts,
+                                       const int* atype, const float* pad_em, void* stream) {
+  return embed_fwd<_Float16>(gr_part, em, frags, consts, nloc, sel_host, ntypes, n1, counts, atype, pad_em, stream);
 }
 
 extern "C" int dpmi_se_a_section_counts(int16_t* counts, const int* nlist, int64_t nloc, const int* sel_host,
@@ -789,12 +851,16 @@ extern "C" int dpmi_se_a_section_counts(int16_t* counts, const int* nlist, int64
 }
 
 extern "C" int dpmi_se_a_descriptor_fwd(float* gr, void* desc, const float* gr_part, int64_t nloc, int ntypes,
-                                        int nnei, int n1, int axis, int desc_bf16, void* stream) {
+                                        int nnei, int n1, int axis, int desc_kind, void* stream) {
   DPMI_CHECK_ARG(n1 > 0 && n1 <= 32 && axis > 0 && axis <= 4 * n1 && axis <= 16, "unsupported widths");
+  DPMI_CHECK_ARG(desc_kind >= 0 && desc_kind <= 2, "desc_kind must be 0 (fp32), 1 (bf16) or 2 (fp16)");
   if (nloc == 0) return DPMI_OK;
-  if (desc_bf16)
+  if (desc_kind == 1)
     hipLaunchKernelGGL(k_descrpt_fwd<__bf16>, dim3(ceil_div(nloc, kAtoms)), dim3(256), 0, as_stream(stream), gr,
                        static_cast<__bf16*>(desc), gr_part, (int)nloc, ntypes, 1.f / (float)nnei, n1, axis);
+  else if (desc_kind == 2)
+    hipLaunchKernelGGL(k_descrpt_fwd<_Float16>, dim3(ceil_div(nloc, kAtoms)), dim3(256), 0, as_stream(stream), gr,
+                       static_cast<_Float16*>(desc), gr_part, (int)nloc, ntypes, 1.f / (float)nnei, n1, axis);
   else
     hipLaunchKernelGGL(k_descrpt_fwd<float>, dim3(ceil_div(nloc, kAtoms)), dim3(256), 0, as_stream(stream), gr,
                        static_cast<float*>(desc), gr_part, (int)nloc, ntypes, 1.f / (float)nnei, n1, axis);
@@ -823,21 +889,11 @@ extern "C" int dpmi_se_a_descriptor_bwd(float* dgr, const float* ddesc, const fl
 extern "C" int dpmi_se_a_embed_bwd_bf16(float* net_deriv, const float* em, const float* dgr, const void* frags,
                                         const float* consts, int64_t nloc, const int* sel_host, int ntypes, int n1,
                                         const int16_t* counts, void* stream) {
-  EmbedParams p;
-  int rc = make_params(&p, sel_host, ntypes);
-  if (rc) return rc;
-  p.counts = counts;
-  DPMI_CHECK_ARG(n1 > 0 && n1 <= 32, "n1 out of range [1, 32]");
-  p.n1 = n1;
-  DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
-  if (nloc == 0) return DPMI_OK;
-  dim3 grid((unsigned)std::min<int64_t>(ceil_div(nloc, kAtoms), 1024), (unsigned)ntypes);
-  if (skip_layout(n1))
-    hipLaunchKernelGGL(k_embed_bwd<true>, grid, dim3(256), 0, as_stream(stream), net_deriv, em, dgr,
-                       static_cast<const bf16x8*>(frags), consts, (int)nloc, p);
-  else
-    hipLaunchKernelGGL(k_embed_bwd<false>, grid, dim3(256), 0, as_stream(stream), net_deriv, em, dgr,
-                       static_cast<const bf16x8*>(frags), consts, (int)nloc, p);
-  DPMI_LAUNCH_CHECK();
-  return DPMI_OK;
+  return embed_bwd<__bf16>(net_deriv, em, dgr, frags, consts, nloc, sel_host, ntypes, n1, counts, stream);
+}
+
+extern "C" int dpmi_se_a_embed_bwd_f16(float* net_deriv, const float* em, const float* dgr, const void* frags,
+                                       const float* consts, int64_t nloc, const int* sel_host, int ntypes, int n1,
+                                       const int16_t* counts, void* stream) {
+  return embed_bwd<_Float16>(net_deriv, em, dgr, frags, consts, nloc, sel_host, ntypes, n1, counts, stream);
 }

@@ -1,5 +1,7 @@
-// Fused energy fitting net on gfx950 MFMA (bf16 operands, fp32 accumulation):
-// atomic energies and dE/dD of every local atom in one pass.
+// Fused energy fitting net on gfx950 MFMA (bf16 or fp16 operands, fp32
+// accumulation): atomic energies and dE/dD of every local atom in one pass.
+// The kernel is a template on the 16-bit operand type H (__bf16 or
+// _Float16); both MFMA forms have the same lane layout.
 //
 // Reference semantics: GeneralFitting._forward_common / EnergyFittingNet
 // (deepmd/pt/model/task/fitting.py:734-900, ener.py) with FittingNet layers
@@ -16,11 +18,11 @@
 // (wavefront w owns features 64w .. 64w+63 of the width padded to 256), so a
 // feature's tanh factor, skip value and adjoint stay in the registers of one
 // lane for the whole pass.  Layers are computed transposed, y^T = W^T x^T,
-// with v_mfma_f32_16x16x32_bf16: the weight fragment is the A operand (M =
+// with v_mfma_f32_16x16x32_bf16 / _f16: the weight fragment is the A operand (M =
 // output feature), the layer input is the B operand (K = input feature,
 // N = atom), read from an LDS buffer [atom][feature] that every layer's
 // output is written back to.  The D matrix streams through LDS in 64-wide K
-// blocks (bf16, as the descriptor kernel writes it), prefetched one block ahead.  The
+// blocks (H, as the descriptor kernel writes it), prefetched one block ahead.  The
 // backward pass runs the transposed products (dh_{l-1} = dz_l W_l^T + skip)
 // through the same LDS buffer and ends with dD = dz1 W1^T, whose B operands
 // (dz1 for all 64 atoms) stay in registers while W1's fragments stream from
@@ -36,22 +38,24 @@
 namespace dpmi {
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+template <typename H>
+using h8 = H __attribute__((ext_vector_type(8)));
+template <typename H>
+using h4 = H __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kH = 256;         // padded hidden width (4 wavefronts x 64 features)
 constexpr int kTile = 64;       // atoms per workgroup
 constexpr int kKB = 64;         // layer-1 input features staged per step
-constexpr int kXS = kKB + 8;    // LDS row stride (bf16) of the staged D block
-constexpr int kHS = kH + 8;     // LDS row stride (bf16) of the hidden-layer buffer
+constexpr int kXS = kKB + 8;    // LDS row stride (16-bit elements) of the staged D block
+constexpr int kHS = kH + 8;     // LDS row stride (16-bit elements) of the hidden-layer buffer
 constexpr int kMaxTypes = 16;
 constexpr int kCst = 7 * kH + 8;  // b1 idt1 b2 idt2 b3 idt3 w4 [kH each], b4
 constexpr int kMaxGR = 104;       // widest G^T R (M) the fused descriptor form stages
 constexpr int kGRS = 4 * kMaxGR + 4;  // LDS floats per atom of the staged GR ([m][c], padded)
 constexpr int kMaxTypesGR = 4;        // neighbor types whose partials the fused form sums
 
-// Fragment offsets (in 64-lane fragments of bf16x8) inside one type's block.
+// Fragment offsets (in 64-lane fragments of 8 16-bit values) inside one type's block.
 struct FitParams {
   int ntypes;
   int nloc;
@@ -70,8 +74,11 @@ __device__ inline float fast_tanh(float x) {
   return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + e);
 }
 
-__device__ inline f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+__device__ inline f32x4 mfma(const h8<__bf16>& a, const h8<__bf16>& b, const f32x4& c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ inline f32x4 mfma(const h8<_Float16>& a, const h8<_Float16>& b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
 // Sum over the four 16-lane groups, the same value in every lane: the gfx950
@@ -85,39 +92,42 @@ __device__ inline float sum_lane_groups(float x) {
 
 // B fragment of 16 atoms (rows 16s..16s+15 of an LDS [atom][stride] buffer),
 // K = features kbase + 8g .. kbase + 8g + 7.
-__device__ inline bf16x8 lds_b(const __bf16* buf, int stride, int s, int n, int kbase, int g) {
-  return *reinterpret_cast<const bf16x8*>(buf + (16 * s + n) * stride + kbase + 8 * g);
+template <typename H>
+__device__ inline h8<H> lds_b(const H* buf, int stride, int s, int n, int kbase, int g) {
+  return *reinterpret_cast<const h8<H>*>(buf + (16 * s + n) * stride + kbase + 8 * g);
 }
 
 // Writes a layer output (D layout: lane -> atom n of subtile s, features
-// 4g + v of M tile mt) to the LDS buffer as bf16.
-__device__ inline void lds_put(__bf16* buf, int w, int mt, int s, int n, int g, const float* v4) {
-  bf16x4 q;
-  q[0] = (__bf16)v4[0];
-  q[1] = (__bf16)v4[1];
-  q[2] = (__bf16)v4[2];
-  q[3] = (__bf16)v4[3];
-  *reinterpret_cast<bf16x4*>(buf + (16 * s + n) * kHS + 16 * (4 * w + mt) + 4 * g) = q;
+// 4g + v of M tile mt) to the LDS buffer as H.
+template <typename H>
+__device__ inline void lds_put(H* buf, int w, int mt, int s, int n, int g, const float* v4) {
+  h4<H> q;
+  q[0] = (H)v4[0];
+  q[1] = (H)v4[1];
+  q[2] = (H)v4[2];
+  q[3] = (H)v4[3];
+  *reinterpret_cast<h4<H>*>(buf + (16 * s + n) * kHS + 16 * (4 * w + mt) + 4 * g) = q;
 }
 
 // One hidden H -> H layer product for this wavefront's four M tiles and the
 // 64 atoms: acc[mt][s] = sum_k W(frag)[feature][k] * in[atom][k].
-__device__ inline void hidden_product(f32x4 (&acc)[4][4], const bf16x8* __restrict__ frag, const __bf16* s_h,
+template <typename H>
+__device__ inline void hidden_product(f32x4 (&acc)[4][4], const h8<H>* __restrict__ frag, const H* s_h,
                                       int w, int lane, int n, int g) {
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
     for (int s = 0; s < 4; ++s) acc[mt][s] = f32x4{0.f, 0.f, 0.f, 0.f};
-  bf16x8 a[4];
+  h8<H> a[4];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) a[mt] = frag[((4 * w + mt) * 8 + 0) * 64 + lane];
 #pragma unroll 1
   for (int kc = 0; kc < 8; ++kc) {
-    bf16x8 an[4];
+    h8<H> an[4];
     const int kn = kc + 1 < 8 ? kc + 1 : kc;
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) an[mt] = frag[((4 * w + mt) * 8 + kn) * 64 + lane];
-    bf16x8 b[4];
+    h8<H> b[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) b[s] = lds_b(s_h, kHS, s, n, 32 * kc, g);
 #pragma unroll
@@ -137,14 +147,14 @@ __device__ inline void hidden_product(f32x4 (&acc)[4][4], const bf16x8* __restri
 //   dGR[c][m] = sum_a dD[m][a] GR[c][a] + [m < 16] sum_m' dD[m'][m] GR[c][m'],
 // written / nnei in the embedding kernels' internal order as dgr
 // [nloc, 128, 4] (component fastest: one 16-byte store per atom and feature).
-template <bool kGR>
+template <typename H, bool kGR>
 __global__ void __launch_bounds__(256)
-    k_fit(double* __restrict__ atom_energy, float* __restrict__ dout, const __bf16* __restrict__ desc,
+    k_fit(double* __restrict__ atom_energy, float* __restrict__ dout, const H* __restrict__ desc,
           const float* __restrict__ gr_part, const int* __restrict__ perm, const int* __restrict__ type_start,
-          const bf16x8* __restrict__ frag_all, const float* __restrict__ consts, const double* __restrict__ ebias,
+          const h8<H>* __restrict__ frag_all, const float* __restrict__ consts, const double* __restrict__ ebias,
           FitParams p) {
-  __shared__ __attribute__((aligned(16))) __bf16 s_x[2][kTile * kXS];
-  __shared__ __attribute__((aligned(16))) __bf16 s_h[kTile * kHS];
+  __shared__ __attribute__((aligned(16))) H s_x[2][kTile * kXS];
+  __shared__ __attribute__((aligned(16))) H s_h[kTile * kHS];
   __shared__ float s_e[4][kTile];
   __shared__ int s_atom[kTile];
   __shared__ __attribute__((aligned(16))) float s_gr[kGR ? kTile * kGRS : 4];
@@ -166,50 +176,50 @@ __global__ void __launch_bounds__(256)
 
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, n = lane & 15, g = lane >> 4;
   if (tid < kTile) s_atom[tid] = tid < cnt ? perm[start + tid] : -1;
-  const bf16x8* frag = frag_all + (int64_t)t * p.type_frags;
+  const h8<H>* frag = frag_all + (int64_t)t * p.type_frags;
   const float* cst = consts + (int64_t)t * kCst;
   __syncthreads();
 
-  // ---- layer 1: D block loads (thread -> atom tid/4, features 16*(tid%4) .. +15 of the block, bf16)
+  // ---- layer 1: D block loads (thread -> atom tid/4, features 16*(tid%4) .. +15 of the block, H)
   const int xl = tid >> 2, xa = s_atom[xl], xq = tid & 3;
-  const __bf16* xrow = desc + (int64_t)(xa < 0 ? 0 : xa) * p.k1;
+  const H* xrow = desc + (int64_t)(xa < 0 ? 0 : xa) * p.k1;
   const int nkb = p.nk1 / 2;
-  auto load_block = [&](int kb, bf16x8 (&r)[2]) {
+  auto load_block = [&](int kb, h8<H> (&r)[2]) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int f = kb * kKB + 16 * xq + 8 * j;
       // streamed once: non-temporal, so it does not evict the weight fragments from L2
-      r[j] = (xa >= 0 && f < p.k1) ? __builtin_nontemporal_load(reinterpret_cast<const bf16x8*>(xrow + f))
-                                   : bf16x8{};
+      r[j] = (xa >= 0 && f < p.k1) ? __builtin_nontemporal_load(reinterpret_cast<const h8<H>*>(xrow + f))
+                                   : h8<H>{};
     }
   };
-  auto store_block = [&](int buf, const bf16x8 (&r)[2]) {
-    __bf16* dst = s_x[buf] + xl * kXS + 16 * xq;
-    *reinterpret_cast<bf16x8*>(dst) = r[0];
-    *reinterpret_cast<bf16x8*>(dst + 8) = r[1];
+  auto store_block = [&](int buf, const h8<H> (&r)[2]) {
+    H* dst = s_x[buf] + xl * kXS + 16 * xq;
+    *reinterpret_cast<h8<H>*>(dst) = r[0];
+    *reinterpret_cast<h8<H>*>(dst + 8) = r[1];
   };
   float4* sgr4 = reinterpret_cast<float4*>(s_gr);
   // D block kb of atom xl, features (m = 4 kb + ml, a = 4 xq .. 4 xq + 3):
   // the thread's GR[.][a] (ga) stay in registers through layer 1
   float4 ga[4];
-  auto gen_block = [&](int kb, bf16x8 (&r)[2]) {
+  auto gen_block = [&](int kb, h8<H> (&r)[2]) {
 #pragma unroll
     for (int ml = 0; ml < 4; ++ml) {
       const float4 gm = sgr4[xl * (kGRS / 4) + 4 * kb + ml];
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        r[ml >> 1][4 * (ml & 1) + j] = (__bf16)(gm.x * ga[j].x + gm.y * ga[j].y + gm.z * ga[j].z + gm.w * ga[j].w);
+        r[ml >> 1][4 * (ml & 1) + j] = (H)(gm.x * ga[j].x + gm.y * ga[j].y + gm.z * ga[j].z + gm.w * ga[j].w);
     }
   };
-  auto store_gen = [&](int buf, const bf16x8 (&r)[2]) {
+  auto store_gen = [&](int buf, const h8<H> (&r)[2]) {
     // r[ml / 2][4 (ml % 2) + j] -> feature 16 ml + 4 xq + j
-    __bf16* dst = s_x[buf] + xl * kXS + 4 * xq;
+    H* dst = s_x[buf] + xl * kXS + 4 * xq;
 #pragma unroll
     for (int ml = 0; ml < 4; ++ml) {
-      bf16x4 q;
+      h4<H> q;
 #pragma unroll
       for (int j = 0; j < 4; ++j) q[j] = r[ml >> 1][4 * (ml & 1) + j];
-      *reinterpret_cast<bf16x4*>(dst + 16 * ml) = q;
+      *reinterpret_cast<h4<H>*>(dst + 16 * ml) = q;
     }
   };
 
@@ -257,26 +267,26 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < 4; ++j) ga[j] = sgr4[xl * (kGRS / 4) + 4 * xq + j];
-    bf16x8 r[2];
+    h8<H> r[2];
     gen_block(0, r);
     store_gen(0, r);
   } else {
-    bf16x8 r[2];
+    h8<H> r[2];
     load_block(0, r);
     store_block(0, r);
   }
   __syncthreads();
-  const bf16x8* w1f = frag + p.w1f;
+  const h8<H>* w1f = frag + p.w1f;
   // weight fragments of the next K block are loaded one block ahead
-  bf16x8 a_cur[2][4];
+  h8<H> a_cur[2][4];
 #pragma unroll
   for (int kc = 0; kc < 2; ++kc)
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) a_cur[kc][mt] = w1f[((4 * w + mt) * p.nk1 + kc) * 64 + lane];
 #pragma unroll 1
   for (int kb = 0; kb < nkb; ++kb) {
-    bf16x8 r[2];
-    bf16x8 a_nxt[2][4];
+    h8<H> r[2];
+    h8<H> a_nxt[2][4];
     const bool more = kb + 1 < nkb;
     // fused form: D block kb + 1 is built from the staged GR
     const int kn = more ? kb + 1 : kb;
@@ -291,7 +301,7 @@ __global__ void __launch_bounds__(256)
       for (int mt = 0; mt < 4; ++mt) a_nxt[kc][mt] = w1f[((4 * w + mt) * p.nk1 + 2 * kn + kc) * 64 + lane];
 #pragma unroll
     for (int kc = 0; kc < 2; ++kc) {
-      bf16x8 b[4];
+      h8<H> b[4];
 #pragma unroll
       for (int s = 0; s < 4; ++s) b[s] = lds_b(s_x[kb & 1], kXS, s, n, 32 * kc, g);
 #pragma unroll
@@ -411,12 +421,12 @@ __global__ void __launch_bounds__(256)
   }
 
   // ---- dD = dz1 W1^T: B operands (dz1 of the 64 atoms, K = 256) in registers
-  bf16x8 bz[8][4];
+  h8<H> bz[8][4];
 #pragma unroll
   for (int kc = 0; kc < 8; ++kc)
 #pragma unroll
     for (int s = 0; s < 4; ++s) bz[kc][s] = lds_b(s_h, kHS, s, n, 32 * kc, g);
-  const bf16x8* w1b = frag + p.w1b;
+  const h8<H>* w1b = frag + p.w1b;
   int atom_s[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) atom_s[s] = s_atom[16 * s + n];
@@ -440,13 +450,13 @@ __global__ void __launch_bounds__(256)
     __syncthreads();  // every wavefront has read dz1 from s_h
   }
   // the next M tile's eight weight fragments are loaded one tile ahead
-  bf16x8 wa[8];
+  h8<H> wa[8];
   if (mt0 < mt1)
 #pragma unroll
     for (int kc = 0; kc < 8; ++kc) wa[kc] = w1b[((int64_t)mt0 * 8 + kc) * 64 + lane];
 #pragma unroll 1
   for (int mt = mt0; mt < mt1; ++mt) {
-    bf16x8 wn[8];
+    h8<H> wn[8];
     const int mn = mt + 1 < mt1 ? mt + 1 : mt;
 #pragma unroll
     for (int kc = 0; kc < 8; ++kc) wn[kc] = w1b[((int64_t)mn * 8 + kc) * 64 + lane];
@@ -554,14 +564,10 @@ void set_offsets(FitParams& p) {
   p.type_frags = p.w1b + (int64_t)p.nm1 * 8 * 64;
 }
 
-}  // namespace
-}  // namespace dpmi
-
-using namespace dpmi;
-
-extern "C" int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const void* desc, const int* perm,
-                                      const int* type_start, const void* frags, const float* consts,
-                                      const double* ebias, int64_t nloc, int ntypes, int k1, void* stream) {
+template <typename H>
+int fitting(double* atom_energy, float* ddesc, const void* desc, const int* perm, const int* type_start,
+            const void* frags, const float* consts, const double* ebias, int64_t nloc, int ntypes, int k1,
+            void* stream) {
   DPMI_CHECK_ARG(ntypes > 0 && ntypes <= kMaxTypes, "ntypes out of range [1, 16]");
   DPMI_CHECK_ARG(k1 > 0 && k1 % 16 == 0, "descriptor width must be a positive multiple of 16");
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
@@ -573,17 +579,17 @@ extern "C" int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const v
   set_offsets(p);
   // one workgroup per 64-atom tile of each type (upper bound; extra ones exit)
   const int64_t grid = ceil_div(nloc, kTile) + ntypes;
-  hipLaunchKernelGGL(k_fit<false>, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), atom_energy, ddesc,
-                     static_cast<const __bf16*>(desc), nullptr, perm, type_start, static_cast<const bf16x8*>(frags),
+  hipLaunchKernelGGL((k_fit<H, false>), dim3((unsigned)grid), dim3(256), 0, as_stream(stream), atom_energy, ddesc,
+                     static_cast<const H*>(desc), nullptr, perm, type_start, static_cast<const h8<H>*>(frags),
                      consts, ebias, p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }
 
-extern "C" int dpmi_se_a_fitting_gr_bf16(double* atom_energy, float* dgr, const float* gr_part, const int* perm,
-                                         const int* type_start, const void* frags, const float* consts,
-                                         const double* ebias, int64_t nloc, int ntypes, int n1, int nnei,
-                                         void* stream) {
+template <typename H>
+int fitting_gr(double* atom_energy, float* dgr, const float* gr_part, const int* perm, const int* type_start,
+               const void* frags, const float* consts, const double* ebias, int64_t nloc, int ntypes, int n1,
+               int nnei, void* stream) {
   DPMI_CHECK_ARG(ntypes > 0 && ntypes <= kMaxTypesGR, "fused descriptor form: ntypes out of range [1, 4]");
   DPMI_CHECK_ARG(n1 >= 4 && 4 * n1 <= kMaxGR, "fused descriptor form needs 16 <= 4 n1 <= 104");
   DPMI_CHECK_ARG(nnei > 0, "nnei must be positive");
@@ -599,10 +605,44 @@ extern "C" int dpmi_se_a_fitting_gr_bf16(double* atom_energy, float* dgr, const 
   p.k1 = 16 * p.M;
   set_offsets(p);
   const int64_t grid = ceil_div(nloc, kTile) + ntypes;
-  hipLaunchKernelGGL(k_fit<true>, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), atom_energy, dgr, nullptr,
-                     gr_part, perm, type_start, static_cast<const bf16x8*>(frags), consts, ebias, p);
+  hipLaunchKernelGGL((k_fit<H, true>), dim3((unsigned)grid), dim3(256), 0, as_stream(stream), atom_energy, dgr,
+                     nullptr, gr_part, perm, type_start, static_cast<const h8<H>*>(frags), consts, ebias, p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
+}
+
+}  // namespace
+}  // namespace dpmi
+
+using namespace dpmi;
+
+extern "C" int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const void* desc, const int* perm,
+                                      const int* type_start, const void* frags, const float* consts,
+                                      const double* ebias, int64_t nloc, int ntypes, int k1, void* stream) {
+  return fitting<__bf16>(atom_energy, ddesc, desc, perm, type_start, frags, consts, ebias, nloc, ntypes, k1, stream);
+}
+
+extern "C" int dpmi_se_a_fitting_f16(double* atom_energy, float* ddesc, const void* desc, const int* perm,
+                                     const int* type_start, const void* frags, const float* consts,
+                                     const double* ebias, int64_t nloc, int ntypes, int k1, void* stream) {
+  return fitting<_Float16>(atom_energy, ddesc, desc, perm, type_start, frags, consts, ebias, nloc, ntypes, k1,
+                           stream);
+}
+
+extern "C" int dpmi_se_a_fitting_gr_bf16(double* atom_energy, float* dgr, const float* gr_part, const int* perm,
+                                         const int* type_start, const void* frags, const float* consts,
+                                         const double* ebias, int64_t nloc, int ntypes, int n1, int nnei,
+                                         void* stream) {
+  return fitting_gr<__bf16>(atom_energy, dgr, gr_part, perm, type_start, frags, consts, ebias, nloc, ntypes, n1,
+                            nnei, stream);
+}
+
+extern "C" int dpmi_se_a_fitting_gr_f16(double* atom_energy, float* dgr, const float* gr_part, const int* perm,
+                                        const int* type_start, const void* frags, const float* consts,
+                                        const double* ebias, int64_t nloc, int ntypes, int n1, int nnei,
+                                        void* stream) {
+  return fitting_gr<_Float16>(atom_energy, dgr, gr_part, perm, type_start, frags, consts, ebias, nloc, ntypes, n1,
+                              nnei, stream);
 }
 
 extern "C" int dpmi_se_a_fitting_frag_count(int64_t* count, int k1) {

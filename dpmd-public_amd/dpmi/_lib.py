@@ -41,14 +41,18 @@ SIGNATURES = {
     "dpmi_force_virial_cells_f64": [_P] * 8 + [_L, _L, _P, _F, _F, _P, _I, _P, _P, _P, _L, _P],
     "dpmi_force_virial_cells_f32": [_P] * 8 + [_L, _L, _P, _F, _F, _P, _I, _P, _P, _P, _L, _P],
     "dpmi_se_a_embed_fwd_bf16": [_P, _P, _P, _P, _L, _P, _I, _I, _P, _P, _P, _P],
+    "dpmi_se_a_embed_fwd_f16": [_P, _P, _P, _P, _L, _P, _I, _I, _P, _P, _P, _P],
     "dpmi_se_a_descriptor_fwd": [_P, _P, _P, _L, _I, _I, _I, _I, _I, _P],
     "dpmi_se_a_descriptor_bwd": [_P, _P, _P, _L, _I, _I, _I, _P],
     "dpmi_se_a_embed_bwd_bf16": [_P, _P, _P, _P, _P, _L, _P, _I, _I, _P, _P],
+    "dpmi_se_a_embed_bwd_f16": [_P, _P, _P, _P, _P, _L, _P, _I, _I, _P, _P],
     "dpmi_se_a_section_counts": [_P, _P, _L, _P, _I, _P],
     "dpmi_se_a_embed_fwd_f64": [_P, _P, _P, _P, _P, _L, _P, _I, _I, _P, _P, _P, _P],
     "dpmi_se_a_embed_bwd_f64": [_P, _P, _P, _P, _P, _P, _L, _P, _I, _I, _P, _P],
     "dpmi_se_a_fitting_bf16": [_P, _P, _P, _P, _P, _P, _P, _P, _L, _I, _I, _P],
+    "dpmi_se_a_fitting_f16": [_P, _P, _P, _P, _P, _P, _P, _P, _L, _I, _I, _P],
     "dpmi_se_a_fitting_gr_bf16": [_P, _P, _P, _P, _P, _P, _P, _P, _L, _I, _I, _I, _P],
+    "dpmi_se_a_fitting_gr_f16": [_P, _P, _P, _P, _P, _P, _P, _P, _L, _I, _I, _I, _P],
     "dpmi_se_a_fitting_frag_count": [_P, _I],
     "dpmi_tabulate_fusion_se_a_f64": [_P, _P, _P, _P, _P, _L, _L, _L, _I, _P],
     "dpmi_tabulate_fusion_se_a_f32": [_P, _P, _P, _P, _P, _L, _L, _L, _I, _P],

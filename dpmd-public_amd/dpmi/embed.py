@@ -1,4 +1,4 @@
-"""Host side of the fused se_e2_a embedding kernels (csrc/embed_bf16.hip).
+"""Host side of the fused se_e2_a embedding kernels (csrc/embed_mfma16.hip).
 
 pack_embedding() lays the embedding-net weights of each neighbor type out
 as MFMA A-operand fragments (bf16) in the order the kernels read them, plus a
@@ -77,14 +77,21 @@ def _layer(l):
     return w, b, idt
 
 
-def pack_embedding(emb_layers: list) -> tuple[np.ndarray, np.ndarray, int]:
+# MFMA operand dtype -> entry-point suffix of the 16-bit kernels
+SUFFIX = {torch.bfloat16: "bf16", torch.float16: "f16"}
+# descriptor output dtype -> desc_kind of dpmi_se_a_descriptor_fwd
+DESC_KIND = {torch.float32: 0, torch.bfloat16: 1, torch.float16: 2}
+
+
+def pack_embedding(emb_layers: list, dtype=torch.bfloat16) -> tuple[np.ndarray, np.ndarray, int]:
     """emb_layers[t] = the three layer dicts of neighbor type t.
-    Returns (frags uint16 [ntypes, 40, 64, 8] holding bf16 bits,
+    Returns (frags uint16 [ntypes, 40, 64, 8] holding the bits of the 16-bit
+    operand dtype (bfloat16 or float16),
     consts float32 [ntypes, 704], n1).  consts: K2 w1 [0:32], K2 b1 [32:64],
     idt1 [64:96], K2 b2 [96:160], idt2 [160:224], K2 b3 [224:352],
     idt3 [352:480], w1 [480:512], idt2 / K2 [512:576], 4 idt3 / K2 [576:704]
     (the backward's dz3 = dG idt3 (1 - t3^2) is formed as dG (4 idt3) r3 (1 - r3));
-    all 40 fragments hold bf16(K2 W)."""
+    all 40 fragments hold dtype(K2 W)."""
     ntypes = len(emb_layers)
     (w1, _, _) = _layer(emb_layers[0][0])
     n1 = w1.shape[1]
@@ -120,7 +127,7 @@ def pack_embedding(emb_layers: list) -> tuple[np.ndarray, np.ndarray, int]:
         for mt in range(8):  # layer 3, A = K2 W3^T (M = q3, K = q2)
             for kc in range(2):
                 f[4 + 2 * mt + kc] = K2 * W3[32 * kc + K, (16 * mt + lr)[:, None]]
-        # the adjoint uses the same bf16(K2 W) as the forward (with dz / K2 as
+        # the adjoint uses the same dtype(K2 W) as the forward (with dz / K2 as
         # its B operand), so forces are the gradient of the evaluated energy
         for mt in range(4):  # dh2 = W3 dz3 (M = q2, K = q3)
             for kc in range(4):
@@ -128,15 +135,21 @@ def pack_embedding(emb_layers: list) -> tuple[np.ndarray, np.ndarray, int]:
         for mt in range(2):  # dh1 = W2 dz2 (M = q1, K = q2)
             for kc in range(2):
                 f[20 + 16 + mt * 2 + kc] = K2 * W2[(16 * mt + lr)[:, None], 32 * kc + K]
-    bits = torch.from_numpy(frags).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    bits = torch.from_numpy(frags).to(dtype).view(torch.int16).numpy().view(np.uint16)
     return bits, consts, n1
 
 
 class FusedEmbedding:
-    """Device-resident packed weights + the four fused launches."""
+    """Device-resident packed weights + the four fused launches.  dtype is
+    the MFMA operand type: torch.bfloat16 or torch.float16."""
 
-    def __init__(self, emb_layers, sel, axis: int, device):
-        bits, consts, n1 = pack_embedding(emb_layers)
+    def __init__(self, emb_layers, sel, axis: int, device, dtype=torch.bfloat16):
+        if dtype not in SUFFIX:
+            raise ValueError("the fused embedding takes bfloat16 or float16 operands")
+        self.dtype = dtype
+        self._fwd = "dpmi_se_a_embed_fwd_" + SUFFIX[dtype]
+        self._bwd = "dpmi_se_a_embed_bwd_" + SUFFIX[dtype]
+        bits, consts, n1 = pack_embedding(emb_layers, dtype)
         self.n1 = n1
         self.M = 4 * n1
         self.axis = int(axis)
@@ -155,20 +168,20 @@ class FusedEmbedding:
         nloc = em.shape[0]
         gr_part = torch.empty((self.ntypes, nloc, 4, self.M), dtype=torch.float32, device=em.device)
         counts, atype, pad_em = pad if pad is not None else (None, None, None)
-        ops._launch("dpmi_se_a_embed_fwd_bf16", ptr(gr_part), ptr(em.contiguous()), ptr(self.frags), ptr(self.consts),
+        ops._launch(self._fwd, ptr(gr_part), ptr(em.contiguous()), ptr(self.frags), ptr(self.consts),
                     nloc, ptr(self.sel), self.ntypes, self.n1, ptr(counts), ptr(atype), ptr(pad_em), ops._stream())
         return gr_part
 
     def forward(self, em: torch.Tensor, desc_dtype=torch.float32, pad=None):
         """em [nloc, nnei, 4] fp32 -> (gr [nloc, 4, M], desc [nloc, M*axis] in
-        desc_dtype: float32, or bfloat16 for the fused fitting kernel)."""
+        desc_dtype: float32, or the operand dtype for the fused fitting kernel)."""
         nloc = em.shape[0]
         dev = em.device
         gr_part = self.forward_part(em, pad)
         gr = torch.empty((nloc, 4, self.M), dtype=torch.float32, device=dev)
         desc = torch.empty((nloc, self.M * self.axis), dtype=desc_dtype, device=dev)
         ops._launch("dpmi_se_a_descriptor_fwd", ptr(gr), ptr(desc), ptr(gr_part), nloc, self.ntypes, self.nnei,
-                    self.n1, self.axis, int(desc_dtype == torch.bfloat16), ops._stream())
+                    self.n1, self.axis, DESC_KIND[desc_dtype], ops._stream())
         return gr, desc
 
     def backward(self, em: torch.Tensor, gr: torch.Tensor, ddesc: torch.Tensor, counts=None) -> torch.Tensor:
@@ -186,7 +199,7 @@ class FusedEmbedding:
         each section's listed neighbors get 0."""
         nloc = em.shape[0]
         nd = torch.empty((nloc, self.nnei, 4), dtype=torch.float32, device=em.device)
-        ops._launch("dpmi_se_a_embed_bwd_bf16", ptr(nd), ptr(em.contiguous()), ptr(dgr.contiguous()),
+        ops._launch(self._bwd, ptr(nd), ptr(em.contiguous()), ptr(dgr.contiguous()),
                     ptr(self.frags), ptr(self.consts), nloc, ptr(self.sel), self.ntypes, self.n1, ptr(counts),
                     ops._stream())
         return nd

@@ -1,4 +1,4 @@
-"""Host side of the fused energy fitting kernel (csrc/fitting_bf16.hip).
+"""Host side of the fused energy fitting kernel (csrc/fitting_mfma16.hip).
 
 pack_fitting() lays each type's fitting net (three hidden layers of width
 H <= 256, the second and third with the MLPLayer skip connection, a linear
@@ -62,8 +62,9 @@ def _frags(A: np.ndarray, nm: int, nk: int) -> np.ndarray:
     return P.reshape(nm, 16, nk, 4, 8).transpose(0, 2, 3, 1, 4).reshape(nm, nk, 64, 8)
 
 
-def pack_fitting(fit_layers: list, bias_atom_e, out_bias=None):
-    """Returns (frags uint16 [ntypes, nfrag, 64, 8] bf16 bits, consts float32
+def pack_fitting(fit_layers: list, bias_atom_e, out_bias=None, dtype=torch.bfloat16):
+    """Returns (frags uint16 [ntypes, nfrag, 64, 8] holding the bits of the
+    16-bit operand dtype (bfloat16 or float16), consts float32
     [ntypes, N_CONST], ebias float64 [ntypes], k1)."""
     if not supported(fit_layers):
         raise ValueError("fused fitting needs hidden widths (H, H, H), H <= 256, and a linear H -> 1 output")
@@ -91,17 +92,24 @@ def pack_fitting(fit_layers: list, bias_atom_e, out_bias=None):
             c[k * KH:k * KH + h] = 1.0 if v is None else v
         c[7 * KH] = b4[0] if b4.size else 0.0
     frags = np.stack(blocks).astype(np.float32)
-    bits = torch.from_numpy(frags).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    bits = torch.from_numpy(frags).to(dtype).view(torch.int16).numpy().view(np.uint16)
     ob = np.zeros(ntypes) if out_bias is None else np.asarray(out_bias, dtype=np.float64).reshape(-1)
     ebias = np.asarray(bias_atom_e, dtype=np.float64).reshape(-1) + ob
     return bits, consts, ebias, k1
 
 
 class FusedFitting:
-    """Device-resident packed fitting nets + the fused launch."""
+    """Device-resident packed fitting nets + the fused launch.  dtype is the
+    MFMA operand type: torch.bfloat16 or torch.float16."""
 
-    def __init__(self, fit_layers, bias_atom_e, out_bias, device):
-        bits, consts, ebias, k1 = pack_fitting(fit_layers, bias_atom_e, out_bias)
+    def __init__(self, fit_layers, bias_atom_e, out_bias, device, dtype=torch.bfloat16):
+        sfx = {torch.bfloat16: "bf16", torch.float16: "f16"}.get(dtype)
+        if sfx is None:
+            raise ValueError("the fused fitting takes bfloat16 or float16 operands")
+        self.dtype = dtype
+        self._fit = "dpmi_se_a_fitting_" + sfx
+        self._fit_gr = "dpmi_se_a_fitting_gr_" + sfx
+        bits, consts, ebias, k1 = pack_fitting(fit_layers, bias_atom_e, out_bias, dtype)
         self.k1 = k1
         self.ntypes = len(fit_layers)
         self.frags = torch.from_numpy(bits.astype(np.int16)).to(device)
@@ -119,17 +127,18 @@ class FusedFitting:
         return perm, start
 
     def __call__(self, desc: torch.Tensor, atype: torch.Tensor, order=None):
-        """desc [nloc, k1] bf16 -> (atom_energy float64 [nloc], d E / d desc fp32 [nloc, k1])."""
+        """desc [nloc, k1] in the operand dtype -> (atom_energy float64 [nloc],
+        d E / d desc fp32 [nloc, k1])."""
         nloc = desc.shape[0]
         if desc.shape[1] != self.k1:
             raise ValueError(f"descriptor width {desc.shape[1]} != fitting input {self.k1}")
-        if desc.dtype != torch.bfloat16:
-            raise TypeError("the fused fitting kernel takes the descriptor in bfloat16")
+        if desc.dtype != self.dtype:
+            raise TypeError(f"the fused fitting kernel takes the descriptor in {self.dtype}")
         perm, start = self.sort_by_type(atype) if order is None else order
         desc = desc.contiguous()
         ae = torch.empty(nloc, dtype=torch.float64, device=desc.device)
         dd = torch.empty((nloc, self.k1), dtype=torch.float32, device=desc.device)
-        ops._launch("dpmi_se_a_fitting_bf16", ptr(ae), ptr(dd), ptr(desc), ptr(perm), ptr(start), ptr(self.frags),
+        ops._launch(self._fit, ptr(ae), ptr(dd), ptr(desc), ptr(perm), ptr(start), ptr(self.frags),
                     ptr(self.consts), ptr(self.ebias), nloc, self.ntypes, self.k1, ops._stream())
         return ae, dd
 
@@ -150,6 +159,6 @@ class FusedFitting:
         gr_part = gr_part.contiguous()
         ae = torch.empty(nloc, dtype=torch.float64, device=gr_part.device)
         dgr = torch.empty((nloc, 128, 4), dtype=torch.float32, device=gr_part.device)
-        ops._launch("dpmi_se_a_fitting_gr_bf16", ptr(ae), ptr(dgr), ptr(gr_part), ptr(perm), ptr(start),
+        ops._launch(self._fit_gr, ptr(ae), ptr(dgr), ptr(gr_part), ptr(perm), ptr(start),
                     ptr(self.frags), ptr(self.consts), ptr(self.ebias), nloc, self.ntypes, n1, nnei, ops._stream())
         return ae, dgr

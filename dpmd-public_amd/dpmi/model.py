@@ -166,24 +166,24 @@ class SeAEnergyModel:
         self.fused64 = None
         if precision == "fp64" and tables is None and embed.supported_f64(neuron):
             self.fused64 = embed.FusedEmbeddingF64(emb, self.sel, self.device)
-        # bf16 form: fused MFMA embedding kernels (else the torch layers above)
+        # bf16 / fp16 forms: fused MFMA embedding kernels (else the torch layers above)
         self.fused = None
-        if precision == "bf16" and embed.supported(neuron) and self.axis <= 16:
-            self.fused = embed.FusedEmbedding(emb, self.sel, self.axis, self.device)
+        if precision in ("bf16", "fp16") and embed.supported(neuron) and self.axis <= 16:
+            self.fused = embed.FusedEmbedding(emb, self.sel, self.axis, self.device, self.gemm_dtype)
         self.fit = [Net(layers, self.device, dt, last_linear=True) for layers in fit]
         self.M = self.emb[0].w[-1].shape[1]
         self.bias_atom_e = torch.as_tensor(np.asarray(bias_atom_e, dtype=np.float64).reshape(-1),
                                            device=self.device)
         ob = np.zeros(self.ntypes) if out_bias is None else np.asarray(out_bias, dtype=np.float64).reshape(-1)
         self.out_bias = torch.as_tensor(ob, device=self.device)
-        # bf16 form: fused MFMA fitting kernel (else the torch layers above)
+        # bf16 / fp16 forms: fused MFMA fitting kernel (else the torch layers above)
         self.fused_fit = None
         if self.fused is not None and fitting.supported(fit):
-            self.fused_fit = fitting.FusedFitting(fit, bias_atom_e, ob, self.device)
+            self.fused_fit = fitting.FusedFitting(fit, bias_atom_e, ob, self.device, self.gemm_dtype)
         # the descriptor fused into the fitting kernel (axis 16, M <= 104)
         self.fused_gr = self.fused_fit is not None and self.fused_fit.supports_gr(self.fused.n1, self.axis)
         self._order_cache = None
-        # padding shortcut of the bf16 embedding kernels (each davg / dstd
+        # padding shortcut of the fused embedding kernels (each davg / dstd
         # section constant, so all padding slots of a section share one em row)
         self.pad_em = self._padding_rows() if (self.fused is not None or self.fused64 is not None) else None
 
@@ -416,7 +416,7 @@ class SeAEnergyModel:
         return order
 
     def _fused_chain(self, em, atype, use_gr=None, order=None, pad=None):
-        """bf16 embedding -> descriptor -> fitting -> adjoints: (dE/d em, atom
+        """bf16 / fp16 embedding -> descriptor -> fitting -> adjoints: (dE/d em, atom
         energies).  With the descriptor fused into the fitting kernel
         (fused_gr) the G^T R partials go straight to it and dE/dGR comes back;
         otherwise the descriptor kernels run between (and torch runs the
@@ -427,7 +427,7 @@ class SeAEnergyModel:
             gr_part = self.fused.forward_part(em, pad)
             ae, dgr = self.fused_fit.from_gr(gr_part, atype, self.fused.n1, self.fused.nnei, order=order)
             return self.fused.backward_dgr(em, dgr, counts), ae
-        gr, desc = self.fused.forward(em, torch.bfloat16 if self.fused_fit is not None else torch.float32, pad)
+        gr, desc = self.fused.forward(em, self.gemm_dtype if self.fused_fit is not None else torch.float32, pad)
         if self.fused_fit is not None:
             ae, ddesc = self.fused_fit(desc, atype, order=order)
         else:
@@ -438,7 +438,7 @@ class SeAEnergyModel:
         return self.fused.backward(em, gr, ddesc, counts), ae
 
     def _compute_fused(self, atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy, order=None):
-        """bf16 form with hand-written adjoints end to end: fused embedding +
+        """bf16 / fp16 forms with hand-written adjoints end to end: fused embedding +
         G^T R forward, fused fitting forward + adjoint (or hipBLASLt GEMMs with
         autograd through the fitting net when its shape is not supported),
         descriptor and embedding adjoints recomputing the forward, env-mat

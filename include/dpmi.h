@@ -242,13 +242,14 @@ int dpmi_tabulate_fusion_se_a_grad_grad_f32(float* dz_dy, const float* table, co
                                             int64_t last_layer_size, int is_sorted, void* stream);
 
 /* ------------------------------------------------------------------------
- * Fused embedding net + G^T R and the descriptor, bf16 MFMA with fp32
- * accumulation (the precision form of DescrptBlockSeA.forward,
+ * Fused embedding net + G^T R and the descriptor, bf16 (or fp16: the _f16
+ * entry points) MFMA with fp32 accumulation (the precision form of
+ * DescrptBlockSeA.forward,
  * deepmd/pt/model/descriptor/se_a.py:802-858, embedding layers of
  * deepmd/pt/model/network/mlp.py:188-219; type_one_side, widths
  * (n1, 2 n1, 4 n1), n1 <= 32, resnet with timestep).  The reference has no
  * single function for this: it chains torch MLP layers and matmuls.
- * frags: [ntypes, 40, 64, 8] bf16 weight fragments and consts
+ * frags: [ntypes, 40, 64, 8] bf16 (fp16 for _f16) weight fragments and consts
  * [ntypes, 704] fp32, both laid out by dpmi.embed.pack_embedding (internal
  * feature order q3; see that module).  em [nloc, nnei, 4] fp32.
  * ------------------------------------------------------------------------ */
@@ -268,10 +269,14 @@ int dpmi_se_a_section_counts(int16_t* counts, const int* nlist, int64_t nloc, co
 int dpmi_se_a_embed_fwd_bf16(float* gr_part, const float* em, const void* frags, const float* consts, int64_t nloc,
                              const int* sel_host, int ntypes, int n1, const int16_t* counts, const int* atype,
                              const float* pad_em, void* stream);
+int dpmi_se_a_embed_fwd_f16(float* gr_part, const float* em, const void* frags, const float* consts, int64_t nloc,
+                            const int* sel_host, int ntypes, int n1, const int16_t* counts, const int* atype,
+                            const float* pad_em, void* stream);
 /* gr [nloc, 4, 4 n1] = sum_t gr_part / nnei; desc [nloc, 4 n1 * axis] =
- * GR^T GR[:, :axis] (se_a.py:851-858), fp32 or (desc_bf16 != 0) bf16. */
+ * GR^T GR[:, :axis] (se_a.py:851-858) in fp32 (desc_kind 0), bf16 (1) or
+ * fp16 (2). */
 int dpmi_se_a_descriptor_fwd(float* gr, void* desc, const float* gr_part, int64_t nloc, int ntypes, int nnei,
-                             int n1, int axis, int desc_bf16, void* stream);
+                             int n1, int axis, int desc_kind, void* stream);
 /* dgr [nloc, 128, 4] = dE/d(sum_t gr_part) from ddesc = dE/d desc, in the
  * embedding kernels' internal feature order, the four components of a
  * feature contiguous (gr as written above). */
@@ -281,10 +286,15 @@ int dpmi_se_a_descriptor_bwd(float* dgr, const float* ddesc, const float* gr, in
  * (n1 = width of the first embedding layer, as packed).  With counts (the
  * padding shortcut of the forward) the rows after each section's listed
  * neighbors are written 0: they are padding slots, which no force term
- * reads. */
+ * reads.  The fp16 form stages each atom's dgr scaled by a power of two
+ * (its largest magnitude brought into [1, 2)) and unscales the result,
+ * exactly, so the fp16 operands of the adjoint stay normal numbers. */
 int dpmi_se_a_embed_bwd_bf16(float* net_deriv, const float* em, const float* dgr, const void* frags,
                              const float* consts, int64_t nloc, const int* sel_host, int ntypes, int n1,
                              const int16_t* counts, void* stream);
+int dpmi_se_a_embed_bwd_f16(float* net_deriv, const float* em, const float* dgr, const void* frags,
+                            const float* consts, int64_t nloc, const int* sel_host, int ntypes, int n1,
+                            const int16_t* counts, void* stream);
 
 /* FP64 form of the same two kernels (v_mfma_f64_16x16x4f64, fp64
  * throughout).  Weights as packed by dpmi.embed.pack_embedding_f64: w2
@@ -301,21 +311,26 @@ int dpmi_se_a_embed_bwd_f64(double* net_deriv, const double* em, const double* d
                             int n1, const int16_t* counts, void* stream);
 
 /* ------------------------------------------------------------------------
- * Fused energy fitting net (bf16 MFMA operands, fp32 accumulation).
+ * Fused energy fitting net (bf16 MFMA operands, or fp16 for the _f16
+ * entry points, fp32 accumulation).
  * Replaces GeneralFitting._forward_common + its autograd backward
  * (deepmd/pt/model/task/fitting.py:734-900; FittingNet layers of
  * deepmd/dpmodel/utils/network.py:1031-1090): per type, three hidden layers
  * of width H <= 256 (the second and third with the skip connection) and a
- * linear H -> 1 output.  desc_bf16 [nloc, k1] is the descriptor in bf16 (as
- * dpmi_se_a_descriptor_fwd writes it with desc_bf16 = 1).  frags / consts / ebias are laid out by
+ * linear H -> 1 output.  desc [nloc, k1] is the descriptor in bf16 (fp16)
+ * as dpmi_se_a_descriptor_fwd writes it with desc_kind 1 (2).  frags /
+ * consts / ebias are laid out by
  * dpmi.fitting.pack_fitting; perm [nloc] lists the local atoms ordered by
  * type and type_start [ntypes + 1] (device) delimits each type's run.
  * ------------------------------------------------------------------------ */
 /* atom_energy [nloc] fp64 = network output + bias_atom_e + out_bias;
  * ddesc [nloc, k1] fp32 = dE/d desc for E = sum atom_energy. */
-int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const void* desc_bf16, const int* perm,
+int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const void* desc, const int* perm,
                            const int* type_start, const void* frags, const float* consts, const double* ebias,
                            int64_t nloc, int ntypes, int k1, void* stream);
+int dpmi_se_a_fitting_f16(double* atom_energy, float* ddesc, const void* desc, const int* perm,
+                          const int* type_start, const void* frags, const float* consts, const double* ebias,
+                          int64_t nloc, int ntypes, int k1, void* stream);
 /* The same fitting net with the descriptor fused in (axis 16, M = 4 n1 with
  * 16 <= M <= 104): gr_part [ntypes, nloc, 4, M] as dpmi_se_a_embed_fwd_bf16
  * writes it; each 64-atom workgroup forms GR = sum_t gr_part / nnei and
@@ -328,7 +343,10 @@ int dpmi_se_a_fitting_bf16(double* atom_energy, float* ddesc, const void* desc_b
 int dpmi_se_a_fitting_gr_bf16(double* atom_energy, float* dgr, const float* gr_part, const int* perm,
                               const int* type_start, const void* frags, const float* consts, const double* ebias,
                               int64_t nloc, int ntypes, int n1, int nnei, void* stream);
-/* Number of bf16x8 lanes (64 per fragment) of one type's packed fitting net. */
+int dpmi_se_a_fitting_gr_f16(double* atom_energy, float* dgr, const float* gr_part, const int* perm,
+                             const int* type_start, const void* frags, const float* consts, const double* ebias,
+                             int64_t nloc, int ntypes, int n1, int nnei, void* stream);
+/* Number of 8 x 16-bit lanes (64 per fragment) of one type's packed fitting net. */
 int dpmi_se_a_fitting_frag_count(int64_t* count, int k1);
 
 #ifdef __cplusplus

@@ -1,4 +1,5 @@
-"""Fused bf16 embedding kernels against the float64 reference math."""
+"""Fused bf16 / fp16 embedding and fitting kernels against the float64
+reference math (every test taking the ``setup`` fixture runs in both forms)."""
 
 import numpy as np
 import pytest
@@ -14,8 +15,8 @@ def _say(msg):
     print(f"[gpu case] {msg}", flush=True)
 
 
-@pytest.fixture(scope="module")
-def setup():
+@pytest.fixture(scope="module", params=["bf16", "fp16"])
+def setup(request):
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     from dpmi.model import SeAEnergyModel
@@ -24,8 +25,8 @@ def setup():
     spec = systems.WATER_SPEC
     params = gpu_model_params(spec, systems.water192(0))
     m64 = SeAEnergyModel.from_params(spec, params, device=DEV)
-    mbf = SeAEnergyModel.from_params(spec, params, device=DEV, precision="bf16")
-    assert mbf.fused is not None
+    mbf = SeAEnergyModel.from_params(spec, params, device=DEV, precision=request.param)
+    assert mbf.fused is not None and mbf.fused.dtype == mbf.gemm_dtype
     s = systems.water192(0)
     c = torch.tensor(s.coord, device=DEV)
     t = torch.tensor(s.atype, device=DEV)
@@ -65,14 +66,14 @@ def test_fused_embedding_backward(setup):
 
 def test_fused_model_force_rmse(setup):
     spec, params, m64, mbf, s, c, t = setup
-    _say("fused bf16 model water192 E+F+V")
+    _say(f"fused {mbf.precision} model water192 E+F+V")
     a = m64.compute(c, t, s.box)
     b = mbf.compute(c, t, s.box)
     rmse = torch.sqrt(((a["force"] - b["force"]) ** 2).mean()).item()
     frms = torch.sqrt((a["force"] ** 2).mean()).item()
     de = abs(a["energy"].item() - b["energy"].item()) / s.nloc
     dv = (torch.abs(a["virial"] - b["virial"]).max() / torch.abs(a["virial"]).max()).item()
-    print(f"fused bf16: force RMSE {rmse:.3e} ({rmse / frms:.2e} rel), |dE|/atom {de:.3e}, virial {dv:.2e}",
+    print(f"fused {mbf.precision}: force RMSE {rmse:.3e} ({rmse / frms:.2e} rel), |dE|/atom {de:.3e}, virial {dv:.2e}",
           flush=True)
     assert rmse <= 1e-3
     assert de <= 5e-3
@@ -85,7 +86,7 @@ def test_fused_fitting(setup, nloc):
     leave partial 64-atom tiles in both types."""
     spec, params, m64, mbf, s, c, t = setup
     assert mbf.fused_fit is not None
-    _say(f"fused fitting nloc={nloc}")
+    _say(f"fused fitting {mbf.precision} nloc={nloc}")
     g = torch.Generator(device="cpu").manual_seed(nloc)
     k1 = mbf.fused_fit.k1
     desc = (torch.randn(nloc, k1, generator=g, dtype=torch.float64) * 0.3).to(DEV)
@@ -94,7 +95,7 @@ def test_fused_fitting(setup, nloc):
     with torch.enable_grad():
         ae64 = m64.atom_energy(d, atype)
         (dd64,) = torch.autograd.grad(ae64.sum(), d)
-    ae, dd = mbf.fused_fit(desc.to(torch.bfloat16), atype)
+    ae, dd = mbf.fused_fit(desc.to(mbf.gemm_dtype), atype)
     torch.cuda.synchronize()
     e_err = (ae - ae64).abs().max().item()
     d_err = _rel_rms(dd.double(), dd64)
@@ -118,7 +119,7 @@ def test_padding_shortcut_matches_full_rows(setup, system):
 
     spec, params, m64, mbf, s, c, t = setup
     assert mbf.pad_em is not None
-    _say(f"padding shortcut {system}")
+    _say(f"padding shortcut {mbf.precision} {system}")
     if system != "water192":
         s = systems.water12k(1)
         c = torch.tensor(s.coord, device=DEV)
@@ -181,6 +182,24 @@ def test_padding_shortcut_fp64(setup):
     assert (nd1[~valid] == 0).all()
 
 
+def test_embedding_backward_scale_exact(setup):
+    """The embedding adjoint is linear in dE/dGR: dgr times 2^-20 must give
+    dE/d em times 2^-20 bit for bit.  In fp16 this holds only because each
+    atom's dgr is staged scaled into [1, 2) (without it the adjoint's fp16
+    operands would underflow); in bf16 the wide exponent range gives it."""
+    spec, params, m64, mbf, s, c, t = setup
+    _say(f"embedding backward scale {mbf.precision} water192")
+    ext_c, ext_t, mapping, nlist, _, _, _ = m64.neighbor_list(c, t.to(torch.int32), s.box)
+    em = m64.env_mat(nlist, ext_c, ext_t).float()
+    gen = torch.Generator(device="cpu").manual_seed(11)
+    dgr = (torch.randn(s.nloc, 128, 4, generator=gen) * 1e-2).to(DEV)
+    nd = mbf.fused.backward_dgr(em, dgr)
+    nd_s = mbf.fused.backward_dgr(em, dgr * 2.0**-20)
+    torch.cuda.synchronize()
+    assert torch.isfinite(nd).all() and nd.abs().max() > 0
+    assert torch.equal(nd_s, nd * 2.0**-20)
+
+
 def test_type_order_cache_follows_atype(setup):
     """The model reuses the type ordering of the fitting kernel while the
     same atype tensor comes back unchanged: an in-place change of the types
@@ -188,20 +207,20 @@ def test_type_order_cache_follows_atype(setup):
     from dpmi.model import SeAEnergyModel
 
     spec, params, m64, mbf, s, c, t = setup
-    _say("type-order cache water192")
+    _say(f"type-order cache {mbf.precision} water192")
     e0 = mbf.compute(c, t, s.box)["energy"].item()
     t2 = t.clone()
     assert mbf.compute(c, t2, s.box)["energy"].item() == e0
     t2[:8] = 1 - t2[:8]
     e1 = mbf.compute(c, t2, s.box)["energy"].item()
-    fresh = SeAEnergyModel.from_params(spec, params, device=DEV, precision="bf16")
+    fresh = SeAEnergyModel.from_params(spec, params, device=DEV, precision=mbf.precision)
     e_ref = fresh.compute(c, t2, s.box)["energy"].item()
     assert e1 == e_ref and e1 != e0
 
 
 def _check_fused_descriptor(fe, ff, gr_part, atype):
-    """dpmi_se_a_fitting_gr_bf16 against the two-kernel chain on the same G^T R
-    partials (descriptor forward -> fitting on the bf16 descriptor ->
+    """dpmi_se_a_fitting_gr_{bf16,f16} against the two-kernel chain on the same
+    G^T R partials (descriptor forward -> fitting on the 16-bit descriptor ->
     descriptor backward): the same D and fitting arithmetic, dE/dGR summed in
     another order; padding slots of dgr exactly zero."""
     from dpmi import embed, ops
@@ -209,9 +228,9 @@ def _check_fused_descriptor(fe, ff, gr_part, atype):
 
     nloc = gr_part.shape[1]
     gr = torch.empty((nloc, 4, fe.M), dtype=torch.float32, device=DEV)
-    desc = torch.empty((nloc, fe.M * 16), dtype=torch.bfloat16, device=DEV)
-    ops._launch("dpmi_se_a_descriptor_fwd", ptr(gr), ptr(desc), ptr(gr_part), nloc, fe.ntypes, fe.nnei, fe.n1, 16, 1,
-                ops._stream())
+    desc = torch.empty((nloc, fe.M * 16), dtype=fe.dtype, device=DEV)
+    ops._launch("dpmi_se_a_descriptor_fwd", ptr(gr), ptr(desc), ptr(gr_part), nloc, fe.ntypes, fe.nnei, fe.n1, 16,
+                embed.DESC_KIND[fe.dtype], ops._stream())
     ae_ref, ddesc = ff(desc, atype)
     dgr_ref = torch.empty((nloc, 128, 4), dtype=torch.float32, device=DEV)
     ops._launch("dpmi_se_a_descriptor_bwd", ptr(dgr_ref), ptr(ddesc), ptr(gr), nloc, fe.nnei, fe.n1, 16,
@@ -239,7 +258,7 @@ def test_fused_descriptor_fitting(setup, nloc):
     spec, params, m64, mbf, s, c, t = setup
     assert mbf.fused_gr
     fe, ff = mbf.fused, mbf.fused_fit
-    _say(f"fused descriptor + fitting nloc={nloc}")
+    _say(f"fused descriptor + fitting {mbf.precision} nloc={nloc}")
     if nloc == s.nloc:
         ext_c, ext_t, mapping, nlist, _, _, _ = m64.neighbor_list(c, t.to(torch.int32), s.box)
         gr_part = fe.forward_part(m64.env_mat(nlist, ext_c, ext_t).float())
@@ -251,8 +270,9 @@ def test_fused_descriptor_fitting(setup, nloc):
     _check_fused_descriptor(fe, ff, gr_part, atype)
 
 
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
 @pytest.mark.parametrize("n1,ntypes", [(16, 3), (26, 4), (4, 1)])
-def test_fused_descriptor_fitting_shapes(n1, ntypes):
+def test_fused_descriptor_fitting_shapes(n1, ntypes, precision):
     """Other widths the fused form takes: M = 64 with three types, M = 104
     (the widest) with four, M = 16 with one type; random partials and types,
     700 atoms; fitting width 96."""
@@ -260,11 +280,11 @@ def test_fused_descriptor_fitting_shapes(n1, ntypes):
         pytest.skip("no GPU")
     from dpmi.model import SeAEnergyModel
 
-    _say(f"fused descriptor shapes n1={n1} ntypes={ntypes}")
+    _say(f"fused descriptor shapes {precision} n1={n1} ntypes={ntypes}")
     sel = tuple(10 + 7 * k for k in range(ntypes))
     spec = systems.ModelSpec(ntypes=ntypes, sel=sel, neuron=(n1, 2 * n1, 4 * n1), fitting=(96, 96, 96))
     params = systems.make_params(spec, seed=n1 + ntypes)
-    m = SeAEnergyModel.from_params(spec, params, device=DEV, precision="bf16")
+    m = SeAEnergyModel.from_params(spec, params, device=DEV, precision=precision)
     assert m.fused_gr
     g = torch.Generator(device="cpu").manual_seed(n1)
     nloc = 700
