@@ -6,7 +6,8 @@ the GPU, inputs resident in HBM.  Both forms run on the same GPU and the same
 coordinates: the plain FP64 path and the BF16-MFMA form (embedding/fitting
 GEMM operands bf16 with fp32 accumulation, env-mat fp32, energy and virial
 reductions fp64).  `value` is the BF16 form's throughput; the FP64 figure,
-the speedup and the BF16 error against FP64 ride along.
+the speedup and the BF16 error against FP64 ride along, and so does the
+FP16-MFMA form (same kernels with fp16 operands: throughput, speedup, error).
 
     python bench.py [--steps K] [--warmup W] [--workload water110k]
 """
@@ -203,17 +204,24 @@ def main() -> int:
         del m64
     mlo = SeAEnergyModel.from_params(spec, params, device=dev, precision=args.precision)
     tlo, outlo, klo = run(mlo, names32)
+    # the other 16-bit form beside the FP64 leg (bf16 line: fp16 too)
+    other = None
+    if out64 is not None and args.precision == "bf16":
+        m16 = SeAEnergyModel.from_params(spec, params, device=dev, precision="fp16")
+        other = run(m16, [n.replace("_bf16", "_f16") for n in names32])
+        del m16
 
-    accuracy = None
-    if out64 is not None:
+    def accuracy_of(out):
         frms = torch.sqrt((out64["force"] ** 2).mean()).item()
-        rmse = torch.sqrt(((out64["force"] - outlo["force"]) ** 2).mean()).item()
-        de_atom = abs(out64["energy"].item() - outlo["energy"].item()) / nloc
+        rmse = torch.sqrt(((out64["force"] - out["force"]) ** 2).mean()).item()
+        de_atom = abs(out64["energy"].item() - out["energy"].item()) / nloc
         vscale = torch.abs(out64["virial"]).max().item()
-        dv = torch.abs(out64["virial"] - outlo["virial"]).max().item() / max(vscale, 1e-300)
-        accuracy = {"force_rmse_eV_per_A": rmse, "force_rms_eV_per_A": frms,
-                    "force_rmse_relative": rmse / frms if frms else None,
-                    "energy_per_atom_abs_eV": de_atom, "virial_max_rel": dv, "force_rmse_ok": rmse <= 1e-3}
+        dv = torch.abs(out64["virial"] - out["virial"]).max().item() / max(vscale, 1e-300)
+        return {"force_rmse_eV_per_A": rmse, "force_rms_eV_per_A": frms,
+                "force_rmse_relative": rmse / frms if frms else None,
+                "energy_per_atom_abs_eV": de_atom, "virial_max_rel": dv, "force_rmse_ok": rmse <= 1e-3}
+
+    accuracy = accuracy_of(outlo) if out64 is not None else None
 
     # rows and 16-row tiles the embedding kernels evaluate: listed neighbors
     # per (atom, type section) when the padding shortcut is on, else all slots
@@ -283,8 +291,12 @@ def main() -> int:
         "fp64": None if t64 is None else {"value": round(nloc / t64, 1), "ms_per_step": round(t64 * 1e3, 3)},
         "speedup_vs_fp64": None if t64 is None else round(t64 / tlo, 3),
         "accuracy_vs_fp64": accuracy,
+        "fp16": None if other is None else {
+            "value": round(nloc / other[0], 1), "ms_per_step": round(other[0] * 1e3, 3),
+            "speedup_vs_fp64": round(t64 / other[0], 3), "accuracy_vs_fp64": accuracy_of(other[1])},
         "kernels_ms_per_step": {"fp64": {k: round(v["total_ms"] / args.steps, 4) for k, v in k64.items()},
-                                args.precision: {k: round(v["total_ms"] / args.steps, 4) for k, v in klo.items()}},
+                                args.precision: {k: round(v["total_ms"] / args.steps, 4) for k, v in klo.items()}}
+        | ({} if other is None else {"fp16": {k: round(v["total_ms"] / args.steps, 4) for k, v in other[2].items()}}),
         "roofline": roofline,
         "hbm_kernels": line_hbm,
         # the env-mat derivative is recomputed inside the force kernel, never
