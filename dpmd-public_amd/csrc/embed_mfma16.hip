@@ -229,7 +229,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   float4 ev_next[2] = {load_em(blockIdx.x * kAtoms + w, r), load_em(blockIdx.x * kAtoms + w, 16 + r)};
   int nt_i = tiles_of(blockIdx.x * kAtoms + w);
   for (int i = blockIdx.x * kAtoms + w; i < nloc; i += stride) {
-    const int nti = nt_i, nchunk = (nti + 1) >> 1, inext = i + stride;
+    // (the counts are the same in every lane: readfirstlane makes the tile
+    // loop's branches scalar; applied here, where the load read ahead one
+    // atom ago has long completed)
+    const int nti = __builtin_amdgcn_readfirstlane(nt_i), nchunk = (nti + 1) >> 1, inext = i + stride;
     nt_i = tiles_of(inext);  // read ahead of its use
     const int ti = p.counts != nullptr ? p.atype[i] : -1;
     if (nchunk == 0) {  // no evaluated tile: the prefetched rows belong to the next atom
@@ -559,7 +562,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   float4 ev_next = load_em(blockIdx.x * kAtoms + w, 0);
   int nv_next = listed(blockIdx.x * kAtoms + w);
   for (int i = blockIdx.x * kAtoms + w; i < nloc; i += stride) {
-    const int nv = nv_next, nti = (nv + 15) >> 4, inext = i + stride;
+    // (wave-uniform: readfirstlane keeps the tile loop's control scalar)
+    const int nv = __builtin_amdgcn_readfirstlane(nv_next), nti = (nv + 15) >> 4, inext = i + stride;
     nv_next = listed(inext);
     if (nti == 0) ev_next = load_em(inext, 0);
     // stage dGR [q3][c] of the atom: sdg[q3 * 4 + c]
