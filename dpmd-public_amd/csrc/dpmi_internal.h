@@ -48,6 +48,12 @@ constexpr int kWave = 64;
 
 __device__ inline int lane_id() { return __lane_id(); }
 
+// Index of the wavefront in its workgroup, as a wave-uniform (scalar) value:
+// the compiler cannot tell that threadIdx.x / 64 is the same in every lane,
+// and everything derived from it (atom indices, per-atom loads, addresses)
+// would otherwise live in vector registers.
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
 // Lanes below this one whose bit is set in mask.
 __device__ inline int mask_prefix(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),

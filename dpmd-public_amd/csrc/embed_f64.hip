@@ -231,7 +231,7 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
   const double* sW2 = smem64;
   const double* sW3 = sW2 + 32 * kP2;
   const double* sC = sW3 + 64 * kP3;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  const int w = wave_id(), lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   const int n1 = p.n1, M = 4 * n1;
   const int sel = p.sec[t + 1] - p.sec[t], ntile = (sel + 15) / 16;
   const int nb = p.blk[t + 1] - p.blk[t], b0 = blockIdx.x - p.blk[t];
@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
   for (int i = b0 * kWaves64 + w; i < nloc; i += nb * kWaves64) {
     const double* emi = em + ((int64_t)i * p.nnei + p.sec[t]) * 4;
     // tiles holding listed neighbors (all of the section without counts)
-    const int nti = p.counts != nullptr ? (p.counts[(int64_t)i * p.ntypes + t] + 15) >> 4 : ntile;
+    const int nti = p.counts != nullptr ? (p.counts[(int64_t)i * p.ntypes + t] + 15) >> 4 : ntile;  // wave-uniform
     // GR[c = g][16 mt + r]: slot 0 of each tile's GR^T product (the other
     // slots are zero rows c >= 4), accumulated here so that no full
     // accumulator stays live across tiles
@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(64 * kWaves64) __attribute__((amdgpu_waves_per
   const double* sW2 = smem64;
   const double* sW3 = sW2 + 32 * kP2;
   const double* sC = sW3 + 64 * kP3;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  const int w = wave_id(), lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   double* sdg = const_cast<double*>(sC) + kC64 + w * 128 * 4;  // [q3][c]
   const int n1 = p.n1, M = 4 * n1;
   const int sel = p.sec[t + 1] - p.sec[t], ntile = (sel + 15) / 16;

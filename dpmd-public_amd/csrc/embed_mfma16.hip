@@ -174,6 +174,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
   __shared__ __attribute__((aligned(16))) float s_em[kAtoms][32 * 4];
   const int t = blockIdx.y;
+  // (threadIdx.x >> 6 here, not wave_id(): the scalar atom index measured
+  // slower in this kernel)
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   const h8<H>* frag_t = frag_all + (int64_t)t * (kFragFwd + kFragBwd) * 64;
   for (int k = threadIdx.x; k < kFragFwd * 64; k += blockDim.x) s_frag[k] = frag_t[k];
@@ -530,7 +532,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   __shared__ __attribute__((aligned(16))) float s_kin[kAtoms][4];
   __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
   const int t = blockIdx.y;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  const int w = wave_id(), lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   const h8<H>* frag_t = frag_all + (int64_t)t * (kFragFwd + kFragBwd) * 64;
   for (int k = threadIdx.x; k < (kFragFwd + kFragBwd) * 64; k += blockDim.x) s_frag[k] = frag_t[k];
   __syncthreads();

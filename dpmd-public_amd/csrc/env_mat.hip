@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(256)
                 const CT* __restrict__ ext_c, const int* __restrict__ ext_t, int nloc, const FP* __restrict__ avg,
                 const FP* __restrict__ std_, float rmin, float rmax, SecParams sp) {
   __shared__ int lim_all[kAtomsPerBlock][kMaxTypes];
-  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int w = wave_id(), lane = lane_id();
   const int i = blockIdx.x * kAtomsPerBlock + w;
   const bool active = i < nloc;
   const int nnei = sp.nnei;
@@ -570,7 +570,7 @@ __global__ void __launch_bounds__(256)
   const int* __restrict__ ghost_off = reinterpret_cast<const int*>(nlws + L.ghost_off);
   const int* __restrict__ misc = reinterpret_cast<const int*>(nlws + L.misc);
   const int* __restrict__ bigcells = reinterpret_cast<const int*>(nlws + L.bigcells);
-  const int tid = threadIdx.x, w = tid / kWave, lane = lane_id();
+  const int tid = threadIdx.x, w = wave_id(), lane = lane_id();
   const int side = 2 * p.nsearch + 1, nnb = side * side * side;
   const int ncells = kBigPass ? misc[3] : p.ncell[0] * p.ncell[1] * p.ncell[2];
   const int nnei = sp.nnei;

@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(256)
   }
   if (t < 0) return;
 
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, n = lane & 15, g = lane >> 4;
+  const int tid = threadIdx.x, w = wave_id(), lane = tid & 63, n = lane & 15, g = lane >> 4;
   if (tid < kTile) s_atom[tid] = tid < cnt ? perm[start + tid] : -1;
   const h8<H>* frag = frag_all + (int64_t)t * p.type_frags;
   const float* cst = consts + (int64_t)t * kCst;
