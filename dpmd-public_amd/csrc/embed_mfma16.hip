@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(256)
     k_descrpt_fwd(float* __restrict__ gr, DT* __restrict__ desc, const float* __restrict__ gr_part, int nloc,
                   int ntypes, float inv_nnei, int n1, int axis) {
   __shared__ float s[kAtoms][4 * kQ3];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = wave_id(), lane = threadIdx.x & 63;
   const int i = blockIdx.x * kAtoms + w;
   if (i >= nloc) return;
   const int M = 4 * n1;
@@ -397,7 +397,7 @@ __global__ void __launch_bounds__(256)
                   float inv_nnei, int n1, int axis) {
   __shared__ float s[kAtoms][4 * kQ3];
   __shared__ float sd[kAtoms][128 * 16];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = wave_id(), lane = threadIdx.x & 63;
   const int i = blockIdx.x * kAtoms + w;
   if (i >= nloc) return;
   const int M = 4 * n1;
@@ -442,7 +442,7 @@ __global__ void __launch_bounds__(256)
   __shared__ float s_out[kAtoms][4 * kQ3];
   __shared__ __attribute__((aligned(16))) float s_ga[kAtoms][AXIS * 4];
   __shared__ float s_dd[kAtoms][kQ3 * kS];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = wave_id(), lane = threadIdx.x & 63;
   const int i = blockIdx.x * kAtoms + w;
   if (i >= nloc) return;
   const int M = 4 * n1;
@@ -749,7 +749,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 // slots env_mat_a_cpu treats as neighbors: it stops at the first -1).
 __global__ void __launch_bounds__(256)
     k_section_counts(int16_t* __restrict__ counts, const int* __restrict__ nlist, int nloc, EmbedParams p) {
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = wave_id(), lane = threadIdx.x & 63;
   const int i = blockIdx.x * kAtoms + w;
   if (i >= nloc) return;
   const int* nl = nlist + (int64_t)i * p.nnei;

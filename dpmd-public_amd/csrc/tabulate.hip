@@ -69,7 +69,7 @@ template <typename FP, int Q>
 __global__ void __launch_bounds__(256)
     k_tab_fwd(FP* __restrict__ out, const FP* __restrict__ table, TabInfo<FP> info, const FP* __restrict__ em_x,
               const FP* __restrict__ em, int nloc, int nnei, int M, int is_sorted) {
-  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int w = wave_id(), lane = lane_id();
   const int i = blockIdx.x * kAtoms + w;
   if (i >= nloc) return;
   const FP* ex = em_x + (int64_t)i * nnei;
@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(256)
     k_tab_grad_grad(FP* __restrict__ dz_dy, const FP* __restrict__ table, TabInfo<FP> info,
                     const FP* __restrict__ em_x, const FP* __restrict__ em, const FP* __restrict__ dz_dem_x,
                     const FP* __restrict__ dz_dem, int nloc, int nnei, int M, int is_sorted) {
-  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int w = wave_id(), lane = lane_id();
   const int i = blockIdx.x * kAtoms + w;
   if (i >= nloc) return;
   const FP* ex = em_x + (int64_t)i * nnei;
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(256)
                int M, int is_sorted) {
   extern __shared__ unsigned char smem_raw[];
   FP* dys = reinterpret_cast<FP*>(smem_raw);  // [kAtoms][4*M]
-  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int w = threadIdx.x / kWave, lane = lane_id();  // (wave_id() measured slower here)
   const int i = blockIdx.x * kAtoms + w;
   FP* dyw = dys + (int64_t)w * 4 * M;
   if (i < nloc) {
