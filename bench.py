@@ -72,9 +72,11 @@ def _issue_bound(workload: str, entry: str):
     if not f.exists() or k is None:
         return None, None
     for line in f.read_text().splitlines():
-        cols = line.split()
-        if len(cols) > 3 and cols[0] == k and cols[1] == "bound":
-            return float(cols[2]), str(f.relative_to(ROOT))
+        t = line.strip()
+        if t.startswith(k + " "):
+            cols = t[len(k):].split()
+            if len(cols) > 1 and cols[0] == "bound":
+                return float(cols[1]), str(f.relative_to(ROOT))
     return None, None
 
 
