@@ -183,6 +183,7 @@ class SeAEnergyModel:
         # the descriptor fused into the fitting kernel (axis 16, M <= 104)
         self.fused_gr = self.fused_fit is not None and self.fused_fit.supports_gr(self.fused.n1, self.axis)
         self._order_cache = None
+        self._host_order_cache = None
         # padding shortcut of the fused embedding kernels (each davg / dstd
         # section constant, so all padding slots of a section share one em row)
         self.pad_em = self._padding_rows() if (self.fused is not None or self.fused64 is not None) else None
@@ -287,7 +288,8 @@ class SeAEnergyModel:
             return self._compute_fused(atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy,
                                        order=self._type_order(atype_in, atype))
         if self.fused64 is not None:
-            return self._compute_fused64(atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy)
+            return self._compute_fused64(atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy,
+                                         atype_in=atype_in)
         em = self.env_mat(nlist, ext_c, ext_t)
         # atoms in blocks: an atom's energy and dE/d em depend on its own em
         # rows only, so the network's working set stays bounded (about
@@ -328,11 +330,7 @@ class SeAEnergyModel:
             nd, ae = self._fused_chain(em, atype, pad=self._pad_args(nlist, atype))
         elif self.fused64 is not None:
             pad = self._pad_args(nlist, atype)
-            g = self.fused64.forward(em, pad).requires_grad_(True)
-            with torch.enable_grad():
-                d = torch.bmm(g.transpose(1, 2), g[:, :, : self.axis]).reshape(nloc, -1)
-                ae = self.atom_energy(d, atype)
-                (dgr,) = torch.autograd.grad(ae.sum(), g)
+            ae, dgr = self._fit_gr64(self.fused64.forward(em, pad), atype)
             nd = self.fused64.backward(em, dgr, pad[0] if pad is not None else None)
         else:
             em = em.detach().requires_grad_(True)
@@ -344,32 +342,63 @@ class SeAEnergyModel:
         ae = ae.detach()
         return {"energy": ae.sum(), "atom_energy": ae, "force": force, "virial": virial}
 
-    def _compute_fused64(self, atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy):
+    def _compute_fused64(self, atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy, atype_in=None):
         """FP64 plain form on hand-written kernels: fused FP64 embedding + G^T R
         and its recompute adjoint; descriptor and fitting (per-type nets,
-        rocBLAS) through torch autograd in atom blocks; env-mat adjoint for
+        rocBLAS) through torch autograd (_fit_gr64); env-mat adjoint for
         forces and virial."""
         em = self.env_mat(nlist, ext_c, ext_t)
         pad = self._pad_args(nlist, atype)
         gr = self.fused64.forward(em, pad)
-        nloc = em.shape[0]
-        blk = max(1024, self.ROWS_PER_BLOCK // 64)
-        dgr = torch.empty_like(gr)
-        ae = torch.empty(nloc, dtype=torch.float64, device=em.device)
-        for a in range(0, nloc, blk):
-            b = min(nloc, a + blk)
-            g = gr[a:b].detach().requires_grad_(True)
-            with torch.enable_grad():
-                d = torch.bmm(g.transpose(1, 2), g[:, :, : self.axis]).reshape(b - a, -1)
-                ae_b = self.atom_energy(d, atype[a:b])
-                (dgr[a:b],) = torch.autograd.grad(ae_b.sum(), g)
-            ae[a:b] = ae_b.detach()
+        ae, dgr = self._fit_gr64(gr, atype, atype_in)
         nd = self.fused64.backward(em, dgr, pad[0] if pad is not None else None)
         force, virial = self.forces(nd, nlist, cells, mapping, ext_c, ext_t, box)
         out = {"energy": ae.sum(), "force": force, "virial": virial}
         if need_atom_energy:
             out["atom_energy"] = ae
         return out
+
+    def _host_type_order(self, atype_in, atype):
+        """(perm int64 [nloc] atoms ordered by type, bounds: host list of each
+        type's first sorted index + nloc), reused while the caller passes the
+        same, unmodified atype tensor (identity and version counter)."""
+        c = self._host_order_cache
+        if atype_in is not None and c is not None and c[0] is atype_in and c[1] == atype_in._version:
+            return c[2]
+        a = atype.to(torch.int64)
+        perm = torch.argsort(a, stable=True)
+        counts = torch.bincount(a, minlength=self.ntypes)[: self.ntypes].cpu().tolist()
+        bounds = [0]
+        for n in counts:
+            bounds.append(bounds[-1] + int(n))
+        order = (perm, bounds)
+        if atype_in is not None:
+            self._host_order_cache = (atype_in, atype_in._version, order)
+        return order
+
+    def _fit_gr64(self, gr, atype, atype_in=None):
+        """Descriptor D = GR^T GR< and the per-type fitting nets in FP64 with
+        their autograd adjoint, on G^T R [nloc, 4, M] gathered once into
+        type-sorted order: each type's net runs on a contiguous slice (blocks
+        of up to 2^17 atoms), and the atomic energies and dE/dGR go back to
+        the original order once.  Returns (atom_energy [nloc], dgr)."""
+        perm, bounds = self._host_type_order(atype_in, atype)
+        gs = gr.index_select(0, perm)
+        dgs = torch.empty_like(gs)
+        aes = torch.empty(gs.shape[0], dtype=torch.float64, device=gr.device)
+        blk = max(1024, self.ROWS_PER_BLOCK // 32)
+        for t in range(self.ntypes):
+            for a in range(bounds[t], bounds[t + 1], blk):
+                b = min(bounds[t + 1], a + blk)
+                g = gs[a:b].detach().requires_grad_(True)
+                with torch.enable_grad():
+                    d = torch.bmm(g.transpose(1, 2), g[:, :, : self.axis]).reshape(b - a, -1)
+                    y = self.fit[t](d, self.gemm_dtype)[:, 0].to(torch.float64)
+                    (dgs[a:b],) = torch.autograd.grad(y.sum(), g)
+                aes[a:b] = y.detach() + self.bias_atom_e[t] + self.out_bias[t]
+        ae = torch.empty_like(aes).index_copy_(0, perm, aes)
+        dgr = torch.empty_like(gr).index_copy_(0, perm, dgs)
+        return ae, dgr
 
     def _padding_rows(self):
         """pad_em [ntypes (center), ntypes (neighbor), 4] on the device, in
