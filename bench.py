@@ -129,7 +129,7 @@ def _mfma_flops(name: str, spec, nloc: int, nrows: int) -> float:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="water110k", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp16"])
@@ -268,6 +268,21 @@ def main() -> int:
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "traffic_source": traffic_src,
                     "algorithmic_bytes_per_launch": int(algo), "avg_launch_ms": round(klo[dom]["avg_ms"], 4)}
+    # whole-step roofline: the step's algorithmic FLOPs at the MFMA peak plus
+    # its HBM-bound kernels' algorithmic bytes at the HBM peak, over the
+    # measured step.  FLOPs per atom: E+F = 2 x forward, forward = embedding
+    # 2 nnei (n1 + n1 n2 + n2 n3) + G^T R 2 nnei 4 M + D 2 4 M axis + own-type
+    # fitting 2 (k1 H + 2 H^2 + H); bytes per atom: list 4 nnei, env-mat and
+    # recompute force 4 nnei (1 + 4) + 48 each (fp32 form)
+    n1, n2, n3 = spec.neuron
+    hid = spec.fitting[0]
+    fwd_flops = (2 * nnei * (n1 + n1 * n2 + n2 * n3) + 2 * nnei * 4 * n3 + 2 * 4 * n3 * spec.axis_neuron
+                 + 2 * (n3 * spec.axis_neuron * hid + 2 * hid * hid + hid))
+    step_flops = 2.0 * fwd_flops * nloc
+    step_bytes = (4 * nnei + 2 * (4 * nnei * 5 + 48)) * nloc
+    bound_s = step_flops / (MFMA_PEAK_TFLOPS[args.precision] * 1e12) + step_bytes / (HBM_PEAK_GBS * 1e9)
+    step_roofline = {"flops_per_step": step_flops, "hbm_bytes_per_step": step_bytes,
+                     "bound_ms": round(bound_s * 1e3, 4), "frac": round(bound_s / tlo, 4)}
     # the HBM-bound kernels of the same step, for reference
     line_hbm = {}
     for n, v in klo.items():
@@ -300,6 +315,7 @@ def main() -> int:
                                 args.precision: {k: round(v["total_ms"] / args.steps, 4) for k, v in klo.items()}}
         | ({} if other is None else {"fp16": {k: round(v["total_ms"] / args.steps, 4) for k, v in other[2].items()}}),
         "roofline": roofline,
+        "step_roofline": step_roofline,
         "hbm_kernels": line_hbm,
         # the env-mat derivative is recomputed inside the force kernel, never
         # stored; what storing it would take (nloc x nnei x 4 x 3 values)
