@@ -691,8 +691,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
             continue;
           }
           const float rl = f.r3[mt * 4 + v], rh = f.r3[(mt + 4) * 4 + v];
-          b_dz3[mt >> 1][(mt & 1) * 4 + v] = (H)(d_lo[v] * cst[576 + q + v] * (rl - rl * rl));
-          b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (H)(d_hi[v] * cst[640 + q + v] * (rh - rh * rh));
+          b_dz3[mt >> 1][(mt & 1) * 4 + v] = (H)(d_lo[v] * cst[576 + q + v] * fmaf(-rl, rl, rl));
+          b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (H)(d_hi[v] * cst[640 + q + v] * fmaf(-rh, rh, rh));
         }
       }
       // dh2 = dz3 W3^T + skip;  dz2 = dh2 * idt2 * (1 - t2^2)
@@ -729,7 +729,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
           if (kSkip && u == 7) continue;
           float x = d[v] + dh2[mt * 4 + v] + dh2[(mt + 2) * 4 + v];
           float tt = f.t1[u];
-          ds += x * cst[64 + q] * (1.f - tt * tt) * cst[480 + q];
+          ds += x * fmaf(-tt, tt, 1.f) * cst[480 + q];  // cst[480 + q] = idt1 w1
         }
       }
       ds += __shfl_xor(ds, 16, 64);

@@ -89,7 +89,7 @@ def pack_embedding(emb_layers: list, dtype=torch.bfloat16) -> tuple[np.ndarray, 
     operand dtype (bfloat16 or float16),
     consts float32 [ntypes, 704], n1).  consts: K2 w1 [0:32], K2 b1 [32:64],
     idt1 [64:96], K2 b2 [96:160], idt2 [160:224], K2 b3 [224:352],
-    idt3 [352:480], w1 [480:512], idt2 / K2 [512:576], 4 idt3 / K2 [576:704]
+    idt3 [352:480], idt1 w1 [480:512], idt2 / K2 [512:576], 4 idt3 / K2 [576:704]
     (the backward's dz3 = dG idt3 (1 - t3^2) is formed as dG (4 idt3) r3 (1 - r3));
     all 40 fragments hold dtype(K2 W)."""
     ntypes = len(emb_layers)
@@ -117,7 +117,7 @@ def pack_embedding(emb_layers: list, dtype=torch.bfloat16) -> tuple[np.ndarray, 
         c[160 + q2] = i2
         c[224 + q3] = K2 * b3
         c[352 + q3] = i3
-        c[480 + q1] = w1[0]
+        c[480 + q1] = i1 * w1[0]
         c[512 + q2] = i2 / K2
         c[576 + q3] = 4.0 * i3 / K2
         K = phi[lg]  # [64, 8] K slot -> feature offset within a 32-chunk

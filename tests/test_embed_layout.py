@@ -245,7 +245,7 @@ def test_backward_tile_matches_autograd(packed):
             u = mt * 4 + v
             q = phi(LG, u)
             x = d[:, v] + dh2[:, mt * 4 + v] + dh2[:, (mt + 2) * 4 + v]
-            ds += x * c[64 + q] * (1 - t1[:, u] ** 2) * c[480 + q]
+            ds += x * (1 - t1[:, u] ** 2) * c[480 + q]  # c[480 + q] = idt1 w1
     ds_tot = np.array([ds[LR == r].sum() for r in range(16)])
     nd = np.zeros((16, 4))
     nd[:, 0] = dem[LG == 0][:, 0][np.argsort(LR[LG == 0])] + ds_tot
