@@ -266,7 +266,21 @@ class SeAEnergyModel:
         d = torch.bmm(gr.transpose(1, 2), gr[:, :, : self.axis])
         return d.reshape(nloc, -1)
 
-    def atom_energy(self, desc, atype):
+    def atom_energy(self, desc, atype, order=None):
+        """Per-atom energies of desc [nloc, k1] with each type's fitting net.
+        order = (perm, bounds) of _host_type_order: one gather into
+        type-sorted rows, contiguous slices per type and one scatter back,
+        without the per-type host synchronisation of nonzero()."""
+        if order is not None:
+            perm, bounds = order
+            ds = desc.index_select(0, perm)
+            ys = []
+            for t in range(self.ntypes):
+                if bounds[t + 1] > bounds[t]:
+                    y = self.fit[t](ds[bounds[t]:bounds[t + 1]], self.gemm_dtype)[:, 0].to(torch.float64)
+                    ys.append(y + self.bias_atom_e[t] + self.out_bias[t])
+            es = torch.cat(ys) if ys else ds.new_zeros(0, dtype=torch.float64)
+            return torch.zeros(desc.shape[0], dtype=torch.float64, device=desc.device).index_add(0, perm, es)
         e = torch.empty(desc.shape[0], dtype=torch.float64, device=desc.device)
         for t in range(self.ntypes):
             idx = (atype == t).nonzero().squeeze(1)
@@ -298,11 +312,15 @@ class SeAEnergyModel:
         blk = max(256, self.ROWS_PER_BLOCK // max(self.nnei, 1))
         net_deriv = torch.empty_like(em)
         ae = torch.empty(nloc, dtype=torch.float64, device=em.device)
+        # one block: the cached type order of the atom-type tensor serves it
+        order = self._host_type_order(atype_in, atype) if nloc <= blk else None
+        if order is not None and order[1][-1] != nloc:  # types outside [0, ntypes)
+            order = None
         for a in range(0, nloc, blk):
             b = min(nloc, a + blk)
             em_b = em[a:b].detach().requires_grad_(True)
             with torch.enable_grad():
-                ae_b = self.atom_energy(self.descriptor(em_b), atype[a:b])
+                ae_b = self.atom_energy(self.descriptor(em_b), atype[a:b], order)
                 (net_deriv[a:b],) = torch.autograd.grad(ae_b.sum(), em_b)
             ae[a:b] = ae_b.detach()
         energy = ae.sum()
