@@ -304,6 +304,9 @@ class SeAEnergyModel:
         if self.fused64 is not None:
             return self._compute_fused64(atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy,
                                          atype_in=atype_in)
+        if self.tables is not None:
+            return self._compute_tab(atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy,
+                                     atype_in=atype_in)
         em = self.env_mat(nlist, ext_c, ext_t)
         # atoms in blocks: an atom's energy and dE/d em depend on its own em
         # rows only, so the network's working set stays bounded (about
@@ -376,6 +379,35 @@ class SeAEnergyModel:
             out["atom_energy"] = ae
         return out
 
+    def _compute_tab(self, atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy, atype_in=None):
+        """Compressed model (tabulated embedding) with direct launches of the
+        table kernels: tabulate_fusion_se_a per neighbor type (se_a.py:833-839)
+        summed into G^T R / nnei, the descriptor and fitting nets with their
+        adjoint (_fit_gr64), then tabulate_fusion_se_a_grad per type on
+        dE/d(em^T G) = dgr / nnei, whose em_x part adds to column 0 of
+        dE/d em (em_x is em[..., 0]); env-mat adjoint for forces and virial."""
+        em = self.env_mat(nlist, ext_c, ext_t)
+        parts, gr = [], None
+        for t in range(self.ntypes):
+            rr = em[:, self.sec[t]:self.sec[t + 1], :].contiguous()
+            ex = rr[:, :, 0].reshape(-1, 1).contiguous()
+            table, info = self.tables[t]
+            c = ops.tabulate_forward(table, info, ex, rr, self.M)
+            parts.append((table, info, ex, rr))
+            gr = c if gr is None else gr + c
+        ae, dgr = self._fit_gr64(gr / self.nnei, atype, atype_in)
+        dy = (dgr / self.nnei).contiguous()
+        nd = torch.empty_like(em)
+        for t, (table, info, ex, rr) in enumerate(parts):
+            dx, dem = ops.tabulate_grad(table, info, ex, rr, dy)
+            dem[:, :, 0] += dx.reshape(dem.shape[0], -1)
+            nd[:, self.sec[t]:self.sec[t + 1], :] = dem
+        force, virial = self.forces(nd, nlist, cells, mapping, ext_c, ext_t, box)
+        out = {"energy": ae.sum(), "force": force, "virial": virial}
+        if need_atom_energy:
+            out["atom_energy"] = ae
+        return out
+
     def _host_type_order(self, atype_in, atype):
         """(perm int64 [nloc] atoms ordered by type, bounds: host list of each
         type's first sorted index + nloc), reused while the caller passes the
@@ -395,7 +427,8 @@ class SeAEnergyModel:
         return order
 
     def _fit_gr64(self, gr, atype, atype_in=None):
-        """Descriptor D = GR^T GR< and the per-type fitting nets in FP64 with
+        """Descriptor D = GR^T GR< and the per-type fitting nets (in the dtype
+        of gr: FP64, or FP32 for an fp32 compressed model) with
         their autograd adjoint, on G^T R [nloc, 4, M] gathered once into
         type-sorted order: each type's net runs on a contiguous slice (blocks
         of up to 2^17 atoms), and the atomic energies and dE/dGR go back to
