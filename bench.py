@@ -147,8 +147,8 @@ def main() -> int:
         return copy_calibration(args.copy_bytes)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: every box in this setup has a single MI355X, so --gpus "
-              f"{max(args.gpus, world)} is not run",
+        print(f"bench.py measures one MI355X; --gpus {max(args.gpus, world)} is not supported (the multi-GPU "
+              "form, spatial shards with a halo exchange, is not built)",
               file=sys.stderr)
         return 2
 
