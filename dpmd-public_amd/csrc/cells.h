@@ -18,7 +18,9 @@ constexpr int kCandCap = 1536;
 constexpr int kCandCapBig = 3072;
 constexpr int kMaxNb = 128;
 
-// workspace status words: misc[0] nall, misc[1] capacity error,
+// workspace status words: misc[0] nall, misc[1] error bits (1: more
+// candidates than a row can hold, 2: the workspace was not set up by
+// dpmi_copy_coord_f64 for this nloc / nall or is smaller than its layout),
 // misc[2] per-atom redo count, misc[3] big-cell count
 
 struct GridParams {
@@ -34,7 +36,10 @@ struct GridParams {
   int nloc;
   long long lay_nlc;   // cell counts the workspace layout was sized for
   long long lay_next;
+  int magic;           // kGridMagic once dpmi_copy_coord_f64 has written the header
 };
+
+constexpr int kGridMagic = 0x494d5044;  // "DPMI"
 
 struct WsLayout {
   int64_t params, misc, bbox, cell_of, cell_count, cell_start, cell_cursor, cell_unordered, cell_atoms, ghost_off,

@@ -18,6 +18,11 @@ void set_error(const std::string& msg);
 // cells the workspace is sized for and the search radius in cells.
 int cell_grid_info(const double* box_host, int64_t nloc, float rcut, int64_t* nlc, int* nsearch);
 
+// Whether dpmi_build_nlist_f64 searches cell by cell for rows of nnei slots
+// (candidate tables and cand_slot), or atom by atom (every cand_slot -1, no
+// big-cell list) when the rows are too long for its LDS budget.
+bool search_uses_cells(int nnei);
+
 #define DPMI_CHECK_ARG(cond, msg)       \
   do {                                  \
     if (!(cond)) {                      \

@@ -551,7 +551,7 @@ __global__ void __launch_bounds__(256)
                   double* __restrict__ vpartial, const FP* __restrict__ nd, const int* __restrict__ nlist,
                   const int16_t* __restrict__ cand_slot, const int* __restrict__ mapping,
                   const double* __restrict__ ext_c, const int* __restrict__ ext_t, const FP* __restrict__ std_,
-                  const char* __restrict__ nlws, float rmin, float rmax, SecParams sp) {
+                  const char* __restrict__ nlws, float rmin, float rmax, SecParams sp, int nloc) {
   __shared__ int s_base[kMaxNb], s_local[kMaxNb], s_pref[kMaxNb + 1];
   __shared__ int lim_all[kAtomsPerBlock][kMaxTypes];
   extern __shared__ unsigned char smem_raw[];
@@ -572,7 +572,10 @@ __global__ void __launch_bounds__(256)
   const int* __restrict__ bigcells = reinterpret_cast<const int*>(nlws + L.bigcells);
   const int tid = threadIdx.x, w = wave_id(), lane = lane_id();
   const int side = 2 * p.nsearch + 1, nnb = side * side * side;
-  const int ncells = kBigPass ? misc[3] : p.ncell[0] * p.ncell[1] * p.ncell[2];
+  // a neighbor-list workspace of another system: no cell is processed (the
+  // gather then writes NaN forces)
+  const bool ws_ok = p.magic == kGridMagic && p.nloc == nloc;
+  const int ncells = !ws_ok ? 0 : kBigPass ? misc[3] : p.ncell[0] * p.ncell[1] * p.ncell[2];
   const int nnei = sp.nnei;
   const float inv_range = 1.f / (rmax - rmin);
   double v[9];
@@ -744,22 +747,37 @@ __global__ void __launch_bounds__(256)
 template <typename FP>
 __global__ void __launch_bounds__(256)
     k_force_gather(double* __restrict__ force, const FP* __restrict__ partial, const int* __restrict__ cell_ok,
-                   const char* __restrict__ nlws) {
+                   const char* __restrict__ nlws, int nloc) {
   const GridParams p = *reinterpret_cast<const GridParams*>(nlws);
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nloc) return;
+  if (p.magic != kGridMagic || p.nloc != nloc) {
+    const double nan = __builtin_nan("");
+    force[3 * (int64_t)o + 0] = nan;
+    force[3 * (int64_t)o + 1] = nan;
+    force[3 * (int64_t)o + 2] = nan;
+    return;
+  }
   const WsLayout L = ws_layout(p.nloc, p.lay_nlc, p.lay_next);
   const int* __restrict__ cell_of = reinterpret_cast<const int*>(nlws + L.cell_of);
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= p.nloc) return;
   const int ns = p.nsearch, side = 2 * ns + 1, nnb = side * side * side;
   const int c = cell_of[o];
   const int c2 = c % p.ncell[2], c1 = (c / p.ncell[2]) % p.ncell[1], c0 = c / (p.ncell[1] * p.ncell[2]);
   double f[3] = {0., 0., 0.};
   for (int t = 0; t < nnb; ++t) {
-    int q0 = c0 - (t / (side * side) - ns), q1 = c1 - ((t / side) % side - ns), q2 = c2 - (t % side - ns);
+    const int o0 = t / (side * side) - ns, o1 = (t / side) % side - ns, o2 = t % side - ns;
+    int q0 = c0 - o0, q1 = c1 - o1, q2 = c2 - o2;
     if (p.pbc) {
       q0 += pbc_shift(q0, p.ncell[0]) * p.ncell[0];
       q1 += pbc_shift(q1, p.ncell[1]) * p.ncell[1];
       q2 += pbc_shift(q2, p.ncell[2]) * p.ncell[2];
+      // cell q's table holds offset t only when q + offset(t) lies in the
+      // ghost range (nb_table); with nsearch > ngcell some offsets fall
+      // outside it and their partial slots are never written
+      const int i0 = q0 + o0, i1 = q1 + o1, i2 = q2 + o2;
+      if (i0 < -p.ngcell[0] || i0 >= p.ncell[0] + p.ngcell[0] || i1 < -p.ngcell[1] ||
+          i1 >= p.ncell[1] + p.ngcell[1] || i2 < -p.ngcell[2] || i2 >= p.ncell[2] + p.ngcell[2])
+        continue;
     } else if (q0 < 0 || q0 >= p.ncell[0] || q1 < 0 || q1 >= p.ncell[1] || q2 < 0 || q2 >= p.ncell[2]) {
       continue;
     }
@@ -1030,6 +1048,21 @@ int force_cells_impl(double* force, double* virial, const FP* net_deriv, const i
   int ns;
   cell_grid_info(box_host, nloc, rcut, &nlc, &ns);
   double* vpartial = reinterpret_cast<double*>(reinterpret_cast<char*>(cell_ok) + align256(4 * nlc));
+  if (!search_uses_cells(sp.nnei)) {
+    // the search went atom by atom (rows too long for its candidate tables):
+    // every cand_slot is -1 and no big-cell list exists, so the per-atom
+    // adjoint takes every neighbor term
+    const int bpa = (int)std::min<int64_t>(kForceBlocks, std::max<int64_t>(1, ceil_div(nloc, kAtomsPerBlock)));
+    if (nloc > 0)
+      hipLaunchKernelGGL((k_force_virial_recompute<FP, double>), dim3(bpa), dim3(256), 0, st, force, vpartial,
+                         net_deriv, nlist, mapping, ext_coord, ext_type, (int)nloc, std_, rcut_smth, rcut, sp);
+    else
+      hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(64), 0, st, vpartial, (int64_t)9, 0.);
+    if (virial)
+      hipLaunchKernelGGL(k_virial_reduce<double>, dim3(1), dim3(576), 0, st, virial, vpartial, nloc > 0 ? bpa : 1);
+    DPMI_LAUNCH_CHECK();
+    return DPMI_OK;
+  }
   const int bpf = (int)std::min<int64_t>(kForceBlocks, std::max<int64_t>(1, nloc));
   // LDS table: fp32 one fixed-point copy (8 B per component), fp64 one copy per wavefront
   const size_t per_slot = sizeof(FP) == 4 ? 3 * sizeof(unsigned long long) : kAtomsPerBlock * 3 * sizeof(FP);
@@ -1044,21 +1077,22 @@ int force_cells_impl(double* force, double* virial, const FP* net_deriv, const i
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
       hipLaunchKernelGGL((k_force_cells<FP, kCandCap, false, kCh>), dim3(bpf), dim3(256), shm, st, force, partial,
                          cell_ok, vpartial, net_deriv, nlist, cand_slot, mapping, ext_coord, ext_type, std_,
-                         static_cast<const char*>(nlist_workspace), rcut_smth, rcut, sp);
+                         static_cast<const char*>(nlist_workspace), rcut_smth, rcut, sp, (int)nloc);
       if constexpr (sizeof(FP) == 4) {
         // the search's big cells (second table size); its virial partials follow the first pass's
         DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_force_cells<FP, kCandCapBig, true, kCh>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_big));
         hipLaunchKernelGGL((k_force_cells<FP, kCandCapBig, true, kCh>), dim3(kBigBlocks), dim3(256), shm_big, st,
                            force, partial, cell_ok, vpartial + 9 * bpf, net_deriv, nlist, cand_slot, mapping,
-                           ext_coord, ext_type, std_, static_cast<const char*>(nlist_workspace), rcut_smth, rcut, sp);
+                           ext_coord, ext_type, std_, static_cast<const char*>(nlist_workspace), rcut_smth, rcut, sp,
+                           (int)nloc);
       }
       return DPMI_OK;
     };
     rc = sp.nnei <= 3 * kWave ? launch(std::integral_constant<int, 3>{}) : launch(std::integral_constant<int, 4>{});
     if (rc) return rc;
     hipLaunchKernelGGL((k_force_gather<FP>), dim3(ceil_div(nloc, 256)), dim3(256), 0, st, force, partial, cell_ok,
-                       static_cast<const char*>(nlist_workspace));
+                       static_cast<const char*>(nlist_workspace), (int)nloc);
   } else {
     hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(64), 0, st, vpartial, (int64_t)9, 0.);
   }

@@ -146,6 +146,19 @@ __global__ void k_write_params(GridParams p, GridParams* dst) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = p;
 }
 
+// Sets error bit 2 of misc[1] unless the workspace header was written by
+// dpmi_copy_coord_f64 for this nloc and nall and its layout fits in
+// ws_bytes; the search kernels then leave every output untouched.
+__global__ void k_ws_validate(char* ws, int nloc, int nall, long long ws_bytes) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const GridParams& p = *reinterpret_cast<const GridParams*>(ws);
+  int* misc = reinterpret_cast<int*>(ws + ws_layout(0, 0, 0).misc);
+  bool ok = p.magic == kGridMagic && p.nloc == nloc && misc[0] == nall && p.lay_nlc > 0 && p.lay_next > 0 &&
+            p.lay_nlc < (1ll << 31) && p.lay_next < (1ll << 31);
+  if (ok) ok = ws_layout(p.nloc, p.lay_nlc, p.lay_next).total <= ws_bytes;
+  if (!ok) atomicOr(&misc[1], 2);
+}
+
 // Bounding box of an open system (one workgroup).
 __global__ void __launch_bounds__(1024) k_bbox(const double* c, int n, double* out) {
   __shared__ double red[6][1024 / kWave];
@@ -369,6 +382,7 @@ __global__ void __launch_bounds__(64) k_search_format(
   const int* __restrict__ cell_atoms = reinterpret_cast<const int*>(ws + L.cell_atoms);
   const int* __restrict__ ghost_off = reinterpret_cast<const int*>(ws + L.ghost_off);
   int* misc = reinterpret_cast<int*>(ws + L.misc);
+  if (misc[1] & 2) return;  // workspace rejected by k_ws_validate
   const int i = atoms ? atoms[blockIdx.x] : (int)blockIdx.x;
   const int lane = lane_id();
   const double rcut2d = (double)(rcut * rcut);
@@ -549,6 +563,7 @@ __global__ void __launch_bounds__(256) k_search_cells(
   int16_t* kslot = kslot_all + w * (kKeyCap + 8);
   int16_t* rslot = rslot_all + w * sp.nnei;
 
+  if (misc[1] & 2) return;  // workspace rejected by k_ws_validate
   for (int ci = blockIdx.x; ci < ncells; ci += gridDim.x) {
     const int cc = kBigPass ? bigcells[ci] : ci;
     const int a0 = cell_start[cc], na = cell_start[cc + 1] - a0;
@@ -841,6 +856,14 @@ __global__ void __launch_bounds__(64) k_format_external(
 
 }  // namespace
 
+// Dynamic LDS of the cell-wise search with a table of cap candidates.
+size_t search_cells_lds(int cap, int nnei) {
+  return (size_t)cap * sizeof(float4) +
+         4 * ((size_t)(kKeyCap + 8) * (sizeof(uint64_t) + sizeof(int16_t)) + (sizeof(int) + sizeof(int16_t)) * nnei);
+}
+
+bool search_uses_cells(int nnei) { return search_cells_lds(kCandCapBig, nnei) <= kMaxCellsLds; }
+
 int cell_grid_info(const double* box_host, int64_t nloc, float rcut, int64_t* nlc, int* nsearch) {
   int64_t next;
   DPMI_CHECK_ARG(rcut > 0.f, "rcut must be positive");
@@ -921,6 +944,7 @@ extern "C" int dpmi_copy_coord_f64(double* out_coord, int* out_type, int* mappin
   }
   g.lay_nlc = nlc;
   g.lay_next = next;
+  g.magic = kGridMagic;
   const int64_t nlc_use = (int64_t)g.ncell[0] * g.ncell[1] * g.ncell[2];
   const int64_t next_use = (int64_t)g.ext[0] * g.ext[1] * g.ext[2];
   hipLaunchKernelGGL(k_write_params, dim3(1), dim3(1), 0, st, g, gp);
@@ -986,12 +1010,9 @@ extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped
   int* misc = reinterpret_cast<int*>(ws + ws_layout(0, 0, 0).misc);
   // status words misc[1..3] (capacity error, redo count, big-cell count)
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, misc + 1, (int64_t)3, 0);
-  auto shm_for = [&](int cap) {
-    return (size_t)cap * sizeof(float4) +
-           4 * ((size_t)(kKeyCap + 8) * (sizeof(uint64_t) + sizeof(int16_t)) + (sizeof(int) + sizeof(int16_t)) * sp.nnei);
-  };
-  const size_t shm_cells = shm_for(kCandCap), shm_big = shm_for(kCandCapBig);
-  if (shm_big <= kMaxCellsLds) {
+  hipLaunchKernelGGL(k_ws_validate, dim3(1), dim3(64), 0, st, ws, (int)nloc, (int)nall, (long long)workspace_bytes);
+  const size_t shm_cells = search_cells_lds(kCandCap, sp.nnei), shm_big = search_cells_lds(kCandCapBig, sp.nnei);
+  if (search_uses_cells(sp.nnei)) {
     // first pass over every cell; cells whose neighborhood exceeds its table
     // are listed for the second pass (larger table, fewer workgroups per CU)
     if (shm_cells > 65536)
@@ -1026,6 +1047,11 @@ extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped
     DPMI_HIP(hipStreamSynchronize(st));
   }
   int err = stat[0];
+  if (err & 2) {
+    set_error("the workspace was not prepared by dpmi_copy_coord_f64 for this nloc and nall, or is smaller than "
+              "dpmi_nlist_workspace_bytes gave for that system");
+    return DPMI_ERR_ARG;
+  }
   if (err) {
     set_error("more than 1024 candidates within rcut for some atom");
     return DPMI_ERR_CAPACITY;

@@ -16,6 +16,7 @@ _P = ctypes.c_void_p
 _I = ctypes.c_int
 _L = ctypes.c_int64
 _F = ctypes.c_float
+_D = ctypes.c_double
 
 # name -> argtypes (every entry point returns int status)
 SIGNATURES = {
@@ -25,6 +26,12 @@ SIGNATURES = {
     "dpmi_prod_env_mat_a_f64": [_P, _P, _P, _P, _P, _P, _L, _P, _P, _F, _F, _P, _I, _P],
     "dpmi_prod_env_mat_a_f32": [_P, _P, _P, _P, _P, _P, _L, _P, _P, _F, _F, _P, _I, _P],
     "dpmi_prod_env_mat_a_f32c64": [_P, _P, _P, _P, _P, _P, _L, _P, _P, _F, _F, _P, _I, _P],
+    "dpmi_env_mat_pt_f64": [_P] * 8 + [_L, _L, _L, _L, _I, _I, _D, _D, _D, _I, _P],
+    "dpmi_env_mat_pt_f32": [_P] * 8 + [_L, _L, _L, _L, _I, _I, _D, _D, _D, _I, _P],
+    "dpmi_env_mat_pt_grad_f64": [_P] * 8 + [_L, _L, _L, _L, _I, _I, _D, _D, _D, _I, _P],
+    "dpmi_env_mat_pt_grad_f32": [_P] * 8 + [_L, _L, _L, _L, _I, _I, _D, _D, _D, _I, _P],
+    "dpmi_env_mat_pt_grad_grad_f64": [_P] * 8 + [_L, _L, _L, _L, _I, _I, _D, _D, _D, _I, _P],
+    "dpmi_env_mat_pt_grad_grad_f32": [_P] * 8 + [_L, _L, _L, _L, _I, _I, _D, _D, _D, _I, _P],
     "dpmi_prod_force_se_a_f64": [_P, _P, _P, _P, _L, _L, _L, _L, _P],
     "dpmi_prod_force_se_a_f32": [_P, _P, _P, _P, _L, _L, _L, _L, _P],
     "dpmi_prod_virial_workspace_bytes": [_P, _L, _L],

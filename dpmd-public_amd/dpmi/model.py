@@ -184,6 +184,7 @@ class SeAEnergyModel:
         self.fused_gr = self.fused_fit is not None and self.fused_fit.supports_gr(self.fused.n1, self.axis)
         self._order_cache = None
         self._host_order_cache = None
+        self._types_checked = None
         # padding shortcut of the fused embedding kernels (each davg / dstd
         # section constant, so all padding slots of a section share one em row)
         self.pad_em = self._padding_rows() if (self.fused is not None or self.fused64 is not None) else None
@@ -291,10 +292,28 @@ class SeAEnergyModel:
         return e
 
     # ----------------------------------------------------------------- E + F + V
+    def check_types(self, atype: torch.Tensor) -> None:
+        """Raise ValueError unless every atom type lies in [0, ntypes).  The
+        check reads the range back once per atom-type tensor (cached while the
+        same, unmodified tensor comes back).  The reference treats type < 0 as
+        a virtual atom (make_atom_mask, deepmd/pt/model/atomic_model/
+        base_atomic_model.py:278-298) and indexes its per-type nets with every
+        other type; this model path takes real atoms only."""
+        c = self._types_checked
+        if c is not None and c[0] is atype and c[1] == atype._version:
+            return
+        if atype.numel():
+            lo, hi = (int(x) for x in torch.aminmax(atype.reshape(-1)))
+            if lo < 0 or hi >= self.ntypes:
+                raise ValueError(f"atom types must lie in [0, {self.ntypes}), got types {lo}..{hi}; virtual atoms "
+                                 "(type < 0) are not supported by the model path: leave them out of the frame")
+        self._types_checked = (atype, atype._version)
+
     def compute(self, coord: torch.Tensor, atype: torch.Tensor, box, need_atom_energy: bool = False):
         """One E+F+V evaluation of one frame.  coord [nloc, 3] float64 on the
-        GPU, atype [nloc] int; box: 9 values or None.  The neighbor list is
-        rebuilt every call."""
+        GPU, atype [nloc] int with every type in [0, ntypes); box: 9 values or
+        None.  The neighbor list is rebuilt every call."""
+        self.check_types(atype)
         atype_in = atype
         atype = atype.to(torch.int32)
         ext_c, ext_t, mapping, nlist, _ovf, _, cells = self.neighbor_list(coord, atype, box)
@@ -342,6 +361,7 @@ class SeAEnergyModel:
         energy, atom_energy [nloc], force [nall, 3] on every extended atom
         (ghost forces left for the caller to fold or communicate; folded onto
         local atoms when mapping is given) and virial [9]."""
+        self.check_types(ext_type)
         ext_c = ext_coord.to(torch.float64).reshape(-1, 3).contiguous()
         ext_t = ext_type.to(torch.int32).reshape(-1).contiguous()
         atype = ext_t[:nloc]

@@ -214,6 +214,126 @@ def env_mat_a(nlist, ext_coord, ext_type, avg, std, rcut, rcut_smth, sel, with_d
     return em, ed, rij
 
 
+# --------------------------------------------------------------------------- env-mat, PyTorch-backend form
+
+
+def _pt_dims(coord, nlist):
+    nf, nloc, nnei = nlist.shape
+    nall = coord.numel() // (3 * nf) if nf > 0 else 0
+    return nf, nloc, nnei, nall
+
+
+def _pt_tail(coord, nlist, cfg):
+    """Size, cutoff and stream arguments shared by the dpmi_env_mat_pt_* entry points."""
+    nf, nloc, nnei, nall = _pt_dims(coord, nlist)
+    rcut, rcut_smth, protection, exp_sw, ncomp, ntypes = cfg
+    return (nf, nloc, nall, nnei, ntypes, ncomp, rcut, rcut_smth, protection, exp_sw, _stream())
+
+
+class _EnvMatPtGrad(torch.autograd.Function):
+    """Coordinate gradient of the PyTorch-backend env-mat; differentiable
+    w.r.t. the incoming gradients (dpmi_env_mat_pt_grad_grad_*), which is
+    what the parameter gradient of a force loss needs.  The second
+    derivative w.r.t. the coordinates is not formed."""
+
+    @staticmethod
+    def forward(ctx, g_env, g_diff, g_sw, coord, nlist, atype, std, cfg):
+        ctx.set_materialize_grads(False)
+        ctx.save_for_backward(coord, nlist, atype, std)
+        ctx.cfg = cfg
+        ctx.has = (g_env is not None, g_diff is not None, g_sw is not None)
+        ctx.shapes = tuple(None if g is None else g.shape for g in (g_env, g_diff, g_sw))
+        nf, nloc, nnei, nall = _pt_dims(coord, nlist)
+        gc = torch.empty((nf, nall, 3), dtype=coord.dtype, device=coord.device)
+        g_env, g_diff, g_sw = (None if g is None else _c(g, coord.dtype) for g in (g_env, g_diff, g_sw))
+        _launch(f"dpmi_env_mat_pt_grad_{_sfx(coord.dtype)}", ptr(gc), ptr(g_env), ptr(g_diff), ptr(g_sw),
+                ptr(coord), ptr(nlist), ptr(atype), ptr(std), *_pt_tail(coord, nlist, cfg))
+        return gc
+
+    @staticmethod
+    def backward(ctx, v):
+        if v is None:
+            return (None,) * 8
+        coord, nlist, atype, std = ctx.saved_tensors
+        cfg = ctx.cfg
+        nf, nloc, nnei, nall = _pt_dims(coord, nlist)
+        ncomp = cfg[4]
+        need = ctx.needs_input_grad
+        ge = torch.empty((nf, nloc, nnei, ncomp), dtype=coord.dtype, device=coord.device) if need[0] else None
+        gd = torch.empty((nf, nloc, nnei, 3), dtype=coord.dtype, device=coord.device) if need[1] else None
+        gs = torch.empty((nf, nloc, nnei, 1), dtype=coord.dtype, device=coord.device) if need[2] else None
+        _launch(f"dpmi_env_mat_pt_grad_grad_{_sfx(coord.dtype)}", ptr(ge), ptr(gd), ptr(gs),
+                ptr(_c(v, coord.dtype).reshape(nf, nall, 3)), ptr(coord), ptr(nlist), ptr(atype), ptr(std),
+                *_pt_tail(coord, nlist, cfg))
+        shp = ctx.shapes
+        return (ge.reshape(shp[0]) if ge is not None else None, gd.reshape(shp[1]) if gd is not None else None,
+                gs.reshape(shp[2]) if gs is not None else None, None, None, None, None, None)
+
+
+class _EnvMatPt(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, coord, nlist, atype, mean, std, cfg):
+        ctx.set_materialize_grads(False)
+        nf, nloc, nnei, nall = _pt_dims(coord, nlist)
+        ncomp = cfg[4]
+        dt, dev = coord.dtype, coord.device
+        env = torch.empty((nf, nloc, nnei, ncomp), dtype=dt, device=dev)
+        diff = torch.empty((nf, nloc, nnei, 3), dtype=dt, device=dev)
+        sw = torch.empty((nf, nloc, nnei, 1), dtype=dt, device=dev)
+        _launch(f"dpmi_env_mat_pt_{_sfx(dt)}", ptr(env), ptr(diff), ptr(sw), ptr(coord), ptr(nlist), ptr(atype),
+                ptr(mean), ptr(std), *_pt_tail(coord, nlist, cfg))
+        ctx.save_for_backward(coord, nlist, atype, std)
+        ctx.cfg = cfg
+        return env, diff, sw
+
+    @staticmethod
+    def backward(ctx, g_env, g_diff, g_sw):
+        coord, nlist, atype, std = ctx.saved_tensors
+        if g_env is None and g_diff is None and g_sw is None:
+            return (None,) * 6
+        gc = _EnvMatPtGrad.apply(g_env, g_diff, g_sw, coord, nlist, atype, std, ctx.cfg)
+        return gc, None, None, None, None, None
+
+
+def prod_env_mat(extended_coord, nlist, atype, mean, stddev, rcut: float, rcut_smth: float, radial_only: bool = False,
+                 protection: float = 0.0, use_exp_switch: bool = False):
+    """Drop-in for deepmd.pt.model.descriptor.env_mat.prod_env_mat
+    (env_mat.py:51-92), which DescrptBlockSeA.forward calls
+    (se_a.py:781-790): (env_mat [nf, nloc, nnei, 4 or 1], diff
+    [nf, nloc, nnei, 3], switch [nf, nloc, nnei, 1]) from extended_coord
+    [nf, nall * 3] (or [nf, nall, 3]), nlist [nf, nloc, nnei] (any order,
+    -1 padding, entries beyond rcut allowed), atype [nf, nloc] and
+    mean / stddev [ntypes, nnei, 4 or 1].  Differentiable w.r.t.
+    extended_coord (HIP kernels for the gradient and for its own backward
+    w.r.t. the incoming gradients, so an E + F loss trains through it);
+    float64 or float32."""
+    _gpu(extended_coord, nlist, atype, mean, stddev)
+    dt = extended_coord.dtype
+    _sfx(dt)
+    if nlist.dim() != 3:
+        raise ValueError("nlist must be [nframes, nloc, nnei]")
+    nf, nloc, nnei = nlist.shape
+    ncomp = 1 if radial_only else 4
+    if mean.shape[-1] != ncomp or stddev.shape[-1] != ncomp:
+        raise ValueError(f"mean / stddev must end in {ncomp} components")
+    ntypes = mean.shape[0]
+    if mean.numel() != ntypes * nnei * ncomp or stddev.numel() != mean.numel():
+        raise ValueError("mean / stddev must be [ntypes, nnei, ncomp]")
+    if atype.numel() != nf * nloc:
+        raise ValueError("atype must be [nframes, nloc]")
+    if extended_coord.numel() % (3 * max(nf, 1)) != 0:
+        raise ValueError("extended_coord must be [nframes, nall * 3]")
+    if float(rcut_smth) >= float(rcut):
+        raise ValueError("rmin should be less than rmax.")
+    coord = extended_coord.reshape(nf, -1, 3)
+    if not coord.is_contiguous():
+        coord = coord.contiguous()
+    cfg = (float(rcut), float(rcut_smth), float(protection), int(bool(use_exp_switch)), ncomp, int(ntypes))
+    env, diff, sw = _EnvMatPt.apply(coord, _c(nlist, torch.int32), _c(atype, torch.int32).reshape(nf, nloc),
+                                    _c(mean, dt), _c(stddev, dt), cfg)
+    return env, diff, sw
+
+
 def force_virial_recompute(net_deriv, nlist, mapping, ext_coord, ext_type, std, rcut, rcut_smth, sel,
                            want_virial=True):
     """Adjoint of the env-mat: forces (-dE/dx) from dE/d em, with the env-mat
@@ -237,10 +357,12 @@ def force_virial_recompute(net_deriv, nlist, mapping, ext_coord, ext_type, std, 
 
 
 def force_virial_cells(net_deriv, nlist, cells, mapping, ext_coord, ext_type, std, rcut, rcut_smth, sel, box,
-                       want_virial=True):
+                       want_virial=True, workspace=None):
     """force_virial_recompute for a list from build_nlist(..., cells=True)
     (cells = its (cand_slot, workspace)): no memory atomics for the neighbor
-    forces, deterministic.  Forces folded onto local atoms."""
+    forces, deterministic.  Forces folded onto local atoms.  workspace
+    (optional): a uint8 device tensor of at least
+    dpmi_force_virial_cells_workspace_bytes, whose contents are ignored."""
     _gpu(net_deriv, nlist, ext_coord, ext_type, std)
     sfx = _sfx(net_deriv.dtype)
     cslot, nlws = cells
@@ -253,7 +375,12 @@ def force_virial_cells(net_deriv, nlist, cells, mapping, ext_coord, ext_type, st
     wsb = ctypes.c_int64(0)
     call("dpmi_force_virial_cells_workspace_bytes", ctypes.byref(wsb), ptr(b) if b is not None else None, nloc,
          rcut, int(net_deriv.dtype == torch.float64))
-    ws = torch.empty(int(wsb.value), dtype=torch.uint8, device=dev)
+    if workspace is None:
+        ws = torch.empty(int(wsb.value), dtype=torch.uint8, device=dev)
+    else:
+        if workspace.numel() < int(wsb.value) or workspace.dtype != torch.uint8:
+            raise ValueError(f"workspace needs {int(wsb.value)} bytes (uint8)")
+        ws = workspace
     s = _sel_host(sel)
     _launch(f"dpmi_force_virial_cells_{sfx}", ptr(force), ptr(virial), ptr(_c(net_deriv)),
          ptr(_c(nlist, torch.int32)), ptr(cslot), ptr(_c(mapping, torch.int32)) if mapping is not None else None,

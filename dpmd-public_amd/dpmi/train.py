@@ -79,35 +79,56 @@ def energy_force(model, batch: Batch, create_graph: bool):
 class EnergyStdLoss:
     """loss = pref_e / N * mean_f (E - E*)^2 + pref_f * mean (F - F*)^2
     (EnergyStdLoss.forward, deepmd/pt/loss/ener.py:243-271,302-328 with
-    loss_func "mse", no Huber, intensive_ener_virial False)."""
+    loss_func "mse", no Huber, intensive_ener_virial False).
 
-    def __init__(self, pref_e: float = 0.02, pref_f: float = 1000.0):
-        self.pref_e = float(pref_e)
-        self.pref_f = float(pref_f)
+    The prefactors follow the learning rate as in the reference
+    (ener.py:229-231): pref = limit + (start - limit) * lr / starter_lr.
+    limit_pref_* default to start_pref_*, which keeps them constant."""
 
-    def __call__(self, energy, force, energy_label, force_label, natoms: int):
+    def __init__(self, start_pref_e: float = 0.02, start_pref_f: float = 1000.0, limit_pref_e: float | None = None,
+                 limit_pref_f: float | None = None, starter_learning_rate: float = 1.0):
+        self.start_pref_e = float(start_pref_e)
+        self.start_pref_f = float(start_pref_f)
+        self.limit_pref_e = self.start_pref_e if limit_pref_e is None else float(limit_pref_e)
+        self.limit_pref_f = self.start_pref_f if limit_pref_f is None else float(limit_pref_f)
+        self.starter_learning_rate = float(starter_learning_rate)
+
+    def prefactors(self, learning_rate: float | None = None):
+        """(pref_e, pref_f) at learning_rate (the starter rate when None)."""
+        lr = self.starter_learning_rate if learning_rate is None else float(learning_rate)
+        coef = lr / self.starter_learning_rate
+        pref_e = self.limit_pref_e + (self.start_pref_e - self.limit_pref_e) * coef
+        pref_f = self.limit_pref_f + (self.start_pref_f - self.limit_pref_f) * coef
+        return pref_e, pref_f
+
+    def __call__(self, energy, force, energy_label, force_label, natoms: int, learning_rate: float | None = None):
+        pref_e, pref_f = self.prefactors(learning_rate)
         l2_e = torch.mean(torch.square(energy - energy_label))
         diff_f = (force_label - force).reshape(-1)
         l2_f = torch.mean(torch.square(diff_f))
-        loss = (1.0 / natoms) * (self.pref_e * l2_e) + (self.pref_f * l2_f).to(torch.float64)
+        loss = (1.0 / natoms) * (pref_e * l2_e) + (pref_f * l2_f).to(torch.float64)
         return loss, {"rmse_e": l2_e.detach().sqrt() / natoms, "rmse_f": l2_f.detach().sqrt()}
 
 
 class Trainer:
     """Adam on the model's trainable tensors (embedding and fitting nets;
-    the fitting nets only with a tabulated embedding)."""
+    the fitting nets only with a tabulated embedding).  The loss prefactors
+    move with the optimizer's current learning rate (lr of its first param
+    group) from start_pref_* at lr to limit_pref_*, as the reference's."""
 
-    def __init__(self, model, lr: float = 1e-3, pref_e: float = 0.02, pref_f: float = 1000.0):
+    def __init__(self, model, lr: float = 1e-3, pref_e: float = 0.02, pref_f: float = 1000.0,
+                 limit_pref_e: float | None = None, limit_pref_f: float | None = None):
         self.model = model
         self.params = model.parameters()
-        self.loss = EnergyStdLoss(pref_e, pref_f)
+        self.loss = EnergyStdLoss(pref_e, pref_f, limit_pref_e, limit_pref_f, starter_learning_rate=lr)
         self.opt = torch.optim.Adam(self.params, lr=lr)
 
     def loss_and_grad(self, batch: Batch, energy_label, force_label):
         for p in self.params:
             p.grad = None
         energy, force = energy_force(self.model, batch, create_graph=True)
-        loss, more = self.loss(energy, force, energy_label, force_label, batch.nloc)
+        lr = self.opt.param_groups[0]["lr"]
+        loss, more = self.loss(energy, force, energy_label, force_label, batch.nloc, learning_rate=lr)
         loss.backward()
         return loss.detach(), more
 

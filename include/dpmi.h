@@ -99,6 +99,61 @@ int dpmi_prod_env_mat_a_f32c64(float* em, float* em_deriv, float* rij, const int
                                const int* ext_type, int64_t nloc, const float* avg, const float* std, float rcut,
                                float rcut_smth, const int* sel_host, int ntypes, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Environment matrix with the PyTorch backend's semantics: replaces
+ * prod_env_mat / _make_env_mat (deepmd/pt/model/descriptor/env_mat.py:11-92,
+ * smooth weights deepmd/pt/utils/preprocess.py:9-34), the function
+ * DescrptBlockSeA.forward calls (deepmd/pt/model/descriptor/se_a.py:781-790),
+ * and the two products its autograd forms.  Any list: nlist [nframes, nloc,
+ * nnei] of extended indices in any order, -1 padding anywhere, neighbors
+ * beyond rcut allowed (smooth weight 0).  ext_coord [nframes, nall, 3];
+ * atype [nframes, nloc] in [0, ntypes); mean / stddev [ntypes, nnei, ncomp]
+ * with ncomp = 4, or 1 for radial_only.  Per slot, d = x_j - x_i, r = |d|,
+ * q = r + protection, w = compute_smooth_weight(r, rcut_smth, rcut) (or
+ * compute_exp_sw with use_exp_switch):
+ *   env_mat [nframes, nloc, nnei, ncomp] = ([1/q, d/q^2] w - mean) / stddev
+ *   diff [nframes, nloc, nnei, 3] = d, sw [nframes, nloc, nnei] = w
+ * (padding slots: env_mat = -mean / stddev, diff = 0, sw = 0).  diff and sw
+ * may be NULL.  A neighbor index >= nall or a type outside [0, ntypes) gives
+ * NaN in that row.  rcut_smth >= rcut is an error, as in the reference.
+ * ------------------------------------------------------------------------ */
+int dpmi_env_mat_pt_f64(double* env_mat, double* diff, double* sw, const double* ext_coord, const int* nlist,
+                        const int* atype, const double* mean, const double* stddev, int64_t nframes, int64_t nloc,
+                        int64_t nall, int64_t nnei, int ntypes, int ncomp, double rcut, double rcut_smth,
+                        double protection, int use_exp_switch, void* stream);
+int dpmi_env_mat_pt_f32(float* env_mat, float* diff, float* sw, const float* ext_coord, const int* nlist,
+                        const int* atype, const float* mean, const float* stddev, int64_t nframes, int64_t nloc,
+                        int64_t nall, int64_t nnei, int ntypes, int ncomp, double rcut, double rcut_smth,
+                        double protection, int use_exp_switch, void* stream);
+/* Coordinate gradient (the backward of the function above):
+ * grad_coord [nframes, nall, 3] = sum over slots of the derivative of
+ * <g_env_mat, env_mat> + <g_diff, diff> + <g_sw, sw> w.r.t. the extended
+ * coordinates.  Any of g_env_mat / g_diff / g_sw may be NULL (zero).
+ * grad_coord is overwritten.  Neighbor terms are added with atomics, so the
+ * last bits depend on timing, as the reference's autograd scatter does. */
+int dpmi_env_mat_pt_grad_f64(double* grad_coord, const double* g_env_mat, const double* g_diff, const double* g_sw,
+                             const double* ext_coord, const int* nlist, const int* atype, const double* stddev,
+                             int64_t nframes, int64_t nloc, int64_t nall, int64_t nnei, int ntypes, int ncomp,
+                             double rcut, double rcut_smth, double protection, int use_exp_switch, void* stream);
+int dpmi_env_mat_pt_grad_f32(float* grad_coord, const float* g_env_mat, const float* g_diff, const float* g_sw,
+                             const float* ext_coord, const int* nlist, const int* atype, const float* stddev,
+                             int64_t nframes, int64_t nloc, int64_t nall, int64_t nnei, int ntypes, int ncomp,
+                             double rcut, double rcut_smth, double protection, int use_exp_switch, void* stream);
+/* Backward of the coordinate gradient w.r.t. (g_env_mat, g_diff, g_sw)
+ * applied to v = dL/d grad_coord [nframes, nall, 3] (the double backward of
+ * an energy + force loss): gg_env_mat [.., ncomp] = J_env v, gg_diff [.., 3],
+ * gg_sw [..] (each may be NULL).  The second derivative w.r.t. the
+ * coordinates is not formed, as the reference's registered op gradients do
+ * not form it. */
+int dpmi_env_mat_pt_grad_grad_f64(double* gg_env_mat, double* gg_diff, double* gg_sw, const double* v,
+                                  const double* ext_coord, const int* nlist, const int* atype, const double* stddev,
+                                  int64_t nframes, int64_t nloc, int64_t nall, int64_t nnei, int ntypes, int ncomp,
+                                  double rcut, double rcut_smth, double protection, int use_exp_switch, void* stream);
+int dpmi_env_mat_pt_grad_grad_f32(float* gg_env_mat, float* gg_diff, float* gg_sw, const float* v,
+                                  const float* ext_coord, const int* nlist, const int* atype, const float* stddev,
+                                  int64_t nframes, int64_t nloc, int64_t nall, int64_t nnei, int ntypes, int ncomp,
+                                  double rcut, double rcut_smth, double protection, int use_exp_switch, void* stream);
+
 /* format_nlist_cpu (source/lib/src/fmt_nlist.cc:145-177; the formatting step
  * of forward_lower with an external list, make_model.py:334-415, and of the
  * LAMMPS / C++ path, DeepPotPT.cc:180-330): row i of cand (width ncand,

@@ -101,6 +101,68 @@ class _TabulateGrad(torch.autograd.Function):
         return None, None, torch.as_tensor(dz), None, None
 
 
+def smooth_weight(distance, rmin, rmax):
+    """compute_smooth_weight (deepmd/pt/utils/preprocess.py:9-19)."""
+    if rmin >= rmax:
+        raise ValueError("rmin should be less than rmax.")
+    distance = torch.clamp(distance, min=rmin, max=rmax)
+    uu = (distance - rmin) / (rmax - rmin)
+    uu2 = uu * uu
+    return uu2 * uu * (-6 * uu2 + 15 * uu - 10) + 1
+
+
+def exp_switch(distance, rmin, rmax):
+    """compute_exp_sw (deepmd/pt/utils/preprocess.py:22-34), C = 20."""
+    if rmin >= rmax:
+        raise ValueError("rmin should be less than rmax.")
+    distance = torch.clamp(distance, min=0.0, max=rmax)
+    a = 20 / rmin
+    return torch.exp(-torch.exp(a * (distance - rmin)))
+
+
+def prod_env_mat_pt(extended_coord, nlist, atype, mean, stddev, rcut, rcut_smth, radial_only=False,
+                    protection=0.0, use_exp_switch=False):
+    """prod_env_mat / _make_env_mat of the PyTorch backend
+    (deepmd/pt/model/descriptor/env_mat.py:11-92), step for step, on CPU
+    tensors (differentiable w.r.t. extended_coord through torch autograd):
+    padding indices point at an extra coordinate coord[-1] + rcut, padded
+    rows get length + 1 and weight 0, and the result is normalised by the
+    centre type's mean / stddev.  Returns (env_mat, diff, switch)."""
+    bsz, natoms, nnei = nlist.shape
+    coord = extended_coord.view(bsz, -1, 3)
+    nall = coord.shape[1]
+    mask = nlist >= 0
+    nl = torch.where(mask, nlist, nall)
+    coord_l = coord[:, :natoms].view(bsz, -1, 1, 3)
+    index = nl.view(bsz, -1).unsqueeze(-1).expand(-1, -1, 3)
+    coord_pad = torch.concat([coord, coord[:, -1:, :] + rcut], dim=1)
+    coord_r = torch.gather(coord_pad, 1, index).view(bsz, natoms, nnei, 3)
+    diff = coord_r - coord_l
+    length = torch.linalg.norm(diff, dim=-1, keepdim=True)
+    length = length + ~mask.unsqueeze(-1)
+    t0 = 1 / (length + protection)
+    t1 = diff / (length + protection) ** 2
+    sw = exp_switch(length, rcut_smth, rcut) if use_exp_switch else smooth_weight(length, rcut_smth, rcut)
+    weight = sw * mask.unsqueeze(-1)
+    env = t0 * weight if radial_only else torch.cat([t0, t1], dim=-1) * weight
+    env = (env - mean[atype]) / stddev[atype]
+    return env, diff * mask.unsqueeze(-1), weight
+
+
+def loss_prefactors(start_pref_e, start_pref_f, limit_pref_e=None, limit_pref_f=None, learning_rate=None,
+                    starter_learning_rate=None):
+    """(pref_e, pref_f) of EnergyStdLoss.forward (deepmd/pt/loss/ener.py:
+    229-231): coef = learning_rate / starter_learning_rate and
+    pref = limit + (start - limit) * coef.  Without a learning rate, or
+    without limits, the start prefactors."""
+    if learning_rate is None:
+        return float(start_pref_e), float(start_pref_f)
+    coef = float(learning_rate) / float(starter_learning_rate)
+    le = start_pref_e if limit_pref_e is None else limit_pref_e
+    lf = start_pref_f if limit_pref_f is None else limit_pref_f
+    return le + (start_pref_e - le) * coef, lf + (start_pref_f - lf) * coef
+
+
 class Model:
     """se_e2_a + energy fitting, from a dict in the reference serialization
     layout (the 'model' block of deeppot_sea.yaml) or from dpmi.systems
@@ -223,10 +285,15 @@ class Model:
         f = torch.zeros((len(atype), 3), dtype=torch.float64).index_add(0, torch.as_tensor(mp, dtype=torch.long), -g)
         return energy, f
 
-    def train_loss(self, frames, energy_label, force_label, pref_e=0.02, pref_f=1000.0, params=None):
+    def train_loss(self, frames, energy_label, force_label, pref_e=0.02, pref_f=1000.0, params=None,
+                   limit_pref_e=None, limit_pref_f=None, learning_rate=None, starter_learning_rate=None):
         """EnergyStdLoss (deepmd/pt/loss/ener.py:243-271,302-328; mse, 1/N
         energy normalisation) over frames [(coord, atype, box)], and with
-        params (from parameters()) its gradient by a double backward."""
+        params (from parameters()) its gradient by a double backward.
+        pref_e / pref_f are the start prefactors; with a learning rate they
+        move to limit_pref_* as ener.py:229-231 moves them."""
+        pref_e, pref_f = loss_prefactors(pref_e, pref_f, limit_pref_e, limit_pref_f, learning_rate,
+                                         starter_learning_rate)
         es, fs = [], []
         for c, t, b in frames:
             e, f = self.energy_force(c, t, b, create_graph=params is not None)

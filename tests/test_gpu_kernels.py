@@ -155,6 +155,30 @@ def test_env_mat_f32(oracle):
     np.testing.assert_allclose(ed.cpu().numpy(), oed, rtol=1e-4, atol=1e-5)
 
 
+@pytest.mark.parametrize("name,make,rcut,rs,sel", [
+    ("water192", lambda: systems.water192(0), 6.0, 0.5, [46, 92]),
+    ("alloy_triclinic_4k", lambda: systems.alloy262k(4, ncell=16), 8.0, 2.0, [60, 60, 60, 60]),
+    ("water192_rcut9", lambda: systems.water192(0), 9.0, 0.5, [110, 220]),
+])
+def test_env_mat_f32c64(oracle, name, make, rcut, rs, sel):
+    """The precision form's env-mat (double coordinates, x_j - x_i in double,
+    the rest in float): within a few float roundings of the FP64 oracle, em,
+    its derivative and r_ij."""
+    s = make()
+    _say(f"env_mat f32c64 {name}")
+    oc, ot, om, onl, avg, std = _env_inputs(oracle, s, rcut, sel)
+    em, ed, rij = ops.env_mat_a(T(onl), T(oc), T(ot), T(avg, torch.float32), T(std, torch.float32), rcut, rs, sel,
+                                dtype=torch.float32)
+    oem, oed, orij = oracle.env_mat_a(onl, oc, ot, avg.astype(np.float32).astype(np.float64),
+                                      std.astype(np.float32).astype(np.float64), rcut, rs, sel)
+    for got, want in ((em, oem), (ed, oed), (rij, orij)):
+        got = got.cpu().numpy().astype(np.float64).reshape(want.shape)
+        err = np.abs(got - want).max() / np.abs(want).max()
+        assert err <= 1e-6, err
+    # r_ij is the double difference rounded once
+    np.testing.assert_array_equal(rij.cpu().numpy().reshape(orij.shape), orij.astype(np.float32))
+
+
 def test_prod_force_virial_f64(oracle):
     s = systems.water192(0)
     _say("prod_force/prod_virial f64 water192")
@@ -218,7 +242,25 @@ CELLS_CASES = [
     # atoms and cells the search handles one by one: neighbor terms by atomics
     ("clustered", _clustered, 6.0, [200, 200]),
     ("water12k_rcut9", lambda: systems.water12k(1), 9.0, [110, 220]),
+    # box side just above 8 rc: cells only just wider than rc search two
+    # cells out (nsearch 2) while the ghost region is one cell deep, so some
+    # neighbor-cell offsets of a cell's table hold no atoms
+    ("box_just_over_8rc", lambda: systems._water_like("w48", 16, 48.0001, 1, True), 6.0, [46, 92]),
 ]
+
+
+def _poisoned_cells_workspace(s, rcut, dtype):
+    """Force workspace filled with 0xFF bytes (NaN in fp32 and fp64), so a
+    partial slot read before it is written shows up as NaN forces."""
+    import ctypes
+
+    from dpmi import _lib
+
+    n = ctypes.c_int64(0)
+    box = np.ascontiguousarray(np.asarray(s.box, dtype=np.float64))
+    _lib.call("dpmi_force_virial_cells_workspace_bytes", ctypes.byref(n), _lib.ptr(box), s.nloc, rcut,
+              int(dtype == torch.float64))
+    return torch.full((int(n.value),), 255, dtype=torch.uint8, device=DEV)
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
@@ -237,12 +279,13 @@ def test_force_virial_cells(oracle, name, make, rcut, sel, dtype):
     nd = rng.normal(size=(s.nloc, nnei, 4))
     f0, v0 = ops.force_virial_recompute(T(nd, dtype), nlist, mapping, ext_c, ext_t, T(std, dtype), rcut, 0.5, sel)
     f1, v1 = ops.force_virial_cells(T(nd, dtype), nlist, cells, mapping, ext_c, ext_t, T(std, dtype), rcut, 0.5,
-                                    sel, s.box)
+                                    sel, s.box, workspace=_poisoned_cells_workspace(s, rcut, dtype))
     f0, v0, f1, v1 = (x.cpu().numpy() for x in (f0, v0, f1, v1))
+    assert np.isfinite(f1).all() and np.isfinite(v1).all()
     tol = 1e-12 if dtype == torch.float64 else 1e-5
     np.testing.assert_allclose(f1, f0, rtol=tol, atol=tol * np.abs(f0).max())
     np.testing.assert_allclose(v1, v0, rtol=tol, atol=tol * np.abs(v0).max())
-    if dtype == torch.float64 and name == "water12k":
+    if dtype == torch.float64 and name in ("water12k", "box_just_over_8rc"):
         oc, ot, om, onl, _ = oracle.build_nlist_pbc(s.coord, s.atype, s.box, rcut, sel)
         avg = np.zeros_like(std)
         _, oed, orij = oracle.env_mat_a(onl, oc, ot, avg, std, rcut, 0.5, sel)
@@ -259,6 +302,52 @@ def test_force_virial_cells(oracle, name, make, rcut, sel, dtype):
                                     sel, s.box)
     np.testing.assert_array_equal(f2.cpu().numpy(), f1)
     np.testing.assert_array_equal(v2.cpu().numpy(), v1)
+
+
+def test_force_virial_cells_long_rows():
+    """sum(sel) = 3600: the search goes atom by atom (its candidate tables do
+    not fit in LDS for such rows) and the cell-wise force entry point must
+    then take every neighbor term itself."""
+    s = systems.water192(0)
+    sel = [1800, 1800]
+    _say("force/virial cells sum(sel)=3600")
+    ext_c, ext_t, mapping, nlist, ovf, _, cells = ops.build_nlist(T(s.coord), T(s.atype), s.box, 6.0, sel,
+                                                                  cells=True)
+    assert (cells[0] == -1).all()
+    rng = np.random.default_rng(3)
+    std = 0.5 + rng.uniform(size=(2, 3600, 4))
+    nd = rng.normal(size=(s.nloc, 3600, 4))
+    for dtype, tol in ((torch.float64, 1e-12), (torch.float32, 1e-5)):
+        f0, v0 = ops.force_virial_recompute(T(nd, dtype), nlist, mapping, ext_c, ext_t, T(std, dtype), 6.0, 0.5, sel)
+        f1, v1 = ops.force_virial_cells(T(nd, dtype), nlist, cells, mapping, ext_c, ext_t, T(std, dtype), 6.0, 0.5,
+                                        sel, s.box, workspace=_poisoned_cells_workspace(s, 6.0, dtype))
+        f0, v0, f1, v1 = (x.cpu().numpy() for x in (f0, v0, f1, v1))
+        assert np.abs(f0).max() > 0
+        np.testing.assert_allclose(f1, f0, rtol=tol, atol=tol * np.abs(f0).max())
+        np.testing.assert_allclose(v1, v0, rtol=tol, atol=tol * np.abs(v0).max())
+
+
+def test_build_nlist_rejects_foreign_workspace():
+    """A neighbor-list workspace prepared for another system (other nloc) or
+    never prepared is refused before any list is written."""
+    import ctypes
+
+    from dpmi import _lib
+
+    a, b = systems.water192(0), systems.water12k(1)
+    _say("build_nlist foreign workspace")
+    sel = np.array([46, 92], dtype=np.int32)
+    ext_c, ext_t, mapping, nlist, ovf, _, (cslot, ws) = ops.build_nlist(T(b.coord), T(b.atype), b.box, 6.0, sel,
+                                                                         cells=True)
+    a_ext, a_t, a_map, a_nl, _, _ = ops.build_nlist(T(a.coord), T(a.atype), a.box, 6.0, sel)
+    out = torch.full((a.nloc, 138), -7, dtype=torch.int32, device=DEV)
+    ovf_a = torch.empty(a.nloc, dtype=torch.int32, device=DEV)
+    for wsp in (ws, torch.zeros_like(ws)):
+        with pytest.raises(_lib.DpmiError, match="workspace"):
+            _lib.call("dpmi_build_nlist_f64", _lib.ptr(out), _lib.ptr(ovf_a), None, None, _lib.ptr(a_ext),
+                      _lib.ptr(a_t), _lib.ptr(a_map), a.nloc, a_ext.shape[0], 6.0, _lib.ptr(sel), 2, _lib.ptr(wsp),
+                      wsp.numel(), ops._stream())
+        assert (out == -7).all()
 
 
 def test_force_virial_cells_open_system():
