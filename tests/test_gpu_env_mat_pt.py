@@ -127,7 +127,7 @@ def test_coordinate_gradient_matches_oracle(oracle, name, kw):
     c, (e, d, w) = _gpu(ext, nl, at, mean, std, kw, grad=True)
     L = (e * torch.as_tensor(ge, device=DEV)).sum() + (d * torch.as_tensor(gd, device=DEV)).sum() + \
         (w * torch.as_tensor(gw, device=DEV)).sum()
-    (g,) = torch.autograd.grad(L, c)
+    (g,) = torch.autograd.grad(L, c, retain_graph=True)
     assert g.shape == c.shape
     np.testing.assert_allclose(g.cpu().numpy(), og.numpy(), rtol=1e-11, atol=1e-11 * np.abs(og.numpy()).max())
     # env-mat alone (diff and switch unused: their gradients are absent)

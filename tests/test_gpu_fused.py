@@ -33,6 +33,16 @@ def setup(request):
     return spec, params, m64, mbf, s, c, t
 
 
+# error budgets, a few times what the water192 frame measures on MI355X
+# (bf16 / fp16): descriptor 1.8e-3 / 2.1e-4 rel RMS, dE/d em 3.4e-3 / 4.5e-4,
+# force RMSE 4.2e-5 / 5.3e-6 eV/A, |dE|/atom 2.3e-5 / 1.4e-5 eV, fitting on
+# random descriptors max |de| 1.7e-3 / 2.1e-4 eV and dE/d desc 2.4e-3 / 3.0e-4
+BUDGET = {
+    "bf16": dict(desc=5e-3, dem=8e-3, rmse=1e-4, de=1e-4, fit_e=5e-3, fit_d=6e-3),
+    "fp16": dict(desc=6e-4, dem=1.2e-3, rmse=2e-5, de=5e-5, fit_e=6e-4, fit_d=1e-3),
+}
+
+
 def _rel_rms(a, b):
     return (torch.sqrt(((a - b) ** 2).mean()) / torch.sqrt((b**2).mean())).item()
 
@@ -46,7 +56,7 @@ def test_fused_descriptor_forward(setup):
     gr, d = mbf.fused.forward(em64.float())
     err = _rel_rms(d.double(), d64)
     print(f"descriptor rel RMS error {err:.2e}", flush=True)
-    assert err < 1e-2
+    assert err < BUDGET[mbf.precision]["desc"]
 
 
 def test_fused_embedding_backward(setup):
@@ -61,7 +71,7 @@ def test_fused_embedding_backward(setup):
     nd = mbf.fused.backward(em64.detach().float(), gr, w.float())
     err = _rel_rms(nd.double(), g64)
     print(f"dE/d em rel RMS error {err:.2e}", flush=True)
-    assert err < 2e-2
+    assert err < BUDGET[mbf.precision]["dem"]
 
 
 def test_fused_model_force_rmse(setup):
@@ -75,8 +85,8 @@ def test_fused_model_force_rmse(setup):
     dv = (torch.abs(a["virial"] - b["virial"]).max() / torch.abs(a["virial"]).max()).item()
     print(f"fused {mbf.precision}: force RMSE {rmse:.3e} ({rmse / frms:.2e} rel), |dE|/atom {de:.3e}, virial {dv:.2e}",
           flush=True)
-    assert rmse <= 1e-3
-    assert de <= 5e-3
+    assert rmse <= BUDGET[mbf.precision]["rmse"]
+    assert de <= BUDGET[mbf.precision]["de"]
 
 
 @pytest.mark.parametrize("nloc", [192, 1000])
@@ -101,8 +111,8 @@ def test_fused_fitting(setup, nloc):
     d_err = _rel_rms(dd.double(), dd64)
     print(f"fitting: max |de| {e_err:.3e} eV, dE/ddesc rel RMS {d_err:.2e}", flush=True)
     assert torch.isfinite(ae).all() and torch.isfinite(dd).all()
-    assert e_err <= 2e-2
-    assert d_err <= 1e-2
+    assert e_err <= BUDGET[mbf.precision]["fit_e"]
+    assert d_err <= BUDGET[mbf.precision]["fit_d"]
 
 
 @pytest.mark.parametrize("system", ["water192", "water12k"])

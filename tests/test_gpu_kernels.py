@@ -295,8 +295,10 @@ def test_force_virial_cells(oracle, name, make, rcut, sel, dtype):
         np.testing.assert_allclose(f1, of_local, rtol=1e-10, atol=1e-10 * np.abs(of_local).max())
     if name in ("clustered", "water12k_rcut9"):
         return  # atoms left to the per-atom search add their neighbor terms with atomics
-    if name.startswith("alloy") and dtype == torch.float64:
-        return  # FP64 cells over the first table size add their neighbor terms with atomics
+    if (name.startswith("alloy") or name == "box_just_over_8rc") and dtype == torch.float64:
+        # FP64 cells over the first table size (here: 5^3-cell neighborhoods
+        # of the two-cell search) add their neighbor terms with atomics
+        return
     # deterministic: a second call gives the same bits
     f2, v2 = ops.force_virial_cells(T(nd, dtype), nlist, cells, mapping, ext_c, ext_t, T(std, dtype), rcut, 0.5,
                                     sel, s.box)

@@ -84,10 +84,11 @@ def test_low_precision_forms_error_budget(water_model, precision):
     frms = torch.sqrt((a["force"] ** 2).mean()).item()
     print(f"{precision}: force RMSE {rmse:.3e} eV/A ({rmse / frms:.2e} of RMS force), |dE|/atom {de:.3e} eV",
           flush=True)
-    # the target is on forces; the energy error is reported (bf16 rounds the
-    # 1600-wide fitting input to 8 mantissa bits, ~1 meV/atom here)
-    assert rmse <= 1e-3
-    assert de <= 5e-3
+    # measured on MI355X: bf16 4.2e-5 eV/A and 2.3e-5 eV/atom, fp16 5.3e-6 and
+    # 1.4e-5; the budgets are a few times that (the benchmark's own bound,
+    # 1e-3 eV/A, would let a tenfold regression through)
+    assert rmse <= {"bf16": 1e-4, "fp16": 2e-5}[precision]
+    assert de <= {"bf16": 1e-4, "fp16": 5e-5}[precision]
 
 
 def test_tabulated_fp64_model_matches_oracle():
@@ -208,7 +209,10 @@ def test_alloy_bf16_error_budget(alloy_model):
     de = abs(a["energy"].item() - b["energy"].item()) / s.nloc
     dv = (torch.abs(a["virial"] - b["virial"]).max() / torch.abs(a["virial"]).max()).item()
     print(f"alloy bf16: force RMSE {rmse:.3e} eV/A, |dE|/atom {de:.3e} eV, virial max rel {dv:.3e}", flush=True)
-    assert rmse <= 1e-3
+    # measured 2.4e-5 eV/A, 1.9e-5 eV/atom, 1.5e-4
+    assert rmse <= 1e-4
+    assert de <= 1e-4
+    assert dv <= 1e-3
 
 
 @pytest.mark.parametrize("precision", ["fp64", "bf16"])
