@@ -10,6 +10,8 @@
 // compensated (Neumaier) sum per component.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include <cstring>
@@ -960,6 +962,61 @@ extern "C" int dpmi_prod_virial_se_a_f32(float* virial, float* atom_virial, cons
                                          int64_t workspace_bytes, void* stream) {
   return prod_virial_impl<float>(virial, atom_virial, net_deriv, in_deriv, rij, nlist, nloc, nall, nnei, nframes,
                                  workspace, workspace_bytes, stream);
+}
+
+// Scratch of dpmi_prod_virial_se_a_noalloc_f64: the per-workgroup virial
+// partials (9 doubles per 4 local atoms per frame) that the workspace of
+// dpmi_prod_virial_se_a_f64 holds, in device memory the library owns.  Every
+// GPU the library runs on keeps its own copy until the process ends; every
+// call on one GPU shares it, so calls that overlap on one GPU race.  Each
+// workgroup writes all of its partials before the reduction reads them, so
+// nothing needs clearing between calls.
+constexpr int64_t kScratchBytes = int64_t(8) << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (the host-side name is only a stand-in for the device array).
+int scratch_address(void** out) {
+  int dev = 0;
+  DPMI_HIP(hipGetDevice(&dev));
+  DPMI_CHECK_ARG(dev >= 0 && dev < kMaxScratchDevices, "device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    DPMI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    DPMI_CHECK_ARG(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return DPMI_OK;
+}
+
+}  // namespace
+
+extern "C" int dpmi_prod_virial_se_a_noalloc_f64(double* virial, double* atom_virial, const double* net_deriv,
+                                                 const double* in_deriv, const double* rij, const int* nlist,
+                                                 int64_t nloc, int64_t nall, int64_t nnei, int64_t nframes,
+                                                 void* stream) {
+  DPMI_CHECK_ARG(nloc >= 0 && nall >= nloc && nnei >= 0 && nframes >= 0, "bad sizes");
+  const int64_t need = virial_ws(nloc, nframes);
+  if (need > kScratchBytes) {
+    set_error("dpmi_prod_virial_se_a_noalloc_f64: nframes * ceil(nloc / 4) * 72 bytes = " + std::to_string(need) +
+              " exceeds the library's scratch of " + std::to_string(kScratchBytes) +
+              " bytes (at most 116,508 four-atom groups over all frames); use dpmi_prod_virial_se_a_f64 with a "
+              "caller workspace");
+    return DPMI_ERR_ARG;
+  }
+  void* ws = nullptr;
+  int rc = scratch_address(&ws);
+  if (rc) return rc;
+  return prod_virial_impl<double>(virial, atom_virial, net_deriv, in_deriv, rij, nlist, nloc, nall, nnei, nframes,
+                                  ws, kScratchBytes, stream);
 }
 
 constexpr int64_t kForceBlocks = 2048;

@@ -37,6 +37,7 @@ SIGNATURES = {
     "dpmi_prod_virial_workspace_bytes": [_P, _L, _L],
     "dpmi_prod_virial_se_a_f64": [_P, _P, _P, _P, _P, _P, _L, _L, _L, _L, _P, _L, _P],
     "dpmi_prod_virial_se_a_f32": [_P, _P, _P, _P, _P, _P, _L, _L, _L, _L, _P, _L, _P],
+    "dpmi_prod_virial_se_a_noalloc_f64": [_P, _P, _P, _P, _P, _P, _L, _L, _L, _L, _P],
     "dpmi_format_nlist_f64": [_P, _P, _P, _P, _P, _L, _P, _P, _P, _L, _L, _F, _P, _I, _P, _L, _P],
     "dpmi_prod_force_grad_a_f64": [_P, _P, _P, _P, _L, _L, _L, _L, _P],
     "dpmi_prod_force_grad_a_f32": [_P, _P, _P, _P, _L, _L, _L, _L, _P],

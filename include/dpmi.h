@@ -192,6 +192,19 @@ int dpmi_prod_virial_se_a_f32(float* virial, float* atom_virial, const float* ne
                               int64_t nall, int64_t nnei, int64_t nframes, void* workspace,
                               int64_t workspace_bytes, void* stream);
 
+/* dpmi_prod_virial_se_a_f64 without a workspace argument: it allocates
+ * nothing, so it suits callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Its per-workgroup partials
+ * live in 8 MiB of device memory the library owns on each GPU (held until
+ * the process ends): inputs need nframes * ceil(nloc / 4) <= 116,508, larger
+ * ones return DPMI_ERR_ARG.  Every call on one GPU shares that memory, so
+ * calls that overlap on one GPU (other streams, other threads) race; calls
+ * queued on one stream do not.  Results are those of the workspace form. */
+int dpmi_prod_virial_se_a_noalloc_f64(double* virial, double* atom_virial, const double* net_deriv,
+                                      const double* in_deriv, const double* rij, const int* nlist, int64_t nloc,
+                                      int64_t nall, int64_t nnei, int64_t nframes, void* stream);
+
 /* ------------------------------------------------------------------------
  * Gradients of force and virial w.r.t. net_deriv (the training step's
  * double backward).  prod_force_grad_a_cpu (source/lib/src/
