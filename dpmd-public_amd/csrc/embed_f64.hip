@@ -29,6 +29,7 @@
 #include <cstring>
 
 #include "dpmi_internal.h"
+#include "tanh64.h"
 
 namespace dpmi {
 namespace {
@@ -60,38 +61,6 @@ struct Embed64Params {
 
 __device__ inline f64x4 mfma64(double a, double b, const f64x4& c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-// tanh(x) = sign(x) E / (E + 2), E = expm1(2|x|); |x| >= 22 gives +-1.
-__device__ inline double tanh64(double x) {
-  const double y = fmin(2.0 * fabs(x), 44.0);
-  const double n = rint(y * 1.4426950408889634);
-  double r = fma(-n, 6.93147180369123816490e-01, y);
-  r = fma(-n, 1.90821492927058770002e-10, r);
-  // expm1(r) = r + r^2 (1/2 + r/3! + ... + r^11/13!),  |r| <= 0.347
-  double q = 1.0 / 6227020800.0;
-  q = fma(q, r, 1.0 / 479001600.0);
-  q = fma(q, r, 1.0 / 39916800.0);
-  q = fma(q, r, 1.0 / 3628800.0);
-  q = fma(q, r, 1.0 / 362880.0);
-  q = fma(q, r, 1.0 / 40320.0);
-  q = fma(q, r, 1.0 / 5040.0);
-  q = fma(q, r, 1.0 / 720.0);
-  q = fma(q, r, 1.0 / 120.0);
-  q = fma(q, r, 1.0 / 24.0);
-  q = fma(q, r, 1.0 / 6.0);
-  q = fma(q, r, 0.5);
-  const double em1r = fma(r * r, q, r);
-  const int ni = (int)n;
-  // expm1(y) = 2^n expm1(r) + (2^n - 1)
-  const double e = ldexp(em1r, ni) + (ldexp(1.0, ni) - 1.0);
-  const double d = e + 2.0;
-  double rc = __builtin_amdgcn_rcp(d);
-  rc = fma(rc, fma(-d, rc, 1.0), rc);
-  rc = fma(rc, fma(-d, rc, 1.0), rc);
-  const double t0 = e * rc;
-  const double t = fma(rc, fma(-d, t0, e), t0);
-  return copysign(t, x);
 }
 
 __device__ inline int orig_of_q3_64(int q, int n1) {

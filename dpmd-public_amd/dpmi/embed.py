@@ -289,3 +289,25 @@ class FusedEmbeddingF64:
                     ptr(self.w3), ptr(self.consts), nloc, ptr(self.sel), self.ntypes, self.n1, ptr(counts),
                     ops._stream())
         return nd
+
+    def forward_part(self, em: torch.Tensor, pad=None) -> torch.Tensor:
+        """em [nloc, nnei, 4] float64 -> per-type partials [ntypes, nloc, 4, M]
+        of sum_rows em_t^T G_t (GR = their sum / nnei), as the FP64 fitting
+        kernel takes them."""
+        nloc = em.shape[0]
+        em = em.contiguous()
+        part = torch.empty((self.ntypes, nloc, 4, self.M), dtype=torch.float64, device=em.device)
+        counts, atype, pad_em = pad if pad is not None else (None, None, None)
+        ops._launch("dpmi_se_a_embed_fwd_f64", ptr(part), ptr(em), ptr(self.w2), ptr(self.w3), ptr(self.consts),
+                    nloc, ptr(self.sel), self.ntypes, self.n1, ptr(counts), ptr(atype), ptr(pad_em), ops._stream())
+        return part
+
+    def backward_part(self, em: torch.Tensor, dpart: torch.Tensor, counts=None) -> torch.Tensor:
+        """dE/d(sum of the partials) [nloc, 4, M] float64 (what the FP64
+        fitting kernel writes) -> dE/d em [nloc, nnei, 4]."""
+        nloc = em.shape[0]
+        nd = torch.empty((nloc, self.nnei, 4), dtype=torch.float64, device=em.device)
+        ops._launch("dpmi_se_a_embed_bwd_f64", ptr(nd), ptr(em.contiguous()), ptr(dpart.contiguous()), ptr(self.w2),
+                    ptr(self.w3), ptr(self.consts), nloc, ptr(self.sel), self.ntypes, self.n1, ptr(counts),
+                    ops._stream())
+        return nd

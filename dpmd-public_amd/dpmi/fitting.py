@@ -162,3 +162,100 @@ class FusedFitting:
         ops._launch(self._fit_gr, ptr(ae), ptr(dgr), ptr(gr_part), ptr(perm), ptr(start),
                     ptr(self.frags), ptr(self.consts), ptr(self.ebias), nloc, self.ntypes, n1, nnei, ops._stream())
         return ae, dgr
+
+
+# --------------------------------------------------------------------------- FP64 form (csrc/fitting_f64.hip)
+
+
+def _frags64(A: np.ndarray, nm: int, nk: int) -> np.ndarray:
+    """A [rows, cols] -> MFMA f64 16x16x4 A fragments [nm, nk, 64] of the
+    zero-padded [16 nm, 4 nk] matrix: lane l = 16 kk + r holds
+    A[16 mt + r, 4 ks + kk]."""
+    P = np.zeros((16 * nm, 4 * nk), dtype=np.float64)
+    r, c = min(A.shape[0], P.shape[0]), min(A.shape[1], P.shape[1])
+    P[:r, :c] = A[:r, :c]
+    return P.reshape(nm, 16, nk, 4).transpose(0, 2, 3, 1).reshape(nm, nk, 64)
+
+
+def supported_f64(fit_layers: list, M: int, axis: int) -> bool:
+    """The FP64 kernel's shapes: the fused-fitting structure above with the
+    descriptor D = GR^T GR[:, :16] of a G^T R width 16 <= M <= 128."""
+    if axis != 16 or not (16 <= M <= 128) or not supported(fit_layers):
+        return False
+    return all(_layer(layers[0])[0].shape[0] == 16 * M for layers in fit_layers)
+
+
+def pack_fitting_f64(fit_layers: list, bias_atom_e, out_bias, M: int):
+    """(frags float64 [ntypes, per-type doubles], consts float64
+    [ntypes, 7*256 + 1], ebias float64 [ntypes], H) in the order the FP64
+    kernel reads them, per type:
+
+        w1f  A = W1^T  [16 tiles, 4 M K steps]   w2f, w3f  A = W2^T, W3^T  [16, nkh]
+        w3b, w2b  A = W3, W2  [16, nkh]          w1b  A = W1  [M tiles, nkh]
+
+    (nkh = ceil(H / 4)); consts b1 idt1 b2 idt2 b3 idt3 w4 [256 each], b4."""
+    if not supported_f64(fit_layers, M, 16):
+        raise ValueError("FP64 fused fitting needs axis 16, 16 <= M <= 128 and hidden widths (H, H, H), H <= 256")
+    ntypes = len(fit_layers)
+    h = _layer(fit_layers[0][0])[0].shape[1]
+    nkh = (h + 3) // 4
+    blocks, consts = [], np.zeros((ntypes, 7 * KH + 1), dtype=np.float64)
+    for t, layers in enumerate(fit_layers):
+        (w1, b1, i1), (w2, b2, i2), (w3, b3, i3), (w4, b4, _) = (_layer(x) for x in layers)
+        if w1.shape[1] != h:
+            raise ValueError("every type's fitting net must have the same hidden width")
+        parts = [_frags64(w1.T, 16, 4 * M), _frags64(w2.T, 16, nkh), _frags64(w3.T, 16, nkh),
+                 _frags64(w3, 16, nkh), _frags64(w2, 16, nkh), _frags64(w1, M, nkh)]
+        blocks.append(np.concatenate([x.reshape(-1) for x in parts]))
+        c = consts[t]
+        for k, v in enumerate((b1, i1, b2, i2, b3, i3, w4[:, 0])):
+            c[k * KH:k * KH + h] = 1.0 if v is None else v
+        c[7 * KH] = b4[0] if b4.size else 0.0
+    ob = np.zeros(ntypes) if out_bias is None else np.asarray(out_bias, dtype=np.float64).reshape(-1)
+    ebias = np.asarray(bias_atom_e, dtype=np.float64).reshape(-1) + ob
+    return np.stack(blocks), consts, ebias, h
+
+
+class FusedFitting64:
+    """Device-resident FP64 fitting nets + the fused descriptor / fitting /
+    adjoint launch (dpmi_se_a_fitting_gr_f64)."""
+
+    def __init__(self, fit_layers, bias_atom_e, out_bias, M: int, device):
+        frags, consts, ebias, h = pack_fitting_f64(fit_layers, bias_atom_e, out_bias, M)
+        import ctypes
+
+        n = ctypes.c_int64(0)
+        ops.call("dpmi_se_a_fitting_f64_frag_doubles", ctypes.byref(n), M, h)
+        if frags.shape[1] != n.value:
+            raise RuntimeError(f"packed {frags.shape[1]} doubles per type, the kernel reads {n.value}")
+        self.M, self.H, self.ntypes = M, h, len(fit_layers)
+        self.frags = torch.from_numpy(frags).to(device)
+        self.consts = torch.from_numpy(consts).to(device)
+        self.ebias = torch.from_numpy(ebias).to(device)
+
+    sort_by_type = FusedFitting.sort_by_type
+
+    def from_parts(self, gr_part: torch.Tensor, atype: torch.Tensor, nnei: int, order=None):
+        """gr_part [nparts, nloc, 4, M] float64 (GR = sum over parts / nnei)
+        -> (atom_energy float64 [nloc], dE/d(sum of parts) float64
+        [nloc, 4, M])."""
+        if gr_part.dtype != torch.float64 or gr_part.dim() != 4 or tuple(gr_part.shape[2:]) != (4, self.M):
+            raise ValueError(f"gr_part must be float64 [nparts, nloc, 4, {self.M}]")
+        nparts, nloc = gr_part.shape[:2]
+        perm, start = self.sort_by_type(atype) if order is None else order
+        gr_part = gr_part.contiguous()
+        ae = torch.empty(nloc, dtype=torch.float64, device=gr_part.device)
+        dgr = torch.empty((nloc, 4, self.M), dtype=torch.float64, device=gr_part.device)
+        ops._launch("dpmi_se_a_fitting_gr_f64", ptr(ae), ptr(dgr), ptr(gr_part), nparts, ptr(perm), ptr(start),
+                    ptr(self.frags), ptr(self.consts), ptr(self.ebias), nloc, self.ntypes, self.M, self.H, nnei,
+                    ops._stream())
+        return ae, dgr
+
+
+def energy_sum(atom_energy: torch.Tensor) -> torch.Tensor:
+    """Compensated (Neumaier), deterministic FP64 sum of atomic energies on
+    the GPU (dpmi_energy_sum_f64): a 0-d float64 tensor."""
+    ae = atom_energy.to(torch.float64).contiguous()
+    out = torch.empty((), dtype=torch.float64, device=ae.device)
+    ops._launch("dpmi_energy_sum_f64", ptr(out), ptr(ae), ae.numel(), ops._stream())
+    return out

@@ -182,6 +182,11 @@ class SeAEnergyModel:
             self.fused_fit = fitting.FusedFitting(fit, bias_atom_e, ob, self.device, self.gemm_dtype)
         # the descriptor fused into the fitting kernel (axis 16, M <= 104)
         self.fused_gr = self.fused_fit is not None and self.fused_fit.supports_gr(self.fused.n1, self.axis)
+        # FP64 form (network or table embedding): the descriptor, the fitting
+        # nets and their adjoint as one FP64-MFMA kernel (else torch, _fit_gr64)
+        self.fused_fit64 = None
+        if precision == "fp64" and fitting.supported_f64(fit, self.M, self.axis):
+            self.fused_fit64 = fitting.FusedFitting64(fit, bias_atom_e, ob, self.M, self.device)
         self._order_cache = None
         self._host_order_cache = None
         self._types_checked = None
@@ -345,7 +350,7 @@ class SeAEnergyModel:
                 ae_b = self.atom_energy(self.descriptor(em_b), atype[a:b], order)
                 (net_deriv[a:b],) = torch.autograd.grad(ae_b.sum(), em_b)
             ae[a:b] = ae_b.detach()
-        energy = ae.sum()
+        energy = fitting.energy_sum(ae)
         force, virial = self.forces(net_deriv, nlist, cells, mapping, ext_c, ext_t, box)
         out = {"energy": energy, "force": force, "virial": virial}
         if need_atom_energy:
@@ -371,8 +376,8 @@ class SeAEnergyModel:
             nd, ae = self._fused_chain(em, atype, pad=self._pad_args(nlist, atype))
         elif self.fused64 is not None:
             pad = self._pad_args(nlist, atype)
-            ae, dgr = self._fit_gr64(self.fused64.forward(em, pad), atype)
-            nd = self.fused64.backward(em, dgr, pad[0] if pad is not None else None)
+            ae, dpart = self._fit_parts64(self.fused64.forward_part(em, pad), atype)
+            nd = self.fused64.backward_part(em, dpart, pad[0] if pad is not None else None)
         else:
             em = em.detach().requires_grad_(True)
             with torch.enable_grad():
@@ -381,20 +386,32 @@ class SeAEnergyModel:
         force, virial = ops.force_virial_recompute(nd, nlist, mapping, ext_c, ext_t, self.dstd, self.rcut,
                                                    self.rcut_smth, self.sel)
         ae = ae.detach()
-        return {"energy": ae.sum(), "atom_energy": ae, "force": force, "virial": virial}
+        return {"energy": fitting.energy_sum(ae), "atom_energy": ae, "force": force, "virial": virial}
+
+    def _fit_parts64(self, parts, atype, atype_in=None):
+        """(atom energies [nloc], dE/d(sum of parts) [nloc, 4, M]) of the
+        per-type G^T R partials parts [nparts, nloc, 4, M] (GR = their sum /
+        nnei): the FP64 fused descriptor / fitting kernel, or the torch nets
+        of _fit_gr64 where its shapes are not covered."""
+        if self.fused_fit64 is not None and parts.dtype == torch.float64:
+            order = self._type_order(atype_in if atype_in is not None else atype, atype)
+            return self.fused_fit64.from_parts(parts, atype, self.nnei, order=order)
+        ae, dgr = self._fit_gr64(parts.sum(dim=0) / self.nnei, atype, atype_in)
+        return ae, dgr / self.nnei
 
     def _compute_fused64(self, atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy, atype_in=None):
-        """FP64 plain form on hand-written kernels: fused FP64 embedding + G^T R
-        and its recompute adjoint; descriptor and fitting (per-type nets,
-        rocBLAS) through torch autograd (_fit_gr64); env-mat adjoint for
-        forces and virial."""
+        """FP64 plain form on hand-written kernels end to end: fused FP64
+        embedding + G^T R partials, the fused FP64 descriptor / fitting /
+        adjoint kernel (dE/d partials), the embedding's recompute adjoint,
+        the env-mat adjoint for forces and virial, and the compensated
+        energy sum."""
         em = self.env_mat(nlist, ext_c, ext_t)
         pad = self._pad_args(nlist, atype)
-        gr = self.fused64.forward(em, pad)
-        ae, dgr = self._fit_gr64(gr, atype, atype_in)
-        nd = self.fused64.backward(em, dgr, pad[0] if pad is not None else None)
+        part = self.fused64.forward_part(em, pad)
+        ae, dpart = self._fit_parts64(part, atype, atype_in)
+        nd = self.fused64.backward_part(em, dpart, pad[0] if pad is not None else None)
         force, virial = self.forces(nd, nlist, cells, mapping, ext_c, ext_t, box)
-        out = {"energy": ae.sum(), "force": force, "virial": virial}
+        out = {"energy": fitting.energy_sum(ae), "force": force, "virial": virial}
         if need_atom_energy:
             out["atom_energy"] = ae
         return out
@@ -402,28 +419,31 @@ class SeAEnergyModel:
     def _compute_tab(self, atype, ext_c, ext_t, mapping, nlist, cells, box, need_atom_energy, atype_in=None):
         """Compressed model (tabulated embedding) with direct launches of the
         table kernels: tabulate_fusion_se_a per neighbor type (se_a.py:833-839)
-        summed into G^T R / nnei, the descriptor and fitting nets with their
-        adjoint (_fit_gr64), then tabulate_fusion_se_a_grad per type on
-        dE/d(em^T G) = dgr / nnei, whose em_x part adds to column 0 of
-        dE/d em (em_x is em[..., 0]); env-mat adjoint for forces and virial."""
+        into the per-type G^T R partials, the descriptor and fitting nets with
+        their adjoint (_fit_parts64: the FP64 fused kernel, or torch for an
+        fp32 table), then tabulate_fusion_se_a_grad per type on
+        dy = dE/d(em^T G) = dE/d(sum of partials), whose em_x part adds to
+        column 0 of dE/d em (em_x is em[..., 0]); env-mat adjoint for forces
+        and virial."""
         em = self.env_mat(nlist, ext_c, ext_t)
-        parts, gr = [], None
+        nloc = em.shape[0]
+        parts = []
+        gr_part = torch.empty((self.ntypes, nloc, 4, self.M), dtype=em.dtype, device=em.device)
         for t in range(self.ntypes):
             rr = em[:, self.sec[t]:self.sec[t + 1], :].contiguous()
             ex = rr[:, :, 0].reshape(-1, 1).contiguous()
             table, info = self.tables[t]
-            c = ops.tabulate_forward(table, info, ex, rr, self.M)
+            gr_part[t] = ops.tabulate_forward(table, info, ex, rr, self.M)
             parts.append((table, info, ex, rr))
-            gr = c if gr is None else gr + c
-        ae, dgr = self._fit_gr64(gr / self.nnei, atype, atype_in)
-        dy = (dgr / self.nnei).contiguous()
+        ae, dy = self._fit_parts64(gr_part, atype, atype_in)
+        dy = dy.to(em.dtype).contiguous()
         nd = torch.empty_like(em)
         for t, (table, info, ex, rr) in enumerate(parts):
             dx, dem = ops.tabulate_grad(table, info, ex, rr, dy)
             dem[:, :, 0] += dx.reshape(dem.shape[0], -1)
             nd[:, self.sec[t]:self.sec[t + 1], :] = dem
         force, virial = self.forces(nd, nlist, cells, mapping, ext_c, ext_t, box)
-        out = {"energy": ae.sum(), "force": force, "virial": virial}
+        out = {"energy": fitting.energy_sum(ae), "force": force, "virial": virial}
         if need_atom_energy:
             out["atom_energy"] = ae
         return out
@@ -506,12 +526,13 @@ class SeAEnergyModel:
         caller passes the same, unmodified atype tensor (the cache holds the
         tensor itself and its version counter, so neither a new tensor nor an
         in-place change can reuse a stale order)."""
-        if self.fused_fit is None:
+        ff = self.fused_fit if self.fused_fit is not None else self.fused_fit64
+        if ff is None:
             return None
         c = self._order_cache
         if c is not None and c[0] is atype_in and c[1] == atype_in._version:
             return c[2]
-        order = self.fused_fit.sort_by_type(atype)
+        order = ff.sort_by_type(atype)
         self._order_cache = (atype_in, atype_in._version, order)
         return order
 
@@ -545,9 +566,9 @@ class SeAEnergyModel:
         adjoint for forces and virial."""
         em = self.env_mat(nlist, ext_c, ext_t)
         nd, ae = self._fused_chain(em, atype, order=order, pad=self._pad_args(nlist, atype))
-        energy = ae.sum()
+        energy = fitting.energy_sum(ae.detach())
         force, virial = self.forces(nd, nlist, cells, mapping, ext_c, ext_t, box)
-        out = {"energy": energy.detach(), "force": force, "virial": virial}
+        out = {"energy": energy, "force": force, "virial": virial}
         if need_atom_energy:
             out["atom_energy"] = ae.detach()
         return out

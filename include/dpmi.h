@@ -417,6 +417,30 @@ int dpmi_se_a_fitting_gr_f16(double* atom_energy, float* dgr, const float* gr_pa
 /* Number of 8 x 16-bit lanes (64 per fragment) of one type's packed fitting net. */
 int dpmi_se_a_fitting_frag_count(int64_t* count, int k1);
 
+/* FP64 form of the fused descriptor + fitting kernel (v_mfma_f64_16x16x4f64,
+ * FP64 throughout): replaces the descriptor of DescrptBlockSeA.forward
+ * (deepmd/pt/model/descriptor/se_a.py:851-858), GeneralFitting._forward_common
+ * (deepmd/pt/model/task/fitting.py:734-900) and its autograd backward.
+ * gr_part [nparts, nloc, 4, M] (GR = sum over the parts / nnei; the parts are
+ * the per-neighbor-type G^T R of the embedding or of tabulate_fusion_se_a);
+ * D = GR^T GR[:, :16] (axis 16, 16 <= M <= 128) is formed inside the kernel,
+ * never stored.  Hidden width H <= 256, three hidden layers as the bf16
+ * kernel.  perm / type_start as above.  frags: per type
+ * dpmi_se_a_fitting_f64_frag_doubles doubles, consts [ntypes, 7*256 + 1],
+ * ebias [ntypes], laid out by dpmi.fitting.pack_fitting_f64.
+ * atom_energy [nloc]; dgr [nloc, 4, M] = dE/d(sum of the parts) for
+ * E = sum atom_energy (the input of dpmi_se_a_embed_bwd_f64 and the dy of
+ * dpmi_tabulate_fusion_se_a_grad_f64).  Deterministic (fixed-order sums). */
+int dpmi_se_a_fitting_gr_f64(double* atom_energy, double* dgr, const double* gr_part, int nparts, const int* perm,
+                             const int* type_start, const double* frags, const double* consts, const double* ebias,
+                             int64_t nloc, int ntypes, int M, int H, int nnei, void* stream);
+int dpmi_se_a_fitting_f64_frag_doubles(int64_t* count, int M, int H);
+
+/* energy [1] = sum of atom_energy [n], FP64 with compensated (Neumaier)
+ * summation in a fixed order (the per-frame energy reduction of
+ * deepmd/pt/model/model/transform_output.py:165,181). */
+int dpmi_energy_sum_f64(double* energy, const double* atom_energy, int64_t n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
