@@ -63,23 +63,6 @@ for _h in ("bf16", "f16"):
                       f"dpmi_se_a_fitting_gr_{_h}": f"k_fit<{_h}, true>"})
 
 
-def _issue_bound(workload: str, entry: str):
-    """SIMD-cycles per 16-row tile that the kernel's VALU instruction mix
-    needs (4 per VALU, 16 per transcendental instruction), from the committed
-    SQ counter summary of this workload, or None."""
-    f = ROOT / "profiles" / f"sq_mfma_{workload}_bf16.txt"
-    k = KERNEL_OF.get(entry)
-    if not f.exists() or k is None:
-        return None, None
-    for line in f.read_text().splitlines():
-        t = line.strip()
-        if t.startswith(k + " "):
-            cols = t[len(k):].split()
-            if len(cols) > 1 and cols[0] == "bound":
-                return float(cols[1]), str(f.relative_to(ROOT))
-    return None, None
-
-
 def _measured_traffic(workload: str, entry: str):
     """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the
     kernel behind `entry`, from the committed rocprofv3 counter summary of
@@ -193,12 +176,13 @@ def main() -> int:
         return (t1 - t0) / args.steps, out, kt.summary()
 
     names64 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f64",
-               "dpmi_force_virial_cells_f64", "dpmi_se_a_embed_fwd_f64", "dpmi_se_a_embed_bwd_f64"]
+               "dpmi_force_virial_cells_f64", "dpmi_se_a_embed_fwd_f64", "dpmi_se_a_embed_bwd_f64",
+               "dpmi_se_a_fitting_gr_f64", "dpmi_energy_sum_f64"]
     h = {"bf16": "bf16", "fp16": "f16"}[args.precision]
     names32 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f32c64",
                "dpmi_force_virial_cells_f32", f"dpmi_se_a_embed_fwd_{h}", "dpmi_se_a_descriptor_fwd",
                "dpmi_se_a_descriptor_bwd", f"dpmi_se_a_embed_bwd_{h}", f"dpmi_se_a_fitting_{h}",
-               f"dpmi_se_a_fitting_gr_{h}"]
+               f"dpmi_se_a_fitting_gr_{h}", "dpmi_energy_sum_f64"]
     t64, out64, k64 = None, None, {}
     if not args.no_fp64:
         m64 = SeAEnergyModel.from_params(spec, params, device=dev, precision="fp64")
@@ -254,13 +238,6 @@ def main() -> int:
                     "algorithmic_flops_per_launch": flops, "embedding_rows_per_launch": nrows,
                     "embedding_tiles_per_launch": tiles,
                     "avg_launch_ms": round(klo[dom]["avg_ms"], 4)}
-        # what actually binds the embedding kernels: VALU issue (tanh = two
-        # quarter-rate transcendentals per element), from the counted mix
-        cyc, cyc_src = _issue_bound(args.workload, dom)
-        if cyc is not None and avg_s > 0:
-            roofline["valu_issue"] = {"frac": round(cyc * tiles / (avg_s * 2.4e9 * 1024), 4),
-                                      "cycles_per_tile_bound": cyc, "tiles_per_launch": tiles,
-                                      "source": cyc_src}
     else:
         algo = _bytes_per_atom(dom, nnei) * nloc
         achieved = algo / avg_s / 1e9 if avg_s > 0 else 0.0
@@ -326,7 +303,7 @@ def main() -> int:
     }
 
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(spec, params, args.cpu_seconds, wl["spec"])
+        line["cpu_baseline"] = cpu_baseline(spec, params, args.cpu_seconds, args.workload, sysm)
     print(json.dumps(line), flush=True)
     return 0
 
@@ -356,7 +333,7 @@ def bench_tabulated(args) -> int:
     coord = torch.tensor(sysm.coord, device=dev)
     atype = torch.tensor(sysm.atype, device=dev)
     names = ["dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f64", "dpmi_tabulate_fusion_se_a_f64",
-             "dpmi_tabulate_fusion_se_a_grad_f64", "dpmi_force_virial_cells_f64"]
+             "dpmi_tabulate_fusion_se_a_grad_f64", "dpmi_se_a_fitting_gr_f64", "dpmi_force_virial_cells_f64"]
 
     def run(model):
         for _ in range(args.warmup):
@@ -456,24 +433,20 @@ def copy_calibration(nbytes: int) -> int:
     return 0
 
 
-def cpu_baseline(spec, params, budget_s: float, family: str = "water") -> dict:
-    """The CPU oracle (C neighbor list + float64 torch-CPU model with autograd
-    forces, the reference PT algorithm) on a bounded sample of the same kind
-    of system (the 12,288-atom water box, or a 13,824-atom alloy box of the
-    same lattice): whole E+F+V steps until budget_s of CPU time has elapsed
-    (at least one)."""
-    import torch
+def _cpu_model_name() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    from dpmi import systems
-    from oracle.model_ref import Model
 
-    threads = torch.get_num_threads()
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    m = Model.from_params(spec, params)
-    if family == "water":
-        s, what = systems.water12k(1), "12,288-atom water box (same model and density)"
-    else:
-        s, what = systems.alloy262k(4, ncell=24), "13,824-atom alloy box (same model, lattice and density)"
+def _whole_step_sample(m, s, budget_s: float):
+    """Whole E+F+V steps of system s with the CPU oracle (C neighbor list +
+    float64 torch-CPU model, autograd forces) until budget_s has elapsed (at
+    least one, at most 8): (atom-steps/s, steps, seconds)."""
     m.evaluate(s.coord[:600], s.atype[:600], None)  # warm up allocators
     steps, t0 = 0, time.perf_counter()
     while True:
@@ -482,9 +455,84 @@ def cpu_baseline(spec, params, budget_s: float, family: str = "water") -> dict:
         el = time.perf_counter() - t0
         if el >= budget_s or steps >= 8:
             break
-    return {"value": round(s.nloc * steps / el, 1), "unit": "atom-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} FP64 E+F+V step(s) of the {what} with the CPU oracle: C neighbor list + "
-                      f"torch-CPU float64 model, autograd forces; {el:.1f} s"}
+    return s.nloc * steps / el, steps, el
+
+
+def cpu_baseline(spec, params, budget_s: float, workload: str, sysm) -> dict:
+    """The CPU oracle on the benchmark's own system: the C restatement of the
+    reference CPU kernels (cell-list neighbor list, env-mat with its
+    derivative, prod_force_a, prod_virial_a: OpenMP over atoms) on all atoms,
+    and the float64 torch-CPU network with autograd dE/d em (the reference PT
+    composition) on blocks of atoms until budget_s has elapsed, its time per
+    atom extrapolated to the whole system.  Beside it: the reference's own
+    O(nloc * nall) candidate search (build_nlist_cpu, neighbor_list.cc:873-916,
+    one thread) on a sample of centres, and whole steps of the smaller box of
+    the same model."""
+    import numpy as np
+    import torch
+
+    from dpmi import systems
+    from oracle import oracle
+    from oracle.model_ref import Model
+
+    threads = torch.get_num_threads()
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    oracle.lib()
+    m = Model.from_params(spec, params)
+    s = sysm
+    sel = list(spec.sel)
+    t0 = time.perf_counter()
+    ec, et, mp, nlist, _ = oracle.build_nlist_pbc(s.coord, s.atype, s.box, spec.rcut, sel)
+    t_nlist = time.perf_counter() - t0
+    avg, std = m.davg.numpy(), m.dstd.numpy()
+    t0 = time.perf_counter()
+    em, ed, rij = oracle.env_mat_a(nlist, ec, et, avg, std, spec.rcut, spec.rcut_smth, sel)
+    t_env = time.perf_counter() - t0
+    nnei = nlist.shape[1]
+    nd = np.zeros((s.nloc, nnei * 4))
+    blk, done, t0 = 2048, 0, time.perf_counter()
+    while done < s.nloc:
+        b = min(s.nloc, done + blk)
+        x = torch.tensor(em[done:b].reshape(b - done, nnei, 4), requires_grad=True)
+        ae = m.atom_energy(m.descriptor(x), s.atype[done:b])
+        (g,) = torch.autograd.grad(ae.sum(), x)
+        nd[done:b] = g.numpy().reshape(b - done, -1)
+        done = b
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    t_net_done = time.perf_counter() - t0
+    t_net = t_net_done * s.nloc / done
+    t0 = time.perf_counter()
+    oracle.prod_force_a(nd, ed.reshape(s.nloc, -1), nlist, ec.shape[0])
+    t_force = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.prod_virial_a(nd, ed.reshape(s.nloc, -1), rij.reshape(s.nloc, -1), nlist, ec.shape[0])
+    t_virial = time.perf_counter() - t0
+    step = t_nlist + t_env + t_net + t_force + t_virial
+    # the reference's all-pairs candidate search (its algorithm, run here on
+    # the oracle's threads) for the first ncent centres against every
+    # extended atom; its cost is linear in the number of centres
+    ncent = min(s.nloc, 8192)
+    t0 = time.perf_counter()
+    oracle.candidates(ec, ncent, spec.rcut, brute=True)
+    t_brute = (time.perf_counter() - t0) * s.nloc / max(ncent, 1)
+    small = systems.water12k(1) if spec.ntypes == 2 else systems.alloy262k(4, ncell=24)
+    ws_rate, ws_steps, ws_el = _whole_step_sample(m, small, min(budget_s, 8.0))
+    return {"value": round(s.nloc / step, 1), "unit": "atom-steps/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "cpu_model": _cpu_model_name(),
+            "sample": f"one FP64 E+F+V step of the {workload} system ({s.nloc} atoms) on the CPU oracle: C "
+                      f"cell-list neighbor list {t_nlist:.2f} s, C env-mat with derivative {t_env:.2f} s, "
+                      f"C prod_force {t_force:.2f} s and prod_virial {t_virial:.2f} s on all atoms; the "
+                      f"float64 torch-CPU network with autograd dE/d em on {done} of the atoms in "
+                      f"{t_net_done:.1f} s, extrapolated to {t_net:.1f} s for all",
+            "parts_s": {"nlist": round(t_nlist, 3), "env_mat": round(t_env, 3), "network": round(t_net, 3),
+                        "prod_force": round(t_force, 3), "prod_virial": round(t_virial, 3)},
+            "reference_all_pairs_candidates": {
+                "seconds_for_all_atoms": round(t_brute, 2), "threads": threads,
+                "sample": f"build_nlist_cpu's O(nloc * nall) search for {ncent} centres against all "
+                          f"{ec.shape[0]} extended atoms, scaled to {s.nloc} centres"},
+            "whole_step_smaller_box": {"atoms": small.nloc, "atom_steps_per_s": round(ws_rate, 1), "steps": ws_steps,
+                                       "seconds": round(ws_el, 1)}}
 
 
 if __name__ == "__main__":
