@@ -65,6 +65,7 @@ SIGNATURES = {
     "dpmi_se_a_fitting_gr_f64": [_P, _P, _P, _I, _P, _P, _P, _P, _P, _L, _I, _I, _I, _I, _P],
     "dpmi_se_a_fitting_f64_frag_doubles": [_P, _I, _I],
     "dpmi_energy_sum_f64": [_P, _P, _L, _P],
+    "dpmi_tabulate_build_f64": [_P, _P, _P, _P, _L, _P, _P, _I, _P],
     "dpmi_tabulate_fusion_se_a_f64": [_P, _P, _P, _P, _P, _L, _L, _L, _I, _P],
     "dpmi_tabulate_fusion_se_a_f32": [_P, _P, _P, _P, _P, _L, _L, _L, _I, _P],
     "dpmi_tabulate_fusion_se_a_grad_f64": [_P, _P, _P, _P, _P, _P, _P, _L, _L, _L, _I, _P],

@@ -35,50 +35,70 @@ def _layer(l):
     return w, b, idt
 
 
-def mlp_value_derivs(x: np.ndarray, layers, fold_idt: bool = False, device=None):
+def mlp_value_derivs(x: np.ndarray, layers, fold_idt: bool = False):
     """Embedding net G(x) for scalar inputs x [n] with dG/dx and d2G/dx2,
-    each [n, M] (float64, tanh activation, MLPLayer skip rules).  With a
-    device (e.g. "cuda") the chain runs there in torch float64; the result
-    is returned as numpy either way."""
-    if device is not None:
-        import torch
-
-        dev = torch.device(device)
-        xp = torch
-        y = torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev).reshape(-1, 1)
-        cat = lambda a: torch.cat([a, a], dim=1)  # noqa: E731
-        arr = lambda a: torch.as_tensor(a, device=dev)  # noqa: E731
-    else:
-        xp = np
-        y = np.asarray(x, dtype=np.float64).reshape(-1, 1)
-        cat = lambda a: np.concatenate([a, a], axis=1)  # noqa: E731
-        arr = lambda a: a  # noqa: E731
-    dy = xp.ones_like(y)
-    d2y = xp.zeros_like(y)
+    each [n, M] (float64 numpy, tanh activation, MLPLayer skip rules): the
+    host form of the chain dpmi_tabulate_build_f64 runs on the GPU."""
+    y = np.asarray(x, dtype=np.float64).reshape(-1, 1)
+    dy = np.ones_like(y)
+    d2y = np.zeros_like(y)
     for l in layers:
         w, b, idt = _layer(l)
         nin, nout = w.shape
-        w, b = arr(w), arr(b)
         z = y @ w + b
         dz = dy @ w
         d2z = d2y @ w
-        t = xp.tanh(z)
+        t = np.tanh(z)
         g1 = 1.0 - t * t
         g2 = -2.0 * t * g1
-        scale = arr(idt) if (fold_idt and idt is not None) else 1.0
+        scale = idt if (fold_idt and idt is not None) else 1.0
         out = t * scale
         dout = g1 * dz * scale
         d2out = (g2 * dz * dz + g1 * d2z) * scale
         if nout == nin:
             out, dout, d2out = out + y, dout + dy, d2out + d2y
         elif nout == 2 * nin:
-            out = out + cat(y)
-            dout = dout + cat(dy)
-            d2out = d2out + cat(d2y)
+            out = out + np.concatenate([y, y], axis=1)
+            dout = dout + np.concatenate([dy, dy], axis=1)
+            d2out = d2out + np.concatenate([d2y, d2y], axis=1)
         y, dy, d2y = out, dout, d2out
-    if device is not None:
-        return y.cpu().numpy(), dy.cpu().numpy(), d2y.cpu().numpy()
     return y, dy, d2y
+
+
+def build_on_gpu(xx: np.ndarray, tt: np.ndarray, layers, fold_idt: bool = False, device="cuda"):
+    """One net's table on the GPU (dpmi_tabulate_build_f64): the value /
+    slope / curvature chain at the grid nodes xx and the quintic Hermite
+    coefficients of the intervals of widths tt.  Returns (table [nspline,
+    6 M], node values [3, nnodes, M]) as float64 device tensors."""
+    import torch
+
+    from . import ops
+    from ._lib import ptr
+
+    dev = torch.device(device)
+    params, dims = [], []
+    for l in layers:
+        w, b, idt = _layer(l)
+        nin, nout = w.shape
+        scale = idt if (fold_idt and idt is not None) else np.ones(nout)
+        params += [w.reshape(-1), b, scale]
+        dims += [nin, nout]
+    p = torch.as_tensor(np.concatenate(params), device=dev)
+    tt = np.asarray(tt, dtype=np.float64).reshape(-1)
+    xx = np.asarray(xx, dtype=np.float64).reshape(-1)
+    if xx.shape[0] < tt.shape[0] + 1:
+        raise ValueError("need a grid node at both ends of every interval")
+    # the intervals' nodes (a grid from arange may carry one more point)
+    x = torch.as_tensor(xx[: tt.shape[0] + 1], device=dev).contiguous()
+    t = torch.as_tensor(tt, device=dev).contiguous()
+    n, M = x.numel(), dims[-1]
+    vdd = torch.empty((3, n, M), dtype=torch.float64, device=dev)
+    tab = torch.empty((n - 1, 6 * M), dtype=torch.float64, device=dev)
+    d = np.ascontiguousarray(np.asarray(dims, dtype=np.int32))
+    ops._gpu(x)
+    ops._launch("dpmi_tabulate_build_f64", ptr(tab), ptr(vdd), ptr(x), ptr(t), n, ptr(p), ptr(d), len(layers),
+                ops._stream())
+    return tab, vdd
 
 
 def hermite_table(vv: np.ndarray, dd: np.ndarray, d2: np.ndarray, tt: np.ndarray) -> np.ndarray:
@@ -127,7 +147,8 @@ def build_tables(emb_layers, davg, dstd, rcut: float, rcut_smth: float, min_nbor
                  check_frequency: int = -1, fold_idt: bool = False, device=None):
     """Tables of a type_one_side se_e2_a model: a list (one per neighbor type)
     of (table [nspline, 6 M] float64, table_info [6] float64).  device: where
-    the value/slope/curvature chain of the nets runs (None = numpy)."""
+    the tables are built: None for the host (numpy), a device for the
+    dpmi_tabulate_build_f64 kernels."""
     lower, upper = env_mat_range(davg, dstd, rcut, rcut_smth, min_nbor_dist)
     live = np.asarray(sel) > 0
     ll = float(np.min(lower[live]))
@@ -140,8 +161,11 @@ def build_tables(emb_layers, davg, dstd, rcut: float, rcut_smth: float, min_nbor
     tt[: int((uu - ll) / stride0)] = stride0
     out = []
     for layers in emb_layers:
-        vv, dd, d2 = mlp_value_derivs(xx, layers, fold_idt, device=device)
-        table = hermite_table(vv, dd, d2, tt)
+        if device is not None:
+            table = build_on_gpu(xx, tt, layers, fold_idt, device)[0].cpu().numpy()
+        else:
+            vv, dd, d2 = mlp_value_derivs(xx, layers, fold_idt)
+            table = hermite_table(vv, dd, d2, tt)
         info = np.array([int(ll), int(uu), int(uu) * extrapolate, stride0, stride1, check_frequency],
                         dtype=np.float64)
         out.append((table, info))

@@ -309,6 +309,22 @@ int dpmi_tabulate_fusion_se_a_grad_grad_f32(float* dz_dy, const float* table, co
                                             const float* dz_dy_dem, int64_t nloc, int64_t nnei,
                                             int64_t last_layer_size, int is_sorted, void* stream);
 
+/* Table builder of the compressed embedding (DPTabulate.build,
+ * deepmd/utils/tabulate.py:67-147, 273-367, with the value / slope /
+ * curvature chain of deepmd/utils/tabulate_math.py:136-251, 353-470), FP64:
+ * the embedding net of one neighbor type evaluated at the grid nodes xx
+ * [nnodes] with its first and second derivative (node_vdd [3, nnodes, M],
+ * written), then the quintic Hermite coefficients of every interval of width
+ * tt [nnodes - 1]: table [nnodes - 1, 6 M] in the layout
+ * dpmi_tabulate_fusion_se_a_f64 reads.  params: per layer w [nin][nout]
+ * (row-major), b [nout], then the factor applied to tanh [nout] (the
+ * timestep idt, or ones: the reference builder ignores idt); dims_host
+ * [nlayers][2] = (nin, nout) with nin of the first layer 1 and widths
+ * <= 1024, nlayers <= 8; nout == nin adds the layer input, nout == 2 nin adds
+ * it twice, as MLPLayer does. */
+int dpmi_tabulate_build_f64(double* table, double* node_vdd, const double* xx, const double* tt, int64_t nnodes,
+                            const double* params, const int* dims_host, int nlayers, void* stream);
+
 /* ------------------------------------------------------------------------
  * Fused embedding net + G^T R and the descriptor, bf16 (or fp16: the _f16
  * entry points) MFMA with fp32 accumulation (the precision form of

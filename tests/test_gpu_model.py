@@ -275,3 +275,28 @@ def test_table_builder_on_gpu():
             xa = ta.reshape(n, -1, 6) * scale
             xb = tb.reshape(n, -1, 6) * scale
             np.testing.assert_allclose(xb, xa, rtol=0, atol=1e-12 * np.abs(xa).max())
+
+
+@pytest.mark.parametrize("neuron,resnet", [((32, 64, 128), True), ((6, 13, 26, 26), True), ((5, 7, 9), False)],
+                         ids=["alloy", "four_layers", "no_skip"])
+def test_table_builder_kernel_node_values(neuron, resnet):
+    """dpmi_tabulate_build_f64's value / slope / curvature at the grid nodes
+    against the host chain (itself pinned to torch autograd), for nets with
+    and without skip connections and timesteps, and its coefficients."""
+    from dpmi import table
+
+    spec = systems.ModelSpec(ntypes=1, sel=(10,), neuron=neuron, resnet_dt=resnet)
+    layers = systems.make_params(spec, seed=3)["embedding"][0]
+    _say(f"table builder kernel {neuron}")
+    xx = np.append(np.arange(-1.0, 2.0, 0.01), np.arange(2.0, 10.0, 0.1))
+    tt = np.diff(xx)
+    for fold in (False, True):
+        tab, vdd = table.build_on_gpu(xx, tt, layers, fold_idt=fold, device=DEV)
+        ref = table.mlp_value_derivs(xx, layers, fold)
+        for k in range(3):
+            np.testing.assert_allclose(vdd[k].cpu().numpy(), ref[k], rtol=1e-13, atol=1e-13 * np.abs(ref[k]).max())
+        want = table.hermite_table(*ref, tt[:, None])
+        n = tt.shape[0]
+        scale = tt[:, None, None] ** np.arange(6)[None, None, :]
+        np.testing.assert_allclose(tab.cpu().numpy().reshape(n, -1, 6) * scale, want.reshape(n, -1, 6) * scale,
+                                   rtol=0, atol=1e-12 * np.abs(want.reshape(n, -1, 6) * scale).max())
