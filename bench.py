@@ -144,7 +144,7 @@ def main() -> int:
     import torch
 
     from dpmi import ops, systems
-    from dpmi.model import SeAEnergyModel
+    from dpmi.model import CapturedStep, SeAEnergyModel
     from dpmi.stat import gpu_model_params
 
     dev = torch.device("cuda:0")
@@ -159,6 +159,7 @@ def main() -> int:
     nnei = int(sum(spec.sel))
 
     peak_mem = {}
+    eager_ms = {}
 
     def run(model, names):
         torch.cuda.synchronize()
@@ -173,13 +174,28 @@ def main() -> int:
                 out = model.compute(coord, atype, sysm.box)
             torch.cuda.synchronize()
             t1 = time.perf_counter()
-        return (t1 - t0) / args.steps, out, kt.summary()
+        eager = (t1 - t0) / args.steps
+        # the same step captured in a HIP graph (list rebuilt in every replay)
+        cap = CapturedStep(model, coord, atype, sysm.box)
+        for _ in range(args.warmup):
+            cap.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            cap.replay()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        graph = (t1 - t0) / args.steps
+        if not torch.isfinite(cap.out["force"]).all():
+            raise RuntimeError("captured step produced non-finite forces")
+        eager_ms[model.precision] = round(eager * 1e3, 3)
+        return graph, cap.out, kt.summary()
 
-    names64 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f64",
+    names64 = ["dpmi_copy_coord_nowait_f64", "dpmi_build_nlist_nowait_f64", "dpmi_prod_env_mat_a_f64",
                "dpmi_force_virial_cells_f64", "dpmi_se_a_embed_fwd_f64", "dpmi_se_a_embed_bwd_f64",
                "dpmi_se_a_fitting_gr_f64", "dpmi_energy_sum_f64"]
     h = {"bf16": "bf16", "fp16": "f16"}[args.precision]
-    names32 = ["dpmi_copy_coord_f64", "dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f32c64",
+    names32 = ["dpmi_copy_coord_nowait_f64", "dpmi_build_nlist_nowait_f64", "dpmi_prod_env_mat_a_f32c64",
                "dpmi_force_virial_cells_f32", f"dpmi_se_a_embed_fwd_{h}", "dpmi_se_a_descriptor_fwd",
                "dpmi_se_a_descriptor_bwd", f"dpmi_se_a_embed_bwd_{h}", f"dpmi_se_a_fitting_{h}",
                f"dpmi_se_a_fitting_gr_{h}", "dpmi_energy_sum_f64"]
@@ -288,6 +304,9 @@ def main() -> int:
         "fp16": None if other is None else {
             "value": round(nloc / other[0], 1), "ms_per_step": round(other[0] * 1e3, 3),
             "speedup_vs_fp64": round(t64 / other[0], 3), "accuracy_vs_fp64": accuracy_of(other[1])},
+        "step_timing": "ms_per_step: the step (list rebuilt, E, F, V) captured once as a HIP graph and replayed; "
+                       "eager_ms_per_step: the same step launched from Python each time",
+        "eager_ms_per_step": eager_ms,
         "kernels_ms_per_step": {"fp64": {k: round(v["total_ms"] / args.steps, 4) for k, v in k64.items()},
                                 args.precision: {k: round(v["total_ms"] / args.steps, 4) for k, v in klo.items()}}
         | ({} if other is None else {"fp16": {k: round(v["total_ms"] / args.steps, 4) for k, v in other[2].items()}}),
@@ -322,7 +341,7 @@ def bench_tabulated(args) -> int:
     import torch
 
     from dpmi import ops, systems, table
-    from dpmi.model import SeAEnergyModel
+    from dpmi.model import CapturedStep, SeAEnergyModel
 
     dev = torch.device("cuda:0")
     spec = systems.WATER_SPEC
@@ -332,10 +351,12 @@ def bench_tabulated(args) -> int:
     sysm = systems.SYSTEMS[args.workload]()
     coord = torch.tensor(sysm.coord, device=dev)
     atype = torch.tensor(sysm.atype, device=dev)
-    names = ["dpmi_build_nlist_f64", "dpmi_prod_env_mat_a_f64", "dpmi_tabulate_fusion_se_a_f64",
+    names = ["dpmi_build_nlist_nowait_f64", "dpmi_prod_env_mat_a_f64", "dpmi_tabulate_fusion_se_a_f64",
              "dpmi_tabulate_fusion_se_a_grad_f64", "dpmi_se_a_fitting_gr_f64", "dpmi_force_virial_cells_f64"]
 
-    def run(model):
+    eager_ms = {}
+
+    def run(model, label):
         for _ in range(args.warmup):
             model.compute(coord, atype, sysm.box)
         torch.cuda.synchronize()
@@ -344,15 +365,29 @@ def bench_tabulated(args) -> int:
             for _ in range(args.steps):
                 out = model.compute(coord, atype, sysm.box)
             torch.cuda.synchronize()
-            dt = (time.perf_counter() - t0) / args.steps
-        return dt, out, kt.summary()
+            eager_ms[label] = round((time.perf_counter() - t0) / args.steps * 1e3, 3)
+        cap = CapturedStep(model, coord, atype, sysm.box)
+        for _ in range(args.warmup):
+            cap.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            cap.replay()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        if not torch.isfinite(cap.out["force"]).all():
+            raise RuntimeError("captured step produced non-finite forces")
+        return dt, cap.out, kt.summary()
 
-    tt, ot, kt = run(SeAEnergyModel.from_params(spec, params, device=dev, tables=tabs))
-    tn, on, _ = run(SeAEnergyModel.from_params(spec, params, device=dev))
+    tt, ot, kt = run(SeAEnergyModel.from_params(spec, params, device=dev, tables=tabs), "tabulated")
+    tn, on, _ = run(SeAEnergyModel.from_params(spec, params, device=dev), "network_fp64")
     frms = torch.sqrt((on["force"] ** 2).mean()).item()
     line = _line(args, sysm.nloc / tt, tt * 1e3, "f64",
                  {"workload": args.workload, "atoms": sysm.nloc, "form": "FP64, tabulated embedding "
                   "(info [0, 2, 10, 0.01, 0.1, -1], 280 intervals)"},
+                 step_timing="ms_per_step: the step captured once as a HIP graph and replayed; "
+                             "eager_ms_per_step: launched from Python each time",
+                 eager_ms_per_step=eager_ms,
                  network_fp64={"value": round(sysm.nloc / tn, 1), "ms_per_step": round(tn * 1e3, 3)},
                  table_vs_network={"force_rmse_eV_per_A": torch.sqrt(((ot["force"] - on["force"]) ** 2).mean()).item(),
                                    "force_rms_eV_per_A": frms,

@@ -753,7 +753,10 @@ __global__ void __launch_bounds__(256)
   const GridParams p = *reinterpret_cast<const GridParams*>(nlws);
   const int o = blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= nloc) return;
-  if (p.magic != kGridMagic || p.nloc != nloc) {
+  // a workspace of another system, or a neighbor list whose search reported
+  // an error (status words of dpmi_nlist_status): NaN forces
+  const int* misc = reinterpret_cast<const int*>(nlws + ws_layout(0, 0, 0).misc);
+  if (p.magic != kGridMagic || p.nloc != nloc || (misc[1] | misc[4]) != 0) {
     const double nan = __builtin_nan("");
     force[3 * (int64_t)o + 0] = nan;
     force[3 * (int64_t)o + 1] = nan;

@@ -399,10 +399,17 @@ __global__ void __launch_bounds__(1024) k_energy_sum(double* __restrict__ out, c
                                                      int64_t n) {
   __shared__ double ss[1024], cs[1024];
   double s = 0., c = 0.;
-  for (int64_t k = threadIdx.x; k < n; k += 1024) {
-    const double x = ae[k], t = s + x;
-    c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
-    s = t;
+  // eight independent loads in flight per thread, then their sums in order
+  for (int64_t k0 = threadIdx.x; k0 < n; k0 += 8 * 1024) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = k0 + 1024 * u < n ? ae[k0 + 1024 * u] : 0.;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double t = s + x[u];
+      c += fabs(s) >= fabs(x[u]) ? (s - t) + x[u] : (x[u] - t) + s;
+      s = t;
+    }
   }
   ss[threadIdx.x] = s;
   cs[threadIdx.x] = c;

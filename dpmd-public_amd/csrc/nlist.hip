@@ -153,10 +153,27 @@ __global__ void k_ws_validate(char* ws, int nloc, int nall, long long ws_bytes) 
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const GridParams& p = *reinterpret_cast<const GridParams*>(ws);
   int* misc = reinterpret_cast<int*>(ws + ws_layout(0, 0, 0).misc);
-  bool ok = p.magic == kGridMagic && p.nloc == nloc && misc[0] == nall && p.lay_nlc > 0 && p.lay_next > 0 &&
-            p.lay_nlc < (1ll << 31) && p.lay_next < (1ll << 31);
+  bool ok = p.magic == kGridMagic && p.nloc == nloc && (nall < 0 || misc[0] == nall) && misc[0] >= nloc &&
+            misc[0] < (1 << 24) && p.lay_nlc > 0 && p.lay_next > 0 && p.lay_nlc < (1ll << 31) &&
+            p.lay_next < (1ll << 31);
   if (ok) ok = ws_layout(p.nloc, p.lay_nlc, p.lay_next).total <= ws_bytes;
+  if (ok) ok = misc[4] == 0;  // the ghost copy ran out of room: nall indexes past the arrays
   if (!ok) atomicOr(&misc[1], 2);
+}
+
+// After a rejected search (status bit 2) the outputs were never written:
+// fill them with -1 (no neighbors), so that every later kernel reads valid
+// indices; its forces are then NaN (dpmi_force_virial_cells_*).
+__global__ void k_poison_rows(int* __restrict__ nlist, int* __restrict__ nlist_mapped, int16_t* __restrict__ cand_slot,
+                              int* __restrict__ overflow, int64_t nslots, int nloc, const char* __restrict__ ws) {
+  const int* misc = reinterpret_cast<const int*>(ws + ws_layout(0, 0, 0).misc);
+  if ((misc[1] & 2) == 0) return;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nslots; k += (int64_t)gridDim.x * blockDim.x) {
+    nlist[k] = -1;
+    if (nlist_mapped) nlist_mapped[k] = -1;
+    if (cand_slot) cand_slot[k] = -1;
+    if (overflow && k < nloc) overflow[k] = -1;
+  }
 }
 
 // Bounding box of an open system (one workgroup).
@@ -334,7 +351,7 @@ __global__ void k_cell_order(const int* __restrict__ cell_of, const int* __restr
 __global__ void __launch_bounds__(64) k_ghosts(double* __restrict__ out_c, int* __restrict__ out_t,
                                                int* __restrict__ mapping, const int* __restrict__ cell_start,
                                                const int* __restrict__ cell_atoms, const int* __restrict__ ghost_off,
-                                               GridParams p) {
+                                               GridParams p, int64_t mem_nall, int* __restrict__ misc) {
   int e = blockIdx.x;
   int e2 = e % p.ext[2], e1 = (e / p.ext[2]) % p.ext[1], e0 = e / (p.ext[1] * p.ext[2]);
   int ii[3] = {e0 - p.ngcell[0], e1 - p.ngcell[1], e2 - p.ngcell[2]};
@@ -353,6 +370,10 @@ __global__ void __launch_bounds__(64) k_ghosts(double* __restrict__ out_c, int* 
   int c = (jj[0] * p.ncell[1] + jj[1]) * p.ncell[2] + jj[2];
   int b = cell_start[c], cnt = cell_start[c + 1] - b;
   int64_t base = (int64_t)p.nloc + ghost_off[e];
+  if (base + cnt > mem_nall) {  // more ghosts than the output holds (the host sized it for fewer)
+    if (threadIdx.x == 0) atomicOr(&misc[4], 4);
+    cnt = (int)max((int64_t)0, mem_nall - base);
+  }
   for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
     int a = cell_atoms[b + k];
     int64_t q = base + k;
@@ -371,7 +392,7 @@ __global__ void __launch_bounds__(64) k_search_format(
     int* __restrict__ nlist, int* __restrict__ overflow, int* __restrict__ nlist_mapped,
     const double* __restrict__ ext_c, const int* __restrict__ ext_t, const int* __restrict__ mapping,
     char* __restrict__ ws, const int* __restrict__ atoms, int16_t* __restrict__ cand_slot, SelParams sp,
-    float rcut) {
+    float rcut, const int* __restrict__ count_dev) {
   extern __shared__ unsigned char smem_raw[];
   uint64_t* keys = reinterpret_cast<uint64_t*>(smem_raw);
   int* row = reinterpret_cast<int*>(keys + CAP);
@@ -383,7 +404,11 @@ __global__ void __launch_bounds__(64) k_search_format(
   const int* __restrict__ ghost_off = reinterpret_cast<const int*>(ws + L.ghost_off);
   int* misc = reinterpret_cast<int*>(ws + L.misc);
   if (misc[1] & 2) return;  // workspace rejected by k_ws_validate
-  const int i = atoms ? atoms[blockIdx.x] : (int)blockIdx.x;
+  // one atom per workgroup, or (count_dev) a fixed grid striding over the
+  // count the cell-wise pass left in device memory
+  const int nwork = count_dev ? *count_dev : (int)gridDim.x;
+  for (int wk = blockIdx.x; wk < nwork; wk += gridDim.x) {
+  const int i = atoms ? atoms[wk] : wk;
   const int lane = lane_id();
   const double rcut2d = (double)(rcut * rcut);
   const float rcut2f = rcut * rcut;
@@ -482,6 +507,8 @@ __global__ void __launch_bounds__(64) k_search_format(
   }
   for (int off = 32; off > 0; off >>= 1) ovf = max(ovf, __shfl_xor(ovf, off, kWave));
   if (overflow && lane == 0) overflow[i] = ovf;
+  __syncthreads();
+  }
 }
 
 // Step 5, cell-wise: one workgroup per local cell (grid-stride over cells).
@@ -890,13 +917,31 @@ extern "C" int dpmi_nlist_workspace_bytes(int64_t* bytes, const double* box_host
   return nlist_workspace(bytes, box_host, nloc, rcut);
 }
 
-extern "C" int dpmi_copy_coord_f64(double* out_coord, int* out_type, int* mapping, int64_t* nall, int64_t mem_nall,
-                                   const double* in_coord, const int* in_type, int64_t nloc,
-                                   const double* box_host, float rcut, void* workspace, int64_t workspace_bytes,
-                                   void* stream) {
-  DPMI_CHECK_ARG(nall != nullptr, "nall is NULL");
+namespace dpmi {
+
+// Upper bound on nall for a periodic box: every local cell has at most
+// prod_d ceil((ncell_d + 2 ngcell_d) / ncell_d) images among the extended
+// cells (itself included), so nall <= nloc times that product.
+int64_t copy_coord_capacity(const double* box_host, int64_t nloc, float rcut) {
+  const GridParams g = pbc_grid(box_host, rcut, (int)nloc);
+  int64_t m = 1;
+  for (int d = 0; d < 3; ++d) m *= (g.ncell[d] + 2 * g.ngcell[d] + g.ncell[d] - 1) / g.ncell[d];
+  return std::max<int64_t>(nloc * m, 1);
+}
+
+static int copy_coord_impl(double* out_coord, int* out_type, int* mapping, int64_t* nall, int64_t mem_nall,
+                           const double* in_coord, const int* in_type, int64_t nloc, const double* box_host,
+                           float rcut, void* workspace, int64_t workspace_bytes, void* stream, bool wait) {
+  DPMI_CHECK_ARG(wait || box_host != nullptr, "the form without waits needs a periodic box");
+  DPMI_CHECK_ARG(!wait || nall != nullptr, "nall is NULL");
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 24), "nloc out of range [0, 2^24)");
   DPMI_CHECK_ARG(mem_nall >= nloc, "mem_nall must be >= nloc");
+  if (!wait) {
+    const int64_t cap = copy_coord_capacity(box_host, nloc, rcut);
+    DPMI_CHECK_ARG(mem_nall >= cap, "mem_nall below dpmi_copy_coord_capacity");
+    DPMI_CHECK_ARG(cap < (int64_t(1) << 24), "the capacity bound reaches 2^24 extended atoms, beyond the 64-bit "
+                                             "neighbor key: use dpmi_copy_coord_f64");
+  }
   int64_t need;
   int rc = nlist_workspace(&need, box_host, nloc, rcut);
   if (rc) return rc;
@@ -967,6 +1012,14 @@ extern "C" int dpmi_copy_coord_f64(double* out_coord, int* out_type, int* mappin
                        cell_atoms, (int)nloc);
     DPMI_LAUNCH_CHECK();
   }
+  if (!wait) {
+    // nall stays in the workspace (status word 0); the capacity bound makes
+    // every ghost fit, and k_ghosts flags it if one would not
+    hipLaunchKernelGGL(k_ghosts, dim3(next_use), dim3(64), 0, st, out_coord, out_type, mapping, cell_start,
+                       cell_atoms, ghost_off, g, mem_nall, misc);
+    DPMI_LAUNCH_CHECK();
+    return DPMI_OK;
+  }
   int h_nall = 0;
   DPMI_HIP(hipMemcpyAsync(&h_nall, misc, sizeof(int), hipMemcpyDeviceToHost, st));
   DPMI_HIP(hipStreamSynchronize(st));
@@ -978,19 +1031,51 @@ extern "C" int dpmi_copy_coord_f64(double* out_coord, int* out_type, int* mappin
   DPMI_CHECK_ARG(h_nall < (1 << 24), "nall >= 2^24 is not supported by the 64-bit neighbor key");
   if (g.pbc && next_use > 0 && h_nall > nloc) {
     hipLaunchKernelGGL(k_ghosts, dim3(next_use), dim3(64), 0, st, out_coord, out_type, mapping, cell_start,
-                       cell_atoms, ghost_off, g);
+                       cell_atoms, ghost_off, g, mem_nall, misc);
     DPMI_LAUNCH_CHECK();
   }
   return DPMI_OK;
 }
 
-extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped, int16_t* cand_slot,
-                                    const double* ext_coord, const int* ext_type, const int* mapping, int64_t nloc,
-                                    int64_t nall, float rcut, const int* sel_host, int ntypes, const void* workspace,
-                                    int64_t workspace_bytes, void* stream) {
+}  // namespace dpmi
+
+extern "C" int dpmi_copy_coord_f64(double* out_coord, int* out_type, int* mapping, int64_t* nall, int64_t mem_nall,
+                                   const double* in_coord, const int* in_type, int64_t nloc,
+                                   const double* box_host, float rcut, void* workspace, int64_t workspace_bytes,
+                                   void* stream) {
+  return copy_coord_impl(out_coord, out_type, mapping, nall, mem_nall, in_coord, in_type, nloc, box_host, rcut,
+                         workspace, workspace_bytes, stream, true);
+}
+
+extern "C" int dpmi_copy_coord_capacity(int64_t* capacity, const double* box_host, int64_t nloc, float rcut) {
+  DPMI_CHECK_ARG(capacity != nullptr && box_host != nullptr && nloc >= 0 && rcut > 0.f, "bad arguments");
+  int64_t nlc, next;
+  DPMI_CHECK_ARG(grid_sizes(box_host, nloc, rcut, &nlc, &next), "degenerate box");
+  *capacity = copy_coord_capacity(box_host, nloc, rcut);
+  return DPMI_OK;
+}
+
+extern "C" int dpmi_copy_coord_nowait_f64(double* out_coord, int* out_type, int* mapping, int64_t mem_nall,
+                                          const double* in_coord, const int* in_type, int64_t nloc,
+                                          const double* box_host, float rcut, void* workspace,
+                                          int64_t workspace_bytes, void* stream) {
+  return copy_coord_impl(out_coord, out_type, mapping, nullptr, mem_nall, in_coord, in_type, nloc, box_host, rcut,
+                         workspace, workspace_bytes, stream, false);
+}
+
+namespace dpmi {
+
+// wait: read the redo count and the status back (the form that returns
+// errors); otherwise the redo pass runs on a fixed grid that reads the count
+// from the workspace, nall comes from the workspace (nall < 0) and errors
+// stay in the status words for dpmi_nlist_status.
+static int build_nlist_impl(int* nlist, int* overflow, int* nlist_mapped, int16_t* cand_slot, const double* ext_coord,
+                            const int* ext_type, const int* mapping, int64_t nloc, int64_t nall, float rcut,
+                            const int* sel_host, int ntypes, const void* workspace, int64_t workspace_bytes,
+                            void* stream, bool wait) {
   DPMI_CHECK_ARG(ntypes > 0 && ntypes <= kMaxTypes, "ntypes out of range [1, 64]");
   DPMI_CHECK_ARG(sel_host != nullptr, "sel is NULL");
-  DPMI_CHECK_ARG(nall >= nloc && nall < (int64_t(1) << 24), "nall out of range");
+  DPMI_CHECK_ARG(!wait || (nall >= nloc && nall < (int64_t(1) << 24)), "nall out of range");
   DPMI_CHECK_ARG(nlist_mapped == nullptr || mapping != nullptr, "nlist_mapped needs mapping");
   SelParams sp;
   std::memset(&sp, 0, sizeof(sp));
@@ -1010,7 +1095,8 @@ extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped
   int* misc = reinterpret_cast<int*>(ws + ws_layout(0, 0, 0).misc);
   // status words misc[1..3] (capacity error, redo count, big-cell count)
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, misc + 1, (int64_t)3, 0);
-  hipLaunchKernelGGL(k_ws_validate, dim3(1), dim3(64), 0, st, ws, (int)nloc, (int)nall, (long long)workspace_bytes);
+  hipLaunchKernelGGL(k_ws_validate, dim3(1), dim3(64), 0, st, ws, (int)nloc, wait ? (int)nall : -1,
+                     (long long)workspace_bytes);
   const size_t shm_cells = search_cells_lds(kCandCap, sp.nnei), shm_big = search_cells_lds(kCandCapBig, sp.nnei);
   if (search_uses_cells(sp.nnei)) {
     // first pass over every cell; cells whose neighborhood exceeds its table
@@ -1030,9 +1116,23 @@ extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped
     // very long rows (sum(sel)): every atom through the per-atom kernel
     size_t shm = 1024 * sizeof(uint64_t) + sizeof(int) * sp.nnei;
     hipLaunchKernelGGL(k_search_format<1024>, dim3((unsigned)nloc), dim3(64), shm, st, nlist, overflow, nlist_mapped,
-                       ext_coord, ext_type, mapping, ws, nullptr, cand_slot, sp, rcut);
+                       ext_coord, ext_type, mapping, ws, nullptr, cand_slot, sp, rcut, nullptr);
   }
   DPMI_LAUNCH_CHECK();
+  if (!wait) {
+    if (search_uses_cells(sp.nnei)) {
+      const int* redo = reinterpret_cast<const int*>(ws + ws_layout(nloc, 0, 0).redo);
+      size_t shm = 1024 * sizeof(uint64_t) + sizeof(int) * sp.nnei;
+      hipLaunchKernelGGL(k_search_format<1024>, dim3((unsigned)std::min<int64_t>(nloc, 2048)), dim3(64), shm, st,
+                         nlist, overflow, nlist_mapped, ext_coord, ext_type, mapping, ws, redo, cand_slot, sp, rcut,
+                         misc + 2);
+      DPMI_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_poison_rows, dim3(256), dim3(256), 0, st, nlist, nlist_mapped, cand_slot, overflow,
+                       nloc * (int64_t)sp.nnei, (int)nloc, ws);
+    DPMI_LAUNCH_CHECK();
+    return DPMI_OK;
+  }
   int stat[2] = {0, 0};
   DPMI_HIP(hipMemcpyAsync(stat, misc + 1, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
   DPMI_HIP(hipStreamSynchronize(st));
@@ -1041,7 +1141,7 @@ extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped
     const int* redo = reinterpret_cast<const int*>(ws + ws_layout(nloc, 0, 0).redo);
     size_t shm = 1024 * sizeof(uint64_t) + sizeof(int) * sp.nnei;
     hipLaunchKernelGGL(k_search_format<1024>, dim3(stat[1]), dim3(64), shm, st, nlist, overflow, nlist_mapped,
-                       ext_coord, ext_type, mapping, ws, redo, cand_slot, sp, rcut);
+                       ext_coord, ext_type, mapping, ws, redo, cand_slot, sp, rcut, nullptr);
     DPMI_LAUNCH_CHECK();
     DPMI_HIP(hipMemcpyAsync(stat, misc + 1, sizeof(int), hipMemcpyDeviceToHost, st));
     DPMI_HIP(hipStreamSynchronize(st));
@@ -1056,6 +1156,41 @@ extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped
     set_error("more than 1024 candidates within rcut for some atom");
     return DPMI_ERR_CAPACITY;
   }
+  return DPMI_OK;
+}
+
+// status word reported by dpmi_nlist_status: the search's error bits and
+// those of the ghost copy (bit 4: more ghosts than the output holds)
+__global__ void k_status(int* __restrict__ out, const char* __restrict__ ws) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const int* misc = reinterpret_cast<const int*>(ws + ws_layout(0, 0, 0).misc);
+    out[0] = misc[1] | misc[4];
+  }
+}
+
+}  // namespace dpmi
+
+extern "C" int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped, int16_t* cand_slot,
+                                    const double* ext_coord, const int* ext_type, const int* mapping, int64_t nloc,
+                                    int64_t nall, float rcut, const int* sel_host, int ntypes, const void* workspace,
+                                    int64_t workspace_bytes, void* stream) {
+  return build_nlist_impl(nlist, overflow, nlist_mapped, cand_slot, ext_coord, ext_type, mapping, nloc, nall, rcut,
+                          sel_host, ntypes, workspace, workspace_bytes, stream, true);
+}
+
+extern "C" int dpmi_build_nlist_nowait_f64(int* nlist, int* overflow, int* nlist_mapped, int16_t* cand_slot,
+                                           const double* ext_coord, const int* ext_type, const int* mapping,
+                                           int64_t nloc, float rcut, const int* sel_host, int ntypes,
+                                           const void* workspace, int64_t workspace_bytes, void* stream) {
+  return build_nlist_impl(nlist, overflow, nlist_mapped, cand_slot, ext_coord, ext_type, mapping, nloc, -1, rcut,
+                          sel_host, ntypes, workspace, workspace_bytes, stream, false);
+}
+
+extern "C" int dpmi_nlist_status(int* status, const void* workspace, int64_t workspace_bytes, void* stream) {
+  DPMI_CHECK_ARG(status != nullptr && workspace != nullptr && workspace_bytes >= (int64_t)(512 + 64),
+                 "bad arguments");
+  hipLaunchKernelGGL(k_status, dim3(1), dim3(64), 0, as_stream(stream), status, static_cast<const char*>(workspace));
+  DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }
 

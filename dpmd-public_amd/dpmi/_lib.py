@@ -23,6 +23,10 @@ SIGNATURES = {
     "dpmi_nlist_workspace_bytes": [_P, _P, _L, _F],
     "dpmi_copy_coord_f64": [_P, _P, _P, _P, _L, _P, _P, _L, _P, _F, _P, _L, _P],
     "dpmi_build_nlist_f64": [_P, _P, _P, _P, _P, _P, _P, _L, _L, _F, _P, _I, _P, _L, _P],
+    "dpmi_copy_coord_capacity": [_P, _P, _L, _F],
+    "dpmi_copy_coord_nowait_f64": [_P, _P, _P, _L, _P, _P, _L, _P, _F, _P, _L, _P],
+    "dpmi_build_nlist_nowait_f64": [_P] * 7 + [_L, _F, _P, _I, _P, _L, _P],
+    "dpmi_nlist_status": [_P, _P, _L, _P],
     "dpmi_prod_env_mat_a_f64": [_P, _P, _P, _P, _P, _P, _L, _P, _P, _F, _F, _P, _I, _P],
     "dpmi_prod_env_mat_a_f32": [_P, _P, _P, _P, _P, _P, _L, _P, _P, _F, _F, _P, _I, _P],
     "dpmi_prod_env_mat_a_f32c64": [_P, _P, _P, _P, _P, _P, _L, _P, _P, _F, _F, _P, _I, _P],

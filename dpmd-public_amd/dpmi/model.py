@@ -190,6 +190,8 @@ class SeAEnergyModel:
         self._order_cache = None
         self._host_order_cache = None
         self._types_checked = None
+        self._status_host = None
+        self._status_event = None
         # padding shortcut of the fused embedding kernels (each davg / dstd
         # section constant, so all padding slots of a section share one em row)
         self.pad_em = self._padding_rows() if (self.fused is not None or self.fused64 is not None) else None
@@ -237,8 +239,42 @@ class SeAEnergyModel:
     # ----------------------------------------------------------------- pieces
     def neighbor_list(self, coord, atype, box):
         """(ext_coord, ext_type, mapping, nlist, overflow, None, cells); cells
-        = (candidate slots, workspace) for the cell-wise force kernel."""
-        return ops.build_nlist(coord, atype, box, self.rcut, self.sel, cells=True)
+        = (candidate slots, workspace) for the cell-wise force kernel.  With a
+        periodic box the pipeline runs without host waits (the extended
+        arrays are sized by the capacity bound); its status is read by
+        check_status."""
+        out = ops.build_nlist(coord, atype, box, self.rcut, self.sel, cells=True, wait=box is None)
+        if box is not None and not torch.cuda.is_current_stream_capturing():
+            st = ops.nlist_status(out[6][1])
+            if self._status_host is None:
+                self._status_host = torch.zeros(1, dtype=torch.int32).pin_memory()
+            self._status_host.copy_(st, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record()
+            self._status_event = ev
+        return out
+
+    def check_status(self) -> None:
+        """Raise if the last neighbor list built without host waits reported
+        an error (its forces are NaN then).  compute() calls it for the
+        previous step; call it after the last step (waits for that step's
+        list).  A step replayed from a captured graph is not tracked: check
+        its forces for NaN."""
+        ev = self._status_event
+        if ev is None or torch.cuda.is_current_stream_capturing():
+            return
+        ev.synchronize()
+        self._status_event = None
+        code = int(self._status_host[0])
+        if code:
+            what = []
+            if code & 1:
+                what.append("an atom has more than 1024 candidates within rcut")
+            if code & 2:
+                what.append("the neighbor-list workspace belongs to another system")
+            if code & 4:
+                what.append("more ghost atoms than the capacity bound")
+            raise RuntimeError(f"neighbor list failed (status {code}: {'; '.join(what)}); its forces are NaN")
 
     def forces(self, net_deriv, nlist, cells, mapping, ext_c, ext_t, box):
         return ops.force_virial_cells(net_deriv, nlist, cells, mapping, ext_c, ext_t, self.dstd, self.rcut,
@@ -319,6 +355,7 @@ class SeAEnergyModel:
         GPU, atype [nloc] int with every type in [0, ntypes); box: 9 values or
         None.  The neighbor list is rebuilt every call."""
         self.check_types(atype)
+        self.check_status()
         atype_in = atype
         atype = atype.to(torch.int32)
         ext_c, ext_t, mapping, nlist, _ovf, _, cells = self.neighbor_list(coord, atype, box)
@@ -572,3 +609,36 @@ class SeAEnergyModel:
         if need_atom_energy:
             out["atom_energy"] = ae.detach()
         return out
+
+
+class CapturedStep:
+    """One E+F+V step of a model captured in a HIP graph (torch.cuda.CUDAGraph):
+    the neighbor list (periodic box: the form without host waits), env-mat,
+    networks, forces and virial replay as one graph launch.  coord and atype
+    are the captured inputs: copy new coordinates into ``self.coord`` (or pass
+    them to ``replay``) and replay; the box and the atom count are fixed.
+    ``self.out`` holds the outputs of the last replay.  The model's status
+    check does not see replays: a failed list shows as NaN forces."""
+
+    def __init__(self, model: SeAEnergyModel, coord: torch.Tensor, atype: torch.Tensor, box, warmup: int = 2):
+        self.model = model
+        self.coord = coord.detach().clone()
+        self.atype = atype.detach().clone()
+        self.box = box
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(max(1, warmup)):
+                model.compute(self.coord, self.atype, box)
+            model.check_status()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.out = model.compute(self.coord, self.atype, box)
+
+    def replay(self, coord: torch.Tensor | None = None) -> dict:
+        if coord is not None:
+            self.coord.copy_(coord)
+        self.graph.replay()
+        return self.out

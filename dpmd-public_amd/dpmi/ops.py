@@ -122,7 +122,7 @@ def nlist_workspace_bytes(box, nloc: int, rcut: float) -> int:
 
 
 def build_nlist(coord: torch.Tensor, atype: torch.Tensor, box, rcut: float, sel, mapped: bool = False,
-                mem_nall: int | None = None, cells: bool = False):
+                mem_nall: int | None = None, cells: bool = False, wait: bool = True):
     """Periodic (box given) or open neighbor list of one frame.
 
     coord [nloc, 3] float64, atype [nloc] int.  Returns (ext_coord [nall, 3],
@@ -130,6 +130,11 @@ def build_nlist(coord: torch.Tensor, atype: torch.Tensor, box, rcut: float, sel,
     extended indices, overflow [nloc] int32, nlist_mapped or None).  With
     cells=True a seventh item (cand_slot [nloc, nnei] int16, workspace) is
     returned for force_virial_cells.
+
+    wait=False (periodic boxes): the pipeline without host waits; the
+    extended arrays then have dpmi_copy_coord_capacity rows, of which the
+    first nall hold atoms (nall stays on the device), and errors are read
+    later through nlist_status(workspace).
     """
     _gpu(coord, atype)
     dev = coord.device
@@ -139,6 +144,26 @@ def build_nlist(coord: torch.Tensor, atype: torch.Tensor, box, rcut: float, sel,
     b = _box_host(box)
     ws_bytes = nlist_workspace_bytes(b, nloc, rcut)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    if not wait and b is not None:
+        capv = ctypes.c_int64(0)
+        call("dpmi_copy_coord_capacity", ctypes.byref(capv), ptr(b), nloc, rcut)
+        cap = int(capv.value)
+        ext_c = torch.empty((cap, 3), dtype=torch.float64, device=dev)
+        ext_t = torch.empty(cap, dtype=torch.int32, device=dev)
+        mapping = torch.empty(cap, dtype=torch.int32, device=dev)
+        _launch("dpmi_copy_coord_nowait_f64", ptr(ext_c), ptr(ext_t), ptr(mapping), cap, ptr(coord), ptr(atype), nloc,
+                ptr(b), rcut, ptr(ws), ws_bytes, _stream())
+        s = _sel_host(sel)
+        nnei = int(s.sum())
+        nlist = torch.empty((nloc, nnei), dtype=torch.int32, device=dev)
+        overflow = torch.empty(nloc, dtype=torch.int32, device=dev)
+        nl_map = torch.empty((nloc, nnei), dtype=torch.int32, device=dev) if mapped else None
+        cslot = torch.empty((nloc, nnei), dtype=torch.int16, device=dev) if cells else None
+        _launch("dpmi_build_nlist_nowait_f64", ptr(nlist), ptr(overflow), ptr(nl_map), ptr(cslot), ptr(ext_c),
+                ptr(ext_t), ptr(mapping), nloc, rcut, ptr(s), len(s), ptr(ws), ws_bytes, _stream())
+        if cells:
+            return ext_c, ext_t, mapping, nlist, overflow, nl_map, (cslot, ws)
+        return ext_c, ext_t, mapping, nlist, overflow, nl_map
     cap = mem_nall if mem_nall is not None else (nloc * 4 + 4096 if b is not None else nloc)
     nall = ctypes.c_int64(0)
     for _ in range(8):
@@ -165,6 +190,16 @@ def build_nlist(coord: torch.Tensor, atype: torch.Tensor, box, rcut: float, sel,
     if cells:
         return ext_c, ext_t, mapping, nlist, overflow, nl_map, (cslot, ws)
     return ext_c, ext_t, mapping, nlist, overflow, nl_map
+
+
+def nlist_status(workspace) -> torch.Tensor:
+    """Status word of the neighbor-list pipeline that used this workspace,
+    as an int32 device tensor (0: no error; bit 1: an atom with more than
+    1024 candidates, 2: a workspace of another system, 4: more ghosts than
+    the extended arrays hold).  Queued on the stream, no wait."""
+    out = torch.empty(1, dtype=torch.int32, device=workspace.device)
+    _launch("dpmi_nlist_status", ptr(out), ptr(workspace), workspace.numel(), _stream())
+    return out
 
 
 def format_nlist(cand, ext_coord, ext_type, nloc: int, rcut: float, sel, numneigh=None, mapping=None):

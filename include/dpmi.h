@@ -54,7 +54,8 @@ int dpmi_nlist_workspace_bytes(int64_t* bytes, const double* box_host, int64_t n
  * dpmi_build_nlist_f64.  Waits for the stream once to learn nall.
  * If nall > mem_nall, returns DPMI_ERR_CAPACITY with *nall set (the caller
  * grows the buffers and calls again, as _norm_copy_coord_cpu does).
- * With box_host == NULL nothing is copied and nall = nloc. */
+ * With box_host == NULL nothing is copied and nall = nloc (the open-system
+ * grid waits once more, for the bounding box of the coordinates). */
 int dpmi_copy_coord_f64(double* out_coord, int* out_type, int* mapping, int64_t* nall, int64_t mem_nall,
                         const double* in_coord, const int* in_type, int64_t nloc,
                         const double* box_host, float rcut, void* workspace, int64_t workspace_bytes,
@@ -70,11 +71,36 @@ int dpmi_copy_coord_f64(double* out_coord, int* out_type, int* mapping, int64_t*
  * 619-622).  cand_slot (optional, [nloc, sum(sel)] int16) receives each
  * neighbor's position in the candidate table of its centre's cell (-1 where
  * none), which dpmi_force_virial_cells_* uses; the workspace then keeps the
- * cell grid for that call. */
+ * cell grid for that call.  Waits for the stream to read its status back
+ * (twice when some atoms need the per-atom pass); the _nowait form below
+ * does not. */
 int dpmi_build_nlist_f64(int* nlist, int* overflow, int* nlist_mapped, int16_t* cand_slot,
                          const double* ext_coord, const int* ext_type, const int* mapping, int64_t nloc,
                          int64_t nall, float rcut, const int* sel_host, int ntypes, const void* workspace,
                          int64_t workspace_bytes, void* stream);
+
+/* The same pipeline without a host wait, for periodic boxes (the form a HIP
+ * graph can capture; same neighbor lists).  dpmi_copy_coord_capacity gives a
+ * bound on nall from the cell grid alone (every local cell has at most
+ * prod_d ceil((ncell_d + 2 ngcell_d) / ncell_d) images among the extended
+ * cells); the output arrays of dpmi_copy_coord_nowait_f64 hold mem_nall >=
+ * that bound entries, of which the first nall are written (nall stays in the
+ * workspace).  dpmi_build_nlist_nowait_f64 takes nall from the workspace and
+ * runs its per-atom pass on a fixed grid.  Neither reads anything back:
+ * errors (more than 1024 candidates for an atom, a workspace of another
+ * system, more ghosts than mem_nall) are bits of the status word that
+ * dpmi_nlist_status copies into device memory (0 = no error; the caller
+ * reads it when it chooses), and dpmi_force_virial_cells_* writes NaN forces
+ * for a list whose status is not 0. */
+int dpmi_copy_coord_capacity(int64_t* capacity, const double* box_host, int64_t nloc, float rcut);
+int dpmi_copy_coord_nowait_f64(double* out_coord, int* out_type, int* mapping, int64_t mem_nall,
+                               const double* in_coord, const int* in_type, int64_t nloc, const double* box_host,
+                               float rcut, void* workspace, int64_t workspace_bytes, void* stream);
+int dpmi_build_nlist_nowait_f64(int* nlist, int* overflow, int* nlist_mapped, int16_t* cand_slot,
+                                const double* ext_coord, const int* ext_type, const int* mapping, int64_t nloc,
+                                float rcut, const int* sel_host, int ntypes, const void* workspace,
+                                int64_t workspace_bytes, void* stream);
+int dpmi_nlist_status(int* status, const void* workspace, int64_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
  * Environment matrix.  Replaces prod_env_mat_a_gpu
@@ -161,7 +187,8 @@ int dpmi_env_mat_pt_grad_grad_f32(float* gg_env_mat, float* gg_diff, float* gg_s
  * skipping -1) is formatted like the search output: neighbors of type in
  * [0, ntypes) with float r^2 <= rc^2, by type section, sorted by (float r^2,
  * index), -1 padding, overflow as above.  Candidates must be distinct.
- * workspace: 4 bytes of device memory (a status word). */
+ * workspace: 4 bytes of device memory (a status word).  Waits for the stream
+ * to read that status back. */
 int dpmi_format_nlist_f64(int* nlist, int* overflow, int* nlist_mapped, const int* cand, const int* numneigh,
                           int64_t ncand, const double* ext_coord, const int* ext_type, const int* mapping,
                           int64_t nloc, int64_t nall, float rcut, const int* sel_host, int ntypes, void* workspace,
