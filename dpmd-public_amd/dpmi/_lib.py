@@ -70,6 +70,11 @@ SIGNATURES = {
     "dpmi_se_a_fitting_f64_frag_doubles": [_P, _I, _I],
     "dpmi_energy_sum_f64": [_P, _P, _L, _P],
     "dpmi_tabulate_build_f64": [_P, _P, _P, _P, _L, _P, _P, _I, _P],
+    "dpmi_lr_tick": [_P, _P, _P, _P],
+    "dpmi_adam_f64": [_P, _P, _P, _P, _L, _P, _D, _D, _D, _D, _P],
+    "dpmi_adam_f32": [_P, _P, _P, _P, _L, _P, _D, _D, _D, _D, _P],
+    "dpmi_row_sum_f64": [_P, _P, _P, _L, _L, _P],
+    "dpmi_row_sum_f32": [_P, _P, _P, _L, _L, _P],
     "dpmi_tabulate_fusion_se_a_f64": [_P, _P, _P, _P, _P, _L, _L, _L, _I, _P],
     "dpmi_tabulate_fusion_se_a_f32": [_P, _P, _P, _P, _P, _L, _L, _L, _I, _P],
     "dpmi_tabulate_fusion_se_a_grad_f64": [_P, _P, _P, _P, _P, _P, _P, _L, _L, _L, _I, _P],

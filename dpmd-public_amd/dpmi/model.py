@@ -75,14 +75,19 @@ _SPLIT_ROWS = 1 << 15
 def _split_k_linear(x, w, b):
     """x @ w + b as a batched product over row blocks with the weight
     broadcast, so the weight gradient is a batched product summed over the
-    blocks (split K) and stays differentiable for the double backward."""
+    blocks (split K) and stays differentiable for the double backward.  The
+    bias is broadcast by ops.RowBroadcast, whose gradient is the fixed-order
+    HIP row sum (no library multi-block reduction: those clear a semaphore
+    with a memory-set node, which a captured training step must not hold)."""
     rows = x.shape[0]
     nb = min(256, rows // 1024)
     pad = (-rows) % nb
     if pad:
         x = torch.cat([x, x.new_zeros((pad, x.shape[1]))])
-    y = torch.bmm(x.view(nb, -1, x.shape[1]), w.unsqueeze(0).expand(nb, -1, -1)).reshape(-1, w.shape[1]) + b
-    return y[:rows] if pad else y
+    y = torch.bmm(x.view(nb, -1, x.shape[1]), w.unsqueeze(0).expand(nb, -1, -1)).reshape(-1, w.shape[1])
+    if pad:
+        y = y[:rows]
+    return y + ops.RowBroadcast.apply(b, rows)
 
 
 class Net:
@@ -103,10 +108,12 @@ class Net:
 
     def __call__(self, x, gemm_dtype=None):
         for w, b, idt, act in zip(self.w, self.b, self.idt, self.act):
+            rows_bc = False
             if gemm_dtype is None and w.requires_grad and x.dim() == 2 and x.shape[0] >= _SPLIT_ROWS:
                 # training on neighbor rows: dL/dW would be one GEMM with a tiny
                 # output and K = all rows (a single workgroup); split K
                 y = _split_k_linear(x, w, b)
+                rows_bc = True
             elif gemm_dtype is None or w.shape[1] < 16:
                 # fp64 path, or the 240 -> 1 output layer: a per-atom dot product
                 # (the low-precision N=1 GEMM path of the BLAS is pathologically slow)
@@ -117,7 +124,7 @@ class Net:
             if act:
                 y = _ACT(y)
             if idt is not None:
-                y = y * idt
+                y = y * (ops.RowBroadcast.apply(idt, y.shape[0]) if rows_bc else idt)
             if act:
                 nin, nout = w.shape
                 if nin == nout:

@@ -675,6 +675,74 @@ def prod_virial_se_a(net_deriv, in_deriv, rij, nlist, natoms, n_a_sel: int, n_r_
     return [v, av]
 
 
+# --------------------------------------------------------------------------- training step
+
+def lr_tick(hyper: torch.Tensor, step: torch.Tensor, sched) -> None:
+    """Start of a training step on the device: hyper [4] float64 = [lr_k,
+    pref_e, pref_f, k + 1] for k = step (int64 [1]), then step = k + 1
+    (dpmi_lr_tick; sched is the 11-value schedule of the C header)."""
+    _gpu(hyper, step)
+    if hyper.dtype != torch.float64 or hyper.numel() < 4 or not hyper.is_contiguous():
+        raise ValueError("hyper must be a contiguous float64 tensor of 4 values")
+    if step.dtype != torch.int64 or step.numel() != 1:
+        raise ValueError("step must be one int64 value")
+    s = np.ascontiguousarray(np.asarray(sched, dtype=np.float64))
+    if s.shape != (11,):
+        raise ValueError("the schedule has 11 values")
+    call("dpmi_lr_tick", ptr(hyper), ptr(step), ptr(s), _stream())
+
+
+def adam_step(param, grad, exp_avg, exp_avg_sq, hyper, betas=(0.9, 0.999), eps: float = 1e-8,
+              weight_decay: float = 0.0) -> None:
+    """In-place Adam over flat contiguous buffers of one dtype (dpmi_adam_*),
+    lr and t read from hyper on the device."""
+    _gpu(param, grad, exp_avg, exp_avg_sq, hyper)
+    ts = (param, grad, exp_avg, exp_avg_sq)
+    if any(t.dtype != param.dtype or not t.is_contiguous() or t.numel() != param.numel() for t in ts):
+        raise ValueError("param, grad, exp_avg and exp_avg_sq must be contiguous, of one dtype and size")
+    _launch(f"dpmi_adam_{_sfx(param.dtype)}", ptr(param), ptr(grad), ptr(exp_avg), ptr(exp_avg_sq), param.numel(),
+            ptr(hyper), float(betas[0]), float(betas[1]), float(eps), float(weight_decay), _stream())
+
+
+def row_sum(x: torch.Tensor) -> torch.Tensor:
+    """Column sums of a 2-D tensor [R, F] in a fixed order (dpmi_row_sum_*)."""
+    _gpu(x)
+    if x.dim() != 2:
+        raise ValueError("row_sum takes a 2-D tensor")
+    x = x.contiguous()
+    R, F = x.shape
+    out = torch.empty(F, dtype=x.dtype, device=x.device)
+    part = torch.empty(256 * F, dtype=x.dtype, device=x.device)
+    _launch(f"dpmi_row_sum_{_sfx(x.dtype)}", ptr(out), ptr(x), ptr(part), R, F, _stream())
+    return out
+
+
+class RowBroadcast(torch.autograd.Function):
+    """v [F] -> [R, F] with every row v; the gradient is the fixed-order
+    row sum (RowSum), whose gradient is this broadcast again, so the pair is
+    differentiable to any order."""
+
+    @staticmethod
+    def forward(ctx, v, rows: int):
+        ctx.rows = rows
+        return v.unsqueeze(0).expand(rows, v.shape[0])
+
+    @staticmethod
+    def backward(ctx, g):
+        return RowSum.apply(g), None
+
+
+class RowSum(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        ctx.rows = x.shape[0]
+        return row_sum(x)
+
+    @staticmethod
+    def backward(ctx, g):
+        return RowBroadcast.apply(g, ctx.rows)
+
+
 # --------------------------------------------------------------------------- registration
 
 _REGISTERED = False

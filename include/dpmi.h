@@ -484,6 +484,42 @@ int dpmi_se_a_fitting_f64_frag_doubles(int64_t* count, int M, int H);
  * deepmd/pt/model/model/transform_output.py:165,181). */
 int dpmi_energy_sum_f64(double* energy, const double* atom_energy, int64_t n, void* stream);
 
+/* ------------------------------------------------------ training step
+ * The optimizer side of the training step (deepmd/pt/train/training.py:
+ * 1393-1460), with the step count and the rate kept in device memory so
+ * that the whole step can be captured and replayed as a HIP graph.  Neither
+ * function waits on the stream. */
+
+/* Learning-rate tick at the start of a step: with k = *step (steps begun so
+ * far, int64 on the device) writes hyper [4] (device) = [lr_k, pref_e,
+ * pref_f, k + 1] and sets *step = k + 1.  lr_k follows the LambdaLR over
+ * LearningRateExp that the reference trainer builds (training.py:1138-1150;
+ * deepmd/dpmodel/utils/learning_rate.py:167-217, 367-397); the prefactors
+ * follow lr as EnergyStdLoss does (deepmd/pt/loss/ener.py:229-231).
+ * sched_host (host) [11] = start_lr, decay_rate, stop_lr, decay_steps,
+ * warmup_steps, warmup_start_lr, smooth (0/1), start_pref_e, limit_pref_e,
+ * start_pref_f, limit_pref_f; its values are read at the call. */
+int dpmi_lr_tick(double* hyper, int64_t* step, const double* sched_host, void* stream);
+
+/* One Adam step over n elements of a flat parameter buffer, in place on
+ * param, exp_avg and exp_avg_sq, with lr = hyper[0] and t = hyper[3] read on
+ * the device: replaces torch.optim.Adam(fused=True) of the reference trainer
+ * (training.py:1028-1030, stepped at 1451-1452; no amsgrad, no maximize).
+ * exp_avg / exp_avg_sq start at zero. */
+int dpmi_adam_f64(double* param, const double* grad, double* exp_avg, double* exp_avg_sq, int64_t n,
+                  const double* hyper, double beta1, double beta2, double eps, double weight_decay, void* stream);
+int dpmi_adam_f32(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
+                  const double* hyper, double beta1, double beta2, double eps, double weight_decay, void* stream);
+
+/* out [F] = column sums of x [R, F] (row-major) in a fixed order (two
+ * passes; part: workspace of 256 * F elements).  The gradient of a bias or
+ * timestep broadcast over the R neighbor rows of an embedding layer
+ * (MLPLayer, deepmd/pt/model/network/mlp.py:188-219) in the training step,
+ * deterministic and free of memory-set nodes when captured in a graph; the
+ * reference leaves this reduction to torch autograd. */
+int dpmi_row_sum_f64(double* out, const double* x, double* part, int64_t R, int64_t F, void* stream);
+int dpmi_row_sum_f32(float* out, const float* x, float* part, int64_t R, int64_t F, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
