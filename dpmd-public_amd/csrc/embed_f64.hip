@@ -413,13 +413,39 @@ int make_params64(Embed64Params* p, const int* sel_host, int ntypes, int n1, int
     p->sec[t + 1] = p->sec[t] + sel_host[t];
   }
   p->nnei = p->sec[ntypes];
-  // one workgroup per CU at a time (the weights fill most of the LDS): two
-  // rounds of 256 workgroups, split over types in proportion to their rows
-  const int64_t total = std::max<int64_t>(ntypes, std::min<int64_t>(512, ceil_div(nloc, kWaves64) * ntypes));
+  // One workgroup per CU at a time (the weights fill most of the LDS): two
+  // full rounds over the CUs, apportioned to the types in proportion to their
+  // rows by largest remainder so that the total is exact (a rounded-up share
+  // per type would leave one workgroup running alone in a third round).
+  static int ncu = 0;
+  if (ncu == 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+      ncu = v;
+    else
+      ncu = 256;
+  }
+  const int64_t cap = ceil_div(std::max<int64_t>(nloc, 1), kWaves64);  // workgroups with an atom for each wave
+  const int64_t total = std::max<int64_t>(ntypes, std::min<int64_t>(2 * ncu, cap * ntypes));
+  int64_t share[kMaxTypes64], rem[kMaxTypes64], used = 0;
+  for (int t = 0; t < ntypes; ++t) {
+    const int64_t num = p->nnei > 0 ? total * sel_host[t] : total;
+    const int64_t den = p->nnei > 0 ? p->nnei : ntypes;
+    share[t] = num / den;
+    rem[t] = num % den;
+    used += share[t];
+  }
+  for (int64_t left = total - used; left > 0; --left) {  // the largest remainders get one more
+    int best = 0;
+    for (int t = 1; t < ntypes; ++t)
+      if (rem[t] > rem[best]) best = t;
+    ++share[best];
+    rem[best] = -1;
+  }
   p->blk[0] = 0;
   for (int t = 0; t < ntypes; ++t) {
-    int64_t b = p->nnei > 0 ? (total * sel_host[t] + p->nnei - 1) / p->nnei : 1;
-    b = std::max<int64_t>(1, std::min<int64_t>(b, ceil_div(std::max<int64_t>(nloc, 1), kWaves64)));
+    const int64_t b = std::max<int64_t>(1, std::min<int64_t>(share[t], cap));
     p->blk[t + 1] = p->blk[t] + (int)b;
   }
   p->nblk_total = p->blk[ntypes];

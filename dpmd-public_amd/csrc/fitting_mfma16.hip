@@ -14,10 +14,12 @@
 // turns into dE/d em.
 //
 // MI355X mapping.  A workgroup owns 64 atoms of one type (the list is sorted
-// by type); its four wavefronts split every hidden layer by output feature
-// (wavefront w owns features 64w .. 64w+63 of the width padded to 256), so a
-// feature's tanh factor, skip value and adjoint stay in the registers of one
-// lane for the whole pass.  Layers are computed transposed, y^T = W^T x^T,
+// by type); its eight wavefronts split every hidden layer by output feature
+// and atom: wavefront w owns features 64 (w % 4) .. 64 (w % 4) + 63 of the
+// width padded to 256 for atoms 32 (w / 4) .. 32 (w / 4) + 31, so a feature's
+// tanh factor, skip value and adjoint stay in the registers of one lane for
+// the whole pass, and two wavefronts share each SIMD (the per-lane state of
+// half the atoms fits the 256 registers of two waves per SIMD).  Layers are computed transposed, y^T = W^T x^T,
 // with v_mfma_f32_16x16x32_bf16 / _f16: the weight fragment is the A operand (M =
 // output feature), the layer input is the B operand (K = input feature,
 // N = atom), read from an LDS buffer [atom][feature] that every layer's
@@ -44,8 +46,10 @@ template <typename H>
 using h4 = H __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kH = 256;         // padded hidden width (4 wavefronts x 64 features)
+constexpr int kH = 256;         // padded hidden width (4 feature groups x 64 features)
 constexpr int kTile = 64;       // atoms per workgroup
+constexpr int kThreads = 512;   // eight wavefronts: 4 feature groups x 2 atom halves
+constexpr int kSub = 2;         // 16-atom subtiles per wavefront
 constexpr int kKB = 64;         // layer-1 input features staged per step
 constexpr int kXS = kKB + 8;    // LDS row stride (16-bit elements) of the staged D block
 constexpr int kHS = kH + 8;     // LDS row stride (16-bit elements) of the hidden-layer buffer
@@ -98,42 +102,43 @@ __device__ inline h8<H> lds_b(const H* buf, int stride, int s, int n, int kbase,
 }
 
 // Writes a layer output (D layout: lane -> atom n of subtile s, features
-// 4g + v of M tile mt) to the LDS buffer as H.
+// 4g + v of M tile mt of feature group fg) to the LDS buffer as H.
 template <typename H>
-__device__ inline void lds_put(H* buf, int w, int mt, int s, int n, int g, const float* v4) {
+__device__ inline void lds_put(H* buf, int fg, int mt, int s, int n, int g, const float* v4) {
   h4<H> q;
   q[0] = (H)v4[0];
   q[1] = (H)v4[1];
   q[2] = (H)v4[2];
   q[3] = (H)v4[3];
-  *reinterpret_cast<h4<H>*>(buf + (16 * s + n) * kHS + 16 * (4 * w + mt) + 4 * g) = q;
+  *reinterpret_cast<h4<H>*>(buf + (16 * s + n) * kHS + 16 * (4 * fg + mt) + 4 * g) = q;
 }
 
-// One hidden H -> H layer product for this wavefront's four M tiles and the
-// 64 atoms: acc[mt][s] = sum_k W(frag)[feature][k] * in[atom][k].
+// One hidden H -> H layer product for this wavefront's four M tiles (feature
+// group fg) and its two subtiles (s0, s0 + 1): acc[mt][s] = sum_k
+// W(frag)[feature][k] * in[atom][k].
 template <typename H>
-__device__ inline void hidden_product(f32x4 (&acc)[4][4], const h8<H>* __restrict__ frag, const H* s_h,
-                                      int w, int lane, int n, int g) {
+__device__ inline void hidden_product(f32x4 (&acc)[4][kSub], const h8<H>* __restrict__ frag, const H* s_h,
+                                      int fg, int s0, int lane, int n, int g) {
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-    for (int s = 0; s < 4; ++s) acc[mt][s] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int s = 0; s < kSub; ++s) acc[mt][s] = f32x4{0.f, 0.f, 0.f, 0.f};
   h8<H> a[4];
 #pragma unroll
-  for (int mt = 0; mt < 4; ++mt) a[mt] = frag[((4 * w + mt) * 8 + 0) * 64 + lane];
+  for (int mt = 0; mt < 4; ++mt) a[mt] = frag[((4 * fg + mt) * 8 + 0) * 64 + lane];
 #pragma unroll 1
   for (int kc = 0; kc < 8; ++kc) {
     h8<H> an[4];
     const int kn = kc + 1 < 8 ? kc + 1 : kc;
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) an[mt] = frag[((4 * w + mt) * 8 + kn) * 64 + lane];
-    h8<H> b[4];
+    for (int mt = 0; mt < 4; ++mt) an[mt] = frag[((4 * fg + mt) * 8 + kn) * 64 + lane];
+    h8<H> b[kSub];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) b[s] = lds_b(s_h, kHS, s, n, 32 * kc, g);
+    for (int s = 0; s < kSub; ++s) b[s] = lds_b(s_h, kHS, s0 + s, n, 32 * kc, g);
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-      for (int s = 0; s < 4; ++s) acc[mt][s] = mfma(a[mt], b[s], acc[mt][s]);
+      for (int s = 0; s < kSub; ++s) acc[mt][s] = mfma(a[mt], b[s], acc[mt][s]);
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) a[mt] = an[mt];
   }
@@ -148,7 +153,7 @@ __device__ inline void hidden_product(f32x4 (&acc)[4][4], const h8<H>* __restric
 // written / nnei in the embedding kernels' internal order as dgr
 // [nloc, 128, 4] (component fastest: one 16-byte store per atom and feature).
 template <typename H, bool kGR>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kThreads)
     k_fit(double* __restrict__ atom_energy, float* __restrict__ dout, const H* __restrict__ desc,
           const float* __restrict__ gr_part, const int* __restrict__ perm, const int* __restrict__ type_start,
           const h8<H>* __restrict__ frag_all, const float* __restrict__ consts, const double* __restrict__ ebias,
@@ -174,140 +179,145 @@ __global__ void __launch_bounds__(256)
   }
   if (t < 0) return;
 
+  // wavefront w: feature group fg (features 64 fg .. 64 fg + 63), subtiles s0, s0 + 1
   const int tid = threadIdx.x, w = wave_id(), lane = tid & 63, n = lane & 15, g = lane >> 4;
+  const int fg = w & 3, s0 = kSub * (w >> 2);
   if (tid < kTile) s_atom[tid] = tid < cnt ? perm[start + tid] : -1;
   const h8<H>* frag = frag_all + (int64_t)t * p.type_frags;
   const float* cst = consts + (int64_t)t * kCst;
   __syncthreads();
 
-  // ---- layer 1: D block loads (thread -> atom tid/4, features 16*(tid%4) .. +15 of the block, H)
-  const int xl = tid >> 2, xa = s_atom[xl], xq = tid & 3;
+  // ---- layer 1: D block loads (thread -> atom tid/8, features 8*(tid%8) .. +7 of the block, H)
+  const int xl = tid >> 3, xa = s_atom[xl], xq = tid & 7;
   const H* xrow = desc + (int64_t)(xa < 0 ? 0 : xa) * p.k1;
   const int nkb = p.nk1 / 2;
-  auto load_block = [&](int kb, h8<H> (&r)[2]) {
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int f = kb * kKB + 16 * xq + 8 * j;
-      // streamed once: non-temporal, so it does not evict the weight fragments from L2
-      r[j] = (xa >= 0 && f < p.k1) ? __builtin_nontemporal_load(reinterpret_cast<const h8<H>*>(xrow + f))
-                                   : h8<H>{};
-    }
+  auto load_block = [&](int kb, h8<H>& r) {
+    const int f = kb * kKB + 8 * xq;
+    // streamed once: non-temporal, so it does not evict the weight fragments from L2
+    r = (xa >= 0 && f < p.k1) ? __builtin_nontemporal_load(reinterpret_cast<const h8<H>*>(xrow + f)) : h8<H>{};
   };
-  auto store_block = [&](int buf, const h8<H> (&r)[2]) {
-    H* dst = s_x[buf] + xl * kXS + 16 * xq;
-    *reinterpret_cast<h8<H>*>(dst) = r[0];
-    *reinterpret_cast<h8<H>*>(dst + 8) = r[1];
+  auto store_block = [&](int buf, const h8<H>& r) {
+    *reinterpret_cast<h8<H>*>(s_x[buf] + xl * kXS + 8 * xq) = r;
   };
   float4* sgr4 = reinterpret_cast<float4*>(s_gr);
-  // D block kb of atom xl, features (m = 4 kb + ml, a = 4 xq .. 4 xq + 3):
-  // the thread's GR[.][a] (ga) stay in registers through layer 1
+  // D block kb of atom xl, features (m = 4 kb + 2 mh + ml, a = 4 aq .. 4 aq + 3)
+  // with aq = xq % 4, mh = xq / 4: the thread's GR[.][a] (ga) stay in
+  // registers through layer 1
+  const int aq = xq & 3, mh = xq >> 2;
   float4 ga[4];
-  auto gen_block = [&](int kb, h8<H> (&r)[2]) {
+  auto gen_block = [&](int kb, h8<H>& r) {
 #pragma unroll
-    for (int ml = 0; ml < 4; ++ml) {
-      const float4 gm = sgr4[xl * (kGRS / 4) + 4 * kb + ml];
+    for (int ml = 0; ml < 2; ++ml) {
+      const float4 gm = sgr4[xl * (kGRS / 4) + 4 * kb + 2 * mh + ml];
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        r[ml >> 1][4 * (ml & 1) + j] = (H)(gm.x * ga[j].x + gm.y * ga[j].y + gm.z * ga[j].z + gm.w * ga[j].w);
+        r[4 * ml + j] = (H)(gm.x * ga[j].x + gm.y * ga[j].y + gm.z * ga[j].z + gm.w * ga[j].w);
     }
   };
-  auto store_gen = [&](int buf, const h8<H> (&r)[2]) {
-    // r[ml / 2][4 (ml % 2) + j] -> feature 16 ml + 4 xq + j
-    H* dst = s_x[buf] + xl * kXS + 4 * xq;
+  auto store_gen = [&](int buf, const h8<H>& r) {
+    // r[4 ml + j] -> feature 16 (2 mh + ml) + 4 aq + j
+    H* dst = s_x[buf] + xl * kXS + 32 * mh + 4 * aq;
 #pragma unroll
-    for (int ml = 0; ml < 4; ++ml) {
+    for (int ml = 0; ml < 2; ++ml) {
       h4<H> q;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) q[j] = r[ml >> 1][4 * (ml & 1) + j];
+      for (int j = 0; j < 4; ++j) q[j] = r[4 * ml + j];
       *reinterpret_cast<h4<H>*>(dst + 16 * ml) = q;
     }
   };
 
-  f32x4 acc[4][4];
+  f32x4 acc[4][kSub];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-    for (int s = 0; s < 4; ++s) acc[mt][s] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int s = 0; s < kSub; ++s) acc[mt][s] = f32x4{0.f, 0.f, 0.f, 0.f};
   if constexpr (kGR) {
-    // all of the tile's GR at once, kBatch float4 of every type in flight per
-    // thread (row-major over [atom][c][m] so each load instruction reads
-    // contiguous memory); GR[c][m] -> s_gr[atom][m][c]
-    constexpr int kBatch = 8;
-    const int m4n = p.M >> 2, n4 = kTile * p.M;  // float4 per (atom, c) row / per tile
-    const float inv_m = 1.f / (float)p.M, inv_m4 = 1.f / (float)m4n;
+    // all of the tile's GR at once: work item (atom la, feature m), kBatch
+    // items per thread in flight; the four components of every type are read
+    // as scalars (consecutive lanes read consecutive m: coalesced) and
+    // written as one float4 GR[.][m] (consecutive lanes write consecutive LDS
+    // addresses: no bank conflicts); GR[c][m] -> s_gr[atom][m][c]
+    constexpr int kBatch = 4;
+    const int nit = kTile * p.M;
+    const float inv_m = 1.f / (float)p.M;
 #pragma unroll 1
-    for (int k0 = tid; k0 < n4; k0 += 256 * kBatch) {
-      f32x4 x[kBatch][kMaxTypesGR];
+    for (int k0 = tid; k0 < nit; k0 += kThreads * kBatch) {
+      float x[kBatch][kMaxTypesGR][4];
 #pragma unroll
       for (int b = 0; b < kBatch; ++b) {
-        const int k = k0 + 256 * b;
-        const int la = k < n4 ? __float2int_rz(((float)k + 0.5f) * inv_m) : 0;
-        const int ai = k < n4 ? s_atom[la] : -1;
+        const int k = k0 + kThreads * b;
+        const int la = k < nit ? __float2int_rz(((float)k + 0.5f) * inv_m) : 0;
+        const int ai = k < nit ? s_atom[la] : -1;
+        const int m = k - la * p.M;
 #pragma unroll
-        for (int tt = 0; tt < kMaxTypesGR; ++tt)
-          x[b][tt] = (ai >= 0 && tt < p.ntypes)
-                         ? __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(
-                               gr_part + ((int64_t)tt * p.nloc + ai) * 4 * p.M) + (k - la * p.M))
-                         : f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int tt = 0; tt < kMaxTypesGR; ++tt) {
+          const float* src = gr_part + ((int64_t)tt * p.nloc + (ai < 0 ? 0 : ai)) * 4 * p.M + m;
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            x[b][tt][c] = (ai >= 0 && tt < p.ntypes) ? __builtin_nontemporal_load(src + c * p.M) : 0.f;
+        }
       }
 #pragma unroll
       for (int b = 0; b < kBatch; ++b) {
-        const int k = k0 + 256 * b;
-        if (k >= n4) break;
-        const int la = __float2int_rz(((float)k + 0.5f) * inv_m), f4 = k - la * p.M;
-        const int c = __float2int_rz(((float)f4 + 0.5f) * inv_m4), m = 4 * (f4 - c * m4n);
-        f32x4 v = x[b][0];
+        const int k = k0 + kThreads * b;
+        if (k >= nit) break;
+        const int la = __float2int_rz(((float)k + 0.5f) * inv_m), m = k - la * p.M;
+        float v[4];
 #pragma unroll
-        for (int tt = 1; tt < kMaxTypesGR; ++tt) v += x[b][tt];
-        float* dst = s_gr + la * kGRS + 4 * m + c;
+        for (int c = 0; c < 4; ++c) {
+          v[c] = x[b][0][c];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) dst[4 * j] = v[j] * p.inv_nnei;
+          for (int tt = 1; tt < kMaxTypesGR; ++tt) v[c] += x[b][tt][c];
+        }
+        sgr4[la * (kGRS / 4) + m] =
+            make_float4(v[0] * p.inv_nnei, v[1] * p.inv_nnei, v[2] * p.inv_nnei, v[3] * p.inv_nnei);
       }
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 4; ++j) ga[j] = sgr4[xl * (kGRS / 4) + 4 * xq + j];
-    h8<H> r[2];
+    for (int j = 0; j < 4; ++j) ga[j] = sgr4[xl * (kGRS / 4) + 4 * aq + j];
+    h8<H> r;
     gen_block(0, r);
     store_gen(0, r);
   } else {
-    h8<H> r[2];
+    h8<H> r;
     load_block(0, r);
     store_block(0, r);
   }
   __syncthreads();
   const h8<H>* w1f = frag + p.w1f;
-  // weight fragments of the next K block are loaded one block ahead
-  h8<H> a_cur[2][4];
+  // weight fragments are loaded two K blocks ahead (an L2 round trip is
+  // longer than one block's products)
+  auto load_w1 = [&](int kb, h8<H> (&a)[2][4]) {
 #pragma unroll
-  for (int kc = 0; kc < 2; ++kc)
+    for (int kc = 0; kc < 2; ++kc)
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) a_cur[kc][mt] = w1f[((4 * w + mt) * p.nk1 + kc) * 64 + lane];
+      for (int mt = 0; mt < 4; ++mt) a[kc][mt] = w1f[((4 * fg + mt) * p.nk1 + 2 * kb + kc) * 64 + lane];
+  };
+  h8<H> a_cur[2][4], a_nxt[2][4];
+  load_w1(0, a_cur);
+  load_w1(nkb > 1 ? 1 : 0, a_nxt);
 #pragma unroll 1
   for (int kb = 0; kb < nkb; ++kb) {
-    h8<H> r[2];
-    h8<H> a_nxt[2][4];
+    h8<H> r;
+    h8<H> a_nn[2][4];
     const bool more = kb + 1 < nkb;
     // fused form: D block kb + 1 is built from the staged GR
-    const int kn = more ? kb + 1 : kb;
     if constexpr (kGR) {
       if (more) gen_block(kb + 1, r);
     } else {
       if (more) load_block(kb + 1, r);
     }
-#pragma unroll
-    for (int kc = 0; kc < 2; ++kc)
-#pragma unroll
-      for (int mt = 0; mt < 4; ++mt) a_nxt[kc][mt] = w1f[((4 * w + mt) * p.nk1 + 2 * kn + kc) * 64 + lane];
+    load_w1(kb + 2 < nkb ? kb + 2 : nkb - 1, a_nn);
 #pragma unroll
     for (int kc = 0; kc < 2; ++kc) {
-      h8<H> b[4];
+      h8<H> b[kSub];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) b[s] = lds_b(s_x[kb & 1], kXS, s, n, 32 * kc, g);
+      for (int s = 0; s < kSub; ++s) b[s] = lds_b(s_x[kb & 1], kXS, s0 + s, n, 32 * kc, g);
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) acc[mt][s] = mfma(a_cur[kc][mt], b[s], acc[mt][s]);
+        for (int s = 0; s < kSub; ++s) acc[mt][s] = mfma(a_cur[kc][mt], b[s], acc[mt][s]);
     }
     if constexpr (kGR) {
       if (more) store_gen((kb + 1) & 1, r);
@@ -318,15 +328,18 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int kc = 0; kc < 2; ++kc)
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt) a_cur[kc][mt] = a_nxt[kc][mt];
+      for (int mt = 0; mt < 4; ++mt) {
+        a_cur[kc][mt] = a_nxt[kc][mt];
+        a_nxt[kc][mt] = a_nn[kc][mt];
+      }
   }
   // ---- layer-1 epilogue: h (fp32, skip of layer 2), tanh factors f1 (fp32)
-  float h[4][4][4], f1[4][4][4], f2[4][4][4], f3[4][4][4];
+  float h[4][kSub][4], f1[4][kSub][4], f2[4][kSub][4], f3[4][kSub][4];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
-    const int j0 = 16 * (4 * w + mt) + 4 * g;
+    const int j0 = 16 * (4 * fg + mt) + 4 * g;
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < kSub; ++s) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const float tt = fast_tanh(acc[mt][s][v] + cst[0 * kH + j0 + v]);
@@ -334,7 +347,7 @@ __global__ void __launch_bounds__(256)
         h[mt][s][v] = tt * idt;
         f1[mt][s][v] = idt * (1.f - tt * tt);
       }
-      lds_put(s_h, w, mt, s, n, g, h[mt][s]);
+      lds_put(s_h, fg, mt, s0 + s, n, g, h[mt][s]);
     }
   }
   __syncthreads();
@@ -342,15 +355,15 @@ __global__ void __launch_bounds__(256)
   // ---- layers 2 and 3 (H -> H with skip)
 #pragma unroll
   for (int layer = 0; layer < 2; ++layer) {
-    hidden_product(acc, frag + (layer == 0 ? p.w2f : p.w3f), s_h, w, lane, n, g);
+    hidden_product(acc, frag + (layer == 0 ? p.w2f : p.w3f), s_h, fg, s0, lane, n, g);
     __syncthreads();  // every wavefront has read the layer input
     const float* bl = cst + (2 + 2 * layer) * kH;
     const float* il = cst + (3 + 2 * layer) * kH;
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-      const int j0 = 16 * (4 * w + mt) + 4 * g;
+      const int j0 = 16 * (4 * fg + mt) + 4 * g;
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
+      for (int s = 0; s < kSub; ++s) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const float tt = fast_tanh(acc[mt][s][v] + bl[j0 + v]);
@@ -358,7 +371,7 @@ __global__ void __launch_bounds__(256)
           h[mt][s][v] += tt * idt;
           (layer == 0 ? f2 : f3)[mt][s][v] = idt * (1.f - tt * tt);
         }
-        if (layer == 0) lds_put(s_h, w, mt, s, n, g, h[mt][s]);
+        if (layer == 0) lds_put(s_h, fg, mt, s0 + s, n, g, h[mt][s]);
       }
     }
     if (layer == 0) __syncthreads();
@@ -367,22 +380,22 @@ __global__ void __launch_bounds__(256)
   // ---- output layer and atomic energy; dz3 = w4 * f3 into the LDS buffer
   const float* w4 = cst + 6 * kH;
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
+  for (int s = 0; s < kSub; ++s) {
     float e = 0.f;
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-      const int j0 = 16 * (4 * w + mt) + 4 * g;
+      const int j0 = 16 * (4 * fg + mt) + 4 * g;
       float dz[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         e += h[mt][s][v] * w4[j0 + v];
         dz[v] = w4[j0 + v] * f3[mt][s][v];
       }
-      lds_put(s_h, w, mt, s, n, g, dz);
+      lds_put(s_h, fg, mt, s0 + s, n, g, dz);
     }
     e += __shfl_xor(e, 16, 64);
     e += __shfl_xor(e, 32, 64);
-    if (g == 0) s_e[w][16 * s + n] = e;
+    if (g == 0) s_e[fg][16 * (s0 + s) + n] = e;
   }
   __syncthreads();
   if (tid < kTile && tid < cnt) {
@@ -394,13 +407,13 @@ __global__ void __launch_bounds__(256)
   // dh3 = w4 (dE/de = 1); the skip of layer 3 carries it to dh2 unchanged
 #pragma unroll
   for (int layer = 0; layer < 2; ++layer) {
-    hidden_product(acc, frag + (layer == 0 ? p.w3b : p.w2b), s_h, w, lane, n, g);
+    hidden_product(acc, frag + (layer == 0 ? p.w3b : p.w2b), s_h, fg, s0, lane, n, g);
     __syncthreads();
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-      const int j0 = 16 * (4 * w + mt) + 4 * g;
+      const int j0 = 16 * (4 * fg + mt) + 4 * g;
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
+      for (int s = 0; s < kSub; ++s) {
         float dz[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -414,37 +427,39 @@ __global__ void __launch_bounds__(256)
             dz[v] = dh * f1[mt][s][v];
           }
         }
-        lds_put(s_h, w, mt, s, n, g, dz);
+        lds_put(s_h, fg, mt, s0 + s, n, g, dz);
       }
     }
     __syncthreads();
   }
 
-  // ---- dD = dz1 W1^T: B operands (dz1 of the 64 atoms, K = 256) in registers
-  h8<H> bz[8][4];
+  // ---- dD = dz1 W1^T: B operands (dz1 of the wavefront's 32 atoms, K = 256)
+  // in registers; the feature group picks the M tiles
+  h8<H> bz[8][kSub];
 #pragma unroll
   for (int kc = 0; kc < 8; ++kc)
 #pragma unroll
-    for (int s = 0; s < 4; ++s) bz[kc][s] = lds_b(s_h, kHS, s, n, 32 * kc, g);
+    for (int s = 0; s < kSub; ++s) bz[kc][s] = lds_b(s_h, kHS, s0 + s, n, 32 * kc, g);
   const h8<H>* w1b = frag + p.w1b;
-  int atom_s[4];
+  int atom_s[kSub];
 #pragma unroll
-  for (int s = 0; s < 4; ++s) atom_s[s] = s_atom[16 * s + n];
-  const int atom_g = s_atom[16 * g + n];
+  for (int s = 0; s < kSub; ++s) atom_s[s] = s_atom[16 * (s0 + s) + n];
+  // lane group g < kSub stores the first term of subtile s0 + g
+  const int atom_g = g < kSub ? s_atom[16 * (s0 + g) + n] : -1;
   const int mt_per_wave = (p.nm1 + 3) / 4;
-  const int mt0 = w * mt_per_wave, mt1 = min(p.nm1, mt0 + mt_per_wave);
+  const int mt0 = fg * mt_per_wave, mt1 = min(p.nm1, mt0 + mt_per_wave);
   // fused form: GR[.][a] of the lane's atoms (a = 4 g + v) in registers,
   // acc2 = this wavefront's part of sum_m dD[m][a] GR[.][m]; s_h is reused
   // (after the barrier) for the first term of m < 16, s_x for the reduction
-  float4 gra[4][4], acc2[4][4];
+  float4 gra[kSub][4], acc2[kSub][4];
   float4* t1buf = reinterpret_cast<float4*>(s_h);
   float4* red = reinterpret_cast<float4*>(&s_x[0][0]);
   if constexpr (kGR) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+    for (int s = 0; s < kSub; ++s)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        gra[s][v] = sgr4[(16 * s + n) * (kGRS / 4) + 4 * g + v];
+        gra[s][v] = sgr4[(16 * (s0 + s) + n) * (kGRS / 4) + 4 * g + v];
         acc2[s][v] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     __syncthreads();  // every wavefront has read dz1 from s_h
@@ -460,22 +475,22 @@ __global__ void __launch_bounds__(256)
     const int mn = mt + 1 < mt1 ? mt + 1 : mt;
 #pragma unroll
     for (int kc = 0; kc < 8; ++kc) wn[kc] = w1b[((int64_t)mn * 8 + kc) * 64 + lane];
-    f32x4 d[4];
+    f32x4 d[kSub];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) d[s] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int s = 0; s < kSub; ++s) d[s] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int kc = 0; kc < 8; ++kc)
 #pragma unroll
-      for (int s = 0; s < 4; ++s) d[s] = mfma(wa[kc], bz[kc][s], d[s]);
+      for (int s = 0; s < kSub; ++s) d[s] = mfma(wa[kc], bz[kc][s], d[s]);
 #pragma unroll
     for (int kc = 0; kc < 8; ++kc) wa[kc] = wn[kc];
     if constexpr (kGR) {
       // M tile mt is m = mt, its rows a = 4 g + v (axis 16)
       const int q3 = mt >= 16 ? q3_of(mt, p.n1) : 0;
-      float4 p1[4];
+      float4 p1[kSub];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const float4 gm = sgr4[(16 * s + n) * (kGRS / 4) + mt];
+      for (int s = 0; s < kSub; ++s) {
+        const float4 gm = sgr4[(16 * (s0 + s) + n) * (kGRS / 4) + mt];
         p1[s] = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -495,33 +510,34 @@ __global__ void __launch_bounds__(256)
         p1[s].z = sum_lane_groups(p1[s].z);
         p1[s].w = sum_lane_groups(p1[s].w);
       }
-      // lane group g holds atom 16 g + n: one 16-byte (four-component) store per lane
-      const float4 x = g == 0 ? p1[0] : g == 1 ? p1[1] : g == 2 ? p1[2] : p1[3];
+      // lane group g < kSub holds atom 16 (s0 + g) + n: one 16-byte
+      // (four-component) store per lane
+      const float4 x = g == 0 ? p1[0] : p1[1];
       if (atom_g >= 0) {
         if (mt >= 16)
           *reinterpret_cast<float4*>(dout + ((int64_t)atom_g * 128 + q3) * 4) =
               make_float4(x.x * p.inv_nnei, x.y * p.inv_nnei, x.z * p.inv_nnei, x.w * p.inv_nnei);
         else
-          t1buf[(16 * g + n) * 16 + mt] = x;
+          t1buf[(16 * (s0 + g) + n) * 16 + mt] = x;
       }
     } else {
       const int f = 16 * mt + 4 * g;
 #pragma unroll
-      for (int s = 0; s < 4; ++s)
+      for (int s = 0; s < kSub; ++s)
         if (atom_s[s] >= 0)
           __builtin_nontemporal_store(d[s], reinterpret_cast<f32x4*>(dout + (int64_t)atom_s[s] * p.k1 + f));
     }
   }
   if constexpr (kGR) {
-    // second term summed over the four wavefronts in a fixed order
+    // second term summed over the four feature groups in a fixed order
 #pragma unroll 1
     for (int ph = 0; ph < 4; ++ph) {
-      if (w == ph)
+      if (fg == ph)
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+        for (int s = 0; s < kSub; ++s)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            float4* r4 = red + (16 * s + n) * 16 + 4 * g + v;
+            float4* r4 = red + (16 * (s0 + s) + n) * 16 + 4 * g + v;
             float4 x = acc2[s][v];
             if (ph > 0) {
               const float4 y = *r4;
@@ -535,7 +551,7 @@ __global__ void __launch_bounds__(256)
       __syncthreads();
     }
     // dGR for m < 16 (both terms) and zeros in the padding slots
-    for (int k = tid; k < kTile * 128; k += 256) {
+    for (int k = tid; k < kTile * 128; k += kThreads) {
       const int la = k >> 7, q = k & 127, ai = s_atom[la];
       if (ai < 0) continue;
       const int m = orig_of_q3(q, p.n1);
@@ -579,7 +595,7 @@ int fitting(double* atom_energy, float* ddesc, const void* desc, const int* perm
   set_offsets(p);
   // one workgroup per 64-atom tile of each type (upper bound; extra ones exit)
   const int64_t grid = ceil_div(nloc, kTile) + ntypes;
-  hipLaunchKernelGGL((k_fit<H, false>), dim3((unsigned)grid), dim3(256), 0, as_stream(stream), atom_energy, ddesc,
+  hipLaunchKernelGGL((k_fit<H, false>), dim3((unsigned)grid), dim3(kThreads), 0, as_stream(stream), atom_energy, ddesc,
                      static_cast<const H*>(desc), nullptr, perm, type_start, static_cast<const h8<H>*>(frags),
                      consts, ebias, p);
   DPMI_LAUNCH_CHECK();
@@ -605,7 +621,7 @@ int fitting_gr(double* atom_energy, float* dgr, const float* gr_part, const int*
   p.k1 = 16 * p.M;
   set_offsets(p);
   const int64_t grid = ceil_div(nloc, kTile) + ntypes;
-  hipLaunchKernelGGL((k_fit<H, true>), dim3((unsigned)grid), dim3(256), 0, as_stream(stream), atom_energy, dgr,
+  hipLaunchKernelGGL((k_fit<H, true>), dim3((unsigned)grid), dim3(kThreads), 0, as_stream(stream), atom_energy, dgr,
                      nullptr, gr_part, perm, type_start, static_cast<const h8<H>*>(frags), consts, ebias, p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;

@@ -115,8 +115,8 @@ int adam_impl(T* param, const T* grad, T* exp_avg, T* exp_avg_sq, int64_t n, con
 
 // Column sums of x [R, F] in a fixed order, in two passes: block b of pass 1
 // sums rows [b chunk, (b + 1) chunk) into part [b, F] (lanes of one column
-// over interleaved rows, combined in lane order), pass 2 sums the parts in
-// block order.  The reduction the training graph needs for the gradients of
+// over interleaved rows, combined in lane order), pass 2 sums the parts of
+// each column in a fixed pairwise tree.  The reduction the training graph needs for the gradients of
 // broadcast biases and timesteps, without the multi-block library reduction
 // (whose semaphore memset must not sit in a captured graph).
 constexpr int kRowSumBlocks = 256;
@@ -141,13 +141,20 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// one workgroup per column: part b in lane b, then a fixed pairwise tree
 template <typename T>
-__global__ void __launch_bounds__(256) k_rowsum_final(T* __restrict__ out, const T* __restrict__ part, int nb, int F) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= F) return;
-  T v = 0;
-  for (int b = 0; b < nb; ++b) v += part[(long long)b * F + c];
-  out[c] = v;
+__global__ void __launch_bounds__(kRowSumBlocks) k_rowsum_final(T* __restrict__ out, const T* __restrict__ part, int nb,
+                                                               int F) {
+  __shared__ T s[kRowSumBlocks];
+  const int c = blockIdx.x, t = threadIdx.x;
+  s[t] = t < nb ? part[(long long)t * F + c] : T(0);
+  __syncthreads();
+#pragma unroll
+  for (int h = kRowSumBlocks / 2; h > 0; h >>= 1) {
+    if (t < h) s[t] += s[t + h];
+    __syncthreads();
+  }
+  if (t == 0) out[c] = s[0];
 }
 
 template <typename T>
@@ -160,7 +167,7 @@ int row_sum_impl(T* out, const T* x, T* part, int64_t R, int64_t F, void* stream
   const int W = F < 256 ? (int)F : 256;
   hipLaunchKernelGGL(k_rowsum_part<T>, dim3((unsigned)nb, (unsigned)ceil_div(F, W)), dim3(256), 0, st, part, x,
                      (long long)R, (int)F, (long long)chunk);
-  hipLaunchKernelGGL(k_rowsum_final<T>, dim3((unsigned)ceil_div(F, 256)), dim3(256), 0, st, out, part, nb, (int)F);
+  hipLaunchKernelGGL(k_rowsum_final<T>, dim3((unsigned)F), dim3(kRowSumBlocks), 0, st, out, part, nb, (int)F);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }
