@@ -17,6 +17,8 @@ constexpr int kMaxGridNoPbc = 64;
 constexpr int kCandCap = 1536;
 constexpr int kCandCapBig = 3072;
 constexpr int kMaxNb = 128;
+// most workgroups sharing one cell in the cell-wise search (small boxes)
+constexpr int kMaxCellSplit = 8;
 
 // workspace status words: misc[0] nall, misc[1] error bits (1: more
 // candidates than a row can hold, 2: the workspace was not set up by

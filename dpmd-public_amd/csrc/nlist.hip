@@ -591,16 +591,23 @@ __global__ void __launch_bounds__(256) k_search_cells(
   int16_t* rslot = rslot_all + w * sp.nnei;
 
   if (misc[1] & 2) return;  // workspace rejected by k_ws_validate
-  for (int ci = blockIdx.x; ci < ncells; ci += gridDim.x) {
+  // Fewer cells than workgroups (small boxes): nsplit workgroups share each
+  // cell, each building the same candidate table and taking every
+  // nsplit-th group of four of its atoms; large boxes keep one per cell.
+  const int nsplit = ncells > 0 ? max(1, min(kMaxCellSplit, (int)gridDim.x / ncells)) : 1;
+  for (int wi = blockIdx.x; wi < ncells * nsplit; wi += gridDim.x) {
+    const int ci = wi % ncells, part = wi / ncells;
     const int cc = kBigPass ? bigcells[ci] : ci;
     const int a0 = cell_start[cc], na = cell_start[cc + 1] - a0;
-    if (na == 0) continue;  // uniform in the workgroup
+    if (na <= 4 * part) continue;  // no atom for this part (uniform in the workgroup)
     const int T = nb_table(p, cell_start, ghost_off, cc, s_base, s_local, s_pref);
     if (T > CAP) {
-      if (!kBigPass) {
-        if (tid == 0) bigcells[atomicAdd(&misc[3], 1)] = cc;  // second pass, larger table
-      } else {
-        for (int k = tid; k < na; k += blockDim.x) redo[atomicAdd(&misc[2], 1)] = cell_atoms[a0 + k];
+      if (part == 0) {
+        if (!kBigPass) {
+          if (tid == 0) bigcells[atomicAdd(&misc[3], 1)] = cc;  // second pass, larger table
+        } else {
+          for (int k = tid; k < na; k += blockDim.x) redo[atomicAdd(&misc[2], 1)] = cell_atoms[a0 + k];
+        }
       }
       __syncthreads();
       continue;
@@ -614,7 +621,7 @@ __global__ void __launch_bounds__(256) k_search_cells(
                             __int_as_float((code << 24) | j));
     }
     __syncthreads();
-    for (int ai = w; ai < na; ai += 4) {
+    for (int ai = 4 * part + w; ai < na; ai += 4 * nsplit) {
       const int i = cell_atoms[a0 + ai];
       const double xi0 = ext_c[3 * i + 0], xi1 = ext_c[3 * i + 1], xi2 = ext_c[3 * i + 2];
       const float fi0 = (float)xi0, fi1 = (float)xi1, fi2 = (float)xi2;
