@@ -51,6 +51,8 @@ constexpr int kHS = kHP + 4;   // LDS row pitch (doubles) of the hidden buffer
 constexpr int kMaxM = 128;     // widest G^T R
 constexpr int kCst64 = 7 * kHP + 1;  // b1 idt1 b2 idt2 b3 idt3 w4 [256 each], b4
 constexpr int kMaxParts = 16;
+constexpr int kHB = 6;         // K steps per prefetched batch of the hidden-layer products
+constexpr int kDB = 6;         // K steps per prefetched batch of the dD product
 
 struct Fit64Params {
   int ntypes;
@@ -84,17 +86,39 @@ __device__ inline void hidden64(f64x4 (&acc)[2][2], const double* __restrict__ w
   const bool live1 = mt0 + 1 < nmt;
   const double* w0 = wf + (int64_t)mt0 * nkh * 64 + lane;
   const double* w1 = w0 + (int64_t)nkh * 64;
-#pragma unroll 4
-  for (int ks = 0; ks < nkh; ++ks) {
-    const double a0 = w0[(int64_t)ks * 64];
-    const double a1 = live1 ? w1[(int64_t)ks * 64] : 0.;
-    const double b0 = s_h[n * kHS + 4 * ks + g];
-    const double b1 = s_h[(16 + n) * kHS + 4 * ks + g];
-    acc[0][0] = mfma64(a0, b0, acc[0][0]);
-    acc[0][1] = mfma64(a0, b1, acc[0][1]);
-    if (live1) {
-      acc[1][0] = mfma64(a1, b0, acc[1][0]);
-      acc[1][1] = mfma64(a1, b1, acc[1][1]);
+  // software pipeline: the fragments of the next kHB K steps load while this
+  // batch's products run (one batch is shorter than an L2 round trip)
+  double an[kHB][2];
+  auto load = [&](int kb, double (&a)[kHB][2]) {
+#pragma unroll
+    for (int u = 0; u < kHB; ++u) {
+      const int ks = kb + u;
+      a[u][0] = ks < nkh ? w0[(int64_t)ks * 64] : 0.;
+      a[u][1] = live1 && ks < nkh ? w1[(int64_t)ks * 64] : 0.;
+    }
+  };
+  load(0, an);
+#pragma unroll 1
+  for (int kb = 0; kb < nkh; kb += kHB) {
+    double a[kHB][2];
+#pragma unroll
+    for (int u = 0; u < kHB; ++u) {
+      a[u][0] = an[u][0];
+      a[u][1] = an[u][1];
+    }
+    if (kb + kHB < nkh) load(kb + kHB, an);
+#pragma unroll
+    for (int u = 0; u < kHB; ++u) {
+      const int ks = kb + u;
+      if (ks >= nkh) break;
+      const double b0 = s_h[n * kHS + 4 * ks + g];
+      const double b1 = s_h[(16 + n) * kHS + 4 * ks + g];
+      acc[0][0] = mfma64(a[u][0], b0, acc[0][0]);
+      acc[0][1] = mfma64(a[u][0], b1, acc[0][1]);
+      if (live1) {
+        acc[1][0] = mfma64(a[u][1], b0, acc[1][0]);
+        acc[1][1] = mfma64(a[u][1], b1, acc[1][1]);
+      }
     }
   }
 }
@@ -130,15 +154,23 @@ __global__ void __launch_bounds__(kThreads)
   const double* cst = consts + (int64_t)t * kCst64;
   __syncthreads();
 
-  // ---- GR = inv_nnei * sum over partials, staged as [atom][m][c]
+  // ---- GR = inv_nnei * sum over partials, staged as [atom][m][c]: thread
+  // per (atom, m), its four components read as coalesced scalars (lanes over
+  // consecutive m) and written as four consecutive doubles
   const int per_atom = 4 * M;
-  for (int k = tid; k < kT * per_atom; k += kThreads) {
-    const int la = k / per_atom, r = k - la * per_atom, c = r / M, m = r - c * M;
+  for (int k = tid; k < kT * M; k += kThreads) {
+    const int la = k / M, m = k - la * M;
     const int ai = s_atom[la];
-    double v = 0.;
+    double v[4] = {0., 0., 0., 0.};
     if (ai >= 0)
-      for (int q = 0; q < p.nparts; ++q) v += gr_part[((int64_t)q * p.nloc + ai) * per_atom + r];
-    s_buf[la * gs + 4 * m + c] = v * p.inv_nnei;
+      for (int q = 0; q < p.nparts; ++q) {
+        const double* src = gr_part + ((int64_t)q * p.nloc + ai) * per_atom + m;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] += src[c * M];
+      }
+    double* dst = s_buf + la * gs + 4 * m;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) dst[c] = v[c] * p.inv_nnei;
   }
   __syncthreads();
   const int mt0 = 2 * w;
@@ -162,12 +194,18 @@ __global__ void __launch_bounds__(kThreads)
     const int nk1 = 4 * M;
     const double* w0 = frag + p.w1f + (int64_t)mt0 * nk1 * 64 + lane;
     const double* w1 = w0 + (int64_t)nk1 * 64;
-    double an[2][4];
+    // W1 fragments two m-steps ahead (one step's products are shorter than
+    // an L2 round trip)
+    auto load_m = [&](int m, double (&a)[2][4]) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      an[0][q] = w0[(int64_t)q * 64];
-      an[1][q] = live1 ? w1[(int64_t)q * 64] : 0.;
-    }
+      for (int q = 0; q < 4; ++q) {
+        a[0][q] = m < M ? w0[(int64_t)(4 * m + q) * 64] : 0.;
+        a[1][q] = live1 && m < M ? w1[(int64_t)(4 * m + q) * 64] : 0.;
+      }
+    };
+    double an[2][4], ann[2][4];
+    load_m(0, an);
+    load_m(1, ann);
 #pragma unroll 1
     for (int m = 0; m < M; ++m) {
       double a[2][4];
@@ -175,14 +213,10 @@ __global__ void __launch_bounds__(kThreads)
       for (int q = 0; q < 4; ++q) {
         a[0][q] = an[0][q];
         a[1][q] = an[1][q];
+        an[0][q] = ann[0][q];
+        an[1][q] = ann[1][q];
       }
-      if (m + 1 < M) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          an[0][q] = w0[(int64_t)(4 * (m + 1) + q) * 64];
-          an[1][q] = live1 ? w1[(int64_t)(4 * (m + 1) + q) * 64] : 0.;
-        }
-      }
+      if (m + 2 < M) load_m(m + 2, ann);
       double gm[2][4];
 #pragma unroll
       for (int s = 0; s < 2; ++s)
@@ -330,22 +364,55 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
       for (int c = 0; c < 4; ++c) ga[s][v][c] = gr_at(ai_s[s], c, 4 * v + g);
   const double* w1b = frag + p.w1b;
+  // software pipeline over (m, K batch): the fragments of the next batch
+  // (or of the first batch of this wavefront's next m) and the GR column of
+  // the next m load while the current products run
+  auto load_d = [&](int md, int kb, double (&a)[kDB]) {
+    const double* wrow = w1b + (int64_t)md * p.nkh * 64 + lane;
+#pragma unroll
+    for (int u = 0; u < kDB; ++u) a[u] = md < M && kb + u < p.nkh ? wrow[(int64_t)(kb + u) * 64] : 0.;
+  };
+  auto load_gm = [&](int md, double (&gm)[2][4]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) gm[s][c] = md < M ? gr_at(ai_s[s], c, md) : 0.;
+  };
+  double an[kDB], gmn[2][4];
+  load_d(w, 0, an);
+  load_gm(w, gmn);
 #pragma unroll 1
   for (int md = w; md < M; md += kWv) {
     f64x4 d[2] = {f64x4{0., 0., 0., 0.}, f64x4{0., 0., 0., 0.}};
-    const double* wrow = w1b + (int64_t)md * p.nkh * 64 + lane;
-#pragma unroll 4
-    for (int ks = 0; ks < p.nkh; ++ks) {
-      const double a = wrow[(int64_t)ks * 64];
-      d[0] = mfma64(a, s_buf[n * kHS + 4 * ks + g], d[0]);
-      d[1] = mfma64(a, s_buf[(16 + n) * kHS + 4 * ks + g], d[1]);
+    double gms[2][4];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) gms[s][c] = gmn[s][c];
+    load_gm(md + kWv, gmn);
+#pragma unroll 1
+    for (int kb = 0; kb < p.nkh; kb += kDB) {
+      double a[kDB];
+#pragma unroll
+      for (int u = 0; u < kDB; ++u) a[u] = an[u];
+      if (kb + kDB < p.nkh)
+        load_d(md, kb + kDB, an);
+      else
+        load_d(md + kWv, 0, an);
+#pragma unroll
+      for (int u = 0; u < kDB; ++u) {
+        const int ks = kb + u;
+        if (ks >= p.nkh) break;
+        d[0] = mfma64(a[u], s_buf[n * kHS + 4 * ks + g], d[0]);
+        d[1] = mfma64(a[u], s_buf[(16 + n) * kHS + 4 * ks + g], d[1]);
+      }
     }
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
       const int ai = ai_s[s];
       double gm[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) gm[c] = gr_at(ai, c, md);
+      for (int c = 0; c < 4; ++c) gm[c] = gms[s][c];
       double t1[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
