@@ -13,13 +13,14 @@
 // and the adjoint dE/dD for E = sum_i e_i, which the descriptor adjoint
 // turns into dE/d em.
 //
-// MI355X mapping.  A workgroup owns 64 atoms of one type (the list is sorted
-// by type); its eight wavefronts split every hidden layer by output feature
-// and atom: wavefront w owns features 64 (w % 4) .. 64 (w % 4) + 63 of the
-// width padded to 256 for atoms 32 (w / 4) .. 32 (w / 4) + 31, so a feature's
-// tanh factor, skip value and adjoint stay in the registers of one lane for
-// the whole pass, and two wavefronts share each SIMD (the per-lane state of
-// half the atoms fits the 256 registers of two waves per SIMD).  Layers are computed transposed, y^T = W^T x^T,
+// MI355X mapping.  A workgroup owns 32 atoms of one type (the list is sorted
+// by type); its four wavefronts split every hidden layer by output feature:
+// wavefront w owns features 64 w .. 64 w + 63 of the width padded to 256, so
+// a feature's tanh factor, skip value and adjoint stay in the registers of
+// one lane for the whole pass.  The per-lane state (four feature tiles x two
+// 16-atom subtiles) fits the 256 registers of two waves per SIMD, and the
+// workgroup's LDS (71 KB) lets two workgroups share a CU, so that one's
+// barrier-separated phases (staging, epilogues) overlap the other's products.  Layers are computed transposed, y^T = W^T x^T,
 // with v_mfma_f32_16x16x32_bf16 / _f16: the weight fragment is the A operand (M =
 // output feature), the layer input is the B operand (K = input feature,
 // N = atom), read from an LDS buffer [atom][feature] that every layer's
@@ -27,7 +28,7 @@
 // blocks (H, as the descriptor kernel writes it), prefetched one block ahead.  The
 // backward pass runs the transposed products (dh_{l-1} = dz_l W_l^T + skip)
 // through the same LDS buffer and ends with dD = dz1 W1^T, whose B operands
-// (dz1 for all 64 atoms) stay in registers while W1's fragments stream from
+// (dz1 for the 32 atoms) stay in registers while W1's fragments stream from
 // L2.
 #include <hip/hip_runtime.h>
 
@@ -47,9 +48,9 @@ using h4 = H __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kH = 256;         // padded hidden width (4 feature groups x 64 features)
-constexpr int kTile = 64;       // atoms per workgroup
-constexpr int kThreads = 512;   // eight wavefronts: 4 feature groups x 2 atom halves
-constexpr int kSub = 2;         // 16-atom subtiles per wavefront
+constexpr int kTile = 32;       // atoms per workgroup
+constexpr int kThreads = 256;   // four wavefronts, one per feature group
+constexpr int kSub = 2;         // 16-atom subtiles per wavefront (all of the tile)
 constexpr int kKB = 64;         // layer-1 input features staged per step
 constexpr int kXS = kKB + 8;    // LDS row stride (16-bit elements) of the staged D block
 constexpr int kHS = kH + 8;     // LDS row stride (16-bit elements) of the hidden-layer buffer
@@ -123,13 +124,17 @@ __device__ inline void hidden_product(f32x4 (&acc)[4][kSub], const h8<H>* __rest
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
     for (int s = 0; s < kSub; ++s) acc[mt][s] = f32x4{0.f, 0.f, 0.f, 0.f};
-  h8<H> a[4];
+  // fragments two K chunks ahead
+  h8<H> a[4], ab[4];
 #pragma unroll
-  for (int mt = 0; mt < 4; ++mt) a[mt] = frag[((4 * fg + mt) * 8 + 0) * 64 + lane];
+  for (int mt = 0; mt < 4; ++mt) {
+    a[mt] = frag[((4 * fg + mt) * 8 + 0) * 64 + lane];
+    ab[mt] = frag[((4 * fg + mt) * 8 + 1) * 64 + lane];
+  }
 #pragma unroll 1
   for (int kc = 0; kc < 8; ++kc) {
     h8<H> an[4];
-    const int kn = kc + 1 < 8 ? kc + 1 : kc;
+    const int kn = kc + 2 < 8 ? kc + 2 : kc;
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) an[mt] = frag[((4 * fg + mt) * 8 + kn) * 64 + lane];
     h8<H> b[kSub];
@@ -140,7 +145,10 @@ __device__ inline void hidden_product(f32x4 (&acc)[4][kSub], const h8<H>* __rest
 #pragma unroll
       for (int s = 0; s < kSub; ++s) acc[mt][s] = mfma(a[mt], b[s], acc[mt][s]);
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) a[mt] = an[mt];
+    for (int mt = 0; mt < 4; ++mt) {
+      a[mt] = ab[mt];
+      ab[mt] = an[mt];
+    }
   }
 }
 
@@ -153,13 +161,18 @@ __device__ inline void hidden_product(f32x4 (&acc)[4][kSub], const h8<H>* __rest
 // written / nnei in the embedding kernels' internal order as dgr
 // [nloc, 128, 4] (component fastest: one 16-byte store per atom and feature).
 template <typename H, bool kGR>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_fit(double* __restrict__ atom_energy, float* __restrict__ dout, const H* __restrict__ desc,
           const float* __restrict__ gr_part, const int* __restrict__ perm, const int* __restrict__ type_start,
           const h8<H>* __restrict__ frag_all, const float* __restrict__ consts, const double* __restrict__ ebias,
           FitParams p) {
-  __shared__ __attribute__((aligned(16))) H s_x[2][kTile * kXS];
-  __shared__ __attribute__((aligned(16))) H s_h[kTile * kHS];
+  // the layer-1 D blocks (two buffers) and, after layer 1, the hidden-layer
+  // buffer share one region; the adjoint's dE/dGR stages reuse it at the end
+  constexpr int kXH = 2 * kTile * kXS > kTile * kHS ? 2 * kTile * kXS : kTile * kHS;
+  static_assert(kTile * kHS * 2 >= 2 * kTile * 16 * 16, "the dE/dGR stages must fit the shared region");
+  __shared__ __attribute__((aligned(16))) H s_xh[kXH];
+  auto s_x = [&](int b) { return s_xh + b * (kTile * kXS); };
+  H* s_h = s_xh;
   __shared__ float s_e[4][kTile];
   __shared__ int s_atom[kTile];
   __shared__ __attribute__((aligned(16))) float s_gr[kGR ? kTile * kGRS : 4];
@@ -179,9 +192,9 @@ __global__ void __launch_bounds__(kThreads)
   }
   if (t < 0) return;
 
-  // wavefront w: feature group fg (features 64 fg .. 64 fg + 63), subtiles s0, s0 + 1
+  // wavefront w: feature group fg = w (features 64 fg .. 64 fg + 63), subtiles s0, s0 + 1
   const int tid = threadIdx.x, w = wave_id(), lane = tid & 63, n = lane & 15, g = lane >> 4;
-  const int fg = w & 3, s0 = kSub * (w >> 2);
+  const int fg = w & 3, s0 = 0;
   if (tid < kTile) s_atom[tid] = tid < cnt ? perm[start + tid] : -1;
   const h8<H>* frag = frag_all + (int64_t)t * p.type_frags;
   const float* cst = consts + (int64_t)t * kCst;
@@ -197,7 +210,7 @@ __global__ void __launch_bounds__(kThreads)
     r = (xa >= 0 && f < p.k1) ? __builtin_nontemporal_load(reinterpret_cast<const h8<H>*>(xrow + f)) : h8<H>{};
   };
   auto store_block = [&](int buf, const h8<H>& r) {
-    *reinterpret_cast<h8<H>*>(s_x[buf] + xl * kXS + 8 * xq) = r;
+    *reinterpret_cast<h8<H>*>(s_x(buf) + xl * kXS + 8 * xq) = r;
   };
   float4* sgr4 = reinterpret_cast<float4*>(s_gr);
   // D block kb of atom xl, features (m = 4 kb + 2 mh + ml, a = 4 aq .. 4 aq + 3)
@@ -216,7 +229,7 @@ __global__ void __launch_bounds__(kThreads)
   };
   auto store_gen = [&](int buf, const h8<H>& r) {
     // r[4 ml + j] -> feature 16 (2 mh + ml) + 4 aq + j
-    H* dst = s_x[buf] + xl * kXS + 32 * mh + 4 * aq;
+    H* dst = s_x(buf) + xl * kXS + 32 * mh + 4 * aq;
 #pragma unroll
     for (int ml = 0; ml < 2; ++ml) {
       h4<H> q;
@@ -313,7 +326,7 @@ __global__ void __launch_bounds__(kThreads)
     for (int kc = 0; kc < 2; ++kc) {
       h8<H> b[kSub];
 #pragma unroll
-      for (int s = 0; s < kSub; ++s) b[s] = lds_b(s_x[kb & 1], kXS, s0 + s, n, 32 * kc, g);
+      for (int s = 0; s < kSub; ++s) b[s] = lds_b(s_x(kb & 1), kXS, s0 + s, n, 32 * kc, g);
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
@@ -449,11 +462,12 @@ __global__ void __launch_bounds__(kThreads)
   const int mt_per_wave = (p.nm1 + 3) / 4;
   const int mt0 = fg * mt_per_wave, mt1 = min(p.nm1, mt0 + mt_per_wave);
   // fused form: GR[.][a] of the lane's atoms (a = 4 g + v) in registers,
-  // acc2 = this wavefront's part of sum_m dD[m][a] GR[.][m]; s_h is reused
-  // (after the barrier) for the first term of m < 16, s_x for the reduction
+  // acc2 = this wavefront's part of sum_m dD[m][a] GR[.][m]; the shared
+  // region is reused (after the barrier) for the first term of m < 16 and
+  // for the reduction over the feature groups
   float4 gra[kSub][4], acc2[kSub][4];
-  float4* t1buf = reinterpret_cast<float4*>(s_h);
-  float4* red = reinterpret_cast<float4*>(&s_x[0][0]);
+  float4* t1buf = reinterpret_cast<float4*>(s_xh);
+  float4* red = reinterpret_cast<float4*>(s_xh) + kTile * 16;
   if constexpr (kGR) {
 #pragma unroll
     for (int s = 0; s < kSub; ++s)
@@ -464,15 +478,18 @@ __global__ void __launch_bounds__(kThreads)
       }
     __syncthreads();  // every wavefront has read dz1 from s_h
   }
-  // the next M tile's eight weight fragments are loaded one tile ahead
-  h8<H> wa[8];
-  if (mt0 < mt1)
+  // the M tiles' eight weight fragments are loaded two tiles ahead (one
+  // tile's products are shorter than an L2 round trip)
+  h8<H> wa[8], wb[8];
 #pragma unroll
-    for (int kc = 0; kc < 8; ++kc) wa[kc] = w1b[((int64_t)mt0 * 8 + kc) * 64 + lane];
+  for (int kc = 0; kc < 8; ++kc) {
+    wa[kc] = mt0 < mt1 ? w1b[((int64_t)mt0 * 8 + kc) * 64 + lane] : h8<H>{};
+    wb[kc] = mt0 + 1 < mt1 ? w1b[((int64_t)(mt0 + 1) * 8 + kc) * 64 + lane] : h8<H>{};
+  }
 #pragma unroll 1
   for (int mt = mt0; mt < mt1; ++mt) {
     h8<H> wn[8];
-    const int mn = mt + 1 < mt1 ? mt + 1 : mt;
+    const int mn = mt + 2 < mt1 ? mt + 2 : mt;
 #pragma unroll
     for (int kc = 0; kc < 8; ++kc) wn[kc] = w1b[((int64_t)mn * 8 + kc) * 64 + lane];
     f32x4 d[kSub];
@@ -483,7 +500,10 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
       for (int s = 0; s < kSub; ++s) d[s] = mfma(wa[kc], bz[kc][s], d[s]);
 #pragma unroll
-    for (int kc = 0; kc < 8; ++kc) wa[kc] = wn[kc];
+    for (int kc = 0; kc < 8; ++kc) {
+      wa[kc] = wb[kc];
+      wb[kc] = wn[kc];
+    }
     if constexpr (kGR) {
       // M tile mt is m = mt, its rows a = 4 g + v (axis 16)
       const int q3 = mt >= 16 ? q3_of(mt, p.n1) : 0;
