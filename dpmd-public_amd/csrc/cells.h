@@ -45,7 +45,7 @@ constexpr int kGridMagic = 0x494d5044;  // "DPMI"
 
 struct WsLayout {
   int64_t params, misc, bbox, cell_of, cell_count, cell_start, cell_cursor, cell_unordered, cell_atoms, ghost_off,
-      redo, bigcells, total;
+      redo, bigcells, pos_of, total;
 };
 
 __host__ __device__ inline int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
@@ -81,6 +81,8 @@ __host__ __device__ inline WsLayout ws_layout(int64_t nloc, int64_t nlc, int64_t
   o = align256(o + 4 * (next + 1));
   w.bigcells = o;  // cells whose neighborhood exceeds the first search pass's table
   o = align256(o + 4 * nlc);
+  w.pos_of = o;  // position of each local atom in cell_atoms (the cell-sorted order)
+  o = align256(o + 4 * na);
   w.total = o;
   return w;
 }

@@ -534,7 +534,8 @@ __global__ void __launch_bounds__(256)
 // neighbor forces.  Pass 1: one workgroup per local cell (grid-stride); the
 // cell's neighbor terms g(i, s) accumulate in LDS at the neighbor's position in
 // the cell's candidate table (cand_slot from the neighbor search), then each
-// table entry is written once to partial[target][neighbor-cell index].  The
+// table entry is written once to partial[neighbor-cell index][position of the
+// target in the cell-sorted order] (whole runs per neighbor cell).  The
 // centre terms and the virial are as in k_force_virial_recompute.  Atoms or
 // cells the search left to its per-atom path (cand_slot < 0, or a table
 // over the table capacity) add their neighbor terms with atomics instead.
@@ -572,6 +573,7 @@ __global__ void __launch_bounds__(256)
   const int* __restrict__ ghost_off = reinterpret_cast<const int*>(nlws + L.ghost_off);
   const int* __restrict__ misc = reinterpret_cast<const int*>(nlws + L.misc);
   const int* __restrict__ bigcells = reinterpret_cast<const int*>(nlws + L.bigcells);
+  const int* __restrict__ pos_of = reinterpret_cast<const int*>(nlws + L.pos_of);
   const int tid = threadIdx.x, w = wave_id(), lane = lane_id();
   const int side = 2 * p.nsearch + 1, nnb = side * side * side;
   // a neighbor-list workspace of another system: no cell is processed (the
@@ -724,7 +726,10 @@ __global__ void __launch_bounds__(256)
         int t;
         const int j = nb_candidate(g, nnb, s_base, s_local, s_pref, cell_atoms, &t);
         const int o = mapping ? mapping[j] : j;
-        FP* dst = partial + ((int64_t)o * nnb + t) * 3;
+        // [offset][cell-sorted position][3]: the candidates of one neighbor
+        // cell (local, or a ghost image in the same member order) are
+        // consecutive positions, so each run of the table is one contiguous write
+        FP* dst = partial + ((int64_t)t * nloc + pos_of[o]) * 3;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           FP x;
@@ -743,28 +748,30 @@ __global__ void __launch_bounds__(256)
   block_virial_partial<FP>(v, vpartial);
 }
 
-// Pass 2: thread per local atom; the neighbor-cell index t of a cell's table
-// holding this atom's image identifies that cell (c - offset(t), wrapped), so
-// the atom sums its partial[o][t] over t in a fixed order.
+// Pass 2: thread per position of the cell-sorted atom order (consecutive
+// threads read consecutive partials); the neighbor-cell index t of a cell's
+// table holding this atom's image identifies that cell (c - offset(t),
+// wrapped), so the atom sums its partial[t][pos] over t in a fixed order.
 template <typename FP>
 __global__ void __launch_bounds__(256)
     k_force_gather(double* __restrict__ force, const FP* __restrict__ partial, const int* __restrict__ cell_ok,
                    const char* __restrict__ nlws, int nloc) {
   const GridParams p = *reinterpret_cast<const GridParams*>(nlws);
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= nloc) return;
+  const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= nloc) return;
   // a workspace of another system, or a neighbor list whose search reported
   // an error (status words of dpmi_nlist_status): NaN forces
   const int* misc = reinterpret_cast<const int*>(nlws + ws_layout(0, 0, 0).misc);
   if (p.magic != kGridMagic || p.nloc != nloc || (misc[1] | misc[4]) != 0) {
     const double nan = __builtin_nan("");
-    force[3 * (int64_t)o + 0] = nan;
-    force[3 * (int64_t)o + 1] = nan;
-    force[3 * (int64_t)o + 2] = nan;
+    force[3 * (int64_t)pos + 0] = nan;
+    force[3 * (int64_t)pos + 1] = nan;
+    force[3 * (int64_t)pos + 2] = nan;
     return;
   }
   const WsLayout L = ws_layout(p.nloc, p.lay_nlc, p.lay_next);
   const int* __restrict__ cell_of = reinterpret_cast<const int*>(nlws + L.cell_of);
+  const int o = reinterpret_cast<const int*>(nlws + L.cell_atoms)[pos];
   const int ns = p.nsearch, side = 2 * ns + 1, nnb = side * side * side;
   const int c = cell_of[o];
   const int c2 = c % p.ncell[2], c1 = (c / p.ncell[2]) % p.ncell[1], c0 = c / (p.ncell[1] * p.ncell[2]);
@@ -787,7 +794,7 @@ __global__ void __launch_bounds__(256)
       continue;
     }
     if (!cell_ok[(q0 * p.ncell[1] + q1) * p.ncell[2] + q2]) continue;
-    const FP* src = partial + ((int64_t)o * nnb + t) * 3;
+    const FP* src = partial + ((int64_t)t * nloc + pos) * 3;
     f[0] += (double)src[0];
     f[1] += (double)src[1];
     f[2] += (double)src[2];

@@ -335,8 +335,10 @@ __global__ void k_cell_fill(const int* __restrict__ cell_of, const int* __restri
 }
 
 // Step 3b: stable order inside each cell (index order, as build_clist pushes).
+// pos_of is the inverse of cell_atoms (position of each atom in it).
 __global__ void k_cell_order(const int* __restrict__ cell_of, const int* __restrict__ cell_start,
-                             const int* __restrict__ unordered, int* __restrict__ cell_atoms, int nloc) {
+                             const int* __restrict__ unordered, int* __restrict__ cell_atoms,
+                             int* __restrict__ pos_of, int nloc) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nloc) return;
   int c = cell_of[i];
@@ -344,6 +346,7 @@ __global__ void k_cell_order(const int* __restrict__ cell_of, const int* __restr
   int rank = 0;
   for (int k = b; k < e; ++k) rank += unordered[k] < i;
   cell_atoms[b + rank] = i;
+  pos_of[i] = b + rank;
 }
 
 // Step 4: ghost images, one wavefront per extended cell (copy_coord,
@@ -964,6 +967,7 @@ static int copy_coord_impl(double* out_coord, int* out_type, int* mapping, int64
   int* cursor = reinterpret_cast<int*>(ws + L.cell_cursor);
   int* unordered = reinterpret_cast<int*>(ws + L.cell_unordered);
   int* cell_atoms = reinterpret_cast<int*>(ws + L.cell_atoms);
+  int* pos_of = reinterpret_cast<int*>(ws + L.pos_of);
   int* ghost_off = reinterpret_cast<int*>(ws + L.ghost_off);
   int* misc = reinterpret_cast<int*>(ws + L.misc);
   GridParams* gp = reinterpret_cast<GridParams*>(ws + L.params);
@@ -1016,7 +1020,7 @@ static int copy_coord_impl(double* out_coord, int* out_type, int* mapping, int64
     hipLaunchKernelGGL(k_cell_fill, dim3(ceil_div(nloc, 256)), dim3(256), 0, st, cell_of, cell_start, cursor,
                        unordered, (int)nloc);
     hipLaunchKernelGGL(k_cell_order, dim3(ceil_div(nloc, 256)), dim3(256), 0, st, cell_of, cell_start, unordered,
-                       cell_atoms, (int)nloc);
+                       cell_atoms, pos_of, (int)nloc);
     DPMI_LAUNCH_CHECK();
   }
   if (!wait) {
