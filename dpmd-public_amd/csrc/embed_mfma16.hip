@@ -156,39 +156,95 @@ __device__ inline h8<H> r3_frag(const TileFwd<H>& f, int kc) {
   return b;
 }
 
-// Forward: per (atom, neighbor type) wavefront, GR partial [t][i][c][m]
-// (original feature order m < 4 n1).
-// Persistent grid: a workgroup loads its type's weight fragments once and its
-// four wavefronts walk atoms independently (no block barrier after set-up).
-// G^T R contracts over neighbor rows while r3 and h2 come out of the layer
-// products with the row as the lane index, so each 16-row tile is transposed
-// by MFMA against an identity fragment (exact for a 16-bit operand): then the
-// lane index is the feature and every lane group holds four rows, and two
-// tiles make the 32-row K operand (row order psi(g, u) = 4g + u for u < 4,
-// 16 + 4g + u - 4 otherwise) of the products em^T r3, em^T h2 and em^T 1.
+// Row stream.  A wavefront takes a contiguous range of atoms for one
+// neighbor type t and walks their rows of section t in atom order: the
+// listed rows with counts (the padding shortcut), all sel rows without.  It
+// reads up to 64 atoms at a time (a window): the window's row counts sit one
+// per lane with their inclusive prefix sum, so the start and end of any atom
+// of the window are readlanes away.  The stream is cut into chunks of at most
+// 32 rows holding rows of at most four consecutive atoms; atom k of the
+// window owns slot k % 4, and the em^T products use M row 4 slot + c, so the
+// four 16-lane groups of the accumulators hold four atoms (instead of one
+// atom in rows c < 4 and twelve unused rows).  Tiles are full except at the
+// end of a range or where a chunk closes on its fourth atom, so the tanh
+// chain runs on listed rows instead of whole 16-row tiles per atom.
+constexpr int kSlots = 4;
+
+struct Chunk {
+  int s0;          // stream position where atom k0 starts
+  int e0, e1, e2;  // where atoms k0, k0 + 1, k0 + 2 end
+  int p1;          // end of the chunk (its rows are p0 .. p1 - 1)
+  int nfin;        // atoms k0 .. k0 + nfin - 1 end inside the chunk
+};
+
+// inclusive prefix of the window's row counts at atom k (the total past it)
+__device__ inline int pre_at(int pre, int k, int n) { return __builtin_amdgcn_readlane(pre, min(k, n - 1)); }
+
+// the chunk that starts at stream position p0 with atom k0 < n unfinished
+__device__ inline Chunk make_chunk(int pre, int k0, int p0, int n, int rows) {
+  Chunk c;
+  c.s0 = k0 > 0 ? pre_at(pre, k0 - 1, n) : 0;
+  c.e0 = pre_at(pre, k0, n);
+  c.e1 = pre_at(pre, k0 + 1, n);
+  c.e2 = pre_at(pre, k0 + 2, n);
+  const int e3 = pre_at(pre, k0 + 3, n);
+  c.p1 = min(p0 + rows, e3);
+  c.nfin = (int)(c.e0 <= c.p1) + (int)(k0 + 1 < n && c.e1 <= c.p1) + (int)(k0 + 2 < n && c.e2 <= c.p1) +
+           (int)(k0 + 3 < n && e3 <= c.p1);
+  return c;
+}
+
+// the same with at most two atoms per chunk (the backward's 16-row tiles)
+__device__ inline Chunk make_chunk2(int pre, int k0, int p0, int n, int rows) {
+  Chunk c;
+  c.s0 = k0 > 0 ? pre_at(pre, k0 - 1, n) : 0;
+  c.e0 = pre_at(pre, k0, n);
+  c.e1 = pre_at(pre, k0 + 1, n);
+  c.e2 = c.e1;
+  c.p1 = min(p0 + rows, c.e1);
+  c.nfin = (int)(c.e0 <= c.p1) + (int)(k0 + 1 < n && c.e1 <= c.p1);
+  return c;
+}
+
+// the atom (window index) of stream position pos inside chunk c, and its row
+__device__ inline int chunk_atom(const Chunk& c, int k0, int pos, int* row) {
+  const int kk = (int)(pos >= c.e0) + (int)(pos >= c.e1) + (int)(pos >= c.e2);
+  const int start = kk == 0 ? c.s0 : kk == 1 ? c.e0 : kk == 2 ? c.e1 : c.e2;
+  *row = pos - start;
+  return k0 + kk;
+}
+
+// Forward: GR partial [t][i][c][m] (original feature order m < 4 n1) of
+// the atoms of each wavefront's range.  A workgroup loads its type's weight
+// fragments once; its four wavefronts stream independently (no block
+// barrier after set-up).  G^T R contracts over neighbor rows while r3 and h2
+// come out of the layer products with the row as the lane index, so each
+// 16-row tile is transposed by MFMA against an identity fragment (exact for
+// a 16-bit operand): then the lane index is the feature and every lane group
+// holds four rows, and two tiles make the 32-row K operand (row order
+// psi(g, u) = 4g + u for u < 4, 16 + 4g + u - 4 otherwise) of the products
+// em^T r3, em^T h2 and em^T 1, whose A operand holds em[row][c] in M row
+// 4 slot(row) + c and zero in the other slots' rows.
 template <typename H, bool kSkip>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_embed_fwd(float* __restrict__ gr_part, const float* __restrict__ em, const h8<H>* __restrict__ frag_all,
-                const float* __restrict__ consts, int nloc, EmbedParams p) {
+                const float* __restrict__ consts, int nloc, int per_wave, EmbedParams p) {
   __shared__ h8<H> s_frag[kFragFwd * 64];
   __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
-  __shared__ __attribute__((aligned(16))) float s_em[kAtoms][32 * 4];
+  __shared__ __attribute__((aligned(16))) float s_em[kAtoms][32 * 16];  // [row][4 slot + c]
   const int t = blockIdx.y;
-  // (threadIdx.x >> 6 here, not wave_id(): the scalar atom index measured
-  // slower in this kernel)
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  const int w = wave_id(), lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   const h8<H>* frag_t = frag_all + (int64_t)t * (kFragFwd + kFragBwd) * 64;
   for (int k = threadIdx.x; k < kFragFwd * 64; k += blockDim.x) s_frag[k] = frag_t[k];
   for (int k = threadIdx.x; k < kConsts; k += blockDim.x) s_cst[k] = consts[(int64_t)t * kConsts + k];
   const float* cst = s_cst;
   const int sel = p.sec[t + 1] - p.sec[t];
-  const int ntile_sel = (sel + 15) / 16;
   __syncthreads();
-  // Padding shortcut: an atom's tiles after its last listed neighbor hold
-  // only padding slots, all with the same em row (its center type's
-  // -davg/dstd); they are not evaluated, their G^T R share is
-  // n_pad em_pad^T G_pad with G_pad evaluated once per center type here
-  // (with the H roundings the tile path applies).
+  // Padding shortcut: an atom's slots after its last listed neighbor all
+  // hold the same em row (its center type's -davg/dstd); they are not
+  // evaluated, their G^T R share is n_pad em_pad^T G_pad with G_pad
+  // evaluated once per center type here (with the H roundings the tile path
+  // applies).
   __shared__ float s_gpad[kMaxTypes][kQ3];
   if (p.counts != nullptr) {
     for (int ti = w; ti < p.ntypes; ti += kAtoms) {
@@ -206,12 +262,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     }
     __syncthreads();
   }
-  // tiles of atom ai that are evaluated: all of its section, or with counts
-  // those holding a listed neighbor
-  auto tiles_of = [&](int ai) -> int {
-    if (p.counts == nullptr || ai >= nloc) return ntile_sel;
-    return (p.counts[(int64_t)ai * p.ntypes + t] + 15) >> 4;
-  };
   // identity fragments: B (K = phi slot of a 32-chunk, N = n), one per 16-wide half
   h8<H> eye[2];
 #pragma unroll
@@ -222,50 +272,65 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
   for (int u = 0; u < 8; ++u) ones[u] = (H)1.f;
   float* sem = s_em[w];
-  const int stride = gridDim.x * kAtoms;
-  auto load_em = [&](int ai, int row) {
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (ai < nloc && row < sel) v = *reinterpret_cast<const float4*>(em + ((int64_t)ai * p.nnei + p.sec[t] + row) * 4);
-    return v;
-  };
-  float4 ev_next[2] = {load_em(blockIdx.x * kAtoms + w, r), load_em(blockIdx.x * kAtoms + w, 16 + r)};
-  int nt_i = tiles_of(blockIdx.x * kAtoms + w);
-  for (int i = blockIdx.x * kAtoms + w; i < nloc; i += stride) {
-    // (the counts are the same in every lane: readfirstlane makes the tile
-    // loop's branches scalar; applied here, where the load read ahead one
-    // atom ago has long completed)
-    const int nti = __builtin_amdgcn_readfirstlane(nt_i), nchunk = (nti + 1) >> 1, inext = i + stride;
-    nt_i = tiles_of(inext);  // read ahead of its use
-    const int ti = p.counts != nullptr ? p.atype[i] : -1;
-    if (nchunk == 0) {  // no evaluated tile: the prefetched rows belong to the next atom
-      ev_next[0] = load_em(inext, r);
-      ev_next[1] = load_em(inext, 16 + r);
+  const float* em_t = em + (int64_t)p.sec[t] * 4;
+  const int a_lo = min((blockIdx.x * kAtoms + w) * per_wave, nloc), a_hi = min(a_lo + per_wave, nloc);
+  for (int wa = a_lo; wa < a_hi; wa += 64) {
+    const int n = min(64, a_hi - wa);
+    int cnt = 0;
+    if (lane < n) cnt = p.counts != nullptr ? (int)p.counts[(int64_t)(wa + lane) * p.ntypes + t] : sel;
+    int pre = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(pre, off, 64);
+      if (lane >= off) pre += y;
     }
-    // D(M = c, N = internal feature 16 nt + r): acc = em^T r3, acc_h = em^T h2,
-    // acc_s = sum over rows of em (the same in every column)
+    // the row of this lane in tile rt of chunk c, its slot (-1: no row)
+    auto load_row = [&](const Chunk& c, int k0, int p0, int rt, int* slot) {
+      const int pos = p0 + 16 * rt + r;
+      int row;
+      const int k = chunk_atom(c, k0, pos, &row);
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      *slot = -1;
+      if (pos < c.p1) {
+        v = *reinterpret_cast<const float4*>(em_t + ((int64_t)(wa + k) * p.nnei + row) * 4);
+        *slot = k & (kSlots - 1);
+      }
+      return v;
+    };
+    // D(M = 4 slot + c, N = internal feature 16 nt + r): acc = em^T r3,
+    // acc_h = em^T h2, acc_s = sum over rows of em (the same in every column)
     f32x4 acc[8], acc_h[4], acc_s = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int nt = 0; nt < 8; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) acc_h[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int ch = 0; ch < nchunk; ++ch) {
-      const float4 ev_cur[2] = {ev_next[0], ev_next[1]};
-      const int ni = ch + 1 < nchunk ? i : inext, nrow = ch + 1 < nchunk ? (ch + 1) * 32 : 0;
-      ev_next[0] = load_em(ni, nrow + r);
-      ev_next[1] = load_em(ni, nrow + 16 + r);
+    int k0 = 0, p0 = 0;
+    Chunk c = make_chunk(pre, 0, 0, n, 32);
+    int sl[2];
+    float4 ev[2] = {load_row(c, 0, 0, 0, &sl[0]), load_row(c, 0, 0, 1, &sl[1])};
+    while (k0 < n) {
+      // the next chunk, and its rows read ahead
+      const int k0n = k0 + c.nfin, p0n = c.p1;
+      Chunk cn = c;
+      int sln[2] = {-1, -1};
+      float4 evn[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+      if (k0n < n) {
+        cn = make_chunk(pre, k0n, p0n, n, 32);
+        evn[0] = load_row(cn, k0n, p0n, 0, &sln[0]);
+        evn[1] = load_row(cn, k0n, p0n, 1, &sln[1]);
+      }
+      const int nrows = c.p1 - p0;
       // weights and constants stay in LDS (re-read per chunk), not in registers
       asm volatile("" ::: "memory");
       // transposed tiles: [tile][pair of 16-feature tiles], 4 rows each
       h8<H> rt3[2][4], ht2[2][2];
 #pragma unroll
       for (int rt = 0; rt < 2; ++rt) {
-        const float4 ev = ev_cur[rt];
-        // a tile not evaluated contributes no em row either (its padding
-        // slots, if any, are counted by the n_pad term)
-        if (g == 0)
-          *reinterpret_cast<float4*>(sem + 4 * (16 * rt + r)) =
-              2 * ch + rt < nti ? ev : make_float4(0.f, 0.f, 0.f, 0.f);
-        if (2 * ch + rt >= nti) {  // wave-uniform: a tile not evaluated (rows past sel or padding slots)
+        const float4 e = ev[rt];
+        // em staged as [row][4 slot + c]: lane group g writes the slot-g quarter of its row
+        *reinterpret_cast<float4*>(sem + 16 * (16 * rt + r) + 4 * g) =
+            sl[rt] == g ? e : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (16 * rt >= nrows) {  // wave-uniform: a tile without rows
 #pragma unroll
           for (int kc = 0; kc < 4; ++kc)
 #pragma unroll
@@ -277,7 +342,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
           continue;
         }
         TileFwd<H> f;
-        forward_tile<H, kSkip>(ev.x, s_frag, cst, g, lane, f);
+        forward_tile<H, kSkip>(e.x, s_frag, cst, g, lane, f);
 #pragma unroll
         for (int kc = 0; kc < 4; ++kc) {
           const h8<H> rb = r3_frag(f, kc);
@@ -299,13 +364,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      // em^T as A (M = c, K = rows in psi order); rows M >= 4 of the products
-      // are never read, so their A values are left as they come
+      // em^T as A (M = 4 slot + c, K = rows in psi order)
       h8<H> a;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int row = u < 4 ? 4 * g + u : 16 + 4 * g + (u - 4);
-        a[u] = (H)sem[4 * row + (r & 3)];
+        a[u] = (H)sem[16 * row + r];
       }
 #pragma unroll
       for (int nt = 0; nt < 8; ++nt) {
@@ -330,33 +394,61 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       acc_s = mfma(a, ones, acc_s);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-    }
-    if (g == 0) {
-      // GR_t^T rows in the original feature order: gr_part[t][i][c][m]
-      // (n1 passes through an opaque copy so the index arithmetic below is
-      // not hoisted out of the atom loop, where registers are scarce)
-      int n1 = p.n1;
-      asm volatile("" : "+s"(n1));
-      const int M = 4 * n1;
-      float* out = gr_part + ((int64_t)t * nloc + i) * 4 * M;
-      // padding rows not evaluated: n_pad em_pad^T G_pad
-      float padc[4] = {0.f, 0.f, 0.f, 0.f};
-      const int n_pad = sel - min(sel, 16 * nti);
-      if (n_pad > 0 && ti >= 0 && ti < p.ntypes)
+      if (c.nfin > 0) {
+        // atoms that ended in this chunk: lane group g holds the atom of slot
+        // g, which is k0 + kk; its GR_t^T rows in the original feature order
+        const int kk = (g - k0) & (kSlots - 1);
+        const bool fin = kk < c.nfin;
+        const int cnt_a = __shfl(cnt, min(k0 + kk, 63), 64);
+        if (fin) {
+          const int ai = wa + k0 + kk;
+          // (n1 passes through an opaque copy so the index arithmetic below is
+          // not hoisted out of the chunk loop, where registers are scarce)
+          int n1 = p.n1;
+          asm volatile("" : "+s"(n1));
+          const int M = 4 * n1;
+          float* out = gr_part + ((int64_t)t * nloc + ai) * 4 * M;
+          // padding rows not evaluated: n_pad em_pad^T G_pad
+          float padc[4] = {0.f, 0.f, 0.f, 0.f};
+          int tp = 0;
+          const int n_pad = sel - cnt_a;
+          if (p.counts != nullptr && n_pad > 0) {
+            const int ti = p.atype[ai];
+            if (ti >= 0 && ti < p.ntypes) {
+              tp = ti;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) padc[v] = (float)n_pad * (float)(H)p.pad_em[(ti * p.ntypes + t) * 4 + v];
-      const int tp = ti >= 0 && ti < p.ntypes ? ti : 0;
+              for (int v = 0; v < 4; ++v) padc[v] = (float)n_pad * (float)(H)p.pad_em[(ti * p.ntypes + t) * 4 + v];
+            }
+          }
 #pragma unroll
-      for (int nt = 0; nt < 8; ++nt) {
-        const int q = 16 * nt + r;
-        const int m = orig_of_q3(q, n1);
-        const float idt3 = cst[352 + q];
-        const float gp = n_pad > 0 ? s_gpad[tp][q] : 0.f;
-        if (m >= 0)
+          for (int nt = 0; nt < 8; ++nt) {
+            const int q = 16 * nt + r;
+            const int m = orig_of_q3(q, n1);
+            const float idt3 = cst[352 + q];
+            const float gp = n_pad > 0 ? s_gpad[tp][q] : 0.f;
+            if (m >= 0)
 #pragma unroll
-          for (int v = 0; v < 4; ++v)
-            out[v * M + m] = idt3 * (acc_s[v] - 2.f * acc[nt][v]) + acc_h[nt & 3][v] + padc[v] * gp;
+              for (int v = 0; v < 4; ++v)
+                out[v * M + m] = idt3 * (acc_s[v] - 2.f * acc[nt][v]) + acc_h[nt & 3][v] + padc[v] * gp;
+          }
+        }
+        // the finished slots start the next atom from zero
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+#pragma unroll
+          for (int nt = 0; nt < 8; ++nt) acc[nt][v] = fin ? 0.f : acc[nt][v];
+#pragma unroll
+          for (int nt = 0; nt < 4; ++nt) acc_h[nt][v] = fin ? 0.f : acc_h[nt][v];
+          acc_s[v] = fin ? 0.f : acc_s[v];
+        }
       }
+      k0 = k0n;
+      p0 = p0n;
+      c = cn;
+      sl[0] = sln[0];
+      sl[1] = sln[1];
+      ev[0] = evn[0];
+      ev[1] = evn[1];
     }
   }
 }
@@ -516,20 +608,25 @@ __global__ void __launch_bounds__(256)
   for (int k = lane; k < 4 * kQ3; k += 64) out[k] = so[(k & 3) * kQ3 + (k >> 2)] * inv_nnei;
 }
 
-// Backward: net_deriv[i][sec_t + row][c] = dE/d em, recomputing the forward.
-// Persistent grid (weight fragments loaded into LDS once per workgroup);
-// each wavefront keeps dGR of its atom in LDS twice: as float [q3][c] for the
-// per-atom operands, and as 16-byte H rows [q3] = (dGR[0..3][q3], 0 x 4),
-// the A operand of dG = em dGR read with one 16-byte load per 16 features.
+// Backward: net_deriv[i][sec_t + row][c] = dE/d em, recomputing the forward,
+// over the same row stream as the forward with 16-row tiles that hold rows
+// of at most two atoms (slot = window index % 2).  A workgroup loads its
+// type's weight fragments into LDS once.  Each wavefront stages an atom's
+// dGR when the atom's first row comes up: as per-slot operands of the direct
+// term, and into one of the two halves of a 16-byte H row per feature
+// [q3] = (dGR slot 0 [0..3][q3], dGR slot 1 [0..3][q3]), the A operand of
+// dG = em dGR (the B operand holds em[row][c] in K slot 4 slot(row) + c).
 template <typename H, bool kSkip>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_embed_bwd(float* __restrict__ net_deriv, const float* __restrict__ em, const float* __restrict__ dgr,
-                const h8<H>* __restrict__ frag_all, const float* __restrict__ consts, int nloc, EmbedParams p) {
+                const h8<H>* __restrict__ frag_all, const float* __restrict__ consts, int nloc, int per_wave,
+                EmbedParams p) {
+  constexpr int kS = 2;  // atoms per tile
   __shared__ h8<H> s_frag[(kFragFwd + kFragBwd) * 64];
   __shared__ __attribute__((aligned(16))) float s_dg[kAtoms][kQ3 * 4];
   __shared__ h8<H> s_dgb[kAtoms][kQ3];
-  __shared__ h8<H> s_dop[kAtoms][6 * 16];
-  __shared__ __attribute__((aligned(16))) float s_kin[kAtoms][4];
+  __shared__ h8<H> s_dop[kAtoms][kS][6 * 16];
+  __shared__ __attribute__((aligned(16))) float s_kin[kAtoms][kS][4];
   __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
   const int t = blockIdx.y;
   const int w = wave_id(), lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
@@ -541,82 +638,70 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   __syncthreads();
   const float* cst = s_cst;
   const int sel = p.sec[t + 1] - p.sec[t];
-  const int ntile = (sel + 15) / 16;
   float* sdg = s_dg[w];
   h8<H>* sdgb = s_dgb[w];
-  h8<H>* sdop = s_dop[w];
-  float* skin = s_kin[w];
-  const int stride = gridDim.x * kAtoms;
-  // em rows are prefetched one tile ahead (across atoms too) so the HBM
-  // latency overlaps the current tile's work
-  auto load_em = [&](int ai, int tl) {
-    const int row = tl * 16 + r;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (ai < nloc && row < sel) v = *reinterpret_cast<const float4*>(em + ((int64_t)ai * p.nnei + p.sec[t] + row) * 4);
-    return v;
-  };
-  // evaluated tiles: those holding listed neighbors (see the forward's
-  // padding shortcut); padding slots, whose dE/d em no force term reads,
-  // are written 0
-  auto listed = [&](int ai) -> int {
-    return (p.counts == nullptr || ai >= nloc) ? sel : (int)p.counts[(int64_t)ai * p.ntypes + t];
-  };
-  float4 ev_next = load_em(blockIdx.x * kAtoms + w, 0);
-  int nv_next = listed(blockIdx.x * kAtoms + w);
-  for (int i = blockIdx.x * kAtoms + w; i < nloc; i += stride) {
-    // (wave-uniform: readfirstlane keeps the tile loop's control scalar)
-    const int nv = __builtin_amdgcn_readfirstlane(nv_next), nti = (nv + 15) >> 4, inext = i + stride;
-    nv_next = listed(inext);
-    if (nti == 0) ev_next = load_em(inext, 0);
-    // stage dGR [q3][c] of the atom: sdg[q3 * 4 + c]
-    // fp16 operands: everything below is linear in dGR, so the atom's dGR is
-    // staged times 2^-e (2^e the power of two at or below its largest
-    // magnitude) and dE/d em multiplied by 2^e at the end, both exact; the
-    // dz operands then sit in fp16's normal range instead of underflowing
-    float unscale = 1.f;
-    {
-      // (the lane index passes through an opaque copy: hoisted address
-      // arithmetic here made the allocator spill inside the atom loop)
-      const float4* dg = reinterpret_cast<const float4*>(dgr + (int64_t)i * 4 * kQ3);
-      int l = lane;
-      asm volatile("" : "+v"(l));
-      float4 x0 = dg[l], x1 = dg[l + 64];
-      if constexpr (std::is_same<H, _Float16>::value) {
-        float mx = fmaxf(fmaxf(fmaxf(fabsf(x0.x), fabsf(x0.y)), fmaxf(fabsf(x0.z), fabsf(x0.w))),
-                         fmaxf(fmaxf(fabsf(x1.x), fabsf(x1.y)), fmaxf(fabsf(x1.z), fabsf(x1.w))));
+  // a slot not staged yet still meets the zero K slots of the other slot's
+  // rows: its half of the dG operand must be finite
+  for (int q = lane; q < kQ3; q += 64)
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-        // exponent field of the maximum (0 for zero and subnormals), kept in
-        // [27, 227] so that both powers of two below are normal floats
-        int ef = (int)((__float_as_uint(mx) >> 23) & 255u);
-        ef = __builtin_amdgcn_readfirstlane(min(max(ef, 27), 227));
-        const float sc = __uint_as_float((uint32_t)(254 - ef) << 23);  // 2^-e
-        unscale = __uint_as_float((uint32_t)ef << 23);                  // 2^e
-        x0 = make_float4(x0.x * sc, x0.y * sc, x0.z * sc, x0.w * sc);
-        x1 = make_float4(x1.x * sc, x1.y * sc, x1.z * sc, x1.w * sc);
-      }
-      reinterpret_cast<float4*>(sdg)[l] = x0;
-      reinterpret_cast<float4*>(sdg)[l + 64] = x1;
+    for (int u = 0; u < 8; ++u) sdgb[q][u] = (H)0.f;
+  const float* em_t = em + (int64_t)p.sec[t] * 4;
+  float* nd_t = net_deriv + (int64_t)p.sec[t] * 4;
+  // fp16 operands: everything below is linear in dGR, so each atom's dGR is
+  // staged times 2^-e (2^e the power of two at or below its largest
+  // magnitude) and dE/d em multiplied by 2^e at the end, both exact; the dz
+  // operands then sit in fp16's normal range instead of underflowing
+  float unscale[kS] = {1.f, 1.f};
+  // Stage atom ai (listed rows cnt) into slot s: the padding slots after its
+  // listed rows get dE/d em = 0 (no force term reads them); then the
+  // operands of the direct term dem[c] = sum_q3 G[q3] dGR[c][q3] with
+  // G = idt3 (1 - 2 r3) + [h2, h2]:  dem = K_c + sum_q3 r3 (-2 idt3 dGR)
+  // + sum_q2 h2 (dGR[q2] + dGR[q2 + 64]), A (M = 4 s + c, K = feature slot):
+  // sdop[s][f][4 g + c], f = 0..3 the r3 chunks, 4..5 the h2 chunks; and
+  // K_c = sum_q3 idt3 dGR[c].
+  auto stage = [&](int ai, int s, int cnt) {
+    if (p.counts != nullptr)
+      for (int row = cnt + lane; row < sel; row += 64)
+        *reinterpret_cast<float4*>(nd_t + ((int64_t)ai * p.nnei + row) * 4) = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (cnt == 0) return;
+    // (the lane index passes through an opaque copy: hoisted address
+    // arithmetic here made the allocator spill inside the loops)
+    const float4* dg = reinterpret_cast<const float4*>(dgr + (int64_t)ai * 4 * kQ3);
+    int l = lane;
+    asm volatile("" : "+v"(l));
+    float4 x0 = dg[l], x1 = dg[l + 64];
+    if constexpr (std::is_same<H, _Float16>::value) {
+      float mx = fmaxf(fmaxf(fmaxf(fabsf(x0.x), fabsf(x0.y)), fmaxf(fabsf(x0.z), fabsf(x0.w))),
+                       fmaxf(fmaxf(fabsf(x1.x), fabsf(x1.y)), fmaxf(fabsf(x1.z), fabsf(x1.w))));
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+      // exponent field of the maximum (0 for zero and subnormals), kept in
+      // [27, 227] so that both powers of two below are normal floats
+      int ef = (int)((__float_as_uint(mx) >> 23) & 255u);
+      ef = __builtin_amdgcn_readfirstlane(min(max(ef, 27), 227));
+      const float sc = __uint_as_float((uint32_t)(254 - ef) << 23);  // 2^-e
+      unscale[s] = __uint_as_float((uint32_t)ef << 23);              // 2^e
+      x0 = make_float4(x0.x * sc, x0.y * sc, x0.z * sc, x0.w * sc);
+      x1 = make_float4(x1.x * sc, x1.y * sc, x1.z * sc, x1.w * sc);
     }
+    // the previous tile's reads of this wavefront's staging are complete
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    reinterpret_cast<float4*>(sdg)[l] = x0;
+    reinterpret_cast<float4*>(sdg)[l + 64] = x1;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     for (int q = lane; q < kQ3; q += 64) {
       const float4 x = *reinterpret_cast<const float4*>(sdg + 4 * q);
-      h8<H> v;
+      typedef H h4 __attribute__((ext_vector_type(4)));
+      h4 v;
       v[0] = (H)x.x;
       v[1] = (H)x.y;
       v[2] = (H)x.z;
       v[3] = (H)x.w;
-#pragma unroll
-      for (int u = 4; u < 8; ++u) v[u] = (H)0.f;
-      sdgb[q] = v;
+      reinterpret_cast<h4*>(sdgb + q)[s] = v;
     }
-    // Operands of the direct term dem[c] = sum_q3 G[q3] dGR[c][q3] with
-    // G = idt3 (1 - 2 r3) + [h2, h2]:  dem = K_c + sum_q3 r3 (-2 idt3 dGR)
-    // + sum_q2 h2 (dGR[q2] + dGR[q2 + 64]), A (M = c, K = feature slot); rows
-    // c >= 4 of the product are never read, so an A fragment depends on the
-    // lane only through (g, c = r % 4): sdop[f][4 g + c], f = 0..3 the r3
-    // chunks, 4..5 the h2 chunks, read per tile (registers are scarce here).
+    h8<H>* sdop = s_dop[w][s];
     for (int k = lane; k < 96; k += 64) {
       const int fi = k >> 4, gg = (k >> 2) & 3, cc = k & 3;
       h8<H> v;
@@ -628,120 +713,162 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       }
       sdop[k] = v;
     }
-    // K_c = sum_q3 idt3 dGR[c]: lane (q0 = lane / 4, c = lane % 4) sums q3 = q0 + 16 j
+    // K_c: lane (q0 = lane / 4, c = lane % 4) sums q3 = q0 + 16 j
     float kpart = 0.f;
 #pragma unroll
     for (int j = 0; j < 8; ++j) kpart += cst[352 + (lane >> 2) + 16 * j] * sdg[lane + 64 * j];
 #pragma unroll
     for (int off = 4; off < 64; off <<= 1) kpart += __shfl_xor(kpart, off, 64);
-    if (lane < 4) skin[lane] = kpart;
+    if (lane < 4) s_kin[w][s][lane] = kpart;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    float* nd_i = net_deriv + ((int64_t)i * p.nnei + p.sec[t]) * 4;
-    if (p.counts != nullptr)
-      for (int row = 16 * nti + lane; row < sel; row += 64)
-        *reinterpret_cast<float4*>(nd_i + 4 * row) = make_float4(0.f, 0.f, 0.f, 0.f);
-    // rows of evaluated tiles past the listed neighbors (padding slots) get 0 too
-    const int nrow_out = p.counts != nullptr ? nv : sel;
-    for (int tl = 0; tl < nti; ++tl) {
-      const int row = tl * 16 + r;
-      const float4 ev = ev_next;
-      ev_next = tl + 1 < nti ? load_em(i, tl + 1) : load_em(inext, 0);
-      // weights and constants are re-read from LDS every tile instead of being
-      // held in registers across the loop: that keeps the kernel at two
-      // wavefronts per SIMD
-      asm volatile("" ::: "memory");
-      TileFwd<H> f;
-      forward_tile<H, kSkip>(ev.x, s_frag, cst, g, lane, f);
-      // direct term -> D(M = c, N = rows)
-      const h8<H>* dop = sdop + 4 * g + (r & 3);
-      f32x4 dem = *reinterpret_cast<const f32x4*>(skin);
+  };
+  const int a_lo = min((blockIdx.x * kAtoms + w) * per_wave, nloc), a_hi = min(a_lo + per_wave, nloc);
+  for (int wa = a_lo; wa < a_hi; wa += 64) {
+    const int n = min(64, a_hi - wa);
+    int cnt = 0;
+    if (lane < n) cnt = p.counts != nullptr ? (int)p.counts[(int64_t)(wa + lane) * p.ntypes + t] : sel;
+    int pre = cnt;
 #pragma unroll
-      for (int kc = 0; kc < 4; ++kc) dem = mfma(dop[16 * kc], r3_frag(f, kc), dem);
-#pragma unroll
-      for (int kc = 0; kc < 2; ++kc) dem = mfma(dop[16 * (4 + kc)], f.b2[kc], dem);
-      // em as B (N = rows, K = c): K slots 0..3 in lane group 0, all others zero
-      h8<H> b_em;
-      b_em[0] = (H)(g == 0 ? ev.x : 0.f);
-      b_em[1] = (H)(g == 0 ? ev.y : 0.f);
-      b_em[2] = (H)(g == 0 ? ev.z : 0.f);
-      b_em[3] = (H)(g == 0 ? ev.w : 0.f);
-#pragma unroll
-      for (int u = 4; u < 8; ++u) b_em[u] = (H)0.f;
-      // dG = em dGR -> D(M = q3, N = rows), A = row q3 of sdgb (its K slots
-      // 4..7 are zero; those of lane groups 1..3 meet the zero K slots of b_em);
-      // dz3 = dG idt3 (1 - t3^2) = dG 4 idt3 r3 (1 - r3); skip = dG[q] + dG[q + 64]
-      f32x4 skip[4];
-      h8<H> b_dz3[4];
-      // keep the sdgb reads below the recompute (hoisted, they would hold 32
-      // registers across it)
-      asm volatile("" ::: "memory");
-#pragma unroll
-      for (int mt = 0; mt < 4; ++mt) {
-        const h8<H> a_lo = sdgb[16 * mt + r], a_hi = sdgb[16 * (mt + 4) + r];
-        const f32x4 d_lo = mfma(a_lo, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
-        const f32x4 d_hi = mfma(a_hi, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
-        skip[mt] = mfma(a_hi, b_em, d_lo);
-        const int q = 16 * mt + 4 * g;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          if (kSkip && (mt & 1) && v == 3) {  // empty slot: no weights read it
-            b_dz3[mt >> 1][(mt & 1) * 4 + v] = (H)0.f;
-            b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (H)0.f;
-            continue;
-          }
-          const float rl = f.r3[mt * 4 + v], rh = f.r3[(mt + 4) * 4 + v];
-          b_dz3[mt >> 1][(mt & 1) * 4 + v] = (H)(d_lo[v] * cst[576 + q + v] * fmaf(-rl, rl, rl));
-          b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (H)(d_hi[v] * cst[640 + q + v] * fmaf(-rh, rh, rh));
-        }
-      }
-      // dh2 = dz3 W3^T + skip;  dz2 = dh2 * idt2 * (1 - t2^2)
-      float dh2[16];
-      h8<H> b_dz2[2];
-#pragma unroll
-      for (int mt = 0; mt < 4; ++mt) {
-        f32x4 d = skip[mt];
-#pragma unroll
-        for (int kc = 0; kc < 4; ++kc) d = mfma(fb[(mt * 4 + kc) * 64 + lane], b_dz3[kc], d);
-        const int q = 16 * mt + 4 * g;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          float x = d[v];
-          dh2[mt * 4 + v] = x;
-          if (kSkip && (mt & 1) && v == 3) {
-            b_dz2[mt >> 1][(mt & 1) * 4 + v] = (H)0.f;
-            continue;
-          }
-          float tt = f.t2[mt * 4 + v];
-          b_dz2[mt >> 1][(mt & 1) * 4 + v] = (H)(x * cst[512 + q + v] * (1.f - tt * tt));
-        }
-      }
-      // dh1 = dz2 W2^T + dh2[q1] + dh2[q1 + 32];  ds = sum_f1 dh1 * idt1 * (1 - t1^2) * w1
-      float ds = 0.f;
-#pragma unroll
-      for (int mt = 0; mt < 2; ++mt) {
-        f32x4 d = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kc = 0; kc < 2; ++kc) d = mfma(fb[(16 + mt * 2 + kc) * 64 + lane], b_dz2[kc], d);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int u = mt * 4 + v, q = phi(g, u);
-          if (kSkip && u == 7) continue;
-          float x = d[v] + dh2[mt * 4 + v] + dh2[(mt + 2) * 4 + v];
-          float tt = f.t1[u];
-          ds += x * fmaf(-tt, tt, 1.f) * cst[480 + q];  // cst[480 + q] = idt1 w1
-        }
-      }
-      ds += __shfl_xor(ds, 16, 64);
-      ds += __shfl_xor(ds, 32, 64);
-      if (g == 0 && row < sel) {
-        float4 o = make_float4((dem[0] + ds) * unscale, dem[1] * unscale, dem[2] * unscale, dem[3] * unscale);
-        if (row >= nrow_out) o = make_float4(0.f, 0.f, 0.f, 0.f);
-        *reinterpret_cast<float4*>(nd_i + 4 * row) = o;
-      }
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(pre, off, 64);
+      if (lane >= off) pre += y;
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    // em row of this lane in the tile of chunk c (read one tile ahead)
+    auto load_row = [&](const Chunk& c, int k0, int p0) {
+      const int pos = p0 + r;
+      int row;
+      const int k = chunk_atom(c, k0, pos, &row);
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (pos < c.p1) v = *reinterpret_cast<const float4*>(em_t + ((int64_t)(wa + k) * p.nnei + row) * 4);
+      return v;
+    };
+    int k0 = 0, p0 = 0, staged = 0;
+    Chunk c = make_chunk2(pre, 0, 0, n, 16);
+    float4 ev_next = load_row(c, 0, 0);
+    while (k0 < n) {
+      const int k0n = k0 + c.nfin, p0n = c.p1;
+      // stage the atoms whose rows start in this tile (and the atoms without
+      // rows before them)
+      const int last = max(c.p1 > c.e0 ? k0 + 1 : k0, k0 + c.nfin - 1);
+      for (; staged <= last; ++staged) stage(wa + staged, staged & 1, __builtin_amdgcn_readlane(cnt, staged));
+      const float4 ev = ev_next;
+      Chunk cn = c;
+      if (k0n < n) {
+        cn = make_chunk2(pre, k0n, p0n, n, 16);
+        ev_next = load_row(cn, k0n, p0n);
+      }
+      if (c.p1 > p0) {
+        // this lane's row: atom k (slot sl, -1 past the tile's rows), row index
+        const int pos = p0 + r;
+        int row;
+        const int k = chunk_atom(c, k0, pos, &row);
+        const int sl = pos < c.p1 ? (k & 1) : -1;
+        // weights and constants are re-read from LDS every tile instead of being
+        // held in registers across the loop: that keeps the kernel at two
+        // wavefronts per SIMD
+        asm volatile("" ::: "memory");
+        TileFwd<H> f;
+        forward_tile<H, kSkip>(ev.x, s_frag, cst, g, lane, f);
+        // direct term -> D(M = 4 slot + c, N = rows); lane groups 2, 3 unused
+        const h8<H>* dop = s_dop[w][(r >> 2) & 1] + 4 * g + (r & 3);
+        f32x4 dem = f32x4{0.f, 0.f, 0.f, 0.f};
+        if (g < kS) dem = *reinterpret_cast<const f32x4*>(s_kin[w][g & 1]);
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc) dem = mfma(dop[16 * kc], r3_frag(f, kc), dem);
+#pragma unroll
+        for (int kc = 0; kc < 2; ++kc) dem = mfma(dop[16 * (4 + kc)], f.b2[kc], dem);
+        // em as B (N = rows, K = 4 slot + c): lane group 0 only, the row's own slot
+        h8<H> b_em;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) b_em[u] = (H)0.f;
+        if (g == 0) {
+          const bool s0 = sl == 0, s1 = sl == 1;
+          b_em[0] = (H)(s0 ? ev.x : 0.f);
+          b_em[1] = (H)(s0 ? ev.y : 0.f);
+          b_em[2] = (H)(s0 ? ev.z : 0.f);
+          b_em[3] = (H)(s0 ? ev.w : 0.f);
+          b_em[4] = (H)(s1 ? ev.x : 0.f);
+          b_em[5] = (H)(s1 ? ev.y : 0.f);
+          b_em[6] = (H)(s1 ? ev.z : 0.f);
+          b_em[7] = (H)(s1 ? ev.w : 0.f);
+        }
+        // dG = em dGR -> D(M = q3, N = rows), A = row q3 of sdgb (lane groups
+        // 1..3 meet the zero K slots of b_em);
+        // dz3 = dG idt3 (1 - t3^2) = dG 4 idt3 r3 (1 - r3); skip = dG[q] + dG[q + 64]
+        f32x4 skip[4];
+        h8<H> b_dz3[4];
+        // keep the sdgb reads below the recompute (hoisted, they would hold 32
+        // registers across it)
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+          const h8<H> a_lo = sdgb[16 * mt + r], a_hi = sdgb[16 * (mt + 4) + r];
+          const f32x4 d_lo = mfma(a_lo, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
+          const f32x4 d_hi = mfma(a_hi, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
+          skip[mt] = mfma(a_hi, b_em, d_lo);
+          const int q = 16 * mt + 4 * g;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            if (kSkip && (mt & 1) && v == 3) {  // empty slot: no weights read it
+              b_dz3[mt >> 1][(mt & 1) * 4 + v] = (H)0.f;
+              b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (H)0.f;
+              continue;
+            }
+            const float rl = f.r3[mt * 4 + v], rh = f.r3[(mt + 4) * 4 + v];
+            b_dz3[mt >> 1][(mt & 1) * 4 + v] = (H)(d_lo[v] * cst[576 + q + v] * fmaf(-rl, rl, rl));
+            b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (H)(d_hi[v] * cst[640 + q + v] * fmaf(-rh, rh, rh));
+          }
+        }
+        // dh2 = dz3 W3^T + skip;  dz2 = dh2 * idt2 * (1 - t2^2)
+        float dh2[16];
+        h8<H> b_dz2[2];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+          f32x4 d = skip[mt];
+#pragma unroll
+          for (int kc = 0; kc < 4; ++kc) d = mfma(fb[(mt * 4 + kc) * 64 + lane], b_dz3[kc], d);
+          const int q = 16 * mt + 4 * g;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            float x = d[v];
+            dh2[mt * 4 + v] = x;
+            if (kSkip && (mt & 1) && v == 3) {
+              b_dz2[mt >> 1][(mt & 1) * 4 + v] = (H)0.f;
+              continue;
+            }
+            float tt = f.t2[mt * 4 + v];
+            b_dz2[mt >> 1][(mt & 1) * 4 + v] = (H)(x * cst[512 + q + v] * (1.f - tt * tt));
+          }
+        }
+        // dh1 = dz2 W2^T + dh2[q1] + dh2[q1 + 32];  ds = sum_f1 dh1 * idt1 * (1 - t1^2) * w1
+        float ds = 0.f;
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+          f32x4 d = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int kc = 0; kc < 2; ++kc) d = mfma(fb[(16 + mt * 2 + kc) * 64 + lane], b_dz2[kc], d);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int u = mt * 4 + v, q = phi(g, u);
+            if (kSkip && u == 7) continue;
+            float x = d[v] + dh2[mt * 4 + v] + dh2[(mt + 2) * 4 + v];
+            float tt = f.t1[u];
+            ds += x * fmaf(-tt, tt, 1.f) * cst[480 + q];  // cst[480 + q] = idt1 w1
+          }
+        }
+        ds += __shfl_xor(ds, 16, 64);
+        ds += __shfl_xor(ds, 32, 64);
+        if (g == sl) {
+          const float us = sl == 0 ? unscale[0] : unscale[1];
+          *reinterpret_cast<float4*>(nd_t + ((int64_t)(wa + k) * p.nnei + row) * 4) =
+              make_float4((dem[0] + ds) * us, dem[1] * us, dem[2] * us, dem[3] * us);
+        }
+      }
+      k0 = k0n;
+      p0 = p0n;
+      c = cn;
+    }
   }
 }
 
@@ -791,14 +918,16 @@ int embed_fwd(float* gr_part, const float* em, const void* frags, const float* c
   p.n1 = n1;
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
   if (nloc == 0) return DPMI_OK;
-  // persistent grid: enough workgroups per type to fill 256 CUs a few times over
+  // up to 1,024 workgroups per type (256 CUs four times over); each
+  // wavefront streams a contiguous range of per_wave atoms
   dim3 grid((unsigned)std::min<int64_t>(ceil_div(nloc, kAtoms), 1024), (unsigned)ntypes);
+  const int per_wave = (int)ceil_div(nloc, (int64_t)grid.x * kAtoms);
   if (skip_layout(n1))
     hipLaunchKernelGGL((k_embed_fwd<H, true>), grid, dim3(256), 0, as_stream(stream), gr_part, em,
-                       static_cast<const h8<H>*>(frags), consts, (int)nloc, p);
+                       static_cast<const h8<H>*>(frags), consts, (int)nloc, per_wave, p);
   else
     hipLaunchKernelGGL((k_embed_fwd<H, false>), grid, dim3(256), 0, as_stream(stream), gr_part, em,
-                       static_cast<const h8<H>*>(frags), consts, (int)nloc, p);
+                       static_cast<const h8<H>*>(frags), consts, (int)nloc, per_wave, p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }
@@ -815,12 +944,13 @@ int embed_bwd(float* net_deriv, const float* em, const float* dgr, const void* f
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
   if (nloc == 0) return DPMI_OK;
   dim3 grid((unsigned)std::min<int64_t>(ceil_div(nloc, kAtoms), 1024), (unsigned)ntypes);
+  const int per_wave = (int)ceil_div(nloc, (int64_t)grid.x * kAtoms);
   if (skip_layout(n1))
     hipLaunchKernelGGL((k_embed_bwd<H, true>), grid, dim3(256), 0, as_stream(stream), net_deriv, em, dgr,
-                       static_cast<const h8<H>*>(frags), consts, (int)nloc, p);
+                       static_cast<const h8<H>*>(frags), consts, (int)nloc, per_wave, p);
   else
     hipLaunchKernelGGL((k_embed_bwd<H, false>), grid, dim3(256), 0, as_stream(stream), net_deriv, em, dgr,
-                       static_cast<const h8<H>*>(frags), consts, (int)nloc, p);
+                       static_cast<const h8<H>*>(frags), consts, (int)nloc, per_wave, p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }
