@@ -83,6 +83,17 @@ __device__ inline f32x4 mfma(const h8<_Float16>& a, const h8<_Float16>& b, const
   return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
+template <typename H>
+using h4 = H __attribute__((ext_vector_type(4)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+__device__ inline f32x4 mfma16(const h4<__bf16>& a, const h4<__bf16>& b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, a), __builtin_bit_cast(s16x4, b), c, 0,
+                                                    0, 0);
+}
+__device__ inline f32x4 mfma16(const h4<_Float16>& a, const h4<_Float16>& b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
 // Layer-1 feature owned by lane group g at B-slot u (also the K order phi).
 __device__ inline int phi(int g, int u) { return 16 * (u >> 2) + 4 * g + (u & 3); }
 
@@ -138,6 +149,9 @@ __device__ inline void forward_tile(float s, const h8<H>* __restrict__ frag, con
   }
 #pragma unroll
   for (int mt = 0; mt < 8; ++mt) {
+    // fragments read per pair of output tiles, not all ahead of the products
+    // (sixteen fragments in flight would hold 64 registers)
+    if ((mt & 1) == 0) asm volatile("" ::: "memory");
     f32x4 z = mfma(frag[(4 + 2 * mt) * 64 + lane], f.b2[0], *reinterpret_cast<const f32x4*>(cst + 224 + 16 * mt + 4 * g));
     z = mfma(frag[(5 + 2 * mt) * 64 + lane], f.b2[1], z);
 #pragma unroll
@@ -215,23 +229,24 @@ __device__ inline int chunk_atom(const Chunk& c, int k0, int pos, int* row) {
 }
 
 // Forward: GR partial [t][i][c][m] (original feature order m < 4 n1) of
-// the atoms of each wavefront's range.  A workgroup loads its type's weight
+// the atoms of each wavefront's range, over 16-row tiles of the row stream
+// holding rows of up to four atoms.  A workgroup loads its type's weight
 // fragments once; its four wavefronts stream independently (no block
 // barrier after set-up).  G^T R contracts over neighbor rows while r3 and h2
 // come out of the layer products with the row as the lane index, so each
-// 16-row tile is transposed by MFMA against an identity fragment (exact for
-// a 16-bit operand): then the lane index is the feature and every lane group
-// holds four rows, and two tiles make the 32-row K operand (row order
-// psi(g, u) = 4g + u for u < 4, 16 + 4g + u - 4 otherwise) of the products
+// 16-feature block is transposed by MFMA against an identity fragment
+// (exact for a 16-bit operand): then the lane index is the feature and lane
+// group g holds rows 4g .. 4g + 3, the K = 16 operand of the products
 // em^T r3, em^T h2 and em^T 1, whose A operand holds em[row][c] in M row
-// 4 slot(row) + c and zero in the other slots' rows.
+// 4 slot(row) + c and zero in the other slots' rows.  Each block is consumed
+// as soon as it is transposed, so no transposed tile is held in registers.
 template <typename H, bool kSkip>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip ? 4 : 3, kSkip ? 4 : 3)))
     k_embed_fwd(float* __restrict__ gr_part, const float* __restrict__ em, const h8<H>* __restrict__ frag_all,
                 const float* __restrict__ consts, int nloc, int per_wave, EmbedParams p) {
   __shared__ h8<H> s_frag[kFragFwd * 64];
   __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
-  __shared__ __attribute__((aligned(16))) float s_em[kAtoms][32 * 16];  // [row][4 slot + c]
+  __shared__ __attribute__((aligned(16))) float s_em[kAtoms][16 * 16];  // [row][4 slot + c]
   const int t = blockIdx.y;
   const int w = wave_id(), lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   const h8<H>* frag_t = frag_all + (int64_t)t * (kFragFwd + kFragBwd) * 64;
@@ -262,20 +277,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     }
     __syncthreads();
   }
-  // identity fragments: B (K = phi slot of a 32-chunk, N = n), one per 16-wide half
-  h8<H> eye[2];
-#pragma unroll
-  for (int j = 0; j < 2; ++j)
-#pragma unroll
-    for (int u = 0; u < 8; ++u) eye[j][u] = (H)(((u >> 2) == j && 4 * g + (u & 3) == r) ? 1.f : 0.f);
-  h8<H> ones;
-#pragma unroll
-  for (int u = 0; u < 8; ++u) ones[u] = (H)1.f;
   float* sem = s_em[w];
   const float* em_t = em + (int64_t)p.sec[t] * 4;
   const int a_lo = min((blockIdx.x * kAtoms + w) * per_wave, nloc), a_hi = min(a_lo + per_wave, nloc);
   for (int wa = a_lo; wa < a_hi; wa += 64) {
     const int n = min(64, a_hi - wa);
+    // identity fragments: B (K = phi slot of a 32-chunk, N = n), one per
+    // 16-wide half (built per window from an opaque lane index: hoisted out
+    // of the loop they would be spilled)
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    h8<H> eye[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) eye[j][u] = (H)(((u >> 2) == j && 4 * (ln >> 4) + (u & 3) == (ln & 15)) ? 1.f : 0.f);
     int cnt = 0;
     if (lane < n) cnt = p.counts != nullptr ? (int)p.counts[(int64_t)(wa + lane) * p.ntypes + t] : sel;
     int pre = cnt;
@@ -284,17 +300,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       const int y = __shfl_up(pre, off, 64);
       if (lane >= off) pre += y;
     }
-    // the row of this lane in tile rt of chunk c, its slot (-1: no row)
-    auto load_row = [&](const Chunk& c, int k0, int p0, int rt, int* slot) {
-      const int pos = p0 + 16 * rt + r;
+    // em row of this lane in the tile of chunk c (read one tile ahead)
+    auto load_row = [&](const Chunk& c, int k0, int p0) {
+      const int pos = p0 + r;
       int row;
       const int k = chunk_atom(c, k0, pos, &row);
       float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      *slot = -1;
-      if (pos < c.p1) {
-        v = *reinterpret_cast<const float4*>(em_t + ((int64_t)(wa + k) * p.nnei + row) * 4);
-        *slot = k & (kSlots - 1);
-      }
+      if (pos < c.p1) v = *reinterpret_cast<const float4*>(em_t + ((int64_t)(wa + k) * p.nnei + row) * 4);
       return v;
     };
     // D(M = 4 slot + c, N = internal feature 16 nt + r): acc = em^T r3,
@@ -305,52 +317,43 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) acc_h[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     int k0 = 0, p0 = 0;
-    Chunk c = make_chunk(pre, 0, 0, n, 32);
-    int sl[2];
-    float4 ev[2] = {load_row(c, 0, 0, 0, &sl[0]), load_row(c, 0, 0, 1, &sl[1])};
+    Chunk c = make_chunk(pre, 0, 0, n, 16);
+    float4 ev_next = load_row(c, 0, 0);
     while (k0 < n) {
-      // the next chunk, and its rows read ahead
       const int k0n = k0 + c.nfin, p0n = c.p1;
+      const float4 ev = ev_next;
       Chunk cn = c;
-      int sln[2] = {-1, -1};
-      float4 evn[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
       if (k0n < n) {
-        cn = make_chunk(pre, k0n, p0n, n, 32);
-        evn[0] = load_row(cn, k0n, p0n, 0, &sln[0]);
-        evn[1] = load_row(cn, k0n, p0n, 1, &sln[1]);
+        cn = make_chunk(pre, k0n, p0n, n, 16);
+        ev_next = load_row(cn, k0n, p0n);
       }
-      const int nrows = c.p1 - p0;
-      // weights and constants stay in LDS (re-read per chunk), not in registers
-      asm volatile("" ::: "memory");
-      // transposed tiles: [tile][pair of 16-feature tiles], 4 rows each
-      h8<H> rt3[2][4], ht2[2][2];
-#pragma unroll
-      for (int rt = 0; rt < 2; ++rt) {
-        const float4 e = ev[rt];
+      if (c.p1 > p0) {
+        // this lane's row: its slot (-1 past the tile's rows)
+        const int pos = p0 + r;
+        int row;
+        const int sl = pos < c.p1 ? (chunk_atom(c, k0, pos, &row) & (kSlots - 1)) : -1;
         // em staged as [row][4 slot + c]: lane group g writes the slot-g quarter of its row
-        *reinterpret_cast<float4*>(sem + 16 * (16 * rt + r) + 4 * g) =
-            sl[rt] == g ? e : make_float4(0.f, 0.f, 0.f, 0.f);
-        if (16 * rt >= nrows) {  // wave-uniform: a tile without rows
-#pragma unroll
-          for (int kc = 0; kc < 4; ++kc)
-#pragma unroll
-            for (int u = 0; u < 8; ++u) rt3[rt][kc][u] = (H)0.f;
-#pragma unroll
-          for (int kc = 0; kc < 2; ++kc)
-#pragma unroll
-            for (int u = 0; u < 8; ++u) ht2[rt][kc][u] = (H)0.f;
-          continue;
-        }
+        *reinterpret_cast<float4*>(sem + 16 * r + 4 * g) = sl == g ? ev : make_float4(0.f, 0.f, 0.f, 0.f);
+        // weights and constants stay in LDS (re-read per tile), not in registers
+        asm volatile("" ::: "memory");
         TileFwd<H> f;
-        forward_tile<H, kSkip>(e.x, s_frag, cst, g, lane, f);
+        forward_tile<H, kSkip>(ev.x, s_frag, cst, g, lane, f);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // em^T as A (M = 4 slot + c, K = row 4 g + u)
+        h4<H> a;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = (H)sem[16 * (4 * g + u) + r];
 #pragma unroll
         for (int kc = 0; kc < 4; ++kc) {
           const h8<H> rb = r3_frag(f, kc);
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
             const f32x4 d = mfma(rb, eye[j], f32x4{0.f, 0.f, 0.f, 0.f});
+            h4<H> b;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) rt3[rt][kc][4 * j + v] = (H)d[v];
+            for (int v = 0; v < 4; ++v) b[v] = (H)d[v];
+            acc[2 * kc + j] = mfma16(a, b, acc[2 * kc + j]);
           }
         }
 #pragma unroll
@@ -358,44 +361,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
             const f32x4 d = mfma(f.b2[kc], eye[j], f32x4{0.f, 0.f, 0.f, 0.f});
+            h4<H> b;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) ht2[rt][kc][4 * j + v] = (H)d[v];
+            for (int v = 0; v < 4; ++v) b[v] = (H)d[v];
+            acc_h[2 * kc + j] = mfma16(a, b, acc_h[2 * kc + j]);
           }
+        h4<H> ones;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ones[u] = (H)1.f;
+        acc_s = mfma16(a, ones, acc_s);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
       }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      // em^T as A (M = 4 slot + c, K = rows in psi order)
-      h8<H> a;
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int row = u < 4 ? 4 * g + u : 16 + 4 * g + (u - 4);
-        a[u] = (H)sem[16 * row + r];
-      }
-#pragma unroll
-      for (int nt = 0; nt < 8; ++nt) {
-        h8<H> b;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          b[v] = rt3[0][nt >> 1][4 * (nt & 1) + v];
-          b[4 + v] = rt3[1][nt >> 1][4 * (nt & 1) + v];
-        }
-        acc[nt] = mfma(a, b, acc[nt]);
-      }
-#pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
-        h8<H> b;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          b[v] = ht2[0][nt >> 1][4 * (nt & 1) + v];
-          b[4 + v] = ht2[1][nt >> 1][4 * (nt & 1) + v];
-        }
-        acc_h[nt] = mfma(a, b, acc_h[nt]);
-      }
-      acc_s = mfma(a, ones, acc_s);
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
       if (c.nfin > 0) {
-        // atoms that ended in this chunk: lane group g holds the atom of slot
+        // atoms that ended in this tile: lane group g holds the atom of slot
         // g, which is k0 + kk; its GR_t^T rows in the original feature order
         const int kk = (g - k0) & (kSlots - 1);
         const bool fin = kk < c.nfin;
@@ -403,7 +382,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
         if (fin) {
           const int ai = wa + k0 + kk;
           // (n1 passes through an opaque copy so the index arithmetic below is
-          // not hoisted out of the chunk loop, where registers are scarce)
+          // not hoisted out of the tile loop, where registers are scarce)
           int n1 = p.n1;
           asm volatile("" : "+s"(n1));
           const int M = 4 * n1;
@@ -445,10 +424,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       k0 = k0n;
       p0 = p0n;
       c = cn;
-      sl[0] = sln[0];
-      sl[1] = sln[1];
-      ev[0] = evn[0];
-      ev[1] = evn[1];
     }
   }
 }
@@ -616,17 +591,19 @@ __global__ void __launch_bounds__(256)
 // term, and into one of the two halves of a 16-byte H row per feature
 // [q3] = (dGR slot 0 [0..3][q3], dGR slot 1 [0..3][q3]), the A operand of
 // dG = em dGR (the B operand holds em[row][c] in K slot 4 slot(row) + c).
+constexpr int kWavesBwd = 12;  // wavefronts per backward workgroup: three per SIMD
+
 template <typename H, bool kSkip>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+__global__ void __launch_bounds__(64 * kWavesBwd) __attribute__((amdgpu_waves_per_eu(3, 3)))
     k_embed_bwd(float* __restrict__ net_deriv, const float* __restrict__ em, const float* __restrict__ dgr,
                 const h8<H>* __restrict__ frag_all, const float* __restrict__ consts, int nloc, int per_wave,
                 EmbedParams p) {
   constexpr int kS = 2;  // atoms per tile
   __shared__ h8<H> s_frag[(kFragFwd + kFragBwd) * 64];
-  __shared__ __attribute__((aligned(16))) float s_dg[kAtoms][kQ3 * 4];
-  __shared__ h8<H> s_dgb[kAtoms][kQ3];
-  __shared__ h8<H> s_dop[kAtoms][kS][6 * 16];
-  __shared__ __attribute__((aligned(16))) float s_kin[kAtoms][kS][4];
+  __shared__ __attribute__((aligned(16))) float s_dg[kWavesBwd][kQ3 * 4];
+  __shared__ h8<H> s_dgb[kWavesBwd][kQ3];
+  __shared__ h8<H> s_dop[kWavesBwd][kS][6 * 16];
+  __shared__ __attribute__((aligned(16))) float s_kin[kWavesBwd][kS][4];
   __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
   const int t = blockIdx.y;
   const int w = wave_id(), lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
@@ -723,7 +700,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   };
-  const int a_lo = min((blockIdx.x * kAtoms + w) * per_wave, nloc), a_hi = min(a_lo + per_wave, nloc);
+  const int a_lo = min((blockIdx.x * kWavesBwd + w) * per_wave, nloc), a_hi = min(a_lo + per_wave, nloc);
   for (int wa = a_lo; wa < a_hi; wa += 64) {
     const int n = min(64, a_hi - wa);
     int cnt = 0;
@@ -943,14 +920,16 @@ int embed_bwd(float* net_deriv, const float* em, const float* dgr, const void* f
   p.n1 = n1;
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
   if (nloc == 0) return DPMI_OK;
-  dim3 grid((unsigned)std::min<int64_t>(ceil_div(nloc, kAtoms), 1024), (unsigned)ntypes);
-  const int per_wave = (int)ceil_div(nloc, (int64_t)grid.x * kAtoms);
+  // one workgroup of twelve wavefronts per CU (127 KB of LDS), up to 1,024
+  // per type; each wavefront streams a contiguous range of per_wave atoms
+  dim3 grid((unsigned)std::min<int64_t>(ceil_div(nloc, kWavesBwd), 1024), (unsigned)ntypes);
+  const int per_wave = (int)ceil_div(nloc, (int64_t)grid.x * kWavesBwd);
   if (skip_layout(n1))
-    hipLaunchKernelGGL((k_embed_bwd<H, true>), grid, dim3(256), 0, as_stream(stream), net_deriv, em, dgr,
-                       static_cast<const h8<H>*>(frags), consts, (int)nloc, per_wave, p);
+    hipLaunchKernelGGL((k_embed_bwd<H, true>), grid, dim3(64 * kWavesBwd), 0, as_stream(stream), net_deriv, em,
+                       dgr, static_cast<const h8<H>*>(frags), consts, (int)nloc, per_wave, p);
   else
-    hipLaunchKernelGGL((k_embed_bwd<H, false>), grid, dim3(256), 0, as_stream(stream), net_deriv, em, dgr,
-                       static_cast<const h8<H>*>(frags), consts, (int)nloc, per_wave, p);
+    hipLaunchKernelGGL((k_embed_bwd<H, false>), grid, dim3(64 * kWavesBwd), 0, as_stream(stream), net_deriv, em,
+                       dgr, static_cast<const h8<H>*>(frags), consts, (int)nloc, per_wave, p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }
