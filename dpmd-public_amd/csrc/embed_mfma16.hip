@@ -195,16 +195,18 @@ struct Chunk {
 __device__ inline int pre_at(int pre, int k, int n) { return __builtin_amdgcn_readlane(pre, min(k, n - 1)); }
 
 // the chunk that starts at stream position p0 with atom k0 < n unfinished
-__device__ inline Chunk make_chunk(int pre, int k0, int p0, int n, int rows) {
+// and may hold rows of atoms k0 .. klast (klast <= k0 + 3)
+__device__ inline Chunk make_chunk(int pre, int k0, int p0, int n, int rows, int klast) {
   Chunk c;
+  klast = min(klast, n - 1);
   c.s0 = k0 > 0 ? pre_at(pre, k0 - 1, n) : 0;
   c.e0 = pre_at(pre, k0, n);
   c.e1 = pre_at(pre, k0 + 1, n);
   c.e2 = pre_at(pre, k0 + 2, n);
   const int e3 = pre_at(pre, k0 + 3, n);
-  c.p1 = min(p0 + rows, e3);
-  c.nfin = (int)(c.e0 <= c.p1) + (int)(k0 + 1 < n && c.e1 <= c.p1) + (int)(k0 + 2 < n && c.e2 <= c.p1) +
-           (int)(k0 + 3 < n && e3 <= c.p1);
+  c.p1 = min(p0 + rows, pre_at(pre, klast, n));
+  c.nfin = (int)(c.e0 <= c.p1) + (int)(k0 + 1 <= klast && c.e1 <= c.p1) + (int)(k0 + 2 <= klast && c.e2 <= c.p1) +
+           (int)(k0 + 3 <= klast && e3 <= c.p1);
   return c;
 }
 
@@ -316,15 +318,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
     for (int nt = 0; nt < 8; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) acc_h[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-    int k0 = 0, p0 = 0;
-    Chunk c = make_chunk(pre, 0, 0, n, 16);
+    // Finished atoms keep their accumulators until three have ended (or the
+    // window is done), and are then written in one pass with one lane group
+    // each; until then the tiles hold rows of atoms up to kf + 3 only, kf the
+    // first atom not written yet.
+    int k0 = 0, p0 = 0, kf = 0;
+    Chunk c = make_chunk(pre, 0, 0, n, 16, 3);
     float4 ev_next = load_row(c, 0, 0);
     while (k0 < n) {
       const int k0n = k0 + c.nfin, p0n = c.p1;
+      const bool flush = k0n >= n || k0n - kf >= 3;
+      const int kfn = flush ? k0n : kf;
       const float4 ev = ev_next;
       Chunk cn = c;
       if (k0n < n) {
-        cn = make_chunk(pre, k0n, p0n, n, 16);
+        cn = make_chunk(pre, k0n, p0n, n, 16, kfn + 3);
         ev_next = load_row(cn, k0n, p0n);
       }
       if (c.p1 > p0) {
@@ -373,14 +381,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
       }
-      if (c.nfin > 0) {
-        // atoms that ended in this tile: lane group g holds the atom of slot
-        // g, which is k0 + kk; its GR_t^T rows in the original feature order
-        const int kk = (g - k0) & (kSlots - 1);
-        const bool fin = kk < c.nfin;
-        const int cnt_a = __shfl(cnt, min(k0 + kk, 63), 64);
+      if (flush) {
+        // the finished atoms kf .. k0n - 1: lane group g holds the atom of
+        // slot g, which is kf + kk; its GR_t^T rows in the original feature order
+        const int kk = (g - kf) & (kSlots - 1);
+        const bool fin = kf + kk < k0n;
+        const int cnt_a = __shfl(cnt, min(kf + kk, 63), 64);
         if (fin) {
-          const int ai = wa + k0 + kk;
+          const int ai = wa + kf + kk;
           // (n1 passes through an opaque copy so the index arithmetic below is
           // not hoisted out of the tile loop, where registers are scarce)
           int n1 = p.n1;
@@ -423,6 +431,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
       }
       k0 = k0n;
       p0 = p0n;
+      kf = kfn;
       c = cn;
     }
   }
