@@ -230,7 +230,7 @@ __device__ inline int chunk_atom(const Chunk& c, int k0, int pos, int* row) {
   return k0 + kk;
 }
 
-// Forward: GR partial [t][i][c][m] (original feature order m < 4 n1) of
+// Forward: GR partial [t][i][m][c] (original feature order m < 4 n1) of
 // the atoms of each wavefront's range, over 16-row tiles of the row stream
 // holding rows of up to four atoms.  A workgroup loads its type's weight
 // fragments once; its four wavefronts stream independently (no block
@@ -394,7 +394,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
           int n1 = p.n1;
           asm volatile("" : "+s"(n1));
           const int M = 4 * n1;
-          float* out = gr_part + ((int64_t)t * nloc + ai) * 4 * M;
+          float4* out = reinterpret_cast<float4*>(gr_part) + ((int64_t)t * nloc + ai) * M;
           // padding rows not evaluated: n_pad em_pad^T G_pad
           float padc[4] = {0.f, 0.f, 0.f, 0.f};
           int tp = 0;
@@ -413,10 +413,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
             const int m = orig_of_q3(q, n1);
             const float idt3 = cst[352 + q];
             const float gp = n_pad > 0 ? s_gpad[tp][q] : 0.f;
-            if (m >= 0)
+            float o[4];
 #pragma unroll
-              for (int v = 0; v < 4; ++v)
-                out[v * M + m] = idt3 * (acc_s[v] - 2.f * acc[nt][v]) + acc_h[nt & 3][v] + padc[v] * gp;
+            for (int v = 0; v < 4; ++v) o[v] = idt3 * (acc_s[v] - 2.f * acc[nt][v]) + acc_h[nt & 3][v] + padc[v] * gp;
+            if (m >= 0) out[m] = make_float4(o[0], o[1], o[2], o[3]);
           }
         }
         // the finished slots start the next atom from zero
@@ -437,7 +437,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
   }
 }
 
-// GR = sum_t GR_t / nnei (original order, saved for the backward) and
+// GR = sum_t GR_t / nnei (partials [t][i][m][c]; GR [i][c][m] in the
+// original order, saved for the backward) and
 // D[m][a] = sum_c GR[c][m] GR[c][a] (se_a.py:851-858).
 template <typename DT>
 __global__ void __launch_bounds__(256)
@@ -448,12 +449,13 @@ __global__ void __launch_bounds__(256)
   const int i = blockIdx.x * kAtoms + w;
   if (i >= nloc) return;
   const int M = 4 * n1;
-  for (int k = lane; k < 4 * M; k += 64) {
+  for (int k = lane; k < 4 * M; k += 64) {  // k = 4 m + c
     float v = 0.f;
     for (int t = 0; t < ntypes; ++t) v += gr_part[((int64_t)t * nloc + i) * 4 * M + k];
     v *= inv_nnei;
-    s[w][k] = v;
-    gr[(int64_t)i * 4 * M + k] = v;
+    const int ck = (k & 3) * M + (k >> 2);
+    s[w][ck] = v;
+    gr[(int64_t)i * 4 * M + ck] = v;
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();

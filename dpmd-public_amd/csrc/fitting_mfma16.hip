@@ -247,15 +247,15 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
   if constexpr (kGR) {
     // all of the tile's GR at once: work item (atom la, feature m), kBatch
     // items per thread in flight; the four components of every type are read
-    // as scalars (consecutive lanes read consecutive m: coalesced) and
-    // written as one float4 GR[.][m] (consecutive lanes write consecutive LDS
-    // addresses: no bank conflicts); GR[c][m] -> s_gr[atom][m][c]
+    // as one float4 (gr_part [t][i][m][c]: consecutive lanes read consecutive
+    // 16-byte words) and written as one float4 GR[.][m] (consecutive lanes
+    // write consecutive LDS addresses: no bank conflicts) -> s_gr[atom][m][c]
     constexpr int kBatch = 4;
     const int nit = kTile * p.M;
     const float inv_m = 1.f / (float)p.M;
 #pragma unroll 1
     for (int k0 = tid; k0 < nit; k0 += kThreads * kBatch) {
-      float x[kBatch][kMaxTypesGR][4];
+      f32x4 x[kBatch][kMaxTypesGR];
 #pragma unroll
       for (int b = 0; b < kBatch; ++b) {
         const int k = k0 + kThreads * b;
@@ -264,10 +264,8 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
         const int m = k - la * p.M;
 #pragma unroll
         for (int tt = 0; tt < kMaxTypesGR; ++tt) {
-          const float* src = gr_part + ((int64_t)tt * p.nloc + (ai < 0 ? 0 : ai)) * 4 * p.M + m;
-#pragma unroll
-          for (int c = 0; c < 4; ++c)
-            x[b][tt][c] = (ai >= 0 && tt < p.ntypes) ? __builtin_nontemporal_load(src + c * p.M) : 0.f;
+          const f32x4* src = reinterpret_cast<const f32x4*>(gr_part) + ((int64_t)tt * p.nloc + (ai < 0 ? 0 : ai)) * p.M + m;
+          x[b][tt] = (ai >= 0 && tt < p.ntypes) ? __builtin_nontemporal_load(src) : f32x4{0.f, 0.f, 0.f, 0.f};
         }
       }
 #pragma unroll

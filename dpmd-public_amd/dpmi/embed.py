@@ -160,13 +160,14 @@ class FusedEmbedding:
         self.consts = torch.from_numpy(consts).to(device)
 
     def forward_part(self, em: torch.Tensor, pad=None) -> torch.Tensor:
-        """em [nloc, nnei, 4] fp32 -> gr_part [ntypes, nloc, 4, M]: each
-        neighbor type's em^T G (original feature order, not yet / nnei).
+        """em [nloc, nnei, 4] fp32 -> gr_part [ntypes, nloc, M, 4]: each
+        neighbor type's (em^T G)^T (original feature order, the four
+        components of a feature contiguous, not yet / nnei).
         pad = (counts int16 [nloc, ntypes], atype int32 [nloc], pad_em fp32
         [ntypes, ntypes, 4]) enables the padding shortcut: the slots after
         each section's listed neighbors all hold pad_em[type(i), t]."""
         nloc = em.shape[0]
-        gr_part = torch.empty((self.ntypes, nloc, 4, self.M), dtype=torch.float32, device=em.device)
+        gr_part = torch.empty((self.ntypes, nloc, self.M, 4), dtype=torch.float32, device=em.device)
         counts, atype, pad_em = pad if pad is not None else (None, None, None)
         ops._launch(self._fwd, ptr(gr_part), ptr(em.contiguous()), ptr(self.frags), ptr(self.consts),
                     nloc, ptr(self.sel), self.ntypes, self.n1, ptr(counts), ptr(atype), ptr(pad_em), ops._stream())

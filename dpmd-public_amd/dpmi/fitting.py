@@ -148,13 +148,13 @@ class FusedFitting:
         return axis == 16 and 4 <= n1 <= 26 and self.k1 == 64 * n1 and self.ntypes <= 4
 
     def from_gr(self, gr_part: torch.Tensor, atype: torch.Tensor, n1: int, nnei: int, order=None):
-        """Fused descriptor form: gr_part [ntypes, nloc, 4, 4 n1] fp32 (the
+        """Fused descriptor form: gr_part [ntypes, nloc, 4 n1, 4] fp32 (the
         embedding forward's partials) -> (atom_energy float64 [nloc],
         dE/d(sum_t gr_part) fp32 [nloc, 128, 4] in the embedding kernels'
         internal order)."""
         nloc = gr_part.shape[1]
-        if gr_part.dtype != torch.float32 or gr_part.shape[2:] != (4, 4 * n1):
-            raise ValueError("gr_part must be float32 [ntypes, nloc, 4, 4 n1]")
+        if gr_part.dtype != torch.float32 or gr_part.shape[2:] != (4 * n1, 4):
+            raise ValueError("gr_part must be float32 [ntypes, nloc, 4 n1, 4]")
         perm, start = self.sort_by_type(atype) if order is None else order
         gr_part = gr_part.contiguous()
         ae = torch.empty(nloc, dtype=torch.float64, device=gr_part.device)

@@ -369,8 +369,9 @@ int dpmi_tabulate_build_f64(double* table, double* node_vdd, const double* xx, c
  * where env_mat_a_cpu stops, source/lib/src/env_mat.cc:158-160). */
 int dpmi_se_a_section_counts(int16_t* counts, const int* nlist, int64_t nloc, const int* sel_host, int ntypes,
                              void* stream);
-/* gr_part [ntypes, nloc, 4, 4 n1]: sum over each type's neighbors of em^T G
- * (original feature order).  Optional padding shortcut (counts non-NULL, as
+/* gr_part [ntypes, nloc, 4 n1, 4]: sum over each type's neighbors of
+ * (em^T G)^T (original feature order, the four components of a feature
+ * contiguous).  Optional padding shortcut (counts non-NULL, as
  * dpmi_se_a_section_counts writes them): the slots after an atom's listed
  * neighbors must all hold the same em row, pad_em [ntypes (center),
  * ntypes (neighbor), 4] for the atom's type atype [nloc] (true when davg /
@@ -443,7 +444,7 @@ int dpmi_se_a_fitting_f16(double* atom_energy, float* ddesc, const void* desc, c
                           const int* type_start, const void* frags, const float* consts, const double* ebias,
                           int64_t nloc, int ntypes, int k1, void* stream);
 /* The same fitting net with the descriptor fused in (axis 16, M = 4 n1 with
- * 16 <= M <= 104): gr_part [ntypes, nloc, 4, M] as dpmi_se_a_embed_fwd_bf16
+ * 16 <= M <= 104): gr_part [ntypes, nloc, M, 4] as dpmi_se_a_embed_fwd_bf16
  * writes it; each 64-atom workgroup forms GR = sum_t gr_part / nnei and
  * D = GR^T GR[:, :16] (se_a.py:851-858) in LDS and, instead of dE/dD, writes
  * dgr [nloc, 128, 4] = dE/d(sum_t gr_part) in the embedding kernels' internal

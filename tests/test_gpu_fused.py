@@ -275,7 +275,7 @@ def test_fused_descriptor_fitting(setup, nloc):
         atype = t.to(torch.int32)
     else:
         g = torch.Generator(device="cpu").manual_seed(nloc)
-        gr_part = (torch.randn(fe.ntypes, nloc, 4, fe.M, generator=g) * 15.0).to(DEV)
+        gr_part = (torch.randn(fe.ntypes, nloc, 4, fe.M, generator=g) * 15.0).transpose(2, 3).contiguous().to(DEV)
         atype = (torch.rand(nloc, generator=g) < 0.35).to(torch.int32).to(DEV)
     _check_fused_descriptor(fe, ff, gr_part, atype)
 
@@ -298,7 +298,7 @@ def test_fused_descriptor_fitting_shapes(n1, ntypes, precision):
     assert m.fused_gr
     g = torch.Generator(device="cpu").manual_seed(n1)
     nloc = 700
-    gr_part = (torch.randn(ntypes, nloc, 4, 4 * n1, generator=g) * 10.0).to(DEV)
+    gr_part = (torch.randn(ntypes, nloc, 4, 4 * n1, generator=g) * 10.0).transpose(2, 3).contiguous().to(DEV)
     atype = torch.randint(0, ntypes, (nloc,), generator=g).to(torch.int32).to(DEV)
     _check_fused_descriptor(m.fused, m.fused_fit, gr_part, atype)
 
