@@ -613,6 +613,7 @@ __global__ void __launch_bounds__(64 * kWavesBwd) __attribute__((amdgpu_waves_pe
   __shared__ h8<H> s_frag[(kFragFwd + kFragBwd) * 64];
   __shared__ __attribute__((aligned(16))) float s_dg[kWavesBwd][kQ3 * 4];
   __shared__ h8<H> s_dgb[kWavesBwd][kQ3];
+  __shared__ h8<H> s_dgbs[kWavesBwd][kQ3];  // the same times 4 idt3 / k (the dz3 factor)
   __shared__ h8<H> s_dop[kWavesBwd][kS][6 * 16];
   __shared__ __attribute__((aligned(16))) float s_kin[kWavesBwd][kS][4];
   __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
@@ -628,11 +629,15 @@ __global__ void __launch_bounds__(64 * kWavesBwd) __attribute__((amdgpu_waves_pe
   const int sel = p.sec[t + 1] - p.sec[t];
   float* sdg = s_dg[w];
   h8<H>* sdgb = s_dgb[w];
+  h8<H>* sdgbs = s_dgbs[w];
   // a slot not staged yet still meets the zero K slots of the other slot's
-  // rows: its half of the dG operand must be finite
+  // rows: its half of the dG operands must be finite
   for (int q = lane; q < kQ3; q += 64)
 #pragma unroll
-    for (int u = 0; u < 8; ++u) sdgb[q][u] = (H)0.f;
+    for (int u = 0; u < 8; ++u) {
+      sdgb[q][u] = (H)0.f;
+      sdgbs[q][u] = (H)0.f;
+    }
   const float* em_t = em + (int64_t)p.sec[t] * 4;
   float* nd_t = net_deriv + (int64_t)p.sec[t] * 4;
   // fp16 operands: everything below is linear in dGR, so each atom's dGR is
@@ -681,13 +686,18 @@ __global__ void __launch_bounds__(64 * kWavesBwd) __attribute__((amdgpu_waves_pe
     __builtin_amdgcn_wave_barrier();
     for (int q = lane; q < kQ3; q += 64) {
       const float4 x = *reinterpret_cast<const float4*>(sdg + 4 * q);
-      typedef H h4 __attribute__((ext_vector_type(4)));
-      h4 v;
+      const float c3 = cst[576 + q];
+      h4<H> v, vs;
       v[0] = (H)x.x;
       v[1] = (H)x.y;
       v[2] = (H)x.z;
       v[3] = (H)x.w;
-      reinterpret_cast<h4*>(sdgb + q)[s] = v;
+      vs[0] = (H)(x.x * c3);
+      vs[1] = (H)(x.y * c3);
+      vs[2] = (H)(x.z * c3);
+      vs[3] = (H)(x.w * c3);
+      reinterpret_cast<h4<H>*>(sdgb + q)[s] = v;
+      reinterpret_cast<h4<H>*>(sdgbs + q)[s] = vs;
     }
     h8<H>* sdop = s_dop[w][s];
     for (int k = lane; k < 96; k += 64) {
@@ -783,7 +793,8 @@ __global__ void __launch_bounds__(64 * kWavesBwd) __attribute__((amdgpu_waves_pe
         }
         // dG = em dGR -> D(M = q3, N = rows), A = row q3 of sdgb (lane groups
         // 1..3 meet the zero K slots of b_em);
-        // dz3 = dG idt3 (1 - t3^2) = dG 4 idt3 r3 (1 - r3); skip = dG[q] + dG[q + 64]
+        // dz3 = dG idt3 (1 - t3^2) = (dG 4 idt3 / k) r3 (1 - r3) / k, the first
+        // factor from the scaled operand sdgbs; skip = dG[q] + dG[q + 64]
         f32x4 skip[4];
         h8<H> b_dz3[4];
         // keep the sdgb reads below the recompute (hoisted, they would hold 32
@@ -792,10 +803,10 @@ __global__ void __launch_bounds__(64 * kWavesBwd) __attribute__((amdgpu_waves_pe
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
           const h8<H> a_lo = sdgb[16 * mt + r], a_hi = sdgb[16 * (mt + 4) + r];
-          const f32x4 d_lo = mfma(a_lo, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
-          const f32x4 d_hi = mfma(a_hi, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
-          skip[mt] = mfma(a_hi, b_em, d_lo);
-          const int q = 16 * mt + 4 * g;
+          const h8<H> as_lo = sdgbs[16 * mt + r], as_hi = sdgbs[16 * (mt + 4) + r];
+          const f32x4 d_lo = mfma(as_lo, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
+          const f32x4 d_hi = mfma(as_hi, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
+          skip[mt] = mfma(a_hi, b_em, mfma(a_lo, b_em, f32x4{0.f, 0.f, 0.f, 0.f}));
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             if (kSkip && (mt & 1) && v == 3) {  // empty slot: no weights read it
@@ -804,8 +815,8 @@ __global__ void __launch_bounds__(64 * kWavesBwd) __attribute__((amdgpu_waves_pe
               continue;
             }
             const float rl = f.r3[mt * 4 + v], rh = f.r3[(mt + 4) * 4 + v];
-            b_dz3[mt >> 1][(mt & 1) * 4 + v] = (H)(d_lo[v] * cst[576 + q + v] * fmaf(-rl, rl, rl));
-            b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (H)(d_hi[v] * cst[640 + q + v] * fmaf(-rh, rh, rh));
+            b_dz3[mt >> 1][(mt & 1) * 4 + v] = (H)(d_lo[v] * fmaf(-rl, rl, rl));
+            b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (H)(d_hi[v] * fmaf(-rh, rh, rh));
           }
         }
         // dh2 = dz3 W3^T + skip;  dz2 = dh2 * idt2 * (1 - t2^2)
