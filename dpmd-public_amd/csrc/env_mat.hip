@@ -604,79 +604,102 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
     FP* accw = acc + w * 3 * kCap;
-    for (int ai = w; ai < na; ai += kAtomsPerBlock) {
-      const int i = cell_atoms[a0 + ai];
-      const int* nl = nlist + (int64_t)i * nnei;
-      const int ti = ext_t[i] < sp.ntypes ? ext_t[i] : -1;
-      const double xi0 = ext_c[3 * i + 0], xi1 = ext_c[3 * i + 1], xi2 = ext_c[3 * i + 2];
-      double ci[3] = {0., 0., 0.};
-      // neighbor term of slot s (neighbor j, its coordinates, dE/d em row,
-      // std row and table slot already loaded)
-      auto term = [&](int s, int j, const double xj[3], const FP n[4], const FP sd[4], int cs) {
-        const double r64[3] = {xj[0] - xi0, xj[1] - xi1, xj[2] - xi2};
-        const FP rr[3] = {(FP)r64[0], (FP)r64[1], (FP)r64[2]};
-        FP val[4], d[12];
+    // neighbor term of slot s of atom i (neighbor j, its coordinates, dE/d em
+    // row, std row and table slot already loaded); centre sum in ci
+    auto term = [&](double ci[3], double xi0, double xi1, double xi2, int j, const double xj[3], const FP n[4],
+                    const FP sd[4], int cs) {
+      const double r64[3] = {xj[0] - xi0, xj[1] - xi1, xj[2] - xi2};
+      const FP rr[3] = {(FP)r64[0], (FP)r64[1], (FP)r64[2]};
+      FP val[4], d[12];
+      if constexpr (kFixed)
+        env_one_fast(rr, rmin, inv_range, val, d);
+      else
+        env_one(rr, rmin, rmax, val, d);
+      FP g[3] = {(FP)0., (FP)0., (FP)0.};
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        FP nc;
         if constexpr (kFixed)
-          env_one_fast(rr, rmin, inv_range, val, d);
+          nc = n[c] * __builtin_amdgcn_rcpf(sd[c]);
         else
-          env_one(rr, rmin, rmax, val, d);
-        FP g[3] = {(FP)0., (FP)0., (FP)0.};
+          nc = n[c] / sd[c];
+        g[0] += nc * d[3 * c + 0];
+        g[1] += nc * d[3 * c + 1];
+        g[2] += nc * d[3 * c + 2];
+      }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          FP nc;
-          if constexpr (kFixed)
-            nc = n[c] * __builtin_amdgcn_rcpf(sd[c]);
-          else
-            nc = n[c] / sd[c];
-          g[0] += nc * d[3 * c + 0];
-          g[1] += nc * d[3 * c + 1];
-          g[2] += nc * d[3 * c + 2];
-        }
+      for (int k = 0; k < 3; ++k) ci[k] += (double)g[k];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) ci[k] += (double)g[k];
+      for (int d0 = 0; d0 < 3; ++d0)
 #pragma unroll
-        for (int d0 = 0; d0 < 3; ++d0)
+        for (int d1 = 0; d1 < 3; ++d1) v[d0 * 3 + d1] += (double)g[d0] * r64[d1];
+      if (cs >= 0 && cs < T) {
+        if constexpr (kFixed) {
 #pragma unroll
-          for (int d1 = 0; d1 < 3; ++d1) v[d0 * 3 + d1] += (double)g[d0] * r64[d1];
-        if (cs >= 0 && cs < T) {
-          if constexpr (kFixed) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-              atomicAdd(&accq[3 * cs + k], (unsigned long long)__float2ll_rn((float)g[k] * kFix));
-          } else {
-            accw[3 * cs + 0] += g[0];
-            accw[3 * cs + 1] += g[1];
-            accw[3 * cs + 2] += g[2];
-          }
+          for (int k = 0; k < 3; ++k)
+            atomicAdd(&accq[3 * cs + k], (unsigned long long)__float2ll_rn((float)g[k] * kFix));
         } else {
-          const int o = mapping ? mapping[j] : j;
-          atomicAdd(&force[3 * (int64_t)o + 0], (double)g[0]);
-          atomicAdd(&force[3 * (int64_t)o + 1], (double)g[1]);
-          atomicAdd(&force[3 * (int64_t)o + 2], (double)g[2]);
+          accw[3 * cs + 0] += g[0];
+          accw[3 * cs + 1] += g[1];
+          accw[3 * cs + 2] += g[2];
         }
-      };
-      if (nnei <= kWave * kChunks) {
-        // the atom's whole row in registers: every load of every chunk is
-        // issued before any arithmetic, so the wave waits for two dependent
-        // round trips (list, then coordinates) instead of two per chunk
-        int jv[kChunks];
+      } else {
+        const int o = mapping ? mapping[j] : j;
+        atomicAdd(&force[3 * (int64_t)o + 0], (double)g[0]);
+        atomicAdd(&force[3 * (int64_t)o + 1], (double)g[1]);
+        atomicAdd(&force[3 * (int64_t)o + 2], (double)g[2]);
+      }
+    };
+    auto centre = [&](int i, double ci[3]) {
 #pragma unroll
-        for (int k = 0; k < kChunks; ++k) jv[k] = lane + kWave * k < nnei ? nl[lane + kWave * k] : 0;
-        FP nv[kChunks][4], sv[kChunks][4];
-        int csv[kChunks];
+      for (int k = 0; k < 3; ++k) ci[k] = wave_sum(ci[k]);
+      if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) atomicAdd(&force[3 * (int64_t)i + k], -ci[k]);
+    };
+    if (nnei <= kWave * kChunks) {
+      // The atom's whole row in registers.  In fp32 the loads that need
+      // nothing but the atom (list, dE/d em, table slots, its coordinates)
+      // are issued one atom ahead, so a wavefront waits for one round trip
+      // per atom (the neighbor coordinates) instead of two.
+      struct Row {
+        int i, ti;
+        double xi[3];
+        int jv[kChunks], csv[kChunks];
+        FP nv[kChunks][4];
+      };
+      auto load_row = [&](int ai, Row& R) {
+        const int i = ai < na ? cell_atoms[a0 + ai] : cell_atoms[a0];
+        R.i = i;
+        const int tr = ext_t[i];
+        R.ti = ai < na && tr < sp.ntypes ? tr : -1;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) R.xi[c] = ext_c[3 * i + c];
+        const int* nl = nlist + (int64_t)i * nnei;
 #pragma unroll
         for (int k = 0; k < kChunks; ++k) {
           const int s = lane + kWave * k;
-          const bool in = s < nnei && ti >= 0;
+          const bool in = s < nnei && R.ti >= 0;
+          R.jv[k] = s < nnei ? nl[s] : 0;
           const FP* n = nd + ((int64_t)i * nnei + (in ? s : 0)) * 4;
-          const FP* sd = std_ + ((int64_t)(ti < 0 ? 0 : ti) * nnei + (in ? s : 0)) * 4;
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            nv[k][c] = in ? n[c] : (FP)0.;
-            sv[k][c] = in ? sd[c] : (FP)1.;
-          }
-          csv[k] = in && ok ? (int)cand_slot[(int64_t)i * nnei + s] : -1;
+          for (int c = 0; c < 4; ++c) R.nv[k][c] = in ? n[c] : (FP)0.;
+          R.csv[k] = in && ok ? (int)cand_slot[(int64_t)i * nnei + s] : -1;
         }
+      };
+      // (fp64 holds twice the registers per row: it loads at use)
+      constexpr bool kAhead = kFixed;
+      Row nx;
+      if constexpr (kAhead) load_row(w, nx);
+      for (int ai = w; ai < na; ai += kAtomsPerBlock) {
+        Row R;
+        if constexpr (kAhead) {
+          R = nx;
+          if (ai + kAtomsPerBlock < na) load_row(ai + kAtomsPerBlock, nx);
+        } else {
+          load_row(ai, R);
+        }
+        const int i = R.i, ti = R.ti;
         // section limits (first -1 of each type section) from the registers
         for (int t = lane; t < sp.ntypes; t += kWave) lim_all[w][t] = sp.sec[t + 1];
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -684,24 +707,38 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int k = 0; k < kChunks; ++k) {
           const int s = lane + kWave * k;
-          if (s < nnei && jv[k] < 0) atomicMin(&lim_all[w][slot_type(sp, s)], s);
+          if (s < nnei && R.jv[k] < 0) atomicMin(&lim_all[w][slot_type(sp, s)], s);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         bool act[kChunks];
         double xj[kChunks][3];
+        FP sv[kChunks][4];
 #pragma unroll
         for (int k = 0; k < kChunks; ++k) {
           const int s = lane + kWave * k;
           act[k] = ti >= 0 && s < nnei && s < lim_all[w][slot_type(sp, s)];
-          const int j = act[k] ? jv[k] : i;
+          const int j = act[k] ? R.jv[k] : i;
 #pragma unroll
           for (int c = 0; c < 3; ++c) xj[k][c] = ext_c[3 * j + c];
+          // the std rows (ntypes x nnei x 4, cache resident) at use
+          const FP* sd = std_ + ((int64_t)(ti < 0 ? 0 : ti) * nnei + (s < nnei ? s : 0)) * 4;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) sv[k][c] = act[k] ? sd[c] : (FP)1.;
         }
+        double ci[3] = {0., 0., 0.};
 #pragma unroll
         for (int k = 0; k < kChunks; ++k)
-          if (act[k]) term(lane + kWave * k, jv[k], xj[k], nv[k], sv[k], csv[k]);
-      } else {
+          if (act[k]) term(ci, R.xi[0], R.xi[1], R.xi[2], R.jv[k], xj[k], R.nv[k], sv[k], R.csv[k]);
+        centre(i, ci);
+      }
+    } else {
+      for (int ai = w; ai < na; ai += kAtomsPerBlock) {
+        const int i = cell_atoms[a0 + ai];
+        const int* nl = nlist + (int64_t)i * nnei;
+        const int ti = ext_t[i] < sp.ntypes ? ext_t[i] : -1;
+        const double xi0 = ext_c[3 * i + 0], xi1 = ext_c[3 * i + 1], xi2 = ext_c[3 * i + 2];
+        double ci[3] = {0., 0., 0.};
         section_limits_wave(nl, sp, lim_all[w]);
         for (int s = lane; s < nnei && ti >= 0; s += kWave) {
           const int t = slot_type(sp, s);
@@ -711,14 +748,10 @@ __global__ void __launch_bounds__(256)
           const FP* sd = std_ + ((int64_t)ti * nnei + s) * 4;
           const FP* n = nd + ((int64_t)i * nnei + s) * 4;
           const FP nn[4] = {n[0], n[1], n[2], n[3]}, ss[4] = {sd[0], sd[1], sd[2], sd[3]};
-          term(s, j, xj, nn, ss, ok ? (int)cand_slot[(int64_t)i * nnei + s] : -1);
+          term(ci, xi0, xi1, xi2, j, xj, nn, ss, ok ? (int)cand_slot[(int64_t)i * nnei + s] : -1);
         }
+        centre(i, ci);
       }
-#pragma unroll
-      for (int k = 0; k < 3; ++k) ci[k] = wave_sum(ci[k]);
-      if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) atomicAdd(&force[3 * (int64_t)i + k], -ci[k]);
     }
     __syncthreads();
     if (ok)
