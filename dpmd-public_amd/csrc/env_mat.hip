@@ -809,6 +809,48 @@ __global__ void __launch_bounds__(256)
   const int c = cell_of[o];
   const int c2 = c % p.ncell[2], c1 = (c / p.ncell[2]) % p.ncell[1], c0 = c / (p.ncell[1] * p.ncell[2]);
   double f[3] = {0., 0., 0.};
+  // the neighbor cell that holds this atom in its table at offset t, or -1
+  // when no table holds it there
+  auto holder = [&](int t) -> int {
+    const int o0 = t / (side * side) - ns, o1 = (t / side) % side - ns, o2 = t % side - ns;
+    int q0 = c0 - o0, q1 = c1 - o1, q2 = c2 - o2;
+    if (p.pbc) {
+      q0 += pbc_shift(q0, p.ncell[0]) * p.ncell[0];
+      q1 += pbc_shift(q1, p.ncell[1]) * p.ncell[1];
+      q2 += pbc_shift(q2, p.ncell[2]) * p.ncell[2];
+      const int i0 = q0 + o0, i1 = q1 + o1, i2 = q2 + o2;
+      if (i0 < -p.ngcell[0] || i0 >= p.ncell[0] + p.ngcell[0] || i1 < -p.ngcell[1] ||
+          i1 >= p.ncell[1] + p.ngcell[1] || i2 < -p.ngcell[2] || i2 >= p.ncell[2] + p.ngcell[2])
+        return -1;
+    } else if (q0 < 0 || q0 >= p.ncell[0] || q1 < 0 || q1 >= p.ncell[1] || q2 < 0 || q2 >= p.ncell[2]) {
+      return -1;
+    }
+    return (q0 * p.ncell[1] + q1) * p.ncell[2] + q2;
+  };
+  if (nnb == 27) {
+    // every flag and partial load issued before the sums (a partial slot
+    // that no table wrote holds stale data and is selected away, not added);
+    // the sums run in the same fixed order t = 0 .. 26
+    int okv[27];
+    FP x[27][3];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+      const int q = holder(t);
+      okv[t] = q >= 0 ? cell_ok[q] : 0;
+      const FP* src = partial + ((int64_t)t * nloc + pos) * 3;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) x[t][k] = src[k];
+    }
+#pragma unroll
+    for (int t = 0; t < 27; ++t)
+      if (okv[t])
+#pragma unroll
+        for (int k = 0; k < 3; ++k) f[k] += (double)x[t][k];
+    force[3 * (int64_t)o + 0] += f[0];
+    force[3 * (int64_t)o + 1] += f[1];
+    force[3 * (int64_t)o + 2] += f[2];
+    return;
+  }
   for (int t = 0; t < nnb; ++t) {
     const int o0 = t / (side * side) - ns, o1 = (t / side) % side - ns, o2 = t % side - ns;
     int q0 = c0 - o0, q1 = c1 - o1, q2 = c2 - o2;
