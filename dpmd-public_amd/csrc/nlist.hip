@@ -624,10 +624,14 @@ __global__ void __launch_bounds__(256) k_search_cells(
                             __int_as_float((code << 24) | j));
     }
     __syncthreads();
+    // the centre cell's members are the table's run at s_pref[nnb / 2], in
+    // cell order: an atom's index and float coordinates come from LDS (its
+    // double coordinates only where the float test cannot decide)
+    const int ctr = s_pref[nnb / 2];
     for (int ai = 4 * part + w; ai < na; ai += 4 * nsplit) {
-      const int i = cell_atoms[a0 + ai];
-      const double xi0 = ext_c[3 * i + 0], xi1 = ext_c[3 * i + 1], xi2 = ext_c[3 * i + 2];
-      const float fi0 = (float)xi0, fi1 = (float)xi1, fi2 = (float)xi2;
+      const float4 me = cand[ctr + ai];
+      const int i = __float_as_int(me.w) & 0xFFFFFF;
+      const float fi0 = me.x, fi1 = me.y, fi2 = me.z;
       int n = 0, n0 = 0;
       for (int g0 = 0; g0 < T; g0 += kWave) {
         const int g = g0 + lane;
@@ -647,7 +651,8 @@ __global__ void __launch_bounds__(256) k_search_cells(
           if (j != i && tj != kBadType && rr2 <= rcut2f) {
             bool in = (double)rr2 < rsure;
             if (!in) {
-              const double e0 = xi0 - ext_c[3 * j + 0], e1 = xi1 - ext_c[3 * j + 1], e2 = xi2 - ext_c[3 * j + 2];
+              const double e0 = ext_c[3 * i + 0] - ext_c[3 * j + 0], e1 = ext_c[3 * i + 1] - ext_c[3 * j + 1],
+                           e2 = ext_c[3 * i + 2] - ext_c[3 * j + 2];
               in = e0 * e0 + e1 * e1 + e2 * e2 < rcut2d;
             }
             if (in) {
