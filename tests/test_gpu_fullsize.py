@@ -115,3 +115,49 @@ def test_alloy262k_nlist_bit_exact_with_overflow(oracle):
     np.testing.assert_array_equal(mapping.cpu().numpy(), om)
     np.testing.assert_array_equal(nlist.cpu().numpy(), onl)
     np.testing.assert_array_equal(ovf.cpu().numpy(), oovf)
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_embedding_row_stream_many_windows(precision):
+    """The embedding kernels on 884,736 atoms: the env-mat rows, section
+    counts and types of the 110,592-atom water box tiled eight times, so each
+    wavefront's atom range spans several 64-atom windows (one at 110,592
+    atoms).  Each copy's G^T R partials equal the 110,592-atom launch's to the
+    fp32 summation order (rows fall in other tiles); dE/d em equal bit for bit
+    (a row's adjoint does not depend on the tile it shares with other atoms)."""
+    from dpmi import embed
+    from dpmi.model import SeAEnergyModel
+    from dpmi.stat import gpu_model_params
+
+    spec = systems.WATER_SPEC
+    _say(f"{precision} embedding row stream, water110k rows tiled 8x")
+    params = gpu_model_params(spec, systems.water192(0))
+    m = SeAEnergyModel.from_params(spec, params, device=DEV, precision=precision)
+    s = systems.water110k()
+    c = torch.tensor(s.coord, device=DEV)
+    at = torch.tensor(s.atype, device=DEV).to(torch.int32)
+    ext_c, ext_t, mapping, nlist, _, _, _ = m.neighbor_list(c, at, s.box)
+    em = m.env_mat(nlist, ext_c, ext_t)
+    counts, atype, pad_em = m._pad_args(nlist, at)
+    reps = 8
+    em8 = em.repeat(reps, 1, 1).contiguous()
+    pad8 = (counts.repeat(reps, 1).contiguous(), atype.repeat(reps).contiguous(), pad_em)
+    g1 = m.fused.forward_part(em, (counts, atype, pad_em))
+    g8 = m.fused.forward_part(em8, pad8)
+    gen = torch.Generator(device="cpu").manual_seed(11)
+    dgr = (torch.randn(s.nloc, 128, 4, generator=gen) * 0.01).to(DEV)
+    q1, q2, q3 = embed.internal_maps(m.fused.n1)
+    mask = torch.zeros(128, dtype=torch.bool)
+    mask[torch.as_tensor(q3)] = True
+    dgr[:, ~mask.to(DEV), :] = 0
+    nd1 = m.fused.backward_dgr(em, dgr, counts)
+    nd8 = m.fused.backward_dgr(em8, dgr.repeat(reps, 1, 1).contiguous(), pad8[0])
+    torch.cuda.synchronize()
+    scale = g1.abs().max().item()
+    gerr = max((g8[:, k * s.nloc:(k + 1) * s.nloc] - g1).abs().max().item() for k in range(reps)) / scale
+    same = all(torch.equal(nd8[k * s.nloc:(k + 1) * s.nloc], nd1) for k in range(reps))
+    print(f"{precision}: G^T R max rel difference over the copies {gerr:.2e}; dE/d em bitwise equal: {same}",
+          flush=True)
+    assert torch.isfinite(g8).all() and torch.isfinite(nd8).all()
+    assert gerr <= 1e-5
+    assert same
