@@ -44,10 +44,19 @@ inline int make_sec(SecParams* sp, const int* sel_host, int ntypes) {
   return DPMI_OK;
 }
 
+// Type section of slot s: the number of section starts at or below s (the
+// loop index is wave-uniform, so the section bounds are scalar loads; a
+// per-lane index into the kernel arguments would be a memory round trip).
 __device__ inline int slot_type(const SecParams& sp, int s) {
   int t = 0;
-  while (t + 1 < sp.ntypes && s >= sp.sec[t + 1]) ++t;
+  for (int k = 1; k < sp.ntypes; ++k) t += (int)(s >= sp.sec[k]);
   return t;
+}
+
+// lim[t] = end of section t, written from uniform indices
+__device__ inline void section_ends(const SecParams& sp, int* lim) {
+  if (lane_id() == 0)
+    for (int t = 0; t < sp.ntypes; ++t) lim[t] = sp.sec[t + 1];
 }
 
 // spline5_switch, source/lib/include/switcher.h:60-85 (rmin, rmax are float).
@@ -148,7 +157,7 @@ __device__ inline void env_one_fast(const float rr[3], float rmin, float inv_ran
 // lim must hold kMaxTypes ints for this wave.
 __device__ inline void section_limits(const int* __restrict__ nl, const SecParams& sp, int* lim) {
   const int lane = lane_id();
-  for (int t = lane; t < sp.ntypes; t += kWave) lim[t] = sp.sec[t + 1];
+  section_ends(sp, lim);
   __syncthreads();
   for (int s = lane; s < sp.nnei; s += kWave)
     if (nl[s] < 0) atomicMin(&lim[slot_type(sp, s)], s);
@@ -158,7 +167,7 @@ __device__ inline void section_limits(const int* __restrict__ nl, const SecParam
 // The same per wavefront (lim is the wavefront's own LDS row).
 __device__ inline void section_limits_wave(const int* __restrict__ nl, const SecParams& sp, int* lim) {
   const int lane = lane_id();
-  for (int t = lane; t < sp.ntypes; t += kWave) lim[t] = sp.sec[t + 1];
+  section_ends(sp, lim);
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   for (int s = lane; s < sp.nnei; s += kWave)
@@ -228,7 +237,7 @@ __global__ void __launch_bounds__(256)
     int jv[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) jv[k] = lane + kWave * k < nnei ? nl[lane + kWave * k] : 0;
-    for (int t = lane; t < sp.ntypes; t += kWave) lim_all[w][t] = sp.sec[t + 1];
+    section_ends(sp, lim_all[w]);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -701,7 +710,7 @@ __global__ void __launch_bounds__(256)
         }
         const int i = R.i, ti = R.ti;
         // section limits (first -1 of each type section) from the registers
-        for (int t = lane; t < sp.ntypes; t += kWave) lim_all[w][t] = sp.sec[t + 1];
+        section_ends(sp, lim_all[w]);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
 #pragma unroll

@@ -47,6 +47,22 @@ struct SelParams {
   int sec[kMaxTypes + 1];
 };
 
+// sel and the section starts, copied to LDS once per workgroup: indexed by a
+// per-lane type, the kernel-argument arrays would be read from memory by
+// every lane (a round trip per key)
+struct SelLds {
+  int sel[kMaxTypes];
+  int sec[kMaxTypes + 1];
+};
+
+__device__ inline void stage_sel(const SelParams& sp, SelLds* sl) {
+  if (threadIdx.x == 0)
+    for (int t = 0; t < sp.ntypes; ++t) {
+      sl->sel[t] = sp.sel[t];
+      sl->sec[t] = sp.sec[t];
+    }
+}
+
 // ---------------------------------------------------------------- host geometry
 // Same formulas as the reference region code (source/lib/src/region.cc:35-122,
 // SimulationRegion_Impl.h:344-470); x86-64 host code has no FMA, so these
@@ -397,6 +413,9 @@ __global__ void __launch_bounds__(64) k_search_format(
     char* __restrict__ ws, const int* __restrict__ atoms, int16_t* __restrict__ cand_slot, SelParams sp,
     float rcut, const int* __restrict__ count_dev) {
   extern __shared__ unsigned char smem_raw[];
+  __shared__ SelLds s_sl;
+  stage_sel(sp, &s_sl);
+  __syncthreads();
   uint64_t* keys = reinterpret_cast<uint64_t*>(smem_raw);
   int* row = reinterpret_cast<int*>(keys + CAP);
   const GridParams p = *reinterpret_cast<const GridParams*>(ws);
@@ -495,8 +514,10 @@ __global__ void __launch_bounds__(64) k_search_format(
     uint64_t rel = kk - tbase;
     int rank = 0;
     for (int m = 0; m < n; ++m) rank += (keys[m] - tbase) < rel;
-    if (rank < sp.sel[t])
-      row[sp.sec[t] + rank] = (int)(kk & 0xFFFFFFull);
+    const int sel_t = s_sl.sel[t], sec_t = s_sl.sec[t];
+    
+    if (rank < sel_t)
+      row[sec_t + rank] = (int)(kk & 0xFFFFFFull);
     else
       ovf = max(ovf, t);
   }
@@ -554,18 +575,27 @@ __device__ inline double screen_margin(double xmax, float rcut) {
   return 2.0 * (3.5 * (double)rcut * dl + 3.0 * dl * dl + 3.0 * url);
 }
 
+// wavefronts per workgroup of the cell-wise search: eight share the candidate
+// table of the first pass (two workgroups per CU), four the larger table of
+// the second
+constexpr int search_waves(bool big) { return big ? 4 : 8; }
+
 template <int CAP, bool kBigPass>
-__global__ void __launch_bounds__(256) k_search_cells(
+__global__ void __launch_bounds__(64 * search_waves(kBigPass)) k_search_cells(
     int* __restrict__ nlist, int* __restrict__ overflow, int* __restrict__ nlist_mapped,
     int16_t* __restrict__ cand_slot, const double* __restrict__ ext_c, const int* __restrict__ ext_t,
     const int* __restrict__ mapping, char* __restrict__ ws, SelParams sp, float rcut) {
   extern __shared__ unsigned char smem_raw[];
+  __shared__ SelLds s_sl;
+  stage_sel(sp, &s_sl);
+  __syncthreads();
   // candidate g: float coordinates and (type code << 24 | extended index)
   float4* cand = reinterpret_cast<float4*>(smem_raw);
   uint64_t* keys_all = reinterpret_cast<uint64_t*>(cand + CAP);
-  int* rows_all = reinterpret_cast<int*>(keys_all + 4 * (kKeyCap + 8));
-  int16_t* kslot_all = reinterpret_cast<int16_t*>(rows_all + 4 * sp.nnei);
-  int16_t* rslot_all = kslot_all + 4 * (kKeyCap + 8);
+  constexpr int kW = search_waves(kBigPass);
+  int* rows_all = reinterpret_cast<int*>(keys_all + kW * (kKeyCap + 8));
+  int16_t* kslot_all = reinterpret_cast<int16_t*>(rows_all + kW * sp.nnei);
+  int16_t* rslot_all = kslot_all + kW * (kKeyCap + 8);
   __shared__ int s_base[kMaxNb], s_local[kMaxNb], s_pref[kMaxNb + 1];
 
   const GridParams p = *reinterpret_cast<const GridParams*>(ws);
@@ -602,7 +632,7 @@ __global__ void __launch_bounds__(256) k_search_cells(
     const int ci = wi % ncells, part = wi / ncells;
     const int cc = kBigPass ? bigcells[ci] : ci;
     const int a0 = cell_start[cc], na = cell_start[cc + 1] - a0;
-    if (na <= 4 * part) continue;  // no atom for this part (uniform in the workgroup)
+    if (na <= kW * part) continue;  // no atom for this part (uniform in the workgroup)
     const int T = nb_table(p, cell_start, ghost_off, cc, s_base, s_local, s_pref);
     if (T > CAP) {
       if (part == 0) {
@@ -628,7 +658,7 @@ __global__ void __launch_bounds__(256) k_search_cells(
     // cell order: an atom's index and float coordinates come from LDS (its
     // double coordinates only where the float test cannot decide)
     const int ctr = s_pref[nnb / 2];
-    for (int ai = 4 * part + w; ai < na; ai += 4 * nsplit) {
+    for (int ai = kW * part + w; ai < na; ai += kW * nsplit) {
       const float4 me = cand[ctr + ai];
       const int i = __float_as_int(me.w) & 0xFFFFFF;
       const float fi0 = me.x, fi1 = me.y, fi2 = me.z;
@@ -731,18 +761,22 @@ __global__ void __launch_bounds__(256) k_search_cells(
           ranked = true;
           if (ka < n) {
             const int t = (int)(a32 >> 31), r = ra - 1;
-            if (r < sp.sel[t]) {
-              row[sp.sec[t] + r] = kj[ka];
-              rslot[sp.sec[t] + r] = kslot[ka];
+            const int sel_t = s_sl.sel[t], sec_t = s_sl.sec[t];
+            
+            if (r < sel_t) {
+              row[sec_t + r] = kj[ka];
+              rslot[sec_t + r] = kslot[ka];
             } else {
               ovf = max(ovf, t);
             }
           }
           if (kb < n) {
             const int t = (int)(b32 >> 31), r = rb - 1;
-            if (r < sp.sel[t]) {
-              row[sp.sec[t] + r] = kj[kb];
-              rslot[sp.sec[t] + r] = kslot[kb];
+            const int sel_t = s_sl.sel[t], sec_t = s_sl.sec[t];
+            
+            if (r < sel_t) {
+              row[sec_t + r] = kj[kb];
+              rslot[sec_t + r] = kslot[kb];
             } else {
               ovf = max(ovf, t);
             }
@@ -778,18 +812,22 @@ __global__ void __launch_bounds__(256) k_search_cells(
         }
         if (ka < n) {
           const int t = (int)(kka >> 56);
-          if (ra < sp.sel[t]) {
-            row[sp.sec[t] + ra] = (int)(kka & 0xFFFFFFull);
-            rslot[sp.sec[t] + ra] = kslot[ka];
+          const int sel_t = s_sl.sel[t], sec_t = s_sl.sec[t];
+          
+          if (ra < sel_t) {
+            row[sec_t + ra] = (int)(kka & 0xFFFFFFull);
+            rslot[sec_t + ra] = kslot[ka];
           } else {
             ovf = max(ovf, t);
           }
         }
         if (kb < n) {
           const int t = (int)(kkb >> 56);
-          if (rb < sp.sel[t]) {
-            row[sp.sec[t] + rb] = (int)(kkb & 0xFFFFFFull);
-            rslot[sp.sec[t] + rb] = kslot[kb];
+          const int sel_t = s_sl.sel[t], sec_t = s_sl.sec[t];
+          
+          if (rb < sel_t) {
+            row[sec_t + rb] = (int)(kkb & 0xFFFFFFull);
+            rslot[sec_t + rb] = kslot[kb];
           } else {
             ovf = max(ovf, t);
           }
@@ -832,6 +870,9 @@ __global__ void __launch_bounds__(64) k_format_external(
     const int* __restrict__ cand, const int* __restrict__ numneigh, int64_t ncand, const double* __restrict__ ext_c,
     const int* __restrict__ ext_t, const int* __restrict__ mapping, int nall, SelParams sp, float rcut) {
   extern __shared__ unsigned char smem_raw[];
+  __shared__ SelLds s_sl;
+  stage_sel(sp, &s_sl);
+  __syncthreads();
   uint64_t* keys = reinterpret_cast<uint64_t*>(smem_raw);
   int* row = reinterpret_cast<int*>(keys + CAP);
   const int i = blockIdx.x;
@@ -880,8 +921,10 @@ __global__ void __launch_bounds__(64) k_format_external(
     const uint64_t rel = kk - tbase;
     int rank = 0;
     for (int m2 = 0; m2 < n; ++m2) rank += (keys[m2] - tbase) < rel;
-    if (rank < sp.sel[t])
-      row[sp.sec[t] + rank] = (int)(kk & 0xFFFFFFull);
+    const int sel_t = s_sl.sel[t], sec_t = s_sl.sec[t];
+    
+    if (rank < sel_t)
+      row[sec_t + rank] = (int)(kk & 0xFFFFFFull);
     else
       ovf = max(ovf, t);
   }
@@ -899,12 +942,15 @@ __global__ void __launch_bounds__(64) k_format_external(
 }  // namespace
 
 // Dynamic LDS of the cell-wise search with a table of cap candidates.
-size_t search_cells_lds(int cap, int nnei) {
+size_t search_cells_lds(int cap, int nnei, int waves) {
   return (size_t)cap * sizeof(float4) +
-         4 * ((size_t)(kKeyCap + 8) * (sizeof(uint64_t) + sizeof(int16_t)) + (sizeof(int) + sizeof(int16_t)) * nnei);
+         waves * ((size_t)(kKeyCap + 8) * (sizeof(uint64_t) + sizeof(int16_t)) + (sizeof(int) + sizeof(int16_t)) * nnei);
 }
 
-bool search_uses_cells(int nnei) { return search_cells_lds(kCandCapBig, nnei) <= kMaxCellsLds; }
+bool search_uses_cells(int nnei) {
+  return search_cells_lds(kCandCapBig, nnei, search_waves(true)) <= kMaxCellsLds &&
+         search_cells_lds(kCandCap, nnei, search_waves(false)) <= kMaxCellsLds;
+}
 
 int cell_grid_info(const double* box_host, int64_t nloc, float rcut, int64_t* nlc, int* nsearch) {
   int64_t next;
@@ -1113,7 +1159,8 @@ static int build_nlist_impl(int* nlist, int* overflow, int* nlist_mapped, int16_
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, misc + 1, (int64_t)3, 0);
   hipLaunchKernelGGL(k_ws_validate, dim3(1), dim3(64), 0, st, ws, (int)nloc, wait ? (int)nall : -1,
                      (long long)workspace_bytes);
-  const size_t shm_cells = search_cells_lds(kCandCap, sp.nnei), shm_big = search_cells_lds(kCandCapBig, sp.nnei);
+  const size_t shm_cells = search_cells_lds(kCandCap, sp.nnei, search_waves(false)),
+               shm_big = search_cells_lds(kCandCapBig, sp.nnei, search_waves(true));
   if (search_uses_cells(sp.nnei)) {
     // first pass over every cell; cells whose neighborhood exceeds its table
     // are listed for the second pass (larger table, fewer workgroups per CU)
@@ -1122,7 +1169,8 @@ static int build_nlist_impl(int* nlist, int* overflow, int* nlist_mapped, int16_
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_cells));
     DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_search_cells<kCandCapBig, true>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_big));
-    hipLaunchKernelGGL((k_search_cells<kCandCap, false>), dim3((unsigned)std::min<int64_t>(nloc, 2048)), dim3(256),
+    hipLaunchKernelGGL((k_search_cells<kCandCap, false>), dim3((unsigned)std::min<int64_t>(nloc, 2048)),
+                       dim3(64 * search_waves(false)),
                        shm_cells, st, nlist, overflow, nlist_mapped, cand_slot, ext_coord, ext_type, mapping, ws, sp,
                        rcut);
     hipLaunchKernelGGL((k_search_cells<kCandCapBig, true>), dim3((unsigned)std::min<int64_t>(nloc, 2048)), dim3(256),
