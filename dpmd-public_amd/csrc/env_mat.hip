@@ -154,17 +154,7 @@ __device__ inline void env_one_fast(const float rr[3], float rmin, float inv_ran
 
 // First -1 in each type section of the wave's row: slots at or after it are
 // empty (env_mat_a_cpu breaks at the first negative entry of a section).
-// lim must hold kMaxTypes ints for this wave.
-__device__ inline void section_limits(const int* __restrict__ nl, const SecParams& sp, int* lim) {
-  const int lane = lane_id();
-  section_ends(sp, lim);
-  __syncthreads();
-  for (int s = lane; s < sp.nnei; s += kWave)
-    if (nl[s] < 0) atomicMin(&lim[slot_type(sp, s)], s);
-  __syncthreads();
-}
-
-// The same per wavefront (lim is the wavefront's own LDS row).
+// lim is the wavefront's own LDS row of kMaxTypes ints.
 __device__ inline void section_limits_wave(const int* __restrict__ nl, const SecParams& sp, int* lim) {
   const int lane = lane_id();
   section_ends(sp, lim);
@@ -237,14 +227,17 @@ __global__ void __launch_bounds__(256)
     int jv[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) jv[k] = lane + kWave * k < nnei ? nl[lane + kWave * k] : 0;
+    // the limits row is the wavefront's own: wave-local ordering suffices
     section_ends(sp, lim_all[w]);
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int s = lane + kWave * k;
       if (s < nnei && jv[k] < 0) atomicMin(&lim_all[w][slot_type(sp, s)], s);
     }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
     if (!active) return;
     bool valid[4];
     CT xj[4][3];
@@ -260,7 +253,7 @@ __global__ void __launch_bounds__(256)
     for (int k = 0; k < 4; ++k)
       if (lane + kWave * k < nnei) slot(lane + kWave * k, jv[k], valid[k], xj[k]);
   } else {
-    section_limits(nl, sp, lim_all[w]);
+    section_limits_wave(nl, sp, lim_all[w]);
     if (!active) return;
     for (int s = lane; s < nnei; s += kWave) {
       const int j = nl[s];
