@@ -626,7 +626,7 @@ __global__ void __launch_bounds__(64 * search_waves(kBigPass)) k_search_cells(
   if (misc[1] & 2) return;  // workspace rejected by k_ws_validate
   // Fewer cells than workgroups (small boxes): nsplit workgroups share each
   // cell, each building the same candidate table and taking every
-  // nsplit-th group of four of its atoms; large boxes keep one per cell.
+  // nsplit-th group of kW of its atoms; large boxes keep one per cell.
   const int nsplit = ncells > 0 ? max(1, min(kMaxCellSplit, (int)gridDim.x / ncells)) : 1;
   for (int wi = blockIdx.x; wi < ncells * nsplit; wi += gridDim.x) {
     const int ci = wi % ncells, part = wi / ncells;
