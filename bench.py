@@ -90,8 +90,9 @@ def _embed_flops_per_row(neuron) -> float:
 def _mfma_flops(name: str, spec, nloc: int, nrows: int) -> float:
     """Per-launch algorithmic FLOPs of the fused MFMA kernels.
     Embedding forward: layer GEMMs + em^T G (2*4*n3 per neighbor row);
-    embedding backward: the recomputed forward, the two transposed GEMMs of
-    the adjoint, dG = em dGR and the direct term G dGR^T (2 * 2*4*n3 per row).
+    embedding backward: the recomputed forward, its tangent in s (the same
+    two GEMMs again), and the contractions G dGR^T and (dG/ds) dGR^T
+    (2 * 2*4*n3 per row).
     ``nrows`` counts the rows the algorithm needs: the listed neighbors when
     the padding rows (all equal within a type section) are folded into one
     closed-form term, else every slot.

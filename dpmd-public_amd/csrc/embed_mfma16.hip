@@ -29,9 +29,9 @@
 // G = idt3 (1 - 2 r3) + [h2, h2], so G^T R = idt3 (em^T 1) - 2 idt3 (em^T r3)
 // + em^T [h2, h2]; r3 and h2 are transposed per 16-row tile by MFMA against
 // an identity fragment and reduced with MFMA against em^T.  The backward pass
-// recomputes the forward per tile and runs the adjoint chain (dG = em dGR,
-// dh2 = dz3 W3^T + skip, dh1 = dz2 W2^T + skip, ds = dz1 . w1) in the same
-// register layout, so it needs no transposes at all.
+// recomputes the forward per tile together with its tangent in the scalar
+// input s (forward-mode: the same fragments, the same register layout) and
+// contracts both with dGR, so it needs no transposes and no adjoint chain.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,10 +49,9 @@ using h8 = H __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxTypes = 16;
-constexpr int kFragFwd = 20;   // 4 (layer 2) + 16 (layer 3)
-constexpr int kFragBwd = 20;   // 16 (dh2) + 4 (dh1)
-constexpr int kConsts = 704;   // k w1, k b1, idt1 [32]; k b2, idt2 [64]; k b3, idt3 [128]; w1 [32];
-                               // idt2 / k [64]; 4 idt3 / k [128]   (k = 2 log2 e)
+constexpr int kFragFwd = 20;   // 4 (layer 2) + 16 (layer 3), per neighbor type
+constexpr int kConsts = 576;   // k w1, k b1, idt1 [32]; k b2, idt2 [64]; k b3, idt3 [128]; idt1 w1 [32];
+                               // idt2 / k [64]   (k = 2 log2 e)
 constexpr int kQ3 = 128;       // internal width of G
 constexpr int kAtoms = 4;      // atoms (wavefronts) per workgroup
 
@@ -251,7 +250,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
   __shared__ __attribute__((aligned(16))) float s_em[kAtoms][16 * 16];  // [row][4 slot + c]
   const int t = blockIdx.y;
   const int w = wave_id(), lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-  const h8<H>* frag_t = frag_all + (int64_t)t * (kFragFwd + kFragBwd) * 64;
+  const h8<H>* frag_t = frag_all + (int64_t)t * kFragFwd * 64;
   for (int k = threadIdx.x; k < kFragFwd * 64; k += blockDim.x) s_frag[k] = frag_t[k];
   for (int k = threadIdx.x; k < kConsts; k += blockDim.x) s_cst[k] = consts[(int64_t)t * kConsts + k];
   const float* cst = s_cst;
@@ -594,64 +593,58 @@ __global__ void __launch_bounds__(256)
   for (int k = lane; k < 4 * kQ3; k += 64) out[k] = so[(k & 3) * kQ3 + (k >> 2)] * inv_nnei;
 }
 
-// Backward: net_deriv[i][sec_t + row][c] = dE/d em, recomputing the forward,
-// over the same row stream as the forward with 16-row tiles that hold rows
-// of at most two atoms (slot = window index % 2).  A workgroup loads its
-// type's weight fragments into LDS once.  Each wavefront stages an atom's
-// dGR when the atom's first row comes up: as per-slot operands of the direct
-// term, and into one of the two halves of a 16-byte H row per feature
-// [q3] = (dGR slot 0 [0..3][q3], dGR slot 1 [0..3][q3]), the A operand of
-// dG = em dGR (the B operand holds em[row][c] in K slot 4 slot(row) + c).
-constexpr int kWavesBwd = 12;  // wavefronts per backward workgroup: three per SIMD
+// Backward: net_deriv[i][sec_t + row][c] = dE/d em, recomputing the forward
+// over the same row stream as the forward, with 16-row tiles that hold rows
+// of at most two atoms (slot = window index % 2).  The embedding net's input
+// is the scalar s = em[row][0], so dE/ds is taken in forward mode: the tile
+// carries the tangents dh1/ds, dh2/ds, dz3/ds beside the recompute (same
+// weight fragments, same lane layout), and with dE/dG[row][q] =
+// sum_c em[row][c] dGR[c][q] (G^T R = em^T G),
+//   dE/ds = sum_c em[row][c] T[c][row],  T[c] = sum_q dGR[c][q] dG[q]/ds,
+//   dE/d em[row][c] = sum_q G[q] dGR[c][q] (+ dE/ds for c = 0).
+// Both contractions run on MFMA with A (M = 4 slot + c, K = feature) built
+// from the atom's dGR when its first row comes up:
+//   G  = idt3 (1 - 2 r3) + [h2, h2]:  K_c + (-2 idt3 dGR) r3 + (dGR[q2] + dGR[q2 + 64]) h2,
+//   dG/ds = idt3 4 r3 (1 - r3) dz3/ds + [h2', h2']:  with z3' = k dz3/ds (the
+//   fragments hold k W3) the same A (-2 idt3 dGR) against r3 (r3 - 1) z3' and
+//   (k / 2) (dGR[q2] + dGR[q2 + 64]) against h2' give (k / 2) T,
+// so the adjoint chain (dG = em dGR, dz3, dh2, dz2, dh1) is never formed.
+// A workgroup loads its type's forward fragments into LDS once.
+constexpr int kWavesBwd = 16;  // wavefronts per backward workgroup: four per SIMD
+constexpr int kDopBwd = 8 * 16;  // A entries per slot: -2 idt3 dGR (4 chunks), dGR[q2] + dGR[q2 + 64] (2), times k / 2 (2)
+constexpr float kK2 = 2.8853900817779268f;  // k = 2 log2(e)
 
 template <typename H, bool kSkip>
-__global__ void __launch_bounds__(64 * kWavesBwd) __attribute__((amdgpu_waves_per_eu(3, 3)))
+__global__ void __launch_bounds__(64 * kWavesBwd) __attribute__((amdgpu_waves_per_eu(4, 4)))
     k_embed_bwd(float* __restrict__ net_deriv, const float* __restrict__ em, const float* __restrict__ dgr,
                 const h8<H>* __restrict__ frag_all, const float* __restrict__ consts, int nloc, int per_wave,
                 EmbedParams p) {
   constexpr int kS = 2;  // atoms per tile
-  __shared__ h8<H> s_frag[(kFragFwd + kFragBwd) * 64];
-  __shared__ __attribute__((aligned(16))) float s_dg[kWavesBwd][kQ3 * 4];
-  __shared__ h8<H> s_dgb[kWavesBwd][kQ3];
-  __shared__ h8<H> s_dgbs[kWavesBwd][kQ3];  // the same times 4 idt3 / k (the dz3 factor)
-  __shared__ h8<H> s_dop[kWavesBwd][kS][6 * 16];
+  __shared__ h8<H> s_frag[kFragFwd * 64];
+  __shared__ h8<H> s_dop[kWavesBwd][kS][kDopBwd];
   __shared__ __attribute__((aligned(16))) float s_kin[kWavesBwd][kS][4];
   __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
   const int t = blockIdx.y;
   const int w = wave_id(), lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-  const h8<H>* frag_t = frag_all + (int64_t)t * (kFragFwd + kFragBwd) * 64;
-  for (int k = threadIdx.x; k < (kFragFwd + kFragBwd) * 64; k += blockDim.x) s_frag[k] = frag_t[k];
-  __syncthreads();
-  const h8<H>* fb = s_frag + kFragFwd * 64;
+  const h8<H>* frag_t = frag_all + (int64_t)t * kFragFwd * 64;
+  for (int k = threadIdx.x; k < kFragFwd * 64; k += blockDim.x) s_frag[k] = frag_t[k];
   for (int k = threadIdx.x; k < kConsts; k += blockDim.x) s_cst[k] = consts[(int64_t)t * kConsts + k];
   __syncthreads();
   const float* cst = s_cst;
   const int sel = p.sec[t + 1] - p.sec[t];
-  float* sdg = s_dg[w];
-  h8<H>* sdgb = s_dgb[w];
-  h8<H>* sdgbs = s_dgbs[w];
-  // a slot not staged yet still meets the zero K slots of the other slot's
-  // rows: its half of the dG operands must be finite
-  for (int q = lane; q < kQ3; q += 64)
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      sdgb[q][u] = (H)0.f;
-      sdgbs[q][u] = (H)0.f;
-    }
   const float* em_t = em + (int64_t)p.sec[t] * 4;
   float* nd_t = net_deriv + (int64_t)p.sec[t] * 4;
   // fp16 operands: everything below is linear in dGR, so each atom's dGR is
   // staged times 2^-e (2^e the power of two at or below its largest
-  // magnitude) and dE/d em multiplied by 2^e at the end, both exact; the dz
+  // magnitude) and dE/d em multiplied by 2^e at the end, both exact; the A
   // operands then sit in fp16's normal range instead of underflowing
   float unscale[kS] = {1.f, 1.f};
   // Stage atom ai (listed rows cnt) into slot s: the padding slots after its
-  // listed rows get dE/d em = 0 (no force term reads them); then the
-  // operands of the direct term dem[c] = sum_q3 G[q3] dGR[c][q3] with
-  // G = idt3 (1 - 2 r3) + [h2, h2]:  dem = K_c + sum_q3 r3 (-2 idt3 dGR)
-  // + sum_q2 h2 (dGR[q2] + dGR[q2 + 64]), A (M = 4 s + c, K = feature slot):
-  // sdop[s][f][4 g + c], f = 0..3 the r3 chunks, 4..5 the h2 chunks; and
-  // K_c = sum_q3 idt3 dGR[c].
+  // listed rows get dE/d em = 0 (no force term reads them); lane l holds
+  // dGR[.][q3 = l] and dGR[.][q3 = l + 64] and writes the A entries of those
+  // features (entry 16 chunk + 4 gg + c, element u: feature 32 chunk +
+  // phi(gg, u)); K_c = sum_q3 idt3 dGR[c] goes to s_kin.  Rows of the other
+  // slot (and of a slot not staged yet) only reach M rows that are not written.
   auto stage = [&](int ai, int s, int cnt) {
     if (p.counts != nullptr)
       for (int row = cnt + lane; row < sel; row += 64)
@@ -677,47 +670,29 @@ __global__ void __launch_bounds__(64 * kWavesBwd) __attribute__((amdgpu_waves_pe
       x0 = make_float4(x0.x * sc, x0.y * sc, x0.z * sc, x0.w * sc);
       x1 = make_float4(x1.x * sc, x1.y * sc, x1.z * sc, x1.w * sc);
     }
-    // the previous tile's reads of this wavefront's staging are complete
+    const float i0 = cst[352 + l], i1 = cst[352 + 64 + l];
+    const float xa[4] = {x0.x, x0.y, x0.z, x0.w}, xb[4] = {x1.x, x1.y, x1.z, x1.w};
+    float kp[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) kp[c] = fmaf(i0, xa[c], i1 * xb[c]);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) kp[c] += __shfl_xor(kp[c], off, 64);
+    // the previous tiles' reads of this slot are complete
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    reinterpret_cast<float4*>(sdg)[l] = x0;
-    reinterpret_cast<float4*>(sdg)[l + 64] = x1;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    for (int q = lane; q < kQ3; q += 64) {
-      const float4 x = *reinterpret_cast<const float4*>(sdg + 4 * q);
-      const float c3 = cst[576 + q];
-      h4<H> v, vs;
-      v[0] = (H)x.x;
-      v[1] = (H)x.y;
-      v[2] = (H)x.z;
-      v[3] = (H)x.w;
-      vs[0] = (H)(x.x * c3);
-      vs[1] = (H)(x.y * c3);
-      vs[2] = (H)(x.z * c3);
-      vs[3] = (H)(x.w * c3);
-      reinterpret_cast<h4<H>*>(sdgb + q)[s] = v;
-      reinterpret_cast<h4<H>*>(sdgbs + q)[s] = vs;
+    H* dop = reinterpret_cast<H*>(s_dop[w][s]);
+    const int at = 32 * ((l >> 2) & 3) + 4 * ((l >> 4) & 1) + (l & 3) + 128 * (l >> 5);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const float a2 = xa[c] + xb[c];
+      dop[at + 8 * c] = (H)(-2.f * i0 * xa[c]);
+      dop[at + 8 * c + 256] = (H)(-2.f * i1 * xb[c]);
+      dop[at + 8 * c + 512] = (H)a2;
+      dop[at + 8 * c + 768] = (H)(0.5f * kK2 * a2);
     }
-    h8<H>* sdop = s_dop[w][s];
-    for (int k = lane; k < 96; k += 64) {
-      const int fi = k >> 4, gg = (k >> 2) & 3, cc = k & 3;
-      h8<H> v;
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int q = 32 * (fi & 3) + phi(gg, u);
-        v[u] = fi < 4 ? (H)(-2.f * cst[352 + q] * sdg[q * 4 + cc])
-                      : (H)(sdg[q * 4 + cc] + sdg[(q + 64) * 4 + cc]);
-      }
-      sdop[k] = v;
-    }
-    // K_c: lane (q0 = lane / 4, c = lane % 4) sums q3 = q0 + 16 j
-    float kpart = 0.f;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) kpart += cst[352 + (lane >> 2) + 16 * j] * sdg[lane + 64 * j];
-#pragma unroll
-    for (int off = 4; off < 64; off <<= 1) kpart += __shfl_xor(kpart, off, 64);
-    if (lane < 4) s_kin[w][s][lane] = kpart;
+    if (lane == 0) *reinterpret_cast<float4*>(s_kin[w][s]) = make_float4(kp[0], kp[1], kp[2], kp[3]);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   };
@@ -762,104 +737,89 @@ __global__ void __launch_bounds__(64 * kWavesBwd) __attribute__((amdgpu_waves_pe
         int row;
         const int k = chunk_atom(c, k0, pos, &row);
         const int sl = pos < c.p1 ? (k & 1) : -1;
-        // weights and constants are re-read from LDS every tile instead of being
-        // held in registers across the loop: that keeps the kernel at two
-        // wavefronts per SIMD
+        // weights and constants are re-read from LDS every tile instead of
+        // being held in registers across the loop
         asm volatile("" ::: "memory");
-        TileFwd<H> f;
-        forward_tile<H, kSkip>(ev.x, s_frag, cst, g, lane, f);
-        // direct term -> D(M = 4 slot + c, N = rows); lane groups 2, 3 unused
-        const h8<H>* dop = s_dop[w][(r >> 2) & 1] + 4 * g + (r & 3);
-        f32x4 dem = f32x4{0.f, 0.f, 0.f, 0.f};
-        if (g < kS) dem = *reinterpret_cast<const f32x4*>(s_kin[w][g & 1]);
+        // layer 1 and its tangent: h1' = idt1 w1 (1 - t1^2)
+        h8<H> b1, b1t;
+        float h1[8], h1t[8];
 #pragma unroll
-        for (int kc = 0; kc < 4; ++kc) dem = mfma(dop[16 * kc], r3_frag(f, kc), dem);
-#pragma unroll
-        for (int kc = 0; kc < 2; ++kc) dem = mfma(dop[16 * (4 + kc)], f.b2[kc], dem);
-        // em as B (N = rows, K = 4 slot + c): lane group 0 only, the row's own slot
-        h8<H> b_em;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) b_em[u] = (H)0.f;
-        if (g == 0) {
-          const bool s0 = sl == 0, s1 = sl == 1;
-          b_em[0] = (H)(s0 ? ev.x : 0.f);
-          b_em[1] = (H)(s0 ? ev.y : 0.f);
-          b_em[2] = (H)(s0 ? ev.z : 0.f);
-          b_em[3] = (H)(s0 ? ev.w : 0.f);
-          b_em[4] = (H)(s1 ? ev.x : 0.f);
-          b_em[5] = (H)(s1 ? ev.y : 0.f);
-          b_em[6] = (H)(s1 ? ev.z : 0.f);
-          b_em[7] = (H)(s1 ? ev.w : 0.f);
-        }
-        // dG = em dGR -> D(M = q3, N = rows), A = row q3 of sdgb (lane groups
-        // 1..3 meet the zero K slots of b_em);
-        // dz3 = dG idt3 (1 - t3^2) = (dG 4 idt3 / k) r3 (1 - r3) / k, the first
-        // factor from the scaled operand sdgbs; skip = dG[q] + dG[q + 64]
-        f32x4 skip[4];
-        h8<H> b_dz3[4];
-        // keep the sdgb reads below the recompute (hoisted, they would hold 32
-        // registers across it)
-        asm volatile("" ::: "memory");
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-          const h8<H> a_lo = sdgb[16 * mt + r], a_hi = sdgb[16 * (mt + 4) + r];
-          const h8<H> as_lo = sdgbs[16 * mt + r], as_hi = sdgbs[16 * (mt + 4) + r];
-          const f32x4 d_lo = mfma(as_lo, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
-          const f32x4 d_hi = mfma(as_hi, b_em, f32x4{0.f, 0.f, 0.f, 0.f});
-          skip[mt] = mfma(a_hi, b_em, mfma(a_lo, b_em, f32x4{0.f, 0.f, 0.f, 0.f}));
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            if (kSkip && (mt & 1) && v == 3) {  // empty slot: no weights read it
-              b_dz3[mt >> 1][(mt & 1) * 4 + v] = (H)0.f;
-              b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (H)0.f;
-              continue;
-            }
-            const float rl = f.r3[mt * 4 + v], rh = f.r3[(mt + 4) * 4 + v];
-            b_dz3[mt >> 1][(mt & 1) * 4 + v] = (H)(d_lo[v] * fmaf(-rl, rl, rl));
-            b_dz3[(mt + 4) >> 1][(mt & 1) * 4 + v] = (H)(d_hi[v] * fmaf(-rh, rh, rh));
+        for (int u = 0; u < 8; ++u) {
+          if (kSkip && u == 7) {  // empty slot in every lane group
+            h1[u] = h1t[u] = 0.f;
+            b1[u] = b1t[u] = (H)0.f;
+            continue;
           }
+          const int q = phi(g, u);
+          const float tt = tanh_scaled(ev.x * cst[q] + cst[32 + q]);
+          h1[u] = tt * cst[64 + q];
+          h1t[u] = fmaf(-tt, tt, 1.f) * cst[480 + q];
+          b1[u] = (H)h1[u];
+          b1t[u] = (H)h1t[u];
         }
-        // dh2 = dz3 W3^T + skip;  dz2 = dh2 * idt2 * (1 - t2^2)
-        float dh2[16];
-        h8<H> b_dz2[2];
+        // layer 2: z2 (k scaled, bias in the accumulator) and z2' = k W2^T h1';
+        // h2' = (idt2 / k) (1 - t2^2) z2' + h1'
+        h8<H> b2[2], b2t[2];
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
-          f32x4 d = skip[mt];
-#pragma unroll
-          for (int kc = 0; kc < 4; ++kc) d = mfma(fb[(mt * 4 + kc) * 64 + lane], b_dz3[kc], d);
+          const h8<H> fr = s_frag[mt * 64 + lane];
+          const f32x4 z = mfma(fr, b1, *reinterpret_cast<const f32x4*>(cst + 96 + 16 * mt + 4 * g));
+          const f32x4 zt = mfma(fr, b1t, f32x4{0.f, 0.f, 0.f, 0.f});
           const int q = 16 * mt + 4 * g;
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            float x = d[v];
-            dh2[mt * 4 + v] = x;
+            const int u = (mt & 1) * 4 + v;
             if (kSkip && (mt & 1) && v == 3) {
-              b_dz2[mt >> 1][(mt & 1) * 4 + v] = (H)0.f;
+              b2[mt >> 1][u] = b2t[mt >> 1][u] = (H)0.f;
               continue;
             }
-            float tt = f.t2[mt * 4 + v];
-            b_dz2[mt >> 1][(mt & 1) * 4 + v] = (H)(x * cst[512 + q + v] * (1.f - tt * tt));
+            const float tt = tanh_scaled(z[v]);
+            b2[mt >> 1][u] = (H)(tt * cst[160 + q + v] + h1[u]);
+            b2t[mt >> 1][u] = (H)fmaf(fmaf(-tt, tt, 1.f) * cst[512 + q + v], zt[v], h1t[u]);
           }
         }
-        // dh1 = dz2 W2^T + dh2[q1] + dh2[q1 + 32];  ds = sum_f1 dh1 * idt1 * (1 - t1^2) * w1
-        float ds = 0.f;
+        // layer 3 by 32-feature chunks, each contracted at once: dem += A r3,
+        // tg += A r3 (r3 - 1) z3' (M = 4 slot + c, N = rows; lane groups 2, 3
+        // hold rows that are not written)
+        const h8<H>* dop = s_dop[w][(r >> 2) & 1] + 4 * g + (r & 3);
+        f32x4 dem = f32x4{0.f, 0.f, 0.f, 0.f}, tg = f32x4{0.f, 0.f, 0.f, 0.f};
+        if (g < kS) dem = *reinterpret_cast<const f32x4*>(s_kin[w][g]);
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) {
-          f32x4 d = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int kc = 0; kc < 4; ++kc) {
+          // fragments read per chunk, not all ahead of the products
+          asm volatile("" ::: "memory");
+          h8<H> br, bt;
 #pragma unroll
-          for (int kc = 0; kc < 2; ++kc) d = mfma(fb[(16 + mt * 2 + kc) * 64 + lane], b_dz2[kc], d);
+          for (int j = 0; j < 2; ++j) {
+            const int mt = 2 * kc + j;
+            const h8<H> f0 = s_frag[(4 + 2 * mt) * 64 + lane], f1 = s_frag[(5 + 2 * mt) * 64 + lane];
+            f32x4 z = mfma(f0, b2[0], *reinterpret_cast<const f32x4*>(cst + 224 + 16 * mt + 4 * g));
+            z = mfma(f1, b2[1], z);
+            f32x4 zt = mfma(f0, b2t[0], f32x4{0.f, 0.f, 0.f, 0.f});
+            zt = mfma(f1, b2t[1], zt);
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int u = mt * 4 + v, q = phi(g, u);
-            if (kSkip && u == 7) continue;
-            float x = d[v] + dh2[mt * 4 + v] + dh2[(mt + 2) * 4 + v];
-            float tt = f.t1[u];
-            ds += x * fmaf(-tt, tt, 1.f) * cst[480 + q];  // cst[480 + q] = idt1 w1
+            for (int v = 0; v < 4; ++v) {
+              if (kSkip && j == 1 && v == 3) {
+                br[4 * j + v] = bt[4 * j + v] = (H)0.f;
+                continue;
+              }
+              const float r3 = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(z[v]));
+              br[4 * j + v] = (H)r3;
+              bt[4 * j + v] = (H)(fmaf(r3, r3, -r3) * zt[v]);
+            }
           }
+          const h8<H> a = dop[16 * kc];
+          dem = mfma(a, br, dem);
+          tg = mfma(a, bt, tg);
         }
-        ds += __shfl_xor(ds, 16, 64);
-        ds += __shfl_xor(ds, 32, 64);
+#pragma unroll
+        for (int kc = 0; kc < 2; ++kc) {
+          dem = mfma(dop[16 * (4 + kc)], b2[kc], dem);
+          tg = mfma(dop[16 * (6 + kc)], b2t[kc], tg);
+        }
         if (g == sl) {
           const float us = sl == 0 ? unscale[0] : unscale[1];
+          const float ds = (2.f / kK2) * (ev.x * tg[0] + ev.y * tg[1] + ev.z * tg[2] + ev.w * tg[3]);
           *reinterpret_cast<float4*>(nd_t + ((int64_t)(wa + k) * p.nnei + row) * 4) =
               make_float4((dem[0] + ds) * us, dem[1] * us, dem[2] * us, dem[3] * us);
         }
@@ -942,7 +902,7 @@ int embed_bwd(float* net_deriv, const float* em, const float* dgr, const void* f
   p.n1 = n1;
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
   if (nloc == 0) return DPMI_OK;
-  // one workgroup of twelve wavefronts per CU (127 KB of LDS), up to 1,024
+  // one workgroup of sixteen wavefronts per CU (87 KB of LDS), up to 1,024
   // per type; each wavefront streams a contiguous range of per_wave atoms
   dim3 grid((unsigned)std::min<int64_t>(ceil_div(nloc, kWavesBwd), 1024), (unsigned)ntypes);
   const int per_wave = (int)ceil_div(nloc, (int64_t)grid.x * kWavesBwd);

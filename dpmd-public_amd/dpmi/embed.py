@@ -31,8 +31,8 @@ import torch
 from . import ops
 from ._lib import ptr
 
-N_FRAG = 40  # 20 forward + 20 backward fragments per type
-N_CONST = 704
+N_FRAG = 20  # layer-2 (4) and layer-3 (16) fragments per type
+N_CONST = 576
 # tanh(x) = 1 - 2 / (2^(K2 x) + 1): the forward weights and biases are stored
 # multiplied by K2 so the kernels' MFMA results feed exp2 directly
 K2 = 2.0 / np.log(2.0)
@@ -85,13 +85,15 @@ DESC_KIND = {torch.float32: 0, torch.bfloat16: 1, torch.float16: 2}
 
 def pack_embedding(emb_layers: list, dtype=torch.bfloat16) -> tuple[np.ndarray, np.ndarray, int]:
     """emb_layers[t] = the three layer dicts of neighbor type t.
-    Returns (frags uint16 [ntypes, 40, 64, 8] holding the bits of the 16-bit
+    Returns (frags uint16 [ntypes, 20, 64, 8] holding the bits of the 16-bit
     operand dtype (bfloat16 or float16),
-    consts float32 [ntypes, 704], n1).  consts: K2 w1 [0:32], K2 b1 [32:64],
+    consts float32 [ntypes, 576], n1).  consts: K2 w1 [0:32], K2 b1 [32:64],
     idt1 [64:96], K2 b2 [96:160], idt2 [160:224], K2 b3 [224:352],
-    idt3 [352:480], idt1 w1 [480:512], idt2 / K2 [512:576], 4 idt3 / K2 [576:704]
-    (the backward's dz3 = dG idt3 (1 - t3^2) is formed as dG (4 idt3) r3 (1 - r3));
-    all 40 fragments hold dtype(K2 W)."""
+    idt3 [352:480], idt1 w1 [480:512], idt2 / K2 [512:576] (the last two
+    give the backward's tangents h1' = idt1 w1 (1 - t1^2) and
+    h2' = (idt2 / K2) (1 - t2^2) z2' + h1'); the fragments hold dtype(K2 W),
+    which the backward's tangent products use as well, so forces are the
+    gradient of the evaluated energy."""
     ntypes = len(emb_layers)
     (w1, _, _) = _layer(emb_layers[0][0])
     n1 = w1.shape[1]
@@ -119,7 +121,6 @@ def pack_embedding(emb_layers: list, dtype=torch.bfloat16) -> tuple[np.ndarray, 
         c[352 + q3] = i3
         c[480 + q1] = i1 * w1[0]
         c[512 + q2] = i2 / K2
-        c[576 + q3] = 4.0 * i3 / K2
         K = phi[lg]  # [64, 8] K slot -> feature offset within a 32-chunk
         f = frags[t]
         for mt in range(4):  # layer 2, A = K2 W2^T (M = q2, K = q1)
@@ -127,14 +128,6 @@ def pack_embedding(emb_layers: list, dtype=torch.bfloat16) -> tuple[np.ndarray, 
         for mt in range(8):  # layer 3, A = K2 W3^T (M = q3, K = q2)
             for kc in range(2):
                 f[4 + 2 * mt + kc] = K2 * W3[32 * kc + K, (16 * mt + lr)[:, None]]
-        # the adjoint uses the same dtype(K2 W) as the forward (with dz / K2 as
-        # its B operand), so forces are the gradient of the evaluated energy
-        for mt in range(4):  # dh2 = W3 dz3 (M = q2, K = q3)
-            for kc in range(4):
-                f[20 + mt * 4 + kc] = K2 * W3[(16 * mt + lr)[:, None], 32 * kc + K]
-        for mt in range(2):  # dh1 = W2 dz2 (M = q1, K = q2)
-            for kc in range(2):
-                f[20 + 16 + mt * 2 + kc] = K2 * W2[(16 * mt + lr)[:, None], 32 * kc + K]
     bits = torch.from_numpy(frags).to(dtype).view(torch.int16).numpy().view(np.uint16)
     return bits, consts, n1
 

@@ -360,8 +360,8 @@ int dpmi_tabulate_build_f64(double* table, double* node_vdd, const double* xx, c
  * deepmd/pt/model/network/mlp.py:188-219; type_one_side, widths
  * (n1, 2 n1, 4 n1), n1 <= 32, resnet with timestep).  The reference has no
  * single function for this: it chains torch MLP layers and matmuls.
- * frags: [ntypes, 40, 64, 8] bf16 (fp16 for _f16) weight fragments and consts
- * [ntypes, 704] fp32, both laid out by dpmi.embed.pack_embedding (internal
+ * frags: [ntypes, 20, 64, 8] bf16 (fp16 for _f16) weight fragments and consts
+ * [ntypes, 576] fp32, both laid out by dpmi.embed.pack_embedding (internal
  * feature order q3; see that module).  em [nloc, nnei, 4] fp32.
  * ------------------------------------------------------------------------ */
 /* counts [nloc, ntypes] int16 = listed neighbors of each atom per type
@@ -395,12 +395,14 @@ int dpmi_se_a_descriptor_fwd(float* gr, void* desc, const float* gr_part, int64_
 int dpmi_se_a_descriptor_bwd(float* dgr, const float* ddesc, const float* gr, int64_t nloc, int nnei, int n1,
                              int axis, void* stream);
 /* net_deriv [nloc, nnei, 4] = dE/d em, recomputing the embedding forward
- * (n1 = width of the first embedding layer, as packed).  With counts (the
+ * with its tangent in s = em[..., 0] (forward mode: dE/ds = sum_c em[c]
+ * sum_q dGR[c][q] dG[q]/ds; n1 = width of the first embedding layer, as
+ * packed).  With counts (the
  * padding shortcut of the forward) the rows after each section's listed
  * neighbors are written 0: they are padding slots, which no force term
  * reads.  The fp16 form stages each atom's dgr scaled by a power of two
  * (its largest magnitude brought into [1, 2)) and unscales the result,
- * exactly, so the fp16 operands of the adjoint stay normal numbers. */
+ * exactly, so its fp16 dgr operands stay normal numbers. */
 int dpmi_se_a_embed_bwd_bf16(float* net_deriv, const float* em, const float* dgr, const void* frags,
                              const float* consts, int64_t nloc, const int* sel_host, int ntypes, int n1,
                              const int16_t* counts, void* stream);

@@ -183,7 +183,11 @@ def test_forward_gr_transpose(packed):
 
 
 def test_backward_tile_matches_autograd(packed):
-    """dE/ds and dE/d em of one tile for E = sum dGR . (em^T G)."""
+    """dE/d em of one tile for E = sum dGR . (em^T G) through k_embed_bwd's
+    dataflow: the forward recompute with its tangent in s (h1', h2', z3'
+    from the same fragments), the A entries written where the stage writes
+    them (lane l: features l and l + 64), dem = K_c + A r3 + A h2 and
+    tg = A r3 (r3 - 1) z3' + A' h2' = (K2 / 2) sum_q dGR dG/ds."""
     import torch
 
     params, frags, consts, n1 = packed
@@ -195,61 +199,51 @@ def test_backward_tile_matches_autograd(packed):
     dgr[:, q3] = rng.normal(size=(4, 4 * n1))
     f = frags[t]
     c = consts[t]
-    t1, h1, t2, h2, r3, G = forward_tile(em[LR, 0], f, c)
     # activations unrounded here: this checks the dataflow, not the bf16 error
-    r3b, h2b = r3, h2
-    # the kernel's backward dataflow: dem = K_c + sum r3 (-2 idt3 dGR) + sum h2 (dGR[q2] + dGR[q2 + 64])
-    kslot = 32 * np.arange(4)[:, None, None] + phi(LG[:, None], np.arange(8)[None, :])[None]  # [kc, 64, 8]
-    crow = (LR & 3)[:, None]
-    a_r = [-2.0 * c[352 + kslot[kc]] * dgr[crow, kslot[kc]] for kc in range(4)]
-    a_h = [dgr[crow, kslot[kc]] + dgr[crow, kslot[kc] + 64] for kc in range(2)]
-    kc_all = (c[352:480][None, :] * dgr).sum(axis=1)  # K_c
-    dem = np.tile(kc_all, (64, 1))
+    t1, h1, t2, h2, r3, G = forward_tile(em[LR, 0], f, c)
+    # the stage: every entry of the slot's 1024 written exactly once
+    dop = np.full(1024, np.nan)
+    at = 32 * ((L >> 2) & 3) + 4 * ((L >> 4) & 1) + (L & 3) + 128 * (L >> 5)
+    for cc in range(4):
+        xa, xb = dgr[cc, L], dgr[cc, L + 64]
+        for off, val in ((0, -2.0 * c[352 + L] * xa), (256, -2.0 * c[352 + 64 + L] * xb), (512, xa + xb),
+                         (768, 0.5 * embed.K2 * (xa + xb))):
+            assert np.isnan(dop[at + 8 * cc + off]).all()
+            dop[at + 8 * cc + off] = val
+    assert not np.isnan(dop).any()
+    # A operand of chunk j, lane (r, g): entry 16 j + 4 g + (r & 3)
+    A = [np.stack([dop[(16 * j + 4 * LG + (LR & 3)) * 8 + u] for u in range(8)], axis=1) for j in range(8)]
+    # tangents
+    h1t = np.zeros((64, 8))
+    for u in range(8):
+        h1t[:, u] = (1.0 - t1[:, u] ** 2) * c[480 + phi(LG, u)]
+    h2t = np.zeros((64, 16))
+    for mt in range(4):
+        z2t = mfma(f[mt], h1t)
+        q = 16 * mt + 4 * LG
+        for v in range(4):
+            h2t[:, mt * 4 + v] = (1.0 - t2[:, mt * 4 + v] ** 2) * c[512 + q + v] * z2t[:, v] + h1t[:, (mt & 1) * 4 + v]
+    bt = np.zeros((64, 32))
+    for mt in range(8):
+        z3t = mfma(f[5 + 2 * mt], frag_of(h2t, 1), mfma(f[4 + 2 * mt], frag_of(h2t, 0)))
+        for v in range(4):
+            rr = r3[:, mt * 4 + v]
+            bt[:, mt * 4 + v] = (rr * rr - rr) * z3t[:, v]
+    dem = np.zeros((64, 4))
+    dem[LG == 0] = (c[352:480][None, :] * dgr).sum(axis=1)  # K_c of slot 0
+    tg = np.zeros((64, 4))
     for kc in range(4):
-        dem = mfma(a_r[kc], frag_of(r3b, kc), dem)
+        dem = mfma(A[kc], frag_of(r3, kc), dem)
+        tg = mfma(A[kc], frag_of(bt, kc), tg)
     for kc in range(2):
-        dem = mfma(a_h[kc], frag_of(h2b, kc), dem)
-    # dG A operand: row q3 of the bf16 table (dGR[0..3][q3], 0 x 4) in every lane group
-    sdgb = np.zeros((128, 8))
-    sdgb[:, :4] = dgr.T
-    b_em = np.zeros((64, 8))
-    b_em[LG == 0, :4] = em[LR[LG == 0]]
-    bdz3 = [np.zeros((64, 8)) for _ in range(4)]
-    skip = []
-    for mt in range(4):
-        a_lo, a_hi = sdgb[16 * mt + LR], sdgb[16 * (mt + 4) + LR]
-        d_lo, d_hi = mfma(a_lo, b_em), mfma(a_hi, b_em)
-        skip.append(mfma(a_hi, b_em, d_lo))
-        q = 16 * mt + 4 * LG
-        for v in range(4):
-            rl, rh = r3[:, mt * 4 + v], r3[:, (mt + 4) * 4 + v]
-            bdz3[mt >> 1][:, (mt & 1) * 4 + v] = d_lo[:, v] * c[576 + q + v] * (rl - rl * rl)
-            bdz3[(mt + 4) >> 1][:, (mt & 1) * 4 + v] = d_hi[:, v] * c[640 + q + v] * (rh - rh * rh)
-    dh2 = np.zeros((64, 16))
-    bdz2 = [np.zeros((64, 8)) for _ in range(2)]
-    for mt in range(4):
-        d = skip[mt]
-        for kc in range(4):
-            d = mfma(f[20 + mt * 4 + kc], bdz3[kc], d)
-        q = 16 * mt + 4 * LG
-        for v in range(4):
-            x = d[:, v]
-            dh2[:, mt * 4 + v] = x
-            bdz2[mt >> 1][:, (mt & 1) * 4 + v] = x * c[512 + q + v] * (1 - t2[:, mt * 4 + v] ** 2)
-    ds = np.zeros(64)
-    for mt in range(2):
-        d = np.zeros((64, 4))
-        for kc in range(2):
-            d = mfma(f[36 + mt * 2 + kc], bdz2[kc], d)
-        for v in range(4):
-            u = mt * 4 + v
-            q = phi(LG, u)
-            x = d[:, v] + dh2[:, mt * 4 + v] + dh2[:, (mt + 2) * 4 + v]
-            ds += x * (1 - t1[:, u] ** 2) * c[480 + q]  # c[480 + q] = idt1 w1
-    ds_tot = np.array([ds[LR == r].sum() for r in range(16)])
-    nd = np.zeros((16, 4))
-    nd[:, 0] = dem[LG == 0][:, 0][np.argsort(LR[LG == 0])] + ds_tot
-    nd[:, 1:] = dem[LG == 0][:, 1:][np.argsort(LR[LG == 0])]
+        dem = mfma(A[4 + kc], frag_of(h2, kc), dem)
+        tg = mfma(A[6 + kc], frag_of(h2t, kc), tg)
+    # lane group 0 holds slot 0: M rows c = 0..3 of each row LR
+    l0 = LG == 0
+    order = np.argsort(LR[l0])
+    ds = (2.0 / embed.K2) * (em[LR[l0]] * tg[l0]).sum(axis=1)
+    nd = dem[l0][order].copy()
+    nd[:, 0] += ds[order]
     # autograd reference with the same (bf16-rounded) weights
     emt = torch.tensor(em, requires_grad=True)
     x = emt[:, :1]
