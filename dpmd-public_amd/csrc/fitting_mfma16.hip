@@ -124,31 +124,30 @@ __device__ inline void hidden_product(f32x4 (&acc)[4][kSub], const h8<H>* __rest
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
     for (int s = 0; s < kSub; ++s) acc[mt][s] = f32x4{0.f, 0.f, 0.f, 0.f};
-  // fragments two K chunks ahead
-  h8<H> a[4], ab[4];
+  // fragments two K chunks ahead, in a ring of three register sets (a loop
+  // body of three chunks, so no set is copied into another)
+  auto load = [&](int kc, h8<H>(&a)[4]) {
 #pragma unroll
-  for (int mt = 0; mt < 4; ++mt) {
-    a[mt] = frag[((4 * fg + mt) * 8 + 0) * 64 + lane];
-    ab[mt] = frag[((4 * fg + mt) * 8 + 1) * 64 + lane];
-  }
-#pragma unroll 1
-  for (int kc = 0; kc < 8; ++kc) {
-    h8<H> an[4];
-    const int kn = kc + 2 < 8 ? kc + 2 : kc;
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) an[mt] = frag[((4 * fg + mt) * 8 + kn) * 64 + lane];
+    for (int mt = 0; mt < 4; ++mt) a[mt] = frag[((4 * fg + mt) * 8 + kc) * 64 + lane];
+  };
+  auto step = [&](int kc, const h8<H>(&cur)[4], h8<H>(&nxt2)[4]) {
+    if (kc + 2 < 8) load(kc + 2, nxt2);
     h8<H> b[kSub];
 #pragma unroll
     for (int s = 0; s < kSub; ++s) b[s] = lds_b(s_h, kHS, s0 + s, n, 32 * kc, g);
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-      for (int s = 0; s < kSub; ++s) acc[mt][s] = mfma(a[mt], b[s], acc[mt][s]);
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-      a[mt] = ab[mt];
-      ab[mt] = an[mt];
-    }
+      for (int s = 0; s < kSub; ++s) acc[mt][s] = mfma(cur[mt], b[s], acc[mt][s]);
+  };
+  h8<H> a0[4], a1[4], a2[4];
+  load(0, a0);
+  load(1, a1);
+#pragma unroll 1
+  for (int kc = 0; kc < 8; kc += 3) {
+    step(kc, a0, a2);
+    if (kc + 1 < 8) step(kc + 1, a1, a0);
+    if (kc + 2 < 8) step(kc + 2, a2, a1);
   }
 }
 
@@ -305,13 +304,11 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt) a[kc][mt] = w1f[((4 * fg + mt) * p.nk1 + 2 * kb + kc) * 64 + lane];
   };
-  h8<H> a_cur[2][4], a_nxt[2][4];
-  load_w1(0, a_cur);
-  load_w1(nkb > 1 ? 1 : 0, a_nxt);
-#pragma unroll 1
-  for (int kb = 0; kb < nkb; ++kb) {
+  // one K block: products with the fragments cur, block kb + 2's fragments
+  // into nxt2 (a ring of three register sets over a loop body of three
+  // blocks, so no set is copied into another)
+  auto l1_step = [&](int kb, const h8<H>(&cur)[2][4], h8<H>(&nxt2)[2][4]) {
     h8<H> r;
-    h8<H> a_nn[2][4];
     const bool more = kb + 1 < nkb;
     // fused form: D block kb + 1 is built from the staged GR
     if constexpr (kGR) {
@@ -319,7 +316,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
     } else {
       if (more) load_block(kb + 1, r);
     }
-    load_w1(kb + 2 < nkb ? kb + 2 : nkb - 1, a_nn);
+    load_w1(kb + 2 < nkb ? kb + 2 : nkb - 1, nxt2);
 #pragma unroll
     for (int kc = 0; kc < 2; ++kc) {
       h8<H> b[kSub];
@@ -328,7 +325,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-        for (int s = 0; s < kSub; ++s) acc[mt][s] = mfma(a_cur[kc][mt], b[s], acc[mt][s]);
+        for (int s = 0; s < kSub; ++s) acc[mt][s] = mfma(cur[kc][mt], b[s], acc[mt][s]);
     }
     if constexpr (kGR) {
       if (more) store_gen((kb + 1) & 1, r);
@@ -336,13 +333,15 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
       if (more) store_block((kb + 1) & 1, r);
     }
     __syncthreads();
-#pragma unroll
-    for (int kc = 0; kc < 2; ++kc)
-#pragma unroll
-      for (int mt = 0; mt < 4; ++mt) {
-        a_cur[kc][mt] = a_nxt[kc][mt];
-        a_nxt[kc][mt] = a_nn[kc][mt];
-      }
+  };
+  h8<H> a0[2][4], a1[2][4], a2[2][4];
+  load_w1(0, a0);
+  load_w1(nkb > 1 ? 1 : 0, a1);
+#pragma unroll 1
+  for (int kb = 0; kb < nkb; kb += 3) {
+    l1_step(kb, a0, a2);
+    if (kb + 1 < nkb) l1_step(kb + 1, a1, a0);
+    if (kb + 2 < nkb) l1_step(kb + 2, a2, a1);
   }
   // ---- layer-1 epilogue: h (fp32, skip of layer 2), tanh factors f1 (fp32)
   float h[4][kSub][4], f1[4][kSub][4], f2[4][kSub][4], f3[4][kSub][4];
