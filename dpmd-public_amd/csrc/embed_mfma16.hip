@@ -27,8 +27,9 @@
 // q3 = q2(f mod 2n1) + 64*[f >= 2n1]; padded features are exact zeros.
 // G itself is never formed: with r3 = 1 / (1 + exp(2 x3)), tanh = 1 - 2 r3 and
 // G = idt3 (1 - 2 r3) + [h2, h2], so G^T R = idt3 (em^T 1) - 2 idt3 (em^T r3)
-// + em^T [h2, h2]; r3 and h2 are transposed per 16-row tile by MFMA against
-// an identity fragment and reduced with MFMA against em^T.  The backward pass
+// + em^T [h2, h2]; layer 3 is computed transposed, so r3 comes out with the
+// rows in the K position of the em^T r3 product, h2 is transposed per
+// 16-row tile by MFMA against an identity fragment.  The backward pass
 // recomputes the forward per tile together with its tangent in the scalar
 // input s (forward-mode: the same fragments, the same register layout) and
 // contracts both with dGR, so it needs no transposes and no adjoint chain.
@@ -96,36 +97,37 @@ __device__ inline f32x4 mfma16(const h4<_Float16>& a, const h4<_Float16>& b, con
 // Layer-1 feature owned by lane group g at B-slot u (also the K order phi).
 __device__ inline int phi(int g, int u) { return 16 * (u >> 2) + 4 * g + (u & 3); }
 
-// Recompute of the forward chain for one 16-row tile, all in the lane's
-// layout.  Layer 3 is returned as r3 = 1 / (1 + 2^z3) only: with
-// tanh = 1 - 2 r3 the embedding output is G = idt3 (1 - 2 r3) + [h2, h2], and
-// the kernels contract r3 and h2 separately (G^T R = idt3 (sum_rows em)
-// - 2 idt3 (em^T r3) + em^T [h2, h2]), so G itself is never formed.
+// The forward chain of one 16-row tile.  Layers 1 and 2 keep the row as the
+// lane index (N); layer 3 is computed transposed, z3^T = h2^T (k W3): h2's
+// registers serve as the A operand (M = row, the same K slots) and the
+// layer-3 weight fragment as the B operand (N = feature, the same lane
+// contents), so r3 comes out with the feature as the lane index and lane
+// group g holding rows 4 g .. 4 g + 3 - the K = 16 operand of the em^T r3
+// product, with no transpose.  Layer 3 is returned as r3 = 1 / (1 + 2^z3)
+// only: with tanh = 1 - 2 r3 the embedding output is G = idt3 (1 - 2 r3) +
+// [h2, h2], and the kernel contracts r3 and h2 separately (G^T R = idt3
+// (sum_rows em) - 2 idt3 (em^T r3) + em^T [h2, h2]), so G is never formed.
 template <typename H>
 struct TileFwd {
-  float t1[8], h1[8];
-  float t2[16];
-  h8<H> b2[2];  // h2 in H: layer 3's B operand (features 32 kc + phi(g, u))
-  float r3[32];  // slot mt * 4 + v: internal feature 16 mt + 4 g + v
+  h8<H> b2[2];   // h2 in H: features 32 kc + phi(g, u) of row lane % 16
+  float r3t[32];  // slot mt * 4 + v: feature 16 mt + lane % 16 of row 4 g + v
 };
 
 template <typename H, bool kSkip>
-__device__ inline void forward_tile(float s, const h8<H>* __restrict__ frag, const float* __restrict__ cst, int g,
-                                    int lane, TileFwd<H>& f) {
+__device__ inline void forward_tile(float s, const h8<H>* __restrict__ frag, const float* __restrict__ cst,
+                                    const f32x4* __restrict__ b3x4, int g, int lane, TileFwd<H>& f) {
   h8<H> b1;
+  float h1[8];
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
     if (kSkip && u == 7) {  // empty slot in every lane group
-      f.t1[u] = 0.f;
-      f.h1[u] = 0.f;
+      h1[u] = 0.f;
       b1[u] = (H)0.f;
       continue;
     }
-    int q = phi(g, u);
-    float t = tanh_scaled(s * cst[q] + cst[32 + q]);
-    f.t1[u] = t;
-    f.h1[u] = t * cst[64 + q];
-    b1[u] = (H)f.h1[u];
+    const int q = phi(g, u);
+    h1[u] = tanh_scaled(s * cst[q] + cst[32 + q]) * cst[64 + q];
+    b1[u] = (H)h1[u];
   }
   f32x4 z2[4];
 #pragma unroll
@@ -137,36 +139,23 @@ __device__ inline void forward_tile(float s, const h8<H>* __restrict__ frag, con
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       if (kSkip && (mt & 1) && v == 3) {
-        f.t2[mt * 4 + v] = 0.f;
         f.b2[mt >> 1][(mt & 1) * 4 + v] = (H)0.f;
         continue;
       }
-      float t = tanh_scaled(z2[mt][v]);
-      f.t2[mt * 4 + v] = t;
-      f.b2[mt >> 1][(mt & 1) * 4 + v] = (H)(t * cst[160 + q + v] + f.h1[(mt & 1) * 4 + v]);
+      f.b2[mt >> 1][(mt & 1) * 4 + v] = (H)(tanh_scaled(z2[mt][v]) * cst[160 + q + v] + h1[(mt & 1) * 4 + v]);
     }
   }
+  const int n = lane & 15;
 #pragma unroll
   for (int mt = 0; mt < 8; ++mt) {
     // fragments read per pair of output tiles, not all ahead of the products
     // (sixteen fragments in flight would hold 64 registers)
     if ((mt & 1) == 0) asm volatile("" ::: "memory");
-    f32x4 z = mfma(frag[(4 + 2 * mt) * 64 + lane], f.b2[0], *reinterpret_cast<const f32x4*>(cst + 224 + 16 * mt + 4 * g));
-    z = mfma(frag[(5 + 2 * mt) * 64 + lane], f.b2[1], z);
+    f32x4 z = mfma(f.b2[0], frag[(4 + 2 * mt) * 64 + lane], b3x4[16 * mt + n]);
+    z = mfma(f.b2[1], frag[(5 + 2 * mt) * 64 + lane], z);
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
-      f.r3[mt * 4 + v] = (kSkip && (mt & 1) && v == 3) ? 0.f
-                                                        : __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(z[v]));
+    for (int v = 0; v < 4; ++v) f.r3t[mt * 4 + v] = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(z[v]));
   }
-}
-
-// r3 of the 32-feature chunk kc as an MFMA operand (K = feature slot phi(g, u))
-template <typename H>
-__device__ inline h8<H> r3_frag(const TileFwd<H>& f, int kc) {
-  h8<H> b;
-#pragma unroll
-  for (int u = 0; u < 8; ++u) b[u] = (H)f.r3[(2 * kc + (u >> 2)) * 4 + (u & 3)];
-  return b;
 }
 
 // Row stream.  A wavefront takes a contiguous range of atoms for one
@@ -233,14 +222,14 @@ __device__ inline int chunk_atom(const Chunk& c, int k0, int pos, int* row) {
 // the atoms of each wavefront's range, over 16-row tiles of the row stream
 // holding rows of up to four atoms.  A workgroup loads its type's weight
 // fragments once; its four wavefronts stream independently (no block
-// barrier after set-up).  G^T R contracts over neighbor rows while r3 and h2
-// come out of the layer products with the row as the lane index, so each
-// 16-feature block is transposed by MFMA against an identity fragment
-// (exact for a 16-bit operand): then the lane index is the feature and lane
-// group g holds rows 4g .. 4g + 3, the K = 16 operand of the products
-// em^T r3, em^T h2 and em^T 1, whose A operand holds em[row][c] in M row
-// 4 slot(row) + c and zero in the other slots' rows.  Each block is consumed
-// as soon as it is transposed, so no transposed tile is held in registers.
+// barrier after set-up).  G^T R contracts over neighbor rows: r3 comes out
+// of the transposed layer-3 product with the feature as the lane index and
+// lane group g holding rows 4g .. 4g + 3, the K = 16 operand of em^T r3; h2
+// comes out of layer 2 with the row as the lane index, so each of its
+// 16-feature blocks is transposed by MFMA against an identity fragment
+// (exact for a 16-bit operand) into the same layout for em^T h2.  The A
+// operand of these products and of em^T 1 holds em[row][c] in M row
+// 4 slot(row) + c and zero in the other slots' rows.
 template <typename H, bool kSkip>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip ? 4 : 3, kSkip ? 4 : 3)))
     k_embed_fwd(float* __restrict__ gr_part, const float* __restrict__ em, const h8<H>* __restrict__ frag_all,
@@ -248,11 +237,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
   __shared__ h8<H> s_frag[kFragFwd * 64];
   __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
   __shared__ __attribute__((aligned(16))) float s_em[kAtoms][16 * 16];  // [row][4 slot + c]
+  // layer-3 bias (k b3) of feature 16 mt + n in all four rows: the
+  // accumulator input of the transposed product, one 16-byte read
+  __shared__ f32x4 s_b3x4[8 * 16];
   const int t = blockIdx.y;
   const int w = wave_id(), lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   const h8<H>* frag_t = frag_all + (int64_t)t * kFragFwd * 64;
   for (int k = threadIdx.x; k < kFragFwd * 64; k += blockDim.x) s_frag[k] = frag_t[k];
   for (int k = threadIdx.x; k < kConsts; k += blockDim.x) s_cst[k] = consts[(int64_t)t * kConsts + k];
+  for (int k = threadIdx.x; k < 8 * 16; k += blockDim.x) {
+    const float b = consts[(int64_t)t * kConsts + 224 + k];
+    s_b3x4[k] = f32x4{b, b, b, b};
+  }
   const float* cst = s_cst;
   const int sel = p.sec[t + 1] - p.sec[t];
   __syncthreads();
@@ -265,16 +261,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
   if (p.counts != nullptr) {
     for (int ti = w; ti < p.ntypes; ti += kAtoms) {
       TileFwd<H> f;
-      forward_tile<H, kSkip>(p.pad_em[(ti * p.ntypes + t) * 4], s_frag, cst, g, lane, f);
+      forward_tile<H, kSkip>(p.pad_em[(ti * p.ntypes + t) * 4], s_frag, cst, s_b3x4, g, lane, f);
+      // every row holds the padding row: row 0 (lane group 0, v = 0) gives
+      // r3 of feature 16 mt + r, row 0 (lane r = 0) h2 of 16 mt + 4 g + v
+      if (g == 0)
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt) {
+          const int q = 16 * mt + r;
+          s_gpad[ti][q] = cst[352 + q] * (1.f - 2.f * (float)(H)f.r3t[mt * 4]);
+        }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
       if (r == 0)
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int q = 16 * mt + 4 * g + v;
-            s_gpad[ti][q] = cst[352 + q] * (1.f - 2.f * (float)(H)f.r3[mt * 4 + v]) +
-                            (float)f.b2[(mt & 3) >> 1][((mt & 3) & 1) * 4 + v];
-          }
+          for (int v = 0; v < 4; ++v)
+            s_gpad[ti][16 * mt + 4 * g + v] += (float)f.b2[(mt & 3) >> 1][((mt & 3) & 1) * 4 + v];
     }
     __syncthreads();
   }
@@ -344,7 +347,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
         // weights and constants stay in LDS (re-read per tile), not in registers
         asm volatile("" ::: "memory");
         TileFwd<H> f;
-        forward_tile<H, kSkip>(ev.x, s_frag, cst, g, lane, f);
+        forward_tile<H, kSkip>(ev.x, s_frag, cst, s_b3x4, g, lane, f);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         // em^T as A (M = 4 slot + c, K = row 4 g + u)
@@ -352,16 +355,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
 #pragma unroll
         for (int u = 0; u < 4; ++u) a[u] = (H)sem[16 * (4 * g + u) + r];
 #pragma unroll
-        for (int kc = 0; kc < 4; ++kc) {
-          const h8<H> rb = r3_frag(f, kc);
+        for (int mt = 0; mt < 8; ++mt) {
+          h4<H> b;
 #pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            const f32x4 d = mfma(rb, eye[j], f32x4{0.f, 0.f, 0.f, 0.f});
-            h4<H> b;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) b[v] = (H)d[v];
-            acc[2 * kc + j] = mfma16(a, b, acc[2 * kc + j]);
-          }
+          for (int v = 0; v < 4; ++v) b[v] = (H)f.r3t[mt * 4 + v];
+          acc[mt] = mfma16(a, b, acc[mt]);
         }
 #pragma unroll
         for (int kc = 0; kc < 2; ++kc)
