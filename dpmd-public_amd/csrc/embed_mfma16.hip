@@ -615,7 +615,7 @@ constexpr float kK2 = 2.8853900817779268f;  // k = 2 log2(e)
 template <typename H, bool kSkip>
 __global__ void __launch_bounds__(64 * kWavesBwd) __attribute__((amdgpu_waves_per_eu(4, 4)))
     k_embed_bwd(float* __restrict__ net_deriv, const float* __restrict__ em, const float* __restrict__ dgr,
-                const h8<H>* __restrict__ frag_all, const float* __restrict__ consts, int nloc, int per_wave,
+                const h8<H>* __restrict__ frag_all, const float* __restrict__ consts, int nloc, int per_wg,
                 EmbedParams p) {
   constexpr int kS = 2;  // atoms per tile
   __shared__ h8<H> s_frag[kFragFwd * 64];
@@ -627,11 +627,47 @@ __global__ void __launch_bounds__(64 * kWavesBwd) __attribute__((amdgpu_waves_pe
   const h8<H>* frag_t = frag_all + (int64_t)t * kFragFwd * 64;
   for (int k = threadIdx.x; k < kFragFwd * 64; k += blockDim.x) s_frag[k] = frag_t[k];
   for (int k = threadIdx.x; k < kConsts; k += blockDim.x) s_cst[k] = consts[(int64_t)t * kConsts + k];
-  __syncthreads();
   const float* cst = s_cst;
   const int sel = p.sec[t + 1] - p.sec[t];
   const float* em_t = em + (int64_t)p.sec[t] * 4;
   float* nd_t = net_deriv + (int64_t)p.sec[t] * 4;
+  // The workgroup's atoms [A0, A1) (at most one per thread) are split among
+  // its wavefronts by rows, not by atoms: with R rows in all, atom i (E_i
+  // rows before it) goes to wavefront min(15, 16 E_i / R), so every
+  // wavefront gets a contiguous range of about R / 16 rows (a wavefront
+  // waits for nothing at the end: the workgroup ends with its last one).
+  __shared__ int s_wsum[kWavesBwd], s_wstart[kWavesBwd + 1];
+  const int A0 = min((int)blockIdx.x * per_wg, nloc), nwg = min(per_wg, nloc - A0);
+  {
+    const int ia = threadIdx.x;
+    const int cnt_i = ia < nwg ? (p.counts != nullptr ? (int)p.counts[(int64_t)(A0 + ia) * p.ntypes + t] : sel) : 0;
+    int inc = cnt_i;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += y;
+    }
+    if (lane == 63) s_wsum[w] = inc;
+    if (threadIdx.x <= kWavesBwd) s_wstart[threadIdx.x] = nwg;
+    __syncthreads();
+    int base = 0, R = 0;
+    for (int ww = 0; ww < kWavesBwd; ++ww) {
+      const int x = s_wsum[ww];
+      base += ww < w ? x : 0;
+      R += x;
+    }
+    const int ex = base + inc - cnt_i;  // E_i
+    auto wave_of = [&](int e) { return R > 0 ? min(kWavesBwd - 1, (int)(((int64_t)kWavesBwd * e) / R)) : 0; };
+    // atom i starts the wavefronts (wave_of(E_{i-1}), wave_of(E_i)]
+    int e_prev = __shfl_up(ex, 1, 64);
+    if (lane == 0) e_prev = ia > 0 ? base - (ia - 1 < nwg ? (p.counts != nullptr ? (int)p.counts[(int64_t)(A0 + ia - 1) * p.ntypes + t] : sel) : 0) : -1;
+    if (ia < nwg) {
+      const int k1 = wave_of(ex), k0 = ia > 0 ? wave_of(e_prev) : -1;
+      for (int k = k0 + 1; k <= k1; ++k) s_wstart[k] = ia;
+    }
+    __syncthreads();
+  }
+
   // fp16 operands: everything below is linear in dGR, so each atom's dGR is
   // staged times 2^-e (2^e the power of two at or below its largest
   // magnitude) and dE/d em multiplied by 2^e at the end, both exact; the A
@@ -694,7 +730,7 @@ __global__ void __launch_bounds__(64 * kWavesBwd) __attribute__((amdgpu_waves_pe
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   };
-  const int a_lo = min((blockIdx.x * kWavesBwd + w) * per_wave, nloc), a_hi = min(a_lo + per_wave, nloc);
+  const int a_lo = A0 + s_wstart[w], a_hi = A0 + s_wstart[w + 1];
   for (int wa = a_lo; wa < a_hi; wa += 64) {
     const int n = min(64, a_hi - wa);
     int cnt = 0;
@@ -900,16 +936,17 @@ int embed_bwd(float* net_deriv, const float* em, const float* dgr, const void* f
   p.n1 = n1;
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
   if (nloc == 0) return DPMI_OK;
-  // one workgroup of sixteen wavefronts per CU (87 KB of LDS), up to 1,024
-  // per type; each wavefront streams a contiguous range of per_wave atoms
-  dim3 grid((unsigned)std::min<int64_t>(ceil_div(nloc, kWavesBwd), 1024), (unsigned)ntypes);
-  const int per_wave = (int)ceil_div(nloc, (int64_t)grid.x * kWavesBwd);
+  // one workgroup of sixteen wavefronts per CU (87 KB of LDS); a workgroup
+  // takes per_wg <= 1,024 consecutive atoms (about 256 workgroups per type)
+  // and splits them among its wavefronts by rows
+  const int per_wg = (int)std::min<int64_t>(64 * kWavesBwd, std::max<int64_t>(kWavesBwd, ceil_div(nloc, 256)));
+  dim3 grid((unsigned)ceil_div(nloc, (int64_t)per_wg), (unsigned)ntypes);
   if (skip_layout(n1))
     hipLaunchKernelGGL((k_embed_bwd<H, true>), grid, dim3(64 * kWavesBwd), 0, as_stream(stream), net_deriv, em,
-                       dgr, static_cast<const h8<H>*>(frags), consts, (int)nloc, per_wave, p);
+                       dgr, static_cast<const h8<H>*>(frags), consts, (int)nloc, per_wg, p);
   else
     hipLaunchKernelGGL((k_embed_bwd<H, false>), grid, dim3(64 * kWavesBwd), 0, as_stream(stream), net_deriv, em,
-                       dgr, static_cast<const h8<H>*>(frags), consts, (int)nloc, per_wave, p);
+                       dgr, static_cast<const h8<H>*>(frags), consts, (int)nloc, per_wg, p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }
