@@ -218,6 +218,47 @@ __device__ inline int chunk_atom(const Chunk& c, int k0, int pos, int* row) {
   return k0 + kk;
 }
 
+// A workgroup's atoms [A0, A0 + nwg) (at most one per thread) split among
+// its kW wavefronts by rows, not by atoms: with R rows in all, atom i (E_i
+// rows before it) goes to wavefront min(kW - 1, kW E_i / R), so every
+// wavefront gets a contiguous range of about R / kW rows and none waits long
+// for the last one at the end.  Wavefront k takes atoms s_wstart[k] ..
+// s_wstart[k + 1] - 1 (relative to A0).  Contains two workgroup barriers.
+template <int kW>
+__device__ inline void split_rows(const EmbedParams& p, int t, int sel, int A0, int nwg, int w, int lane,
+                                  int* s_wsum, int* s_wstart) {
+  auto count = [&](int i) {
+    return i < nwg ? (p.counts != nullptr ? (int)p.counts[(int64_t)(A0 + i) * p.ntypes + t] : sel) : 0;
+  };
+  const int ia = threadIdx.x;
+  const int cnt_i = count(ia);
+  int inc = cnt_i;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += y;
+  }
+  if (lane == 63) s_wsum[w] = inc;
+  if (threadIdx.x <= kW) s_wstart[threadIdx.x] = nwg;
+  __syncthreads();
+  int base = 0, R = 0;
+  for (int ww = 0; ww < kW; ++ww) {
+    const int x = s_wsum[ww];
+    base += ww < w ? x : 0;
+    R += x;
+  }
+  const int ex = base + inc - cnt_i;  // E_i
+  auto wave_of = [&](int e) { return R > 0 ? min(kW - 1, (int)(((int64_t)kW * e) / R)) : 0; };
+  // atom i starts the wavefronts (wave_of(E_{i-1}), wave_of(E_i)]
+  int e_prev = __shfl_up(ex, 1, 64);
+  if (lane == 0) e_prev = ia > 0 ? base - count(ia - 1) : -1;
+  if (ia < nwg) {
+    const int k1 = wave_of(ex), k0 = ia > 0 ? wave_of(e_prev) : -1;
+    for (int k = k0 + 1; k <= k1; ++k) s_wstart[k] = ia;
+  }
+  __syncthreads();
+}
+
 // Forward: GR partial [t][i][m][c] (original feature order m < 4 n1) of
 // the atoms of each wavefront's range, over 16-row tiles of the row stream
 // holding rows of up to four atoms.  A workgroup loads its type's weight
@@ -631,42 +672,10 @@ __global__ void __launch_bounds__(64 * kWavesBwd) __attribute__((amdgpu_waves_pe
   const int sel = p.sec[t + 1] - p.sec[t];
   const float* em_t = em + (int64_t)p.sec[t] * 4;
   float* nd_t = net_deriv + (int64_t)p.sec[t] * 4;
-  // The workgroup's atoms [A0, A1) (at most one per thread) are split among
-  // its wavefronts by rows, not by atoms: with R rows in all, atom i (E_i
-  // rows before it) goes to wavefront min(15, 16 E_i / R), so every
-  // wavefront gets a contiguous range of about R / 16 rows (a wavefront
-  // waits for nothing at the end: the workgroup ends with its last one).
+  // this workgroup's atoms, split among its wavefronts by rows
   __shared__ int s_wsum[kWavesBwd], s_wstart[kWavesBwd + 1];
   const int A0 = min((int)blockIdx.x * per_wg, nloc), nwg = min(per_wg, nloc - A0);
-  {
-    const int ia = threadIdx.x;
-    const int cnt_i = ia < nwg ? (p.counts != nullptr ? (int)p.counts[(int64_t)(A0 + ia) * p.ntypes + t] : sel) : 0;
-    int inc = cnt_i;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int y = __shfl_up(inc, off, 64);
-      if (lane >= off) inc += y;
-    }
-    if (lane == 63) s_wsum[w] = inc;
-    if (threadIdx.x <= kWavesBwd) s_wstart[threadIdx.x] = nwg;
-    __syncthreads();
-    int base = 0, R = 0;
-    for (int ww = 0; ww < kWavesBwd; ++ww) {
-      const int x = s_wsum[ww];
-      base += ww < w ? x : 0;
-      R += x;
-    }
-    const int ex = base + inc - cnt_i;  // E_i
-    auto wave_of = [&](int e) { return R > 0 ? min(kWavesBwd - 1, (int)(((int64_t)kWavesBwd * e) / R)) : 0; };
-    // atom i starts the wavefronts (wave_of(E_{i-1}), wave_of(E_i)]
-    int e_prev = __shfl_up(ex, 1, 64);
-    if (lane == 0) e_prev = ia > 0 ? base - (ia - 1 < nwg ? (p.counts != nullptr ? (int)p.counts[(int64_t)(A0 + ia - 1) * p.ntypes + t] : sel) : 0) : -1;
-    if (ia < nwg) {
-      const int k1 = wave_of(ex), k0 = ia > 0 ? wave_of(e_prev) : -1;
-      for (int k = k0 + 1; k <= k1; ++k) s_wstart[k] = ia;
-    }
-    __syncthreads();
-  }
+  split_rows<kWavesBwd>(p, t, sel, A0, nwg, w, lane, s_wsum, s_wstart);
 
   // fp16 operands: everything below is linear in dGR, so each atom's dGR is
   // staged times 2^-e (2^e the power of two at or below its largest
@@ -912,7 +921,8 @@ int embed_fwd(float* gr_part, const float* em, const void* frags, const float* c
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
   if (nloc == 0) return DPMI_OK;
   // up to 1,024 workgroups per type (256 CUs four times over); each
-  // wavefront streams a contiguous range of per_wave atoms
+  // wavefront streams a contiguous range of per_wave atoms (four workgroups
+  // per CU: one whose wavefronts end early is soon followed by another)
   dim3 grid((unsigned)std::min<int64_t>(ceil_div(nloc, kAtoms), 1024), (unsigned)ntypes);
   const int per_wave = (int)ceil_div(nloc, (int64_t)grid.x * kAtoms);
   if (skip_layout(n1))

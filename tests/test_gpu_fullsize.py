@@ -121,10 +121,14 @@ def test_alloy262k_nlist_bit_exact_with_overflow(oracle):
 def test_embedding_row_stream_many_windows(precision):
     """The embedding kernels on 884,736 atoms: the env-mat rows, section
     counts and types of the 110,592-atom water box tiled eight times, so each
-    wavefront's atom range spans several 64-atom windows (one at 110,592
-    atoms).  Each copy's G^T R partials equal the 110,592-atom launch's to the
-    fp32 summation order (rows fall in other tiles); dE/d em equal bit for bit
-    (a row's adjoint does not depend on the tile it shares with other atoms)."""
+    forward wavefront's atom range spans several 64-atom windows (one at
+    110,592 atoms).  Each copy's G^T R partials equal the 110,592-atom
+    launch's to the fp32 summation order (rows fall in other tiles); dE/d em
+    equal bit for bit (a row's adjoint does not depend on the tile it shares
+    with other atoms).  The backward splits its workgroups' atoms by rows:
+    with the rows of 15 atoms in 16 dropped (counts 0), a wavefront's range
+    holds about 16 times the atoms and spans several windows; the kept atoms'
+    dE/d em still equal the full launch's bit for bit, the others are 0."""
     from dpmi import embed
     from dpmi.model import SeAEnergyModel
     from dpmi.stat import gpu_model_params
@@ -161,3 +165,10 @@ def test_embedding_row_stream_many_windows(precision):
     assert torch.isfinite(g8).all() and torch.isfinite(nd8).all()
     assert gerr <= 1e-5
     assert same
+    keep = torch.arange(reps * s.nloc, device=DEV) % 16 == 0
+    cs = torch.where(keep[:, None], pad8[0], torch.zeros_like(pad8[0]))
+    nds = m.fused.backward_dgr(em8, dgr.repeat(reps, 1, 1).contiguous(), cs)
+    torch.cuda.synchronize()
+    kept = keep[: s.nloc]
+    assert all(torch.equal(nds[k * s.nloc:(k + 1) * s.nloc][kept], nd1[kept]) for k in range(reps))
+    assert (nds[~keep] == 0).all()
