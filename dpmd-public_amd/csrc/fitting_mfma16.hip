@@ -165,9 +165,9 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
           const float* __restrict__ gr_part, const int* __restrict__ perm, const int* __restrict__ type_start,
           const h8<H>* __restrict__ frag_all, const float* __restrict__ consts, const double* __restrict__ ebias,
           FitParams p) {
-  // the layer-1 D blocks (two buffers) and, after layer 1, the hidden-layer
+  // the layer-1 D blocks (four buffers) and, after layer 1, the hidden-layer
   // buffer share one region; the adjoint's dE/dGR stages reuse it at the end
-  constexpr int kXH = 2 * kTile * kXS > kTile * kHS ? 2 * kTile * kXS : kTile * kHS;
+  constexpr int kXH = 4 * kTile * kXS > kTile * kHS ? 4 * kTile * kXS : kTile * kHS;
   static_assert(kTile * kHS * 2 >= 2 * kTile * 16 * 16, "the dE/dGR stages must fit the shared region");
   __shared__ __attribute__((aligned(16))) H s_xh[kXH];
   auto s_x = [&](int b) { return s_xh + b * (kTile * kXS); };
@@ -289,10 +289,18 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
     h8<H> r;
     gen_block(0, r);
     store_gen(0, r);
+    if (nkb > 1) {
+      gen_block(1, r);
+      store_gen(1, r);
+    }
   } else {
     h8<H> r;
     load_block(0, r);
     store_block(0, r);
+    if (nkb > 1) {
+      load_block(1, r);
+      store_block(1, r);
+    }
   }
   __syncthreads();
   const h8<H>* w1f = frag + p.w1f;
@@ -306,33 +314,36 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
   };
   // one K block: products with the fragments cur, block kb + 2's fragments
   // into nxt2 (a ring of three register sets over a loop body of three
-  // blocks, so no set is copied into another)
+  // blocks, so no set is copied into another).  The D blocks are in a ring
+  // of four LDS buffers, block kb + 2 written while block kb is read, so
+  // the workgroup meets at a barrier after every second block: a buffer is
+  // rewritten two blocks after it was read, with a barrier in between.
   auto l1_step = [&](int kb, const h8<H>(&cur)[2][4], h8<H>(&nxt2)[2][4]) {
     h8<H> r;
-    const bool more = kb + 1 < nkb;
-    // fused form: D block kb + 1 is built from the staged GR
+    const bool more = kb + 2 < nkb;
+    // fused form: D block kb + 2 is built from the staged GR
     if constexpr (kGR) {
-      if (more) gen_block(kb + 1, r);
+      if (more) gen_block(kb + 2, r);
     } else {
-      if (more) load_block(kb + 1, r);
+      if (more) load_block(kb + 2, r);
     }
     load_w1(kb + 2 < nkb ? kb + 2 : nkb - 1, nxt2);
 #pragma unroll
     for (int kc = 0; kc < 2; ++kc) {
       h8<H> b[kSub];
 #pragma unroll
-      for (int s = 0; s < kSub; ++s) b[s] = lds_b(s_x(kb & 1), kXS, s0 + s, n, 32 * kc, g);
+      for (int s = 0; s < kSub; ++s) b[s] = lds_b(s_x(kb & 3), kXS, s0 + s, n, 32 * kc, g);
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
         for (int s = 0; s < kSub; ++s) acc[mt][s] = mfma(cur[kc][mt], b[s], acc[mt][s]);
     }
     if constexpr (kGR) {
-      if (more) store_gen((kb + 1) & 1, r);
+      if (more) store_gen((kb + 2) & 3, r);
     } else {
-      if (more) store_block((kb + 1) & 1, r);
+      if (more) store_block((kb + 2) & 3, r);
     }
-    __syncthreads();
+    if (kb & 1) __syncthreads();  // uniform
   };
   h8<H> a0[2][4], a1[2][4], a2[2][4];
   load_w1(0, a0);
@@ -343,6 +354,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
     if (kb + 1 < nkb) l1_step(kb + 1, a1, a0);
     if (kb + 2 < nkb) l1_step(kb + 2, a2, a1);
   }
+  __syncthreads();  // every wavefront is done with the D blocks
   // ---- layer-1 epilogue: h (fp32, skip of layer 2), tanh factors f1 (fp32)
   float h[4][kSub][4], f1[4][kSub][4], f2[4][kSub][4], f3[4][kSub][4];
 #pragma unroll
