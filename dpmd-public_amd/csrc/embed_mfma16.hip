@@ -55,6 +55,7 @@ constexpr int kConsts = 576;   // k w1, k b1, idt1 [32]; k b2, idt2 [64]; k b3, 
                                // idt2 / k [64]   (k = 2 log2 e)
 constexpr int kQ3 = 128;       // internal width of G
 constexpr int kAtoms = 4;      // atoms (wavefronts) per workgroup
+constexpr int kH2P = 80;       // row pitch (16-bit elements) of a tile's h2 in LDS: conflict-free transposed reads
 
 struct EmbedParams {
   int ntypes;
@@ -277,7 +278,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
                 const float* __restrict__ consts, int nloc, int per_wave, EmbedParams p) {
   __shared__ h8<H> s_frag[kFragFwd * 64];
   __shared__ __attribute__((aligned(16))) float s_cst[kConsts];
-  __shared__ __attribute__((aligned(16))) float s_em[kAtoms][16 * 16];  // [row][4 slot + c]
+  __shared__ __attribute__((aligned(16))) H s_em[kAtoms][16 * 16];  // em in H, [row][4 slot + c]
+  // h2 of the tile in H, [row][feature] (pitch kH2P): read back transposed
+  __shared__ __attribute__((aligned(16))) H s_h2[kAtoms][16 * kH2P];
   // layer-3 bias (k b3) of feature 16 mt + n in all four rows: the
   // accumulator input of the transposed product, one 16-byte read
   __shared__ f32x4 s_b3x4[8 * 16];
@@ -298,7 +301,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
   // evaluated, their G^T R share is n_pad em_pad^T G_pad with G_pad
   // evaluated once per center type here (with the H roundings the tile path
   // applies).
-  __shared__ float s_gpad[kMaxTypes][kQ3];
+  extern __shared__ float s_gpad[];  // [ntypes][kQ3] (dynamic: sized by the launch)
   if (p.counts != nullptr) {
     for (int ti = w; ti < p.ntypes; ti += kAtoms) {
       TileFwd<H> f;
@@ -309,7 +312,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt) {
           const int q = 16 * mt + r;
-          s_gpad[ti][q] = cst[352 + q] * (1.f - 2.f * (float)(H)f.r3t[mt * 4]);
+          s_gpad[ti * kQ3 + q] = cst[352 + q] * (1.f - 2.f * (float)(H)f.r3t[mt * 4]);
         }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -318,25 +321,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
         for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
           for (int v = 0; v < 4; ++v)
-            s_gpad[ti][16 * mt + 4 * g + v] += (float)f.b2[(mt & 3) >> 1][((mt & 3) & 1) * 4 + v];
+            s_gpad[ti * kQ3 + 16 * mt + 4 * g + v] += (float)f.b2[(mt & 3) >> 1][((mt & 3) & 1) * 4 + v];
     }
     __syncthreads();
   }
-  float* sem = s_em[w];
+  H* sem = s_em[w];
+  H* sh2 = s_h2[w];
   const float* em_t = em + (int64_t)p.sec[t] * 4;
   const int a_lo = min((blockIdx.x * kAtoms + w) * per_wave, nloc), a_hi = min(a_lo + per_wave, nloc);
   for (int wa = a_lo; wa < a_hi; wa += 64) {
     const int n = min(64, a_hi - wa);
-    // identity fragments: B (K = phi slot of a 32-chunk, N = n), one per
-    // 16-wide half (built per window from an opaque lane index: hoisted out
-    // of the loop they would be spilled)
-    int ln = lane;
-    asm volatile("" : "+v"(ln));
-    h8<H> eye[2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int u = 0; u < 8; ++u) eye[j][u] = (H)(((u >> 2) == j && 4 * (ln >> 4) + (u & 3) == (ln & 15)) ? 1.f : 0.f);
     int cnt = 0;
     if (lane < n) cnt = p.counts != nullptr ? (int)p.counts[(int64_t)(wa + lane) * p.ntypes + t] : sel;
     int pre = cnt;
@@ -384,7 +378,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
         int row;
         const int sl = pos < c.p1 ? (chunk_atom(c, k0, pos, &row) & (kSlots - 1)) : -1;
         // em staged as [row][4 slot + c]: lane group g writes the slot-g quarter of its row
-        *reinterpret_cast<float4*>(sem + 16 * r + 4 * g) = sl == g ? ev : make_float4(0.f, 0.f, 0.f, 0.f);
+        {
+          h4<H> e4;
+          e4[0] = (H)(sl == g ? ev.x : 0.f);
+          e4[1] = (H)(sl == g ? ev.y : 0.f);
+          e4[2] = (H)(sl == g ? ev.z : 0.f);
+          e4[3] = (H)(sl == g ? ev.w : 0.f);
+          *reinterpret_cast<h4<H>*>(sem + 16 * r + 4 * g) = e4;
+        }
         // weights and constants stay in LDS (re-read per tile), not in registers
         asm volatile("" ::: "memory");
         TileFwd<H> f;
@@ -394,7 +395,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
         // em^T as A (M = 4 slot + c, K = row 4 g + u)
         h4<H> a;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) a[u] = (H)sem[16 * (4 * g + u) + r];
+        for (int u = 0; u < 4; ++u) a[u] = sem[16 * (4 * g + u) + r];
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt) {
           h4<H> b;
@@ -402,16 +403,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
           for (int v = 0; v < 4; ++v) b[v] = (H)f.r3t[mt * 4 + v];
           acc[mt] = mfma16(a, b, acc[mt]);
         }
+        // h2 (lane = row r, features 32 kc + 16 hh + 4 g + 0..3 in b2[kc]
+        // slots 4 hh ..) into [row][feature], read back transposed: in lane
+        // group G, lane i addresses row 4 G + i / 4, features 16 j + 4 (i % 4)
+        // .. + 3, and ds_read_b64_tr_b16 hands lane i rows 4 G .. 4 G + 3 of
+        // feature 16 j + i, the K = 16 operand of em^T h2
 #pragma unroll
         for (int kc = 0; kc < 2; ++kc)
 #pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            const f32x4 d = mfma(f.b2[kc], eye[j], f32x4{0.f, 0.f, 0.f, 0.f});
-            h4<H> b;
+          for (int hh = 0; hh < 2; ++hh) {
+            h4<H> x;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) b[v] = (H)d[v];
-            acc_h[2 * kc + j] = mfma16(a, b, acc_h[2 * kc + j]);
+            for (int v = 0; v < 4; ++v) x[v] = f.b2[kc][4 * hh + v];
+            *reinterpret_cast<h4<H>*>(sh2 + r * kH2P + 32 * kc + 16 * hh + 4 * g) = x;
           }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          typedef short s16x4_lds __attribute__((ext_vector_type(4)));
+          auto* src = (__attribute__((address_space(3))) s16x4_lds*)(sh2 + (4 * g + (r >> 2)) * kH2P + 16 * j + 4 * (r & 3));
+          const h4<H> b = __builtin_bit_cast(h4<H>, __builtin_amdgcn_ds_read_tr16_b64_v4i16(src));
+          acc_h[j] = mfma16(a, b, acc_h[j]);
+        }
         h4<H> ones;
 #pragma unroll
         for (int u = 0; u < 4; ++u) ones[u] = (H)1.f;
@@ -450,7 +464,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip 
             const int q = 16 * nt + r;
             const int m = orig_of_q3(q, n1);
             const float idt3 = cst[352 + q];
-            const float gp = n_pad > 0 ? s_gpad[tp][q] : 0.f;
+            const float gp = n_pad > 0 ? s_gpad[tp * kQ3 + q] : 0.f;
             float o[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) o[v] = idt3 * (acc_s[v] - 2.f * acc[nt][v]) + acc_h[nt & 3][v] + padc[v] * gp;
@@ -925,11 +939,13 @@ int embed_fwd(float* gr_part, const float* em, const void* frags, const float* c
   // per CU: one whose wavefronts end early is soon followed by another)
   dim3 grid((unsigned)std::min<int64_t>(ceil_div(nloc, kAtoms), 1024), (unsigned)ntypes);
   const int per_wave = (int)ceil_div(nloc, (int64_t)grid.x * kAtoms);
+  // G of the padding row per center type (dynamic LDS, ntypes x 128 floats)
+  const size_t shm_gpad = (size_t)ntypes * kQ3 * sizeof(float);
   if (skip_layout(n1))
-    hipLaunchKernelGGL((k_embed_fwd<H, true>), grid, dim3(256), 0, as_stream(stream), gr_part, em,
+    hipLaunchKernelGGL((k_embed_fwd<H, true>), grid, dim3(256), shm_gpad, as_stream(stream), gr_part, em,
                        static_cast<const h8<H>*>(frags), consts, (int)nloc, per_wave, p);
   else
-    hipLaunchKernelGGL((k_embed_fwd<H, false>), grid, dim3(256), 0, as_stream(stream), gr_part, em,
+    hipLaunchKernelGGL((k_embed_fwd<H, false>), grid, dim3(256), shm_gpad, as_stream(stream), gr_part, em,
                        static_cast<const h8<H>*>(frags), consts, (int)nloc, per_wave, p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
