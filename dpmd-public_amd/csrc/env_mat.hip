@@ -184,6 +184,7 @@ __global__ void __launch_bounds__(256)
   if (ti >= sp.ntypes) ti = -1;  // no statistics for such a type: treated as virtual
   const CT xi0 = ext_c[3 * ii + 0], xi1 = ext_c[3 * ii + 1], xi2 = ext_c[3 * ii + 2];
   const int nem = nnei * 4;
+  const float inv_range = 1.f / (rmax - rmin);
   // one neighbor slot: coordinates loaded, limits known
   auto slot = [&](int s, int j, bool valid, const CT xj[3]) {
     FP rr[3] = {(FP)0., (FP)0., (FP)0.};
@@ -195,7 +196,11 @@ __global__ void __launch_bounds__(256)
       rr[0] = (FP)(xj[0] - xi0);
       rr[1] = (FP)(xj[1] - xi1);
       rr[2] = (FP)(xj[2] - xi2);
-      env_one(rr, rmin, rmax, v, d);
+      // fp32: the reciprocal-based core the fp32 force recompute uses too
+      if constexpr (sizeof(FP) == 4)
+        env_one_fast(rr, rmin, inv_range, v, d);
+      else
+        env_one(rr, rmin, rmax, v, d);
     }
     int64_t o = (int64_t)i * nem + 4 * s;
     if (ti >= 0) {

@@ -120,8 +120,9 @@ int dpmi_prod_env_mat_a_f32(float* em, float* em_deriv, float* rij, const int* n
                             const float* std, float rcut, float rcut_smth, const int* sel_host, int ntypes,
                             void* stream);
 /* Precision form: double coordinates, the difference x_j - x_i formed in
- * double and everything after it in float.  (Both float forms scale by the
- * hardware reciprocal of std, within 1 ulp of the division.) */
+ * double and everything after it in float.  (Both float forms evaluate with
+ * hardware reciprocals - 1/r from rsq, 1/(rcut - rcut_smth) once, times
+ * 1/std - a few ulp from the IEEE divisions of the double form.) */
 int dpmi_prod_env_mat_a_f32c64(float* em, float* em_deriv, float* rij, const int* nlist, const double* ext_coord,
                                const int* ext_type, int64_t nloc, const float* avg, const float* std, float rcut,
                                float rcut_smth, const int* sel_host, int ntypes, void* stream);
