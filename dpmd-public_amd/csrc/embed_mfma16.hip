@@ -29,7 +29,7 @@
 // G = idt3 (1 - 2 r3) + [h2, h2], so G^T R = idt3 (em^T 1) - 2 idt3 (em^T r3)
 // + em^T [h2, h2]; layer 3 is computed transposed, so r3 comes out with the
 // rows in the K position of the em^T r3 product, h2 is transposed per
-// 16-row tile by MFMA against an identity fragment.  The backward pass
+// 16-row tile through LDS (ds_read_b64_tr_b16).  The backward pass
 // recomputes the forward per tile together with its tangent in the scalar
 // input s (forward-mode: the same fragments, the same register layout) and
 // contracts both with dGR, so it needs no transposes and no adjoint chain.
@@ -267,9 +267,9 @@ __device__ inline void split_rows(const EmbedParams& p, int t, int sel, int A0, 
 // barrier after set-up).  G^T R contracts over neighbor rows: r3 comes out
 // of the transposed layer-3 product with the feature as the lane index and
 // lane group g holding rows 4g .. 4g + 3, the K = 16 operand of em^T r3; h2
-// comes out of layer 2 with the row as the lane index, so each of its
-// 16-feature blocks is transposed by MFMA against an identity fragment
-// (exact for a 16-bit operand) into the same layout for em^T h2.  The A
+// comes out of layer 2 with the row as the lane index, so the tile's h2 is
+// stored to LDS as 16-bit [row][feature] and read back with the transposing
+// ds_read_b64_tr_b16 into the same layout for em^T h2.  The A
 // operand of these products and of em^T 1 holds em[row][c] in M row
 // 4 slot(row) + c and zero in the other slots' rows.
 template <typename H, bool kSkip>
