@@ -468,11 +468,22 @@ __global__ void __launch_bounds__(576)
   const int f = blockIdx.x;
   const double* p = partial + (int64_t)f * blocks_per_frame * 9;
   double s = 0., c = 0.;
-  for (int b = lane; b < blocks_per_frame; b += kWave) {
-    double x = p[(int64_t)b * 9 + k];
-    double t = s + x;
-    c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
-    s = t;
+  // eight loads in flight, summed in block order (b = lane, lane + 64, ...)
+  constexpr int kB = 8;
+  for (int b0 = lane; b0 < blocks_per_frame; b0 += kB * kWave) {
+    double xs[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int b = b0 + u * kWave;
+      xs[u] = b < blocks_per_frame ? p[(int64_t)b * 9 + k] : 0.;
+    }
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      if (b0 + u * kWave >= blocks_per_frame) break;
+      const double x = xs[u], t = s + x;
+      c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+      s = t;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     double so = __shfl_xor(s, off, kWave), co = __shfl_xor(c, off, kWave);
