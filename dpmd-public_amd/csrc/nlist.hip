@@ -543,6 +543,10 @@ __global__ void __launch_bounds__(64) k_search_format(
 // more than kKeyCap candidates, or a cell whose neighborhood exceeds
 // kCandCap atoms, is appended to the redo list for k_search_format.
 constexpr int kKeyCap = 512;  // plus 8 padding slots per wavefront
+// three or four types: each type's 64-bit keys in a segment of its own
+constexpr int kSegTypes = 4;
+constexpr int kSeg4 = kKeyCap / kSegTypes;
+constexpr int kSeg4Cap = kSeg4 - 8;  // room for the padding in each segment
 constexpr size_t kMaxCellsLds = 150 * 1024;  // dynamic LDS the cell-wise search may take
 constexpr int kBadType = 127;                // packed type code of a candidate that never passes
 
@@ -580,7 +584,7 @@ __device__ inline double screen_margin(double xmax, float rcut) {
 // the second
 constexpr int search_waves(bool big) { return big ? 4 : 8; }
 
-template <int CAP, bool kBigPass>
+template <int CAP, bool kBigPass, bool kSeg>
 __global__ void __launch_bounds__(64 * search_waves(kBigPass)) k_search_cells(
     int* __restrict__ nlist, int* __restrict__ overflow, int* __restrict__ nlist_mapped,
     int16_t* __restrict__ cand_slot, const double* __restrict__ ext_c, const int* __restrict__ ext_t,
@@ -617,6 +621,8 @@ __global__ void __launch_bounds__(64 * search_waves(kBigPass)) k_search_cells(
   // bits of r^2), which order the candidates like the 64-bit keys except for
   // exact r^2 ties, followed by the candidates' extended indices.
   const bool short_keys = sp.ntypes <= 2;
+  // (the host picks kSeg for three or four types)
+  constexpr bool seg_keys = kSeg;
   uint32_t* k32 = reinterpret_cast<uint32_t*>(keys);
   int* kj = reinterpret_cast<int*>(k32 + (kKeyCap + 8));
   int* row = rows_all + w * sp.nnei;
@@ -663,6 +669,7 @@ __global__ void __launch_bounds__(64 * search_waves(kBigPass)) k_search_cells(
       const int i = __float_as_int(me.w) & 0xFFFFFF;
       const float fi0 = me.x, fi1 = me.y, fi2 = me.z;
       int n = 0, n0 = 0;
+      int ns[kSegTypes] = {0, 0, 0, 0};  // keys per type segment (seg_keys)
       for (int g0 = 0; g0 < T; g0 += kWave) {
         const int g = g0 + lane;
         bool pass = false;
@@ -694,8 +701,22 @@ __global__ void __launch_bounds__(64 * search_waves(kBigPass)) k_search_cells(
           }
         }
         const uint64_t m = __ballot(pass);
+        if (seg_keys) {
+          const int tp = (int)(key >> 56);
+          int r = 0;
+#pragma unroll
+          for (int t = 0; t < kSegTypes; ++t) {
+            const uint64_t mt = __ballot(pass && tp == t);
+            if (tp == t) r = ns[t] + mask_prefix(mt);
+            ns[t] += __popcll(mt);
+          }
+          if (pass && r < kSeg4Cap) {
+            keys[kSeg4 * tp + r] = key;
+            kslot[kSeg4 * tp + r] = (int16_t)g;
+          }
+        }
         const int pos = n + mask_prefix(m);
-        if (pass && pos < kKeyCap) {
+        if (!seg_keys && pass && pos < kKeyCap) {
           if (short_keys) {
             k32[pos] = key32;
             kj[pos] = jpass;
@@ -707,7 +728,11 @@ __global__ void __launch_bounds__(64 * search_waves(kBigPass)) k_search_cells(
         n += __popcll(m);
         n0 += __popcll(__ballot(pass && (key >> 56) == 0));
       }
-      if (n > kKeyCap) {  // wave-uniform: leave this atom to the per-atom kernel
+      int nsmax = 0;
+#pragma unroll
+      for (int t = 0; t < kSegTypes; ++t) nsmax = max(nsmax, ns[t]);
+      // wave-uniform: leave this atom to the per-atom kernel
+      if (seg_keys ? nsmax > kSeg4Cap : n > kKeyCap) {
         if (lane == 0) redo[atomicAdd(&misc[2], 1)] = i;
         continue;
       }
@@ -717,7 +742,17 @@ __global__ void __launch_bounds__(64 * search_waves(kBigPass)) k_search_cells(
       }
       // rank of each key within its type: two keys per lane per pass over the
       // key list, read 8 at a time (independent LDS reads)
-      if (n & 7) {  // pad to a multiple of 8 with keys that rank after everything
+      const bool seg_fast = seg_keys && nsmax <= kWave;
+      if (seg_keys) {
+        // each segment padded to a multiple of 8 (all of them to the longest
+        // when every type fits one lane per key)
+#pragma unroll
+        for (int t = 0; t < kSegTypes; ++t) {
+          const int e = seg_fast ? (nsmax + 7) & ~7 : (ns[t] + 7) & ~7;
+          if (t < sp.ntypes)
+            for (int k = ns[t] + lane; k < e; k += kWave) keys[kSeg4 * t + k] = ~0ull;
+        }
+      } else if (n & 7) {  // pad to a multiple of 8 with keys that rank after everything
         if (lane < 8 - (n & 7)) {
           if (short_keys) {
             k32[n + lane] = ~0u;
@@ -793,52 +828,95 @@ __global__ void __launch_bounds__(64 * search_waves(kBigPass)) k_search_cells(
           return keys[q];
         }
       };
-      auto rank64 = [&](auto shortk) {
-      for (int k0 = 0; k0 < n; k0 += 2 * kWave) {
+      // keys base .. base + cnt - 1, the list padded to a multiple of 8
+      auto rank64 = [&](auto shortk, int base, int cnt) {
+      const int cnt8 = (cnt + 7) & ~7;
+      for (int k0 = 0; k0 < cnt; k0 += 2 * kWave) {
         const int ka = k0 + lane, kb = k0 + kWave + lane;
-        const uint64_t kka = ka < n ? key_at(ka, shortk) : ~0ull, kkb = kb < n ? key_at(kb, shortk) : ~0ull;
+        const uint64_t kka = ka < cnt ? key_at(base + ka, shortk) : ~0ull,
+                       kkb = kb < cnt ? key_at(base + kb, shortk) : ~0ull;
         const uint64_t ta = kka & (0xFFull << 56), tb = kkb & (0xFFull << 56);
         const uint64_t rela = kka - ta, relb = kkb - tb;
         int ra = 0, rb = 0;
-        for (int q = 0; q < n8; q += 8) {
+        for (int q = 0; q < cnt8; q += 8) {
           uint64_t kq[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) kq[u] = key_at(q + u, shortk);
+          for (int u = 0; u < 8; ++u) kq[u] = key_at(base + q + u, shortk);
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             ra += (kq[u] - ta) < rela;
             rb += (kq[u] - tb) < relb;
           }
         }
-        if (ka < n) {
+        if (ka < cnt) {
           const int t = (int)(kka >> 56);
           const int sel_t = s_sl.sel[t], sec_t = s_sl.sec[t];
-          
           if (ra < sel_t) {
             row[sec_t + ra] = (int)(kka & 0xFFFFFFull);
-            rslot[sec_t + ra] = kslot[ka];
+            rslot[sec_t + ra] = kslot[base + ka];
           } else {
             ovf = max(ovf, t);
           }
         }
-        if (kb < n) {
+        if (kb < cnt) {
           const int t = (int)(kkb >> 56);
           const int sel_t = s_sl.sel[t], sec_t = s_sl.sec[t];
-          
           if (rb < sel_t) {
             row[sec_t + rb] = (int)(kkb & 0xFFFFFFull);
-            rslot[sec_t + rb] = kslot[kb];
+            rslot[sec_t + rb] = kslot[base + kb];
           } else {
             ovf = max(ovf, t);
           }
         }
       }
       };
+      // three or four types: each key against its own segment only; lane l
+      // ranks key l of every segment when each fits one lane per key (the
+      // rank is the number of smaller keys: keys of one type differ in j)
+      if (seg_fast) {
+        ranked = true;
+        const int q8 = (nsmax + 7) & ~7;
+        uint64_t kt[kSegTypes];
+        int rk[kSegTypes];
+#pragma unroll
+        for (int t = 0; t < kSegTypes; ++t) {
+          kt[t] = t < sp.ntypes && lane < ns[t] ? keys[kSeg4 * t + lane] : ~0ull;
+          rk[t] = 0;
+        }
+        for (int q = 0; q < q8; q += 8) {
+#pragma unroll
+          for (int t = 0; t < kSegTypes; ++t) {
+            if (t >= sp.ntypes) break;  // uniform
+            uint64_t kq[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) kq[u] = keys[kSeg4 * t + q + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) rk[t] += kq[u] < kt[t];
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < kSegTypes; ++t) {
+          if (t < sp.ntypes && lane < ns[t]) {
+            const int sel_t = s_sl.sel[t], sec_t = s_sl.sec[t];
+            if (rk[t] < sel_t) {
+              row[sec_t + rk[t]] = (int)(kt[t] & 0xFFFFFFull);
+              rslot[sec_t + rk[t]] = kslot[kSeg4 * t + lane];
+            } else {
+              ovf = max(ovf, t);
+            }
+          }
+        }
+      }
       if (!ranked) {
-        if (short_keys)
-          rank64(std::true_type{});
-        else
-          rank64(std::false_type{});
+        if (short_keys) {
+          rank64(std::true_type{}, 0, n);
+        } else if (seg_keys) {
+#pragma unroll
+          for (int t = 0; t < kSegTypes; ++t)
+            if (t < sp.ntypes) rank64(std::false_type{}, kSeg4 * t, ns[t]);
+        } else {
+          rank64(std::false_type{}, 0, n);
+        }
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -1163,19 +1241,26 @@ static int build_nlist_impl(int* nlist, int* overflow, int* nlist_mapped, int16_
                shm_big = search_cells_lds(kCandCapBig, sp.nnei, search_waves(true));
   if (search_uses_cells(sp.nnei)) {
     // first pass over every cell; cells whose neighborhood exceeds its table
-    // are listed for the second pass (larger table, fewer workgroups per CU)
-    if (shm_cells > 65536)
-      DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_search_cells<kCandCap, false>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_cells));
-    DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_search_cells<kCandCapBig, true>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_big));
-    hipLaunchKernelGGL((k_search_cells<kCandCap, false>), dim3((unsigned)std::min<int64_t>(nloc, 2048)),
-                       dim3(64 * search_waves(false)),
-                       shm_cells, st, nlist, overflow, nlist_mapped, cand_slot, ext_coord, ext_type, mapping, ws, sp,
-                       rcut);
-    hipLaunchKernelGGL((k_search_cells<kCandCapBig, true>), dim3((unsigned)std::min<int64_t>(nloc, 2048)), dim3(256),
-                       shm_big, st, nlist, overflow, nlist_mapped, cand_slot, ext_coord, ext_type, mapping, ws, sp,
-                       rcut);
+    // are listed for the second pass (larger table, fewer workgroups per CU);
+    // three or four types keep each type's keys in a segment of its own
+    auto passes = [&](auto first, auto second) -> int {
+      if (shm_cells > 65536)
+        DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(first), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)shm_cells));
+      DPMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(second), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)shm_big));
+      hipLaunchKernelGGL(first, dim3((unsigned)std::min<int64_t>(nloc, 2048)), dim3(64 * search_waves(false)),
+                         shm_cells, st, nlist, overflow, nlist_mapped, cand_slot, ext_coord, ext_type, mapping, ws, sp,
+                         rcut);
+      hipLaunchKernelGGL(second, dim3((unsigned)std::min<int64_t>(nloc, 2048)), dim3(64 * search_waves(true)),
+                         shm_big, st, nlist, overflow, nlist_mapped, cand_slot, ext_coord, ext_type, mapping, ws, sp,
+                         rcut);
+      return DPMI_OK;
+    };
+    const int rc = sp.ntypes > 2 && sp.ntypes <= kSegTypes
+                       ? passes(k_search_cells<kCandCap, false, true>, k_search_cells<kCandCapBig, true, true>)
+                       : passes(k_search_cells<kCandCap, false, false>, k_search_cells<kCandCapBig, true, false>);
+    if (rc) return rc;
   } else {
     // very long rows (sum(sel)): every atom through the per-atom kernel
     size_t shm = 1024 * sizeof(uint64_t) + sizeof(int) * sp.nnei;
