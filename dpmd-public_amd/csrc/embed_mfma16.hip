@@ -273,7 +273,7 @@ __device__ inline void split_rows(const EmbedParams& p, int t, int sel, int A0, 
 // operand of these products and of em^T 1 holds em[row][c] in M row
 // 4 slot(row) + c and zero in the other slots' rows.
 template <typename H, bool kSkip>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSkip ? 4 : 3, kSkip ? 4 : 3)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
     k_embed_fwd(float* __restrict__ gr_part, const float* __restrict__ em, const h8<H>* __restrict__ frag_all,
                 const float* __restrict__ consts, int nloc, int per_wave, EmbedParams p) {
   __shared__ h8<H> s_frag[kFragFwd * 64];
