@@ -59,8 +59,8 @@ KERNEL_OF = {"dpmi_force_virial_cells_f32": "k_force_cells<float, 1536, false, 3
 for _h in ("bf16", "f16"):
     KERNEL_OF.update({f"dpmi_se_a_embed_bwd_{_h}": f"k_embed_bwd<{_h}, skip>",
                       f"dpmi_se_a_embed_fwd_{_h}": f"k_embed_fwd<{_h}, skip>",
-                      f"dpmi_se_a_fitting_{_h}": f"k_fit<{_h}, false>",
-                      f"dpmi_se_a_fitting_gr_{_h}": f"k_fit<{_h}, true>"})
+                      f"dpmi_se_a_fitting_{_h}": f"k_fit<{_h}, 0>",
+                      f"dpmi_se_a_fitting_gr_{_h}": f"k_fit<{_h}, 1>"})
 
 
 def _measured_traffic(workload: str, entry: str):

@@ -56,8 +56,12 @@ constexpr int kXS = kKB + 8;    // LDS row stride (16-bit elements) of the stage
 constexpr int kHS = kH + 8;     // LDS row stride (16-bit elements) of the hidden-layer buffer
 constexpr int kMaxTypes = 16;
 constexpr int kCst = 7 * kH + 8;  // b1 idt1 b2 idt2 b3 idt3 w4 [kH each], b4
-constexpr int kMaxGR = 104;       // widest G^T R (M) the fused descriptor form stages
+constexpr int kMaxGR = 104;       // widest G^T R (M) the fused descriptor form stages in fp32
 constexpr int kGRS = 4 * kMaxGR + 4;  // LDS floats per atom of the staged GR ([m][c], padded)
+// 104 < M <= 128: GR staged in fp16 (fp32 would take 66 KB of LDS per
+// workgroup, and two workgroups per CU would no longer fit)
+constexpr int kMaxGR16 = 128;
+constexpr int kGRS16 = 4 * kMaxGR16 + 8;  // LDS halves per atom of the fp16-staged GR
 constexpr int kMaxTypesGR = 4;        // neighbor types whose partials the fused form sums
 
 // Fragment offsets (in 64-lane fragments of 8 16-bit values) inside one type's block.
@@ -151,15 +155,16 @@ __device__ inline void hidden_product(f32x4 (&acc)[4][kSub], const h8<H>* __rest
   }
 }
 
-// With kGR the descriptor is formed inside the kernel (the fused descriptor
-// form, axis 16): the workgroup stages GR = sum_t gr_part / nnei of its 64
-// atoms in LDS ([atom][m][c], fp32), builds each 64-wide block of
+// With kGR > 0 the descriptor is formed inside the kernel (the fused
+// descriptor form, axis 16): the workgroup stages GR = sum_t gr_part / nnei
+// of its 32 atoms in LDS ([atom][m][c]; fp32 for kGR = 1, M <= 104, fp16 for
+// kGR = 2, M <= 128, products and sums in fp32 either way), builds each 64-wide block of
 // D[m][a] = sum_c GR[c][m] GR[c][a] (se_a.py:851-858) into the layer-1
 // staging buffer, and ends with dE/dGR instead of dE/dD:
 //   dGR[c][m] = sum_a dD[m][a] GR[c][a] + [m < 16] sum_m' dD[m'][m] GR[c][m'],
 // written / nnei in the embedding kernels' internal order as dgr
 // [nloc, 128, 4] (component fastest: one 16-byte store per atom and feature).
-template <typename H, bool kGR>
+template <typename H, int kGR>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_fit(double* __restrict__ atom_energy, float* __restrict__ dout, const H* __restrict__ desc,
           const float* __restrict__ gr_part, const int* __restrict__ perm, const int* __restrict__ type_start,
@@ -174,7 +179,8 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
   H* s_h = s_xh;
   __shared__ float s_e[4][kTile];
   __shared__ int s_atom[kTile];
-  __shared__ __attribute__((aligned(16))) float s_gr[kGR ? kTile * kGRS : 4];
+  constexpr int kGRFloats = kGR == 1 ? kTile * kGRS : kGR == 2 ? kTile * kGRS16 / 2 : 4;
+  __shared__ __attribute__((aligned(16))) float s_gr[kGRFloats];
 
   // tile -> (type, first sorted index); every thread of the workgroup
   // computes the same answer, so the early exit is uniform
@@ -211,7 +217,28 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
   auto store_block = [&](int buf, const h8<H>& r) {
     *reinterpret_cast<h8<H>*>(s_x(buf) + xl * kXS + 8 * xq) = r;
   };
-  float4* sgr4 = reinterpret_cast<float4*>(s_gr);
+  // GR[atom][m][0..3] of the staged tile, as fp32 or fp16 words
+  auto gr_ld = [&](int atom, int m) -> float4 {
+    if constexpr (kGR == 2) {
+      const h4<_Float16> q = *reinterpret_cast<const h4<_Float16>*>(
+          reinterpret_cast<const _Float16*>(s_gr) + atom * kGRS16 + 4 * m);
+      return make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
+    } else {
+      return reinterpret_cast<const float4*>(s_gr)[atom * (kGRS / 4) + m];
+    }
+  };
+  auto gr_st = [&](int atom, int m, const float4& x) {
+    if constexpr (kGR == 2) {
+      h4<_Float16> q;
+      q[0] = (_Float16)x.x;
+      q[1] = (_Float16)x.y;
+      q[2] = (_Float16)x.z;
+      q[3] = (_Float16)x.w;
+      *reinterpret_cast<h4<_Float16>*>(reinterpret_cast<_Float16*>(s_gr) + atom * kGRS16 + 4 * m) = q;
+    } else {
+      reinterpret_cast<float4*>(s_gr)[atom * (kGRS / 4) + m] = x;
+    }
+  };
   // D block kb of atom xl, features (m = 4 kb + 2 mh + ml, a = 4 aq .. 4 aq + 3)
   // with aq = xq % 4, mh = xq / 4: the thread's GR[.][a] (ga) stay in
   // registers through layer 1
@@ -220,7 +247,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
   auto gen_block = [&](int kb, h8<H>& r) {
 #pragma unroll
     for (int ml = 0; ml < 2; ++ml) {
-      const float4 gm = sgr4[xl * (kGRS / 4) + 4 * kb + 2 * mh + ml];
+      const float4 gm = gr_ld(xl, 4 * kb + 2 * mh + ml);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         r[4 * ml + j] = (H)(gm.x * ga[j].x + gm.y * ga[j].y + gm.z * ga[j].z + gm.w * ga[j].w);
@@ -243,7 +270,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
     for (int s = 0; s < kSub; ++s) acc[mt][s] = f32x4{0.f, 0.f, 0.f, 0.f};
-  if constexpr (kGR) {
+  if constexpr (kGR > 0) {
     // all of the tile's GR at once: work item (atom la, feature m), kBatch
     // items per thread in flight; the four components of every type are read
     // as one float4 (gr_part [t][i][m][c]: consecutive lanes read consecutive
@@ -279,13 +306,12 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
 #pragma unroll
           for (int tt = 1; tt < kMaxTypesGR; ++tt) v[c] += x[b][tt][c];
         }
-        sgr4[la * (kGRS / 4) + m] =
-            make_float4(v[0] * p.inv_nnei, v[1] * p.inv_nnei, v[2] * p.inv_nnei, v[3] * p.inv_nnei);
+        gr_st(la, m, make_float4(v[0] * p.inv_nnei, v[1] * p.inv_nnei, v[2] * p.inv_nnei, v[3] * p.inv_nnei));
       }
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 4; ++j) ga[j] = sgr4[xl * (kGRS / 4) + 4 * aq + j];
+    for (int j = 0; j < 4; ++j) ga[j] = gr_ld(xl, 4 * aq + j);
     h8<H> r;
     gen_block(0, r);
     store_gen(0, r);
@@ -322,7 +348,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
     h8<H> r;
     const bool more = kb + 2 < nkb;
     // fused form: D block kb + 2 is built from the staged GR
-    if constexpr (kGR) {
+    if constexpr (kGR > 0) {
       if (more) gen_block(kb + 2, r);
     } else {
       if (more) load_block(kb + 2, r);
@@ -338,7 +364,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
 #pragma unroll
         for (int s = 0; s < kSub; ++s) acc[mt][s] = mfma(cur[kc][mt], b[s], acc[mt][s]);
     }
-    if constexpr (kGR) {
+    if constexpr (kGR > 0) {
       if (more) store_gen((kb + 2) & 3, r);
     } else {
       if (more) store_block((kb + 2) & 3, r);
@@ -477,12 +503,12 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
   float4 gra[kSub][4], acc2[kSub][4];
   float4* t1buf = reinterpret_cast<float4*>(s_xh);
   float4* red = reinterpret_cast<float4*>(s_xh) + kTile * 16;
-  if constexpr (kGR) {
+  if constexpr (kGR > 0) {
 #pragma unroll
     for (int s = 0; s < kSub; ++s)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        gra[s][v] = sgr4[(16 * (s0 + s) + n) * (kGRS / 4) + 4 * g + v];
+        gra[s][v] = gr_ld(16 * (s0 + s) + n, 4 * g + v);
         acc2[s][v] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     __syncthreads();  // every wavefront has read dz1 from s_h
@@ -513,13 +539,13 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
       wa[kc] = wb[kc];
       wb[kc] = wn[kc];
     }
-    if constexpr (kGR) {
+    if constexpr (kGR > 0) {
       // M tile mt is m = mt, its rows a = 4 g + v (axis 16)
       const int q3 = mt >= 16 ? q3_of(mt, p.n1) : 0;
       float4 p1[kSub];
 #pragma unroll
       for (int s = 0; s < kSub; ++s) {
-        const float4 gm = sgr4[(16 * (s0 + s) + n) * (kGRS / 4) + mt];
+        const float4 gm = gr_ld(16 * (s0 + s) + n, mt);
         p1[s] = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -557,7 +583,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
           __builtin_nontemporal_store(d[s], reinterpret_cast<f32x4*>(dout + (int64_t)atom_s[s] * p.k1 + f));
     }
   }
-  if constexpr (kGR) {
+  if constexpr (kGR > 0) {
     // second term summed over the four feature groups in a fixed order
 #pragma unroll 1
     for (int ph = 0; ph < 4; ++ph) {
@@ -624,7 +650,7 @@ int fitting(double* atom_energy, float* ddesc, const void* desc, const int* perm
   set_offsets(p);
   // one workgroup per 64-atom tile of each type (upper bound; extra ones exit)
   const int64_t grid = ceil_div(nloc, kTile) + ntypes;
-  hipLaunchKernelGGL((k_fit<H, false>), dim3((unsigned)grid), dim3(kThreads), 0, as_stream(stream), atom_energy, ddesc,
+  hipLaunchKernelGGL((k_fit<H, 0>), dim3((unsigned)grid), dim3(kThreads), 0, as_stream(stream), atom_energy, ddesc,
                      static_cast<const H*>(desc), nullptr, perm, type_start, static_cast<const h8<H>*>(frags),
                      consts, ebias, p);
   DPMI_LAUNCH_CHECK();
@@ -636,7 +662,7 @@ int fitting_gr(double* atom_energy, float* dgr, const float* gr_part, const int*
                const void* frags, const float* consts, const double* ebias, int64_t nloc, int ntypes, int n1,
                int nnei, void* stream) {
   DPMI_CHECK_ARG(ntypes > 0 && ntypes <= kMaxTypesGR, "fused descriptor form: ntypes out of range [1, 4]");
-  DPMI_CHECK_ARG(n1 >= 4 && 4 * n1 <= kMaxGR, "fused descriptor form needs 16 <= 4 n1 <= 104");
+  DPMI_CHECK_ARG(n1 >= 4 && 4 * n1 <= kMaxGR16, "fused descriptor form needs 16 <= 4 n1 <= 128");
   DPMI_CHECK_ARG(nnei > 0, "nnei must be positive");
   DPMI_CHECK_ARG(nloc >= 0 && nloc < (int64_t(1) << 31), "nloc out of range");
   if (nloc == 0) return DPMI_OK;
@@ -650,8 +676,12 @@ int fitting_gr(double* atom_energy, float* dgr, const float* gr_part, const int*
   p.k1 = 16 * p.M;
   set_offsets(p);
   const int64_t grid = ceil_div(nloc, kTile) + ntypes;
-  hipLaunchKernelGGL((k_fit<H, true>), dim3((unsigned)grid), dim3(kThreads), 0, as_stream(stream), atom_energy, dgr,
-                     nullptr, gr_part, perm, type_start, static_cast<const h8<H>*>(frags), consts, ebias, p);
+  if (p.M <= kMaxGR)
+    hipLaunchKernelGGL((k_fit<H, 1>), dim3((unsigned)grid), dim3(kThreads), 0, as_stream(stream), atom_energy, dgr,
+                       nullptr, gr_part, perm, type_start, static_cast<const h8<H>*>(frags), consts, ebias, p);
+  else
+    hipLaunchKernelGGL((k_fit<H, 2>), dim3((unsigned)grid), dim3(kThreads), 0, as_stream(stream), atom_energy, dgr,
+                       nullptr, gr_part, perm, type_start, static_cast<const h8<H>*>(frags), consts, ebias, p);
   DPMI_LAUNCH_CHECK();
   return DPMI_OK;
 }

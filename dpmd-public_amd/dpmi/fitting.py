@@ -143,9 +143,9 @@ class FusedFitting:
         return ae, dd
 
     def supports_gr(self, n1: int, axis: int) -> bool:
-        """Whether the fused descriptor form applies: axis 16, 16 <= 4 n1 <= 104,
-        at most 4 types."""
-        return axis == 16 and 4 <= n1 <= 26 and self.k1 == 64 * n1 and self.ntypes <= 4
+        """Whether the fused descriptor form applies: axis 16, 16 <= 4 n1 <= 128,
+        at most 4 types (GR staged in fp32 up to 4 n1 = 104, in fp16 above)."""
+        return axis == 16 and 4 <= n1 <= 32 and self.k1 == 64 * n1 and self.ntypes <= 4
 
     def from_gr(self, gr_part: torch.Tensor, atype: torch.Tensor, n1: int, nnei: int, order=None):
         """Fused descriptor form: gr_part [ntypes, nloc, 4 n1, 4] fp32 (the

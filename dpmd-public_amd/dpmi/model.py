@@ -187,7 +187,7 @@ class SeAEnergyModel:
         self.fused_fit = None
         if self.fused is not None and fitting.supported(fit):
             self.fused_fit = fitting.FusedFitting(fit, bias_atom_e, ob, self.device, self.gemm_dtype)
-        # the descriptor fused into the fitting kernel (axis 16, M <= 104)
+        # the descriptor fused into the fitting kernel (axis 16, M <= 128)
         self.fused_gr = self.fused_fit is not None and self.fused_fit.supports_gr(self.fused.n1, self.axis)
         # FP64 form (network or table embedding): the descriptor, the fitting
         # nets and their adjoint as one FP64-MFMA kernel (else torch, _fit_gr64)

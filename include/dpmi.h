@@ -448,14 +448,15 @@ int dpmi_se_a_fitting_f16(double* atom_energy, float* ddesc, const void* desc, c
                           const int* type_start, const void* frags, const float* consts, const double* ebias,
                           int64_t nloc, int ntypes, int k1, void* stream);
 /* The same fitting net with the descriptor fused in (axis 16, M = 4 n1 with
- * 16 <= M <= 104): gr_part [ntypes, nloc, M, 4] as dpmi_se_a_embed_fwd_bf16
- * writes it; each 64-atom workgroup forms GR = sum_t gr_part / nnei and
+ * 16 <= M <= 128): gr_part [ntypes, nloc, M, 4] as dpmi_se_a_embed_fwd_bf16
+ * writes it; each 32-atom workgroup forms GR = sum_t gr_part / nnei and
  * D = GR^T GR[:, :16] (se_a.py:851-858) in LDS and, instead of dE/dD, writes
  * dgr [nloc, 128, 4] = dE/d(sum_t gr_part) in the embedding kernels' internal
  * feature order (what dpmi_se_a_descriptor_bwd produces, the input of
  * dpmi_se_a_embed_bwd_bf16).  Replaces the descriptor kernels and
  * dpmi_se_a_fitting_bf16 on that path; frags / consts / ebias as above with
- * k1 = 16 M. */
+ * k1 = 16 M.  GR is held in LDS as fp32 for M <= 104 and rounded to fp16
+ * there for 104 < M <= 128 (the D products and sums stay fp32). */
 int dpmi_se_a_fitting_gr_bf16(double* atom_energy, float* dgr, const float* gr_part, const int* perm,
                               const int* type_start, const void* frags, const float* consts, const double* ebias,
                               int64_t nloc, int ntypes, int n1, int nnei, void* stream);

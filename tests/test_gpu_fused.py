@@ -228,11 +228,9 @@ def test_type_order_cache_follows_atype(setup):
     assert e1 == e_ref and e1 != e0
 
 
-def _check_fused_descriptor(fe, ff, gr_part, atype):
-    """dpmi_se_a_fitting_gr_{bf16,f16} against the two-kernel chain on the same
-    G^T R partials (descriptor forward -> fitting on the 16-bit descriptor ->
-    descriptor backward): the same D and fitting arithmetic, dE/dGR summed in
-    another order; padding slots of dgr exactly zero."""
+def _chain(fe, ff, gr_part, atype):
+    """The two-kernel chain on G^T R partials: descriptor forward -> fitting
+    on the 16-bit descriptor -> descriptor backward; (atom energies, dgr)."""
     from dpmi import embed, ops
     from dpmi._lib import ptr
 
@@ -245,6 +243,16 @@ def _check_fused_descriptor(fe, ff, gr_part, atype):
     dgr_ref = torch.empty((nloc, 128, 4), dtype=torch.float32, device=DEV)
     ops._launch("dpmi_se_a_descriptor_bwd", ptr(dgr_ref), ptr(ddesc), ptr(gr), nloc, fe.nnei, fe.n1, 16,
                 ops._stream())
+    return ae_ref, dgr_ref
+
+
+def _check_fused_descriptor(fe, ff, gr_part, atype):
+    """dpmi_se_a_fitting_gr_{bf16,f16} against the two-kernel chain on the same
+    G^T R partials: the same D and fitting arithmetic, dE/dGR summed in
+    another order; padding slots of dgr exactly zero."""
+    from dpmi import embed
+
+    ae_ref, dgr_ref = _chain(fe, ff, gr_part, atype)
     ae, dgr_out = ff.from_gr(gr_part, atype, fe.n1, fe.nnei)
     torch.cuda.synchronize()
     dgr = dgr_out.double().cpu()
@@ -284,7 +292,7 @@ def test_fused_descriptor_fitting(setup, nloc):
 @pytest.mark.parametrize("n1,ntypes", [(16, 3), (26, 4), (4, 1)])
 def test_fused_descriptor_fitting_shapes(n1, ntypes, precision):
     """Other widths the fused form takes: M = 64 with three types, M = 104
-    (the widest) with four, M = 16 with one type; random partials and types,
+    (the widest staged in fp32) with four, M = 16 with one type; random partials and types,
     700 atoms; fitting width 96."""
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
@@ -332,6 +340,50 @@ def test_descriptor_backward_widths(axis):
     out = out.double().cpu().transpose(1, 2)  # [nloc, 4, 128] like ref
     assert torch.isfinite(out).all()
     assert torch.allclose(out, ref, rtol=1e-4, atol=1e-5 * ref.abs().max().item())
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_fused_descriptor_fitting_m128(precision):
+    """M = 128 (embedding [32, 64, 128], the alloy's), where the fused kernel
+    holds GR in LDS as fp16.  (1) Partials whose sum / nnei is an fp16 value
+    (nnei = 256, one nonzero type): the fused kernel and the two-kernel
+    chain see the same GR, so the fp32-staged form's bounds hold.  (2)
+    Random partials over four types, both against the FP64 fused kernel on
+    the same partials: the fused kernel's energy and dE/dGR errors stay
+    within twice the chain's (whose GR is fp32)."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dpmi import embed
+    from dpmi.model import SeAEnergyModel
+
+    _say(f"fused descriptor M=128 {precision}")
+    spec = systems.ModelSpec(ntypes=4, sel=(64, 64, 64, 64), neuron=(32, 64, 128), fitting=(240, 240, 240))
+    params = systems.make_params(spec, seed=11)
+    m = SeAEnergyModel.from_params(spec, params, device=DEV, precision=precision)
+    fe, ff = m.fused, m.fused_fit
+    assert m.fused_gr and fe.M == 128 and fe.nnei == 256
+    g = torch.Generator(device="cpu").manual_seed(128)
+    nloc = 777
+    atype = torch.randint(0, 4, (nloc,), generator=g).to(torch.int32).to(DEV)
+    exact = torch.zeros(4, nloc, 128, 4)
+    exact[0] = (torch.randn(nloc, 128, 4, generator=g) * 0.3).half().float() * 256.0
+    _check_fused_descriptor(fe, ff, exact.to(DEV), atype)
+
+    parts = (torch.randn(4, nloc, 128, 4, generator=g) * 20.0).to(DEV)
+    ae_c, dgr_c = _chain(fe, ff, parts, atype)
+    ae, dgr = ff.from_gr(parts, atype, fe.n1, fe.nnei)
+    m64 = SeAEnergyModel.from_params(spec, params, device=DEV)
+    ae64, d64 = m64.fused_fit64.from_parts(parts.double().transpose(2, 3).contiguous(), atype, fe.nnei)
+    torch.cuda.synchronize()
+    q3 = torch.as_tensor(embed.internal_maps(fe.n1)[2], device=DEV)
+    ref = torch.zeros((nloc, 128, 4), dtype=torch.float64, device=DEV)
+    ref[:, q3, :] = d64.transpose(1, 2)
+    e_f, e_c = (ae - ae64).abs().max().item(), (ae_c - ae64).abs().max().item()
+    d_f = ((dgr.double() - ref).abs().max() / ref.abs().max()).item()
+    d_c = ((dgr_c.double() - ref).abs().max() / ref.abs().max()).item()
+    print(f"M=128 {precision}: energy err fused {e_f:.2e} chain {e_c:.2e}; dgr rel fused {d_f:.2e} chain {d_c:.2e}",
+          flush=True)
+    assert e_f <= 2.0 * e_c and d_f <= 2.0 * d_c
 
 
 def _gr_ref(m, em):
